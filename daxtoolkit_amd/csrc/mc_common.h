@@ -1,0 +1,225 @@
+// Marching-cubes data layout shared by the forward, backward and batched
+// kernels.
+//
+// Inside-bits: one bit per grid point, bit = (f > iso) (strict, as
+// GetHexahedronClassification, dax/worklet/MarchingCubes.h:36-48).  Bits are
+// stored per x-row in 64-bit "row words": word q of row (j,k) holds points
+// x = 64q .. 64q+63 (bit x%64), padding bits beyond nx are 0.  The words of a
+// volume are in flat (k, j, q) order, so word index w = q + wpr*(j + ny*k)
+// with wpr = ceil(nx/64) words per row.  A row word is the unit of every
+// count: its active edges (points of the word, axes x<y<z) and its active
+// cells (cells whose lowest corner is in the word).  The exclusive scan over
+// words of those two counts gives, for every word, the first output vertex
+// id and the first output triangle id -- the order of the reference's output
+// (vertices sorted by (lower point id, upper point id), triangles by cell then
+// visit index, DispatcherGenerateInterpolatedCells.h:180-290).
+#pragma once
+
+#include "common.h"
+#include "mc_case_table.h"
+
+namespace daxhip {
+namespace mc {
+
+constexpr int kTileWords = 1024;     // row words per scan tile
+constexpr int kScanThreads = 256;    // threads of the count/scan and generate blocks
+constexpr int kWordsPerThread = kTileWords / kScanThreads;
+constexpr int kMaxWpr = 256;         // nx <= 16384
+
+// Geometry of one volume in the (possibly batched) word space.
+struct VolGeom {
+  int32_t nx, ny, nz, wpr;
+  float iso;
+  float origin[3];
+  float spacing[3];
+  int64_t field_off;  // first field element of the volume
+  int64_t word_off;   // first row word of the volume
+  int64_t nwords;     // wpr*ny*nz
+};
+
+// Edge e of a cell with lowest corner (i,j,k): its lower endpoint is
+// (i+dx, j+dy, k+dz) and it runs along `axis`.  Derived from the vertex pairs
+// of MarchingCubes.h:131-135 and the corner order of
+// TopologyUniform.h:139-159; the lower endpoint is always the smaller point
+// id (MarchingCubes.h:148-157).  Packed as dx | dy<<1 | dz<<2 | axis<<3.
+__host__ __device__ constexpr uint8_t edge_code(int e) {
+  return (uint8_t)(e == 0 ? (0 | 0 << 1 | 0 << 2 | 0 << 3)
+                 : e == 1 ? (1 | 0 << 1 | 0 << 2 | 1 << 3)
+                 : e == 2 ? (0 | 1 << 1 | 0 << 2 | 0 << 3)
+                 : e == 3 ? (0 | 0 << 1 | 0 << 2 | 1 << 3)
+                 : e == 4 ? (0 | 0 << 1 | 1 << 2 | 0 << 3)
+                 : e == 5 ? (1 | 0 << 1 | 1 << 2 | 1 << 3)
+                 : e == 6 ? (0 | 1 << 1 | 1 << 2 | 0 << 3)
+                 : e == 7 ? (0 | 0 << 1 | 1 << 2 | 1 << 3)
+                 : e == 8 ? (0 | 0 << 1 | 0 << 2 | 2 << 3)
+                 : e == 9 ? (1 | 0 << 1 | 0 << 2 | 2 << 3)
+                 : e == 10 ? (1 | 1 << 1 | 0 << 2 | 2 << 3)
+                           : (0 | 1 << 1 | 0 << 2 | 2 << 3));
+}
+
+__device__ __forceinline__ uint64_t shr1(uint64_t a, uint64_t a_next) {
+  return (a >> 1) | (a_next << 63);
+}
+
+// Mask of the first n bits (n in [0,64]).
+__device__ __forceinline__ uint64_t low_mask(int n) {
+  return n >= 64 ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
+}
+
+__device__ __forceinline__ int popc(uint64_t x) { return __popcll(x); }
+
+// Per-word geometry: row coordinates and validity masks.
+struct WordPos {
+  int q, j, k;
+  bool has_y, has_z;   // a +y / +z neighbour row exists
+  uint64_t pmask;      // bits that are grid points
+  uint64_t xmask;      // bits whose +x neighbour exists (x-edges, cells in x)
+};
+
+__device__ __forceinline__ WordPos word_pos(const VolGeom& g, int64_t lw) {
+  WordPos p;
+  const int64_t row = lw / g.wpr;
+  p.q = (int)(lw - row * g.wpr);
+  p.j = (int)(row % g.ny);
+  p.k = (int)(row / g.ny);
+  p.has_y = p.j + 1 < g.ny;
+  p.has_z = p.k + 1 < g.nz;
+  const int x0 = p.q * 64;
+  p.pmask = low_mask(g.nx - x0);
+  p.xmask = low_mask(g.nx - 1 - x0);
+  return p;
+}
+
+// Active-edge flags of a word from its row word A, the next word A1 in x,
+// and the rows B (+y) and C (+z).
+__device__ __forceinline__ void edge_flags(const WordPos& p, uint64_t A, uint64_t A1, uint64_t B,
+                                           uint64_t C, uint64_t& fx, uint64_t& fy, uint64_t& fz) {
+  fx = (A ^ shr1(A, A1)) & p.xmask;
+  fy = p.has_y ? ((A ^ B) & p.pmask) : 0ull;
+  fz = p.has_z ? ((A ^ C) & p.pmask) : 0ull;
+}
+
+// The eight corner bit-planes of the 64 cells of a word (corner order of
+// TopologyUniform.h:139-159) and the mask of cells with mixed corners.
+struct CellPlanes {
+  uint64_t c[8];
+  uint64_t active;
+};
+
+__device__ __forceinline__ void cell_planes(const WordPos& p, uint64_t A, uint64_t A1, uint64_t B,
+                                            uint64_t B1, uint64_t C, uint64_t C1, uint64_t D,
+                                            uint64_t D1, CellPlanes& cp) {
+  cp.c[0] = A;
+  cp.c[1] = shr1(A, A1);
+  cp.c[2] = shr1(B, B1);
+  cp.c[3] = B;
+  cp.c[4] = C;
+  cp.c[5] = shr1(C, C1);
+  cp.c[6] = shr1(D, D1);
+  cp.c[7] = D;
+  uint64_t all = ~0ull, any = 0ull;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    all &= cp.c[v];
+    any |= cp.c[v];
+  }
+  cp.active = (p.has_y && p.has_z) ? ((any & ~all) & p.xmask) : 0ull;
+}
+
+__device__ __forceinline__ int cell_case(const CellPlanes& cp, int i) {
+  int c = 0;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) c |= (int)((cp.c[v] >> i) & 1ull) << v;
+  return c;
+}
+
+// Rank of the edge (bit i, axis a) among the active edges of a word.
+__device__ __forceinline__ int edge_rank(uint64_t fx, uint64_t fy, uint64_t fz, int i, int a) {
+  const uint64_t lo = (1ull << i) - 1ull;
+  int r = popc(fx & lo) + popc(fy & lo) + popc(fz & lo);
+  if (a > 0) r += (int)((fx >> i) & 1ull);
+  if (a > 1) r += (int)((fy >> i) & 1ull);
+  return r;
+}
+
+// Inverse of edge_rank: the k-th active edge of a word (k < count).
+__device__ __forceinline__ void edge_select(uint64_t fx, uint64_t fy, uint64_t fz, int k, int& i,
+                                            int& a) {
+  // largest i with (#edges at bits < i) <= k
+  int lo = 0, hi = 63;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const uint64_t m = (1ull << mid) - 1ull;
+    const int c = popc(fx & m) + popc(fy & m) + popc(fz & m);
+    if (c <= k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  i = lo;
+  const uint64_t m = (i == 0) ? 0ull : ((1ull << i) - 1ull);
+  int r = k - (popc(fx & m) + popc(fy & m) + popc(fz & m));
+  const int bx = (int)((fx >> i) & 1ull), by = (int)((fy >> i) & 1ull);
+  if (bx) {
+    if (r == 0) {
+      a = 0;
+      return;
+    }
+    --r;
+  }
+  if (by) {
+    if (r == 0) {
+      a = 1;
+      return;
+    }
+    --r;
+  }
+  a = 2;
+}
+
+// Decoupled look-back tile status: flag in bits 62-63 (0 = not ready,
+// 1 = aggregate, 2 = inclusive prefix), triangle count in bits 31-61, vertex
+// count in bits 0-30.  Totals above 2^31-1 overflow dax::Id anyway; the
+// 64-bit totals accumulated beside the scan detect that case.
+constexpr uint64_t kStatusAgg = 1ull;
+constexpr uint64_t kStatusPrefix = 2ull;
+
+__device__ __forceinline__ uint64_t pack_status(uint32_t v, uint32_t t, uint64_t flag) {
+  return (flag << 62) | ((uint64_t)(t & 0x7fffffffu) << 31) | (uint64_t)(v & 0x7fffffffu);
+}
+
+// Control block at the end of the workspace.
+struct Ctrl {
+  uint32_t tile_counter;
+  uint32_t pad;
+  unsigned long long total_v;  // 64-bit totals: vertices (active edges), triangles
+  unsigned long long total_t;
+};
+
+// Workspace layout for a word space of `nwords` row words.
+struct Workspace {
+  size_t off_bits, off_base_v, off_base_t, off_status, off_ctrl, bytes;
+  int64_t nwords, ntiles;
+};
+
+inline Workspace workspace_layout(int64_t nwords) {
+  Workspace w;
+  w.nwords = nwords;
+  w.ntiles = ceil_div(nwords, kTileWords);
+  size_t off = 0;
+  w.off_bits = off;
+  off = align_up(off + (size_t)(nwords + 1) * 8, 256);
+  w.off_base_v = off;
+  off = align_up(off + (size_t)(nwords + 1) * 4, 256);
+  w.off_base_t = off;
+  off = align_up(off + (size_t)(nwords + 1) * 4, 256);
+  w.off_status = off;  // status then ctrl: one contiguous block zeroed per run
+  off = off + (size_t)w.ntiles * 8;
+  w.off_ctrl = off;
+  off = align_up(off + sizeof(Ctrl), 256);
+  w.bytes = off;
+  return w;
+}
+
+}  // namespace mc
+}  // namespace daxhip

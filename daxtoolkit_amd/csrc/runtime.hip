@@ -1,0 +1,146 @@
+// Runtime part of the C-ABI: errors, memory (the ArrayManagerExecution
+// backend), synchronisation and the event timer.
+#include <stdarg.h>
+
+#include "common.h"
+
+namespace daxhip {
+
+static thread_local char g_last_error[1024] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+}
+
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return DAX_HIP_OK;
+  // clear the sticky error so later calls start clean
+  // (as ArrayManagerExecutionCuda.h:46-84 does for out-of-memory)
+  (void)hipGetLastError();
+  set_error("%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+  return e == hipErrorOutOfMemory ? DAX_HIP_OOM : DAX_HIP_EXEC_ERROR;
+}
+
+}  // namespace daxhip
+
+using namespace daxhip;
+
+namespace {
+struct Timer {
+  hipEvent_t start, stop;
+};
+}  // namespace
+
+extern "C" {
+
+const char* dax_hip_last_error(void) { return g_last_error; }
+
+const char* dax_hip_version(void) { return "daxhip 0.1 gfx950"; }
+
+int dax_hip_device_count(int* h_count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    *h_count = 0;
+    return check_hip(e, "hipGetDeviceCount");
+  }
+  *h_count = n;
+  return DAX_HIP_OK;
+}
+
+int dax_hip_malloc(void** d_ptr, size_t bytes) {
+  if (!d_ptr) {
+    set_error("dax_hip_malloc: NULL output pointer");
+    return DAX_HIP_BAD_VALUE;
+  }
+  *d_ptr = nullptr;
+  if (bytes == 0) return DAX_HIP_OK;
+  hipError_t e = hipMalloc(d_ptr, bytes);
+  if (e != hipSuccess) {
+    *d_ptr = nullptr;
+    int rc = check_hip(e, "hipMalloc");
+    if (rc == DAX_HIP_OOM)
+      set_error("Failed to allocate execution array with %zu bytes.", bytes);
+    return rc;
+  }
+  return DAX_HIP_OK;
+}
+
+int dax_hip_free(void* d_ptr) {
+  if (!d_ptr) return DAX_HIP_OK;
+  DAXHIP_CHECK(hipFree(d_ptr));
+  return DAX_HIP_OK;
+}
+
+int dax_hip_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes, void* stream) {
+  if (bytes == 0) return DAX_HIP_OK;
+  DAXHIP_CHECK(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  return DAX_HIP_OK;
+}
+
+int dax_hip_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes, void* stream) {
+  if (bytes == 0) return DAX_HIP_OK;
+  DAXHIP_CHECK(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  DAXHIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return DAX_HIP_OK;
+}
+
+int dax_hip_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes, void* stream) {
+  if (bytes == 0) return DAX_HIP_OK;
+  DAXHIP_CHECK(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return DAX_HIP_OK;
+}
+
+int dax_hip_memset(void* d_dst, int value, size_t bytes, void* stream) {
+  if (bytes == 0) return DAX_HIP_OK;
+  DAXHIP_CHECK(hipMemsetAsync(d_dst, value, bytes, (hipStream_t)stream));
+  return DAX_HIP_OK;
+}
+
+int dax_hip_synchronize(void* stream) {
+  DAXHIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return DAX_HIP_OK;
+}
+
+int dax_hip_timer_create(void** h_timer) {
+  Timer* t = new Timer();
+  hipError_t e = hipEventCreate(&t->start);
+  if (e == hipSuccess) e = hipEventCreate(&t->stop);
+  if (e != hipSuccess) {
+    delete t;
+    *h_timer = nullptr;
+    return check_hip(e, "hipEventCreate");
+  }
+  *h_timer = t;
+  return DAX_HIP_OK;
+}
+
+int dax_hip_timer_destroy(void* timer) {
+  Timer* t = (Timer*)timer;
+  if (!t) return DAX_HIP_OK;
+  (void)hipEventDestroy(t->start);
+  (void)hipEventDestroy(t->stop);
+  delete t;
+  return DAX_HIP_OK;
+}
+
+int dax_hip_timer_reset(void* timer, void* stream) {
+  Timer* t = (Timer*)timer;
+  DAXHIP_CHECK(hipEventRecord(t->start, (hipStream_t)stream));
+  return DAX_HIP_OK;
+}
+
+int dax_hip_timer_elapsed(void* timer, void* stream, double* h_seconds) {
+  Timer* t = (Timer*)timer;
+  DAXHIP_CHECK(hipEventRecord(t->stop, (hipStream_t)stream));
+  DAXHIP_CHECK(hipEventSynchronize(t->stop));
+  float ms = 0.0f;
+  DAXHIP_CHECK(hipEventElapsedTime(&ms, t->start, t->stop));
+  *h_seconds = (double)ms / 1000.0;
+  return DAX_HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+"""Synthetic scalar fields of the benchmark workloads (seeded, reproducible).
+
+All fields are float32 arrays of shape (nz, ny, nx) with point (i, j, k) at
+coordinates origin + spacing*(i, j, k).  Parameters are drawn on the host from
+numpy's PCG64 ``default_rng(seed)``; the values are computed in float64 and
+rounded once to float32.  Device generation (torch) and host generation
+(numpy) follow the same formulas; tests that compare two implementations
+always feed both the same float32 bits.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+def sphere_distance(n: int = 128, center: float = 63.5) -> np.ndarray:
+    """|p - (c, c, c)| on an n^3 grid, spacing 1, origin 0 (float64 -> float32)."""
+    ax = np.arange(n, dtype=np.float64) - center
+    z, y, x = np.meshgrid(ax, ax, ax, indexing="ij")
+    return np.sqrt(x * x + y * y + z * z).astype(np.float32)
+
+
+def linear(nx: int, ny: int, nz: int, g=(1.0, 1.0, 1.0)) -> np.ndarray:
+    """dot(p, g): the reference's MC test field (x+y+z for g=(1,1,1))."""
+    x = np.arange(nx, dtype=np.float64)
+    y = np.arange(ny, dtype=np.float64)
+    z = np.arange(nz, dtype=np.float64)
+    return (g[0] * x[None, None, :] + g[1] * y[None, :, None] + g[2] * z[:, None, None]).astype(
+        np.float32)
+
+
+@dataclass
+class GaussianMixture:
+    centers: np.ndarray    # [G, 3] (x, y, z) in voxels
+    sigmas: np.ndarray     # [G]
+    amps: np.ndarray       # [G]
+    noise_seed: int
+    noise: float = 0.01
+
+
+def gaussian_mixture_params(seed: int, dims, n_gauss: int = 16, sigma=(20.0, 60.0),
+                            amp=(0.5, 1.5), noise: float = 0.01) -> GaussianMixture:
+    """Centres uniform in the box [0, n-1] per axis, sigma ~ U[20,60] voxels,
+    amplitude ~ U[0.5,1.5], plus uniform noise in [-noise, noise]."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(seed)
+    hi = np.array([nx - 1, ny - 1, nz - 1], dtype=np.float64)
+    centers = rng.uniform(0.0, 1.0, size=(n_gauss, 3)) * hi
+    sigmas = rng.uniform(sigma[0], sigma[1], size=n_gauss)
+    amps = rng.uniform(amp[0], amp[1], size=n_gauss)
+    noise_seed = int(rng.integers(0, 2**31 - 1))
+    return GaussianMixture(centers, sigmas, amps, noise_seed, noise)
+
+
+def gaussian_mixture_numpy(dims, p: GaussianMixture) -> np.ndarray:
+    nx, ny, nz = dims
+    x = np.arange(nx, dtype=np.float64)
+    y = np.arange(ny, dtype=np.float64)
+    z = np.arange(nz, dtype=np.float64)
+    f = np.zeros((nz, ny, nx), dtype=np.float64)
+    for c, s, a in zip(p.centers, p.sigmas, p.amps):
+        k = -0.5 / (s * s)
+        ex = np.exp(k * (x - c[0]) ** 2)
+        ey = np.exp(k * (y - c[1]) ** 2)
+        ez = a * np.exp(k * (z - c[2]) ** 2)
+        f += ez[:, None, None] * ey[None, :, None] * ex[None, None, :]
+    rng = np.random.default_rng(p.noise_seed)
+    f += rng.uniform(-p.noise, p.noise, size=f.shape)
+    return f.astype(np.float32)
+
+
+def gaussian_mixture_torch(dims, p: GaussianMixture, device="cuda"):
+    """Same mixture generated on the device (noise from torch's generator)."""
+    import torch
+    nx, ny, nz = dims
+    dev = torch.device(device)
+    x = torch.arange(nx, dtype=torch.float64, device=dev)
+    y = torch.arange(ny, dtype=torch.float64, device=dev)
+    f = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(p.noise_seed)
+    exs = [torch.exp((-0.5 / (s * s)) * (x - c[0]) ** 2) for c, s in zip(p.centers, p.sigmas)]
+    eys = [torch.exp((-0.5 / (s * s)) * (y - c[1]) ** 2) for c, s in zip(p.centers, p.sigmas)]
+    chunk = max(1, (1 << 26) // (nx * ny))
+    for z0 in range(0, nz, chunk):
+        z1 = min(nz, z0 + chunk)
+        z = torch.arange(z0, z1, dtype=torch.float64, device=dev)
+        acc = torch.zeros((z1 - z0, ny, nx), dtype=torch.float64, device=dev)
+        for c, s, a, ex, ey in zip(p.centers, p.sigmas, p.amps, exs, eys):
+            ez = a * torch.exp((-0.5 / (s * s)) * (z - c[2]) ** 2)
+            acc += ez[:, None, None] * (ey[:, None] * ex[None, :])[None]
+        noise = torch.rand((z1 - z0, ny, nx), dtype=torch.float64, device=dev, generator=gen)
+        acc += (2.0 * noise - 1.0) * p.noise
+        f[z0:z1] = acc.to(torch.float32)
+    return f
+
+
+def mandelbulb_torch(n: int, power: float = 8.0, max_iter: int = 10, bailout: float = 2.0,
+                     extent: float = 1.2, device="cuda"):
+    """Escape-time Mandelbulb (White-Nylander, z0 = c) over [-extent, extent]^3.
+
+    z <- z^power + c in spherical form (r^p, p*theta, p*phi with
+    theta = acos(z_z/r), phi = atan2(z_y, z_x)); the value is the number of
+    iterations done while |z| <= bailout, at most max_iter, divided by
+    max_iter.  Point (i, j, k) is c = -extent + 2*extent*(i, j, k)/(n-1).
+    """
+    import torch
+    dev = torch.device(device)
+    out = torch.empty((n, n, n), dtype=torch.float32, device=dev)
+    ax = torch.linspace(-extent, extent, n, dtype=torch.float64, device=dev)
+    chunk = max(1, (1 << 24) // (n * n))
+    for z0 in range(0, n, chunk):
+        z1 = min(n, z0 + chunk)
+        cz, cy, cx = torch.meshgrid(ax[z0:z1], ax, ax, indexing="ij")
+        zx, zy, zz = cx.clone(), cy.clone(), cz.clone()
+        it = torch.zeros_like(cx)
+        alive = torch.ones_like(cx, dtype=torch.bool)
+        for _ in range(max_iter):
+            r = torch.sqrt(zx * zx + zy * zy + zz * zz)
+            alive = alive & (r <= bailout)
+            it += alive.to(torch.float64)
+            theta = torch.acos(torch.clamp(zz / torch.clamp(r, min=1e-300), -1.0, 1.0))
+            phi = torch.atan2(zy, zx)
+            rp = r ** power
+            nzx = rp * torch.sin(power * theta) * torch.cos(power * phi) + cx
+            nzy = rp * torch.sin(power * theta) * torch.sin(power * phi) + cy
+            nzz = rp * torch.cos(power * theta) + cz
+            zx = torch.where(alive, nzx, zx)
+            zy = torch.where(alive, nzy, zy)
+            zz = torch.where(alive, nzz, zz)
+        out[z0:z1] = (it / max_iter).to(torch.float32)
+    return out

@@ -1,0 +1,219 @@
+"""Python driver of the marching-cubes C-ABI over torch device tensors.
+
+torch only provides device memory and the stream; all computation is in
+libdaxhip.so.  Field tensors are float32, C-contiguous, shape (nz, ny, nx) on
+a HIP device, so the flat point index is i + nx*(j + ny*k) as in Dax
+(dax/Extent.h:104-112).
+
+The call sequence mirrors the reference's two dispatchers
+(Benchmarks/MarchingCubes/Pipeline.h:115-130):
+``DispatcherMapCell<MarchingCubesCount>`` -> :func:`classify` and
+``DispatcherGenerateInterpolatedCells<MarchingCubesGenerate>`` ->
+:func:`forward`.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import torch
+
+from . import _lib
+from ._lib import MC_DEDUP, MC_NORMALS
+
+
+def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _f3(v: Sequence[float]):
+    import ctypes
+    return (ctypes.c_float * 3)(*[float(x) for x in v])
+
+
+def _check_field(field: torch.Tensor) -> tuple[int, int, int]:
+    if field.dim() != 3:
+        raise ValueError("field must have shape (nz, ny, nx)")
+    if field.dtype != torch.float32 or not field.is_cuda or not field.is_contiguous():
+        raise ValueError("field must be a contiguous float32 tensor on a HIP device")
+    nz, ny, nx = field.shape
+    return int(nx), int(ny), int(nz)
+
+
+def workspace_bytes(nx: int, ny: int, nz: int) -> int:
+    return int(_lib.load().dax_hip_mc_workspace_bytes(nx, ny, nz))
+
+
+def new_workspace(nx: int, ny: int, nz: int, device=None) -> torch.Tensor:
+    return torch.empty(workspace_bytes(nx, ny, nz), dtype=torch.uint8, device=device or "cuda")
+
+
+@dataclass
+class Contour:
+    """Output of one forward: an UnstructuredGrid<CellTagTriangle> plus extras."""
+    vertices: torch.Tensor            # [U, 3] f32
+    connectivity: torch.Tensor        # [T, 3] i32
+    normals: Optional[torch.Tensor]   # [U, 3] f32
+    edges: Optional[torch.Tensor]     # [U] u32 as int32 bits: (axis << 30) | lower point id
+    interp_first: Optional[torch.Tensor]
+    interp_second: Optional[torch.Tensor]
+    interp_weight: Optional[torch.Tensor]
+
+    @property
+    def num_triangles(self) -> int:
+        return int(self.connectivity.shape[0])
+
+    @property
+    def num_points(self) -> int:
+        return int(self.vertices.shape[0])
+
+
+def classify(field: torch.Tensor, iso: float, workspace: Optional[torch.Tensor] = None,
+             counts: Optional[torch.Tensor] = None):
+    """Classify + count/scan.  Returns (workspace, counts[2] = {T, U})."""
+    nx, ny, nz = _check_field(field)
+    ws = workspace if workspace is not None else new_workspace(nx, ny, nz, field.device)
+    cnt = counts if counts is not None else torch.empty(2, dtype=torch.int64, device=field.device)
+    _lib.call("dax_hip_mc_classify", field.data_ptr(), nx, ny, nz, float(iso), ws.data_ptr(),
+              ws.numel(), cnt.data_ptr(), _stream())
+    return ws, cnt
+
+
+def read_counts(counts: torch.Tensor, cap_points: int = 2**62, cap_tris: int = 2**62):
+    import ctypes
+    t, u = ctypes.c_int64(), ctypes.c_int64()
+    _lib.call("dax_hip_mc_read_counts", counts.data_ptr(), cap_points, cap_tris,
+              ctypes.byref(t), ctypes.byref(u), _stream())
+    return int(t.value), int(u.value)
+
+
+def cell_outputs(field: torch.Tensor, workspace: torch.Tensor):
+    """Per-cell case (u8), NumFaces (i32) and exclusive triangle offset (i32)."""
+    nx, ny, nz = _check_field(field)
+    C = max(nx - 1, 0) * max(ny - 1, 0) * max(nz - 1, 0)
+    dev = field.device
+    cases = torch.empty(C, dtype=torch.uint8, device=dev)
+    count = torch.empty(C, dtype=torch.int32, device=dev)
+    offs = torch.empty(C, dtype=torch.int32, device=dev)
+    _lib.call("dax_hip_mc_cell_outputs", nx, ny, nz, workspace.data_ptr(), workspace.numel(),
+              cases.data_ptr(), count.data_ptr(), offs.data_ptr(), _stream())
+    return cases, count, offs
+
+
+class ForwardBuffers:
+    """Preallocated outputs of :func:`forward_into` (for timed, graph-friendly runs)."""
+
+    def __init__(self, nx, ny, nz, cap_points, cap_tris, device="cuda", normals=True,
+                 edges=False, interp=False):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.cap_points, self.cap_tris = int(cap_points), int(cap_tris)
+        self.workspace = new_workspace(nx, ny, nz, device)
+        self.vertices = torch.empty((self.cap_points, 3), dtype=torch.float32, device=device)
+        self.normals = (torch.empty((self.cap_points, 3), dtype=torch.float32, device=device)
+                        if normals else None)
+        self.conn = torch.empty((self.cap_tris, 3), dtype=torch.int32, device=device)
+        self.edges = torch.empty(self.cap_points, dtype=torch.int32, device=device) if edges else None
+        if interp:
+            self.interp_first = torch.empty(self.cap_points, dtype=torch.int32, device=device)
+            self.interp_second = torch.empty(self.cap_points, dtype=torch.int32, device=device)
+            self.interp_weight = torch.empty(self.cap_points, dtype=torch.float32, device=device)
+        else:
+            self.interp_first = self.interp_second = self.interp_weight = None
+        self.counts = torch.zeros(2, dtype=torch.int64, device=device)
+
+
+def forward_into(field: torch.Tensor, iso: float, buf: ForwardBuffers, origin=(0, 0, 0),
+                 spacing=(1, 1, 1), dedup: bool = True, normals: bool = True) -> None:
+    """Enqueue one forward into preallocated buffers (no host synchronisation)."""
+    nx, ny, nz = _check_field(field)
+    flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and buf.normals is not None) else 0)
+    _lib.call("dax_hip_mc_forward", field.data_ptr(), nx, ny, nz, _f3(origin), _f3(spacing),
+              float(iso), flags, buf.workspace.data_ptr(), buf.workspace.numel(),
+              buf.vertices.data_ptr(), _ptr(buf.normals), buf.conn.data_ptr(),
+              _ptr(buf.edges) if dedup else None, _ptr(buf.interp_first),
+              _ptr(buf.interp_second), _ptr(buf.interp_weight), buf.cap_points, buf.cap_tris,
+              buf.counts.data_ptr(), _stream())
+
+
+def forward_phases(field: torch.Tensor, iso: float, buf: ForwardBuffers, origin=(0, 0, 0),
+                   spacing=(1, 1, 1), dedup: bool = True, normals: bool = True,
+                   events=None) -> None:
+    """The same forward as three calls (one kernel each); if `events` (four
+    torch.cuda.Event) is given they are recorded before, between and after
+    the kernels on the current stream."""
+    nx, ny, nz = _check_field(field)
+    flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and buf.normals is not None) else 0)
+    ws = buf.workspace
+    s = _stream()
+    if events:
+        events[0].record()
+    _lib.call("dax_hip_mc_bits", field.data_ptr(), nx, ny, nz, float(iso), ws.data_ptr(),
+              ws.numel(), s)
+    if events:
+        events[1].record()
+    _lib.call("dax_hip_mc_scan", nx, ny, nz, ws.data_ptr(), ws.numel(), s)
+    if events:
+        events[2].record()
+    _lib.call("dax_hip_mc_generate", field.data_ptr(), nx, ny, nz, _f3(origin), _f3(spacing),
+              float(iso), flags, ws.data_ptr(), ws.numel(), buf.vertices.data_ptr(),
+              _ptr(buf.normals), buf.conn.data_ptr(), _ptr(buf.edges) if dedup else None,
+              _ptr(buf.interp_first), _ptr(buf.interp_second), _ptr(buf.interp_weight),
+              buf.cap_points, buf.cap_tris, buf.counts.data_ptr(), s)
+    if events:
+        events[3].record()
+
+
+def forward(field: torch.Tensor, iso: float, origin=(0, 0, 0), spacing=(1, 1, 1),
+            dedup: bool = True, normals: bool = True, edges: bool = False,
+            interp: bool = False) -> Contour:
+    """Classify, size the outputs, then run the full forward."""
+    nx, ny, nz = _check_field(field)
+    ws, cnt = classify(field, iso)
+    T, U = read_counts(cnt)
+    if not dedup:
+        U = 3 * T
+    buf = ForwardBuffers(nx, ny, nz, max(U, 1), max(T, 1), field.device, normals=normals,
+                         edges=edges and dedup, interp=interp)
+    forward_into(field, iso, buf, origin, spacing, dedup, normals)
+    T2, U2 = read_counts(buf.counts, buf.cap_points, buf.cap_tris)
+    assert (T2, U2) == (T, U), (T2, U2, T, U)
+    return Contour(
+        vertices=buf.vertices[:U], connectivity=buf.conn[:T],
+        normals=None if buf.normals is None else buf.normals[:U],
+        edges=None if buf.edges is None else buf.edges[:U],
+        interp_first=None if buf.interp_first is None else buf.interp_first[:U],
+        interp_second=None if buf.interp_second is None else buf.interp_second[:U],
+        interp_weight=None if buf.interp_weight is None else buf.interp_weight[:U])
+
+
+def interpolate_scalar(values: torch.Tensor, contour: Contour) -> torch.Tensor:
+    """CompactPointField of a scalar point field onto the contour points."""
+    if contour.interp_first is None:
+        raise ValueError("forward(..., interp=True) is needed to interpolate point fields")
+    n = contour.num_points
+    out = torch.empty(n, dtype=torch.float32, device=values.device)
+    _lib.call("dax_hip_mc_interpolate_scalar", values.data_ptr(), contour.interp_first.data_ptr(),
+              contour.interp_second.data_ptr(), contour.interp_weight.data_ptr(), n,
+              out.data_ptr(), _stream())
+    return out
+
+
+def point_gradient(field: torch.Tensor, spacing=(1, 1, 1)) -> torch.Tensor:
+    nx, ny, nz = _check_field(field)
+    out = torch.empty((nx * ny * nz, 3), dtype=torch.float32, device=field.device)
+    _lib.call("dax_hip_point_gradient", field.data_ptr(), nx, ny, nz, _f3(spacing),
+              out.data_ptr(), _stream())
+    return out
+
+
+def cell_gradient(field: torch.Tensor, origin=(0, 0, 0), spacing=(1, 1, 1)) -> torch.Tensor:
+    nx, ny, nz = _check_field(field)
+    C = max(nx - 1, 0) * max(ny - 1, 0) * max(nz - 1, 0)
+    out = torch.empty((C, 3), dtype=torch.float32, device=field.device)
+    _lib.call("dax_hip_cell_gradient", field.data_ptr(), nx, ny, nz, _f3(origin), _f3(spacing),
+              out.data_ptr(), _stream())
+    return out

@@ -1,0 +1,696 @@
+// ORACLE -- test infrastructure only.
+//
+// CPU restatement of Dax's Serial device adapter on the marching-cubes
+// isosurface path (classify -> inclusive scan -> upper bounds -> visit index
+// -> generate -> sort/unique/lower-bounds dedup -> coordinate interpolation),
+// plus Dax's CellGradient worklet, the central-difference point normals and an
+// FP64 adjoint / finite-difference checker for the backward pass.  Only the
+// test-suite, the smoke check and the CPU-baseline leg of the benchmark load
+// this library; the product never does.
+//
+// Every function names the reference lines it follows.  The FP32 operation
+// order of the reference is kept (weight, Lerp, CellGradient sum), and the file
+// is compiled with -ffp-contract=off so no multiply-add is fused.
+//
+// Pinning (what checks each function; see tests/test_oracle.py):
+//   tables            SHA-256 of the expanded TriTable/NumFaces bytes
+//   oracle_mc_count   brute force over all 256 corner sign patterns; survey
+//                     cross-check values for the 128^3 sphere
+//   scans / bounds    the reference conformance suite's golden vectors
+//                     (TestingDeviceAdapter.h:621-717, 940-1001)
+//   oracle_mc_generate  UnitTestWorkletMarchingCubes.cxx (26^3, x+y+z, iso 70:
+//                     3T points without merge, 45 with merge, secondary field
+//                     interpolated exactly); closed-form plane intersections;
+//                     dedup == 3T-point set with duplicates removed
+//   oracle_point_gradient / normals   exact on linear and quadratic fields,
+//                     sphere normals vs analytic (p-c)/|p-c|
+//   oracle_cell_gradient  linear fields give the exact gradient
+//                     (UnitTestWorkletCellGradient.cxx)
+//   oracle_mc_backward_f64  oracle_mc_fd_f64 central differences, and the
+//                     torch float64 autograd restatement in the tests
+//   oracle_mc_pipeline_omp  equality with the serial oracle
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#include <parallel/algorithm>
+#endif
+
+#include "mc_tables_oracle.h"
+
+namespace {
+
+typedef int32_t Id;      // dax::Id (dax/internal/Configure.h.in:67-75, 32-bit default)
+typedef float Scalar;    // dax::Scalar (dax/Types.h:258-268)
+
+struct Tables {
+  uint8_t num_faces[256];
+  uint8_t tri[256][16];
+  Tables() {
+    for (int c = 0; c < 256; ++c) {
+      const char* s = kOracleCaseEdges[c];
+      int n = (int)std::strlen(s);
+      num_faces[c] = (uint8_t)(n / 3);
+      for (int k = 0; k < 16; ++k) {
+        if (k < n) {
+          char ch = s[k];
+          tri[c][k] = (uint8_t)(ch <= '9' ? ch - '0' : ch - 'a' + 10);
+        } else {
+          tri[c][k] = 255;
+        }
+      }
+    }
+  }
+};
+const Tables& tables() {
+  static Tables t;
+  return t;
+}
+
+// dax/worklet/MarchingCubes.h:131-135 -- cell vertex pairs of the 12 edges.
+const unsigned char kCellVertEdges[12][2] = {
+    {0, 1}, {1, 2}, {3, 2}, {0, 3}, {4, 5}, {5, 6},
+    {7, 6}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+struct Grid {
+  int64_t nx, ny, nz;
+  float origin[3];
+  float spacing[3];
+  int64_t points() const { return nx * ny * nz; }
+  int64_t cx() const { return nx > 1 ? nx - 1 : 0; }
+  int64_t cy() const { return ny > 1 ? ny - 1 : 0; }
+  int64_t cz() const { return nz > 1 ? nz - 1 : 0; }
+  int64_t cells() const { return cx() * cy() * cz(); }
+};
+
+Grid make_grid(const int64_t dims[3], const float* origin, const float* spacing) {
+  Grid g;
+  g.nx = dims[0];
+  g.ny = dims[1];
+  g.nz = dims[2];
+  for (int d = 0; d < 3; ++d) {
+    g.origin[d] = origin ? origin[d] : 0.0f;
+    g.spacing[d] = spacing ? spacing[d] : 1.0f;
+  }
+  return g;
+}
+
+// dax/exec/internal/TopologyUniform.h:139-159 with the flat cell index
+// decomposed as in dax/Extent.h:127-137 (x fastest).
+void cell_vertices(const Grid& g, int64_t cell, Id verts[8]) {
+  const int64_t cx = g.cx(), cy = g.cy();
+  const int64_t i = cell % cx, j = (cell / cx) % cy, k = cell / (cx * cy);
+  const Id first = (Id)(i + g.nx * (j + g.ny * k));
+  const Id second = (Id)(first + g.nx * g.ny);
+  const Id xdim = (Id)g.nx;
+  verts[0] = first;
+  verts[1] = first + 1;
+  verts[2] = first + xdim + 1;
+  verts[3] = first + xdim;
+  verts[4] = second;
+  verts[5] = second + 1;
+  verts[6] = second + xdim + 1;
+  verts[7] = second + xdim;
+}
+
+// dax/worklet/MarchingCubes.h:36-48 (strict '>').
+int classification(Scalar iso, const Scalar v[8]) {
+  return ((v[0] > iso) << 0 | (v[1] > iso) << 1 | (v[2] > iso) << 2 |
+          (v[3] > iso) << 3 | (v[4] > iso) << 4 | (v[5] > iso) << 5 |
+          (v[6] > iso) << 6 | (v[7] > iso) << 7);
+}
+
+// dax/cont/UniformGrid.h:82-89 (origin + spacing*ijk, Id converted to Scalar).
+void point_coords(const Grid& g, Id p, Scalar out[3]) {
+  const int64_t i = p % g.nx, j = (p / g.nx) % g.ny, k = p / (g.nx * g.ny);
+  out[0] = g.origin[0] + g.spacing[0] * (Scalar)i;
+  out[1] = g.origin[1] + g.spacing[1] * (Scalar)j;
+  out[2] = g.origin[2] + g.spacing[2] * (Scalar)k;
+}
+
+// dax/cont/internal/EdgeInterpolatedGrid.h:28-74
+struct PointAsEdgeInterpolation {
+  Id first, second;
+  Scalar weight;
+  bool operator==(const PointAsEdgeInterpolation& o) const {
+    return first == o.first && second == o.second;
+  }
+  bool operator<(const PointAsEdgeInterpolation& o) const {
+    return first < o.first || (first == o.first && second < o.second);
+  }
+};
+
+// dax/worklet/MarchingCubes.h:122-168 -- one output triangle of one cell.
+void build_triangle(const Grid& g, const Scalar* field, Scalar iso, int64_t cell,
+                    int visit, PointAsEdgeInterpolation out[3]) {
+  const Tables& T = tables();
+  Id verts[8];
+  cell_vertices(g, cell, verts);
+  Scalar values[8];
+  for (int v = 0; v < 8; ++v) values[v] = field[verts[v]];
+  const int voxel_class = classification(iso, values);
+  for (int ov = 0; ov < 3; ++ov) {
+    const unsigned char edge = T.tri[voxel_class][visit * 3 + ov];
+    int a, b;
+    if (verts[kCellVertEdges[edge][0]] < verts[kCellVertEdges[edge][1]]) {
+      a = kCellVertEdges[edge][0];
+      b = kCellVertEdges[edge][1];
+    } else {
+      a = kCellVertEdges[edge][1];
+      b = kCellVertEdges[edge][0];
+    }
+    const Scalar weight = (iso - values[a]) / (values[b] - values[a]);
+    out[ov].first = verts[a];
+    out[ov].second = verts[b];
+    out[ov].weight = weight;
+  }
+}
+
+// dax/math/VectorAnalysis.h:33-49,60-74 -- Lerp = a + w*(b-a), per component.
+inline Scalar lerp(Scalar a, Scalar b, Scalar w) { return a + w * (b - a); }
+
+// Central-difference point gradient (no reference counterpart).  Interior:
+// (f(p+e)-f(p-e))/(2h); first plane (f(p+e)-f(p))/h; last plane
+// (f(p)-f(p-e))/h; an axis with one point has zero derivative.
+void point_gradient_at(const Grid& g, const Scalar* f, int64_t p, Scalar out[3]) {
+  const int64_t idx[3] = {p % g.nx, (p / g.nx) % g.ny, p / (g.nx * g.ny)};
+  const int64_t n[3] = {g.nx, g.ny, g.nz};
+  const int64_t stride[3] = {1, g.nx, g.nx * g.ny};
+  for (int d = 0; d < 3; ++d) {
+    const Scalar h = g.spacing[d];
+    if (n[d] < 2) {
+      out[d] = 0.0f;
+    } else if (idx[d] == 0) {
+      out[d] = (f[p + stride[d]] - f[p]) / h;
+    } else if (idx[d] == n[d] - 1) {
+      out[d] = (f[p] - f[p - stride[d]]) / h;
+    } else {
+      out[d] = (f[p + stride[d]] - f[p - stride[d]]) / (2.0f * h);
+    }
+  }
+}
+
+// Vertex normal: Lerp of the two endpoint gradients with the vertex weight
+// (the CompactPointField semantics of DispatcherGenerateInterpolatedCells.h:
+// 107-141 applied to the gradient field), then normalised; a zero vector stays
+// zero.
+void normal_from(const Scalar gA[3], const Scalar gB[3], Scalar w, Scalar out[3]) {
+  Scalar m[3];
+  for (int d = 0; d < 3; ++d) m[d] = lerp(gA[d], gB[d], w);
+  const Scalar len2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+  if (len2 > 0.0f) {
+    const Scalar len = std::sqrt(len2);
+    for (int d = 0; d < 3; ++d) out[d] = m[d] / len;
+  } else {
+    out[0] = out[1] = out[2] = 0.0f;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Rebuilds the reference's NumFaces[256] and TriTable[256][16] byte layouts.
+void oracle_tables_expand(uint8_t* num_faces, uint8_t* tri) {
+  const Tables& T = tables();
+  std::memcpy(num_faces, T.num_faces, 256);
+  std::memcpy(tri, T.tri, 4096);
+}
+
+// MarchingCubesCount over every cell (dax/worklet/MarchingCubes.h:53-86),
+// scheduled as the Serial adapter's Id3 triple loop
+// (dax/cont/internal/DeviceAdapterAlgorithmSerial.h:146-176).  `cases` (one
+// byte per cell) is optional; the reference computes but does not store it.
+int oracle_mc_count(const float* field, const int64_t* dims, float iso,
+                    uint8_t* cases, int32_t* counts) {
+  const Grid g = make_grid(dims, nullptr, nullptr);
+  const Tables& T = tables();
+  const int64_t cx = g.cx(), cy = g.cy(), cz = g.cz();
+  int64_t cell = 0;
+  for (int64_t k = 0; k < cz; ++k)
+    for (int64_t j = 0; j < cy; ++j)
+      for (int64_t i = 0; i < cx; ++i, ++cell) {
+        Id verts[8];
+        cell_vertices(g, cell, verts);
+        Scalar values[8];
+        for (int v = 0; v < 8; ++v) values[v] = field[verts[v]];
+        const int c = classification(iso, values);
+        if (cases) cases[cell] = (uint8_t)c;
+        counts[cell] = T.num_faces[c];
+      }
+  return 0;
+}
+
+// ScanInclusive: std::partial_sum, returns the last value
+// (DeviceAdapterAlgorithmSerial.h:46-68).
+int64_t oracle_scan_inclusive_i32(const int32_t* in, int32_t* out, int64_t n) {
+  if (n <= 0) return 0;
+  std::partial_sum(in, in + n, out);
+  return out[n - 1];
+}
+
+// ScanExclusive: partial_sum, shift right, out[0]=0, returns the full sum
+// (DeviceAdapterAlgorithmSerial.h:70-99).
+int64_t oracle_scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n) {
+  if (n <= 0) return 0;
+  std::partial_sum(in, in + n, out);
+  const int32_t full = out[n - 1];
+  std::copy_backward(out, out + n - 1, out + n);
+  out[0] = 0;
+  return full;
+}
+
+// LowerBounds / UpperBounds: std::lower_bound / std::upper_bound per value
+// (DeviceAdapterAlgorithmGeneral.h:258-308, 1044-1092).
+void oracle_lower_bounds_i32(const int32_t* sorted, int64_t n, const int32_t* vals,
+                             int64_t m, int32_t* out) {
+  for (int64_t q = 0; q < m; ++q)
+    out[q] = (int32_t)(std::lower_bound(sorted, sorted + n, vals[q]) - sorted);
+}
+void oracle_upper_bounds_i32(const int32_t* sorted, int64_t n, const int32_t* vals,
+                             int64_t m, int32_t* out) {
+  for (int64_t q = 0; q < m; ++q)
+    out[q] = (int32_t)(std::upper_bound(sorted, sorted + n, vals[q]) - sorted);
+}
+
+// The whole generate step for MarchingCubesGenerate on a uniform grid,
+// restating DispatcherGenerateInterpolatedCells::GenerateNewTopology and
+// ResolveCoordinates (dax/cont/DispatcherGenerateInterpolatedCells.h:158-301):
+//   count (MarchingCubes.h:53-86) -> ScanInclusive (:180-183) -> T
+//   UpperBounds(scan, Counting(0,T)) = validCellRange (:199-202)
+//   visit index = t - LowerBounds(vcr, vcr)[t]  (AddVisitIndexArg.h:44-64,
+//     VisitIndexWorklets.h:48-58)
+//   BuildTriangle per t into slots 3t..3t+2 (MarchingCubes.h:122-168,
+//     GeometryCell.h:80-91)
+//   dedup: Copy, Sort, Unique, LowerBounds -> connectivity (:273-290);
+//   otherwise connectivity = 0..3T-1 (:292-300)
+//   coordinates = InterpolateFieldToField over the kept interpolation records
+//     (GenerateWorklets.h:65-100).
+// Sizes are always reported.  Outputs are written only when the capacities
+// suffice (cap_pts >= 3T always does); returns 1 when they do not.
+int oracle_mc_generate(const float* field, const int64_t* dims, const float* origin,
+                       const float* spacing, float iso, int dedup, int64_t cap_tris,
+                       int64_t cap_pts, int64_t* n_tris, int64_t* n_pts,
+                       int32_t* conn, int32_t* edge_first, int32_t* edge_second,
+                       float* weight, float* coords) {
+  const Grid g = make_grid(dims, origin, spacing);
+  const int64_t C = g.cells();
+  std::vector<int32_t> count(C);
+  if (C > 0) oracle_mc_count(field, dims, iso, nullptr, count.data());
+  std::vector<int32_t> scan(C);
+  const int64_t T = oracle_scan_inclusive_i32(count.data(), scan.data(), C);
+  *n_tris = T;
+  *n_pts = 0;
+  if (T == 0) return 0;
+  std::vector<int32_t> tids(T);
+  std::iota(tids.begin(), tids.end(), 0);
+  std::vector<int32_t> vcr(T);
+  oracle_upper_bounds_i32(scan.data(), C, tids.data(), T, vcr.data());
+  std::vector<int32_t> lb(T);
+  oracle_lower_bounds_i32(vcr.data(), T, vcr.data(), T, lb.data());
+  std::vector<PointAsEdgeInterpolation> interp(3 * T);
+  for (int64_t t = 0; t < T; ++t) {
+    const int visit = (int)(t - lb[t]);
+    build_triangle(g, field, iso, vcr[t], visit, &interp[3 * t]);
+  }
+  std::vector<PointAsEdgeInterpolation> weights(interp);
+  std::vector<int32_t> connections(3 * T);
+  if (dedup) {
+    std::sort(weights.begin(), weights.end());
+    weights.erase(std::unique(weights.begin(), weights.end()), weights.end());
+    for (int64_t q = 0; q < 3 * T; ++q)
+      connections[q] = (int32_t)(std::lower_bound(weights.begin(), weights.end(), interp[q]) -
+                                 weights.begin());
+  } else {
+    std::iota(connections.begin(), connections.end(), 0);
+  }
+  const int64_t U = (int64_t)weights.size();
+  *n_pts = U;
+  if (cap_tris < T || cap_pts < U) return 1;
+  if (conn) std::memcpy(conn, connections.data(), sizeof(int32_t) * 3 * T);
+  for (int64_t u = 0; u < U; ++u) {
+    const PointAsEdgeInterpolation& pe = weights[u];
+    if (edge_first) edge_first[u] = pe.first;
+    if (edge_second) edge_second[u] = pe.second;
+    if (weight) weight[u] = pe.weight;
+    if (coords) {
+      Scalar a[3], b[3];
+      point_coords(g, pe.first, a);
+      point_coords(g, pe.second, b);
+      for (int d = 0; d < 3; ++d) coords[3 * u + d] = lerp(a[d], b[d], pe.weight);
+    }
+  }
+  return 0;
+}
+
+// CompactPointField for a scalar point field (GenerateWorklets.h:65-100).
+void oracle_interpolate_scalar(const float* in, const int32_t* first, const int32_t* second,
+                               const float* w, int64_t n, float* out) {
+  for (int64_t u = 0; u < n; ++u) out[u] = lerp(in[first[u]], in[second[u]], w[u]);
+}
+
+// Central-difference point gradient of the whole field, float32, [P][3].
+void oracle_point_gradient(const float* field, const int64_t* dims, const float* spacing,
+                           float* grad) {
+  const Grid g = make_grid(dims, nullptr, spacing);
+  const int64_t P = g.points();
+  for (int64_t p = 0; p < P; ++p) point_gradient_at(g, field, p, grad + 3 * p);
+}
+
+// Vertex normals for interpolation records (first, second, w).
+void oracle_mc_normals(const float* field, const int64_t* dims, const float* spacing,
+                       const int32_t* first, const int32_t* second, const float* w,
+                       int64_t n, float* normals) {
+  const Grid g = make_grid(dims, nullptr, spacing);
+  for (int64_t u = 0; u < n; ++u) {
+    Scalar gA[3], gB[3];
+    point_gradient_at(g, field, first[u], gA);
+    point_gradient_at(g, field, second[u], gB);
+    normal_from(gA, gB, w[u], normals + 3 * u);
+  }
+}
+
+// CellGradient on a uniform grid (dax/worklet/CellGradient.h:28-48):
+// CellDerivative(Voxel) -> CellDerivativeAxisAligned (dax/exec/Derivative.h:
+// 37-75) with DerivativeWeights(Voxel) at the parametric centre
+// (dax/exec/internal/DerivativeWeights.h:30-69, ParametricCoordinates.h:77-81);
+// widths = coords[6] - coords[0].
+void oracle_cell_gradient(const float* field, const int64_t* dims, const float* origin,
+                          const float* spacing, float* out) {
+  const Grid g = make_grid(dims, origin, spacing);
+  const int64_t C = g.cells();
+  const Scalar pc[3] = {0.5f, 0.5f, 0.5f};
+  const Scalar rc[3] = {1.0f - pc[0], 1.0f - pc[1], 1.0f - pc[2]};
+  Scalar w[8][3];
+  w[0][0] = -rc[1] * rc[2]; w[0][1] = -rc[0] * rc[2]; w[0][2] = -rc[0] * rc[1];
+  w[1][0] = rc[1] * rc[2];  w[1][1] = -pc[0] * rc[2]; w[1][2] = -pc[0] * rc[1];
+  w[2][0] = pc[1] * rc[2];  w[2][1] = pc[0] * rc[2];  w[2][2] = -pc[0] * pc[1];
+  w[3][0] = -pc[1] * rc[2]; w[3][1] = rc[0] * rc[2];  w[3][2] = -rc[0] * pc[1];
+  w[4][0] = -rc[1] * pc[2]; w[4][1] = -rc[0] * pc[2]; w[4][2] = rc[0] * rc[1];
+  w[5][0] = rc[1] * pc[2];  w[5][1] = -pc[0] * pc[2]; w[5][2] = pc[0] * rc[1];
+  w[6][0] = pc[1] * pc[2];  w[6][1] = pc[0] * pc[2];  w[6][2] = pc[0] * pc[1];
+  w[7][0] = -pc[1] * pc[2]; w[7][1] = rc[0] * pc[2];  w[7][2] = rc[0] * pc[1];
+  for (int64_t c = 0; c < C; ++c) {
+    Id verts[8];
+    cell_vertices(g, c, verts);
+    Scalar sum[3] = {0.0f, 0.0f, 0.0f};
+    for (int v = 0; v < 8; ++v) {
+      const Scalar f = field[verts[v]];
+      for (int d = 0; d < 3; ++d) sum[d] = sum[d] + f * w[v][d];
+    }
+    Scalar c0[3], c6[3];
+    point_coords(g, verts[0], c0);
+    point_coords(g, verts[6], c6);
+    for (int d = 0; d < 3; ++d) out[3 * c + d] = sum[d] / (c6[d] - c0[d]);
+  }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// FP64 adjoint and finite differences (no reference counterpart).
+//
+// Forward in FP64 for vertex u on edge (A,B):
+//   t = (iso - fA)/(fB - fA),  x = cA + t (cB - cA)
+//   m = gA + t (gB - gA),      n = m/|m|
+// with g the central-difference gradient above.  L = sum_u dLdx_u.x_u +
+// dLdn_u.n_u with topology (the edge list) held fixed.
+namespace {
+
+void point_gradient_at_f64(const Grid& g, const float* f, const std::unordered_map<int64_t, double>* over,
+                           int64_t p, double out[3]) {
+  auto val = [&](int64_t q) -> double {
+    if (over) {
+      auto it = over->find(q);
+      if (it != over->end()) return it->second;
+    }
+    return (double)f[q];
+  };
+  const int64_t idx[3] = {p % g.nx, (p / g.nx) % g.ny, p / (g.nx * g.ny)};
+  const int64_t n[3] = {g.nx, g.ny, g.nz};
+  const int64_t stride[3] = {1, g.nx, g.nx * g.ny};
+  for (int d = 0; d < 3; ++d) {
+    const double h = (double)g.spacing[d];
+    if (n[d] < 2)
+      out[d] = 0.0;
+    else if (idx[d] == 0)
+      out[d] = (val(p + stride[d]) - val(p)) / h;
+    else if (idx[d] == n[d] - 1)
+      out[d] = (val(p) - val(p - stride[d])) / h;
+    else
+      out[d] = (val(p + stride[d]) - val(p - stride[d])) / (2.0 * h);
+  }
+}
+
+double vertex_loss_f64(const Grid& g, const float* f, const std::unordered_map<int64_t, double>* over,
+                       float iso, int64_t A, int64_t B, const float* dldx, const float* dldn) {
+  auto val = [&](int64_t q) -> double {
+    if (over) {
+      auto it = over->find(q);
+      if (it != over->end()) return it->second;
+    }
+    return (double)f[q];
+  };
+  const double fA = val(A), fB = val(B);
+  const double t = ((double)iso - fA) / (fB - fA);
+  double loss = 0.0;
+  const int64_t ia[3] = {A % g.nx, (A / g.nx) % g.ny, A / (g.nx * g.ny)};
+  const int64_t ib[3] = {B % g.nx, (B / g.nx) % g.ny, B / (g.nx * g.ny)};
+  for (int d = 0; d < 3; ++d) {
+    const double ca = (double)g.origin[d] + (double)g.spacing[d] * (double)ia[d];
+    const double cb = (double)g.origin[d] + (double)g.spacing[d] * (double)ib[d];
+    loss += (double)dldx[d] * (ca + t * (cb - ca));
+  }
+  double gA[3], gB[3], m[3];
+  point_gradient_at_f64(g, f, over, A, gA);
+  point_gradient_at_f64(g, f, over, B, gB);
+  for (int d = 0; d < 3; ++d) m[d] = gA[d] + t * (gB[d] - gA[d]);
+  const double len = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  if (len > 0.0)
+    for (int d = 0; d < 3; ++d) loss += (double)dldn[d] * (m[d] / len);
+  return loss;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Analytic adjoint in FP64, scatter form: dL/df for every point.
+void oracle_mc_backward_f64(const float* field, const int64_t* dims, const float* origin,
+                            const float* spacing, float iso, const int32_t* first,
+                            const int32_t* second, int64_t n, const float* dldx,
+                            const float* dldn, double* dldf) {
+  const Grid g = make_grid(dims, origin, spacing);
+  const int64_t P = g.points();
+  std::fill(dldf, dldf + P, 0.0);
+  const int64_t n3[3] = {g.nx, g.ny, g.nz};
+  const int64_t stride[3] = {1, g.nx, g.nx * g.ny};
+  // d g_p[d] / d f : list of (point, coefficient)
+  auto grad_adj = [&](int64_t p, const double G[3]) {
+    const int64_t idx[3] = {p % g.nx, (p / g.nx) % g.ny, p / (g.nx * g.ny)};
+    for (int d = 0; d < 3; ++d) {
+      const double h = (double)g.spacing[d];
+      if (n3[d] < 2) continue;
+      if (idx[d] == 0) {
+        dldf[p + stride[d]] += G[d] / h;
+        dldf[p] -= G[d] / h;
+      } else if (idx[d] == n3[d] - 1) {
+        dldf[p] += G[d] / h;
+        dldf[p - stride[d]] -= G[d] / h;
+      } else {
+        dldf[p + stride[d]] += G[d] / (2.0 * h);
+        dldf[p - stride[d]] -= G[d] / (2.0 * h);
+      }
+    }
+  };
+  for (int64_t u = 0; u < n; ++u) {
+    const int64_t A = first[u], B = second[u];
+    const double fA = field[A], fB = field[B];
+    const double t = ((double)iso - fA) / (fB - fA);
+    const int64_t ia[3] = {A % g.nx, (A / g.nx) % g.ny, A / (g.nx * g.ny)};
+    const int64_t ib[3] = {B % g.nx, (B / g.nx) % g.ny, B / (g.nx * g.ny)};
+    double dLdt = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      const double ca = (double)g.origin[d] + (double)g.spacing[d] * (double)ia[d];
+      const double cb = (double)g.origin[d] + (double)g.spacing[d] * (double)ib[d];
+      dLdt += (double)dldx[3 * u + d] * (cb - ca);
+    }
+    double gA[3], gB[3], m[3];
+    point_gradient_at_f64(g, field, nullptr, A, gA);
+    point_gradient_at_f64(g, field, nullptr, B, gB);
+    for (int d = 0; d < 3; ++d) m[d] = gA[d] + t * (gB[d] - gA[d]);
+    const double len = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+    double dLdm[3] = {0.0, 0.0, 0.0};
+    if (len > 0.0) {
+      double nv[3], dot = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        nv[d] = m[d] / len;
+        dot += nv[d] * (double)dldn[3 * u + d];
+      }
+      for (int d = 0; d < 3; ++d) dLdm[d] = ((double)dldn[3 * u + d] - nv[d] * dot) / len;
+    }
+    for (int d = 0; d < 3; ++d) dLdt += dLdm[d] * (gB[d] - gA[d]);
+    const double den = fB - fA;
+    dldf[A] += dLdt * (-(1.0 - t) / den);
+    dldf[B] += dLdt * (-t / den);
+    double GA[3], GB[3];
+    for (int d = 0; d < 3; ++d) {
+      GA[d] = (1.0 - t) * dLdm[d];
+      GB[d] = t * dLdm[d];
+    }
+    grad_adj(A, GA);
+    grad_adj(B, GB);
+  }
+}
+
+// FP64 central differences of L with respect to selected points (step h),
+// evaluating only the vertices whose value depends on the perturbed point.
+void oracle_mc_fd_f64(const float* field, const int64_t* dims, const float* origin,
+                      const float* spacing, float iso, const int32_t* first,
+                      const int32_t* second, int64_t n, const float* dldx, const float* dldn,
+                      const int64_t* points, int64_t npoints, double h, double* out) {
+  const Grid g = make_grid(dims, origin, spacing);
+  // point -> incident vertices
+  std::unordered_map<int64_t, std::vector<int64_t>> incident;
+  incident.reserve((size_t)(2 * n));
+  for (int64_t u = 0; u < n; ++u) {
+    incident[first[u]].push_back(u);
+    incident[second[u]].push_back(u);
+  }
+  const int64_t stride[3] = {1, g.nx, g.nx * g.ny};
+  const int64_t n3[3] = {g.nx, g.ny, g.nz};
+  for (int64_t s = 0; s < npoints; ++s) {
+    const int64_t q = points[s];
+    const int64_t idx[3] = {q % g.nx, (q / g.nx) % g.ny, q / (g.nx * g.ny)};
+    std::vector<int64_t> nb{q};
+    for (int d = 0; d < 3; ++d) {
+      if (idx[d] > 0) nb.push_back(q - stride[d]);
+      if (idx[d] < n3[d] - 1) nb.push_back(q + stride[d]);
+    }
+    std::vector<int64_t> verts;
+    for (int64_t p : nb) {
+      auto it = incident.find(p);
+      if (it != incident.end()) verts.insert(verts.end(), it->second.begin(), it->second.end());
+    }
+    std::sort(verts.begin(), verts.end());
+    verts.erase(std::unique(verts.begin(), verts.end()), verts.end());
+    double lp = 0.0, lm = 0.0;
+    std::unordered_map<int64_t, double> over;
+    over[q] = (double)field[q] + h;
+    for (int64_t u : verts)
+      lp += vertex_loss_f64(g, field, &over, iso, first[u], second[u], dldx + 3 * u, dldn + 3 * u);
+    over[q] = (double)field[q] - h;
+    for (int64_t u : verts)
+      lm += vertex_loss_f64(g, field, &over, iso, first[u], second[u], dldx + 3 * u, dldn + 3 * u);
+    out[s] = (lp - lm) / (2.0 * h);
+  }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// CPU baseline: the same step sequence as Dax's OpenMP adapter
+// (dax/openmp/cont/internal/DeviceAdapterAlgorithmOpenMP.h:34-144 over the
+// Thrust algorithms of DeviceAdapterAlgorithmThrust.h): parallel classify,
+// parallel inclusive scan, parallel upper/lower bounds, parallel generate,
+// parallel sort, unique, parallel lower bounds and coordinate interpolation.
+// Timed like Benchmarks/MarchingCubes/Pipeline.h:110-132 (classify + generate
+// with dedup, field already resident).  Returns seconds.
+extern "C" double oracle_mc_pipeline_omp(const float* field, const int64_t* dims,
+                                         const float* origin, const float* spacing,
+                                         float iso, int dedup, int threads,
+                                         int64_t* n_tris, int64_t* n_pts) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+  const double t0 = omp_get_wtime();
+#endif
+  const Grid g = make_grid(dims, origin, spacing);
+  const Tables& TB = tables();
+  const int64_t C = g.cells();
+  std::vector<int32_t> count(C), scan(C);
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < C; ++c) {
+    Id verts[8];
+    cell_vertices(g, c, verts);
+    Scalar values[8];
+    for (int v = 0; v < 8; ++v) values[v] = field[verts[v]];
+    count[c] = TB.num_faces[classification(iso, values)];
+  }
+  // blocked two-pass inclusive scan
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
+  std::vector<int64_t> part(nth + 1, 0);
+#pragma omp parallel num_threads(nth)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    const int64_t lo = C * tid / nth, hi = C * (tid + 1) / nth;
+    int64_t s = 0;
+    for (int64_t c = lo; c < hi; ++c) s += count[c];
+    part[tid + 1] = s;
+#pragma omp barrier
+#pragma omp single
+    for (int i = 0; i < nth; ++i) part[i + 1] += part[i];
+    int64_t run = part[tid];
+    for (int64_t c = lo; c < hi; ++c) {
+      run += count[c];
+      scan[c] = (int32_t)run;
+    }
+  }
+  const int64_t T = C > 0 ? scan[C - 1] : 0;
+  *n_tris = T;
+  *n_pts = 0;
+  if (T > 0) {
+    std::vector<int32_t> vcr(T), visit(T);
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < T; ++t)
+      vcr[t] = (int32_t)(std::upper_bound(scan.begin(), scan.end(), (int32_t)t) - scan.begin());
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < T; ++t)
+      visit[t] = (int32_t)(t - (std::lower_bound(vcr.begin(), vcr.end(), vcr[t]) - vcr.begin()));
+    std::vector<PointAsEdgeInterpolation> interp(3 * T);
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < T; ++t) build_triangle(g, field, iso, vcr[t], visit[t], &interp[3 * t]);
+    std::vector<PointAsEdgeInterpolation> weights(interp);
+    std::vector<int32_t> connections(3 * T);
+    if (dedup) {
+#ifdef _OPENMP
+      __gnu_parallel::sort(weights.begin(), weights.end());
+#else
+      std::sort(weights.begin(), weights.end());
+#endif
+      weights.erase(std::unique(weights.begin(), weights.end()), weights.end());
+#pragma omp parallel for schedule(static)
+      for (int64_t q = 0; q < 3 * T; ++q)
+        connections[q] = (int32_t)(std::lower_bound(weights.begin(), weights.end(), interp[q]) -
+                                   weights.begin());
+    } else {
+#pragma omp parallel for schedule(static)
+      for (int64_t q = 0; q < 3 * T; ++q) connections[q] = (int32_t)q;
+    }
+    const int64_t U = (int64_t)weights.size();
+    std::vector<float> coords(3 * U);
+#pragma omp parallel for schedule(static)
+    for (int64_t u = 0; u < U; ++u) {
+      Scalar a[3], b[3];
+      point_coords(g, weights[u].first, a);
+      point_coords(g, weights[u].second, b);
+      for (int d = 0; d < 3; ++d) coords[3 * u + d] = lerp(a[d], b[d], weights[u].weight);
+    }
+    *n_pts = U;
+  }
+#ifdef _OPENMP
+  return omp_get_wtime() - t0;
+#else
+  return -1.0;
+#endif
+}

@@ -1,0 +1,227 @@
+"""ctypes wrapper of the oracle library (test infrastructure only).
+
+The oracle is a CPU restatement of Dax's Serial-adapter marching-cubes path
+(see mc_oracle.cpp for the reference lines each function follows).  Only the
+tests, the smoke check and the CPU-baseline leg of bench.py use it, always as
+the checker or the baseline, never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_SO = _HERE / "liboracle_mc.so"
+_lib = None
+
+_P = np.ctypeslib.ndpointer
+_i64p = C.POINTER(C.c_int64)
+
+
+def build() -> None:
+    subprocess.run(["make", "-C", str(_HERE)], check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not _SO.exists():
+            build()
+        L = C.CDLL(str(_SO))
+        L.oracle_tables_expand.argtypes = [C.c_void_p, C.c_void_p]
+        L.oracle_mc_count.argtypes = [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p]
+        L.oracle_mc_count.restype = C.c_int
+        for n in ("oracle_scan_inclusive_i32", "oracle_scan_exclusive_i32"):
+            getattr(L, n).argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+            getattr(L, n).restype = C.c_int64
+        for n in ("oracle_lower_bounds_i32", "oracle_upper_bounds_i32"):
+            getattr(L, n).argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p]
+        L.oracle_mc_generate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
+                                         C.c_int, C.c_int64, C.c_int64, _i64p, _i64p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]
+        L.oracle_mc_generate.restype = C.c_int
+        L.oracle_interpolate_scalar.argtypes = [C.c_void_p] * 4 + [C.c_int64, C.c_void_p]
+        L.oracle_point_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_mc_normals.argtypes = [C.c_void_p] * 6 + [C.c_int64, C.c_void_p]
+        L.oracle_cell_gradient.argtypes = [C.c_void_p] * 5
+        L.oracle_mc_backward_f64.argtypes = ([C.c_void_p] * 4 + [C.c_float] + [C.c_void_p] * 2 +
+                                             [C.c_int64] + [C.c_void_p] * 3)
+        L.oracle_mc_fd_f64.argtypes = ([C.c_void_p] * 4 + [C.c_float] + [C.c_void_p] * 2 +
+                                       [C.c_int64] + [C.c_void_p] * 3 + [C.c_int64, C.c_double,
+                                                                          C.c_void_p])
+        L.oracle_mc_pipeline_omp.argtypes = [C.c_void_p] * 4 + [C.c_float, C.c_int, C.c_int,
+                                                                 _i64p, _i64p]
+        L.oracle_mc_pipeline_omp.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _dims(field: np.ndarray) -> np.ndarray:
+    nz, ny, nx = field.shape
+    return np.array([nx, ny, nz], dtype=np.int64)
+
+
+def _f3(v) -> np.ndarray:
+    return np.asarray(v, dtype=np.float32).reshape(3)
+
+
+def _field(field) -> np.ndarray:
+    f = np.ascontiguousarray(field, dtype=np.float32)
+    assert f.ndim == 3
+    return f
+
+
+def tables():
+    nf = np.zeros(256, np.uint8)
+    tri = np.zeros(4096, np.uint8)
+    lib().oracle_tables_expand(_p(nf), _p(tri))
+    return nf, tri.reshape(256, 16)
+
+
+def count(field, iso):
+    """Per-cell case (u8) and triangle count (i32) in flat cell order."""
+    f = _field(field)
+    nz, ny, nx = f.shape
+    C_ = max(nx - 1, 0) * max(ny - 1, 0) * max(nz - 1, 0)
+    cases = np.zeros(C_, np.uint8)
+    cnt = np.zeros(C_, np.int32)
+    lib().oracle_mc_count(_p(f), _p(_dims(f)), float(iso), _p(cases), _p(cnt))
+    return cases, cnt
+
+
+def scan_inclusive(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    out = np.zeros_like(a)
+    total = lib().oracle_scan_inclusive_i32(_p(a), _p(out), a.size)
+    return out, int(total)
+
+
+def scan_exclusive(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    out = np.zeros_like(a)
+    total = lib().oracle_scan_exclusive_i32(_p(a), _p(out), a.size)
+    return out, int(total)
+
+
+def lower_bounds(sorted_, vals):
+    s = np.ascontiguousarray(sorted_, dtype=np.int32)
+    v = np.ascontiguousarray(vals, dtype=np.int32)
+    out = np.zeros(v.size, np.int32)
+    lib().oracle_lower_bounds_i32(_p(s), s.size, _p(v), v.size, _p(out))
+    return out
+
+
+def upper_bounds(sorted_, vals):
+    s = np.ascontiguousarray(sorted_, dtype=np.int32)
+    v = np.ascontiguousarray(vals, dtype=np.int32)
+    out = np.zeros(v.size, np.int32)
+    lib().oracle_upper_bounds_i32(_p(s), s.size, _p(v), v.size, _p(out))
+    return out
+
+
+def generate(field, iso, dedup=True, origin=(0, 0, 0), spacing=(1, 1, 1)):
+    """The reference's generate step; returns dict with n_tris, n_pts, conn,
+    first, second, weight (interpolation records) and coords."""
+    f = _field(field)
+    dims = _dims(f)
+    o, s = _f3(origin), _f3(spacing)
+    _, cnt = count(f, iso)
+    T = int(cnt.sum(dtype=np.int64))
+    cap_p = max(3 * T, 1)
+    conn = np.zeros(max(3 * T, 1), np.int32)
+    first = np.zeros(cap_p, np.int32)
+    second = np.zeros(cap_p, np.int32)
+    weight = np.zeros(cap_p, np.float32)
+    coords = np.zeros((cap_p, 3), np.float32)
+    nt, npt = C.c_int64(), C.c_int64()
+    rc = lib().oracle_mc_generate(_p(f), _p(dims), _p(o), _p(s), float(iso), int(dedup), T, cap_p,
+                                  C.byref(nt), C.byref(npt), _p(conn), _p(first), _p(second),
+                                  _p(weight), _p(coords))
+    assert rc == 0
+    T, U = int(nt.value), int(npt.value)
+    return dict(n_tris=T, n_pts=U, conn=conn[:3 * T], first=first[:U], second=second[:U],
+                weight=weight[:U], coords=coords[:U])
+
+
+def interpolate_scalar(values, first, second, weight):
+    v = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
+    first = np.ascontiguousarray(first, np.int32)
+    second = np.ascontiguousarray(second, np.int32)
+    weight = np.ascontiguousarray(weight, np.float32)
+    out = np.zeros(first.size, np.float32)
+    lib().oracle_interpolate_scalar(_p(v), _p(first), _p(second), _p(weight), first.size, _p(out))
+    return out
+
+
+def point_gradient(field, spacing=(1, 1, 1)):
+    f = _field(field)
+    g = np.zeros((f.size, 3), np.float32)
+    lib().oracle_point_gradient(_p(f), _p(_dims(f)), _p(_f3(spacing)), _p(g))
+    return g
+
+
+def normals(field, first, second, weight, spacing=(1, 1, 1)):
+    f = _field(field)
+    first = np.ascontiguousarray(first, np.int32)
+    second = np.ascontiguousarray(second, np.int32)
+    weight = np.ascontiguousarray(weight, np.float32)
+    out = np.zeros((first.size, 3), np.float32)
+    lib().oracle_mc_normals(_p(f), _p(_dims(f)), _p(_f3(spacing)), _p(first), _p(second),
+                            _p(weight), first.size, _p(out))
+    return out
+
+
+def cell_gradient(field, origin=(0, 0, 0), spacing=(1, 1, 1)):
+    f = _field(field)
+    nz, ny, nx = f.shape
+    C_ = max(nx - 1, 0) * max(ny - 1, 0) * max(nz - 1, 0)
+    out = np.zeros((C_, 3), np.float32)
+    lib().oracle_cell_gradient(_p(f), _p(_dims(f)), _p(_f3(origin)), _p(_f3(spacing)), _p(out))
+    return out
+
+
+def backward_f64(field, iso, first, second, dldx, dldn, origin=(0, 0, 0), spacing=(1, 1, 1)):
+    f = _field(field)
+    first = np.ascontiguousarray(first, np.int32)
+    second = np.ascontiguousarray(second, np.int32)
+    dldx = np.ascontiguousarray(dldx, np.float32)
+    dldn = np.ascontiguousarray(dldn, np.float32)
+    out = np.zeros(f.size, np.float64)
+    lib().oracle_mc_backward_f64(_p(f), _p(_dims(f)), _p(_f3(origin)), _p(_f3(spacing)),
+                                 float(iso), _p(first), _p(second), first.size, _p(dldx),
+                                 _p(dldn), _p(out))
+    return out
+
+
+def fd_f64(field, iso, first, second, dldx, dldn, points, h=1e-3, origin=(0, 0, 0),
+           spacing=(1, 1, 1)):
+    f = _field(field)
+    first = np.ascontiguousarray(first, np.int32)
+    second = np.ascontiguousarray(second, np.int32)
+    dldx = np.ascontiguousarray(dldx, np.float32)
+    dldn = np.ascontiguousarray(dldn, np.float32)
+    pts = np.ascontiguousarray(points, np.int64)
+    out = np.zeros(pts.size, np.float64)
+    lib().oracle_mc_fd_f64(_p(f), _p(_dims(f)), _p(_f3(origin)), _p(_f3(spacing)), float(iso),
+                           _p(first), _p(second), first.size, _p(dldx), _p(dldn), _p(pts),
+                           pts.size, float(h), _p(out))
+    return out
+
+
+def pipeline_omp(field, iso, dedup=True, threads=0, origin=(0, 0, 0), spacing=(1, 1, 1)):
+    """Time the OpenMP restatement of Dax's OpenMP-adapter MC pipeline.
+    Returns (seconds, T, U)."""
+    f = _field(field)
+    nt, npt = C.c_int64(), C.c_int64()
+    sec = lib().oracle_mc_pipeline_omp(_p(f), _p(_dims(f)), _p(_f3(origin)), _p(_f3(spacing)),
+                                       float(iso), int(dedup), int(threads), C.byref(nt),
+                                       C.byref(npt))
+    return float(sec), int(nt.value), int(npt.value)

@@ -1,0 +1,215 @@
+"""Pins the oracle: golden vectors of the reference's own tests, closed forms
+and brute force."""
+import numpy as np
+import pytest
+
+from daxtoolkit_amd import fields
+from helpers import dax_test_equal
+from oracle import oracle as orc
+
+ARRAY_SIZE, OFFSET = 500, 1000  # TestingDeviceAdapter.h:25-26
+
+
+def test_scan_golden():
+    """TestingDeviceAdapter.h:940-1001: OFFSET-filled array scans to
+    triangle numbers times OFFSET; both scans return the full sum."""
+    a = np.full(ARRAY_SIZE, OFFSET, np.int32)
+    inc, total = orc.scan_inclusive(a)
+    assert total == OFFSET * ARRAY_SIZE
+    pos = np.arange(1, ARRAY_SIZE + 1)
+    assert np.array_equal(np.cumsum(inc.astype(np.int64)), pos * (pos + 1) // 2 * OFFSET)
+    exc, total = orc.scan_exclusive(a)
+    assert total == OFFSET * ARRAY_SIZE
+    pos = np.arange(ARRAY_SIZE)
+    assert np.array_equal(np.cumsum(exc.astype(np.int64)), pos * (pos + 1) // 2 * OFFSET)
+    assert orc.scan_inclusive(np.zeros(0, np.int32))[1] == 0
+
+
+def test_bounds_golden():
+    """TestingDeviceAdapter.h:621-717 golden random data."""
+    data = np.array([500, 955, 955, 120, 320, 955, 986, 281], np.int32)
+    uniq = np.unique(data)
+    assert list(orc.lower_bounds(uniq, data)) == [3, 4, 4, 0, 2, 4, 5, 1]
+    assert list(orc.upper_bounds(uniq, data)) == [4, 5, 5, 1, 3, 5, 6, 2]
+    test = OFFSET + (np.arange(ARRAY_SIZE) % 50)
+    u = np.unique(test).astype(np.int32)
+    assert np.array_equal(orc.lower_bounds(u, test), np.arange(ARRAY_SIZE) % 50)
+    assert np.array_equal(orc.upper_bounds(u, test), np.arange(ARRAY_SIZE) % 50 + 1)
+
+
+def test_count_brute_force_256_cases():
+    """A1/A2: every sign pattern on one cell gives its case index and the
+    number of triangles a brute-force count of sign-change edges implies."""
+    corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1),
+               (0, 1, 1)]
+    nf, _ = orc.tables()
+    for c in range(256):
+        cube = np.zeros((2, 2, 2), np.float32)
+        for v, (x, y, z) in enumerate(corners):
+            cube[z, y, x] = 2.0 if (c >> v) & 1 else 0.0
+        cases, cnt = orc.count(cube, 1.0)
+        assert cases[0] == c and cnt[0] == nf[c]
+
+
+def test_count_strict_greater():
+    """Values equal to the iso are 'below' (strict >)."""
+    cube = np.ones((2, 2, 2), np.float32)
+    cases, cnt = orc.count(cube, 1.0)
+    assert cases[0] == 0 and cnt[0] == 0
+
+
+def test_reference_mc_unit_test():
+    """UnitTestWorkletMarchingCubes.cxx:39-200 (26^3 points, f=x+y+z,
+    iso 70): 3T points without merge, 45 with merge (closed form: 21
+    lattice points on the plane x+y+z=70 inside the box are ... plus the
+    crossing edges), secondary field interpolated to test_equal."""
+    f = fields.linear(26, 26, 26)
+    r0 = orc.generate(f, 70.0, dedup=False)
+    assert r0["n_pts"] == 3 * r0["n_tris"] and r0["n_tris"] > 0
+    assert np.array_equal(r0["conn"], np.arange(3 * r0["n_tris"]))
+    r1 = orc.generate(f, 70.0, dedup=True)
+    assert r1["n_pts"] == 45 and r1["n_tris"] == r0["n_tris"]
+    sec = fields.linear(26, 26, 26, (-0.5, 1.0, 2.0)).reshape(-1)
+    for r in (r0, r1):
+        vals = orc.interpolate_scalar(sec, r["first"], r["second"], r["weight"])
+        assert dax_test_equal(vals, r["coords"].astype(np.float64) @ [-0.5, 1.0, 2.0])
+
+
+def test_unique_points_closed_form():
+    """Dedup count equals the number of grid edges crossing the plane
+    x+y+z = iso, counted directly (independent of sort/unique)."""
+    n, iso = 26, 70.0
+    f = fields.linear(n, n, n)
+    r = orc.generate(f, iso, dedup=True)
+    ax = np.arange(n)
+    s = ax[:, None, None] + ax[None, :, None] + ax[None, None, :]
+    inside = s > iso
+    crossings = 0
+    for d in range(3):
+        a = np.take(inside, range(n - 1), axis=d)
+        b = np.take(inside, range(1, n), axis=d)
+        crossings += int((a != b).sum())
+    assert r["n_pts"] == crossings
+
+
+def test_dedup_equals_set_of_corners():
+    """Dedup output = the 3T corner records with duplicates removed, sorted
+    by (first, second); connectivity maps each corner to its record."""
+    rng = np.random.default_rng(1)
+    f = rng.standard_normal((7, 9, 11)).astype(np.float32)
+    r0 = orc.generate(f, 0.2, dedup=False)
+    r1 = orc.generate(f, 0.2, dedup=True)
+    keys = sorted(set(zip(r0["first"].tolist(), r0["second"].tolist())))
+    assert list(zip(r1["first"].tolist(), r1["second"].tolist())) == keys
+    back = np.array(keys)[r1["conn"]]
+    assert np.array_equal(back[:, 0], r0["first"]) and np.array_equal(back[:, 1], r0["second"])
+    assert np.array_equal(r1["coords"][r1["conn"]], r0["coords"])
+
+
+def test_edge_flag_numbering_brute_force():
+    """The dedup vertex order is the active edges in (lower point id, axis)
+    order -- the numbering the GPU kernels use instead of a sort."""
+    rng = np.random.default_rng(2)
+    for dims in [(5, 6, 7), (9, 3, 4), (2, 2, 2)]:
+        nx, ny, nz = dims
+        f = rng.standard_normal((nz, ny, nx)).astype(np.float32)
+        r = orc.generate(f, 0.0, dedup=True)
+        inside = (f > 0.0).reshape(-1)
+        edges = []
+        for p in range(f.size):
+            i, j, k = p % nx, (p // nx) % ny, p // (nx * ny)
+            for axis, (ok, q) in enumerate([(i + 1 < nx, p + 1), (j + 1 < ny, p + nx),
+                                            (k + 1 < nz, p + nx * ny)]):
+                if ok and inside[p] != inside[q]:
+                    edges.append((p, q))
+        assert list(zip(r["first"].tolist(), r["second"].tolist())) == edges
+
+
+def test_sphere_survey_values():
+    f = fields.sphere_distance(128)
+    cases, cnt = orc.count(f, 40.0)
+    assert cnt.size == 2048383
+    assert int((cnt > 0).sum()) == 30146 and int(cnt.sum()) == 60284
+    r = orc.generate(f, 40.0)
+    assert r["n_tris"] == 60284 and r["n_pts"] == 30144
+
+
+def test_plane_closed_form_coordinates():
+    """Vertices of f = x (iso 2.5) lie exactly on x = 2.5."""
+    f = fields.linear(6, 5, 4, (1.0, 0.0, 0.0))
+    r = orc.generate(f, 2.5, origin=(0, 0, 0), spacing=(1, 1, 1))
+    assert np.all(r["coords"][:, 0] == 2.5)
+    assert r["n_pts"] == 5 * 4
+
+
+def test_point_gradient_exact_on_linear_and_quadratic():
+    g = (0.5, -2.0, 3.0)
+    sp = (0.5, 1.0, 2.0)
+    nx, ny, nz = 7, 6, 5
+    x = np.arange(nx) * sp[0]
+    y = np.arange(ny) * sp[1]
+    z = np.arange(nz) * sp[2]
+    f = (g[0] * x[None, None, :] + g[1] * y[None, :, None] + g[2] * z[:, None, None]).astype(
+        np.float32)
+    gr = orc.point_gradient(f, spacing=sp)
+    assert np.allclose(gr, np.array(g, np.float32)[None, :], rtol=1e-6, atol=1e-5)
+    # quadratic: central differences exact in the interior
+    fq = (x[None, None, :] ** 2 + 0 * y[None, :, None] + 0 * z[:, None, None]).astype(np.float32)
+    gq = orc.point_gradient(fq, spacing=sp).reshape(nz, ny, nx, 3)
+    assert np.allclose(gq[:, :, 1:-1, 0], 2 * x[None, None, 1:-1], rtol=1e-6)
+    # one-sided on the boundary planes
+    assert np.allclose(gq[:, :, 0, 0], (x[1] ** 2 - x[0] ** 2) / sp[0])
+    assert np.allclose(gq[:, :, -1, 0], (x[-1] ** 2 - x[-2] ** 2) / sp[0])
+
+
+def test_sphere_normals_match_analytic():
+    f = fields.sphere_distance(128)
+    r = orc.generate(f, 40.0)
+    n = orc.normals(f, r["first"], r["second"], r["weight"])
+    p = r["coords"].astype(np.float64) - 63.5
+    ana = p / np.linalg.norm(p, axis=1, keepdims=True)
+    assert np.abs(np.linalg.norm(n, axis=1) - 1.0).max() < 1e-6
+    assert np.abs(n - ana).max() < 2e-3   # O(h^2) stencil error at radius 40
+
+
+def test_cell_gradient_linear_exact():
+    """UnitTestWorkletCellGradient.cxx: linear field -> exact gradient."""
+    sp, org = (0.5, 2.0, 1.0), (-1.0, 0.5, 3.0)
+    nx, ny, nz = 6, 5, 4
+    g = (1.0, 1.0, 1.0)
+    x = org[0] + np.arange(nx) * sp[0]
+    y = org[1] + np.arange(ny) * sp[1]
+    z = org[2] + np.arange(nz) * sp[2]
+    f = (x[None, None, :] + y[None, :, None] + z[:, None, None]).astype(np.float32)
+    cg = orc.cell_gradient(f, origin=org, spacing=sp)
+    assert cg.shape == (5 * 4 * 3, 3)
+    assert dax_test_equal(cg, np.tile(g, (cg.shape[0], 1)))
+
+
+def test_backward_matches_finite_differences():
+    """Analytic FP64 adjoint vs FP64 central differences (step 1e-3)."""
+    dims = (24, 20, 18)
+    p = fields.gaussian_mixture_params(11, dims, n_gauss=4, sigma=(4.0, 8.0))
+    f = fields.gaussian_mixture_numpy(dims, p)
+    iso = 0.5
+    r = orc.generate(f, iso)
+    U = r["n_pts"]
+    assert U > 100
+    rng = np.random.default_rng(4)
+    dldx = rng.standard_normal((U, 3)).astype(np.float32)
+    dldn = rng.standard_normal((U, 3)).astype(np.float32)
+    g = orc.backward_f64(f, iso, r["first"], r["second"], dldx, dldn)
+    flat = f.reshape(-1)
+    cand = np.unique(np.concatenate([r["first"], r["second"]]))
+    cand = cand[np.abs(flat[cand] - iso) > 1e-2]
+    pts = rng.choice(cand, size=min(300, cand.size), replace=False)
+    fd = orc.fd_f64(f, iso, r["first"], r["second"], dldx, dldn, pts, h=1e-4)
+    err = np.abs(fd - g[pts]) / np.maximum(np.abs(fd), 1e-2)
+    assert err.max() < 1e-3, err.max()
+
+
+def test_pipeline_omp_matches_serial():
+    f = fields.sphere_distance(64, 31.5)
+    sec, T, U = orc.pipeline_omp(f, 20.0, dedup=True, threads=2)
+    r = orc.generate(f, 20.0)
+    assert (T, U) == (r["n_tris"], r["n_pts"]) and sec >= 0
