@@ -85,8 +85,12 @@ __device__ __forceinline__ WordPos word_pos(const VolGeom& g, int64_t lw) {
   p.has_y = p.j + 1 < g.ny;
   p.has_z = p.k + 1 < g.nz;
   const int x0 = p.q * 64;
-  p.pmask = low_mask(g.nx - x0);
-  p.xmask = low_mask(g.nx - 1 - x0);
+  // A grid without cells (fewer than 2 points along an axis) has no output:
+  // the reference returns before generating when the triangle count is 0
+  // (DispatcherGenerateInterpolatedCells.h:190-194), so no edge counts.
+  const bool has_cells = g.nx > 1 && g.ny > 1 && g.nz > 1;
+  p.pmask = has_cells ? low_mask(g.nx - x0) : 0ull;
+  p.xmask = has_cells ? low_mask(g.nx - 1 - x0) : 0ull;
   return p;
 }
 

@@ -62,6 +62,8 @@ class Contour:
     interp_first: Optional[torch.Tensor]
     interp_second: Optional[torch.Tensor]
     interp_weight: Optional[torch.Tensor]
+    workspace: Optional[torch.Tensor] = None   # inside-bits + vertex bases (for backward)
+    counts: Optional[torch.Tensor] = None      # device {T, U}
 
     @property
     def num_triangles(self) -> int:
@@ -187,7 +189,34 @@ def forward(field: torch.Tensor, iso: float, origin=(0, 0, 0), spacing=(1, 1, 1)
         edges=None if buf.edges is None else buf.edges[:U],
         interp_first=None if buf.interp_first is None else buf.interp_first[:U],
         interp_second=None if buf.interp_second is None else buf.interp_second[:U],
-        interp_weight=None if buf.interp_weight is None else buf.interp_weight[:U])
+        interp_weight=None if buf.interp_weight is None else buf.interp_weight[:U],
+        workspace=buf.workspace, counts=buf.counts)
+
+
+def backward(field: torch.Tensor, iso: float, contour: Contour, dl_dvertices: torch.Tensor,
+             dl_dnormals: torch.Tensor, origin=(0, 0, 0), spacing=(1, 1, 1),
+             out: Optional[torch.Tensor] = None, scratch: Optional[torch.Tensor] = None):
+    """dL/dfield for upstream dL/dvertices, dL/dnormals ([U, 3] f32) of a
+    forward run with edges=True on the same field and iso."""
+    nx, ny, nz = _check_field(field)
+    if contour.edges is None or contour.workspace is None:
+        raise ValueError("backward needs a forward with edges=True (dedup)")
+    U = contour.num_points
+    for t in (dl_dvertices, dl_dnormals):
+        if t.shape != (U, 3) or t.dtype != torch.float32 or not t.is_contiguous():
+            raise ValueError("upstream gradients must be contiguous float32 [U, 3]")
+    lib = _lib.load()
+    nbytes = int(lib.dax_hip_mc_backward_scratch_bytes(U))
+    if scratch is None or scratch.numel() < nbytes:
+        scratch = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=field.device)
+    if out is None:
+        out = torch.empty_like(field)
+    _lib.call("dax_hip_mc_backward", field.data_ptr(), nx, ny, nz, _f3(origin), _f3(spacing),
+              float(iso), contour.workspace.data_ptr(), contour.workspace.numel(),
+              contour.edges.data_ptr(), contour.counts.data_ptr(), U, dl_dvertices.data_ptr(),
+              dl_dnormals.data_ptr(), scratch.data_ptr(), scratch.numel(), out.data_ptr(),
+              _stream())
+    return out
 
 
 def interpolate_scalar(values: torch.Tensor, contour: Contour) -> torch.Tensor:

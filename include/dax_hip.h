@@ -171,6 +171,24 @@ int dax_hip_mc_interpolate_scalar(const float* d_in, const int32_t* d_first,
                                   const int32_t* d_second, const float* d_weight,
                                   int64_t n, float* d_out, void* stream);
 
+/* ---- adjoint (no reference counterpart) ---------------------------------
+ * dL/dfield of L(vertices, normals) given dL/dvertices [U][3] and
+ * dL/dnormals [U][3], with the topology of the forward held fixed.  Uses the
+ * forward's workspace (inside-bits and vertex bases must still hold the
+ * forward of this field and iso) and its compact edge list d_edges; the
+ * interpolants (t, endpoint gradients) are recomputed from the field.
+ * U = min(d_counts[1], n_points) if d_counts is given (device {T, U} of the
+ * forward), else n_points.  d_scratch needs
+ * dax_hip_mc_backward_scratch_bytes(n_points) bytes.  Deterministic: no
+ * atomics, fixed summation order.  d_dl_dfield [P] f32 is fully written. */
+size_t dax_hip_mc_backward_scratch_bytes(int64_t n_points);
+int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
+                        const float* h_origin3, const float* h_spacing3, float iso,
+                        const void* d_workspace, size_t workspace_bytes, const uint32_t* d_edges,
+                        const int64_t* d_counts, int64_t n_points, const float* d_dl_dvertices,
+                        const float* d_dl_dnormals, void* d_scratch, size_t scratch_bytes,
+                        float* d_dl_dfield, void* stream);
+
 /* Central-difference point gradient of the whole field, [P][3] f32 (interior
  * (f(p+e)-f(p-e))/(2h), one-sided on the first/last plane).  No reference
  * counterpart; the normals of dax_hip_mc_forward use the same stencil. */

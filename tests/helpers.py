@@ -7,10 +7,10 @@ import numpy as np
 def rel_err_vec(a: np.ndarray, b: np.ndarray, floor: float = 1.0) -> float:
     """Max over rows of |a-b|_inf / max(|b|_inf, floor): relative error of
     each vector against its magnitude (floor: the grid-spacing scale)."""
+    if np.asarray(a).size == 0 and np.asarray(b).size == 0:
+        return 0.0
     a = np.asarray(a, np.float64).reshape(len(a), -1)
     b = np.asarray(b, np.float64).reshape(len(b), -1)
-    if a.size == 0:
-        return 0.0
     num = np.abs(a - b).max(axis=1)
     den = np.maximum(np.abs(b).max(axis=1), floor)
     return float((num / den).max())
