@@ -85,6 +85,7 @@ def main():
     T, U = mc.read_counts(cnt)
     del ws
     buf = mc.ForwardBuffers(n, n, n, U, T, normals=True, edges=True)
+    tuned, tuned_ms = mc.autotune(field, iso, buf, reps=10)
 
     for _ in range(args.warmup):
         mc.forward_into(field, iso, buf)
@@ -141,6 +142,8 @@ def main():
         "kernel_ms": {"classify": round(k_ms[0], 5), "count_scan": round(k_ms[1], 5),
                       "generate": round(k_ms[2], 5)},
         "wall_s": round(wall, 4),
+        "tuning": {"classify_unroll": tuned[0], "classify_vec": tuned[1], "scan_sub": tuned[2],
+                   "gen_tile": tuned[3], "autotune_ms": [round(x, 5) for x in tuned_ms]},
         "roofline": {"bound": "hbm", "kernel": "k_classify", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},

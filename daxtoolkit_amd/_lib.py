@@ -73,6 +73,12 @@ _SIGNATURES = {
                                       vp, vp, vp, vp, vp, vp, vp, i64, i64, vp, vp]),
     "dax_hip_mc_cell_outputs": (C.c_int, [i64, i64, i64, vp, sz, vp, vp, vp, vp]),
     "dax_hip_mc_interpolate_scalar": (C.c_int, [vp, vp, vp, vp, i64, vp, vp]),
+    "dax_hip_mc_set_tuning": (C.c_int, [i64, i64, i64, vp]),
+    "dax_hip_mc_get_tuning": (C.c_int, [i64, i64, i64, vp]),
+    "dax_hip_mc_autotune": (C.c_int, [vp, i64, i64, i64, f32, C.c_uint, vp, sz, vp, vp, vp, vp,
+                                      i64, i64, vp, C.c_int, vp, vp, vp]),
+    "dax_hip_mc_tuning_save": (C.c_int, [C.c_char_p]),
+    "dax_hip_mc_tuning_load": (C.c_int, [C.c_char_p]),
     "dax_hip_mc_backward_scratch_bytes": (sz, [i64]),
     "dax_hip_mc_backward": (C.c_int, [vp, i64, i64, i64, vp, vp, f32, vp, sz, vp, vp, i64,
                                       vp, vp, vp, sz, vp, vp]),

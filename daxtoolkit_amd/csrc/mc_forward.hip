@@ -15,327 +15,65 @@
 // produces, so no sort is needed.  The count/scan kernel is a single pass
 // with decoupled look-back; the generate kernel revisits only tiles that
 // have output and touches the field only around the surface.
+//
+// Tile shapes (classify unroll and load width, scan sub-tiles, generate tile
+// words) are chosen per grid shape by the autotuner at the end of this file.
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
 
-#include "mc_common.h"
+#include "mc_kernels.h"
 
 namespace daxhip {
 namespace mc {
 
-__constant__ uint64_t c_case_table[256] = DAX_HIP_MC_CASE_TABLE_INIT;
-
-// ---------------------------------------------------------------------------
-// K1: classify.  One wavefront builds UNROLL row words; lane l reads point
-// 64q+l of the word's row, a ballot of (f > iso) is the word.  The first
-// threads also clear the scan state of the following count/scan launch.
 template <int UNROLL, bool ALIGNED>
 __global__ __launch_bounds__(256) void k_classify(VolGeom g, const float* __restrict__ field,
                                                   uint64_t* __restrict__ bits,
                                                   unsigned long long* __restrict__ zero_buf,
                                                   int64_t zero_words) {
-  const int lane = threadIdx.x & 63;
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gtid < zero_words) zero_buf[gtid] = 0ull;
-  const int64_t wave = gtid >> 6;
-  const int64_t w0 = wave * UNROLL;
-  const float* f = field + g.field_off;
-  float v[UNROLL];
-  bool ok[UNROLL];
-#pragma unroll
-  for (int u = 0; u < UNROLL; ++u) {
-    const int64_t w = w0 + u;
-    ok[u] = false;
-    v[u] = 0.0f;
-    if (w < g.nwords) {
-      if (ALIGNED) {
-        v[u] = __builtin_nontemporal_load(f + w * 64 + lane);
-        ok[u] = true;
-      } else {
-        const int64_t row = w / g.wpr;
-        const int x = (int)(w - row * g.wpr) * 64 + lane;
-        if (x < g.nx) {
-          v[u] = __builtin_nontemporal_load(f + row * g.nx + x);
-          ok[u] = true;
-        }
-      }
-    }
-  }
-  uint64_t mine = 0;
-#pragma unroll
-  for (int u = 0; u < UNROLL; ++u) {
-    const uint64_t b = __ballot(ok[u] && v[u] > g.iso);
-    if (lane == u) mine = b;
-  }
-  if (lane < UNROLL && w0 + lane < g.nwords) bits[g.word_off + w0 + lane] = mine;
+  const int64_t w0 = (gtid >> 6) * UNROLL;
+  classify_words<UNROLL, ALIGNED>(g, field + g.field_off, bits + g.word_off, w0, g.nwords);
 }
 
-// ---------------------------------------------------------------------------
-// K2: per-word counts and the single-pass exclusive scan.
-//
-// One block per tile of kTileWords words; tiles are taken in order from an
-// atomic counter so every predecessor of a tile has started.  The tile's
-// words (plus the halo words of the +x/+y/+z neighbours) are staged in LDS.
-// Counts: active edges = popcount of the x/y/z flags; triangles =
-// sum of NumFaces[case] over the active cells (mixed-corner cells found with
-// 64-bit word ops, so all-in / all-out cells cost nothing).
-// Look-back: wave 0 reads 64 predecessor tile states at a time.
-__device__ __forceinline__ uint32_t faces_of(const uint8_t* s_faces, int c) { return s_faces[c]; }
+template <int UNROLL>
+__global__ __launch_bounds__(256) void k_classify_v4(VolGeom g, const float* __restrict__ field,
+                                                     uint64_t* __restrict__ bits,
+                                                     unsigned long long* __restrict__ zero_buf,
+                                                     int64_t zero_words) {
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gtid < zero_words) zero_buf[gtid] = 0ull;
+  const int64_t c0 = (gtid >> 6) * UNROLL;
+  classify_chunks_v4<UNROLL>(g, field + g.field_off, bits + g.word_off, c0, g.nwords);
+}
 
+template <int SUB>
 __global__ __launch_bounds__(kScanThreads) void k_count_scan(
     VolGeom g, const uint64_t* __restrict__ bits, uint32_t* __restrict__ base_v,
     uint32_t* __restrict__ base_t, unsigned long long* __restrict__ status, Ctrl* __restrict__ ctrl,
     int64_t ntiles) {
-  __shared__ uint64_t s_b0[kTileWords + kMaxWpr + 2];
-  __shared__ uint64_t s_b1[kTileWords + kMaxWpr + 2];
-  __shared__ uint8_t s_faces[256];
-  __shared__ uint32_t s_wave_v[kScanThreads / 64], s_wave_t[kScanThreads / 64];
-  __shared__ uint32_t s_prefix[2];
+  __shared__ ScanShared sh;
   __shared__ int64_t s_tile;
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6;
-  if (tid == 0) s_tile = (int64_t)atomicAdd(&ctrl->tile_counter, 1u);
-  s_faces[tid] = (uint8_t)(c_case_table[tid] >> 60);
+  if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(&ctrl->tile_counter, 1u);
+  sh.faces[threadIdx.x] = (uint8_t)(c_case_table[threadIdx.x] >> 60);
   __syncthreads();
   const int64_t tile = s_tile;
   if (tile >= ntiles) return;  // scan state not reset by a classify launch
-  const int64_t w_begin = tile * kTileWords;
-  const int nw = (int)min((int64_t)kTileWords, g.nwords - w_begin);
-  const uint64_t* b = bits + g.word_off;
-
-  // stage plane k (rows j, j+1 and one word past) and plane k+1
-  const int seg = nw + g.wpr + 1;
-  const int64_t plane = (int64_t)g.wpr * g.ny;
-  for (int i = tid; i < seg; i += kScanThreads) {
-    const int64_t w = w_begin + i;
-    s_b0[i] = (w < g.nwords) ? b[w] : 0ull;
-    s_b1[i] = (w + plane < g.nwords) ? b[w + plane] : 0ull;
-  }
-  __syncthreads();
-
-  uint32_t cv[kWordsPerThread], ct[kWordsPerThread];
-  uint32_t sum_v = 0, sum_t = 0;
-#pragma unroll
-  for (int u = 0; u < kWordsPerThread; ++u) {
-    const int i = tid * kWordsPerThread + u;
-    cv[u] = 0;
-    ct[u] = 0;
-    if (i < nw) {
-      const WordPos p = word_pos(g, w_begin + i);
-      const bool nx1 = p.q + 1 < g.wpr;
-      const uint64_t A = s_b0[i], A1 = nx1 ? s_b0[i + 1] : 0ull;
-      const uint64_t B = s_b0[i + g.wpr], B1 = nx1 ? s_b0[i + g.wpr + 1] : 0ull;
-      const uint64_t C = s_b1[i], C1 = nx1 ? s_b1[i + 1] : 0ull;
-      const uint64_t D = s_b1[i + g.wpr], D1 = nx1 ? s_b1[i + g.wpr + 1] : 0ull;
-      uint64_t fx, fy, fz;
-      edge_flags(p, A, A1, B, C, fx, fy, fz);
-      cv[u] = (uint32_t)(popc(fx) + popc(fy) + popc(fz));
-      CellPlanes cp;
-      cell_planes(p, A, A1, B, B1, C, C1, D, D1, cp);
-      uint64_t act = cp.active;
-      uint32_t t = 0;
-      while (act) {
-        const int bit = __builtin_ctzll(act);
-        act &= act - 1;
-        t += faces_of(s_faces, cell_case(cp, bit));
-      }
-      ct[u] = t;
-    }
-    sum_v += cv[u];
-    sum_t += ct[u];
-  }
-
-  // block exclusive scan of the per-thread sums
-  const uint32_t inc_v = wave_incl_scan(sum_v), inc_t = wave_incl_scan(sum_t);
-  if (lane == 63) {
-    s_wave_v[wv] = inc_v;
-    s_wave_t[wv] = inc_t;
-  }
-  __syncthreads();
-  uint32_t wave_off_v = 0, wave_off_t = 0, agg_v = 0, agg_t = 0;
-#pragma unroll
-  for (int q = 0; q < kScanThreads / 64; ++q) {
-    if (q < wv) {
-      wave_off_v += s_wave_v[q];
-      wave_off_t += s_wave_t[q];
-    }
-    agg_v += s_wave_v[q];
-    agg_t += s_wave_t[q];
-  }
-
-  // decoupled look-back (wave 0)
-  if (wv == 0) {
-    uint32_t ex_v = 0, ex_t = 0;
-    if (tile == 0) {
-      if (lane == 0)
-        __hip_atomic_store(&status[0], pack_status(agg_v, agg_t, kStatusPrefix), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      if (lane == 0)
-        __hip_atomic_store(&status[tile], pack_status(agg_v, agg_t, kStatusAgg), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      int64_t pos = tile - 1;
-      while (true) {
-        const int64_t idx = pos - lane;
-        unsigned long long s = pack_status(0, 0, kStatusPrefix);
-        if (idx >= 0) {
-          s = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          while ((s >> 62) == 0ull) {
-            __builtin_amdgcn_s_sleep(1);
-            s = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          }
-        }
-        const uint64_t pm = __ballot((s >> 62) == kStatusPrefix);
-        uint32_t sv = (uint32_t)(s & 0x7fffffffull), st = (uint32_t)((s >> 31) & 0x7fffffffull);
-        if (pm) {
-          const int first = __builtin_ctzll(pm);
-          if (lane > first) {
-            sv = 0;
-            st = 0;
-          }
-        }
-        ex_v += wave_sum(sv);
-        ex_t += wave_sum(st);
-        if (pm) break;
-        pos -= 64;
-      }
-      if (lane == 0)
-        __hip_atomic_store(&status[tile], pack_status(ex_v + agg_v, ex_t + agg_t, kStatusPrefix),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (lane == 0) {
-      s_prefix[0] = ex_v;
-      s_prefix[1] = ex_t;
-      atomicAdd(&ctrl->total_v, (unsigned long long)agg_v);
-      atomicAdd(&ctrl->total_t, (unsigned long long)agg_t);
-    }
-  }
-  __syncthreads();
-
-  uint32_t run_v = s_prefix[0] + wave_off_v + inc_v - sum_v;
-  uint32_t run_t = s_prefix[1] + wave_off_t + inc_t - sum_t;
-  const int64_t gw = g.word_off + w_begin;
-#pragma unroll
-  for (int u = 0; u < kWordsPerThread; ++u) {
-    const int i = tid * kWordsPerThread + u;
-    if (i < nw) {
-      base_v[gw + i] = run_v;
-      base_t[gw + i] = run_t;
-      run_v += cv[u];
-      run_t += ct[u];
-      if (w_begin + i == g.nwords - 1) {
-        base_v[gw + i + 1] = run_v;
-        base_t[gw + i + 1] = run_t;
-      }
-    }
-  }
+  count_scan_tile<SUB>(g, bits + g.word_off, base_v, base_t, status, ctrl, tile,
+                       tile * (int64_t)SUB * kTileWords, sh);
 }
 
-// ---------------------------------------------------------------------------
-// K3: generate.  One block per tile.  Stages the tile's bits for planes
-// k, k+1, k+2 and the vertex bases for planes k, k+1 in LDS, then
-//   vertices : thread per output vertex (consecutive ids -> coalesced
-//              writes): select its edge in its word, weight, coordinates
-//              (Lerp of the corner coordinates, MarchingCubes.h:160-161 and
-//              GenerateWorklets.h:65-100), central-difference normal
-//   triangles: thread per output triangle: find its cell and visit index in
-//              its word, look up the three edges in the case table
-//              (MarchingCubes.h:141-167) and turn each into its vertex id.
-struct OutPtrs {
-  float* vertices;
-  float* normals;
-  int32_t* conn;
-  uint32_t* edges;
-  int32_t* interp_first;
-  int32_t* interp_second;
-  float* interp_weight;
-};
-
-__device__ __forceinline__ float load_f(const float* f, int64_t i) { return f[i]; }
-
-// Central-difference gradient at point (x,y,z) of a volume.
-__device__ __forceinline__ void point_gradient(const VolGeom& g, const float* f, int x, int y, int z,
-                                               float out[3]) {
-  const int64_t sx = 1, sy = g.nx, sz = (int64_t)g.nx * g.ny;
-  const int64_t p = x + sy * y + sz * z;
-  const int c[3] = {x, y, z};
-  const int n[3] = {g.nx, g.ny, g.nz};
-  const int64_t st[3] = {sx, sy, sz};
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const float h = g.spacing[d];
-    if (n[d] < 2) {
-      out[d] = 0.0f;
-    } else if (c[d] == 0) {
-      out[d] = (load_f(f, p + st[d]) - load_f(f, p)) / h;
-    } else if (c[d] == n[d] - 1) {
-      out[d] = (load_f(f, p) - load_f(f, p - st[d])) / h;
-    } else {
-      out[d] = (load_f(f, p + st[d]) - load_f(f, p - st[d])) / (2.0f * h);
-    }
-  }
-}
-
-// Output record of one interpolated point on edge (lower point (x,y,z), axis).
-__device__ __forceinline__ void emit_point(const VolGeom& g, const float* f, int x, int y, int z,
-                                           int axis, bool normals, float xyz[3], float nrm[3],
-                                           float& w, int64_t& pa, int64_t& pb) {
-  const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-  pa = x + sy * y + sz * z;
-  const int64_t st = axis == 0 ? 1 : (axis == 1 ? sy : sz);
-  pb = pa + st;
-  const float fa = load_f(f, pa), fb = load_f(f, pb);
-  w = (g.iso - fa) / (fb - fa);
-  const int ca[3] = {x, y, z};
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const float a = g.origin[d] + g.spacing[d] * (float)ca[d];
-    const float b = g.origin[d] + g.spacing[d] * (float)(ca[d] + (d == axis ? 1 : 0));
-    xyz[d] = a + w * (b - a);
-  }
-  if (normals) {
-    float ga[3], gb[3];
-    point_gradient(g, f, x, y, z, ga);
-    point_gradient(g, f, x + (axis == 0), y + (axis == 1), z + (axis == 2), gb);
-    float m[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) m[d] = ga[d] + w * (gb[d] - ga[d]);
-    const float len2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
-    if (len2 > 0.0f) {
-      const float len = sqrtf(len2);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) nrm[d] = m[d] / len;
-    } else {
-      nrm[0] = nrm[1] = nrm[2] = 0.0f;
-    }
-  }
-}
-
-// upper_bound over s[0..n] (inclusive end) minus one: index i with
-// s[i] <= v < s[i+1].
-__device__ __forceinline__ int find_word(const uint32_t* s, int n, uint32_t v) {
-  int lo = 0, hi = n;  // s[lo] <= v < s[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (s[mid] <= v)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
+template <int TW>
 __global__ __launch_bounds__(kScanThreads) void k_generate(
     VolGeom g, const float* __restrict__ field, const uint64_t* __restrict__ bits,
     const uint32_t* __restrict__ base_v, const uint32_t* __restrict__ base_t,
     const Ctrl* __restrict__ ctrl, OutPtrs out, unsigned flags, int64_t cap_points,
-    int64_t cap_tris, int64_t* __restrict__ counts, int64_t vol_base_v, int64_t vol_base_t) {
-  constexpr int kSeg = kTileWords + 2 * kMaxWpr + 2;
-  __shared__ uint64_t s_b[3][kSeg];
-  __shared__ uint32_t s_v[2][kTileWords + kMaxWpr + 2];
-  __shared__ uint32_t s_t[kTileWords + 1];
-  __shared__ uint64_t s_case[256];
-
+    int64_t cap_tris, int64_t* __restrict__ counts) {
+  extern __shared__ uint64_t s_dyn[];
   const bool dedup = (flags & DAX_HIP_MC_DEDUP) != 0;
   const bool normals = (flags & DAX_HIP_MC_NORMALS) != 0 && out.normals != nullptr;
   const uint64_t total_v = ctrl->total_v, total_t = ctrl->total_t;
@@ -346,151 +84,38 @@ __global__ __launch_bounds__(kScanThreads) void k_generate(
   }
   if (total_v > 0x7fffffffull || 3ull * total_t > 0x7fffffffull) return;
   if ((int64_t)total_t > cap_tris || (int64_t)n_points > cap_points) return;
+  generate_tile<TW>(g, field + g.field_off, bits + g.word_off, base_v + g.word_off,
+                    base_t + g.word_off, (int64_t)blockIdx.x * TW, out, dedup, normals, 0, 0,
+                    s_dyn);
+}
 
-  const int tid = threadIdx.x;
-  const int64_t tile = blockIdx.x;
-  const int64_t w_begin = tile * kTileWords;
-  const int nw = (int)min((int64_t)kTileWords, g.nwords - w_begin);
-  const int64_t gw = g.word_off + w_begin;
-  const uint32_t tv0 = base_v[gw], tv1 = base_v[gw + nw];
-  const uint32_t tt0 = base_t[gw], tt1 = base_t[gw + nw];
-  if (tv0 == tv1 && tt0 == tt1) return;  // no output in this tile
+// ---------------------------------------------------------------------------
+// Tuning: per grid shape, the classify variant, scan sub-tiles and generate
+// tile words.  Defaults are used until dax_hip_mc_autotune (or
+// dax_hip_mc_set_tuning) records a choice for a shape.
+struct Tuning {
+  int cls_unroll = 16;  // words (vec 1) or 256-point chunks (vec 4) per wave
+  int cls_vec = 1;      // floats per lane per load
+  int scan_sub = 4;     // scan tile = scan_sub * 1024 words
+  int gen_tile = 512;   // generate tile words
+};
 
-  const uint64_t* b = bits + g.word_off;
-  const uint32_t* bv = base_v + g.word_off;
-  const int64_t plane = (int64_t)g.wpr * g.ny;
-  const int seg_b = nw + 2 * g.wpr + 2;
-  const int seg_v = nw + g.wpr + 2;
-  for (int i = tid; i < seg_b; i += kScanThreads) {
-    const int64_t w = w_begin + i;
-#pragma unroll
-    for (int z = 0; z < 3; ++z) {
-      const int64_t wz = w + z * plane;
-      s_b[z][i] = (wz < g.nwords) ? b[wz] : 0ull;
-    }
-  }
-  for (int i = tid; i < seg_v; i += kScanThreads) {
-    const int64_t w = w_begin + i;
-#pragma unroll
-    for (int z = 0; z < 2; ++z) {
-      const int64_t wz = w + z * plane;
-      s_v[z][i] = (wz <= g.nwords) ? bv[wz] : 0u;
-    }
-  }
-  for (int i = tid; i <= nw; i += kScanThreads) s_t[i] = base_t[gw + i];
-  s_case[tid] = c_case_table[tid];
-  __syncthreads();
+std::mutex g_tune_mu;
+Tuning g_tune_default;
+std::map<std::tuple<int64_t, int64_t, int64_t>, Tuning> g_tune_cache;
 
-  const float* f = field + g.field_off;
+Tuning tuning_for(int64_t nx, int64_t ny, int64_t nz) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune_cache.find(std::make_tuple(nx, ny, nz));
+  return it == g_tune_cache.end() ? g_tune_default : it->second;
+}
 
-  // ---- vertices (dedup: one per active edge)
-  if (dedup) {
-    for (uint32_t v = tv0 + tid; v < tv1; v += kScanThreads) {
-      const int li = find_word(s_v[0], nw, v);
-      const WordPos p = word_pos(g, w_begin + li);
-      const bool nx1 = p.q + 1 < g.wpr;
-      const uint64_t A = s_b[0][li], A1 = nx1 ? s_b[0][li + 1] : 0ull;
-      const uint64_t B = s_b[0][li + g.wpr], C = s_b[1][li];
-      uint64_t fx, fy, fz;
-      edge_flags(p, A, A1, B, C, fx, fy, fz);
-      int bit, axis;
-      edge_select(fx, fy, fz, (int)(v - s_v[0][li]), bit, axis);
-      const int x = p.q * 64 + bit;
-      float xyz[3], nrm[3], w;
-      int64_t pa, pb;
-      emit_point(g, f, x, p.j, p.k, axis, normals, xyz, nrm, w, pa, pb);
-      const int64_t ov = (int64_t)v - vol_base_v;
-      const int64_t gv = (int64_t)v;
-      out.vertices[3 * gv + 0] = xyz[0];
-      out.vertices[3 * gv + 1] = xyz[1];
-      out.vertices[3 * gv + 2] = xyz[2];
-      if (normals) {
-        out.normals[3 * gv + 0] = nrm[0];
-        out.normals[3 * gv + 1] = nrm[1];
-        out.normals[3 * gv + 2] = nrm[2];
-      }
-      if (out.edges) out.edges[gv] = ((uint32_t)axis << 30) | (uint32_t)pa;
-      if (out.interp_first) out.interp_first[gv] = (int32_t)pa;
-      if (out.interp_second) out.interp_second[gv] = (int32_t)pb;
-      if (out.interp_weight) out.interp_weight[gv] = w;
-      (void)ov;
-    }
-  }
-
-  // ---- triangles
-  for (uint32_t t = tt0 + tid; t < tt1; t += kScanThreads) {
-    const int li = find_word(s_t, nw, t);
-    const WordPos p = word_pos(g, w_begin + li);
-    const bool nx1 = p.q + 1 < g.wpr;
-    CellPlanes cp;
-    cell_planes(p, s_b[0][li], nx1 ? s_b[0][li + 1] : 0ull, s_b[0][li + g.wpr],
-                nx1 ? s_b[0][li + g.wpr + 1] : 0ull, s_b[1][li], nx1 ? s_b[1][li + 1] : 0ull,
-                s_b[1][li + g.wpr], nx1 ? s_b[1][li + g.wpr + 1] : 0ull, cp);
-    int k = (int)(t - s_t[li]);
-    uint64_t act = cp.active;
-    int bit = 0, cs = 0, nf = 0;
-    bool found = false;
-    while (act) {
-      bit = __builtin_ctzll(act);
-      act &= act - 1;
-      cs = cell_case(cp, bit);
-      nf = (int)(s_case[cs] >> 60);
-      if (k < nf) {
-        found = true;
-        break;
-      }
-      k -= nf;
-    }
-    if (!found) continue;  // unreachable when the scan and this kernel agree
-    const uint64_t packed = s_case[cs];
-    int32_t ids[3];
-#pragma unroll
-    for (int v = 0; v < 3; ++v) {
-      const int e = (int)((packed >> (4 * (3 * k + v))) & 0xF);
-      const uint8_t code = edge_code(e);
-      const int dx = code & 1, dy = (code >> 1) & 1, dz = (code >> 2) & 1, axis = code >> 3;
-      const int x = p.q * 64 + bit + dx;
-      if (dedup) {
-        // the edge's word: same row word or the next one in x, row +dy, plane +dz
-        const int qq = x >> 6, bb = x & 63;
-        const int lw2 = li + (qq - p.q) + dy * g.wpr;
-        WordPos p2;
-        p2.q = qq;
-        p2.j = p.j + dy;
-        p2.k = p.k + dz;
-        p2.has_y = p2.j + 1 < g.ny;
-        p2.has_z = p2.k + 1 < g.nz;
-        p2.pmask = low_mask(g.nx - qq * 64);
-        p2.xmask = low_mask(g.nx - 1 - qq * 64);
-        const bool nx2 = qq + 1 < g.wpr;
-        const uint64_t A = s_b[dz][lw2], A1 = nx2 ? s_b[dz][lw2 + 1] : 0ull;
-        const uint64_t B = s_b[dz][lw2 + g.wpr], C = s_b[dz + 1][lw2];
-        uint64_t fx, fy, fz;
-        edge_flags(p2, A, A1, B, C, fx, fy, fz);
-        ids[v] = (int32_t)((int64_t)s_v[dz][lw2] + edge_rank(fx, fy, fz, bb, axis) - vol_base_v);
-      } else {
-        const int64_t corner = 3 * (int64_t)t + v;
-        float xyz[3], nrm[3], w;
-        int64_t pa, pb;
-        emit_point(g, f, x, p.j + dy, p.k + dz, axis, normals, xyz, nrm, w, pa, pb);
-        out.vertices[3 * corner + 0] = xyz[0];
-        out.vertices[3 * corner + 1] = xyz[1];
-        out.vertices[3 * corner + 2] = xyz[2];
-        if (normals) {
-          out.normals[3 * corner + 0] = nrm[0];
-          out.normals[3 * corner + 1] = nrm[1];
-          out.normals[3 * corner + 2] = nrm[2];
-        }
-        if (out.interp_first) out.interp_first[corner] = (int32_t)pa;
-        if (out.interp_second) out.interp_second[corner] = (int32_t)pb;
-        if (out.interp_weight) out.interp_weight[corner] = w;
-        ids[v] = (int32_t)(corner - 3 * vol_base_t);
-      }
-    }
-    out.conn[3 * (int64_t)t + 0] = ids[0];
-    out.conn[3 * (int64_t)t + 1] = ids[1];
-    out.conn[3 * (int64_t)t + 2] = ids[2];
-  }
+bool valid_tuning(const Tuning& t) {
+  const bool cls = (t.cls_vec == 1 && (t.cls_unroll == 8 || t.cls_unroll == 16 || t.cls_unroll == 32)) ||
+                   (t.cls_vec == 4 && (t.cls_unroll == 2 || t.cls_unroll == 4 || t.cls_unroll == 8));
+  const bool scan = t.scan_sub == 1 || t.scan_sub == 2 || t.scan_sub == 4 || t.scan_sub == 8;
+  const bool gen = t.gen_tile == 256 || t.gen_tile == 512 || t.gen_tile == 1024;
+  return cls && scan && gen;
 }
 
 // ---------------------------------------------------------------------------
@@ -637,44 +262,84 @@ int check_ws(const VolGeom& g, size_t ws_bytes, Workspace& L, void* ws) {
   return DAX_HIP_OK;
 }
 
-constexpr int kClassifyUnroll = 16;
-
-int launch_bits(const VolGeom& g, const float* field, char* ws, const Workspace& L, hipStream_t s) {
+int launch_bits(const VolGeom& g, const float* field, char* ws, const Workspace& L,
+                const Tuning& tu, hipStream_t s) {
   uint64_t* bits = (uint64_t*)(ws + L.off_bits);
   unsigned long long* status = (unsigned long long*)(ws + L.off_status);
   const int64_t zero_words = L.ntiles + (int64_t)(sizeof(Ctrl) / 8);
-  const int64_t waves = ceil_div(g.nwords, kClassifyUnroll);
-  int64_t blocks = ceil_div(waves * 64, 256);
-  blocks = std::max<int64_t>(blocks, ceil_div(zero_words, 256));
   const bool aligned = (g.nx % 64) == 0;
-  if (aligned)
-    k_classify<kClassifyUnroll, true><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
-        g, field, bits, status, zero_words);
-  else
-    k_classify<kClassifyUnroll, false><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
-        g, field, bits, status, zero_words);
+  const bool aligned16 = aligned && (g.field_off % 4) == 0 && ((uintptr_t)field % 16) == 0;
+  const int vec = aligned16 ? tu.cls_vec : 1;
+  const int unroll = (vec == 1 && tu.cls_vec != 1) ? 16 : tu.cls_unroll;
+  const int64_t units = vec == 4 ? ceil_div(g.nwords, 4) : g.nwords;  // chunks or words
+  const int64_t waves = ceil_div(units, unroll);
+  int64_t blocks = std::max<int64_t>(ceil_div(waves * 64, 256), ceil_div(zero_words, 256));
+  const dim3 grid((unsigned)blocks), block(256);
+#define DAXHIP_CLS(U, A) k_classify<U, A><<<grid, block, 0, s>>>(g, field, bits, status, zero_words)
+#define DAXHIP_CLS4(U) k_classify_v4<U><<<grid, block, 0, s>>>(g, field, bits, status, zero_words)
+  if (vec == 4) {
+    if (unroll == 2) DAXHIP_CLS4(2);
+    else if (unroll == 4) DAXHIP_CLS4(4);
+    else DAXHIP_CLS4(8);
+  } else if (aligned) {
+    if (unroll == 8) DAXHIP_CLS(8, true);
+    else if (unroll == 32) DAXHIP_CLS(32, true);
+    else DAXHIP_CLS(16, true);
+  } else {
+    if (unroll == 8) DAXHIP_CLS(8, false);
+    else if (unroll == 32) DAXHIP_CLS(32, false);
+    else DAXHIP_CLS(16, false);
+  }
+#undef DAXHIP_CLS
+#undef DAXHIP_CLS4
   DAXHIP_LAUNCH_CHECK("k_classify");
   return DAX_HIP_OK;
 }
 
-int launch_scan(const VolGeom& g, char* ws, const Workspace& L, hipStream_t s) {
-  k_count_scan<<<dim3((unsigned)L.ntiles), dim3(kScanThreads), 0, s>>>(
-      g, (const uint64_t*)(ws + L.off_bits), (uint32_t*)(ws + L.off_base_v),
-      (uint32_t*)(ws + L.off_base_t), (unsigned long long*)(ws + L.off_status),
-      (Ctrl*)(ws + L.off_ctrl), L.ntiles);
+int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu, hipStream_t s) {
+  const int sub = tu.scan_sub;
+  const int64_t ntiles = ceil_div(g.nwords, (int64_t)sub * kTileWords);
+  const dim3 grid((unsigned)ntiles), block(kScanThreads);
+  const uint64_t* bits = (const uint64_t*)(ws + L.off_bits);
+  uint32_t* bv = (uint32_t*)(ws + L.off_base_v);
+  uint32_t* bt = (uint32_t*)(ws + L.off_base_t);
+  unsigned long long* st = (unsigned long long*)(ws + L.off_status);
+  Ctrl* ctrl = (Ctrl*)(ws + L.off_ctrl);
+  switch (sub) {
+    case 1: k_count_scan<1><<<grid, block, 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
+    case 2: k_count_scan<2><<<grid, block, 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
+    case 8: k_count_scan<8><<<grid, block, 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
+    default: k_count_scan<4><<<grid, block, 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
+  }
   DAXHIP_LAUNCH_CHECK("k_count_scan");
+  return DAX_HIP_OK;
+}
+
+template <int TW>
+int launch_generate_tw(const VolGeom& g, const float* field, const char* ws, const Workspace& L,
+                       unsigned flags, const OutPtrs& o, int64_t cap_points, int64_t cap_tris,
+                       int64_t* counts, hipStream_t s) {
+  const size_t lds = GenLayout<TW>(g.wpr).bytes();
+  const int64_t ntiles = ceil_div(g.nwords, (int64_t)TW);
+  k_generate<TW><<<dim3((unsigned)std::max<int64_t>(ntiles, 1)), dim3(kScanThreads), lds, s>>>(
+      g, field, (const uint64_t*)(ws + L.off_bits), (const uint32_t*)(ws + L.off_base_v),
+      (const uint32_t*)(ws + L.off_base_t), (const Ctrl*)(ws + L.off_ctrl), o, flags, cap_points,
+      cap_tris, counts);
+  DAXHIP_LAUNCH_CHECK("k_generate");
   return DAX_HIP_OK;
 }
 
 int launch_generate(const VolGeom& g, const float* field, const char* ws, const Workspace& L,
                     unsigned flags, const OutPtrs& o, int64_t cap_points, int64_t cap_tris,
-                    int64_t* counts, hipStream_t s) {
-  k_generate<<<dim3((unsigned)L.ntiles), dim3(kScanThreads), 0, s>>>(
-      g, field, (const uint64_t*)(ws + L.off_bits), (const uint32_t*)(ws + L.off_base_v),
-      (const uint32_t*)(ws + L.off_base_t), (const Ctrl*)(ws + L.off_ctrl), o, flags, cap_points,
-      cap_tris, counts, 0, 0);
-  DAXHIP_LAUNCH_CHECK("k_generate");
-  return DAX_HIP_OK;
+                    int64_t* counts, const Tuning& tu, hipStream_t s) {
+  switch (tu.gen_tile) {
+    case 256:
+      return launch_generate_tw<256>(g, field, ws, L, flags, o, cap_points, cap_tris, counts, s);
+    case 1024:
+      return launch_generate_tw<1024>(g, field, ws, L, flags, o, cap_points, cap_tris, counts, s);
+    default:
+      return launch_generate_tw<512>(g, field, ws, L, flags, o, cap_points, cap_tris, counts, s);
+  }
 }
 
 int check_outputs(const VolGeom& g, const float* field, unsigned flags, const float* vertices,
@@ -716,7 +381,8 @@ int dax_hip_mc_bits(const float* d_field, int64_t nx, int64_t ny, int64_t nz, fl
     set_error("dax_hip_mc_bits: field is NULL");
     return DAX_HIP_BAD_VALUE;
   }
-  return launch_bits(g, d_field, (char*)d_workspace, L, (hipStream_t)stream);
+  return launch_bits(g, d_field, (char*)d_workspace, L, tuning_for(nx, ny, nz),
+                     (hipStream_t)stream);
 }
 
 int dax_hip_mc_scan(int64_t nx, int64_t ny, int64_t nz, void* d_workspace, size_t workspace_bytes,
@@ -726,7 +392,7 @@ int dax_hip_mc_scan(int64_t nx, int64_t ny, int64_t nz, void* d_workspace, size_
   if (rc) return rc;
   Workspace L;
   if ((rc = check_ws(g, workspace_bytes, L, d_workspace))) return rc;
-  return launch_scan(g, (char*)d_workspace, L, (hipStream_t)stream);
+  return launch_scan(g, (char*)d_workspace, L, tuning_for(nx, ny, nz), (hipStream_t)stream);
 }
 
 int dax_hip_mc_generate(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
@@ -745,7 +411,7 @@ int dax_hip_mc_generate(const float* d_field, int64_t nx, int64_t ny, int64_t nz
   OutPtrs o{d_vertices, d_normals, d_conn, d_edges, d_interp_first, d_interp_second,
             d_interp_weight};
   return launch_generate(g, d_field, (const char*)d_workspace, L, flags, o, cap_points, cap_tris,
-                         d_counts, (hipStream_t)stream);
+                         d_counts, tuning_for(nx, ny, nz), (hipStream_t)stream);
 }
 
 int dax_hip_mc_classify(const float* d_field, int64_t nx, int64_t ny, int64_t nz, float iso,
@@ -762,8 +428,9 @@ int dax_hip_mc_classify(const float* d_field, int64_t nx, int64_t ny, int64_t nz
   }
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)d_workspace;
-  if ((rc = launch_bits(g, d_field, ws, L, s))) return rc;
-  if ((rc = launch_scan(g, ws, L, s))) return rc;
+  const Tuning tu = tuning_for(nx, ny, nz);
+  if ((rc = launch_bits(g, d_field, ws, L, tu, s))) return rc;
+  if ((rc = launch_scan(g, ws, L, tu, s))) return rc;
   if (d_counts) {
     k_write_counts<<<1, 1, 0, s>>>((const Ctrl*)(ws + L.off_ctrl), d_counts, DAX_HIP_MC_DEDUP);
     DAXHIP_LAUNCH_CHECK("k_write_counts");
@@ -786,11 +453,12 @@ int dax_hip_mc_forward(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
   if ((rc = check_outputs(g, d_field, flags, d_vertices, d_conn, d_edges, d_counts))) return rc;
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)d_workspace;
-  if ((rc = launch_bits(g, d_field, ws, L, s))) return rc;
-  if ((rc = launch_scan(g, ws, L, s))) return rc;
+  const Tuning tu = tuning_for(nx, ny, nz);
+  if ((rc = launch_bits(g, d_field, ws, L, tu, s))) return rc;
+  if ((rc = launch_scan(g, ws, L, tu, s))) return rc;
   OutPtrs o{d_vertices, d_normals, d_conn, d_edges, d_interp_first, d_interp_second,
             d_interp_weight};
-  return launch_generate(g, d_field, ws, L, flags, o, cap_points, cap_tris, d_counts, s);
+  return launch_generate(g, d_field, ws, L, flags, o, cap_points, cap_tris, d_counts, tu, s);
 }
 
 int dax_hip_mc_read_counts(const int64_t* d_counts, int64_t cap_points, int64_t cap_tris,
@@ -864,6 +532,195 @@ int dax_hip_cell_gradient(const float* d_field, int64_t nx, int64_t ny, int64_t 
   k_cell_gradient<<<dim3((unsigned)ceil_div(C, 256)), dim3(256), 0, (hipStream_t)stream>>>(
       g, d_field, d_grad);
   DAXHIP_LAUNCH_CHECK("k_cell_gradient");
+  return DAX_HIP_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Tuning and autotuning
+namespace {
+
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  EventPair() {
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+  }
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  float ms() {
+    float m = 0.0f;
+    (void)hipEventSynchronize(b);
+    (void)hipEventElapsedTime(&m, a, b);
+    return m;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int dax_hip_mc_set_tuning(int64_t nx, int64_t ny, int64_t nz, const int* h_params4) {
+  if (!h_params4) {
+    set_error("dax_hip_mc_set_tuning: NULL parameters");
+    return DAX_HIP_BAD_VALUE;
+  }
+  Tuning t;
+  t.cls_unroll = h_params4[0];
+  t.cls_vec = h_params4[1];
+  t.scan_sub = h_params4[2];
+  t.gen_tile = h_params4[3];
+  if (!valid_tuning(t)) {
+    set_error("dax_hip_mc_set_tuning: invalid parameters {%d,%d,%d,%d}", t.cls_unroll, t.cls_vec,
+              t.scan_sub, t.gen_tile);
+    return DAX_HIP_BAD_VALUE;
+  }
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (nx == 0 && ny == 0 && nz == 0)
+    g_tune_default = t;
+  else
+    g_tune_cache[std::make_tuple(nx, ny, nz)] = t;
+  return DAX_HIP_OK;
+}
+
+int dax_hip_mc_get_tuning(int64_t nx, int64_t ny, int64_t nz, int* h_params4) {
+  const Tuning t = (nx == 0 && ny == 0 && nz == 0) ? g_tune_default : tuning_for(nx, ny, nz);
+  h_params4[0] = t.cls_unroll;
+  h_params4[1] = t.cls_vec;
+  h_params4[2] = t.scan_sub;
+  h_params4[3] = t.gen_tile;
+  return DAX_HIP_OK;
+}
+
+int dax_hip_mc_tuning_save(const char* path) {
+  FILE* fp = fopen(path, "w");
+  if (!fp) {
+    set_error("dax_hip_mc_tuning_save: cannot open %s", path);
+    return DAX_HIP_BAD_VALUE;
+  }
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  fprintf(fp, "# nx ny nz classify_unroll classify_vec scan_sub gen_tile\n");
+  for (const auto& kv : g_tune_cache)
+    fprintf(fp, "%lld %lld %lld %d %d %d %d\n", (long long)std::get<0>(kv.first),
+            (long long)std::get<1>(kv.first), (long long)std::get<2>(kv.first), kv.second.cls_unroll,
+            kv.second.cls_vec, kv.second.scan_sub, kv.second.gen_tile);
+  fclose(fp);
+  return DAX_HIP_OK;
+}
+
+int dax_hip_mc_tuning_load(const char* path) {
+  FILE* fp = fopen(path, "r");
+  if (!fp) {
+    set_error("dax_hip_mc_tuning_load: cannot open %s", path);
+    return DAX_HIP_BAD_VALUE;
+  }
+  char line[256];
+  while (fgets(line, sizeof(line), fp)) {
+    if (line[0] == '#') continue;
+    long long x, y, z;
+    Tuning t;
+    if (sscanf(line, "%lld %lld %lld %d %d %d %d", &x, &y, &z, &t.cls_unroll, &t.cls_vec,
+               &t.scan_sub, &t.gen_tile) == 7 &&
+        valid_tuning(t)) {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune_cache[std::make_tuple((int64_t)x, (int64_t)y, (int64_t)z)] = t;
+    }
+  }
+  fclose(fp);
+  return DAX_HIP_OK;
+}
+
+int dax_hip_mc_autotune(const float* d_field, int64_t nx, int64_t ny, int64_t nz, float iso,
+                        unsigned flags, void* d_workspace, size_t workspace_bytes,
+                        float* d_vertices, float* d_normals, int32_t* d_conn, uint32_t* d_edges,
+                        int64_t cap_points, int64_t cap_tris, int64_t* d_counts, int reps,
+                        int* h_best4, float* h_best_ms3, void* stream) {
+  VolGeom g;
+  int rc = make_geom(nx, ny, nz, nullptr, nullptr, iso, g);
+  if (rc) return rc;
+  Workspace L;
+  if ((rc = check_ws(g, workspace_bytes, L, d_workspace))) return rc;
+  if ((rc = check_outputs(g, d_field, flags, d_vertices, d_conn, d_edges, d_counts))) return rc;
+  if (reps < 1) reps = 1;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)d_workspace;
+  OutPtrs o{d_vertices, d_normals, d_conn, d_edges, nullptr, nullptr, nullptr};
+  Tuning best = tuning_for(nx, ny, nz);
+  EventPair ev;
+
+  // classify variants
+  std::vector<std::pair<int, int>> cls = {{8, 1}, {16, 1}, {32, 1}};
+  if (nx % 64 == 0) cls.insert(cls.end(), {{2, 4}, {4, 4}, {8, 4}});
+  float best_ms[3] = {1e30f, 1e30f, 1e30f};
+  for (auto uv : cls) {
+    Tuning t = best;
+    t.cls_unroll = uv.first;
+    t.cls_vec = uv.second;
+    if ((rc = launch_bits(g, d_field, ws, L, t, s))) return rc;  // warm
+    DAXHIP_CHECK(hipEventRecord(ev.a, s));
+    for (int r = 0; r < reps; ++r)
+      if ((rc = launch_bits(g, d_field, ws, L, t, s))) return rc;
+    DAXHIP_CHECK(hipEventRecord(ev.b, s));
+    const float ms = ev.ms() / reps;
+    if (ms < best_ms[0]) {
+      best_ms[0] = ms;
+      best.cls_unroll = t.cls_unroll;
+      best.cls_vec = t.cls_vec;
+    }
+  }
+  // scan variants (each timed launch follows a classify that resets its state)
+  for (int sub : {1, 2, 4, 8}) {
+    Tuning t = best;
+    t.scan_sub = sub;
+    float total = 0.0f;
+    for (int r = 0; r < reps; ++r) {
+      if ((rc = launch_bits(g, d_field, ws, L, best, s))) return rc;
+      DAXHIP_CHECK(hipEventRecord(ev.a, s));
+      if ((rc = launch_scan(g, ws, L, t, s))) return rc;
+      DAXHIP_CHECK(hipEventRecord(ev.b, s));
+      total += ev.ms();
+    }
+    const float ms = total / reps;
+    if (ms < best_ms[1]) {
+      best_ms[1] = ms;
+      best.scan_sub = sub;
+    }
+  }
+  // generate variants over one classify + scan
+  if ((rc = launch_bits(g, d_field, ws, L, best, s))) return rc;
+  if ((rc = launch_scan(g, ws, L, best, s))) return rc;
+  for (int tw : {256, 512, 1024}) {
+    Tuning t = best;
+    t.gen_tile = tw;
+    if ((rc = launch_generate(g, d_field, ws, L, flags, o, cap_points, cap_tris, d_counts, t, s)))
+      return rc;
+    DAXHIP_CHECK(hipEventRecord(ev.a, s));
+    for (int r = 0; r < reps; ++r)
+      if ((rc = launch_generate(g, d_field, ws, L, flags, o, cap_points, cap_tris, d_counts, t,
+                                s)))
+        return rc;
+    DAXHIP_CHECK(hipEventRecord(ev.b, s));
+    const float ms = ev.ms() / reps;
+    if (ms < best_ms[2]) {
+      best_ms[2] = ms;
+      best.gen_tile = tw;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[std::make_tuple(nx, ny, nz)] = best;
+  }
+  if (h_best4) {
+    h_best4[0] = best.cls_unroll;
+    h_best4[1] = best.cls_vec;
+    h_best4[2] = best.scan_sub;
+    h_best4[3] = best.gen_tile;
+  }
+  if (h_best_ms3)
+    for (int i = 0; i < 3; ++i) h_best_ms3[i] = best_ms[i];
   return DAX_HIP_OK;
 }
 

@@ -1,0 +1,648 @@
+// Device bodies of the marching-cubes kernels, shared by the single-volume
+// launches (mc_forward.hip) and the batched persistent launches
+// (mc_batched.hip).
+//
+//   classify   : field -> inside-bits                         (one pass over f)
+//   count/scan : per-word edge and triangle counts, exclusive scan with
+//                decoupled look-back over scan tiles
+//   generate   : per generate tile, maps of the tile's output vertices and
+//                triangles built in LDS, then one thread per vertex and per
+//                triangle with coalesced writes
+#pragma once
+
+#include "mc_common.h"
+
+namespace daxhip {
+namespace mc {
+
+// Case table in constant memory (one copy per translation unit).
+static __constant__ uint64_t c_case_table[256] = DAX_HIP_MC_CASE_TABLE_INIT;
+
+// ---------------------------------------------------------------------------
+// Incremental word positions (32-bit arithmetic, one division per start).
+struct WordIter {
+  int q, j, k;
+  __device__ __forceinline__ void init(const VolGeom& g, int lw) {
+    const int row = lw / g.wpr;
+    q = lw - row * g.wpr;
+    j = row % g.ny;
+    k = row / g.ny;
+  }
+  __device__ __forceinline__ void next(const VolGeom& g) {
+    if (++q == g.wpr) {
+      q = 0;
+      if (++j == g.ny) {
+        j = 0;
+        ++k;
+      }
+    }
+  }
+  __device__ __forceinline__ WordPos pos(const VolGeom& g) const {
+    WordPos p;
+    p.q = q;
+    p.j = j;
+    p.k = k;
+    p.has_y = j + 1 < g.ny;
+    p.has_z = k + 1 < g.nz;
+    const bool has_cells = g.nx > 1 && g.ny > 1 && g.nz > 1;
+    const int x0 = q * 64;
+    p.pmask = has_cells ? low_mask(g.nx - x0) : 0ull;
+    p.xmask = has_cells ? low_mask(g.nx - 1 - x0) : 0ull;
+    return p;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// classify, one float per lane: a wave builds UNROLL consecutive words.
+template <int UNROLL, bool ALIGNED>
+__device__ __forceinline__ void classify_words(const VolGeom& g, const float* __restrict__ f,
+                                               uint64_t* __restrict__ bits, int64_t w0,
+                                               int64_t w_end) {
+  const int lane = __lane_id();
+  float v[UNROLL];
+  bool ok[UNROLL];
+#pragma unroll
+  for (int u = 0; u < UNROLL; ++u) {
+    const int64_t w = w0 + u;
+    ok[u] = false;
+    v[u] = 0.0f;
+    if (w < w_end) {
+      if (ALIGNED) {
+        v[u] = __builtin_nontemporal_load(f + w * 64 + lane);
+        ok[u] = true;
+      } else {
+        const int64_t row = w / g.wpr;
+        const int x = (int)(w - row * g.wpr) * 64 + lane;
+        if (x < g.nx) {
+          v[u] = __builtin_nontemporal_load(f + row * g.nx + x);
+          ok[u] = true;
+        }
+      }
+    }
+  }
+  uint64_t mine = 0;
+#pragma unroll
+  for (int u = 0; u < UNROLL; ++u) {
+    const uint64_t b = __ballot(ok[u] && v[u] > g.iso);
+    if (lane == u) mine = b;
+  }
+  if (lane < UNROLL && w0 + lane < w_end) bits[w0 + lane] = mine;
+}
+
+// classify, float4 per lane (rows of a multiple of 64 points): a wave reads
+// UNROLL chunks of 256 consecutive points; lane l holds points 4l..4l+3 of a
+// chunk, and a DPP OR-scan over each 16-lane row assembles the chunk's four
+// 64-bit words.
+template <int UNROLL>
+__device__ __forceinline__ void classify_chunks_v4(const VolGeom& g, const float* __restrict__ f,
+                                                   uint64_t* __restrict__ bits, int64_t c0,
+                                                   int64_t nwords) {
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  const int lane = __lane_id();
+  const f32x4* f4 = reinterpret_cast<const f32x4*>(f);
+  f32x4 v[UNROLL];
+  bool ok[UNROLL];
+#pragma unroll
+  for (int u = 0; u < UNROLL; ++u) {
+    const int64_t c = c0 + u;
+    ok[u] = (4 * c + (lane >> 4)) < nwords;
+    v[u] = ok[u] ? __builtin_nontemporal_load(f4 + c * 64 + lane) : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  }
+#pragma unroll
+  for (int u = 0; u < UNROLL; ++u) {
+    const float iso = g.iso;
+    const uint32_t nib = (uint32_t)(v[u].x > iso) | ((uint32_t)(v[u].y > iso) << 1) |
+                         ((uint32_t)(v[u].z > iso) << 2) | ((uint32_t)(v[u].w > iso) << 3);
+    const int sh = 4 * (lane & 15);
+    uint32_t lo = sh < 32 ? nib << sh : 0u;
+    uint32_t hi = sh >= 32 ? nib << (sh - 32) : 0u;
+    lo |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x111, 0xf, 0xf, false);
+    hi |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x111, 0xf, 0xf, false);
+    lo |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x112, 0xf, 0xf, false);
+    hi |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x112, 0xf, 0xf, false);
+    lo |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x114, 0xf, 0xf, false);
+    hi |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x114, 0xf, 0xf, false);
+    lo |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x118, 0xf, 0xf, false);
+    hi |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x118, 0xf, 0xf, false);
+    if ((lane & 15) == 15 && ok[u]) bits[4 * (c0 + u) + (lane >> 4)] = ((uint64_t)hi << 32) | lo;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// count/scan: SUB sub-tiles of kTileWords words per scan tile.
+struct ScanShared {
+  uint64_t b0[kTileWords + kMaxWpr + 2];
+  uint64_t b1[kTileWords + kMaxWpr + 2];
+  uint8_t faces[256];
+  uint32_t wave_v[kScanThreads / 64], wave_t[kScanThreads / 64];
+  uint32_t prefix[2];
+};
+
+// Counts of one row word from the staged bits (i = index in the staging).
+__device__ __forceinline__ void word_counts(const VolGeom& g, const WordPos& p, const uint64_t* b0,
+                                            const uint64_t* b1, const uint8_t* faces, int i,
+                                            uint32_t& nv, uint32_t& nt) {
+  const bool nx1 = p.q + 1 < g.wpr;
+  const uint64_t A = b0[i], A1 = nx1 ? b0[i + 1] : 0ull;
+  const uint64_t B = b0[i + g.wpr], B1 = nx1 ? b0[i + g.wpr + 1] : 0ull;
+  const uint64_t C = b1[i], C1 = nx1 ? b1[i + 1] : 0ull;
+  const uint64_t D = b1[i + g.wpr], D1 = nx1 ? b1[i + g.wpr + 1] : 0ull;
+  uint64_t fx, fy, fz;
+  edge_flags(p, A, A1, B, C, fx, fy, fz);
+  nv = (uint32_t)(popc(fx) + popc(fy) + popc(fz));
+  CellPlanes cp;
+  cell_planes(p, A, A1, B, B1, C, C1, D, D1, cp);
+  uint64_t act = cp.active;
+  uint32_t t = 0;
+  while (act) {
+    const int bit = __builtin_ctzll(act);
+    act &= act - 1;
+    t += faces[cell_case(cp, bit)];
+  }
+  nt = t;
+}
+
+// Decoupled look-back by wave 0 over the tile states; returns the exclusive
+// prefix of `tile` (vertex, triangle) in sh.prefix.
+__device__ __forceinline__ void look_back(unsigned long long* __restrict__ status, int64_t tile,
+                                          uint32_t agg_v, uint32_t agg_t, Ctrl* __restrict__ ctrl,
+                                          ScanShared& sh) {
+  const int lane = __lane_id();
+  uint32_t ex_v = 0, ex_t = 0;
+  if (tile == 0) {
+    if (lane == 0)
+      __hip_atomic_store(&status[0], pack_status(agg_v, agg_t, kStatusPrefix), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    if (lane == 0)
+      __hip_atomic_store(&status[tile], pack_status(agg_v, agg_t, kStatusAgg), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    int64_t pos = tile - 1;
+    while (true) {
+      const int64_t idx = pos - lane;
+      unsigned long long s = pack_status(0, 0, kStatusPrefix);
+      if (idx >= 0) {
+        s = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while ((s >> 62) == 0ull) {
+          __builtin_amdgcn_s_sleep(1);
+          s = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      const uint64_t pm = __ballot((s >> 62) == kStatusPrefix);
+      uint32_t sv = (uint32_t)(s & 0x7fffffffull), st = (uint32_t)((s >> 31) & 0x7fffffffull);
+      if (pm) {
+        const int first = __builtin_ctzll(pm);
+        if (lane > first) {
+          sv = 0;
+          st = 0;
+        }
+      }
+      ex_v += wave_sum(sv);
+      ex_t += wave_sum(st);
+      if (pm) break;
+      pos -= 64;
+    }
+    if (lane == 0)
+      __hip_atomic_store(&status[tile], pack_status(ex_v + agg_v, ex_t + agg_t, kStatusPrefix),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (lane == 0) {
+    sh.prefix[0] = ex_v;
+    sh.prefix[1] = ex_t;
+    atomicAdd(&ctrl->total_v, (unsigned long long)agg_v);
+    atomicAdd(&ctrl->total_t, (unsigned long long)agg_t);
+  }
+}
+
+// One scan tile of a volume: words [w0, w0 + SUB*kTileWords) (volume-local),
+// status slot `tile` in the global tile order.
+template <int SUB>
+__device__ __forceinline__ void count_scan_tile(const VolGeom& g, const uint64_t* __restrict__ b,
+                                                uint32_t* __restrict__ base_v,
+                                                uint32_t* __restrict__ base_t,
+                                                unsigned long long* __restrict__ status,
+                                                Ctrl* __restrict__ ctrl, int64_t tile, int64_t w0,
+                                                ScanShared& sh) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int64_t plane = (int64_t)g.wpr * g.ny;
+  uint32_t packed[SUB][kWordsPerThread];
+  uint32_t ex_v[SUB], ex_t[SUB];
+  uint32_t run_v = 0, run_t = 0;
+#pragma unroll
+  for (int s = 0; s < SUB; ++s) {
+    const int64_t sb = w0 + (int64_t)s * kTileWords;
+    const int nw = (int)max((int64_t)0, min((int64_t)kTileWords, g.nwords - sb));
+    ex_v[s] = run_v;
+    ex_t[s] = run_t;
+#pragma unroll
+    for (int u = 0; u < kWordsPerThread; ++u) packed[s][u] = 0;
+    if (nw == 0) continue;  // uniform
+    const int seg = nw + g.wpr + 1;
+    for (int i = tid; i < seg; i += kScanThreads) {
+      const int64_t w = sb + i;
+      sh.b0[i] = (w < g.nwords) ? b[w] : 0ull;
+      sh.b1[i] = (w + plane < g.nwords) ? b[w + plane] : 0ull;
+    }
+    __syncthreads();
+    uint32_t sum_v = 0, sum_t = 0;
+    const int i0 = tid * kWordsPerThread;
+    if (i0 < nw) {
+      WordIter it;
+      it.init(g, (int)(sb + i0));
+#pragma unroll
+      for (int u = 0; u < kWordsPerThread; ++u) {
+        if (i0 + u < nw) {
+          uint32_t nv, nt;
+          word_counts(g, it.pos(g), sh.b0, sh.b1, sh.faces, i0 + u, nv, nt);
+          packed[s][u] = nv | (nt << 16);
+          sum_v += nv;
+          sum_t += nt;
+          it.next(g);
+        }
+      }
+    }
+    const uint32_t inc_v = wave_incl_scan(sum_v), inc_t = wave_incl_scan(sum_t);
+    if (lane == 63) {
+      sh.wave_v[wv] = inc_v;
+      sh.wave_t[wv] = inc_t;
+    }
+    __syncthreads();
+    uint32_t off_v = 0, off_t = 0, tot_v = 0, tot_t = 0;
+#pragma unroll
+    for (int q = 0; q < kScanThreads / 64; ++q) {
+      if (q < wv) {
+        off_v += sh.wave_v[q];
+        off_t += sh.wave_t[q];
+      }
+      tot_v += sh.wave_v[q];
+      tot_t += sh.wave_t[q];
+    }
+    ex_v[s] = run_v + off_v + inc_v - sum_v;
+    ex_t[s] = run_t + off_t + inc_t - sum_t;
+    run_v += tot_v;
+    run_t += tot_t;
+    __syncthreads();
+  }
+  if (wv == 0) look_back(status, tile, run_v, run_t, ctrl, sh);
+  __syncthreads();
+  const uint32_t pv = sh.prefix[0], pt = sh.prefix[1];
+#pragma unroll
+  for (int s = 0; s < SUB; ++s) {
+    const int64_t sb = w0 + (int64_t)s * kTileWords;
+    uint32_t rv = pv + ex_v[s], rt = pt + ex_t[s];
+#pragma unroll
+    for (int u = 0; u < kWordsPerThread; ++u) {
+      const int64_t w = sb + tid * kWordsPerThread + u;
+      if (w < g.nwords) {
+        base_v[g.word_off + w] = rv;
+        base_t[g.word_off + w] = rt;
+        rv += packed[s][u] & 0xffffu;
+        rt += packed[s][u] >> 16;
+        if (w == g.nwords - 1) {
+          base_v[g.word_off + w + 1] = rv;
+          base_t[g.word_off + w + 1] = rt;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// generate
+struct OutPtrs {
+  float* vertices;
+  float* normals;
+  int32_t* conn;
+  uint32_t* edges;
+  int32_t* interp_first;
+  int32_t* interp_second;
+  float* interp_weight;
+};
+
+// Central-difference gradient at (x,y,z): interior (f(p+e)-f(p-e))/(2h),
+// first plane (f(p+e)-f(p))/h, last plane (f(p)-f(p-e))/h, and 0 along an
+// axis with a single point.  Written branch-free so all loads issue at once.
+__device__ __forceinline__ void point_gradient(const VolGeom& g, const float* __restrict__ f, int x,
+                                               int y, int z, float out[3]) {
+  const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
+  const int64_t p = x + sy * y + sz * z;
+  const int c[3] = {x, y, z};
+  const int n[3] = {g.nx, g.ny, g.nz};
+  const int64_t st[3] = {1, sy, sz};
+  float lo_v[3], hi_v[3];
+  bool wide[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int64_t lo = c[d] > 0 ? p - st[d] : p;
+    const int64_t hi = c[d] < n[d] - 1 ? p + st[d] : p;
+    wide[d] = c[d] > 0 && c[d] < n[d] - 1;
+    lo_v[d] = f[lo];
+    hi_v[d] = f[hi];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float h = g.spacing[d];
+    out[d] = n[d] < 2 ? 0.0f : (hi_v[d] - lo_v[d]) / (wide[d] ? 2.0f * h : h);
+  }
+}
+
+// One interpolated point on the edge with lower endpoint (x,y,z) along axis.
+__device__ __forceinline__ void emit_point(const VolGeom& g, const float* __restrict__ f, int x,
+                                           int y, int z, int axis, bool normals, float xyz[3],
+                                           float nrm[3], float& w, int64_t& pa, int64_t& pb) {
+  const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
+  pa = x + sy * y + sz * z;
+  pb = pa + (axis == 0 ? 1 : (axis == 1 ? sy : sz));
+  const float fa = f[pa], fb = f[pb];
+  float ga[3], gb[3];
+  if (normals) {
+    point_gradient(g, f, x, y, z, ga);
+    point_gradient(g, f, x + (axis == 0), y + (axis == 1), z + (axis == 2), gb);
+  }
+  w = (g.iso - fa) / (fb - fa);
+  const int ca[3] = {x, y, z};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float a = g.origin[d] + g.spacing[d] * (float)ca[d];
+    const float b = g.origin[d] + g.spacing[d] * (float)(ca[d] + (d == axis ? 1 : 0));
+    xyz[d] = a + w * (b - a);
+  }
+  if (normals) {
+    float m[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) m[d] = ga[d] + w * (gb[d] - ga[d]);
+    const float len2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    if (len2 > 0.0f) {
+      const float len = sqrtf(len2);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) nrm[d] = m[d] / len;
+    } else {
+      nrm[0] = nrm[1] = nrm[2] = 0.0f;
+    }
+  }
+}
+
+// index i with s[i] <= v < s[i+1], s[0..n]
+__device__ __forceinline__ int find_word(const uint32_t* s, int n, uint32_t v) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (s[mid] <= v)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// LDS layout of a generate tile of TW words (wpr known at run time).
+template <int TW>
+struct GenLayout {
+  static constexpr int kVCap = 3 * TW;   // vertex map entries
+  static constexpr int kTCap = 6 * TW;   // triangle map entries
+  int segb, segv;
+  __host__ __device__ GenLayout(int wpr) : segb(TW + 2 * wpr + 2), segv(TW + wpr + 2) {}
+  __host__ __device__ size_t bytes() const {
+    return (size_t)(3 * segb + 256) * 8 + (size_t)(2 * segv + TW + 1 + kVCap + kTCap) * 4;
+  }
+};
+
+template <int TW>
+__device__ __forceinline__ void generate_tile(const VolGeom& g, const float* __restrict__ f,
+                                              const uint64_t* __restrict__ b,
+                                              const uint32_t* __restrict__ bv,
+                                              const uint32_t* __restrict__ bt, int64_t w_begin,
+                                              const OutPtrs& out, bool dedup, bool normals,
+                                              int64_t vol_base_v, int64_t vol_base_t,
+                                              uint64_t* s_dyn) {
+  const int tid = threadIdx.x;
+  const int nw = (int)min((int64_t)TW, g.nwords - w_begin);
+  const uint32_t tv0 = bv[w_begin], tv1 = bv[w_begin + nw];
+  const uint32_t tt0 = bt[w_begin], tt1 = bt[w_begin + nw];
+  if (tv0 == tv1 && tt0 == tt1) return;  // uniform: no output in this tile
+
+  const GenLayout<TW> L(g.wpr);
+  uint64_t* s_b0 = s_dyn;
+  uint64_t* s_b1 = s_b0 + L.segb;
+  uint64_t* s_b2 = s_b1 + L.segb;
+  uint64_t* s_case = s_b2 + L.segb;
+  uint32_t* s_v0 = reinterpret_cast<uint32_t*>(s_case + 256);
+  uint32_t* s_v1 = s_v0 + L.segv;
+  uint32_t* s_t = s_v1 + L.segv;
+  uint32_t* s_vmap = s_t + TW + 1;
+  uint32_t* s_tmap = s_vmap + GenLayout<TW>::kVCap;
+  uint64_t* s_bz[3] = {s_b0, s_b1, s_b2};
+  uint32_t* s_vz[2] = {s_v0, s_v1};
+
+  const int64_t plane = (int64_t)g.wpr * g.ny;
+  for (int i = tid; i < L.segb; i += blockDim.x) {
+    const int64_t w = w_begin + i;
+    s_b0[i] = (w < g.nwords) ? b[w] : 0ull;
+    s_b1[i] = (w + plane < g.nwords) ? b[w + plane] : 0ull;
+    s_b2[i] = (w + 2 * plane < g.nwords) ? b[w + 2 * plane] : 0ull;
+  }
+  for (int i = tid; i < L.segv; i += blockDim.x) {
+    const int64_t w = w_begin + i;
+    s_v0[i] = (w <= g.nwords) ? bv[w] : 0u;
+    s_v1[i] = (w + plane <= g.nwords) ? bv[w + plane] : 0u;
+  }
+  for (int i = tid; i <= nw; i += blockDim.x) s_t[i] = bt[w_begin + i];
+  for (int i = tid; i < 256; i += blockDim.x) s_case[i] = c_case_table[i];
+  __syncthreads();
+
+  // maps: vertex -> (word, bit, axis), triangle -> (word, cell bit, visit)
+  const int nV = (int)(tv1 - tv0), nT = (int)(tt1 - tt0);
+  constexpr int kWpt = (TW + kScanThreads - 1) / kScanThreads;
+  {
+    const int i0 = tid * kWpt;
+    if (i0 < nw) {
+      WordIter it;
+      it.init(g, (int)(w_begin + i0));
+#pragma unroll
+      for (int u = 0; u < kWpt; ++u) {
+        const int li = i0 + u;
+        if (li < nw) {
+          const WordPos p = it.pos(g);
+          const bool nx1 = p.q + 1 < g.wpr;
+          const uint64_t A = s_b0[li], A1 = nx1 ? s_b0[li + 1] : 0ull;
+          const uint64_t B = s_b0[li + g.wpr], B1 = nx1 ? s_b0[li + g.wpr + 1] : 0ull;
+          const uint64_t C = s_b1[li], C1 = nx1 ? s_b1[li + 1] : 0ull;
+          const uint64_t D = s_b1[li + g.wpr], D1 = nx1 ? s_b1[li + g.wpr + 1] : 0ull;
+          if (dedup) {
+            uint64_t fx, fy, fz;
+            edge_flags(p, A, A1, B, C, fx, fy, fz);
+            uint64_t any = fx | fy | fz;
+            int r = (int)(s_v0[li] - tv0);
+            while (any && r < GenLayout<TW>::kVCap) {
+              const int bit = __builtin_ctzll(any);
+              any &= any - 1;
+#pragma unroll
+              for (int a = 0; a < 3; ++a) {
+                const uint64_t fa = a == 0 ? fx : (a == 1 ? fy : fz);
+                if ((fa >> bit) & 1ull) {
+                  if (r < GenLayout<TW>::kVCap)
+                    s_vmap[r] = ((uint32_t)li << 8) | ((uint32_t)bit << 2) | (uint32_t)a;
+                  ++r;
+                }
+              }
+            }
+          }
+          CellPlanes cp;
+          cell_planes(p, A, A1, B, B1, C, C1, D, D1, cp);
+          uint64_t act = cp.active;
+          int r = (int)(s_t[li] - tt0);
+          while (act && r < GenLayout<TW>::kTCap) {
+            const int bit = __builtin_ctzll(act);
+            act &= act - 1;
+            const int nf = (int)(s_case[cell_case(cp, bit)] >> 60);
+            for (int s = 0; s < nf; ++s, ++r)
+              if (r < GenLayout<TW>::kTCap)
+                s_tmap[r] = ((uint32_t)li << 9) | ((uint32_t)bit << 3) | (uint32_t)s;
+          }
+          it.next(g);
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- vertices (dedup: one per active edge, in id order)
+  if (dedup) {
+    for (int vl = tid; vl < nV; vl += blockDim.x) {
+      int li, bit, axis;
+      if (vl < GenLayout<TW>::kVCap) {
+        const uint32_t e = s_vmap[vl];
+        li = (int)(e >> 8);
+        bit = (int)((e >> 2) & 63);
+        axis = (int)(e & 3);
+      } else {
+        li = find_word(s_v0, nw, tv0 + (uint32_t)vl);
+        WordIter it;
+        it.init(g, (int)(w_begin + li));
+        const WordPos p = it.pos(g);
+        const bool nx1 = p.q + 1 < g.wpr;
+        uint64_t fx, fy, fz;
+        edge_flags(p, s_b0[li], nx1 ? s_b0[li + 1] : 0ull, s_b0[li + g.wpr], s_b1[li], fx, fy, fz);
+        edge_select(fx, fy, fz, (int)(tv0 + (uint32_t)vl - s_v0[li]), bit, axis);
+      }
+      WordIter it;
+      it.init(g, (int)(w_begin + li));
+      const int x = it.q * 64 + bit;
+      float xyz[3], nrm[3], w;
+      int64_t pa, pb;
+      emit_point(g, f, x, it.j, it.k, axis, normals, xyz, nrm, w, pa, pb);
+      const int64_t gv = (int64_t)tv0 + vl;
+      out.vertices[3 * gv + 0] = xyz[0];
+      out.vertices[3 * gv + 1] = xyz[1];
+      out.vertices[3 * gv + 2] = xyz[2];
+      if (normals) {
+        out.normals[3 * gv + 0] = nrm[0];
+        out.normals[3 * gv + 1] = nrm[1];
+        out.normals[3 * gv + 2] = nrm[2];
+      }
+      if (out.edges) out.edges[gv] = ((uint32_t)axis << 30) | (uint32_t)pa;
+      if (out.interp_first) out.interp_first[gv] = (int32_t)pa;
+      if (out.interp_second) out.interp_second[gv] = (int32_t)pb;
+      if (out.interp_weight) out.interp_weight[gv] = w;
+    }
+  }
+
+  // ---- triangles
+  for (int tl = tid; tl < nT; tl += blockDim.x) {
+    int li, bit, visit, cs;
+    WordIter it;
+    CellPlanes cp;
+    if (tl < GenLayout<TW>::kTCap) {
+      const uint32_t e = s_tmap[tl];
+      li = (int)(e >> 9);
+      bit = (int)((e >> 3) & 63);
+      visit = (int)(e & 7);
+      it.init(g, (int)(w_begin + li));
+      const WordPos p = it.pos(g);
+      const bool nx1 = p.q + 1 < g.wpr;
+      cell_planes(p, s_b0[li], nx1 ? s_b0[li + 1] : 0ull, s_b0[li + g.wpr],
+                  nx1 ? s_b0[li + g.wpr + 1] : 0ull, s_b1[li], nx1 ? s_b1[li + 1] : 0ull,
+                  s_b1[li + g.wpr], nx1 ? s_b1[li + g.wpr + 1] : 0ull, cp);
+      cs = cell_case(cp, bit);
+    } else {
+      li = find_word(s_t, nw, tt0 + (uint32_t)tl);
+      it.init(g, (int)(w_begin + li));
+      const WordPos p = it.pos(g);
+      const bool nx1 = p.q + 1 < g.wpr;
+      cell_planes(p, s_b0[li], nx1 ? s_b0[li + 1] : 0ull, s_b0[li + g.wpr],
+                  nx1 ? s_b0[li + g.wpr + 1] : 0ull, s_b1[li], nx1 ? s_b1[li + 1] : 0ull,
+                  s_b1[li + g.wpr], nx1 ? s_b1[li + g.wpr + 1] : 0ull, cp);
+      int k = (int)(tt0 + (uint32_t)tl - s_t[li]);
+      uint64_t act = cp.active;
+      bool found = false;
+      bit = 0;
+      cs = 0;
+      while (act) {
+        bit = __builtin_ctzll(act);
+        act &= act - 1;
+        cs = cell_case(cp, bit);
+        const int nf = (int)(s_case[cs] >> 60);
+        if (k < nf) {
+          found = true;
+          break;
+        }
+        k -= nf;
+      }
+      if (!found) continue;  // unreachable when the scan and this kernel agree
+      visit = k;
+    }
+    const uint64_t packed = s_case[cs];
+    const int64_t t = (int64_t)tt0 + tl;
+    int32_t ids[3];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      const int e = (int)((packed >> (4 * (3 * visit + v))) & 0xF);
+      const uint8_t code = edge_code(e);
+      const int dx = code & 1, dy = (code >> 1) & 1, dz = (code >> 2) & 1, axis = code >> 3;
+      const int x = it.q * 64 + bit + dx;
+      if (dedup) {
+        const int qq = x >> 6, bb = x & 63;
+        const int lw2 = li + (qq - it.q) + dy * g.wpr;
+        WordPos p2;
+        p2.q = qq;
+        p2.j = it.j + dy;
+        p2.k = it.k + dz;
+        p2.has_y = p2.j + 1 < g.ny;
+        p2.has_z = p2.k + 1 < g.nz;
+        p2.pmask = low_mask(g.nx - qq * 64);
+        p2.xmask = low_mask(g.nx - 1 - qq * 64);
+        const bool nx2 = qq + 1 < g.wpr;
+        const uint64_t* sb = s_bz[dz];
+        const uint64_t A = sb[lw2], A1 = nx2 ? sb[lw2 + 1] : 0ull;
+        const uint64_t B = sb[lw2 + g.wpr], C = s_bz[dz + 1][lw2];
+        uint64_t fx, fy, fz;
+        edge_flags(p2, A, A1, B, C, fx, fy, fz);
+        ids[v] = (int32_t)((int64_t)s_vz[dz][lw2] + edge_rank(fx, fy, fz, bb, axis) - vol_base_v);
+      } else {
+        const int64_t corner = 3 * t + v;
+        float xyz[3], nrm[3], w;
+        int64_t pa, pb;
+        emit_point(g, f, x, it.j + dy, it.k + dz, axis, normals, xyz, nrm, w, pa, pb);
+        out.vertices[3 * corner + 0] = xyz[0];
+        out.vertices[3 * corner + 1] = xyz[1];
+        out.vertices[3 * corner + 2] = xyz[2];
+        if (normals) {
+          out.normals[3 * corner + 0] = nrm[0];
+          out.normals[3 * corner + 1] = nrm[1];
+          out.normals[3 * corner + 2] = nrm[2];
+        }
+        if (out.interp_first) out.interp_first[corner] = (int32_t)pa;
+        if (out.interp_second) out.interp_second[corner] = (int32_t)pb;
+        if (out.interp_weight) out.interp_weight[corner] = w;
+        ids[v] = (int32_t)(corner - 3 * vol_base_t);
+      }
+    }
+    out.conn[3 * t + 0] = ids[0];
+    out.conn[3 * t + 1] = ids[1];
+    out.conn[3 * t + 2] = ids[2];
+  }
+}
+
+}  // namespace mc
+}  // namespace daxhip

@@ -219,6 +219,36 @@ def backward(field: torch.Tensor, iso: float, contour: Contour, dl_dvertices: to
     return out
 
 
+def autotune(field: torch.Tensor, iso: float, buf: ForwardBuffers, reps: int = 10,
+             dedup: bool = True, normals: bool = True):
+    """Run the native autotuner for this shape; returns (params4, ms3)."""
+    import ctypes
+    nx, ny, nz = _check_field(field)
+    flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and buf.normals is not None) else 0)
+    best = (ctypes.c_int * 4)()
+    ms = (ctypes.c_float * 3)()
+    _lib.call("dax_hip_mc_autotune", field.data_ptr(), nx, ny, nz, float(iso), flags,
+              buf.workspace.data_ptr(), buf.workspace.numel(), buf.vertices.data_ptr(),
+              _ptr(buf.normals), buf.conn.data_ptr(), _ptr(buf.edges) if dedup else None,
+              buf.cap_points, buf.cap_tris, buf.counts.data_ptr(), int(reps), best, ms, _stream())
+    return list(best), list(ms)
+
+
+def set_tuning(shape, params4) -> None:
+    import ctypes
+    nx, ny, nz = shape
+    arr = (ctypes.c_int * 4)(*params4)
+    _lib.call("dax_hip_mc_set_tuning", nx, ny, nz, arr)
+
+
+def get_tuning(shape):
+    import ctypes
+    nx, ny, nz = shape
+    arr = (ctypes.c_int * 4)()
+    _lib.call("dax_hip_mc_get_tuning", nx, ny, nz, arr)
+    return list(arr)
+
+
 def interpolate_scalar(values: torch.Tensor, contour: Contour) -> torch.Tensor:
     """CompactPointField of a scalar point field onto the contour points."""
     if contour.interp_first is None:

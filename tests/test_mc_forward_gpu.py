@@ -165,3 +165,36 @@ def test_point_and_cell_gradient():
     cg = mc.cell_gradient(field, origin=org, spacing=sp).cpu().numpy()
     cr = orc.cell_gradient(f, origin=org, spacing=sp)
     assert np.array_equal(cg.view(np.uint32), cr.view(np.uint32))
+
+
+@pytest.mark.parametrize("shape_kind", ["aligned", "ragged"])
+def test_every_tuning_variant_is_bit_identical(shape_kind):
+    """Each classify / scan / generate variant gives identical outputs."""
+    if shape_kind == "aligned":
+        f = fields.sphere_distance(128)
+        iso = 40.0
+    else:
+        dims = (100, 70, 45)
+        p = fields.gaussian_mixture_params(9, dims, n_gauss=6, sigma=(6.0, 15.0))
+        f = fields.gaussian_mixture_numpy(dims, p)
+        iso = 0.5
+    nz, ny, nx = f.shape
+    field = torch.from_numpy(f).cuda()
+    default = mc.get_tuning((0, 0, 0))
+    variants = ([[u, 1, 4, 512] for u in (8, 16, 32)] + [[u, 4, 4, 512] for u in (2, 4, 8)] +
+                [[16, 1, s, 512] for s in (1, 2, 4, 8)] + [[16, 1, 4, t] for t in (256, 512, 1024)])
+    ref = None
+    try:
+        for v in variants:
+            print(f"[case] tuning {shape_kind} {v}", flush=True)
+            mc.set_tuning((nx, ny, nz), v)
+            out = mc.forward(field, iso, edges=True, interp=True)
+            got = [out.vertices.cpu().numpy(), out.normals.cpu().numpy(),
+                   out.connectivity.cpu().numpy(), out.edges.cpu().numpy()]
+            if ref is None:
+                ref = got
+            else:
+                for a, b in zip(got, ref):
+                    assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    finally:
+        mc.set_tuning((nx, ny, nz), default)
