@@ -36,7 +36,27 @@ def parse():
     ap.add_argument("--n", type=int, default=512, help="points per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--tuning", type=str, default=None,
+                    help="classify_unroll,classify_vec,scan_sub,gen_tile (skips the autotuner)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the device copy kernel on 512 MiB (counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe(steps):
+    """Plain copy of 512 MiB with the adapter's Copy kernel: the reference
+    point for FETCH_SIZE / WRITE_SIZE scaling."""
+    import torch
+
+    from daxtoolkit_amd import _lib
+    n = 512 << 20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    s = torch.cuda.current_stream().cuda_stream
+    for _ in range(steps):
+        _lib.call("dax_hip_copy", dst.data_ptr(), src.data_ptr(), n, s)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n, "steps": steps}), flush=True)
 
 
 def cpu_baseline(f_host, iso, cells, budget_s):
@@ -72,6 +92,9 @@ def main():
     from daxtoolkit_amd import fields, mc
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        copy_probe(args.steps)
+        return
     n = args.n
     dims = (n, n, n)
     iso = 0.5
@@ -85,7 +108,12 @@ def main():
     T, U = mc.read_counts(cnt)
     del ws
     buf = mc.ForwardBuffers(n, n, n, U, T, normals=True, edges=True)
-    tuned, tuned_ms = mc.autotune(field, iso, buf, reps=10)
+    if args.tuning:
+        tuned = [int(x) for x in args.tuning.split(",")]
+        mc.set_tuning((n, n, n), tuned)
+        tuned_ms = [0.0, 0.0, 0.0]
+    else:
+        tuned, tuned_ms = mc.autotune(field, iso, buf, reps=10)
 
     for _ in range(args.warmup):
         mc.forward_into(field, iso, buf)

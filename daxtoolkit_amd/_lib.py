@@ -58,6 +58,42 @@ _SIGNATURES = {
     "dax_hip_memcpy_d2d": (C.c_int, [vp, vp, sz, vp]),
     "dax_hip_memset": (C.c_int, [vp, C.c_int, sz, vp]),
     "dax_hip_synchronize": (C.c_int, [vp]),
+    "dax_hip_copy": (C.c_int, [vp, vp, sz, vp]),
+    "dax_hip_scan_inclusive_i32": (C.c_int, [vp, vp, i64, vp, vp]),
+    "dax_hip_scan_exclusive_i32": (C.c_int, [vp, vp, i64, vp, vp]),
+    "dax_hip_lower_bounds_i32": (C.c_int, [vp, i64, vp, i64, C.c_int, vp, vp, vp]),
+    "dax_hip_upper_bounds_i32": (C.c_int, [vp, i64, vp, i64, C.c_int, vp, vp, vp]),
+    "dax_hip_scan_inclusive_i64": (C.c_int, [vp, vp, i64, vp, vp]),
+    "dax_hip_scan_exclusive_i64": (C.c_int, [vp, vp, i64, vp, vp]),
+    "dax_hip_lower_bounds_i64": (C.c_int, [vp, i64, vp, i64, C.c_int, vp, vp, vp]),
+    "dax_hip_upper_bounds_i64": (C.c_int, [vp, i64, vp, i64, C.c_int, vp, vp, vp]),
+    "dax_hip_scan_inclusive_f32": (C.c_int, [vp, vp, i64, vp, vp]),
+    "dax_hip_scan_exclusive_f32": (C.c_int, [vp, vp, i64, vp, vp]),
+    "dax_hip_lower_bounds_f32": (C.c_int, [vp, i64, vp, i64, C.c_int, vp, vp, vp]),
+    "dax_hip_upper_bounds_f32": (C.c_int, [vp, i64, vp, i64, C.c_int, vp, vp, vp]),
+    "dax_hip_scan_inclusive_f64": (C.c_int, [vp, vp, i64, vp, vp]),
+    "dax_hip_scan_exclusive_f64": (C.c_int, [vp, vp, i64, vp, vp]),
+    "dax_hip_lower_bounds_f64": (C.c_int, [vp, i64, vp, i64, C.c_int, vp, vp, vp]),
+    "dax_hip_upper_bounds_f64": (C.c_int, [vp, i64, vp, i64, C.c_int, vp, vp, vp]),
+    "dax_hip_sort_i32": (C.c_int, [vp, i64, C.c_int, vp]),
+    "dax_hip_sort_by_key_i32": (C.c_int, [vp, vp, sz, i64, C.c_int, vp]),
+    "dax_hip_unique_i32": (C.c_int, [vp, i64, P_i64, vp]),
+    "dax_hip_sort_i64": (C.c_int, [vp, i64, C.c_int, vp]),
+    "dax_hip_sort_by_key_i64": (C.c_int, [vp, vp, sz, i64, C.c_int, vp]),
+    "dax_hip_unique_i64": (C.c_int, [vp, i64, P_i64, vp]),
+    "dax_hip_sort_u32": (C.c_int, [vp, i64, C.c_int, vp]),
+    "dax_hip_sort_by_key_u32": (C.c_int, [vp, vp, sz, i64, C.c_int, vp]),
+    "dax_hip_unique_u32": (C.c_int, [vp, i64, P_i64, vp]),
+    "dax_hip_sort_u64": (C.c_int, [vp, i64, C.c_int, vp]),
+    "dax_hip_sort_by_key_u64": (C.c_int, [vp, vp, sz, i64, C.c_int, vp]),
+    "dax_hip_unique_u64": (C.c_int, [vp, i64, P_i64, vp]),
+    "dax_hip_sort_f32": (C.c_int, [vp, i64, C.c_int, vp]),
+    "dax_hip_sort_by_key_f32": (C.c_int, [vp, vp, sz, i64, C.c_int, vp]),
+    "dax_hip_unique_f32": (C.c_int, [vp, i64, P_i64, vp]),
+    "dax_hip_sort_f64": (C.c_int, [vp, i64, C.c_int, vp]),
+    "dax_hip_sort_by_key_f64": (C.c_int, [vp, vp, sz, i64, C.c_int, vp]),
+    "dax_hip_unique_f64": (C.c_int, [vp, i64, P_i64, vp]),
+    "dax_hip_stream_compact": (C.c_int, [vp, sz, vp, C.c_int, i64, vp, vp, vp, P_i64, vp]),
     "dax_hip_timer_create": (C.c_int, [C.POINTER(vp)]),
     "dax_hip_timer_destroy": (C.c_int, [vp]),
     "dax_hip_timer_reset": (C.c_int, [vp, vp]),
@@ -79,6 +115,15 @@ _SIGNATURES = {
                                       i64, i64, vp, C.c_int, vp, vp, vp]),
     "dax_hip_mc_tuning_save": (C.c_int, [C.c_char_p]),
     "dax_hip_mc_tuning_load": (C.c_int, [C.c_char_p]),
+    "dax_hip_mc_batch_workspace_bytes": (sz, [i64, vp]),
+    "dax_hip_mc_batch_header_bytes": (sz, []),
+    "dax_hip_mc_batch_prepare": (C.c_int, [i64, vp, vp, vp, vp, vp, vp, sz, vp, vp]),
+    "dax_hip_mc_batch_forward": (C.c_int, [vp, vp, vp, sz, C.c_uint, vp, vp, vp, vp, i64, i64,
+                                           vp, vp, vp]),
+    "dax_hip_mc_batch_graph_create": (C.c_int, [vp, vp, vp, sz, C.c_uint, vp, vp, vp, vp, i64,
+                                                i64, vp, vp, vp, C.POINTER(vp)]),
+    "dax_hip_graph_launch": (C.c_int, [vp, vp]),
+    "dax_hip_graph_destroy": (C.c_int, [vp]),
     "dax_hip_mc_backward_scratch_bytes": (sz, [i64]),
     "dax_hip_mc_backward": (C.c_int, [vp, i64, i64, i64, vp, vp, f32, vp, sz, vp, vp, i64,
                                       vp, vp, vp, sz, vp, vp]),

@@ -1,0 +1,365 @@
+// Batched marching cubes: many independent volumes (uneven extents,
+// per-volume isovalue, origin and spacing) in one word space.  No reference
+// counterpart (Dax runs one grid per dispatcher call).
+//
+// Layout: the volumes' fields are concatenated (volume v at field_off[v]);
+// their row words are concatenated in volume order (word_off[v]).  Scan and
+// generate tiles never straddle a volume; tile tables map tile -> (volume,
+// first local word).  One global exclusive scan over all words gives every
+// word its vertex/triangle base; subtracting the base of the volume's first
+// word makes the numbering per volume -- a segmented scan without segment
+// flags.  Outputs are concatenated in volume order; connectivity is local to
+// each volume; per-volume {first triangle, first point} offsets are written
+// beside the counts.
+//
+// Each phase is one persistent launch (grid = a few blocks per CU, looping
+// over tiles); the four launches of a forward are graph-capturable (no host
+// read-back, scan state cleared by a kernel).
+#include <algorithm>
+#include <vector>
+
+#include "mc_kernels.h"
+
+namespace daxhip {
+namespace mc {
+
+struct TileRef {
+  int32_t vol;
+  int32_t w0;  // first row word, local to the volume
+};
+
+constexpr int kBatchScanNT = 256, kBatchScanWPT = 4;
+constexpr int kBatchScanTile = kBatchScanNT * kBatchScanWPT;
+constexpr int kBatchGenTile = 256;
+constexpr int kBatchClsUnroll = 16;
+
+struct BatchLayout {
+  size_t off_vols, off_scan_tiles, off_gen_tiles, off_bits, off_base_v, off_base_t, off_status,
+      off_ctrl, bytes;
+  int64_t nvols, nwords, nscan, ngen;
+};
+
+BatchLayout batch_layout(int64_t nvols, const int64_t* dims) {
+  BatchLayout L;
+  L.nvols = nvols;
+  L.nwords = 0;
+  L.nscan = 0;
+  L.ngen = 0;
+  for (int64_t v = 0; v < nvols; ++v) {
+    const int64_t w = ceil_div(dims[3 * v], 64) * dims[3 * v + 1] * dims[3 * v + 2];
+    L.nwords += w;
+    L.nscan += ceil_div(w, kBatchScanTile);
+    L.ngen += ceil_div(w, kBatchGenTile);
+  }
+  size_t off = 0;
+  L.off_vols = off;
+  off = align_up(off + (size_t)nvols * sizeof(VolGeom), 256);
+  L.off_scan_tiles = off;
+  off = align_up(off + (size_t)L.nscan * sizeof(TileRef), 256);
+  L.off_gen_tiles = off;
+  off = align_up(off + (size_t)L.ngen * sizeof(TileRef), 256);
+  L.off_bits = off;
+  off = align_up(off + (size_t)(L.nwords + 1) * 8, 256);
+  L.off_base_v = off;
+  off = align_up(off + (size_t)(L.nwords + 1) * 4, 256);
+  L.off_base_t = off;
+  off = align_up(off + (size_t)(L.nwords + 1) * 4, 256);
+  L.off_status = off;
+  off += (size_t)L.nscan * 8;
+  L.off_ctrl = off;
+  off = align_up(off + sizeof(Ctrl), 256);
+  L.bytes = off;
+  return L;
+}
+
+// Header at the start of the workspace: the layout, so launches need only
+// the workspace pointer after prepare.
+struct BatchHeader {
+  int64_t magic, nvols, nwords, nscan, ngen;
+  size_t off_vols, off_scan_tiles, off_gen_tiles, off_bits, off_base_v, off_base_t, off_status,
+      off_ctrl, bytes;
+};
+constexpr int64_t kBatchMagic = 0x4441584842415443LL;  // "DAXHBATC"
+constexpr size_t kBatchHeaderBytes = 256;
+
+__global__ __launch_bounds__(256) void k_batch_classify(const VolGeom* __restrict__ vols,
+                                                        const TileRef* __restrict__ tiles,
+                                                        int64_t ntiles,
+                                                        const float* __restrict__ fields,
+                                                        uint64_t* __restrict__ bits,
+                                                        unsigned long long* __restrict__ zero_buf,
+                                                        int64_t zero_words) {
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = gtid; i < zero_words; i += nthreads) zero_buf[i] = 0ull;
+  const int wv = threadIdx.x >> 6;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const TileRef tr = tiles[t];
+    const VolGeom g = vols[tr.vol];
+    const int64_t w_end = min((int64_t)tr.w0 + kBatchGenTile, g.nwords);
+    for (int64_t w0 = tr.w0 + (int64_t)wv * kBatchClsUnroll; w0 < w_end;
+         w0 += (int64_t)(blockDim.x >> 6) * kBatchClsUnroll)
+      classify_words<kBatchClsUnroll, false>(g, fields + g.field_off, bits + g.word_off, w0, w_end);
+  }
+}
+
+__global__ __launch_bounds__(kBatchScanNT) void k_batch_count_scan(
+    const VolGeom* __restrict__ vols, const TileRef* __restrict__ tiles, int64_t ntiles,
+    const uint64_t* __restrict__ bits, uint32_t* __restrict__ base_v,
+    uint32_t* __restrict__ base_t, unsigned long long* __restrict__ status,
+    Ctrl* __restrict__ ctrl) {
+  __shared__ ScanShared sh;
+  __shared__ int64_t s_tile;
+  for (int i = threadIdx.x; i < 256; i += kBatchScanNT) sh.faces[i] = (uint8_t)(c_case_table[i] >> 60);
+  while (true) {
+    __syncthreads();  // s_tile and sh reused across iterations
+    if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(&ctrl->tile_counter, 1u);
+    __syncthreads();
+    const int64_t t = s_tile;
+    if (t >= ntiles) break;
+    const TileRef tr = tiles[t];
+    const VolGeom g = vols[tr.vol];
+    count_scan_tile<kBatchScanNT, kBatchScanWPT>(g, bits + g.word_off, base_v, base_t, status, ctrl,
+                                                t, tr.w0, sh);
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_batch_generate(
+    const VolGeom* __restrict__ vols, const TileRef* __restrict__ tiles, int64_t ntiles,
+    const float* __restrict__ fields, const uint64_t* __restrict__ bits,
+    const uint32_t* __restrict__ base_v, const uint32_t* __restrict__ base_t,
+    const Ctrl* __restrict__ ctrl, OutPtrs out, unsigned flags, int64_t cap_points,
+    int64_t cap_tris, int64_t* __restrict__ counts) {
+  extern __shared__ uint64_t s_dyn[];
+  const bool dedup = (flags & DAX_HIP_MC_DEDUP) != 0;
+  const bool normals = (flags & DAX_HIP_MC_NORMALS) != 0 && out.normals != nullptr;
+  const uint64_t total_v = ctrl->total_v, total_t = ctrl->total_t;
+  const uint64_t n_points = dedup ? total_v : 3ull * total_t;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && counts) {
+    counts[0] = (int64_t)total_t;
+    counts[1] = (int64_t)n_points;
+  }
+  if (total_v > 0x7fffffffull || 3ull * total_t > 0x7fffffffull) return;
+  if ((int64_t)total_t > cap_tris || (int64_t)n_points > cap_points) return;
+  for (int64_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
+    const TileRef tr = tiles[xcd_tile(b, ntiles)];
+    const VolGeom g = vols[tr.vol];
+    const int64_t vbv = base_v[g.word_off], vbt = base_t[g.word_off];
+    generate_tile<kBatchGenTile>(g, fields + g.field_off, bits + g.word_off, base_v + g.word_off,
+                                 base_t + g.word_off, tr.w0, out, dedup, normals, vbv, vbt, s_dyn);
+    __syncthreads();  // LDS reused by the next tile
+  }
+}
+
+// Per-volume {first triangle, first point} in the concatenated outputs.
+__global__ void k_batch_offsets(const VolGeom* __restrict__ vols, int64_t nvols,
+                                const uint32_t* __restrict__ base_v,
+                                const uint32_t* __restrict__ base_t, unsigned flags,
+                                int64_t* __restrict__ vol_offsets) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nvols) return;
+  const int64_t w = vols[v].word_off;
+  const bool dedup = (flags & DAX_HIP_MC_DEDUP) != 0;
+  vol_offsets[2 * v + 0] = base_t[w];
+  vol_offsets[2 * v + 1] = dedup ? (int64_t)base_v[w] : 3 * (int64_t)base_t[w];
+}
+
+}  // namespace mc
+}  // namespace daxhip
+
+using namespace daxhip;
+using namespace daxhip::mc;
+
+namespace {
+
+int persistent_blocks(int per_cu) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  return cus * per_cu;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t dax_hip_mc_batch_workspace_bytes(int64_t n_volumes, const int64_t* h_dims3) {
+  if (n_volumes <= 0 || !h_dims3) return 0;
+  return kBatchHeaderBytes + batch_layout(n_volumes, h_dims3).bytes;
+}
+
+// Writes the volume and tile tables into the workspace (synchronous upload).
+int dax_hip_mc_batch_prepare(int64_t n_volumes, const int64_t* h_dims3,
+                             const int64_t* h_field_offsets, const float* h_isos,
+                             const float* h_origins3, const float* h_spacings3, void* d_workspace,
+                             size_t workspace_bytes, void* h_header_out, void* stream) {
+  if (n_volumes <= 0 || n_volumes > 0x7fffffffLL || !h_dims3 || !h_field_offsets || !h_isos) {
+    set_error("dax_hip_mc_batch_prepare: bad volume description");
+    return DAX_HIP_BAD_VALUE;
+  }
+  const BatchLayout L = batch_layout(n_volumes, h_dims3);
+  if (!d_workspace || workspace_bytes < kBatchHeaderBytes + L.bytes) {
+    set_error("dax_hip_mc_batch_prepare: workspace of %zu bytes, %zu needed", workspace_bytes,
+              kBatchHeaderBytes + L.bytes);
+    return DAX_HIP_BAD_VALUE;
+  }
+  std::vector<VolGeom> vols((size_t)n_volumes);
+  std::vector<TileRef> scan_tiles, gen_tiles;
+  scan_tiles.reserve((size_t)L.nscan);
+  gen_tiles.reserve((size_t)L.ngen);
+  int64_t word_off = 0;
+  for (int64_t v = 0; v < n_volumes; ++v) {
+    const int64_t nx = h_dims3[3 * v], ny = h_dims3[3 * v + 1], nz = h_dims3[3 * v + 2];
+    if (nx < 1 || ny < 1 || nz < 1 || nx * ny * nz > (1LL << 30) || ceil_div(nx, 64) > kMaxWpr) {
+      set_error("dax_hip_mc_batch_prepare: volume %lld has unsupported dims %lld x %lld x %lld",
+                (long long)v, (long long)nx, (long long)ny, (long long)nz);
+      return DAX_HIP_BAD_VALUE;
+    }
+    VolGeom& g = vols[(size_t)v];
+    g.nx = (int32_t)nx;
+    g.ny = (int32_t)ny;
+    g.nz = (int32_t)nz;
+    g.wpr = (int32_t)ceil_div(nx, 64);
+    g.iso = h_isos[v];
+    for (int d = 0; d < 3; ++d) {
+      g.origin[d] = h_origins3 ? h_origins3[3 * v + d] : 0.0f;
+      g.spacing[d] = h_spacings3 ? h_spacings3[3 * v + d] : 1.0f;
+    }
+    g.field_off = h_field_offsets[v];
+    g.word_off = word_off;
+    g.nwords = (int64_t)g.wpr * ny * nz;
+    for (int64_t w = 0; w < g.nwords; w += kBatchScanTile) scan_tiles.push_back({(int32_t)v, (int32_t)w});
+    for (int64_t w = 0; w < g.nwords; w += kBatchGenTile) gen_tiles.push_back({(int32_t)v, (int32_t)w});
+    word_off += g.nwords;
+  }
+  BatchHeader h;
+  h.magic = kBatchMagic;
+  h.nvols = n_volumes;
+  h.nwords = L.nwords;
+  h.nscan = L.nscan;
+  h.ngen = L.ngen;
+  h.off_vols = L.off_vols;
+  h.off_scan_tiles = L.off_scan_tiles;
+  h.off_gen_tiles = L.off_gen_tiles;
+  h.off_bits = L.off_bits;
+  h.off_base_v = L.off_base_v;
+  h.off_base_t = L.off_base_t;
+  h.off_status = L.off_status;
+  h.off_ctrl = L.off_ctrl;
+  h.bytes = L.bytes;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)d_workspace;
+  char* body = ws + kBatchHeaderBytes;
+  DAXHIP_CHECK(hipMemcpyAsync(ws, &h, sizeof(h), hipMemcpyHostToDevice, s));
+  DAXHIP_CHECK(hipMemcpyAsync(body + L.off_vols, vols.data(), vols.size() * sizeof(VolGeom),
+                              hipMemcpyHostToDevice, s));
+  DAXHIP_CHECK(hipMemcpyAsync(body + L.off_scan_tiles, scan_tiles.data(),
+                              scan_tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice, s));
+  DAXHIP_CHECK(hipMemcpyAsync(body + L.off_gen_tiles, gen_tiles.data(),
+                              gen_tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice, s));
+  DAXHIP_CHECK(hipStreamSynchronize(s));
+  if (h_header_out) memcpy(h_header_out, &h, sizeof(h));
+  return DAX_HIP_OK;
+}
+
+size_t dax_hip_mc_batch_header_bytes(void) { return sizeof(BatchHeader); }
+
+// The batch forward: four persistent launches, graph-capturable.  h_header is
+// the header returned by prepare.  d_counts {T, U} totals; d_vol_offsets
+// [n][2] {first triangle, first point} of each volume.
+int dax_hip_mc_batch_forward(const float* d_fields, const void* h_header, void* d_workspace,
+                             size_t workspace_bytes, unsigned flags, float* d_vertices,
+                             float* d_normals, int32_t* d_conn, uint32_t* d_edges,
+                             int64_t cap_points, int64_t cap_tris, int64_t* d_counts,
+                             int64_t* d_vol_offsets, void* stream) {
+  const BatchHeader* h = (const BatchHeader*)h_header;
+  if (!h || h->magic != kBatchMagic || !d_workspace ||
+      workspace_bytes < kBatchHeaderBytes + h->bytes) {
+    set_error("dax_hip_mc_batch_forward: header/workspace not prepared");
+    return DAX_HIP_BAD_VALUE;
+  }
+  if (!d_fields || !d_vertices || !d_conn || !d_counts) {
+    set_error("dax_hip_mc_batch_forward: fields, vertices, connectivity and counts are required");
+    return DAX_HIP_BAD_VALUE;
+  }
+  if (d_edges && !(flags & DAX_HIP_MC_DEDUP)) {
+    set_error("dax_hip_mc_batch_forward: the compact edge list needs dedup");
+    return DAX_HIP_BAD_VALUE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  char* body = (char*)d_workspace + kBatchHeaderBytes;
+  const VolGeom* vols = (const VolGeom*)(body + h->off_vols);
+  const TileRef* scan_tiles = (const TileRef*)(body + h->off_scan_tiles);
+  const TileRef* gen_tiles = (const TileRef*)(body + h->off_gen_tiles);
+  uint64_t* bits = (uint64_t*)(body + h->off_bits);
+  uint32_t* bv = (uint32_t*)(body + h->off_base_v);
+  uint32_t* bt = (uint32_t*)(body + h->off_base_t);
+  unsigned long long* status = (unsigned long long*)(body + h->off_status);
+  Ctrl* ctrl = (Ctrl*)(body + h->off_ctrl);
+  const int64_t zero_words = h->nscan + (int64_t)(sizeof(Ctrl) / 8);
+  const int grid = persistent_blocks(4);
+  k_batch_classify<<<dim3(grid), dim3(256), 0, s>>>(vols, gen_tiles, h->ngen, d_fields, bits, status,
+                                                     zero_words);
+  DAXHIP_LAUNCH_CHECK("k_batch_classify");
+  k_batch_count_scan<<<dim3((unsigned)std::min<int64_t>(grid, h->nscan)), dim3(kBatchScanNT), 0, s>>>(
+      vols, scan_tiles, h->nscan, bits, bv, bt, status, ctrl);
+  DAXHIP_LAUNCH_CHECK("k_batch_count_scan");
+  OutPtrs o{d_vertices, d_normals, d_conn, d_edges, nullptr, nullptr, nullptr};
+  const size_t lds = GenLayout<kBatchGenTile>(kMaxWpr).bytes();
+  k_batch_generate<<<dim3((unsigned)std::min<int64_t>(grid, h->ngen)), dim3(kScanThreads), lds, s>>>(
+      vols, gen_tiles, h->ngen, d_fields, bits, bv, bt, ctrl, o, flags, cap_points, cap_tris,
+      d_counts);
+  DAXHIP_LAUNCH_CHECK("k_batch_generate");
+  if (d_vol_offsets) {
+    k_batch_offsets<<<dim3((unsigned)ceil_div(h->nvols, 256)), dim3(256), 0, s>>>(
+        vols, h->nvols, bv, bt, flags, d_vol_offsets);
+    DAXHIP_LAUNCH_CHECK("k_batch_offsets");
+  }
+  return DAX_HIP_OK;
+}
+
+// ---- graphs: capture a launch sequence once, replay it -------------------
+// Captures dax_hip_mc_batch_forward(args) on `stream` into an executable
+// graph (h_graph_exec receives a hipGraphExec_t).
+int dax_hip_mc_batch_graph_create(const float* d_fields, const void* h_header, void* d_workspace,
+                                  size_t workspace_bytes, unsigned flags, float* d_vertices,
+                                  float* d_normals, int32_t* d_conn, uint32_t* d_edges,
+                                  int64_t cap_points, int64_t cap_tris, int64_t* d_counts,
+                                  int64_t* d_vol_offsets, void* stream, void** h_graph_exec) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!s) {
+    set_error("dax_hip_mc_batch_graph_create: capture needs a non-null stream");
+    return DAX_HIP_BAD_VALUE;
+  }
+  DAXHIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = dax_hip_mc_batch_forward(d_fields, h_header, d_workspace, workspace_bytes, flags,
+                                    d_vertices, d_normals, d_conn, d_edges, cap_points, cap_tris,
+                                    d_counts, d_vol_offsets, stream);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return check_hip(e, "hipGraphInstantiate");
+  *h_graph_exec = exec;
+  return DAX_HIP_OK;
+}
+
+int dax_hip_graph_launch(void* graph_exec, void* stream) {
+  DAXHIP_CHECK(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+  return DAX_HIP_OK;
+}
+
+int dax_hip_graph_destroy(void* graph_exec) {
+  if (graph_exec) DAXHIP_CHECK(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+  return DAX_HIP_OK;
+}
+
+}  // extern "C"

@@ -51,20 +51,21 @@ __global__ __launch_bounds__(256) void k_classify_v4(VolGeom g, const float* __r
   classify_chunks_v4<UNROLL>(g, field + g.field_off, bits + g.word_off, c0, g.nwords);
 }
 
-template <int SUB>
-__global__ __launch_bounds__(kScanThreads) void k_count_scan(
-    VolGeom g, const uint64_t* __restrict__ bits, uint32_t* __restrict__ base_v,
-    uint32_t* __restrict__ base_t, unsigned long long* __restrict__ status, Ctrl* __restrict__ ctrl,
-    int64_t ntiles) {
+template <int NT, int WPT>
+__global__ __launch_bounds__(NT) void k_count_scan(VolGeom g, const uint64_t* __restrict__ bits,
+                                                   uint32_t* __restrict__ base_v,
+                                                   uint32_t* __restrict__ base_t,
+                                                   unsigned long long* __restrict__ status,
+                                                   Ctrl* __restrict__ ctrl, int64_t ntiles) {
   __shared__ ScanShared sh;
   __shared__ int64_t s_tile;
   if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(&ctrl->tile_counter, 1u);
-  sh.faces[threadIdx.x] = (uint8_t)(c_case_table[threadIdx.x] >> 60);
+  for (int i = threadIdx.x; i < 256; i += NT) sh.faces[i] = (uint8_t)(c_case_table[i] >> 60);
   __syncthreads();
   const int64_t tile = s_tile;
   if (tile >= ntiles) return;  // scan state not reset by a classify launch
-  count_scan_tile<SUB>(g, bits + g.word_off, base_v, base_t, status, ctrl, tile,
-                       tile * (int64_t)SUB * kTileWords, sh);
+  count_scan_tile<NT, WPT>(g, bits + g.word_off, base_v, base_t, status, ctrl, tile,
+                           tile * (int64_t)NT * WPT, sh);
 }
 
 template <int TW>
@@ -84,9 +85,10 @@ __global__ __launch_bounds__(kScanThreads) void k_generate(
   }
   if (total_v > 0x7fffffffull || 3ull * total_t > 0x7fffffffull) return;
   if ((int64_t)total_t > cap_tris || (int64_t)n_points > cap_points) return;
+  const int64_t ntiles = (g.nwords + TW - 1) / TW;
   generate_tile<TW>(g, field + g.field_off, bits + g.word_off, base_v + g.word_off,
-                    base_t + g.word_off, (int64_t)blockIdx.x * TW, out, dedup, normals, 0, 0,
-                    s_dyn);
+                    base_t + g.word_off, xcd_tile(blockIdx.x, ntiles) * TW, out, dedup, normals,
+                    0, 0, s_dyn);
 }
 
 // ---------------------------------------------------------------------------
@@ -297,19 +299,19 @@ int launch_bits(const VolGeom& g, const float* field, char* ws, const Workspace&
 }
 
 int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu, hipStream_t s) {
-  const int sub = tu.scan_sub;
+  const int sub = tu.scan_sub;  // scan tile = sub * 1024 row words
   const int64_t ntiles = ceil_div(g.nwords, (int64_t)sub * kTileWords);
-  const dim3 grid((unsigned)ntiles), block(kScanThreads);
+  const dim3 grid((unsigned)std::max<int64_t>(ntiles, 1));
   const uint64_t* bits = (const uint64_t*)(ws + L.off_bits);
   uint32_t* bv = (uint32_t*)(ws + L.off_base_v);
   uint32_t* bt = (uint32_t*)(ws + L.off_base_t);
   unsigned long long* st = (unsigned long long*)(ws + L.off_status);
   Ctrl* ctrl = (Ctrl*)(ws + L.off_ctrl);
   switch (sub) {
-    case 1: k_count_scan<1><<<grid, block, 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
-    case 2: k_count_scan<2><<<grid, block, 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
-    case 8: k_count_scan<8><<<grid, block, 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
-    default: k_count_scan<4><<<grid, block, 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
+    case 1: k_count_scan<256, 4><<<grid, dim3(256), 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
+    case 2: k_count_scan<512, 4><<<grid, dim3(512), 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
+    case 8: k_count_scan<1024, 8><<<grid, dim3(1024), 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
+    default: k_count_scan<1024, 4><<<grid, dim3(1024), 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
   }
   DAXHIP_LAUNCH_CHECK("k_count_scan");
   return DAX_HIP_OK;

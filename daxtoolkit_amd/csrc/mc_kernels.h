@@ -129,24 +129,21 @@ __device__ __forceinline__ void classify_chunks_v4(const VolGeom& g, const float
 }
 
 // ---------------------------------------------------------------------------
-// count/scan: SUB sub-tiles of kTileWords words per scan tile.
+// count/scan.  A scan tile is NT threads x WPT consecutive row words.  Each
+// thread loads its words and their +x/+y/+z neighbour words straight from
+// global memory (L2), counts, and the block scans the per-thread sums; wave 0
+// then runs the decoupled look-back for the tile's global prefix.
 struct ScanShared {
-  uint64_t b0[kTileWords + kMaxWpr + 2];
-  uint64_t b1[kTileWords + kMaxWpr + 2];
   uint8_t faces[256];
-  uint32_t wave_v[kScanThreads / 64], wave_t[kScanThreads / 64];
+  uint32_t wave_v[16], wave_t[16];
   uint32_t prefix[2];
 };
 
-// Counts of one row word from the staged bits (i = index in the staging).
-__device__ __forceinline__ void word_counts(const VolGeom& g, const WordPos& p, const uint64_t* b0,
-                                            const uint64_t* b1, const uint8_t* faces, int i,
-                                            uint32_t& nv, uint32_t& nt) {
-  const bool nx1 = p.q + 1 < g.wpr;
-  const uint64_t A = b0[i], A1 = nx1 ? b0[i + 1] : 0ull;
-  const uint64_t B = b0[i + g.wpr], B1 = nx1 ? b0[i + g.wpr + 1] : 0ull;
-  const uint64_t C = b1[i], C1 = nx1 ? b1[i + 1] : 0ull;
-  const uint64_t D = b1[i + g.wpr], D1 = nx1 ? b1[i + g.wpr + 1] : 0ull;
+__device__ __forceinline__ void word_counts_regs(const VolGeom& g, const WordPos& p, uint64_t A,
+                                                 uint64_t A1, uint64_t B, uint64_t B1, uint64_t C,
+                                                 uint64_t C1, uint64_t D, uint64_t D1,
+                                                 const uint8_t* faces, uint32_t& nv,
+                                                 uint32_t& nt) {
   uint64_t fx, fy, fz;
   edge_flags(p, A, A1, B, C, fx, fy, fz);
   nv = (uint32_t)(popc(fx) + popc(fy) + popc(fz));
@@ -166,7 +163,7 @@ __device__ __forceinline__ void word_counts(const VolGeom& g, const WordPos& p, 
 // prefix of `tile` (vertex, triangle) in sh.prefix.
 __device__ __forceinline__ void look_back(unsigned long long* __restrict__ status, int64_t tile,
                                           uint32_t agg_v, uint32_t agg_t, Ctrl* __restrict__ ctrl,
-                                          ScanShared& sh) {
+                                          uint32_t* prefix) {
   const int lane = __lane_id();
   uint32_t ex_v = 0, ex_t = 0;
   if (tile == 0) {
@@ -207,105 +204,99 @@ __device__ __forceinline__ void look_back(unsigned long long* __restrict__ statu
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (lane == 0) {
-    sh.prefix[0] = ex_v;
-    sh.prefix[1] = ex_t;
+    prefix[0] = ex_v;
+    prefix[1] = ex_t;
     atomicAdd(&ctrl->total_v, (unsigned long long)agg_v);
     atomicAdd(&ctrl->total_t, (unsigned long long)agg_t);
   }
 }
 
-// One scan tile of a volume: words [w0, w0 + SUB*kTileWords) (volume-local),
-// status slot `tile` in the global tile order.
-template <int SUB>
+// One scan tile of a volume: words [w0, w0 + NT*WPT) (volume-local), status
+// slot `tile` in the global tile order.
+template <int NT, int WPT>
 __device__ __forceinline__ void count_scan_tile(const VolGeom& g, const uint64_t* __restrict__ b,
                                                 uint32_t* __restrict__ base_v,
                                                 uint32_t* __restrict__ base_t,
                                                 unsigned long long* __restrict__ status,
-                                                Ctrl* __restrict__ ctrl, int64_t tile, int64_t w0,
-                                                ScanShared& sh) {
+                                                Ctrl* __restrict__ ctrl, int64_t tile,
+                                                int64_t tile_w0, ScanShared& sh) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int64_t plane = (int64_t)g.wpr * g.ny;
-  uint32_t packed[SUB][kWordsPerThread];
-  uint32_t ex_v[SUB], ex_t[SUB];
-  uint32_t run_v = 0, run_t = 0;
+  const int64_t w0 = tile_w0 + (int64_t)tid * WPT;
+  uint32_t packed[WPT];
+  uint32_t sum_v = 0, sum_t = 0;
 #pragma unroll
-  for (int s = 0; s < SUB; ++s) {
-    const int64_t sb = w0 + (int64_t)s * kTileWords;
-    const int nw = (int)max((int64_t)0, min((int64_t)kTileWords, g.nwords - sb));
-    ex_v[s] = run_v;
-    ex_t[s] = run_t;
+  for (int u = 0; u < WPT; ++u) packed[u] = 0;
+  if (w0 < g.nwords) {
+    uint64_t A[WPT + 1], B[WPT + 1], C[WPT + 1], D[WPT + 1];
 #pragma unroll
-    for (int u = 0; u < kWordsPerThread; ++u) packed[s][u] = 0;
-    if (nw == 0) continue;  // uniform
-    const int seg = nw + g.wpr + 1;
-    for (int i = tid; i < seg; i += kScanThreads) {
-      const int64_t w = sb + i;
-      sh.b0[i] = (w < g.nwords) ? b[w] : 0ull;
-      sh.b1[i] = (w + plane < g.nwords) ? b[w + plane] : 0ull;
+    for (int u = 0; u <= WPT; ++u) {
+      const int64_t w = w0 + u;
+      A[u] = w < g.nwords ? b[w] : 0ull;
+      B[u] = w + g.wpr < g.nwords ? b[w + g.wpr] : 0ull;
+      C[u] = w + plane < g.nwords ? b[w + plane] : 0ull;
+      D[u] = w + plane + g.wpr < g.nwords ? b[w + plane + g.wpr] : 0ull;
     }
-    __syncthreads();
-    uint32_t sum_v = 0, sum_t = 0;
-    const int i0 = tid * kWordsPerThread;
-    if (i0 < nw) {
-      WordIter it;
-      it.init(g, (int)(sb + i0));
+    WordIter it;
+    it.init(g, (int)w0);
 #pragma unroll
-      for (int u = 0; u < kWordsPerThread; ++u) {
-        if (i0 + u < nw) {
-          uint32_t nv, nt;
-          word_counts(g, it.pos(g), sh.b0, sh.b1, sh.faces, i0 + u, nv, nt);
-          packed[s][u] = nv | (nt << 16);
-          sum_v += nv;
-          sum_t += nt;
-          it.next(g);
-        }
+    for (int u = 0; u < WPT; ++u) {
+      if (w0 + u < g.nwords) {
+        const WordPos p = it.pos(g);
+        const bool nx1 = p.q + 1 < g.wpr;
+        uint32_t nv, nt;
+        word_counts_regs(g, p, A[u], nx1 ? A[u + 1] : 0ull, B[u], nx1 ? B[u + 1] : 0ull, C[u],
+                         nx1 ? C[u + 1] : 0ull, D[u], nx1 ? D[u + 1] : 0ull, sh.faces, nv, nt);
+        packed[u] = nv | (nt << 16);
+        sum_v += nv;
+        sum_t += nt;
+        it.next(g);
       }
     }
-    const uint32_t inc_v = wave_incl_scan(sum_v), inc_t = wave_incl_scan(sum_t);
-    if (lane == 63) {
-      sh.wave_v[wv] = inc_v;
-      sh.wave_t[wv] = inc_t;
-    }
-    __syncthreads();
-    uint32_t off_v = 0, off_t = 0, tot_v = 0, tot_t = 0;
-#pragma unroll
-    for (int q = 0; q < kScanThreads / 64; ++q) {
-      if (q < wv) {
-        off_v += sh.wave_v[q];
-        off_t += sh.wave_t[q];
-      }
-      tot_v += sh.wave_v[q];
-      tot_t += sh.wave_t[q];
-    }
-    ex_v[s] = run_v + off_v + inc_v - sum_v;
-    ex_t[s] = run_t + off_t + inc_t - sum_t;
-    run_v += tot_v;
-    run_t += tot_t;
-    __syncthreads();
   }
-  if (wv == 0) look_back(status, tile, run_v, run_t, ctrl, sh);
+  const uint32_t inc_v = wave_incl_scan(sum_v), inc_t = wave_incl_scan(sum_t);
+  if (lane == 63) {
+    sh.wave_v[wv] = inc_v;
+    sh.wave_t[wv] = inc_t;
+  }
   __syncthreads();
-  const uint32_t pv = sh.prefix[0], pt = sh.prefix[1];
+  uint32_t off_v = 0, off_t = 0, agg_v = 0, agg_t = 0;
 #pragma unroll
-  for (int s = 0; s < SUB; ++s) {
-    const int64_t sb = w0 + (int64_t)s * kTileWords;
-    uint32_t rv = pv + ex_v[s], rt = pt + ex_t[s];
+  for (int q = 0; q < NT / 64; ++q) {
+    if (q < wv) {
+      off_v += sh.wave_v[q];
+      off_t += sh.wave_t[q];
+    }
+    agg_v += sh.wave_v[q];
+    agg_t += sh.wave_t[q];
+  }
+  if (wv == 0) look_back(status, tile, agg_v, agg_t, ctrl, sh.prefix);
+  __syncthreads();
+  uint32_t rv = sh.prefix[0] + off_v + inc_v - sum_v;
+  uint32_t rt = sh.prefix[1] + off_t + inc_t - sum_t;
 #pragma unroll
-    for (int u = 0; u < kWordsPerThread; ++u) {
-      const int64_t w = sb + tid * kWordsPerThread + u;
-      if (w < g.nwords) {
-        base_v[g.word_off + w] = rv;
-        base_t[g.word_off + w] = rt;
-        rv += packed[s][u] & 0xffffu;
-        rt += packed[s][u] >> 16;
-        if (w == g.nwords - 1) {
-          base_v[g.word_off + w + 1] = rv;
-          base_t[g.word_off + w + 1] = rt;
-        }
+  for (int u = 0; u < WPT; ++u) {
+    const int64_t w = w0 + u;
+    if (w < g.nwords) {
+      base_v[g.word_off + w] = rv;
+      base_t[g.word_off + w] = rt;
+      rv += packed[u] & 0xffffu;
+      rt += packed[u] >> 16;
+      if (w == g.nwords - 1) {
+        base_v[g.word_off + w + 1] = rv;
+        base_t[g.word_off + w + 1] = rt;
       }
     }
   }
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so neighbouring tiles share its L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n) {
+  const int64_t per = n / 8, rem = n % 8;
+  const int64_t x = b % 8, i = b / 8;
+  return x * per + (x < rem ? x : rem) + i;
 }
 
 // ---------------------------------------------------------------------------

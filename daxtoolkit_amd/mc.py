@@ -276,3 +276,97 @@ def cell_gradient(field: torch.Tensor, origin=(0, 0, 0), spacing=(1, 1, 1)) -> t
     _lib.call("dax_hip_cell_gradient", field.data_ptr(), nx, ny, nz, _f3(origin), _f3(spacing),
               out.data_ptr(), _stream())
     return out
+
+
+class Batch:
+    """Many independent volumes run as one batched forward (optionally as a
+    captured hipGraph).  `fields` is one flat float32 device tensor holding
+    the volumes back to back (volume v at ``field_offsets[v]``)."""
+
+    def __init__(self, dims, isos, origins=None, spacings=None, device="cuda"):
+        import ctypes
+
+        import numpy as np
+        self.n = len(dims)
+        self.dims = np.ascontiguousarray(np.asarray(dims, dtype=np.int64).reshape(self.n, 3))
+        sizes = self.dims.prod(axis=1)
+        self.field_offsets = np.ascontiguousarray(np.concatenate([[0], np.cumsum(sizes)[:-1]]),
+                                                  dtype=np.int64)
+        self.total_points = int(sizes.sum())
+        self.isos = np.ascontiguousarray(np.asarray(isos, dtype=np.float32).reshape(self.n))
+        self.origins = (None if origins is None else
+                        np.ascontiguousarray(np.asarray(origins, np.float32).reshape(self.n, 3)))
+        self.spacings = (None if spacings is None else
+                         np.ascontiguousarray(np.asarray(spacings, np.float32).reshape(self.n, 3)))
+        lib = _lib.load()
+        p = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+        nbytes = int(lib.dax_hip_mc_batch_workspace_bytes(self.n, p(self.dims)))
+        self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        self.header = ctypes.create_string_buffer(int(lib.dax_hip_mc_batch_header_bytes()))
+        _lib.call("dax_hip_mc_batch_prepare", self.n, p(self.dims), p(self.field_offsets),
+                  p(self.isos), p(self.origins), p(self.spacings), self.workspace.data_ptr(),
+                  self.workspace.numel(), self.header, _stream())
+        self.device = device
+        self.counts = torch.zeros(2, dtype=torch.int64, device=device)
+        self.vol_offsets = torch.zeros((self.n, 2), dtype=torch.int64, device=device)
+
+    def _call(self, name, fields, bufs, flags, extra=()):
+        return _lib.call(name, fields.data_ptr(), self.header, self.workspace.data_ptr(),
+                         self.workspace.numel(), flags, bufs["vertices"].data_ptr(),
+                         _ptr(bufs.get("normals")), bufs["conn"].data_ptr(), _ptr(bufs.get("edges")),
+                         bufs["cap_points"], bufs["cap_tris"], self.counts.data_ptr(),
+                         self.vol_offsets.data_ptr(), _stream(), *extra)
+
+    def alloc(self, cap_points, cap_tris, normals=True, edges=True):
+        dev = self.device
+        return {"vertices": torch.empty((max(cap_points, 1), 3), dtype=torch.float32, device=dev),
+                "normals": (torch.empty((max(cap_points, 1), 3), dtype=torch.float32, device=dev)
+                            if normals else None),
+                "conn": torch.empty((max(cap_tris, 1), 3), dtype=torch.int32, device=dev),
+                "edges": torch.empty(max(cap_points, 1), dtype=torch.int32, device=dev) if edges else None,
+                "cap_points": int(cap_points), "cap_tris": int(cap_tris)}
+
+    def sizes(self, fields, dedup=True):
+        """Run once without room for output to learn {T, U}."""
+        probe = self.alloc(0, 0, normals=False, edges=False)
+        self._call("dax_hip_mc_batch_forward", fields, probe, MC_DEDUP if dedup else 0)
+        return read_counts(self.counts)
+
+    def forward_into(self, fields, bufs, dedup=True, normals=True):
+        flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and bufs.get("normals") is not None) else 0)
+        if not dedup and bufs.get("edges") is not None:
+            bufs = dict(bufs, edges=None)
+        self._call("dax_hip_mc_batch_forward", fields, bufs, flags)
+
+    def capture(self, fields, bufs, dedup=True, normals=True, stream=None):
+        """Capture the batch forward into a graph on `stream` (a non-default
+        torch stream); returns the graph handle."""
+        import ctypes
+        flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and bufs.get("normals") is not None) else 0)
+        h = ctypes.c_void_p()
+        with torch.cuda.stream(stream):
+            self._call("dax_hip_mc_batch_graph_create", fields, bufs, flags, extra=(ctypes.byref(h),))
+        return h
+
+    @staticmethod
+    def launch(graph, stream=None):
+        s = (stream or torch.cuda.current_stream()).cuda_stream
+        _lib.call("dax_hip_graph_launch", graph, s)
+
+    @staticmethod
+    def destroy(graph):
+        _lib.call("dax_hip_graph_destroy", graph)
+
+    def split(self, bufs, dedup=True):
+        """Per-volume views (vertices, normals, connectivity) of the outputs."""
+        T, U = read_counts(self.counts, bufs["cap_points"], bufs["cap_tris"])
+        offs = self.vol_offsets.cpu().numpy()
+        out = []
+        for v in range(self.n):
+            t0, p0 = int(offs[v, 0]), int(offs[v, 1])
+            t1 = int(offs[v + 1, 0]) if v + 1 < self.n else T
+            p1 = int(offs[v + 1, 1]) if v + 1 < self.n else U
+            out.append((bufs["vertices"][p0:p1],
+                        None if bufs.get("normals") is None else bufs["normals"][p0:p1],
+                        bufs["conn"][t0:t1]))
+        return out

@@ -63,6 +63,73 @@ int dax_hip_memset(void* d_dst, int value, size_t bytes, void* stream);
  * (dax/cuda/cont/internal/DeviceAdapterAlgorithmCuda.h:40-43). */
 int dax_hip_synchronize(void* stream);
 
+/* ---- DeviceAdapterAlgorithm<DeviceAdapterTagHip> ----------------------- */
+
+/* Copy (DeviceAdapterAlgorithm.h:56-60; Thrust CopyPortal
+ * DeviceAdapterAlgorithmThrust.h:76-83): a device copy kernel. */
+int dax_hip_copy(void* d_dst, const void* d_src, size_t bytes, void* stream);
+
+/* ScanInclusive / ScanExclusive (DeviceAdapterAlgorithm.h:114-130; Serial
+ * DeviceAdapterAlgorithmSerial.h:46-99): d_out may equal d_in; *h_total
+ * (may be NULL) receives the full sum, as both reference scans return it.
+ * Synchronous.  Deterministic summation order (reduce-then-scan). */
+int dax_hip_scan_inclusive_i32(const int32_t* d_in, int32_t* d_out, int64_t n, int32_t* h_total, void* stream);
+int dax_hip_scan_exclusive_i32(const int32_t* d_in, int32_t* d_out, int64_t n, int32_t* h_total, void* stream);
+int dax_hip_scan_inclusive_i64(const int64_t* d_in, int64_t* d_out, int64_t n, int64_t* h_total, void* stream);
+int dax_hip_scan_exclusive_i64(const int64_t* d_in, int64_t* d_out, int64_t n, int64_t* h_total, void* stream);
+int dax_hip_scan_inclusive_f32(const float* d_in, float* d_out, int64_t n, float* h_total, void* stream);
+int dax_hip_scan_exclusive_f32(const float* d_in, float* d_out, int64_t n, float* h_total, void* stream);
+int dax_hip_scan_inclusive_f64(const double* d_in, double* d_out, int64_t n, double* h_total, void* stream);
+int dax_hip_scan_exclusive_f64(const double* d_in, double* d_out, int64_t n, double* h_total, void* stream);
+
+/* LowerBounds / UpperBounds (DeviceAdapterAlgorithm.h:62-78, 297-317;
+ * General :258-308, :1044-1092): for each value, the first index of the
+ * sorted array whose element is not less (lower) / is greater (upper) than
+ * it; `descending` != 0 for arrays sorted with SortGreater.  Output as int32
+ * (d_out32) and/or int64 (d_out64). */
+int dax_hip_lower_bounds_i32(const int32_t* d_sorted, int64_t n, const int32_t* d_values, int64_t m, int descending, int32_t* d_out32, int64_t* d_out64, void* stream);
+int dax_hip_upper_bounds_i32(const int32_t* d_sorted, int64_t n, const int32_t* d_values, int64_t m, int descending, int32_t* d_out32, int64_t* d_out64, void* stream);
+int dax_hip_lower_bounds_i64(const int64_t* d_sorted, int64_t n, const int64_t* d_values, int64_t m, int descending, int32_t* d_out32, int64_t* d_out64, void* stream);
+int dax_hip_upper_bounds_i64(const int64_t* d_sorted, int64_t n, const int64_t* d_values, int64_t m, int descending, int32_t* d_out32, int64_t* d_out64, void* stream);
+int dax_hip_lower_bounds_f32(const float* d_sorted, int64_t n, const float* d_values, int64_t m, int descending, int32_t* d_out32, int64_t* d_out64, void* stream);
+int dax_hip_upper_bounds_f32(const float* d_sorted, int64_t n, const float* d_values, int64_t m, int descending, int32_t* d_out32, int64_t* d_out64, void* stream);
+int dax_hip_lower_bounds_f64(const double* d_sorted, int64_t n, const double* d_values, int64_t m, int descending, int32_t* d_out32, int64_t* d_out64, void* stream);
+int dax_hip_upper_bounds_f64(const double* d_sorted, int64_t n, const double* d_values, int64_t m, int descending, int32_t* d_out32, int64_t* d_out64, void* stream);
+
+/* Sort / SortByKey / Unique (DeviceAdapterAlgorithm.h:183-250, 271-281):
+ * stable LSD radix sort, ascending (SortLess) or descending (SortGreater);
+ * SortByKey permutes value records of value_bytes bytes with the keys;
+ * Unique removes consecutive duplicates in place and returns the new size.
+ * Synchronous. */
+int dax_hip_sort_i32(int32_t* d_values, int64_t n, int descending, void* stream);
+int dax_hip_sort_i64(int64_t* d_values, int64_t n, int descending, void* stream);
+int dax_hip_sort_u32(uint32_t* d_values, int64_t n, int descending, void* stream);
+int dax_hip_sort_u64(uint64_t* d_values, int64_t n, int descending, void* stream);
+int dax_hip_sort_f32(float* d_values, int64_t n, int descending, void* stream);
+int dax_hip_sort_f64(double* d_values, int64_t n, int descending, void* stream);
+int dax_hip_sort_by_key_i32(int32_t* d_keys, void* d_values, size_t value_bytes, int64_t n, int descending, void* stream);
+int dax_hip_sort_by_key_i64(int64_t* d_keys, void* d_values, size_t value_bytes, int64_t n, int descending, void* stream);
+int dax_hip_sort_by_key_u32(uint32_t* d_keys, void* d_values, size_t value_bytes, int64_t n, int descending, void* stream);
+int dax_hip_sort_by_key_u64(uint64_t* d_keys, void* d_values, size_t value_bytes, int64_t n, int descending, void* stream);
+int dax_hip_sort_by_key_f32(float* d_keys, void* d_values, size_t value_bytes, int64_t n, int descending, void* stream);
+int dax_hip_sort_by_key_f64(double* d_keys, void* d_values, size_t value_bytes, int64_t n, int descending, void* stream);
+int dax_hip_unique_i32(int32_t* d_values, int64_t n, int64_t* h_new_n, void* stream);
+int dax_hip_unique_i64(int64_t* d_values, int64_t n, int64_t* h_new_n, void* stream);
+int dax_hip_unique_u32(uint32_t* d_values, int64_t n, int64_t* h_new_n, void* stream);
+int dax_hip_unique_u64(uint64_t* d_values, int64_t n, int64_t* h_new_n, void* stream);
+int dax_hip_unique_f32(float* d_values, int64_t n, int64_t* h_new_n, void* stream);
+int dax_hip_unique_f64(double* d_values, int64_t n, int64_t* h_new_n, void* stream);
+
+/* StreamCompact (DeviceAdapterAlgorithm.h:252-269; General :769-830):
+ * keeps the entries whose stencil element is non-zero.  With d_in == NULL
+ * the kept indices are written (d_indices32 and/or d_indices64, the
+ * reference's StreamCompact(stencil, output)); otherwise the kept records of
+ * value_bytes bytes go to d_out.  stencil_type: 0 int32, 1 int64, 2 float,
+ * 3 uint8, 4 double.  *h_count = number kept.  Synchronous. */
+int dax_hip_stream_compact(const void* d_in, size_t value_bytes, const void* d_stencil,
+                           int stencil_type, int64_t n, void* d_out, int32_t* d_indices32,
+                           int64_t* d_indices64, int64_t* h_count, void* stream);
+
 /* ---- timer: DeviceAdapterTimerImplementation<Hip> ---------------------
  * Replaces the CUDA event timer (DeviceAdapterAlgorithmCuda.h:49-88). */
 int dax_hip_timer_create(void** h_timer);
@@ -192,6 +259,43 @@ int dax_hip_mc_tuning_load(const char* path);
 int dax_hip_mc_interpolate_scalar(const float* d_in, const int32_t* d_first,
                                   const int32_t* d_second, const float* d_weight,
                                   int64_t n, float* d_out, void* stream);
+
+/* ---- batched marching cubes (no reference counterpart) ------------------
+ * Many independent volumes: volume v has point dims h_dims3[3v..3v+2],
+ * its field at d_fields + h_field_offsets[v] (float32, flat i + nx*(j+ny*k)),
+ * isovalue h_isos[v], origin/spacing h_origins3/h_spacings3 (NULL = 0 / 1).
+ * Outputs are concatenated in volume order; connectivity is local to each
+ * volume; d_vol_offsets [n][2] receives {first triangle, first point} per
+ * volume and d_counts {T, U} the totals.  Per volume the results equal a
+ * single-volume dax_hip_mc_forward.
+ *   dax_hip_mc_batch_workspace_bytes  size for this set of volume dims
+ *   dax_hip_mc_batch_prepare          uploads the volume and tile tables
+ *                                     (synchronous); copies a header of
+ *                                     dax_hip_mc_batch_header_bytes() bytes
+ *                                     to h_header_out for the calls below
+ *   dax_hip_mc_batch_forward          four persistent launches (classify,
+ *                                     count/scan, generate, offsets)
+ *   dax_hip_mc_batch_graph_create     the same, captured into a hipGraphExec
+ *                                     on a non-null stream
+ *   dax_hip_graph_launch / _destroy   replay / free a captured graph */
+size_t dax_hip_mc_batch_workspace_bytes(int64_t n_volumes, const int64_t* h_dims3);
+size_t dax_hip_mc_batch_header_bytes(void);
+int dax_hip_mc_batch_prepare(int64_t n_volumes, const int64_t* h_dims3,
+                             const int64_t* h_field_offsets, const float* h_isos,
+                             const float* h_origins3, const float* h_spacings3, void* d_workspace,
+                             size_t workspace_bytes, void* h_header_out, void* stream);
+int dax_hip_mc_batch_forward(const float* d_fields, const void* h_header, void* d_workspace,
+                             size_t workspace_bytes, unsigned flags, float* d_vertices,
+                             float* d_normals, int32_t* d_conn, uint32_t* d_edges,
+                             int64_t cap_points, int64_t cap_tris, int64_t* d_counts,
+                             int64_t* d_vol_offsets, void* stream);
+int dax_hip_mc_batch_graph_create(const float* d_fields, const void* h_header, void* d_workspace,
+                                  size_t workspace_bytes, unsigned flags, float* d_vertices,
+                                  float* d_normals, int32_t* d_conn, uint32_t* d_edges,
+                                  int64_t cap_points, int64_t cap_tris, int64_t* d_counts,
+                                  int64_t* d_vol_offsets, void* stream, void** h_graph_exec);
+int dax_hip_graph_launch(void* graph_exec, void* stream);
+int dax_hip_graph_destroy(void* graph_exec);
 
 /* ---- adjoint (no reference counterpart) ---------------------------------
  * dL/dfield of L(vertices, normals) given dL/dvertices [U][3] and
