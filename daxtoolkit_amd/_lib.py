@@ -127,6 +127,7 @@ _SIGNATURES = {
     "dax_hip_mc_backward_scratch_bytes": (sz, [i64]),
     "dax_hip_mc_backward": (C.c_int, [vp, i64, i64, i64, vp, vp, f32, vp, sz, vp, vp, i64,
                                       vp, vp, vp, sz, vp, vp]),
+    "dax_hip_mc_interpolate_vec3": (C.c_int, [vp, vp, vp, vp, i64, vp, vp]),
     "dax_hip_point_gradient": (C.c_int, [vp, i64, i64, i64, vp, vp, vp]),
     "dax_hip_cell_gradient": (C.c_int, [vp, i64, i64, i64, vp, vp, vp, vp]),
 }

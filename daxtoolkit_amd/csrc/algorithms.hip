@@ -366,8 +366,10 @@ __global__ __launch_bounds__(kNT) void k_radix_scatter(const U* __restrict__ key
       uint32_t off = s_base[d];
       for (int q = 0; q < wv; ++q) off += s_wcnt[q][d];
       const uint32_t pos = off + rank;
-      keys_out[pos] = key;
-      if (WITH_IDX) idx_out[pos] = idx_in[k];
+      if (pos < (uint64_t)n) {  // always true when the histograms are consistent
+        keys_out[pos] = key;
+        if (WITH_IDX) idx_out[pos] = idx_in[k];
+      }
     }
     __syncthreads();
     // advance the digit bases by this round's counts
@@ -504,6 +506,7 @@ __global__ void k_scatter_flagged(const uint8_t* __restrict__ in, int64_t rec,
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || !flags[i]) return;
   const int64_t p = pos[i];
+  if (p < 0 || p >= n) return;  // positions come from an exclusive scan of 0/1 flags
   if (in)
     for (int64_t b = 0; b < rec; ++b) out[p * rec + b] = in[i * rec + b];
   if (idx32) idx32[p] = (int32_t)i;

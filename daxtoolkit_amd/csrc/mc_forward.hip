@@ -162,6 +162,18 @@ __global__ void k_interpolate_scalar(const float* __restrict__ in, const int32_t
   out[u] = a + weight[u] * (b - a);
 }
 
+__global__ void k_interpolate_vec3(const float* __restrict__ in, const int32_t* __restrict__ first,
+                                   const int32_t* __restrict__ second,
+                                   const float* __restrict__ weight, int64_t n,
+                                   float* __restrict__ out) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const int64_t a = 3 * (int64_t)first[u], b = 3 * (int64_t)second[u];
+  const float w = weight[u];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) out[3 * u + d] = in[a + d] + w * (in[b + d] - in[a + d]);
+}
+
 __global__ void k_point_gradient(VolGeom g, const float* __restrict__ f, float* __restrict__ grad) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t P = (int64_t)g.nx * g.ny * g.nz;
@@ -508,6 +520,15 @@ int dax_hip_mc_interpolate_scalar(const float* d_in, const int32_t* d_first,
   k_interpolate_scalar<<<dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream>>>(
       d_in, d_first, d_second, d_weight, n, d_out);
   DAXHIP_LAUNCH_CHECK("k_interpolate_scalar");
+  return DAX_HIP_OK;
+}
+
+int dax_hip_mc_interpolate_vec3(const float* d_in, const int32_t* d_first, const int32_t* d_second,
+                                const float* d_weight, int64_t n, float* d_out, void* stream) {
+  if (n <= 0) return DAX_HIP_OK;
+  k_interpolate_vec3<<<dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream>>>(
+      d_in, d_first, d_second, d_weight, n, d_out);
+  DAXHIP_LAUNCH_CHECK("k_interpolate_vec3");
   return DAX_HIP_OK;
 }
 

@@ -1,0 +1,77 @@
+// DispatcherMapCell (dax/cont/DispatcherMapCell.h:27-64) for the worklets of
+// the marching-cubes path on a UniformGrid with the HIP adapter:
+//   MarchingCubesCount(iso): Invoke(grid, field, count)   count[C] = NumFaces[case]
+//   CellGradient:            Invoke(grid, coords, field, gradient)
+#ifndef DAX_HIP_SLICE_DISPATCHERMAPCELL_H
+#define DAX_HIP_SLICE_DISPATCHERMAPCELL_H
+
+#include <dax/cont/ArrayHandle.h>
+#include <dax/cont/UniformGrid.h>
+#include <dax/cont/internal/HipWorkspace.h>
+#include <dax/worklet/CellGradient.h>
+#include <dax/worklet/MarchingCubes.h>
+
+namespace dax {
+namespace cont {
+
+template <class WorkletType, class DeviceAdapterTag = DAX_DEFAULT_DEVICE_ADAPTER_TAG>
+class DispatcherMapCell;
+
+template <>
+class DispatcherMapCell<dax::worklet::MarchingCubesCount, dax::hip::cont::DeviceAdapterTagHip> {
+ public:
+  typedef dax::hip::cont::DeviceAdapterTagHip Tag;
+  explicit DispatcherMapCell(const dax::worklet::MarchingCubesCount& w =
+                                 dax::worklet::MarchingCubesCount())
+      : Worklet(w) {}
+
+  template <class CField, class CCount>
+  void Invoke(const UniformGrid<Tag>& grid, const ArrayHandle<dax::Scalar, CField, Tag>& field,
+              ArrayHandle<dax::Id, CCount, Tag>& count) const {
+    static_assert(sizeof(dax::Id) == 4, "the HIP marching-cubes path writes 32-bit ids");
+    const dax::Id3 d = grid.GetPointDimensions();
+    if (field.GetNumberOfValues() != grid.GetNumberOfPoints())
+      throw ErrorControlBadValue("field size does not match the grid's point count");
+    const float* f = field.PrepareForInput().GetRawPointer();
+    internal::HipWorkspace ws(dax_hip_mc_workspace_bytes(d[0], d[1], d[2]));
+    internal::HipCheck(
+        dax_hip_mc_classify(f, d[0], d[1], d[2], Worklet.GetIsoValue(), ws.Data, ws.Bytes, nullptr, nullptr));
+    int32_t* c = count.PrepareForOutput(grid.GetNumberOfCells()).GetRawPointer();
+    internal::HipCheck(dax_hip_mc_cell_outputs(d[0], d[1], d[2], ws.Data, ws.Bytes, nullptr, c,
+                                               nullptr, nullptr));
+    internal::HipCheck(dax_hip_synchronize(nullptr));
+  }
+
+ private:
+  dax::worklet::MarchingCubesCount Worklet;
+};
+
+template <>
+class DispatcherMapCell<dax::worklet::CellGradient, dax::hip::cont::DeviceAdapterTagHip> {
+ public:
+  typedef dax::hip::cont::DeviceAdapterTagHip Tag;
+  explicit DispatcherMapCell(const dax::worklet::CellGradient& = dax::worklet::CellGradient()) {}
+
+  // coords: the grid's point coordinates (implicit on a uniform grid; only
+  // the grid's origin and spacing are used).
+  template <class CCoords, class CField, class COut>
+  void Invoke(const UniformGrid<Tag>& grid, const ArrayHandle<dax::Vector3, CCoords, Tag>& coords,
+              const ArrayHandle<dax::Scalar, CField, Tag>& field,
+              ArrayHandle<dax::Vector3, COut, Tag>& gradient) const {
+    (void)coords;
+    const dax::Id3 d = grid.GetPointDimensions();
+    if (field.GetNumberOfValues() != grid.GetNumberOfPoints())
+      throw ErrorControlBadValue("field size does not match the grid's point count");
+    const float* f = field.PrepareForInput().GetRawPointer();
+    const dax::Vector3 o = grid.GetOrigin(), s = grid.GetSpacing();
+    const float origin[3] = {o[0], o[1], o[2]}, spacing[3] = {s[0], s[1], s[2]};
+    float* g = (float*)gradient.PrepareForOutput(grid.GetNumberOfCells()).GetRawPointer();
+    internal::HipCheck(dax_hip_cell_gradient(f, d[0], d[1], d[2], origin, spacing, g, nullptr));
+    internal::HipCheck(dax_hip_synchronize(nullptr));
+  }
+};
+
+}  // namespace cont
+}  // namespace dax
+
+#endif

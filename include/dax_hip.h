@@ -315,6 +315,11 @@ int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz
                         const float* d_dl_dnormals, void* d_scratch, size_t scratch_bytes,
                         float* d_dl_dfield, void* stream);
 
+/* The same for a Vector3 point field ([n][3] f32, e.g. coordinates). */
+int dax_hip_mc_interpolate_vec3(const float* d_in, const int32_t* d_first,
+                                const int32_t* d_second, const float* d_weight, int64_t n,
+                                float* d_out, void* stream);
+
 /* Central-difference point gradient of the whole field, [P][3] f32 (interior
  * (f(p+e)-f(p-e))/(2h), one-sided on the first/last plane).  No reference
  * counterpart; the normals of dax_hip_mc_forward use the same stencil. */
