@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tuning", type=str, default=None,
-                    help="classify_unroll,classify_vec,scan_sub,gen_tile (skips the autotuner)")
+                    help="classify_unroll,classify_vec,scan_sub,gen_tile[,gen_xcd] (skips the autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the device copy kernel on 512 MiB (counter calibration)")
     return ap.parse_args()
@@ -171,7 +171,8 @@ def main():
                       "generate": round(k_ms[2], 5)},
         "wall_s": round(wall, 4),
         "tuning": {"classify_unroll": tuned[0], "classify_vec": tuned[1], "scan_sub": tuned[2],
-                   "gen_tile": tuned[3], "autotune_ms": [round(x, 5) for x in tuned_ms]},
+                   "gen_tile": tuned[3], "gen_xcd": (tuned + [1])[4],
+                   "autotune_ms": [round(x, 5) for x in tuned_ms]},
         "roofline": {"bound": "hbm", "kernel": "k_classify", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kScanThreads) void k_generate(
     VolGeom g, const float* __restrict__ field, const uint64_t* __restrict__ bits,
     const uint32_t* __restrict__ base_v, const uint32_t* __restrict__ base_t,
     const Ctrl* __restrict__ ctrl, OutPtrs out, unsigned flags, int64_t cap_points,
-    int64_t cap_tris, int64_t* __restrict__ counts) {
+    int64_t cap_tris, int64_t* __restrict__ counts, int xcd_order) {
   extern __shared__ uint64_t s_dyn[];
   const bool dedup = (flags & DAX_HIP_MC_DEDUP) != 0;
   const bool normals = (flags & DAX_HIP_MC_NORMALS) != 0 && out.normals != nullptr;
@@ -86,9 +86,9 @@ __global__ __launch_bounds__(kScanThreads) void k_generate(
   if (total_v > 0x7fffffffull || 3ull * total_t > 0x7fffffffull) return;
   if ((int64_t)total_t > cap_tris || (int64_t)n_points > cap_points) return;
   const int64_t ntiles = (g.nwords + TW - 1) / TW;
+  const int64_t tile = xcd_order ? xcd_tile(blockIdx.x, ntiles) : (int64_t)blockIdx.x;
   generate_tile<TW>(g, field + g.field_off, bits + g.word_off, base_v + g.word_off,
-                    base_t + g.word_off, xcd_tile(blockIdx.x, ntiles) * TW, out, dedup, normals,
-                    0, 0, s_dyn);
+                    base_t + g.word_off, tile * TW, out, dedup, normals, 0, 0, s_dyn);
 }
 
 // ---------------------------------------------------------------------------
@@ -100,6 +100,7 @@ struct Tuning {
   int cls_vec = 1;      // floats per lane per load
   int scan_sub = 4;     // scan tile = scan_sub * 1024 words
   int gen_tile = 512;   // generate tile words
+  int gen_xcd = 0;      // 1: each XCD takes a contiguous range of generate tiles
 };
 
 std::mutex g_tune_mu;
@@ -116,7 +117,8 @@ bool valid_tuning(const Tuning& t) {
   const bool cls = (t.cls_vec == 1 && (t.cls_unroll == 8 || t.cls_unroll == 16 || t.cls_unroll == 32)) ||
                    (t.cls_vec == 4 && (t.cls_unroll == 2 || t.cls_unroll == 4 || t.cls_unroll == 8));
   const bool scan = t.scan_sub == 1 || t.scan_sub == 2 || t.scan_sub == 4 || t.scan_sub == 8;
-  const bool gen = t.gen_tile == 256 || t.gen_tile == 512 || t.gen_tile == 1024;
+  const bool gen = (t.gen_tile == 256 || t.gen_tile == 512 || t.gen_tile == 1024) &&
+                   (t.gen_xcd == 0 || t.gen_xcd == 1);
   return cls && scan && gen;
 }
 
@@ -332,13 +334,13 @@ int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu
 template <int TW>
 int launch_generate_tw(const VolGeom& g, const float* field, const char* ws, const Workspace& L,
                        unsigned flags, const OutPtrs& o, int64_t cap_points, int64_t cap_tris,
-                       int64_t* counts, hipStream_t s) {
+                       int64_t* counts, int xcd, hipStream_t s) {
   const size_t lds = GenLayout<TW>(g.wpr).bytes();
   const int64_t ntiles = ceil_div(g.nwords, (int64_t)TW);
   k_generate<TW><<<dim3((unsigned)std::max<int64_t>(ntiles, 1)), dim3(kScanThreads), lds, s>>>(
       g, field, (const uint64_t*)(ws + L.off_bits), (const uint32_t*)(ws + L.off_base_v),
       (const uint32_t*)(ws + L.off_base_t), (const Ctrl*)(ws + L.off_ctrl), o, flags, cap_points,
-      cap_tris, counts);
+      cap_tris, counts, xcd);
   DAXHIP_LAUNCH_CHECK("k_generate");
   return DAX_HIP_OK;
 }
@@ -348,11 +350,14 @@ int launch_generate(const VolGeom& g, const float* field, const char* ws, const 
                     int64_t* counts, const Tuning& tu, hipStream_t s) {
   switch (tu.gen_tile) {
     case 256:
-      return launch_generate_tw<256>(g, field, ws, L, flags, o, cap_points, cap_tris, counts, s);
+      return launch_generate_tw<256>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
+                                     tu.gen_xcd, s);
     case 1024:
-      return launch_generate_tw<1024>(g, field, ws, L, flags, o, cap_points, cap_tris, counts, s);
+      return launch_generate_tw<1024>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
+                                      tu.gen_xcd, s);
     default:
-      return launch_generate_tw<512>(g, field, ws, L, flags, o, cap_points, cap_tris, counts, s);
+      return launch_generate_tw<512>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
+                                     tu.gen_xcd, s);
   }
 }
 
@@ -586,19 +591,20 @@ struct EventPair {
 
 extern "C" {
 
-int dax_hip_mc_set_tuning(int64_t nx, int64_t ny, int64_t nz, const int* h_params4) {
-  if (!h_params4) {
+int dax_hip_mc_set_tuning(int64_t nx, int64_t ny, int64_t nz, const int* h_params5) {
+  if (!h_params5) {
     set_error("dax_hip_mc_set_tuning: NULL parameters");
     return DAX_HIP_BAD_VALUE;
   }
   Tuning t;
-  t.cls_unroll = h_params4[0];
-  t.cls_vec = h_params4[1];
-  t.scan_sub = h_params4[2];
-  t.gen_tile = h_params4[3];
+  t.cls_unroll = h_params5[0];
+  t.cls_vec = h_params5[1];
+  t.scan_sub = h_params5[2];
+  t.gen_tile = h_params5[3];
+  t.gen_xcd = h_params5[4];
   if (!valid_tuning(t)) {
-    set_error("dax_hip_mc_set_tuning: invalid parameters {%d,%d,%d,%d}", t.cls_unroll, t.cls_vec,
-              t.scan_sub, t.gen_tile);
+    set_error("dax_hip_mc_set_tuning: invalid parameters {%d,%d,%d,%d,%d}", t.cls_unroll,
+              t.cls_vec, t.scan_sub, t.gen_tile, t.gen_xcd);
     return DAX_HIP_BAD_VALUE;
   }
   std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -609,12 +615,13 @@ int dax_hip_mc_set_tuning(int64_t nx, int64_t ny, int64_t nz, const int* h_param
   return DAX_HIP_OK;
 }
 
-int dax_hip_mc_get_tuning(int64_t nx, int64_t ny, int64_t nz, int* h_params4) {
+int dax_hip_mc_get_tuning(int64_t nx, int64_t ny, int64_t nz, int* h_params5) {
   const Tuning t = (nx == 0 && ny == 0 && nz == 0) ? g_tune_default : tuning_for(nx, ny, nz);
-  h_params4[0] = t.cls_unroll;
-  h_params4[1] = t.cls_vec;
-  h_params4[2] = t.scan_sub;
-  h_params4[3] = t.gen_tile;
+  h_params5[0] = t.cls_unroll;
+  h_params5[1] = t.cls_vec;
+  h_params5[2] = t.scan_sub;
+  h_params5[3] = t.gen_tile;
+  h_params5[4] = t.gen_xcd;
   return DAX_HIP_OK;
 }
 
@@ -625,11 +632,11 @@ int dax_hip_mc_tuning_save(const char* path) {
     return DAX_HIP_BAD_VALUE;
   }
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  fprintf(fp, "# nx ny nz classify_unroll classify_vec scan_sub gen_tile\n");
+  fprintf(fp, "# nx ny nz classify_unroll classify_vec scan_sub gen_tile gen_xcd\n");
   for (const auto& kv : g_tune_cache)
-    fprintf(fp, "%lld %lld %lld %d %d %d %d\n", (long long)std::get<0>(kv.first),
+    fprintf(fp, "%lld %lld %lld %d %d %d %d %d\n", (long long)std::get<0>(kv.first),
             (long long)std::get<1>(kv.first), (long long)std::get<2>(kv.first), kv.second.cls_unroll,
-            kv.second.cls_vec, kv.second.scan_sub, kv.second.gen_tile);
+            kv.second.cls_vec, kv.second.scan_sub, kv.second.gen_tile, kv.second.gen_xcd);
   fclose(fp);
   return DAX_HIP_OK;
 }
@@ -645,8 +652,8 @@ int dax_hip_mc_tuning_load(const char* path) {
     if (line[0] == '#') continue;
     long long x, y, z;
     Tuning t;
-    if (sscanf(line, "%lld %lld %lld %d %d %d %d", &x, &y, &z, &t.cls_unroll, &t.cls_vec,
-               &t.scan_sub, &t.gen_tile) == 7 &&
+    if (sscanf(line, "%lld %lld %lld %d %d %d %d %d", &x, &y, &z, &t.cls_unroll, &t.cls_vec,
+               &t.scan_sub, &t.gen_tile, &t.gen_xcd) == 8 &&
         valid_tuning(t)) {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       g_tune_cache[std::make_tuple((int64_t)x, (int64_t)y, (int64_t)z)] = t;
@@ -660,7 +667,7 @@ int dax_hip_mc_autotune(const float* d_field, int64_t nx, int64_t ny, int64_t nz
                         unsigned flags, void* d_workspace, size_t workspace_bytes,
                         float* d_vertices, float* d_normals, int32_t* d_conn, uint32_t* d_edges,
                         int64_t cap_points, int64_t cap_tris, int64_t* d_counts, int reps,
-                        int* h_best4, float* h_best_ms3, void* stream) {
+                        int* h_best5, float* h_best_ms3, void* stream) {
   VolGeom g;
   int rc = make_geom(nx, ny, nz, nullptr, nullptr, iso, g);
   if (rc) return rc;
@@ -715,9 +722,11 @@ int dax_hip_mc_autotune(const float* d_field, int64_t nx, int64_t ny, int64_t nz
   // generate variants over one classify + scan
   if ((rc = launch_bits(g, d_field, ws, L, best, s))) return rc;
   if ((rc = launch_scan(g, ws, L, best, s))) return rc;
-  for (int tw : {256, 512, 1024}) {
+  for (int tv = 0; tv < 6; ++tv) {
+    const int tw = tv % 3 == 0 ? 256 : (tv % 3 == 1 ? 512 : 1024);
     Tuning t = best;
     t.gen_tile = tw;
+    t.gen_xcd = tv / 3;
     if ((rc = launch_generate(g, d_field, ws, L, flags, o, cap_points, cap_tris, d_counts, t, s)))
       return rc;
     DAXHIP_CHECK(hipEventRecord(ev.a, s));
@@ -730,17 +739,19 @@ int dax_hip_mc_autotune(const float* d_field, int64_t nx, int64_t ny, int64_t nz
     if (ms < best_ms[2]) {
       best_ms[2] = ms;
       best.gen_tile = tw;
+      best.gen_xcd = t.gen_xcd;
     }
   }
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[std::make_tuple(nx, ny, nz)] = best;
   }
-  if (h_best4) {
-    h_best4[0] = best.cls_unroll;
-    h_best4[1] = best.cls_vec;
-    h_best4[2] = best.scan_sub;
-    h_best4[3] = best.gen_tile;
+  if (h_best5) {
+    h_best5[0] = best.cls_unroll;
+    h_best5[1] = best.cls_vec;
+    h_best5[2] = best.scan_sub;
+    h_best5[3] = best.gen_tile;
+    h_best5[4] = best.gen_xcd;
   }
   if (h_best_ms3)
     for (int i = 0; i < 3; ++i) h_best_ms3[i] = best_ms[i];

@@ -387,18 +387,71 @@ __device__ __forceinline__ int find_word(const uint32_t* s, int n, uint32_t v) {
   return lo;
 }
 
+// Position of the k-th set bit of m (k < popcount(m)).
+__device__ __forceinline__ int select_bit(uint64_t m, int k) {
+  int lo = 0, hi = 63;  // largest i with popc(m & ((1<<i)-1)) <= k
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (popc(m & ((1ull << mid) - 1ull)) <= k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 // LDS layout of a generate tile of TW words (wpr known at run time).
+constexpr int kMaxGenThreads = 256;
 template <int TW>
 struct GenLayout {
-  static constexpr int kVCap = 3 * TW;   // vertex map entries
-  static constexpr int kTCap = 6 * TW;   // triangle map entries
   int segb, segv;
   __host__ __device__ GenLayout(int wpr) : segb(TW + 2 * wpr + 2), segv(TW + wpr + 2) {}
+  // u64: bits (3 planes), case table, own-word flags fx/fy/fz, active cells,
+  // packed positions; u32: vertex bases (2 planes), triangle bases,
+  // active-cell slots, scan scratch
   __host__ __device__ size_t bytes() const {
-    return (size_t)(3 * segb + 256) * 8 + (size_t)(2 * segv + TW + 1 + kVCap + kTCap) * 4;
+    return (size_t)(3 * segb + 256 + 5 * TW) * 8 +
+           (size_t)(2 * segv + 2 * (TW + 1) + 32 + 2 * kMaxGenThreads + 1) * 4;
   }
 };
 
+__device__ __forceinline__ uint64_t pack_pos(int q, int j, int k) {
+  return (uint64_t)(uint32_t)q | ((uint64_t)(uint32_t)j << 16) | ((uint64_t)(uint32_t)k << 40);
+}
+__device__ __forceinline__ void unpack_pos(uint64_t v, int& q, int& j, int& k) {
+  q = (int)(v & 0xffffu);
+  j = (int)((v >> 16) & 0xffffffu);
+  k = (int)(v >> 40);
+}
+
+// Exclusive scan of one value per thread over the block (thread order);
+// returns the thread's offset and the block total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total, uint32_t* s_w) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  const uint32_t inc = wave_incl_scan(v);
+  if (lane == 63) s_w[wv] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int q = 0; q < nwv; ++q) {
+    if (q < wv) off += s_w[q];
+    tot += s_w[q];
+  }
+  __syncthreads();
+  total = tot;
+  return off + inc - v;
+}
+
+// One generate tile: words [w_begin, w_begin + TW) of a volume.
+//   1. stage bits (planes k, k+1, k+2), vertex bases (k, k+1), triangle bases
+//   2. per word: edge flags, active-cell mask, position; block scan of the
+//      active-cell counts gives every active cell a slot
+//   3. one thread per output vertex: word by binary search over the vertex
+//      bases, edge by popcount select; coordinates, normal, records
+//   4. one thread per active cell: word by binary search over the slots,
+//      cell by popcount select; a block scan of NumFaces gives its first
+//      triangle; its triangles' corners become vertex ids through the edge
+//      flags of the neighbouring words
+// No per-thread loop depends on how many cells or edges a word holds.
 template <int TW>
 __device__ __forceinline__ void generate_tile(const VolGeom& g, const float* __restrict__ f,
                                               const uint64_t* __restrict__ b,
@@ -407,7 +460,7 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const float* __r
                                               const OutPtrs& out, bool dedup, bool normals,
                                               int64_t vol_base_v, int64_t vol_base_t,
                                               uint64_t* s_dyn) {
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, nt = blockDim.x;
   const int nw = (int)min((int64_t)TW, g.nwords - w_begin);
   const uint32_t tv0 = bv[w_begin], tv1 = bv[w_begin + nw];
   const uint32_t tt0 = bt[w_begin], tt1 = bt[w_begin + nw];
@@ -418,111 +471,89 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const float* __r
   uint64_t* s_b1 = s_b0 + L.segb;
   uint64_t* s_b2 = s_b1 + L.segb;
   uint64_t* s_case = s_b2 + L.segb;
-  uint32_t* s_v0 = reinterpret_cast<uint32_t*>(s_case + 256);
+  uint64_t* s_fx = s_case + 256;
+  uint64_t* s_fy = s_fx + TW;
+  uint64_t* s_fz = s_fy + TW;
+  uint64_t* s_act = s_fz + TW;
+  uint64_t* s_pos = s_act + TW;
+  uint32_t* s_v0 = reinterpret_cast<uint32_t*>(s_pos + TW);
   uint32_t* s_v1 = s_v0 + L.segv;
   uint32_t* s_t = s_v1 + L.segv;
-  uint32_t* s_vmap = s_t + TW + 1;
-  uint32_t* s_tmap = s_vmap + GenLayout<TW>::kVCap;
+  uint32_t* s_cs = s_t + TW + 1;
+  uint32_t* s_w = s_cs + TW + 1;  // scan scratch (<= 16 waves)
   uint64_t* s_bz[3] = {s_b0, s_b1, s_b2};
   uint32_t* s_vz[2] = {s_v0, s_v1};
 
   const int64_t plane = (int64_t)g.wpr * g.ny;
-  for (int i = tid; i < L.segb; i += blockDim.x) {
+  for (int i = tid; i < L.segb; i += nt) {
     const int64_t w = w_begin + i;
     s_b0[i] = (w < g.nwords) ? b[w] : 0ull;
     s_b1[i] = (w + plane < g.nwords) ? b[w + plane] : 0ull;
     s_b2[i] = (w + 2 * plane < g.nwords) ? b[w + 2 * plane] : 0ull;
   }
-  for (int i = tid; i < L.segv; i += blockDim.x) {
+  for (int i = tid; i < L.segv; i += nt) {
     const int64_t w = w_begin + i;
     s_v0[i] = (w <= g.nwords) ? bv[w] : 0u;
     s_v1[i] = (w + plane <= g.nwords) ? bv[w + plane] : 0u;
   }
-  for (int i = tid; i <= nw; i += blockDim.x) s_t[i] = bt[w_begin + i];
-  for (int i = tid; i < 256; i += blockDim.x) s_case[i] = c_case_table[i];
+  for (int i = tid; i <= nw; i += nt) s_t[i] = bt[w_begin + i];
+  for (int i = tid; i < 256; i += nt) s_case[i] = c_case_table[i];
   __syncthreads();
 
-  // maps: vertex -> (word, bit, axis), triangle -> (word, cell bit, visit)
-  const int nV = (int)(tv1 - tv0), nT = (int)(tt1 - tt0);
-  constexpr int kWpt = (TW + kScanThreads - 1) / kScanThreads;
+  // 2. per word (each thread a run of consecutive words)
+  const int wpt = (TW + nt - 1) / nt;
+  uint32_t ncell = 0;
   {
-    const int i0 = tid * kWpt;
+    const int i0 = tid * wpt;
     if (i0 < nw) {
       WordIter it;
       it.init(g, (int)(w_begin + i0));
-#pragma unroll
-      for (int u = 0; u < kWpt; ++u) {
+      for (int u = 0; u < wpt && i0 + u < nw; ++u) {
         const int li = i0 + u;
-        if (li < nw) {
-          const WordPos p = it.pos(g);
-          const bool nx1 = p.q + 1 < g.wpr;
-          const uint64_t A = s_b0[li], A1 = nx1 ? s_b0[li + 1] : 0ull;
-          const uint64_t B = s_b0[li + g.wpr], B1 = nx1 ? s_b0[li + g.wpr + 1] : 0ull;
-          const uint64_t C = s_b1[li], C1 = nx1 ? s_b1[li + 1] : 0ull;
-          const uint64_t D = s_b1[li + g.wpr], D1 = nx1 ? s_b1[li + g.wpr + 1] : 0ull;
-          if (dedup) {
-            uint64_t fx, fy, fz;
-            edge_flags(p, A, A1, B, C, fx, fy, fz);
-            uint64_t any = fx | fy | fz;
-            int r = (int)(s_v0[li] - tv0);
-            while (any && r < GenLayout<TW>::kVCap) {
-              const int bit = __builtin_ctzll(any);
-              any &= any - 1;
-#pragma unroll
-              for (int a = 0; a < 3; ++a) {
-                const uint64_t fa = a == 0 ? fx : (a == 1 ? fy : fz);
-                if ((fa >> bit) & 1ull) {
-                  if (r < GenLayout<TW>::kVCap)
-                    s_vmap[r] = ((uint32_t)li << 8) | ((uint32_t)bit << 2) | (uint32_t)a;
-                  ++r;
-                }
-              }
-            }
-          }
-          CellPlanes cp;
-          cell_planes(p, A, A1, B, B1, C, C1, D, D1, cp);
-          uint64_t act = cp.active;
-          int r = (int)(s_t[li] - tt0);
-          while (act && r < GenLayout<TW>::kTCap) {
-            const int bit = __builtin_ctzll(act);
-            act &= act - 1;
-            const int nf = (int)(s_case[cell_case(cp, bit)] >> 60);
-            for (int s = 0; s < nf; ++s, ++r)
-              if (r < GenLayout<TW>::kTCap)
-                s_tmap[r] = ((uint32_t)li << 9) | ((uint32_t)bit << 3) | (uint32_t)s;
-          }
-          it.next(g);
-        }
+        const WordPos p = it.pos(g);
+        const bool nx1 = p.q + 1 < g.wpr;
+        const uint64_t A = s_b0[li], A1 = nx1 ? s_b0[li + 1] : 0ull;
+        const uint64_t B = s_b0[li + g.wpr], B1 = nx1 ? s_b0[li + g.wpr + 1] : 0ull;
+        const uint64_t C = s_b1[li], C1 = nx1 ? s_b1[li + 1] : 0ull;
+        const uint64_t D = s_b1[li + g.wpr], D1 = nx1 ? s_b1[li + g.wpr + 1] : 0ull;
+        uint64_t fx, fy, fz;
+        edge_flags(p, A, A1, B, C, fx, fy, fz);
+        CellPlanes cp;
+        cell_planes(p, A, A1, B, B1, C, C1, D, D1, cp);
+        s_fx[li] = fx;
+        s_fy[li] = fy;
+        s_fz[li] = fz;
+        s_act[li] = cp.active;
+        s_pos[li] = pack_pos(p.q, p.j, p.k);
+        ncell += (uint32_t)popc(cp.active);
+        it.next(g);
       }
     }
   }
+  uint32_t n_cells;
+  {
+    uint32_t run = block_excl_scan(ncell, n_cells, s_w);
+    const int i0 = tid * wpt;
+    for (int u = 0; u < wpt && i0 + u < nw; ++u) {
+      s_cs[i0 + u] = run;
+      run += (uint32_t)popc(s_act[i0 + u]);
+    }
+    if (tid == 0) s_cs[nw] = n_cells;
+  }
   __syncthreads();
 
-  // ---- vertices (dedup: one per active edge, in id order)
+  // 3. vertices (dedup: one per active edge, in id order)
+  const int nV = (int)(tv1 - tv0);
   if (dedup) {
-    for (int vl = tid; vl < nV; vl += blockDim.x) {
-      int li, bit, axis;
-      if (vl < GenLayout<TW>::kVCap) {
-        const uint32_t e = s_vmap[vl];
-        li = (int)(e >> 8);
-        bit = (int)((e >> 2) & 63);
-        axis = (int)(e & 3);
-      } else {
-        li = find_word(s_v0, nw, tv0 + (uint32_t)vl);
-        WordIter it;
-        it.init(g, (int)(w_begin + li));
-        const WordPos p = it.pos(g);
-        const bool nx1 = p.q + 1 < g.wpr;
-        uint64_t fx, fy, fz;
-        edge_flags(p, s_b0[li], nx1 ? s_b0[li + 1] : 0ull, s_b0[li + g.wpr], s_b1[li], fx, fy, fz);
-        edge_select(fx, fy, fz, (int)(tv0 + (uint32_t)vl - s_v0[li]), bit, axis);
-      }
-      WordIter it;
-      it.init(g, (int)(w_begin + li));
-      const int x = it.q * 64 + bit;
+    for (int vl = tid; vl < nV; vl += nt) {
+      const int li = find_word(s_v0, nw, tv0 + (uint32_t)vl);
+      int bit, axis;
+      edge_select(s_fx[li], s_fy[li], s_fz[li], (int)(tv0 + (uint32_t)vl - s_v0[li]), bit, axis);
+      int q, j, k;
+      unpack_pos(s_pos[li], q, j, k);
       float xyz[3], nrm[3], w;
       int64_t pa, pb;
-      emit_point(g, f, x, it.j, it.k, axis, normals, xyz, nrm, w, pa, pb);
+      emit_point(g, f, q * 64 + bit, j, k, axis, normals, xyz, nrm, w, pa, pb);
       const int64_t gv = (int64_t)tv0 + vl;
       out.vertices[3 * gv + 0] = xyz[0];
       out.vertices[3 * gv + 1] = xyz[1];
@@ -539,99 +570,107 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const float* __r
     }
   }
 
-  // ---- triangles
-  for (int tl = tid; tl < nT; tl += blockDim.x) {
-    int li, bit, visit, cs;
-    WordIter it;
-    CellPlanes cp;
-    if (tl < GenLayout<TW>::kTCap) {
-      const uint32_t e = s_tmap[tl];
-      li = (int)(e >> 9);
-      bit = (int)((e >> 3) & 63);
-      visit = (int)(e & 7);
-      it.init(g, (int)(w_begin + li));
-      const WordPos p = it.pos(g);
-      const bool nx1 = p.q + 1 < g.wpr;
+  // 4. triangles: rounds of up to blockDim active cells; a block scan of
+  //    NumFaces gives each cell its first triangle, then one thread per
+  //    triangle finds its cell by binary search over those offsets
+  uint32_t* s_rfirst = s_cs + TW + 1 + 32;  // [nt + 1]
+  uint32_t* s_rinfo = s_rfirst + nt + 1;    // [nt]: li << 16 | bit << 8 | case
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < n_cells; c0 += (uint32_t)nt) {  // uniform
+    const uint32_t slot = c0 + (uint32_t)tid;
+    uint32_t nf = 0;
+    if (slot < n_cells) {
+      const int li = find_word(s_cs, nw, slot);
+      const int bit = select_bit(s_act[li], (int)(slot - s_cs[li]));
+      int q, j, k;
+      unpack_pos(s_pos[li], q, j, k);
+      WordPos p;
+      p.q = q;
+      p.j = j;
+      p.k = k;
+      p.has_y = j + 1 < g.ny;
+      p.has_z = k + 1 < g.nz;
+      p.pmask = low_mask(g.nx - q * 64);
+      p.xmask = low_mask(g.nx - 1 - q * 64);
+      const bool nx1 = q + 1 < g.wpr;
+      CellPlanes cp;
       cell_planes(p, s_b0[li], nx1 ? s_b0[li + 1] : 0ull, s_b0[li + g.wpr],
                   nx1 ? s_b0[li + g.wpr + 1] : 0ull, s_b1[li], nx1 ? s_b1[li + 1] : 0ull,
                   s_b1[li + g.wpr], nx1 ? s_b1[li + g.wpr + 1] : 0ull, cp);
-      cs = cell_case(cp, bit);
-    } else {
-      li = find_word(s_t, nw, tt0 + (uint32_t)tl);
-      it.init(g, (int)(w_begin + li));
-      const WordPos p = it.pos(g);
-      const bool nx1 = p.q + 1 < g.wpr;
-      cell_planes(p, s_b0[li], nx1 ? s_b0[li + 1] : 0ull, s_b0[li + g.wpr],
-                  nx1 ? s_b0[li + g.wpr + 1] : 0ull, s_b1[li], nx1 ? s_b1[li + 1] : 0ull,
-                  s_b1[li + g.wpr], nx1 ? s_b1[li + g.wpr + 1] : 0ull, cp);
-      int k = (int)(tt0 + (uint32_t)tl - s_t[li]);
-      uint64_t act = cp.active;
-      bool found = false;
-      bit = 0;
-      cs = 0;
-      while (act) {
-        bit = __builtin_ctzll(act);
-        act &= act - 1;
-        cs = cell_case(cp, bit);
-        const int nf = (int)(s_case[cs] >> 60);
-        if (k < nf) {
-          found = true;
-          break;
-        }
-        k -= nf;
-      }
-      if (!found) continue;  // unreachable when the scan and this kernel agree
-      visit = k;
+      const int cs = cell_case(cp, bit);
+      nf = (uint32_t)(s_case[cs] >> 60);
+      s_rinfo[tid] = ((uint32_t)li << 16) | ((uint32_t)bit << 8) | (uint32_t)cs;
     }
-    const uint64_t packed = s_case[cs];
-    const int64_t t = (int64_t)tt0 + tl;
-    int32_t ids[3];
+    uint32_t round_tris;
+    s_rfirst[tid] = block_excl_scan(nf, round_tris, s_w);
+    if (tid == 0) s_rfirst[nt] = round_tris;
+    __syncthreads();
+    const int ncell_round = (int)min((uint32_t)nt, n_cells - c0);
+    for (uint32_t tl = (uint32_t)tid; tl < round_tris; tl += (uint32_t)nt) {
+      const int c = find_word(s_rfirst, ncell_round, tl);
+      const uint32_t info = s_rinfo[c];
+      const int li = (int)(info >> 16), bit = (int)((info >> 8) & 0xff), cs = (int)(info & 0xff);
+      const int visit = (int)(tl - s_rfirst[c]);
+      int q, j, k;
+      unpack_pos(s_pos[li], q, j, k);
+      const uint64_t packed = s_case[cs];
+      const int64_t t = (int64_t)tt0 + carry + tl;
+      int32_t ids[3];
 #pragma unroll
-    for (int v = 0; v < 3; ++v) {
-      const int e = (int)((packed >> (4 * (3 * visit + v))) & 0xF);
-      const uint8_t code = edge_code(e);
-      const int dx = code & 1, dy = (code >> 1) & 1, dz = (code >> 2) & 1, axis = code >> 3;
-      const int x = it.q * 64 + bit + dx;
-      if (dedup) {
-        const int qq = x >> 6, bb = x & 63;
-        const int lw2 = li + (qq - it.q) + dy * g.wpr;
-        WordPos p2;
-        p2.q = qq;
-        p2.j = it.j + dy;
-        p2.k = it.k + dz;
-        p2.has_y = p2.j + 1 < g.ny;
-        p2.has_z = p2.k + 1 < g.nz;
-        p2.pmask = low_mask(g.nx - qq * 64);
-        p2.xmask = low_mask(g.nx - 1 - qq * 64);
-        const bool nx2 = qq + 1 < g.wpr;
-        const uint64_t* sb = s_bz[dz];
-        const uint64_t A = sb[lw2], A1 = nx2 ? sb[lw2 + 1] : 0ull;
-        const uint64_t B = sb[lw2 + g.wpr], C = s_bz[dz + 1][lw2];
-        uint64_t fx, fy, fz;
-        edge_flags(p2, A, A1, B, C, fx, fy, fz);
-        ids[v] = (int32_t)((int64_t)s_vz[dz][lw2] + edge_rank(fx, fy, fz, bb, axis) - vol_base_v);
-      } else {
-        const int64_t corner = 3 * t + v;
-        float xyz[3], nrm[3], w;
-        int64_t pa, pb;
-        emit_point(g, f, x, it.j + dy, it.k + dz, axis, normals, xyz, nrm, w, pa, pb);
-        out.vertices[3 * corner + 0] = xyz[0];
-        out.vertices[3 * corner + 1] = xyz[1];
-        out.vertices[3 * corner + 2] = xyz[2];
-        if (normals) {
-          out.normals[3 * corner + 0] = nrm[0];
-          out.normals[3 * corner + 1] = nrm[1];
-          out.normals[3 * corner + 2] = nrm[2];
+      for (int v = 0; v < 3; ++v) {
+        const int e = (int)((packed >> (4 * (3 * visit + v))) & 0xF);
+        const uint8_t code = edge_code(e);
+        const int dx = code & 1, dy = (code >> 1) & 1, dz = (code >> 2) & 1, axis = code >> 3;
+        const int x = q * 64 + bit + dx;
+        if (dedup) {
+          const int qq = x >> 6, bb = x & 63;
+          const int lw2 = li + (qq - q) + dy * g.wpr;
+          uint64_t fx, fy, fz;
+          if (dz == 0 && lw2 < nw) {
+            fx = s_fx[lw2];
+            fy = s_fy[lw2];
+            fz = s_fz[lw2];
+          } else {
+            WordPos p2;
+            p2.q = qq;
+            p2.j = j + dy;
+            p2.k = k + dz;
+            p2.has_y = p2.j + 1 < g.ny;
+            p2.has_z = p2.k + 1 < g.nz;
+            p2.pmask = low_mask(g.nx - qq * 64);
+            p2.xmask = low_mask(g.nx - 1 - qq * 64);
+            const bool nx2 = qq + 1 < g.wpr;
+            const uint64_t* sb = s_bz[dz];
+            edge_flags(p2, sb[lw2], nx2 ? sb[lw2 + 1] : 0ull, sb[lw2 + g.wpr], s_bz[dz + 1][lw2],
+                       fx, fy, fz);
+          }
+          ids[v] =
+              (int32_t)((int64_t)s_vz[dz][lw2] + edge_rank(fx, fy, fz, bb, axis) - vol_base_v);
+        } else {
+          const int64_t corner = 3 * t + v;
+          float xyz[3], nrm[3], w;
+          int64_t pa, pb;
+          emit_point(g, f, x, j + dy, k + dz, axis, normals, xyz, nrm, w, pa, pb);
+          out.vertices[3 * corner + 0] = xyz[0];
+          out.vertices[3 * corner + 1] = xyz[1];
+          out.vertices[3 * corner + 2] = xyz[2];
+          if (normals) {
+            out.normals[3 * corner + 0] = nrm[0];
+            out.normals[3 * corner + 1] = nrm[1];
+            out.normals[3 * corner + 2] = nrm[2];
+          }
+          if (out.interp_first) out.interp_first[corner] = (int32_t)pa;
+          if (out.interp_second) out.interp_second[corner] = (int32_t)pb;
+          if (out.interp_weight) out.interp_weight[corner] = w;
+          ids[v] = (int32_t)(corner - 3 * vol_base_t);
         }
-        if (out.interp_first) out.interp_first[corner] = (int32_t)pa;
-        if (out.interp_second) out.interp_second[corner] = (int32_t)pb;
-        if (out.interp_weight) out.interp_weight[corner] = w;
-        ids[v] = (int32_t)(corner - 3 * vol_base_t);
       }
+      out.conn[3 * t + 0] = ids[0];
+      out.conn[3 * t + 1] = ids[1];
+      out.conn[3 * t + 2] = ids[2];
     }
-    out.conn[3 * t + 0] = ids[0];
-    out.conn[3 * t + 1] = ids[1];
-    out.conn[3 * t + 2] = ids[2];
+    carry += round_tris;
+    __syncthreads();  // s_rfirst / s_rinfo reused by the next round
   }
 }
 

@@ -221,11 +221,11 @@ def backward(field: torch.Tensor, iso: float, contour: Contour, dl_dvertices: to
 
 def autotune(field: torch.Tensor, iso: float, buf: ForwardBuffers, reps: int = 10,
              dedup: bool = True, normals: bool = True):
-    """Run the native autotuner for this shape; returns (params4, ms3)."""
+    """Run the native autotuner for this shape; returns (params5, ms3)."""
     import ctypes
     nx, ny, nz = _check_field(field)
     flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and buf.normals is not None) else 0)
-    best = (ctypes.c_int * 4)()
+    best = (ctypes.c_int * 5)()
     ms = (ctypes.c_float * 3)()
     _lib.call("dax_hip_mc_autotune", field.data_ptr(), nx, ny, nz, float(iso), flags,
               buf.workspace.data_ptr(), buf.workspace.numel(), buf.vertices.data_ptr(),
@@ -234,17 +234,19 @@ def autotune(field: torch.Tensor, iso: float, buf: ForwardBuffers, reps: int = 1
     return list(best), list(ms)
 
 
-def set_tuning(shape, params4) -> None:
+def set_tuning(shape, params) -> None:
+    """params: classify_unroll, classify_vec, scan_sub, gen_tile[, gen_xcd=1]."""
     import ctypes
     nx, ny, nz = shape
-    arr = (ctypes.c_int * 4)(*params4)
+    p = list(params) + [1] * (5 - len(params))
+    arr = (ctypes.c_int * 5)(*p)
     _lib.call("dax_hip_mc_set_tuning", nx, ny, nz, arr)
 
 
 def get_tuning(shape):
     import ctypes
     nx, ny, nz = shape
-    arr = (ctypes.c_int * 4)()
+    arr = (ctypes.c_int * 5)()
     _lib.call("dax_hip_mc_get_tuning", nx, ny, nz, arr)
     return list(arr)
 

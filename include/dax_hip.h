@@ -231,23 +231,25 @@ int dax_hip_mc_cell_outputs(int64_t nx, int64_t ny, int64_t nz, const void* d_wo
                             int32_t* d_offsets, void* stream);
 
 /* ---- tile-shape tuning --------------------------------------------------
- * params4 = {classify_unroll, classify_vec, scan_sub, gen_tile}:
+ * params5 = {classify_unroll, classify_vec, scan_sub, gen_tile, gen_xcd}:
  *   classify_vec 1 (one float per lane, unroll 8/16/32 row words per wave)
  *   or 4 (float4 per lane, unroll 2/4/8 chunks of 256 points; rows of a
  *   multiple of 64 points); scan_sub 1/2/4/8 (scan tile = scan_sub*1024
- *   row words); gen_tile 256/512/1024 (row words per generate block).
+ *   row words); gen_tile 256/512/1024 (row words per generate block);
+ *   gen_xcd 1 gives each XCD a contiguous range of generate tiles, 0 keeps
+ *   the plain block order.
  * A shape of (0,0,0) sets or gets the default for shapes not tuned. */
-int dax_hip_mc_set_tuning(int64_t nx, int64_t ny, int64_t nz, const int* h_params4);
-int dax_hip_mc_get_tuning(int64_t nx, int64_t ny, int64_t nz, int* h_params4);
+int dax_hip_mc_set_tuning(int64_t nx, int64_t ny, int64_t nz, const int* h_params5);
+int dax_hip_mc_get_tuning(int64_t nx, int64_t ny, int64_t nz, int* h_params5);
 /* Times every variant of each kernel with HIP events (reps launches each),
- * keeps the fastest per kernel for this shape and returns it (h_best4) with
+ * keeps the fastest per kernel for this shape and returns it (h_best5) with
  * the per-kernel milliseconds (h_best_ms3, may be NULL).  Synchronous.  The
  * outputs and workspace are those of dax_hip_mc_forward. */
 int dax_hip_mc_autotune(const float* d_field, int64_t nx, int64_t ny, int64_t nz, float iso,
                         unsigned flags, void* d_workspace, size_t workspace_bytes,
                         float* d_vertices, float* d_normals, int32_t* d_conn, uint32_t* d_edges,
                         int64_t cap_points, int64_t cap_tris, int64_t* d_counts, int reps,
-                        int* h_best4, float* h_best_ms3, void* stream);
+                        int* h_best5, float* h_best_ms3, void* stream);
 /* Persist / restore the per-shape choices (text, one shape per line). */
 int dax_hip_mc_tuning_save(const char* path);
 int dax_hip_mc_tuning_load(const char* path);

@@ -16,6 +16,14 @@ ARRAY_SIZE, OFFSET = 500, 1000  # TestingDeviceAdapter.h:25-26
 S = lambda: torch.cuda.current_stream().cuda_stream
 
 
+def stable_order(k, desc):
+    """Permutation of a stable sort of k, ascending or descending."""
+    if not desc:
+        return np.argsort(k, kind="stable")
+    _, rank = np.unique(k, return_inverse=True)
+    return np.argsort(-rank.astype(np.int64), kind="stable")
+
+
 def dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
@@ -105,9 +113,7 @@ def test_sort_and_sort_by_key(suf, dt, n):
         keys = dev(k.copy())
         vals = dev(np.arange(n, dtype=np.int32))
         _lib.call(f"dax_hip_sort_by_key_{suf}", keys.data_ptr(), vals.data_ptr(), 4, n, desc, S())
-        order = np.argsort(-k if desc and np.issubdtype(dt, np.signedinteger | np.floating) else k,
-                           kind="stable") if not (desc and dt == np.uint32) else \
-            np.argsort(~k, kind="stable")
+        order = stable_order(k, desc)
         assert np.array_equal(keys.cpu().numpy(), k[order])
         assert np.array_equal(vals.cpu().numpy(), order.astype(np.int32))
         s = dev(k.copy())
