@@ -1,6 +1,6 @@
 """Benchmark of the marching-cubes isosurface path on one MI355X.
 
-Workload (default): 512^3-point float32 field, sum of 16 isotropic Gaussians
+Headline workload: 512^3-point float32 field, sum of 16 isotropic Gaussians
 (centres uniform in the box, sigma ~ U[20,60] voxels, amplitude ~ U[0.5,1.5],
 seed 0) plus uniform noise in [-0.01, 0.01]; isovalue 0.5; marching cubes
 with duplicate-point removal, central-difference point-gradient normals and
@@ -10,7 +10,16 @@ generate) over the resident field into preallocated outputs.
 Prints one JSON line.  `value` is Gcells/s of the forward; `roofline` is the
 classify kernel (the dominant one) measured with HIP events on the stream the
 kernels run on; `cpu_baseline` times the OpenMP restatement of Dax's
-OpenMP-adapter pipeline on the host cores on the same field.
+OpenMP-adapter pipeline on the host cores on the same field.  Beside the
+headline the line carries
+  fwd_bwd  the same field, forward + adjoint (upstream dL/dvertex and
+           dL/dnormal N(0,1), seed 1), and its ratio to the forward;
+  batched  1024 volumes with per-axis extents U{32..160}, 4-16 Gaussians each,
+           iso U[0.3,0.7]: the batched forward and forward+backward replayed
+           as captured graphs, against a native launch loop of single-volume
+           forwards (+ backwards) into the same outputs;
+  large    (--large) 1024^3 Mandelbulb (power 8, 10 iterations, bailout 2,
+           iso 0.5) forward and forward+backward.
 """
 from __future__ import annotations
 
@@ -40,6 +49,11 @@ def parse():
                     help="classify_unroll,classify_vec,scan_sub,gen_tile[,gen_xcd] (skips the autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the device copy kernel on 512 MiB (counter calibration)")
+    ap.add_argument("--batch", type=int, default=1024, help="volumes of the batched run (0: skip)")
+    ap.add_argument("--no-bwd", action="store_true", help="skip the forward+backward timing")
+    ap.add_argument("--large", action="store_true", help="also run the 1024^3 Mandelbulb")
+    ap.add_argument("--only", choices=["fwd", "batch", "large"], default=None,
+                    help="run one section only (profiling); prints its JSON")
     return ap.parse_args()
 
 
@@ -81,6 +95,164 @@ def cpu_baseline(f_host, iso, cells, budget_s):
     }
 
 
+def _timed(fn, steps):
+    """Mean ms of fn() over `steps` calls, HIP events on the current stream."""
+    import torch
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def _upstream(U, seed):
+    import torch
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    dv = torch.randn((max(U, 1), 3), generator=gen, device="cuda", dtype=torch.float32)
+    dn = torch.randn((max(U, 1), 3), generator=gen, device="cuda", dtype=torch.float32)
+    return dv, dn
+
+
+def single_volume(field, iso, label, args, tuning=None):
+    """Forward (+ phases, + fwd+bwd) of one resident volume."""
+    import torch
+
+    from daxtoolkit_amd import mc
+    nz, ny, nx = field.shape
+    P, C = nx * ny * nz, (nx - 1) * (ny - 1) * (nz - 1)
+    ws, cnt = mc.classify(field, iso)
+    T, U = mc.read_counts(cnt)
+    del ws
+    buf = mc.ForwardBuffers(nx, ny, nz, max(U, 1), max(T, 1), normals=True, edges=True)
+    if tuning:
+        mc.set_tuning((nx, ny, nz), tuning)
+        tuned, tuned_ms = tuning, [0.0, 0.0, 0.0]
+    else:
+        tuned, tuned_ms = mc.autotune(field, iso, buf, reps=10)
+    for _ in range(args.warmup):
+        mc.forward_into(field, iso, buf)
+    torch.cuda.synchronize()
+    print(f"[bench] {label}: T={T} U={U} tuning={tuned}", file=sys.stderr, flush=True)
+    wall0 = time.perf_counter()
+    ms = _timed(lambda: mc.forward_into(field, iso, buf), args.steps)
+    wall = time.perf_counter() - wall0
+    assert mc.read_counts(buf.counts, buf.cap_points, buf.cap_tris) == (T, U)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    for k in range(args.steps):
+        mc.forward_phases(field, iso, buf, events=ev[k])
+    torch.cuda.synchronize()
+    k_ms = [statistics.mean(e[i].elapsed_time(e[i + 1]) for e in ev) for i in range(3)]
+    r = {"P": P, "C": C, "T": T, "U": U, "ms": ms, "wall": wall, "k_ms": k_ms,
+         "tuned": tuned, "tuned_ms": tuned_ms}
+    if not args.no_bwd:
+        dv, dn = _upstream(U, 1)
+        scratch = torch.empty(max(int(mc._lib.load().dax_hip_mc_backward_scratch_bytes(U)), 1),
+                              dtype=torch.uint8, device="cuda")
+        dldf = torch.empty_like(field)
+
+        def fb():
+            mc.forward_into(field, iso, buf)
+            mc.backward_into(field, iso, buf, dv, dn, scratch, dldf)
+        for _ in range(2):
+            fb()
+        r["fb_ms"] = _timed(fb, args.steps)
+        r["bwd_ms"] = _timed(lambda: mc.backward_into(field, iso, buf, dv, dn, scratch, dldf),
+                             args.steps)
+        print(f"[bench] {label}: fwd {ms:.4f} ms, fwd+bwd {r['fb_ms']:.4f} ms",
+              file=sys.stderr, flush=True)
+    return r
+
+
+def batched(args):
+    """1024 uneven volumes: batched graphs vs a native per-volume loop."""
+    import torch
+
+    from daxtoolkit_amd import fields, mc
+    vb = fields.volume_batch(args.batch, seed=0)
+    flat = fields.volume_batch_torch(vb)
+    b = mc.Batch(vb.dims, vb.isos)
+    T, U = b.sizes(flat)
+    P = int(flat.numel())
+    C = int(sum((d[0] - 1) * (d[1] - 1) * (d[2] - 1) for d in vb.dims))
+    print(f"[bench] batch: {args.batch} volumes, {P} points, T={T} U={U}", file=sys.stderr,
+          flush=True)
+    bufs = b.alloc(U, T)
+    dv, dn = _upstream(U, 2)
+    scratch = torch.empty(max(b.scratch_bytes(U), 1), dtype=torch.uint8, device="cuda")
+    dldf = torch.empty_like(flat)
+    b.forward_into(flat, bufs)
+    b.backward(flat, bufs, dv, dn, scratch, dldf)
+    torch.cuda.synchronize()
+    offs = b.vol_offsets.cpu().numpy()
+    s = torch.cuda.Stream()
+    g_f = mc.capture(lambda: b.forward_into(flat, bufs), s)
+    g_fb = mc.capture(lambda: (b.forward_into(flat, bufs),
+                               b.backward(flat, bufs, dv, dn, scratch, dldf)), s)
+    steps = max(3, min(args.steps, 10))
+    try:
+        with torch.cuda.stream(s):
+            for g in (g_f, g_fb):
+                mc.Batch.launch(g, s)
+            ms_f = _timed(lambda: mc.Batch.launch(g_f, s), steps)
+            ms_fb = _timed(lambda: mc.Batch.launch(g_fb, s), steps)
+    finally:
+        mc.Batch.destroy(g_f)
+        mc.Batch.destroy(g_fb)
+    assert mc.read_counts(b.counts) == (T, U)
+    lbufs = b.alloc(U, T)
+    ldldf = torch.empty_like(flat)
+    lws = b.loop_workspace()
+    lcnt = torch.zeros(2, dtype=torch.int64, device="cuda")
+    b.loop(flat, lbufs, offs, lws, lcnt)
+    ms_lf = _timed(lambda: b.loop(flat, lbufs, offs, lws, lcnt), steps)
+    grads = (dv, dn, torch.empty_like(scratch), ldldf)
+    b.loop(flat, lbufs, offs, lws, lcnt, grads=grads)
+    ms_lfb = _timed(lambda: b.loop(flat, lbufs, offs, lws, lcnt, grads=grads), steps)
+    same = all(torch.equal(bufs[k][:n], lbufs[k][:n]) for k, n in
+               (("vertices", U), ("normals", U), ("conn", T), ("edges", U)))
+    same = bool(same and torch.equal(dldf, ldldf))
+    return {
+        "workload": f"{args.batch} volumes, per-axis point extents U{{32..160}} (seed 0), "
+                    f"4-16 Gaussians each, iso U[0.3,0.7], dedup, normals, edge list",
+        "volumes": args.batch, "points": P, "cells": C, "triangles": T, "vertices": U,
+        "graph_fwd_ms": round(ms_f, 4), "graph_fwd_bwd_ms": round(ms_fb, 4),
+        "loop_fwd_ms": round(ms_lf, 4), "loop_fwd_bwd_ms": round(ms_lfb, 4),
+        "speedup_fwd": round(ms_lf / ms_f, 3), "speedup_fwd_bwd": round(ms_lfb / ms_fb, 3),
+        "volumes_per_s_fwd": round(args.batch / (ms_f * 1e-3), 1),
+        "volumes_per_s_fwd_bwd": round(args.batch / (ms_fb * 1e-3), 1),
+        "Gcells_per_s_fwd": round(C / (ms_f * 1e-3) / 1e9, 3),
+        "loop_outputs_bit_identical": same, "steps": steps,
+    }
+
+
+def large(args):
+    import torch
+
+    from daxtoolkit_amd import fields
+    n = 1024
+    t0 = time.perf_counter()
+    field = fields.mandelbulb_torch(n)
+    torch.cuda.synchronize()
+    print(f"[bench] mandelbulb {n}^3 generated in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    a = argparse.Namespace(**vars(args))
+    a.steps = max(3, min(args.steps, 10))
+    r = single_volume(field, 0.5, "mandelbulb", a)
+    out = {"workload": "1024^3 Mandelbulb escape-time field (power 8, <=10 iterations, bailout 2, "
+                       "[-1.2,1.2]^3), iso 0.5, dedup, normals, edge list",
+           "points": r["P"], "cells": r["C"], "triangles": r["T"], "vertices": r["U"],
+           "fwd_ms": round(r["ms"], 4), "Gcells_per_s_fwd": round(r["C"] / (r["ms"] * 1e-3) / 1e9, 3),
+           "tris_per_s": round(r["T"] / (r["ms"] * 1e-3), 1),
+           "kernel_ms": [round(x, 4) for x in r["k_ms"]], "tuning": r["tuned"]}
+    if "fb_ms" in r:
+        out.update({"fwd_bwd_ms": round(r["fb_ms"], 4), "bwd_ms": round(r["bwd_ms"], 4),
+                    "fwd_bwd_over_fwd": round(r["fb_ms"] / r["ms"], 3)})
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -89,57 +261,27 @@ def main():
         sys.exit(2)
     import torch
 
-    from daxtoolkit_amd import fields, mc
+    from daxtoolkit_amd import fields
 
     torch.cuda.set_device(0)
     if args.copy_probe:
         copy_probe(args.steps)
         return
+    if args.only == "batch":
+        print(json.dumps({"batched": batched(args)}), flush=True)
+        return
+    if args.only == "large":
+        print(json.dumps({"large": large(args)}), flush=True)
+        return
     n = args.n
     dims = (n, n, n)
     iso = 0.5
-    P = n ** 3
-    C = (n - 1) ** 3
     params = fields.gaussian_mixture_params(0, dims)
     field = fields.gaussian_mixture_torch(dims, params)
     torch.cuda.synchronize()
-
-    ws, cnt = mc.classify(field, iso)
-    T, U = mc.read_counts(cnt)
-    del ws
-    buf = mc.ForwardBuffers(n, n, n, U, T, normals=True, edges=True)
-    if args.tuning:
-        tuned = [int(x) for x in args.tuning.split(",")]
-        mc.set_tuning((n, n, n), tuned)
-        tuned_ms = [0.0, 0.0, 0.0]
-    else:
-        tuned, tuned_ms = mc.autotune(field, iso, buf, reps=10)
-
-    for _ in range(args.warmup):
-        mc.forward_into(field, iso, buf)
-    torch.cuda.synchronize()
-    mc.read_counts(buf.counts, buf.cap_points, buf.cap_tris)
-
-    # timed region: K forwards
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    wall0 = time.perf_counter()
-    e0.record()
-    for _ in range(args.steps):
-        mc.forward_into(field, iso, buf)
-    e1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    ms = e0.elapsed_time(e1) / args.steps
-    T2, U2 = mc.read_counts(buf.counts, buf.cap_points, buf.cap_tris)
-    assert (T2, U2) == (T, U)
-
-    # per-kernel durations (same work, one kernel per call, events between)
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
-    for k in range(args.steps):
-        mc.forward_phases(field, iso, buf, events=ev[k])
-    torch.cuda.synchronize()
-    k_ms = [statistics.mean(e[i].elapsed_time(e[i + 1]) for e in ev) for i in range(3)]
+    tuning = [int(x) for x in args.tuning.split(",")] if args.tuning else None
+    r = single_volume(field, iso, f"{n}^3", args, tuning)
+    P, C, T, U, ms, k_ms, tuned = r["P"], r["C"], r["T"], r["U"], r["ms"], r["k_ms"], r["tuned"]
 
     b_classify = 4 * P + P // 8                      # field read once + inside-bits written
     b_fwd = 4 * P + 12 * U + 12 * U + 4 * 3 * T + 4 * U  # field + verts + normals + conn + edges
@@ -169,15 +311,24 @@ def main():
         "fwd_frac_hbm_peak": round(fwd_gbs / HBM_PEAK_GBS, 4),
         "kernel_ms": {"classify": round(k_ms[0], 5), "count_scan": round(k_ms[1], 5),
                       "generate": round(k_ms[2], 5)},
-        "wall_s": round(wall, 4),
+        "wall_s": round(r["wall"], 4),
         "tuning": {"classify_unroll": tuned[0], "classify_vec": tuned[1], "scan_sub": tuned[2],
-                   "gen_tile": tuned[3], "gen_xcd": (tuned + [1])[4],
-                   "autotune_ms": [round(x, 5) for x in tuned_ms]},
+                   "gen_tile": tuned[3], "gen_xcd": (list(tuned) + [0])[4],
+                   "autotune_ms": [round(x, 5) for x in r["tuned_ms"]]},
         "roofline": {"bound": "hbm", "kernel": "k_classify", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
     }
-    if not args.no_cpu_baseline:
+    if "fb_ms" in r:
+        line["fwd_bwd"] = {"ms": round(r["fb_ms"], 5), "bwd_ms": round(r["bwd_ms"], 5),
+                           "fwd_bwd_over_fwd": round(r["fb_ms"] / ms, 3),
+                           "Gcells_per_s": round(C / (r["fb_ms"] * 1e-3) / 1e9, 3)}
+    if args.only != "fwd":
+        if args.batch > 0:
+            line["batched"] = batched(args)
+        if args.large:
+            line["large"] = large(args)
+    if not args.no_cpu_baseline and args.only is None:
         f_host = field.cpu().numpy()
         line["cpu_baseline"] = cpu_baseline(f_host, iso, C, args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "mc_adjoint.h"
 #include "mc_kernels.h"
 
 namespace daxhip {
@@ -162,6 +163,45 @@ __global__ void k_batch_offsets(const VolGeom* __restrict__ vols, int64_t nvols,
   const bool dedup = (flags & DAX_HIP_MC_DEDUP) != 0;
   vol_offsets[2 * v + 0] = base_t[w];
   vol_offsets[2 * v + 1] = dedup ? (int64_t)base_v[w] : 3 * (int64_t)base_t[w];
+}
+
+// ---- batched adjoint --------------------------------------------------------
+// Same tile table as generate: a block takes a tile of row words, first the
+// adjoint records of the tile's vertices (global ids base_v[w0] ..
+// base_v[w_end]), then, in a second launch, dL/dfield of the tile's points.
+__global__ __launch_bounds__(256) void k_batch_vertex_adjoint(
+    const VolGeom* __restrict__ vols, const TileRef* __restrict__ tiles, int64_t ntiles,
+    const float* __restrict__ fields, const uint32_t* __restrict__ base_v,
+    const uint32_t* __restrict__ edges, const float* __restrict__ dldx,
+    const float* __restrict__ dldn, const int64_t* __restrict__ counts, int64_t n_points,
+    VertexAdj* __restrict__ out) {
+  const int64_t U = counts ? min(counts[1], n_points) : n_points;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const TileRef tr = tiles[t];
+    const VolGeom g = vols[tr.vol];
+    const int64_t w_end = min((int64_t)tr.w0 + kBatchGenTile, g.nwords);
+    const int64_t u0 = base_v[g.word_off + tr.w0];
+    const int64_t u1 = min((int64_t)base_v[g.word_off + w_end], U);
+    for (int64_t u = u0 + threadIdx.x; u < u1; u += blockDim.x)
+      vertex_adjoint(g, fields, edges, dldx, dldn, u, out);
+  }
+}
+
+__global__ __launch_bounds__(kGatherTile) void k_batch_point_gather(
+    const VolGeom* __restrict__ vols, const TileRef* __restrict__ tiles, int64_t ntiles,
+    const uint64_t* __restrict__ bits, const uint32_t* __restrict__ base_v,
+    const VertexAdj* __restrict__ adj, const int64_t* __restrict__ counts, int64_t n_points,
+    float* __restrict__ dldf) {
+  static_assert(kGatherTile == kBatchGenTile, "gather tiles are the generate tiles");
+  __shared__ GatherShared sh;
+  const int64_t n_adj = counts ? min(counts[1], n_points) : n_points;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const TileRef tr = tiles[t];
+    const VolGeom g = vols[tr.vol];
+    gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj, dldf + g.field_off, tr.w0,
+                min((int64_t)tr.w0 + kBatchGenTile, g.nwords), sh);
+    __syncthreads();  // LDS reused by the next tile
+  }
 }
 
 }  // namespace mc
@@ -316,6 +356,124 @@ int dax_hip_mc_batch_forward(const float* d_fields, const void* h_header, void* 
     k_batch_offsets<<<dim3((unsigned)ceil_div(h->nvols, 256)), dim3(256), 0, s>>>(
         vols, h->nvols, bv, bt, flags, d_vol_offsets);
     DAXHIP_LAUNCH_CHECK("k_batch_offsets");
+  }
+  return DAX_HIP_OK;
+}
+
+// Batched adjoint: dL/dfield of every volume (at the fields' offsets) from
+// dL/dvertex and dL/dnormal of the concatenated outputs of the last batch
+// forward on this workspace.  d_scratch: dax_hip_mc_backward_scratch_bytes
+// (n_points).
+int dax_hip_mc_batch_backward(const float* d_fields, const void* h_header,
+                              const void* d_workspace, size_t workspace_bytes,
+                              const uint32_t* d_edges, const int64_t* d_counts, int64_t n_points,
+                              const float* d_dl_dvertices, const float* d_dl_dnormals,
+                              void* d_scratch, size_t scratch_bytes, float* d_dl_dfields,
+                              void* stream) {
+  const BatchHeader* h = (const BatchHeader*)h_header;
+  if (!h || h->magic != kBatchMagic || !d_workspace ||
+      workspace_bytes < kBatchHeaderBytes + h->bytes) {
+    set_error("dax_hip_mc_batch_backward: header/workspace not prepared");
+    return DAX_HIP_BAD_VALUE;
+  }
+  if (n_points < 0 || scratch_bytes < (size_t)n_points * sizeof(VertexAdj) ||
+      (n_points > 0 && (!d_scratch || !d_edges || !d_dl_dvertices || !d_dl_dnormals)) ||
+      !d_fields || !d_dl_dfields) {
+    set_error("dax_hip_mc_batch_backward: missing buffers or scratch smaller than %zu bytes",
+              (size_t)n_points * sizeof(VertexAdj));
+    return DAX_HIP_BAD_VALUE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const char* body = (const char*)d_workspace + kBatchHeaderBytes;
+  const VolGeom* vols = (const VolGeom*)(body + h->off_vols);
+  const TileRef* gen_tiles = (const TileRef*)(body + h->off_gen_tiles);
+  const uint64_t* bits = (const uint64_t*)(body + h->off_bits);
+  const uint32_t* bv = (const uint32_t*)(body + h->off_base_v);
+  const int grid = (int)std::min<int64_t>(persistent_blocks(4), h->ngen);
+  if (n_points > 0) {
+    k_batch_vertex_adjoint<<<dim3(grid), dim3(256), 0, s>>>(vols, gen_tiles, h->ngen, d_fields, bv,
+                                                            d_edges, d_dl_dvertices, d_dl_dnormals,
+                                                            d_counts, n_points,
+                                                            (VertexAdj*)d_scratch);
+    DAXHIP_LAUNCH_CHECK("k_batch_vertex_adjoint");
+  }
+  k_batch_point_gather<<<dim3(grid), dim3(kGatherTile), 0, s>>>(vols, gen_tiles, h->ngen, bits, bv,
+                                                        (const VertexAdj*)d_scratch, d_counts,
+                                                        n_points, d_dl_dfields);
+  DAXHIP_LAUNCH_CHECK("k_batch_point_gather");
+  return DAX_HIP_OK;
+}
+
+// Capture of any sequence of this library's launches on `stream` (e.g. a
+// batch forward followed by its backward) into one executable graph.
+int dax_hip_graph_begin_capture(void* stream) {
+  if (!stream) {
+    set_error("dax_hip_graph_begin_capture: capture needs a non-null stream");
+    return DAX_HIP_BAD_VALUE;
+  }
+  DAXHIP_CHECK(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+  return DAX_HIP_OK;
+}
+
+int dax_hip_graph_end_capture(void* stream, void** h_graph_exec) {
+  hipGraph_t graph = nullptr;
+  DAXHIP_CHECK(hipStreamEndCapture((hipStream_t)stream, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return check_hip(e, "hipGraphInstantiate");
+  *h_graph_exec = exec;
+  return DAX_HIP_OK;
+}
+
+// Baseline of the batched path: the same volumes, one single-volume forward
+// (three launches) and, when upstream gradients are given, one single-volume
+// backward (two launches) per volume, written at the offsets of the batched
+// outputs (h_vol_offsets, as the batched forward reports them).
+int dax_hip_mc_volume_loop(int64_t n_volumes, const int64_t* h_dims3,
+                           const int64_t* h_field_offsets, const float* h_isos,
+                           const float* h_origins3, const float* h_spacings3,
+                           const float* d_fields, unsigned flags, void* d_workspace,
+                           size_t workspace_bytes, const int64_t* h_vol_offsets,
+                           int64_t cap_points, int64_t cap_tris, float* d_vertices,
+                           float* d_normals, int32_t* d_conn, uint32_t* d_edges,
+                           int64_t* d_counts_scratch, const float* d_dl_dvertices,
+                           const float* d_dl_dnormals, void* d_scratch, size_t scratch_bytes,
+                           float* d_dl_dfields, void* stream) {
+  if (n_volumes <= 0 || !h_dims3 || !h_field_offsets || !h_isos || !h_vol_offsets ||
+      !d_counts_scratch) {
+    set_error("dax_hip_mc_volume_loop: bad arguments");
+    return DAX_HIP_BAD_VALUE;
+  }
+  const bool backward = d_dl_dfields != nullptr;
+  for (int64_t v = 0; v < n_volumes; ++v) {
+    const int64_t t0 = h_vol_offsets[2 * v], p0 = h_vol_offsets[2 * v + 1];
+    const int64_t t1 = v + 1 < n_volumes ? h_vol_offsets[2 * v + 2] : cap_tris;
+    const int64_t p1 = v + 1 < n_volumes ? h_vol_offsets[2 * v + 3] : cap_points;
+    if (t0 < 0 || p0 < 0 || t1 < t0 || p1 < p0 || t1 > cap_tris || p1 > cap_points) {
+      set_error("dax_hip_mc_volume_loop: volume %lld offsets outside the outputs", (long long)v);
+      return DAX_HIP_BAD_VALUE;
+    }
+    const int64_t* d = h_dims3 + 3 * v;
+    const float* o = h_origins3 ? h_origins3 + 3 * v : nullptr;
+    const float* sp = h_spacings3 ? h_spacings3 + 3 * v : nullptr;
+    const float* f = d_fields + h_field_offsets[v];
+    int rc = dax_hip_mc_forward(f, d[0], d[1], d[2], o, sp, h_isos[v], flags, d_workspace,
+                                workspace_bytes, d_vertices + 3 * p0,
+                                d_normals ? d_normals + 3 * p0 : nullptr, d_conn + 3 * t0,
+                                d_edges ? d_edges + p0 : nullptr, nullptr, nullptr, nullptr,
+                                p1 - p0, t1 - t0, d_counts_scratch, stream);
+    if (rc) return rc;
+    if (backward) {
+      rc = dax_hip_mc_backward(f, d[0], d[1], d[2], o, sp, h_isos[v], d_workspace,
+                               workspace_bytes, d_edges ? d_edges + p0 : nullptr,
+                               d_counts_scratch, p1 - p0,
+                               d_dl_dvertices ? d_dl_dvertices + 3 * p0 : nullptr,
+                               d_dl_dnormals ? d_dl_dnormals + 3 * p0 : nullptr,
+                               d_scratch, scratch_bytes,
+                               d_dl_dfields + h_field_offsets[v], stream);
+      if (rc) return rc;
+    }
   }
   return DAX_HIP_OK;
 }

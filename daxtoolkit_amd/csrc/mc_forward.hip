@@ -759,3 +759,4 @@ int dax_hip_mc_autotune(const float* d_field, int64_t nx, int64_t ny, int64_t nz
 }
 
 }  // extern "C"
+

@@ -131,3 +131,39 @@ def mandelbulb_torch(n: int, power: float = 8.0, max_iter: int = 10, bailout: fl
             zz = torch.where(alive, nzz, zz)
         out[z0:z1] = (it / max_iter).to(torch.float32)
     return out
+
+
+@dataclass
+class VolumeBatch:
+    dims: list            # [(nx, ny, nz)]
+    isos: list            # per-volume isovalue
+    params: list          # per-volume GaussianMixture
+
+
+def volume_batch(n: int, lo: int = 32, hi: int = 160, seed: int = 0, gauss=(4, 16),
+                 sigma=(20.0, 60.0), iso=(0.3, 0.7)) -> VolumeBatch:
+    """n independent volumes: per-axis point extents ~ U{lo..hi}, each a
+    seeded mixture of U{gauss} Gaussians (sigma ~ U[sigma] voxels, amplitude
+    ~ U[0.5,1.5], noise U(-0.01,0.01)), isovalue ~ U[iso]."""
+    rng = np.random.default_rng(seed)
+    dims, isos, params = [], [], []
+    for v in range(n):
+        d = tuple(int(x) for x in rng.integers(lo, hi + 1, size=3))
+        g = int(rng.integers(gauss[0], gauss[1] + 1))
+        dims.append(d)
+        isos.append(float(np.float32(rng.uniform(iso[0], iso[1]))))
+        params.append(gaussian_mixture_params(int(rng.integers(0, 2**31 - 1)), d, n_gauss=g,
+                                              sigma=sigma))
+    return VolumeBatch(dims, isos, params)
+
+
+def volume_batch_torch(b: VolumeBatch, device="cuda"):
+    """The batch's fields back to back in one flat float32 device tensor."""
+    import torch
+    sizes = [d[0] * d[1] * d[2] for d in b.dims]
+    flat = torch.empty(sum(sizes), dtype=torch.float32, device=device)
+    off = 0
+    for d, p, s in zip(b.dims, b.params, sizes):
+        flat[off:off + s] = gaussian_mixture_torch(d, p, device).reshape(-1)
+        off += s
+    return flat

@@ -219,6 +219,22 @@ def backward(field: torch.Tensor, iso: float, contour: Contour, dl_dvertices: to
     return out
 
 
+def backward_into(field: torch.Tensor, iso: float, buf: ForwardBuffers,
+                  dl_dvertices: torch.Tensor, dl_dnormals: torch.Tensor, scratch: torch.Tensor,
+                  out: torch.Tensor, origin=(0, 0, 0), spacing=(1, 1, 1)) -> None:
+    """Enqueue the backward of the last :func:`forward_into` on `buf` (run
+    with edges) with preallocated scratch and output (no synchronisation).
+    Upstream gradients cover at least the forward's U points."""
+    nx, ny, nz = _check_field(field)
+    if buf.edges is None:
+        raise ValueError("backward needs forward buffers with edges")
+    _lib.call("dax_hip_mc_backward", field.data_ptr(), nx, ny, nz, _f3(origin), _f3(spacing),
+              float(iso), buf.workspace.data_ptr(), buf.workspace.numel(), buf.edges.data_ptr(),
+              buf.counts.data_ptr(), buf.cap_points, dl_dvertices.data_ptr(),
+              dl_dnormals.data_ptr(), scratch.data_ptr(), scratch.numel(), out.data_ptr(),
+              _stream())
+
+
 def autotune(field: torch.Tensor, iso: float, buf: ForwardBuffers, reps: int = 10,
              dedup: bool = True, normals: bool = True):
     """Run the native autotuner for this shape; returns (params5, ms3)."""
@@ -359,6 +375,52 @@ class Batch:
     def destroy(graph):
         _lib.call("dax_hip_graph_destroy", graph)
 
+    def scratch_bytes(self, n_points: int) -> int:
+        return int(_lib.load().dax_hip_mc_backward_scratch_bytes(int(n_points)))
+
+    def backward(self, fields, bufs, dl_dvertices, dl_dnormals, scratch, out):
+        """dL/dfields (flat, like `fields`) of the last batch forward into
+        `bufs` (run with edges) for upstream [U, 3] gradients."""
+        U = bufs["cap_points"]
+        for t in (dl_dvertices, dl_dnormals):
+            if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() < 3 * U:
+                raise ValueError("upstream gradients must be contiguous float32 [U, 3]")
+        if bufs.get("edges") is None:
+            raise ValueError("the batch backward needs a forward with the edge list")
+        _lib.call("dax_hip_mc_batch_backward", fields.data_ptr(), self.header,
+                  self.workspace.data_ptr(), self.workspace.numel(), bufs["edges"].data_ptr(),
+                  self.counts.data_ptr(), U, dl_dvertices.data_ptr(), dl_dnormals.data_ptr(),
+                  scratch.data_ptr(), scratch.numel(), out.data_ptr(), _stream())
+
+    def loop_workspace(self):
+        lib = _lib.load()
+        need = max(int(lib.dax_hip_mc_workspace_bytes(*map(int, d))) for d in self.dims)
+        return torch.empty(need, dtype=torch.uint8, device=self.device)
+
+    def loop(self, fields, bufs, vol_offsets_host, workspace, counts_scratch, dedup=True,
+             normals=True, grads=None):
+        """Baseline: one single-volume forward (and, with grads =
+        (dl_dvertices, dl_dnormals, scratch, dl_dfields), one single-volume
+        backward) per volume, as a native launch loop, into the same
+        concatenated outputs."""
+        import ctypes
+
+        import numpy as np
+        p = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+        offs = np.ascontiguousarray(vol_offsets_host, dtype=np.int64).reshape(self.n, 2)
+        flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and bufs.get("normals") is not None) else 0)
+        dv = dn = sc = df = None
+        sc_bytes = 0
+        if grads is not None:
+            dv, dn, scr, dfo = grads
+            dv, dn, sc, df, sc_bytes = dv.data_ptr(), dn.data_ptr(), scr.data_ptr(), dfo.data_ptr(), scr.numel()
+        _lib.call("dax_hip_mc_volume_loop", self.n, p(self.dims), p(self.field_offsets),
+                  p(self.isos), p(self.origins), p(self.spacings), fields.data_ptr(), flags,
+                  workspace.data_ptr(), workspace.numel(), p(offs), bufs["cap_points"],
+                  bufs["cap_tris"], bufs["vertices"].data_ptr(), _ptr(bufs.get("normals")),
+                  bufs["conn"].data_ptr(), _ptr(bufs.get("edges")) if dedup else None,
+                  counts_scratch.data_ptr(), dv, dn, sc, sc_bytes, df, _stream())
+
     def split(self, bufs, dedup=True):
         """Per-volume views (vertices, normals, connectivity) of the outputs."""
         T, U = read_counts(self.counts, bufs["cap_points"], bufs["cap_tris"])
@@ -372,3 +434,19 @@ class Batch:
                         None if bufs.get("normals") is None else bufs["normals"][p0:p1],
                         bufs["conn"][t0:t1]))
         return out
+
+
+def capture(fn, stream: torch.cuda.Stream):
+    """Capture every launch `fn()` makes through this library on `stream`
+    (a non-default torch stream) into one executable graph; returns its
+    handle for :meth:`Batch.launch` / :meth:`Batch.destroy`.  `fn` must not
+    allocate or synchronise."""
+    import ctypes
+    h = ctypes.c_void_p()
+    with torch.cuda.stream(stream):
+        _lib.call("dax_hip_graph_begin_capture", stream.cuda_stream)
+        try:
+            fn()
+        finally:
+            _lib.call("dax_hip_graph_end_capture", stream.cuda_stream, ctypes.byref(h))
+    return h

@@ -296,6 +296,39 @@ int dax_hip_mc_batch_graph_create(const float* d_fields, const void* h_header, v
                                   float* d_normals, int32_t* d_conn, uint32_t* d_edges,
                                   int64_t cap_points, int64_t cap_tris, int64_t* d_counts,
                                   int64_t* d_vol_offsets, void* stream, void** h_graph_exec);
+/* Batched adjoint: dL/dfield of every volume (written at the fields'
+ * offsets) from dL/dvertex and dL/dnormal [U][3] of the concatenated outputs
+ * of the last dax_hip_mc_batch_forward on this workspace (run with the edge
+ * list).  d_scratch holds dax_hip_mc_backward_scratch_bytes(n_points).
+ * No reference counterpart. */
+int dax_hip_mc_batch_backward(const float* d_fields, const void* h_header,
+                              const void* d_workspace, size_t workspace_bytes,
+                              const uint32_t* d_edges, const int64_t* d_counts, int64_t n_points,
+                              const float* d_dl_dvertices, const float* d_dl_dnormals,
+                              void* d_scratch, size_t scratch_bytes, float* d_dl_dfields,
+                              void* stream);
+
+/* Baseline of the batched path: the same volumes as a loop of
+ * dax_hip_mc_forward (and, when d_dl_dfields is non-null,
+ * dax_hip_mc_backward) calls, one per volume, into the concatenated outputs
+ * at h_vol_offsets [n][2] (as the batched forward reports them).
+ * d_workspace holds the largest volume's dax_hip_mc_workspace_bytes;
+ * d_counts_scratch holds 2 int64.  No reference counterpart. */
+int dax_hip_mc_volume_loop(int64_t n_volumes, const int64_t* h_dims3,
+                           const int64_t* h_field_offsets, const float* h_isos,
+                           const float* h_origins3, const float* h_spacings3,
+                           const float* d_fields, unsigned flags, void* d_workspace,
+                           size_t workspace_bytes, const int64_t* h_vol_offsets,
+                           int64_t cap_points, int64_t cap_tris, float* d_vertices,
+                           float* d_normals, int32_t* d_conn, uint32_t* d_edges,
+                           int64_t* d_counts_scratch, const float* d_dl_dvertices,
+                           const float* d_dl_dnormals, void* d_scratch, size_t scratch_bytes,
+                           float* d_dl_dfields, void* stream);
+
+/* Capture every launch this library makes on `stream` between the two calls
+ * into one executable graph (e.g. a batch forward and its backward). */
+int dax_hip_graph_begin_capture(void* stream);
+int dax_hip_graph_end_capture(void* stream, void** h_graph_exec);
 int dax_hip_graph_launch(void* graph_exec, void* stream);
 int dax_hip_graph_destroy(void* graph_exec);
 
