@@ -72,7 +72,10 @@ __device__ __forceinline__ void vertex_adjoint(const VolGeom& g, const float* __
   const int axis = (int)(e >> 30);
   const int64_t A = (int64_t)(e & 0x3fffffffu);
   const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-  const int x = (int)(A % g.nx), y = (int)((A / g.nx) % g.ny), z = (int)(A / sz);
+  const uint32_t A32 = (uint32_t)A, row = fast_div(A32, g.div_nx);  // A < 2^30
+  const uint32_t zz = fast_div(row, g.div_ny);
+  const int x = (int)(A32 - row * (uint32_t)g.nx), y = (int)(row - zz * (uint32_t)g.ny),
+            z = (int)zz;
   // an edge code outside the grid (edge list not written by a forward that
   // fit its outputs) contributes nothing and is never dereferenced
   if (axis > 2 || A >= sz * g.nz || x + (axis == 0) >= g.nx || y + (axis == 1) >= g.ny ||
@@ -126,100 +129,111 @@ __device__ __forceinline__ void vertex_adjoint(const VolGeom& g, const float* __
 // ---- per-point gather ------------------------------------------------------
 // A point receives a non-zero dL/df only within L1 distance 1 of an endpoint
 // of an active edge (its own t-terms and the central-difference stencils of
-// the endpoint gradients).  The gather of one point reads the inside-bits and
-// vertex bases of the volume from global memory (they stay in L2 / MALL).
-struct GatherCtx {
-  const VolGeom* g;
-  const uint64_t* bits;    // volume's inside-bits
-  const uint32_t* base_v;  // volume's word bases (global vertex ids)
-  const VertexAdj* adj;
-  int64_t n_adj;           // records in adj
-  int x, j, k;
+// the endpoint gradients).  A gather tile of up to kGatherTile consecutive row
+// words stages, for the planes k-2..k+1 around it, the active-edge flags
+// (fx, fy, fz: the same flags the forward numbers its vertices with) and the
+// vertex bases of the words within two rows (and one word) of the tile:
+// segment dz holds words [w0 + dz*plane - 2*wpr - 1, w_end + dz*plane +
+// 2*wpr + 1).  Every lower endpoint of an edge a tile point reads lies at
+// |dq| <= 1, -2 <= dy, dz <= 1 from its own word, inside its segment.
+constexpr int kGatherTile = 256;     // row words per tile
+constexpr int kGatherThreads = 512;  // threads per tile
+// Endpoint records a tile keeps (2 per thread, held in registers until the
+// flags are no longer needed, then written over them).
+constexpr int kRecPerThread = 2;
+constexpr int kMaxEndpoints = kRecPerThread * kGatherThreads;
+
+__host__ __device__ inline int gather_seg(int wpr) { return kGatherTile + 4 * wpr + 2; }
+// G-words: the tile's words with one row above and below in its plane, and
+// the tile's words in the planes below and above.
+__host__ __device__ inline int gather_gwords(int nw, int wpr) { return 3 * nw + 2 * wpr; }
+__host__ __device__ inline size_t gather_lds_bytes(int wpr) {
+  const size_t flags = (size_t)4 * gather_seg(wpr) * (3 * sizeof(uint64_t) + sizeof(uint32_t));
+  const size_t ng = (size_t)gather_gwords(kGatherTile, wpr);
+  return flags + ng * sizeof(uint64_t) + (ng + 1) * sizeof(uint32_t);
+}
+
+struct Stage {
+  const uint64_t* fx;  // LDS [4][seg] each
+  const uint64_t* fy;
+  const uint64_t* fz;
+  const uint32_t* bv;
+  int seg;
+  int64_t w0;          // tile's first word
 };
 
-__device__ __forceinline__ uint64_t row_word(const GatherCtx& c, int qq, int jj, int kk) {
+// A point's view of the staged words: its word at (q, j, k) lies dzp planes
+// from the tile's plane, at tile-plane word w (its own word minus dzp*plane).
+struct GatherCtx {
+  const VolGeom* g;
+  const Stage* st;
+  const VertexAdj* adj;
+  int64_t n_adj;  // records in adj
+  int64_t w;
+  int dzp;
+  int q, x, j, k;
+
+  // LDS index of word (q+dq, j+dy, k+dz); dzp + dz in -2..1
+  __device__ __forceinline__ int idx(int dq, int dy, int dz) const {
+    return (dzp + dz + 2) * st->seg + (int)(w - st->w0) + dq + dy * g->wpr + 2 * g->wpr + 1;
+  }
+};
+
+// Vertex id of the edge with lower endpoint (x,j,k) + (dx,dy,dz) along
+// `axis`, or -1 if that edge is not active (LDS only).
+__device__ __forceinline__ int64_t edge_vid(const GatherCtx& c, int dx, int dy, int dz, int axis) {
   const VolGeom& g = *c.g;
-  if (qq < 0 || qq >= g.wpr || jj < 0 || jj >= g.ny || kk < 0 || kk >= g.nz) return 0ull;
-  return c.bits[(int64_t)qq + (int64_t)g.wpr * (jj + (int64_t)g.ny * kk)];
+  const int xl = c.x + dx, jl = c.j + dy, kl = c.k + dz;
+  if (xl < 0 || xl >= g.nx || jl < 0 || jl >= g.ny || kl < 0 || kl >= g.nz) return -1;
+  const int qq = xl >> 6, bb = xl & 63;
+  const int i = c.idx(qq - c.q, dy, dz);
+  const uint64_t fa = (axis == 0 ? c.st->fx : (axis == 1 ? c.st->fy : c.st->fz))[i];
+  if (!((fa >> bb) & 1ull)) return -1;
+  const uint64_t fx = axis == 0 ? fa : c.st->fx[i];
+  const uint64_t fy = axis == 1 ? fa : c.st->fy[i];
+  const uint64_t fz = axis == 2 ? fa : c.st->fz[i];
+  const int64_t vid = (int64_t)c.st->bv[i] + edge_rank(fx, fy, fz, bb, axis);
+  return vid < c.n_adj ? vid : -1;
 }
 
-// Does point (x,j,k) + (dx, dy, dz) exist?
-__device__ __forceinline__ bool exists(const GatherCtx& c, int dx, int dy, int dz) {
-  const int x = c.x + dx, y = c.j + dy, z = c.k + dz;
-  return x >= 0 && x < c.g->nx && y >= 0 && y < c.g->ny && z >= 0 && z < c.g->nz;
-}
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ int bit_at(const GatherCtx& c, int dx, int dy, int dz) {
-  const int x = c.x + dx;
-  return (int)((row_word(c, x >> 6, c.j + dy, c.k + dz) >> (x & 63)) & 1ull);
-}
-
-// Vertex record of the edge with lower endpoint (x,j,k) + (dx,dy,dz) along
-// `axis`, or nullptr if that edge is not active.
-__device__ __forceinline__ const VertexAdj* edge_vertex(const GatherCtx& c, int dx, int dy, int dz,
-                                                        int axis) {
-  const int ex = axis == 0, ey = axis == 1, ez = axis == 2;
-  if (!exists(c, dx, dy, dz) || !exists(c, dx + ex, dy + ey, dz + ez)) return nullptr;
-  if (bit_at(c, dx, dy, dz) == bit_at(c, dx + ex, dy + ey, dz + ez)) return nullptr;
-  // rank of the edge within its row word
-  const VolGeom& g = *c.g;
-  const int xg = c.x + dx;
-  const int qq = xg >> 6, bb = xg & 63;
-  WordPos p;
-  p.q = qq;
-  p.j = c.j + dy;
-  p.k = c.k + dz;
-  p.has_y = p.j + 1 < g.ny;
-  p.has_z = p.k + 1 < g.nz;
-  p.pmask = low_mask(g.nx - qq * 64);
-  p.xmask = low_mask(g.nx - 1 - qq * 64);
-  const uint64_t A = row_word(c, qq, p.j, p.k);
-  const uint64_t A1 = row_word(c, qq + 1, p.j, p.k);
-  const uint64_t B = row_word(c, qq, p.j + 1, p.k);
-  const uint64_t C = row_word(c, qq, p.j, p.k + 1);
-  uint64_t fx, fy, fz;
-  edge_flags(p, A, A1, B, C, fx, fy, fz);
-  const int64_t w = (int64_t)qq + (int64_t)g.wpr * (p.j + (int64_t)g.ny * p.k);
-  const uint32_t vid = c.base_v[w] + (uint32_t)edge_rank(fx, fy, fz, bb, axis);
-  return (int64_t)vid < c.n_adj ? c.adj + vid : nullptr;
-}
-
-// dL/dg_p (3 components) and the t-path sum S_p at p = (x,j,k) + (dx,dy,dz):
-// over the 3 edges leaving p and the 3 edges entering p.
-__device__ __forceinline__ void point_sums(const GatherCtx& c, int dx, int dy, int dz, float G[3],
-                                           float& S) {
-  G[0] = G[1] = G[2] = 0.0f;
-  S = 0.0f;
+// (S_p, dL/dg_p) at p = (x,j,k) + (dx,dy,dz): over the 3 edges leaving p
+// (lower halves of their records) and the 3 edges entering p (upper halves),
+// summed in that order.  The six record loads are issued together.
+__device__ __forceinline__ f32x4 point_sums(const GatherCtx& c, int dx, int dy, int dz) {
+  int64_t vid[6];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const VertexAdj* v = edge_vertex(c, dx, dy, dz, a);
-    if (v) {
-      S += v->sA;
-      G[0] += v->gA[0];
-      G[1] += v->gA[1];
-      G[2] += v->gA[2];
-    }
+    vid[a] = edge_vid(c, dx, dy, dz, a);
+    vid[3 + a] = edge_vid(c, dx - (a == 0), dy - (a == 1), dz - (a == 2), a);
   }
+  const f32x4* rec = reinterpret_cast<const f32x4*>(c.adj);
+  f32x4 r[6];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const VertexAdj* v =
-        edge_vertex(c, dx - (a == 0), dy - (a == 1), dz - (a == 2), a);
-    if (v) {
-      S += v->sB;
-      G[0] += v->gB[0];
-      G[1] += v->gB[1];
-      G[2] += v->gB[2];
+  for (int i = 0; i < 6; ++i)
+    r[i] = vid[i] >= 0 ? rec[2 * vid[i] + (i < 3 ? 0 : 1)] : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  f32x4 acc{0.0f, 0.0f, 0.0f, 0.0f};  // S, G[0..2]
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    if (vid[i] >= 0) {
+      acc.x += r[i].x;
+      acc.y += r[i].y;
+      acc.z += r[i].z;
+      acc.w += r[i].w;
     }
   }
+  return acc;
 }
 
-// dL/df of the point (x, j, k).
-__device__ __forceinline__ float point_adjoint(const GatherCtx& c) {
-  const VolGeom& g = *c.g;
-  float G[3], S;
-  point_sums(c, 0, 0, 0, G, S);
-  float acc = S;
-  const int co[3] = {c.x, c.j, c.k};
+// dL/df of the point q = (x, j, k) from (S, G) of q and of its 6 neighbours:
+// at(d, s) returns (S_p, G_p) of p = q + s e_d (s = 0: q itself).
+template <class At>
+__device__ __forceinline__ float combine_point(const VolGeom& g, int x, int j, int k, At at) {
+  const f32x4 own = at(0, 0);
+  const float G[3] = {own.y, own.z, own.w};
+  float acc = own.x;
+  const int co[3] = {x, j, k};
   const int n[3] = {g.nx, g.ny, g.nz};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -235,92 +249,295 @@ __device__ __forceinline__ float point_adjoint(const GatherCtx& c) {
   for (int d = 0; d < 3; ++d) {
     const float h = g.spacing[d];
     if (n[d] < 2) continue;
-    const int ex = d == 0, ey = d == 1, ez = d == 2;
-    float Gn[3], Sn;
     if (co[d] >= 1) {  // p = q - e_d: f_q = f(p + e_d)
-      point_sums(c, -ex, -ey, -ez, Gn, Sn);
-      acc += (co[d] - 1 == 0) ? Gn[d] / h : Gn[d] / (2.0f * h);
+      const f32x4 r = at(d, -1);
+      const float Gd = d == 0 ? r.y : (d == 1 ? r.z : r.w);
+      acc += (co[d] - 1 == 0) ? Gd / h : Gd / (2.0f * h);
     }
     if (co[d] <= n[d] - 2) {  // p = q + e_d: f_q = f(p - e_d)
-      point_sums(c, ex, ey, ez, Gn, Sn);
-      acc += (co[d] + 1 == n[d] - 1) ? -(Gn[d] / h) : -(Gn[d] / (2.0f * h));
+      const f32x4 r = at(d, 1);
+      const float Gd = d == 0 ? r.y : (d == 1 ? r.z : r.w);
+      acc += (co[d] + 1 == n[d] - 1) ? -(Gd / h) : -(Gd / (2.0f * h));
     }
   }
   return acc;
 }
 
-// Endpoint mask of row word (qq, jj, kk): points with an active incident
-// edge (0 outside the grid and in grids without cells).
-__device__ __forceinline__ uint64_t endpoint_mask(const VolGeom& g, const uint64_t* __restrict__ b,
-                                                  int qq, int jj, int kk) {
+// dL/df of the point (x, j, k), every sum from the staged flags.
+__device__ __forceinline__ float point_adjoint(const GatherCtx& c) {
+  return combine_point(*c.g, c.x, c.j, c.k, [&](int d, int s) {
+    return point_sums(c, d == 0 ? s : 0, d == 1 ? s : 0, d == 2 ? s : 0);
+  });
+}
+
+// Endpoint mask of row word (q+dq, j+dy, k+dz) of the context's word
+// (dzp + dz in -1..1): points with an active incident edge, from the staged
+// flags (0 outside the grid; the flags are 0 in grids without cells).
+__device__ __forceinline__ uint64_t endpoint_mask(const GatherCtx& c, int dq, int dy, int dz) {
+  const VolGeom& g = *c.g;
+  const int qq = c.q + dq, jj = c.j + dy, kk = c.k + dz;
   if (qq < 0 || qq >= g.wpr || jj < 0 || jj >= g.ny || kk < 0 || kk >= g.nz) return 0ull;
-  if (!(g.nx > 1 && g.ny > 1 && g.nz > 1)) return 0ull;
-  const int64_t plane = (int64_t)g.wpr * g.ny;
-  const int64_t w = (int64_t)qq + (int64_t)g.wpr * jj + plane * kk;
-  const uint64_t a = b[w];
-  const uint64_t an = qq + 1 < g.wpr ? b[w + 1] : 0ull;
-  const int x0 = qq * 64;
-  const uint64_t fx = (a ^ shr1(a, an)) & low_mask(g.nx - 1 - x0);  // x -> x+1
-  uint64_t e = fx | (fx << 1);
-  if (qq > 0) e |= ((b[w - 1] >> 63) ^ a) & 1ull;  // edge entering bit 0
-  if (jj + 1 < g.ny) e |= a ^ b[w + g.wpr];
-  if (jj >= 1) e |= a ^ b[w - g.wpr];
-  if (kk + 1 < g.nz) e |= a ^ b[w + plane];
-  if (kk >= 1) e |= a ^ b[w - plane];
-  return e & low_mask(g.nx - x0);
+  const Stage& st = *c.st;
+  const int i = c.idx(dq, dy, dz);
+  const uint64_t fx = st.fx[i];
+  uint64_t e = fx | (fx << 1) | st.fy[i] | st.fz[i];
+  if (qq > 0) e |= st.fx[i - 1] >> 63;  // x-edge entering bit 0
+  if (jj > 0) e |= st.fy[i - g.wpr];     // y-edges entering the row
+  if (kk > 0) e |= st.fz[i - st.seg];    // z-edges entering the row
+  return e;
 }
 
-// Points of word p within L1 distance 1 of an endpoint.
-__device__ __forceinline__ uint64_t receiving_mask(const VolGeom& g, const uint64_t* __restrict__ b,
-                                                   const WordPos& p) {
-  const uint64_t e = endpoint_mask(g, b, p.q, p.j, p.k);
+// Points of the context's word within L1 distance 1 of an endpoint.
+__device__ __forceinline__ uint64_t receiving_mask(const GatherCtx& c) {
+  const uint64_t e = endpoint_mask(c, 0, 0, 0);
   uint64_t r = e | (e << 1) | (e >> 1);
-  r |= endpoint_mask(g, b, p.q - 1, p.j, p.k) >> 63;
-  r |= endpoint_mask(g, b, p.q + 1, p.j, p.k) << 63;
-  r |= endpoint_mask(g, b, p.q, p.j - 1, p.k) | endpoint_mask(g, b, p.q, p.j + 1, p.k);
-  r |= endpoint_mask(g, b, p.q, p.j, p.k - 1) | endpoint_mask(g, b, p.q, p.j, p.k + 1);
-  return r & low_mask(g.nx - p.q * 64);
+  r |= endpoint_mask(c, -1, 0, 0) >> 63;
+  r |= endpoint_mask(c, 1, 0, 0) << 63;
+  r |= endpoint_mask(c, 0, -1, 0) | endpoint_mask(c, 0, 1, 0);
+  r |= endpoint_mask(c, 0, 0, -1) | endpoint_mask(c, 0, 0, 1);
+  return r & low_mask(c.g->nx - c.q * 64);
 }
 
-// LDS of one gather tile of up to kGatherTile words.
-constexpr int kGatherTile = 256;
 struct GatherShared {
   uint64_t recv[kGatherTile];
   uint32_t first[kGatherTile + 1];  // exclusive scan of popcount(recv)
   uint32_t scan[16];
 };
 
+// Staging of one flag slot: the word's inside-bits and those it needs.
+struct SlotLoad {
+  uint64_t A, A1, B, C;
+  uint32_t bv;
+  WordPos p;
+  bool in;
+};
+
 // dL/df of the points of words [w0, w_end) of a volume (w_end - w0 <=
-// kGatherTile, blockDim.x == kGatherTile): receiving masks (thread per word),
-// zeros for the other points (a wave per word, coalesced), then one thread per
-// receiving point.  Every thread of the block calls this.
+// kGatherTile, blockDim.x == kGatherThreads).  Every thread of the block
+// calls this; `lds` holds gather_lds_bytes(g.wpr).
+//   1. stage flags and vertex bases of the planes k-2..k+1
+//   2. receiving masks of the tile's words (thread per word)
+//   3. endpoint masks of the G-words, restricted to the points some
+//      receiving point's stencil reads; block scan -> compact endpoint slots
+//   4. (S, G) of every such endpoint (thread per endpoint, 6 edge records)
+//   5. zeros for the other points (coalesced) and dL/df of every receiving
+//      point from the 7 (S, G) of its stencil; no barrier follows, so these
+//      stores drain while the next tiles run
+// A tile with more than kMaxEndpoints endpoints computes every receiving
+// point's stencil sums directly from the flags instead of 3-4.
 __device__ __forceinline__ void gather_tile(const VolGeom& g, const uint64_t* __restrict__ b,
                                             const uint32_t* __restrict__ base_v,
                                             const VertexAdj* __restrict__ adj, int64_t n_adj,
                                             float* __restrict__ out, int64_t w0, int64_t w_end,
-                                            GatherShared& sh) {
+                                            GatherShared& sh, void* lds) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int nw = (int)(w_end - w0);
+  const int wpr = g.wpr;
+  const int seg = gather_seg(wpr);
+  const int64_t plane = (int64_t)wpr * g.ny;
+
+  // 1. flags and vertex bases: every load of a round is issued before any
+  //    is used
+  uint64_t* s_fx = reinterpret_cast<uint64_t*>(lds);
+  uint64_t* s_fy = s_fx + 4 * seg;
+  uint64_t* s_fz = s_fy + 4 * seg;
+  uint32_t* s_bv = reinterpret_cast<uint32_t*>(s_fz + 4 * seg);
+  const int ng = gather_gwords(nw, wpr);
+  uint64_t* s_gm = reinterpret_cast<uint64_t*>(s_bv + 4 * seg);
+  uint32_t* s_gp = reinterpret_cast<uint32_t*>(s_gm + gather_gwords(kGatherTile, wpr));
+  constexpr int kSlots = 3;
+  for (int base = 0; base < 4 * seg; base += kSlots * kGatherThreads) {
+    SlotLoad L[kSlots];
+#pragma unroll
+    for (int u = 0; u < kSlots; ++u) {
+      const int i = base + u * kGatherThreads + tid;
+      const int dz = i / seg - 2;
+      const int64_t w = w0 + dz * plane - 2 * wpr - 1 + (i - (dz + 2) * seg);
+      L[u].in = i < 4 * seg && w >= 0 && w < g.nwords;
+      L[u].A = L[u].A1 = L[u].B = L[u].C = 0ull;
+      L[u].bv = 0u;
+      if (L[u].in) {
+        L[u].p = word_pos(g, w);
+        L[u].A = b[w];
+        if (L[u].p.q + 1 < wpr) L[u].A1 = b[w + 1];
+        if (L[u].p.has_y) L[u].B = b[w + wpr];
+        if (L[u].p.has_z) L[u].C = b[w + plane];
+        L[u].bv = base_v[w];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSlots; ++u) {
+      const int i = base + u * kGatherThreads + tid;
+      if (i < 4 * seg) {
+        uint64_t fx = 0ull, fy = 0ull, fz = 0ull;
+        if (L[u].in) edge_flags(L[u].p, L[u].A, L[u].A1, L[u].B, L[u].C, fx, fy, fz);
+        s_fx[i] = fx;
+        s_fy[i] = fy;
+        s_fz[i] = fz;
+        s_bv[i] = L[u].bv;
+      }
+    }
+  }
+  __syncthreads();
+  const Stage st{s_fx, s_fy, s_fz, s_bv, seg, w0};
+
+  // 2. receiving masks
   uint64_t r = 0ull;
-  if (tid < nw) r = receiving_mask(g, b, word_pos(g, w0 + tid));
-  sh.recv[tid] = r;
+  if (tid < nw) {
+    const WordPos p = word_pos(g, w0 + tid);
+    const GatherCtx c{&g, &st, adj, n_adj, w0 + tid, 0, p.q, p.q * 64, p.j, p.k};
+    r = receiving_mask(c);
+    sh.recv[tid] = r;
+  }
   uint32_t total;
   const uint32_t off = block_excl_scan((uint32_t)popc(r), total, sh.scan);
   if (tid < nw) sh.first[tid] = off;
   if (tid == 0) sh.first[nw] = total;
   __syncthreads();
-  for (int i = wv; i < nw; i += kGatherTile / 64) {
-    const WordPos p = word_pos(g, w0 + i);
-    const int x = p.q * 64 + lane;
-    if (x < g.nx && !((sh.recv[i] >> lane) & 1ull))
-      out[(int64_t)x + (int64_t)g.nx * (p.j + (int64_t)g.ny * p.k)] = 0.0f;
+
+  // 3. endpoint masks of the G-words: gi in [0, nw + 2 wpr) is word
+  //    w0 - wpr + gi of the tile's plane; then nw words of the plane below,
+  //    then nw of the plane above.
+  auto recv_of = [&](int64_t w) -> uint64_t {  // receiving mask of a tile word, else 0
+    return (w >= w0 && w < w_end) ? sh.recv[w - w0] : 0ull;
+  };
+  uint32_t n_end = 0;
+  if (total > 0) {
+    const int ch = (ng + kGatherThreads - 1) / kGatherThreads;
+    uint32_t cnt = 0;
+    for (int u = 0; u < ch; ++u) {
+      const int gi = tid * ch + u;
+      if (gi >= ng) break;
+      int dzp;
+      int64_t wt;  // tile-plane word
+      if (gi < nw + 2 * wpr) {
+        dzp = 0;
+        wt = w0 - wpr + gi;
+      } else if (gi < 2 * nw + 2 * wpr) {
+        dzp = -1;
+        wt = w0 + (gi - nw - 2 * wpr);
+      } else {
+        dzp = 1;
+        wt = w0 + (gi - 2 * nw - 2 * wpr);
+      }
+      const int64_t wa = wt + dzp * plane;  // the word itself
+      uint64_t m = 0ull;
+      if (wa >= 0 && wa < g.nwords) {
+        const WordPos p = word_pos(g, wa);
+        uint64_t need;
+        if (dzp == 0) {
+          const uint64_t rw = recv_of(wa);
+          need = rw | (rw << 1) | (rw >> 1);
+          if (p.q > 0) need |= recv_of(wa - 1) >> 63;
+          if (p.q + 1 < wpr) need |= recv_of(wa + 1) << 63;
+          if (p.j > 0) need |= recv_of(wa - wpr);
+          if (p.j + 1 < g.ny) need |= recv_of(wa + wpr);
+        } else {
+          need = recv_of(wt);
+        }
+        if (need) {
+          const GatherCtx c{&g, &st, adj, n_adj, wt, dzp, p.q, p.q * 64, p.j, p.k};
+          m = endpoint_mask(c, 0, 0, 0) & need;
+        }
+      }
+      s_gm[gi] = m;
+      cnt += (uint32_t)popc(m);
+    }
+    uint32_t run = block_excl_scan(cnt, n_end, sh.scan);
+    for (int u = 0; u < ch; ++u) {
+      const int gi = tid * ch + u;
+      if (gi >= ng) break;
+      s_gp[gi] = run;
+      run += (uint32_t)popc(s_gm[gi]);
+    }
+    if (tid == 0) s_gp[ng] = n_end;
+    __syncthreads();
   }
-  for (uint32_t e = (uint32_t)tid; e < total; e += kGatherTile) {
+  const bool shared_sums = n_end <= (uint32_t)kMaxEndpoints;  // block-uniform
+  f32x4* s_rec = reinterpret_cast<f32x4*>(lds);
+  if (total > 0 && shared_sums) {
+    // 4. (S, G) per endpoint, kept in registers until the flags are done
+    f32x4 rr[kRecPerThread];
+#pragma unroll
+    for (int u = 0; u < kRecPerThread; ++u) {
+      const uint32_t e = (uint32_t)tid + (uint32_t)u * kGatherThreads;
+      rr[u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      if (e < n_end) {
+        const int gi = find_word(s_gp, ng, e);
+        const int bit = select_bit(s_gm[gi], (int)(e - s_gp[gi]));
+        const int dzp = gi < nw + 2 * wpr ? 0 : (gi < 2 * nw + 2 * wpr ? -1 : 1);
+        const int64_t wt = dzp == 0 ? w0 - wpr + gi
+                                    : w0 + (gi - nw - 2 * wpr - (dzp == 1 ? nw : 0));
+        const WordPos p = word_pos(g, wt + dzp * plane);
+        const GatherCtx c{&g, &st, adj, n_adj, wt, dzp, p.q, p.q * 64 + bit, p.j, p.k};
+        rr[u] = point_sums(c, 0, 0, 0);
+      }
+    }
+    __syncthreads();  // the flags region now holds the endpoint records
+#pragma unroll
+    for (int u = 0; u < kRecPerThread; ++u) {
+      const uint32_t e = (uint32_t)tid + (uint32_t)u * kGatherThreads;
+      if (e < n_end) s_rec[e] = rr[u];
+    }
+    __syncthreads();
+  }
+
+  // 5. zeros for the points that receive nothing (four per lane where rows
+  //    are 16-byte aligned: 16 lanes per word, 4 words per wave step) ...
+  if ((g.nx & 3) == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0) {
+    const int sub = lane >> 4, x4 = (lane & 15) * 4;
+    for (int i0 = wv * 4; i0 < nw; i0 += kGatherThreads / 16) {
+      const int i = i0 + sub;
+      if (i >= nw) continue;
+      const WordPos p = word_pos(g, w0 + i);
+      const int x = p.q * 64 + x4;
+      if (x >= g.nx) continue;
+      const uint32_t rb = (uint32_t)(sh.recv[i] >> x4) & 0xfu;
+      float* o = out + (int64_t)x + (int64_t)g.nx * (p.j + (int64_t)g.ny * p.k);
+      if (rb == 0u) {
+        *reinterpret_cast<f32x4*>(o) = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      } else {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          if (!((rb >> t) & 1u)) o[t] = 0.0f;
+      }
+    }
+  } else {
+    for (int i = wv; i < nw; i += kGatherThreads / 64) {
+      const WordPos p = word_pos(g, w0 + i);
+      const int x = p.q * 64 + lane;
+      if (x < g.nx && !((sh.recv[i] >> lane) & 1ull))
+        out[(int64_t)x + (int64_t)g.nx * (p.j + (int64_t)g.ny * p.k)] = 0.0f;
+    }
+  }
+  // ... and dL/df of the receiving points
+  for (uint32_t e = (uint32_t)tid; e < total; e += kGatherThreads) {
     const int i = find_word(sh.first, nw, e);
     const int bit = select_bit(sh.recv[i], (int)(e - sh.first[i]));
     const WordPos p = word_pos(g, w0 + i);
-    const GatherCtx c{&g, b, base_v, adj, n_adj, p.q * 64 + bit, p.j, p.k};
-    out[(int64_t)c.x + (int64_t)g.nx * (p.j + (int64_t)g.ny * p.k)] = point_adjoint(c);
+    const int x = p.q * 64 + bit;
+    float v;
+    if (shared_sums) {
+      const int g0 = i + wpr;  // G-word of the point's own word
+      auto rec = [&](int gi, int bb) -> f32x4 {
+        const uint64_t m = s_gm[gi];
+        if (!((m >> bb) & 1ull)) return f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        return s_rec[s_gp[gi] + (uint32_t)popc(m & ((1ull << bb) - 1ull))];
+      };
+      v = combine_point(g, x, p.j, p.k, [&](int d, int s) -> f32x4 {
+        if (s == 0) return rec(g0, bit);
+        if (d == 0) {
+          const int xb = bit + s;  // -1..64
+          return rec(g0 + (xb < 0 ? -1 : (xb > 63 ? 1 : 0)), xb & 63);
+        }
+        if (d == 1) return rec(g0 + s * wpr, bit);
+        return rec((s < 0 ? nw + 2 * wpr : 2 * nw + 2 * wpr) + i, bit);
+      });
+    } else {
+      const GatherCtx c{&g, &st, adj, n_adj, w0 + i, 0, p.q, x, p.j, p.k};
+      v = point_adjoint(c);
+    }
+    out[(int64_t)x + (int64_t)g.nx * (p.j + (int64_t)g.ny * p.k)] = v;
   }
 }
 

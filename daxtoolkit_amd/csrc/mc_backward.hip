@@ -20,15 +20,16 @@ __global__ __launch_bounds__(256) void k_vertex_adjoint(VolGeom g, const float* 
 }
 
 // One tile of kGatherTile row words per block.
-__global__ __launch_bounds__(kGatherTile) void k_point_gather(
+__global__ __launch_bounds__(kGatherThreads) void k_point_gather(
     VolGeom g, const uint64_t* __restrict__ bits, const uint32_t* __restrict__ base_v,
     const VertexAdj* __restrict__ adj, const int64_t* __restrict__ counts, int64_t n_points,
     float* __restrict__ dldf) {
   __shared__ GatherShared sh;
+  extern __shared__ __align__(16) uint64_t s_dyn[];
   const int64_t n_adj = counts ? min(counts[1], n_points) : n_points;
   const int64_t w0 = (int64_t)blockIdx.x * kGatherTile;
   gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj, dldf + g.field_off, w0,
-              min(w0 + kGatherTile, g.nwords), sh);
+              min(w0 + kGatherTile, g.nwords), sh, s_dyn);
 }
 
 }  // namespace mc
@@ -67,6 +68,7 @@ int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz
   g.field_off = 0;
   g.word_off = 0;
   g.nwords = (int64_t)g.wpr * ny * nz;
+  set_geom_divs(g);
   const Workspace L = workspace_layout(g.nwords);
   if (!d_workspace || workspace_bytes < L.bytes) {
     set_error("dax_hip_mc_backward: workspace of %zu bytes, %zu needed", workspace_bytes, L.bytes);
@@ -87,7 +89,8 @@ int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz
         (VertexAdj*)d_scratch);
     DAXHIP_LAUNCH_CHECK("k_vertex_adjoint");
   }
-  k_point_gather<<<dim3((unsigned)ceil_div(g.nwords, kGatherTile)), dim3(kGatherTile), 0, s>>>(
+  k_point_gather<<<dim3((unsigned)ceil_div(g.nwords, kGatherTile)), dim3(kGatherThreads),
+                   gather_lds_bytes(g.wpr), s>>>(
       g, (const uint64_t*)(ws + L.off_bits), (const uint32_t*)(ws + L.off_base_v),
       (const VertexAdj*)d_scratch, d_counts, n_points, d_dl_dfield);
   DAXHIP_LAUNCH_CHECK("k_point_gather");

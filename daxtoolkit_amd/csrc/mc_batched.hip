@@ -76,12 +76,13 @@ BatchLayout batch_layout(int64_t nvols, const int64_t* dims) {
 // Header at the start of the workspace: the layout, so launches need only
 // the workspace pointer after prepare.
 struct BatchHeader {
-  int64_t magic, nvols, nwords, nscan, ngen;
+  int64_t magic, nvols, nwords, nscan, ngen, max_wpr;
   size_t off_vols, off_scan_tiles, off_gen_tiles, off_bits, off_base_v, off_base_t, off_status,
       off_ctrl, bytes;
 };
 constexpr int64_t kBatchMagic = 0x4441584842415443LL;  // "DAXHBATC"
 constexpr size_t kBatchHeaderBytes = 256;
+static_assert(sizeof(BatchHeader) <= kBatchHeaderBytes, "header fits its reserved bytes");
 
 __global__ __launch_bounds__(256) void k_batch_classify(const VolGeom* __restrict__ vols,
                                                         const TileRef* __restrict__ tiles,
@@ -187,19 +188,20 @@ __global__ __launch_bounds__(256) void k_batch_vertex_adjoint(
   }
 }
 
-__global__ __launch_bounds__(kGatherTile) void k_batch_point_gather(
+__global__ __launch_bounds__(kGatherThreads) void k_batch_point_gather(
     const VolGeom* __restrict__ vols, const TileRef* __restrict__ tiles, int64_t ntiles,
     const uint64_t* __restrict__ bits, const uint32_t* __restrict__ base_v,
     const VertexAdj* __restrict__ adj, const int64_t* __restrict__ counts, int64_t n_points,
     float* __restrict__ dldf) {
   static_assert(kGatherTile == kBatchGenTile, "gather tiles are the generate tiles");
   __shared__ GatherShared sh;
+  extern __shared__ __align__(16) uint64_t s_dyn[];
   const int64_t n_adj = counts ? min(counts[1], n_points) : n_points;
   for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
     const VolGeom g = vols[tr.vol];
     gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj, dldf + g.field_off, tr.w0,
-                min((int64_t)tr.w0 + kBatchGenTile, g.nwords), sh);
+                min((int64_t)tr.w0 + kBatchGenTile, g.nwords), sh, s_dyn);
     __syncthreads();  // LDS reused by the next tile
   }
 }
@@ -270,6 +272,7 @@ int dax_hip_mc_batch_prepare(int64_t n_volumes, const int64_t* h_dims3,
     g.field_off = h_field_offsets[v];
     g.word_off = word_off;
     g.nwords = (int64_t)g.wpr * ny * nz;
+    set_geom_divs(g);
     for (int64_t w = 0; w < g.nwords; w += kBatchScanTile) scan_tiles.push_back({(int32_t)v, (int32_t)w});
     for (int64_t w = 0; w < g.nwords; w += kBatchGenTile) gen_tiles.push_back({(int32_t)v, (int32_t)w});
     word_off += g.nwords;
@@ -280,6 +283,8 @@ int dax_hip_mc_batch_prepare(int64_t n_volumes, const int64_t* h_dims3,
   h.nwords = L.nwords;
   h.nscan = L.nscan;
   h.ngen = L.ngen;
+  h.max_wpr = 1;
+  for (const VolGeom& g : vols) h.max_wpr = std::max<int64_t>(h.max_wpr, g.wpr);
   h.off_vols = L.off_vols;
   h.off_scan_tiles = L.off_scan_tiles;
   h.off_gen_tiles = L.off_gen_tiles;
@@ -397,7 +402,7 @@ int dax_hip_mc_batch_backward(const float* d_fields, const void* h_header,
                                                             (VertexAdj*)d_scratch);
     DAXHIP_LAUNCH_CHECK("k_batch_vertex_adjoint");
   }
-  k_batch_point_gather<<<dim3(grid), dim3(kGatherTile), 0, s>>>(vols, gen_tiles, h->ngen, bits, bv,
+  k_batch_point_gather<<<dim3(grid), dim3(kGatherThreads), gather_lds_bytes((int)h->max_wpr), s>>>(vols, gen_tiles, h->ngen, bits, bv,
                                                         (const VertexAdj*)d_scratch, d_counts,
                                                         n_points, d_dl_dfields);
   DAXHIP_LAUNCH_CHECK("k_batch_point_gather");

@@ -27,6 +27,27 @@ constexpr int kWordsPerThread = kTileWords / kScanThreads;
 constexpr int kMaxWpr = 256;         // nx <= 16384
 
 // Geometry of one volume in the (possibly batched) word space.
+// Division by an invariant d >= 1 for numerators below 2^31: q = (hi32(n*m)
+// + n) >> l with l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1
+// (Granlund & Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, section 4).
+struct FastDiv {
+  uint32_t m;
+  int32_t l;
+};
+
+inline FastDiv make_fast_div(uint32_t d) {
+  FastDiv f;
+  f.l = 0;
+  while ((1ull << f.l) < d) ++f.l;
+  f.m = (uint32_t)((((1ull << 32) * ((1ull << f.l) - d)) / d) + 1);
+  return f;
+}
+
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, FastDiv f) {
+  return (uint32_t)(((uint64_t)__umulhi(n, f.m) + n) >> f.l);
+}
+
 struct VolGeom {
   int32_t nx, ny, nz, wpr;
   float iso;
@@ -35,7 +56,15 @@ struct VolGeom {
   int64_t field_off;  // first field element of the volume
   int64_t word_off;   // first row word of the volume
   int64_t nwords;     // wpr*ny*nz
+  FastDiv div_wpr, div_ny, div_nx;
 };
+
+// Host: fill the division constants after nx, ny, wpr are set.
+inline void set_geom_divs(VolGeom& g) {
+  g.div_wpr = make_fast_div((uint32_t)g.wpr);
+  g.div_ny = make_fast_div((uint32_t)g.ny);
+  g.div_nx = make_fast_div((uint32_t)g.nx);
+}
 
 // Edge e of a cell with lowest corner (i,j,k): its lower endpoint is
 // (i+dx, j+dy, k+dz) and it runs along `axis`.  Derived from the vertex pairs
@@ -76,12 +105,16 @@ struct WordPos {
   uint64_t xmask;      // bits whose +x neighbour exists (x-edges, cells in x)
 };
 
+// lw < 2^31: a volume holds at most 2^30 points and wpr <= nx, so its word
+// count wpr*ny*nz is at most 2^30 (32-bit division is ~4x cheaper).
 __device__ __forceinline__ WordPos word_pos(const VolGeom& g, int64_t lw) {
   WordPos p;
-  const int64_t row = lw / g.wpr;
-  p.q = (int)(lw - row * g.wpr);
-  p.j = (int)(row % g.ny);
-  p.k = (int)(row / g.ny);
+  const uint32_t w = (uint32_t)lw;
+  const uint32_t row = fast_div(w, g.div_wpr);
+  const uint32_t k = fast_div(row, g.div_ny);
+  p.q = (int)(w - row * (uint32_t)g.wpr);
+  p.j = (int)(row - k * (uint32_t)g.ny);
+  p.k = (int)k;
   p.has_y = p.j + 1 < g.ny;
   p.has_z = p.k + 1 < g.nz;
   const int x0 = p.q * 64;

@@ -180,7 +180,9 @@ __global__ void k_point_gradient(VolGeom g, const float* __restrict__ f, float* 
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t P = (int64_t)g.nx * g.ny * g.nz;
   if (p >= P) return;
-  const int x = (int)(p % g.nx), y = (int)((p / g.nx) % g.ny), z = (int)(p / ((int64_t)g.nx * g.ny));
+  const uint32_t row = fast_div((uint32_t)p, g.div_nx), zz = fast_div(row, g.div_ny);
+  const int x = (int)((uint32_t)p - row * (uint32_t)g.nx), y = (int)(row - zz * (uint32_t)g.ny),
+            z = (int)zz;
   float gr[3];
   point_gradient(g, f, x, y, z, gr);
   grad[3 * p + 0] = gr[0];
@@ -265,6 +267,7 @@ int make_geom(int64_t nx, int64_t ny, int64_t nz, const float* origin, const flo
   g.field_off = 0;
   g.word_off = 0;
   g.nwords = (int64_t)g.wpr * ny * nz;
+  set_geom_divs(g);
   return DAX_HIP_OK;
 }
 

@@ -23,10 +23,11 @@ static __constant__ uint64_t c_case_table[256] = DAX_HIP_MC_CASE_TABLE_INIT;
 struct WordIter {
   int q, j, k;
   __device__ __forceinline__ void init(const VolGeom& g, int lw) {
-    const int row = lw / g.wpr;
-    q = lw - row * g.wpr;
-    j = row % g.ny;
-    k = row / g.ny;
+    const uint32_t row = fast_div((uint32_t)lw, g.div_wpr);
+    const uint32_t kk = fast_div(row, g.div_ny);
+    q = lw - (int)row * g.wpr;
+    j = (int)(row - kk * (uint32_t)g.ny);
+    k = (int)kk;
   }
   __device__ __forceinline__ void next(const VolGeom& g) {
     if (++q == g.wpr) {
@@ -71,7 +72,7 @@ __device__ __forceinline__ void classify_words(const VolGeom& g, const float* __
         v[u] = __builtin_nontemporal_load(f + w * 64 + lane);
         ok[u] = true;
       } else {
-        const int64_t row = w / g.wpr;
+        const int64_t row = fast_div((uint32_t)w, g.div_wpr);
         const int x = (int)(w - row * g.wpr) * 64 + lane;
         if (x < g.nx) {
           v[u] = __builtin_nontemporal_load(f + row * g.nx + x);
@@ -415,13 +416,15 @@ struct GenLayout {
   }
 };
 
+// Word position of a word with cells: q < kMaxWpr (8 bits); every axis has
+// at least 2 points and the grid at most 2^30, so j, k < 2^28.
 __device__ __forceinline__ uint64_t pack_pos(int q, int j, int k) {
-  return (uint64_t)(uint32_t)q | ((uint64_t)(uint32_t)j << 16) | ((uint64_t)(uint32_t)k << 40);
+  return (uint64_t)(uint32_t)q | ((uint64_t)(uint32_t)j << 8) | ((uint64_t)(uint32_t)k << 36);
 }
 __device__ __forceinline__ void unpack_pos(uint64_t v, int& q, int& j, int& k) {
-  q = (int)(v & 0xffffu);
-  j = (int)((v >> 16) & 0xffffffu);
-  k = (int)(v >> 40);
+  q = (int)(v & 0xffu);
+  j = (int)((v >> 8) & 0xfffffffu);
+  k = (int)(v >> 36);
 }
 
 // Exclusive scan of one value per thread over the block (thread order);

@@ -114,3 +114,45 @@ def test_backward_deterministic_and_fd_256():
     assert e5[conv].max() <= 1e-3
     # away from the surface the gradient is exactly zero
     assert np.all(got[uni][fd5[:2048] == 0.0] == 0.0)
+
+
+def test_backward_dense_surface():
+    """A field with surface everywhere (sum of short-period sines, 64^3 and a
+    ragged 70x45x38): most gather tiles then hold more stencil endpoints than
+    their LDS record space and take the direct per-point path, the others the
+    shared per-endpoint path.  Checked against the oracle's FP64 adjoint and
+    for run-to-run determinism."""
+    for dims in ((64, 64, 64), (70, 45, 38)):
+        nx, ny, nz = dims
+        z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        f = (np.sin(2 * np.pi * x / 6.0) + np.sin(2 * np.pi * y / 6.5) +
+             np.sin(2 * np.pi * z / 7.0)).astype(np.float32)
+        iso = 0.3
+        print(f"[case] adjoint dense surface {dims}", flush=True)
+        field = torch.from_numpy(f).cuda()
+        out = mc.forward(field, iso, edges=True)
+        U = out.num_points
+        # endpoints (points with an active incident edge) per 16384 points
+        b = f > iso
+        ends = np.zeros_like(b)
+        for ax in range(3):
+            d = np.diff(b, axis=ax) != 0
+            sl_lo = [slice(None)] * 3
+            sl_hi = [slice(None)] * 3
+            sl_lo[ax] = slice(0, -1)
+            sl_hi[ax] = slice(1, None)
+            ends[tuple(sl_lo)] |= d
+            ends[tuple(sl_hi)] |= d
+        assert ends.sum() / f.size * 16384 > 1024
+        gen = torch.Generator().manual_seed(7)
+        dx = torch.randn((U, 3), generator=gen, dtype=torch.float32)
+        dn = torch.randn((U, 3), generator=gen, dtype=torch.float32)
+        g1 = mc.backward(field, iso, out, dx.cuda(), dn.cuda())
+        g2 = mc.backward(field, iso, out, dx.cuda(), dn.cuda())
+        assert torch.equal(g1, g2)
+        r = orc.generate(f, iso)
+        assert r["n_pts"] == U
+        ref = orc.backward_f64(f, iso, r["first"], r["second"], dx.numpy(), dn.numpy())
+        got = g1.cpu().numpy().reshape(-1).astype(np.float64)
+        assert np.array_equal(got == 0.0, ref == 0.0) or np.abs(got[ref == 0.0]).max() == 0.0
+        assert _close(got, ref, 1e-4, 1e-3) <= 1e-4
