@@ -298,6 +298,24 @@ __device__ __forceinline__ uint64_t receiving_mask(const GatherCtx& c) {
   return r & low_mask(c.g->nx - c.q * 64);
 }
 
+// Zeros over the points of row words [w0, w_end) of a volume: one
+// contiguous run of the field (16-byte stores between scalar head and tail).
+__device__ __forceinline__ void zero_run(const VolGeom& g, float* __restrict__ out, int64_t w0,
+                                         int64_t w_end, int tid, int nt) {
+  const WordPos pa = word_pos(g, w0), pb = word_pos(g, w_end - 1);
+  const int64_t p0 = (int64_t)pa.q * 64 + (int64_t)g.nx * (pa.j + (int64_t)g.ny * pa.k);
+  const int64_t p1 = min((int64_t)pb.q * 64 + 64, (int64_t)g.nx) +
+                     (int64_t)g.nx * (pb.j + (int64_t)g.ny * pb.k);
+  float* o = out + p0;
+  const int64_t n = p1 - p0;
+  const int64_t head = min(n, (int64_t)(((16 - (reinterpret_cast<uintptr_t>(o) & 15)) & 15) / 4));
+  for (int64_t t = tid; t < head; t += nt) o[t] = 0.0f;
+  f32x4* o4 = reinterpret_cast<f32x4*>(o + head);
+  const int64_t n4 = (n - head) >> 2;
+  for (int64_t t = tid; t < n4; t += nt) o4[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int64_t t = head + 4 * n4 + tid; t < n; t += nt) o[t] = 0.0f;
+}
+
 struct GatherShared {
   uint64_t recv[kGatherTile];
   uint32_t first[kGatherTile + 1];  // exclusive scan of popcount(recv)
@@ -315,6 +333,7 @@ struct SlotLoad {
 // dL/df of the points of words [w0, w_end) of a volume (w_end - w0 <=
 // kGatherTile, blockDim.x == kGatherThreads).  Every thread of the block
 // calls this; `lds` holds gather_lds_bytes(g.wpr).
+//   0. a tile without active edges nearby only writes zeros
 //   1. stage flags and vertex bases of the planes k-2..k+1
 //   2. receiving masks of the tile's words (thread per word)
 //   3. endpoint masks of the G-words, restricted to the points some
@@ -335,6 +354,19 @@ __device__ __forceinline__ void gather_tile(const VolGeom& g, const uint64_t* __
   const int wpr = g.wpr;
   const int seg = gather_seg(wpr);
   const int64_t plane = (int64_t)wpr * g.ny;
+
+  // 0. a tile with no active edge among its staged words (the vertex bases
+  //    count the edges of a word range) has no receiving point: zeros only
+  bool active = false;
+  if (tid < 4) {
+    const int64_t ws = max(w0 + (tid - 2) * plane - 2 * wpr - 1, (int64_t)0);
+    const int64_t we = min(w0 + (tid - 2) * plane - 2 * wpr - 1 + seg, g.nwords);
+    active = we > ws && base_v[we] != base_v[ws];
+  }
+  if (!__syncthreads_or(active)) {
+    zero_run(g, out, w0, w_end, tid, kGatherThreads);
+    return;
+  }
 
   // 1. flags and vertex bases: every load of a round is issued before any
   //    is used
