@@ -7,6 +7,11 @@ with duplicate-point removal, central-difference point-gradient normals and
 the compact active-edge list.  One step = one forward (classify, count/scan,
 generate) over the resident field into preallocated outputs.
 
+Timed regions are the K steps captured as one HIP graph and launched once
+(bracketed by device synchronisation), so host-side jitter between Python
+calls cannot leave the GPU idle; the same K steps launched one by one from
+Python are reported beside it.
+
 Prints one JSON line.  `value` is Gcells/s of the forward; `roofline` is the
 classify kernel (the dominant one) measured with HIP events on the stream the
 kernels run on; `cpu_baseline` times the OpenMP restatement of Dax's
@@ -95,17 +100,21 @@ def cpu_baseline(f_host, iso, cells, budget_s):
     }
 
 
-def _timed(fn, steps):
-    """Mean ms of fn() over `steps` calls, HIP events on the current stream."""
+def _timed(fn, steps, per_step=None):
+    """Mean ms of fn() over `steps` calls, HIP events on the current stream
+    (one event between consecutive steps; the per-step times go into
+    `per_step` when a list is given)."""
     import torch
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     torch.cuda.synchronize()
-    e0.record()
-    for _ in range(steps):
+    ev[0].record()
+    for i in range(steps):
         fn()
-    e1.record()
+        ev[i + 1].record()
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / steps
+    if per_step is not None:
+        per_step.extend(ev[i].elapsed_time(ev[i + 1]) for i in range(steps))
+    return ev[0].elapsed_time(ev[steps]) / steps
 
 
 def _upstream(U, seed):
@@ -136,9 +145,18 @@ def single_volume(field, iso, label, args, tuning=None):
         mc.forward_into(field, iso, buf)
     torch.cuda.synchronize()
     print(f"[bench] {label}: T={T} U={U} tuning={tuned}", file=sys.stderr, flush=True)
-    wall0 = time.perf_counter()
-    ms = _timed(lambda: mc.forward_into(field, iso, buf), args.steps)
-    wall = time.perf_counter() - wall0
+    # timed region: the K forwards captured as one graph (one host launch, so
+    # host-side jitter cannot idle the GPU between steps)
+    s = torch.cuda.Stream()
+    g_fwd = mc.capture(lambda: [mc.forward_into(field, iso, buf) for _ in range(args.steps)], s)
+    with torch.cuda.stream(s):
+        mc.launch_graph(g_fwd, s)
+        wall0 = time.perf_counter()
+        ms = _timed(lambda: mc.launch_graph(g_fwd, s), 1) / args.steps
+        wall = time.perf_counter() - wall0
+    mc.destroy_graph(g_fwd)
+    steps_ms = []  # the same K forwards launched one by one from Python
+    loop_ms = _timed(lambda: mc.forward_into(field, iso, buf), args.steps, steps_ms)
     assert mc.read_counts(buf.counts, buf.cap_points, buf.cap_tris) == (T, U)
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
     for k in range(args.steps):
@@ -146,7 +164,7 @@ def single_volume(field, iso, label, args, tuning=None):
     torch.cuda.synchronize()
     k_ms = [statistics.mean(e[i].elapsed_time(e[i + 1]) for e in ev) for i in range(3)]
     r = {"P": P, "C": C, "T": T, "U": U, "ms": ms, "wall": wall, "k_ms": k_ms,
-         "tuned": tuned, "tuned_ms": tuned_ms}
+         "tuned": tuned, "tuned_ms": tuned_ms, "steps_ms": steps_ms, "loop_ms": loop_ms}
     if not args.no_bwd:
         dv, dn = _upstream(U, 1)
         scratch = torch.empty(max(int(mc._lib.load().dax_hip_mc_backward_scratch_bytes(U)), 1),
@@ -158,9 +176,15 @@ def single_volume(field, iso, label, args, tuning=None):
             mc.backward_into(field, iso, buf, dv, dn, scratch, dldf)
         for _ in range(2):
             fb()
-        r["fb_ms"] = _timed(fb, args.steps)
-        r["bwd_ms"] = _timed(lambda: mc.backward_into(field, iso, buf, dv, dn, scratch, dldf),
-                             args.steps)
+        g_fb = mc.capture(lambda: [fb() for _ in range(args.steps)], s)
+        g_b = mc.capture(lambda: [mc.backward_into(field, iso, buf, dv, dn, scratch, dldf)
+                                  for _ in range(args.steps)], s)
+        with torch.cuda.stream(s):
+            mc.launch_graph(g_fb, s)
+            r["fb_ms"] = _timed(lambda: mc.launch_graph(g_fb, s), 1) / args.steps
+            r["bwd_ms"] = _timed(lambda: mc.launch_graph(g_b, s), 1) / args.steps
+        mc.destroy_graph(g_fb)
+        mc.destroy_graph(g_b)
         print(f"[bench] {label}: fwd {ms:.4f} ms, fwd+bwd {r['fb_ms']:.4f} ms",
               file=sys.stderr, flush=True)
     return r
@@ -312,6 +336,10 @@ def main():
         "kernel_ms": {"classify": round(k_ms[0], 5), "count_scan": round(k_ms[1], 5),
                       "generate": round(k_ms[2], 5)},
         "wall_s": round(r["wall"], 4),
+        "python_loop_ms_per_step": round(r["loop_ms"], 5),
+        "python_loop_step_ms": {"min": round(min(r["steps_ms"]), 5),
+                                "median": round(statistics.median(r["steps_ms"]), 5),
+                                "max": round(max(r["steps_ms"]), 5)},
         "tuning": {"classify_unroll": tuned[0], "classify_vec": tuned[1], "scan_sub": tuned[2],
                    "gen_tile": tuned[3], "gen_xcd": (list(tuned) + [0])[4],
                    "autotune_ms": [round(x, 5) for x in r["tuned_ms"]]},

@@ -111,6 +111,7 @@ __global__ __launch_bounds__(kBatchScanNT) void k_batch_count_scan(
     uint32_t* __restrict__ base_t, unsigned long long* __restrict__ status,
     Ctrl* __restrict__ ctrl) {
   __shared__ ScanShared sh;
+  __shared__ uint32_t s_pk[ScanTileLds<kBatchScanNT, kBatchScanWPT>::kSize];
   __shared__ int64_t s_tile;
   for (int i = threadIdx.x; i < 256; i += kBatchScanNT) sh.faces[i] = (uint8_t)(c_case_table[i] >> 60);
   while (true) {
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(kBatchScanNT) void k_batch_count_scan(
     const TileRef tr = tiles[t];
     const VolGeom g = vols[tr.vol];
     count_scan_tile<kBatchScanNT, kBatchScanWPT>(g, bits + g.word_off, base_v, base_t, status, ctrl,
-                                                t, tr.w0, sh);
+                                                t, tr.w0, sh, s_pk);
   }
 }
 

@@ -58,6 +58,7 @@ __global__ __launch_bounds__(NT) void k_count_scan(VolGeom g, const uint64_t* __
                                                    unsigned long long* __restrict__ status,
                                                    Ctrl* __restrict__ ctrl, int64_t ntiles) {
   __shared__ ScanShared sh;
+  __shared__ uint32_t s_pk[ScanTileLds<NT, WPT>::kSize];
   __shared__ int64_t s_tile;
   if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(&ctrl->tile_counter, 1u);
   for (int i = threadIdx.x; i < 256; i += NT) sh.faces[i] = (uint8_t)(c_case_table[i] >> 60);
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(NT) void k_count_scan(VolGeom g, const uint64_t* __
   const int64_t tile = s_tile;
   if (tile >= ntiles) return;  // scan state not reset by a classify launch
   count_scan_tile<NT, WPT>(g, bits + g.word_off, base_v, base_t, status, ctrl, tile,
-                           tile * (int64_t)NT * WPT, sh);
+                           tile * (int64_t)NT * WPT, sh, s_pk);
 }
 
 template <int TW>
@@ -117,7 +118,8 @@ bool valid_tuning(const Tuning& t) {
   const bool cls = (t.cls_vec == 1 && (t.cls_unroll == 8 || t.cls_unroll == 16 || t.cls_unroll == 32)) ||
                    (t.cls_vec == 4 && (t.cls_unroll == 2 || t.cls_unroll == 4 || t.cls_unroll == 8));
   const bool scan = t.scan_sub == 1 || t.scan_sub == 2 || t.scan_sub == 4 || t.scan_sub == 8;
-  const bool gen = (t.gen_tile == 256 || t.gen_tile == 512 || t.gen_tile == 1024) &&
+  const bool gen = (t.gen_tile == 128 || t.gen_tile == 256 || t.gen_tile == 512 ||
+                    t.gen_tile == 1024) &&
                    (t.gen_xcd == 0 || t.gen_xcd == 1);
   return cls && scan && gen;
 }
@@ -352,6 +354,9 @@ int launch_generate(const VolGeom& g, const float* field, const char* ws, const 
                     unsigned flags, const OutPtrs& o, int64_t cap_points, int64_t cap_tris,
                     int64_t* counts, const Tuning& tu, hipStream_t s) {
   switch (tu.gen_tile) {
+    case 128:
+      return launch_generate_tw<128>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
+                                     tu.gen_xcd, s);
     case 256:
       return launch_generate_tw<256>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
                                      tu.gen_xcd, s);
@@ -725,11 +730,11 @@ int dax_hip_mc_autotune(const float* d_field, int64_t nx, int64_t ny, int64_t nz
   // generate variants over one classify + scan
   if ((rc = launch_bits(g, d_field, ws, L, best, s))) return rc;
   if ((rc = launch_scan(g, ws, L, best, s))) return rc;
-  for (int tv = 0; tv < 6; ++tv) {
-    const int tw = tv % 3 == 0 ? 256 : (tv % 3 == 1 ? 512 : 1024);
+  for (int tv = 0; tv < 8; ++tv) {
+    const int tw = 128 << (tv % 4);
     Tuning t = best;
     t.gen_tile = tw;
-    t.gen_xcd = tv / 3;
+    t.gen_xcd = tv / 4;
     if ((rc = launch_generate(g, d_field, ws, L, flags, o, cap_points, cap_tris, d_counts, t, s)))
       return rc;
     DAXHIP_CHECK(hipEventRecord(ev.a, s));

@@ -212,49 +212,65 @@ __device__ __forceinline__ void look_back(unsigned long long* __restrict__ statu
   }
 }
 
+// LDS words of a count/scan tile: one u32 per row word, one pad word per 32
+// so that both the coalesced (word u*NT + tid) and the per-thread
+// (word tid*WPT + u) access patterns are free of bank conflicts.
+template <int NT, int WPT>
+struct ScanTileLds {
+  static constexpr int kWords = NT * WPT;
+  static constexpr int kSize = kWords + kWords / 32;
+};
+__device__ __forceinline__ int pad32(int i) { return i + (i >> 5); }
+
 // One scan tile of a volume: words [w0, w0 + NT*WPT) (volume-local), status
 // slot `tile` in the global tile order.
+//   1. per-word vertex / triangle counts, consecutive threads on consecutive
+//      words (coalesced row loads), packed into LDS
+//   2. each thread sums WPT consecutive words; block scan; decoupled
+//      look-back for the tile's prefix
+//   3. the bases go back through LDS and out with coalesced stores
 template <int NT, int WPT>
 __device__ __forceinline__ void count_scan_tile(const VolGeom& g, const uint64_t* __restrict__ b,
                                                 uint32_t* __restrict__ base_v,
                                                 uint32_t* __restrict__ base_t,
                                                 unsigned long long* __restrict__ status,
                                                 Ctrl* __restrict__ ctrl, int64_t tile,
-                                                int64_t tile_w0, ScanShared& sh) {
+                                                int64_t tile_w0, ScanShared& sh,
+                                                uint32_t* __restrict__ s_pk) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int64_t plane = (int64_t)g.wpr * g.ny;
+  const int wpr = g.wpr;
+#pragma unroll
+  for (int u = 0; u < WPT; ++u) {
+    const int i = u * NT + tid;
+    const int64_t w = tile_w0 + i;
+    uint32_t pk = 0;
+    if (w < g.nwords) {
+      const WordPos p = word_pos(g, w);
+      const bool nx1 = p.q + 1 < wpr;
+      const bool hy = w + wpr < g.nwords, hz = w + plane < g.nwords,
+                 hyz = w + plane + wpr < g.nwords;
+      const uint64_t A = b[w], A1 = nx1 ? b[w + 1] : 0ull;
+      const uint64_t B = hy ? b[w + wpr] : 0ull, B1 = (hy && nx1) ? b[w + wpr + 1] : 0ull;
+      const uint64_t C = hz ? b[w + plane] : 0ull, C1 = (hz && nx1) ? b[w + plane + 1] : 0ull;
+      const uint64_t D = hyz ? b[w + plane + wpr] : 0ull;
+      const uint64_t D1 = (hyz && nx1) ? b[w + plane + wpr + 1] : 0ull;
+      uint32_t nv, nt;
+      word_counts_regs(g, p, A, A1, B, B1, C, C1, D, D1, sh.faces, nv, nt);
+      pk = nv | (nt << 16);
+    }
+    s_pk[pad32(i)] = pk;
+  }
+  __syncthreads();
   const int64_t w0 = tile_w0 + (int64_t)tid * WPT;
   uint32_t packed[WPT];
   uint32_t sum_v = 0, sum_t = 0;
 #pragma unroll
-  for (int u = 0; u < WPT; ++u) packed[u] = 0;
-  if (w0 < g.nwords) {
-    uint64_t A[WPT + 1], B[WPT + 1], C[WPT + 1], D[WPT + 1];
-#pragma unroll
-    for (int u = 0; u <= WPT; ++u) {
-      const int64_t w = w0 + u;
-      A[u] = w < g.nwords ? b[w] : 0ull;
-      B[u] = w + g.wpr < g.nwords ? b[w + g.wpr] : 0ull;
-      C[u] = w + plane < g.nwords ? b[w + plane] : 0ull;
-      D[u] = w + plane + g.wpr < g.nwords ? b[w + plane + g.wpr] : 0ull;
-    }
-    WordIter it;
-    it.init(g, (int)w0);
-#pragma unroll
-    for (int u = 0; u < WPT; ++u) {
-      if (w0 + u < g.nwords) {
-        const WordPos p = it.pos(g);
-        const bool nx1 = p.q + 1 < g.wpr;
-        uint32_t nv, nt;
-        word_counts_regs(g, p, A[u], nx1 ? A[u + 1] : 0ull, B[u], nx1 ? B[u + 1] : 0ull, C[u],
-                         nx1 ? C[u + 1] : 0ull, D[u], nx1 ? D[u + 1] : 0ull, sh.faces, nv, nt);
-        packed[u] = nv | (nt << 16);
-        sum_v += nv;
-        sum_t += nt;
-        it.next(g);
-      }
-    }
+  for (int u = 0; u < WPT; ++u) {
+    packed[u] = s_pk[pad32(tid * WPT + u)];
+    sum_v += packed[u] & 0xffffu;
+    sum_t += packed[u] >> 16;
   }
   const uint32_t inc_v = wave_incl_scan(sum_v), inc_t = wave_incl_scan(sum_t);
   if (lane == 63) {
@@ -274,21 +290,25 @@ __device__ __forceinline__ void count_scan_tile(const VolGeom& g, const uint64_t
   }
   if (wv == 0) look_back(status, tile, agg_v, agg_t, ctrl, sh.prefix);
   __syncthreads();
-  uint32_t rv = sh.prefix[0] + off_v + inc_v - sum_v;
-  uint32_t rt = sh.prefix[1] + off_t + inc_t - sum_t;
+  // vertex bases, then triangle bases, each through LDS
 #pragma unroll
-  for (int u = 0; u < WPT; ++u) {
-    const int64_t w = w0 + u;
-    if (w < g.nwords) {
-      base_v[g.word_off + w] = rv;
-      base_t[g.word_off + w] = rt;
-      rv += packed[u] & 0xffffu;
-      rt += packed[u] >> 16;
-      if (w == g.nwords - 1) {
-        base_v[g.word_off + w + 1] = rv;
-        base_t[g.word_off + w + 1] = rt;
-      }
+  for (int pass = 0; pass < 2; ++pass) {
+    uint32_t run = sh.prefix[pass] + (pass == 0 ? off_v + inc_v - sum_v : off_t + inc_t - sum_t);
+    uint32_t* base = pass == 0 ? base_v : base_t;
+#pragma unroll
+    for (int u = 0; u < WPT; ++u) {
+      s_pk[pad32(tid * WPT + u)] = run;
+      run += pass == 0 ? (packed[u] & 0xffffu) : (packed[u] >> 16);
+      if (w0 + u == g.nwords - 1) base[g.word_off + g.nwords] = run;
     }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < WPT; ++u) {
+      const int i = u * NT + tid;
+      const int64_t w = tile_w0 + i;
+      if (w < g.nwords) base[g.word_off + w] = s_pk[pad32(i)];
+    }
+    __syncthreads();
   }
 }
 

@@ -251,10 +251,10 @@ def autotune(field: torch.Tensor, iso: float, buf: ForwardBuffers, reps: int = 1
 
 
 def set_tuning(shape, params) -> None:
-    """params: classify_unroll, classify_vec, scan_sub, gen_tile[, gen_xcd=1]."""
+    """params: classify_unroll, classify_vec, scan_sub, gen_tile[, gen_xcd=0]."""
     import ctypes
     nx, ny, nz = shape
-    p = list(params) + [1] * (5 - len(params))
+    p = list(params) + [0] * (5 - len(params))
     arr = (ctypes.c_int * 5)(*p)
     _lib.call("dax_hip_mc_set_tuning", nx, ny, nz, arr)
 
@@ -450,3 +450,12 @@ def capture(fn, stream: torch.cuda.Stream):
         finally:
             _lib.call("dax_hip_graph_end_capture", stream.cuda_stream, ctypes.byref(h))
     return h
+
+
+def launch_graph(graph, stream: torch.cuda.Stream) -> None:
+    """Replay a graph made by :func:`capture` on `stream`."""
+    _lib.call("dax_hip_graph_launch", graph, stream.cuda_stream)
+
+
+def destroy_graph(graph) -> None:
+    _lib.call("dax_hip_graph_destroy", graph)
