@@ -42,6 +42,28 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 
+def _kernel_variants(n, tuned):
+    """Template names of the three forward kernels a tuning launches."""
+    unroll, vec, sub, tile = tuned[0], tuned[1], tuned[2], tuned[3]
+    cls = (f"k_classify_v4<{unroll}>" if vec == 4 else
+           f"k_classify<{unroll}, {'true' if n % 64 == 0 else 'false'}>")
+    scan = {1: "<256, 4>", 2: "<512, 4>", 4: "<1024, 4>", 8: "<1024, 8>"}[sub]
+    return cls, f"k_count_scan{scan}", f"k_generate<{tile}>"
+
+
+def _measured_traffic(n, tuned):
+    """HBM bytes per dispatch of the forward's kernels (by role) from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the 512^3
+    workload, scaled by the copy-kernel calibration; None for another
+    workload or a kernel variant the summary does not hold."""
+    path = ROOT / "profiles" / "forward_512_traffic.json"
+    if n != 512 or not path.exists():
+        return None
+    d = json.loads(path.read_text())["hbm_bytes_per_dispatch"]
+    roles = ("k_classify_v4", "k_count_scan", "k_generate")
+    return {r: d.get(v) for r, v in zip(roles, _kernel_variants(n, tuned))}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +139,20 @@ def _timed(fn, steps, per_step=None):
     return ev[0].elapsed_time(ev[steps]) / steps
 
 
+def _graph_ms(fn, steps, stream):
+    """Mean ms of fn() when `steps` calls are captured as one graph."""
+    import torch
+
+    from daxtoolkit_amd import mc
+    g = mc.capture(lambda: [fn() for _ in range(steps)], stream)
+    try:
+        with torch.cuda.stream(stream):
+            mc.launch_graph(g, stream)
+            return _timed(lambda: mc.launch_graph(g, stream), 1) / steps
+    finally:
+        mc.destroy_graph(g)
+
+
 def _upstream(U, seed):
     import torch
     gen = torch.Generator(device="cuda").manual_seed(seed)
@@ -158,11 +194,15 @@ def single_volume(field, iso, label, args, tuning=None):
     steps_ms = []  # the same K forwards launched one by one from Python
     loop_ms = _timed(lambda: mc.forward_into(field, iso, buf), args.steps, steps_ms)
     assert mc.read_counts(buf.counts, buf.cap_points, buf.cap_tris) == (T, U)
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
-    for k in range(args.steps):
-        mc.forward_phases(field, iso, buf, events=ev[k])
-    torch.cuda.synchronize()
-    k_ms = [statistics.mean(e[i].elapsed_time(e[i + 1]) for e in ev) for i in range(3)]
+    # per-kernel durations, each from K launches captured as one graph:
+    # classify alone; classify + count/scan (count/scan needs the state
+    # classify clears) minus classify; generate alone (it only reads the
+    # workspace)
+    t_cls = _graph_ms(lambda: mc.classify_into(field, iso, buf), args.steps, s)
+    t_cs = _graph_ms(lambda: (mc.classify_into(field, iso, buf), mc.count_scan_into(buf)),
+                     args.steps, s) - t_cls
+    t_gen = _graph_ms(lambda: mc.generate_into(field, iso, buf), args.steps, s)
+    k_ms = [t_cls, t_cs, t_gen]
     r = {"P": P, "C": C, "T": T, "U": U, "ms": ms, "wall": wall, "k_ms": k_ms,
          "tuned": tuned, "tuned_ms": tuned_ms, "steps_ms": steps_ms, "loop_ms": loop_ms}
     if not args.no_bwd:
@@ -307,10 +347,20 @@ def main():
     r = single_volume(field, iso, f"{n}^3", args, tuning)
     P, C, T, U, ms, k_ms, tuned = r["P"], r["C"], r["T"], r["U"], r["ms"], r["k_ms"], r["tuned"]
 
-    b_classify = 4 * P + P // 8                      # field read once + inside-bits written
+    W = ((n + 63) // 64) * n * n                       # row words
     b_fwd = 4 * P + 12 * U + 12 * U + 4 * 3 * T + 4 * U  # field + verts + normals + conn + edges
-    achieved = b_classify / (k_ms[0] * 1e-3) / 1e9
     fwd_gbs = b_fwd / (ms * 1e-3) / 1e9
+    names = ["k_classify_v4", "k_count_scan", "k_generate"]
+    alg = [4 * P + P // 8,                 # field read once, one inside bit per point written
+           P // 8 + 8 * W,                 # inside-bits read, two u32 bases per row word written
+           12 * U + 12 * U + 4 * 3 * T + 4 * U]  # the forward's outputs
+    traffic = _measured_traffic(n, tuned)
+    kernels = {}
+    for nm, t_k, a_k in zip(names, k_ms, alg):
+        kernels[nm] = {"ms": round(t_k, 5), "alg_bytes": a_k, "GBps": round(a_k / (t_k * 1e-3) / 1e9, 1),
+                       "frac": round(a_k / (t_k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "traffic": traffic.get(nm) if traffic else None}
+    dominant = names[max(range(3), key=lambda i: k_ms[i])]
 
     line = {
         "metric": "MC Gcells/s at 512^3 fwd and fwd+bwd; % HBM roofline; batched volumes/s",
@@ -343,9 +393,13 @@ def main():
         "tuning": {"classify_unroll": tuned[0], "classify_vec": tuned[1], "scan_sub": tuned[2],
                    "gen_tile": tuned[3], "gen_xcd": (list(tuned) + [0])[4],
                    "autotune_ms": [round(x, 5) for x in r["tuned_ms"]]},
-        "roofline": {"bound": "hbm", "kernel": "k_classify", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "roofline": {"bound": "hbm", "scope": "forward graph: k_classify_v4, k_count_scan, k_generate",
+                     "achieved": round(fwd_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(fwd_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": (sum(traffic[k] for k in names)
+                                 if traffic and all(traffic.get(k) for k in names) else None),
+                     "kernel_variants": list(_kernel_variants(n, tuned)),
+                     "dominant_kernel": dominant, "kernels": kernels},
     }
     if "fb_ms" in r:
         line["fwd_bwd"] = {"ms": round(r["fb_ms"], 5), "bwd_ms": round(r["bwd_ms"], 5),

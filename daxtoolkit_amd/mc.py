@@ -141,6 +141,33 @@ def forward_into(field: torch.Tensor, iso: float, buf: ForwardBuffers, origin=(0
               buf.counts.data_ptr(), _stream())
 
 
+def classify_into(field: torch.Tensor, iso: float, buf: ForwardBuffers) -> None:
+    """The forward's first kernel alone: inside-bits into buf.workspace (and
+    the count/scan state cleared)."""
+    nx, ny, nz = _check_field(field)
+    _lib.call("dax_hip_mc_bits", field.data_ptr(), nx, ny, nz, float(iso), buf.workspace.data_ptr(),
+              buf.workspace.numel(), _stream())
+
+
+def count_scan_into(buf: ForwardBuffers) -> None:
+    """The forward's second kernel alone (after :func:`classify_into`)."""
+    _lib.call("dax_hip_mc_scan", buf.nx, buf.ny, buf.nz, buf.workspace.data_ptr(),
+              buf.workspace.numel(), _stream())
+
+
+def generate_into(field: torch.Tensor, iso: float, buf: ForwardBuffers, origin=(0, 0, 0),
+                  spacing=(1, 1, 1), dedup: bool = True, normals: bool = True) -> None:
+    """The forward's third kernel alone (after classify and count/scan)."""
+    nx, ny, nz = _check_field(field)
+    flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and buf.normals is not None) else 0)
+    _lib.call("dax_hip_mc_generate", field.data_ptr(), nx, ny, nz, _f3(origin), _f3(spacing),
+              float(iso), flags, buf.workspace.data_ptr(), buf.workspace.numel(),
+              buf.vertices.data_ptr(), _ptr(buf.normals), buf.conn.data_ptr(),
+              _ptr(buf.edges) if dedup else None, _ptr(buf.interp_first),
+              _ptr(buf.interp_second), _ptr(buf.interp_weight), buf.cap_points, buf.cap_tris,
+              buf.counts.data_ptr(), _stream())
+
+
 def forward_phases(field: torch.Tensor, iso: float, buf: ForwardBuffers, origin=(0, 0, 0),
                    spacing=(1, 1, 1), dedup: bool = True, normals: bool = True,
                    events=None) -> None:
