@@ -73,7 +73,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tuning", type=str, default=None,
-                    help="classify_unroll,classify_vec,scan_sub,gen_tile[,gen_xcd] (skips the autotuner)")
+                    help="classify_unroll,classify_vec,scan_sub,gen_tile[,gen_xcd] "
+                         "(skips the autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the device copy kernel on 512 MiB (counter calibration)")
     ap.add_argument("--batch", type=int, default=1024, help="volumes of the batched run (0: skip)")
@@ -174,7 +175,7 @@ def single_volume(field, iso, label, args, tuning=None):
     buf = mc.ForwardBuffers(nx, ny, nz, max(U, 1), max(T, 1), normals=True, edges=True)
     if tuning:
         mc.set_tuning((nx, ny, nz), tuning)
-        tuned, tuned_ms = tuning, [0.0, 0.0, 0.0]
+        tuned, tuned_ms = mc.get_tuning((nx, ny, nz)), [0.0, 0.0, 0.0]
     else:
         tuned, tuned_ms = mc.autotune(field, iso, buf, reps=10)
     for _ in range(args.warmup):
@@ -391,7 +392,7 @@ def main():
                                 "median": round(statistics.median(r["steps_ms"]), 5),
                                 "max": round(max(r["steps_ms"]), 5)},
         "tuning": {"classify_unroll": tuned[0], "classify_vec": tuned[1], "scan_sub": tuned[2],
-                   "gen_tile": tuned[3], "gen_xcd": (list(tuned) + [0])[4],
+                   "gen_tile": tuned[3], "gen_xcd": tuned[4],
                    "autotune_ms": [round(x, 5) for x in r["tuned_ms"]]},
         "roofline": {"bound": "hbm", "scope": "forward graph: k_classify_v4, k_count_scan, k_generate",
                      "achieved": round(fwd_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -353,7 +353,7 @@ int dax_hip_mc_batch_forward(const float* d_fields, const void* h_header, void* 
       vols, scan_tiles, h->nscan, bits, bv, bt, status, ctrl);
   DAXHIP_LAUNCH_CHECK("k_batch_count_scan");
   OutPtrs o{d_vertices, d_normals, d_conn, d_edges, nullptr, nullptr, nullptr};
-  const size_t lds = GenLayout<kBatchGenTile>(kMaxWpr).bytes();
+  const size_t lds = GenLayout<kBatchGenTile>((int)h->max_wpr).bytes();
   k_batch_generate<<<dim3((unsigned)std::min<int64_t>(grid, h->ngen)), dim3(kScanThreads), lds, s>>>(
       vols, gen_tiles, h->ngen, d_fields, bits, bv, bt, ctrl, o, flags, cap_points, cap_tris,
       d_counts);

@@ -57,13 +57,29 @@ struct VolGeom {
   int64_t word_off;   // first row word of the volume
   int64_t nwords;     // wpr*ny*nz
   FastDiv div_wpr, div_ny, div_nx;
+  // Spacings that are powers of two (bit d): then x / (2h) == x * (rcp_h/2)
+  // and x / h == x * rcp_h exactly in IEEE arithmetic.
+  uint32_t pow2_h;
+  float rcp_h[3];
 };
 
 // Host: fill the division constants after nx, ny, wpr are set.
+// Host: fill the division constants after nx, ny, wpr and spacing are set.
 inline void set_geom_divs(VolGeom& g) {
   g.div_wpr = make_fast_div((uint32_t)g.wpr);
   g.div_ny = make_fast_div((uint32_t)g.ny);
   g.div_nx = make_fast_div((uint32_t)g.nx);
+  g.pow2_h = 0;
+  for (int d = 0; d < 3; ++d) {
+    const float h = g.spacing[d];
+    int e = 0;
+    const float m = frexpf(h, &e);  // h = m * 2^e, m in [0.5, 1)
+    g.rcp_h[d] = 1.0f;
+    if (h > 0.0f && m == 0.5f && e >= -120 && e <= 120) {  // h = 2^(e-1), far from the range limits
+      g.pow2_h |= 1u << d;
+      g.rcp_h[d] = ldexpf(1.0f, 1 - e);
+    }
+  }
 }
 
 // Edge e of a cell with lowest corner (i,j,k): its lower endpoint is
@@ -235,7 +251,7 @@ struct Ctrl {
 
 // Workspace layout for a word space of `nwords` row words.
 struct Workspace {
-  size_t off_bits, off_base_v, off_base_t, off_status, off_ctrl, bytes;
+  size_t off_bits, off_base_v, off_base_t, off_status, off_ctrl, off_counts, bytes;
   int64_t nwords, ntiles;
 };
 
@@ -254,6 +270,8 @@ inline Workspace workspace_layout(int64_t nwords) {
   off = off + (size_t)w.ntiles * 8;
   w.off_ctrl = off;
   off = align_up(off + sizeof(Ctrl), 256);
+  w.off_counts = off;  // packed per-word counts of the reduce-then-scan path
+  off = align_up(off + (size_t)w.ntiles * kTileWords * 4, 256);
   w.bytes = off;
   return w;
 }

@@ -29,26 +29,29 @@
 namespace daxhip {
 namespace mc {
 
+// Classify row words [w_lo, w_hi) (w_lo a multiple of 4 for the float4 form)
+// and clear zero_words words of count/scan state.
 template <int UNROLL, bool ALIGNED>
 __global__ __launch_bounds__(256) void k_classify(VolGeom g, const float* __restrict__ field,
                                                   uint64_t* __restrict__ bits,
                                                   unsigned long long* __restrict__ zero_buf,
-                                                  int64_t zero_words) {
+                                                  int64_t zero_words, int64_t w_lo, int64_t w_hi) {
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gtid < zero_words) zero_buf[gtid] = 0ull;
-  const int64_t w0 = (gtid >> 6) * UNROLL;
-  classify_words<UNROLL, ALIGNED>(g, field + g.field_off, bits + g.word_off, w0, g.nwords);
+  const int64_t w0 = w_lo + (gtid >> 6) * UNROLL;
+  classify_words<UNROLL, ALIGNED>(g, field + g.field_off, bits + g.word_off, w0, w_hi);
 }
 
 template <int UNROLL>
 __global__ __launch_bounds__(256) void k_classify_v4(VolGeom g, const float* __restrict__ field,
                                                      uint64_t* __restrict__ bits,
                                                      unsigned long long* __restrict__ zero_buf,
-                                                     int64_t zero_words) {
+                                                     int64_t zero_words, int64_t w_lo,
+                                                     int64_t w_hi) {
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gtid < zero_words) zero_buf[gtid] = 0ull;
-  const int64_t c0 = (gtid >> 6) * UNROLL;
-  classify_chunks_v4<UNROLL>(g, field + g.field_off, bits + g.word_off, c0, g.nwords);
+  const int64_t c0 = w_lo / 4 + (gtid >> 6) * UNROLL;
+  classify_chunks_v4<UNROLL>(g, field + g.field_off, bits + g.word_off, c0, w_hi);
 }
 
 template <int NT, int WPT>
@@ -69,12 +72,143 @@ __global__ __launch_bounds__(NT) void k_count_scan(VolGeom g, const uint64_t* __
                            tile * (int64_t)NT * WPT, sh, s_pk);
 }
 
+// ---- reduce-then-scan count/scan (tiles of kTileWords row words) ----------
+// 1. k_count_words: a thread per row word (consecutive threads on
+//    consecutive words): packed vertex / triangle counts, and the tile's sums
+//    in its status slot.  No inter-tile dependency.
+// 2. k_scan_tiles: one block scans the tiles' sums, continuing the totals
+//    already in the control block (so consecutive launches over consecutive
+//    tile ranges scan the whole volume), and stores each tile's prefix in
+//    its slot.
+// 3. k_apply_bases: a block per tile turns its prefix and the word counts
+//    into the vertex and triangle base of every word.
+constexpr int kCountThreads = 256;
+__global__ __launch_bounds__(kCountThreads) void k_count_words(VolGeom g,
+                                                               const uint64_t* __restrict__ b,
+                                                               uint32_t* __restrict__ counts,
+                                                               unsigned long long* __restrict__ slot,
+                                                               int64_t t_lo) {
+  __shared__ uint8_t faces[256];
+  __shared__ uint32_t s_v[kCountThreads / 64], s_t[kCountThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int i = tid; i < 256; i += kCountThreads) faces[i] = (uint8_t)(c_case_table[i] >> 60);
+  __syncthreads();
+  const int64_t tile = t_lo + blockIdx.x;
+  const int64_t plane = (int64_t)g.wpr * g.ny;
+  const int wpr = g.wpr;
+  uint32_t sv = 0, st = 0;
+#pragma unroll
+  for (int u = 0; u < kTileWords / kCountThreads; ++u) {
+    const int64_t w = tile * kTileWords + u * kCountThreads + tid;
+    uint32_t pk = 0;
+    if (w < g.nwords) {
+      const WordPos p = word_pos(g, w);
+      const bool nx1 = p.q + 1 < wpr;
+      const bool hy = w + wpr < g.nwords, hz = w + plane < g.nwords,
+                 hyz = w + plane + wpr < g.nwords;
+      const uint64_t A = b[w], A1 = nx1 ? b[w + 1] : 0ull;
+      const uint64_t B = hy ? b[w + wpr] : 0ull, B1 = (hy && nx1) ? b[w + wpr + 1] : 0ull;
+      const uint64_t C = hz ? b[w + plane] : 0ull, C1 = (hz && nx1) ? b[w + plane + 1] : 0ull;
+      const uint64_t D = hyz ? b[w + plane + wpr] : 0ull;
+      const uint64_t D1 = (hyz && nx1) ? b[w + plane + wpr + 1] : 0ull;
+      uint32_t nv, nt;
+      word_counts_regs(g, p, A, A1, B, B1, C, C1, D, D1, faces, nv, nt);
+      pk = nv | (nt << 16);
+      sv += nv;
+      st += nt;
+    }
+    counts[tile * kTileWords + u * kCountThreads + tid] = pk;
+  }
+  sv = wave_sum(sv);
+  st = wave_sum(st);
+  if (lane == 0) {
+    s_v[wv] = sv;
+    s_t[wv] = st;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t tv = 0, tt = 0;
+    for (int q = 0; q < kCountThreads / 64; ++q) {
+      tv += s_v[q];
+      tt += s_t[q];
+    }
+    slot[tile] = (unsigned long long)tv | ((unsigned long long)tt << 32);
+  }
+}
+
+constexpr int kScanTileThreads = 1024;
+__global__ __launch_bounds__(kScanTileThreads) void k_scan_tiles(unsigned long long* __restrict__ slot,
+                                                                 int64_t t_lo, int64_t t_hi,
+                                                                 Ctrl* __restrict__ ctrl) {
+  __shared__ uint32_t s_w[32];
+  const int tid = threadIdx.x;
+  uint64_t run_v = ctrl->total_v, run_t = ctrl->total_t;
+  __syncthreads();  // every thread read the running totals before thread 0 updates them
+  for (int64_t base = t_lo; base < t_hi; base += kScanTileThreads) {
+    const int64_t t = base + tid;
+    const unsigned long long a = t < t_hi ? slot[t] : 0ull;
+    const uint32_t v = (uint32_t)(a & 0xffffffffull), tt = (uint32_t)(a >> 32);
+    uint32_t tot_v, tot_t;
+    const uint32_t ex_v = block_excl_scan(v, tot_v, s_w);
+    const uint32_t ex_t = block_excl_scan(tt, tot_t, s_w + 16);
+    if (t < t_hi)
+      slot[t] = (unsigned long long)(uint32_t)(run_v + ex_v) |
+                ((unsigned long long)(uint32_t)(run_t + ex_t) << 32);
+    run_v += tot_v;
+    run_t += tot_t;
+  }
+  if (tid == 0) {
+    ctrl->total_v = run_v;
+    ctrl->total_t = run_t;
+  }
+}
+
+__global__ __launch_bounds__(kCountThreads) void k_apply_bases(VolGeom g,
+                                                               const uint32_t* __restrict__ counts,
+                                                               const unsigned long long* __restrict__ slot,
+                                                               uint32_t* __restrict__ base_v,
+                                                               uint32_t* __restrict__ base_t,
+                                                               int64_t t_lo) {
+  __shared__ uint32_t s_w[32];
+  constexpr int WPT = kTileWords / kCountThreads;
+  const int tid = threadIdx.x;
+  const int64_t tile = t_lo + blockIdx.x;
+  const unsigned long long pre = slot[tile];
+  const int64_t w0 = tile * kTileWords + (int64_t)tid * WPT;
+  uint32_t pk[WPT];
+  uint32_t sv = 0, st = 0;
+#pragma unroll
+  for (int u = 0; u < WPT; ++u) {
+    pk[u] = counts[w0 + u];  // 0 beyond the last word
+    sv += pk[u] & 0xffffu;
+    st += pk[u] >> 16;
+  }
+  uint32_t tot_v, tot_t;
+  uint32_t rv = (uint32_t)(pre & 0xffffffffull) + block_excl_scan(sv, tot_v, s_w);
+  uint32_t rt = (uint32_t)(pre >> 32) + block_excl_scan(st, tot_t, s_w + 16);
+#pragma unroll
+  for (int u = 0; u < WPT; ++u) {
+    const int64_t w = w0 + u;
+    if (w < g.nwords) {
+      base_v[g.word_off + w] = rv;
+      base_t[g.word_off + w] = rt;
+    }
+    rv += pk[u] & 0xffffu;
+    rt += pk[u] >> 16;
+  }
+  if (w0 < g.nwords && g.nwords <= w0 + WPT) {  // owner of the last word: entry nwords
+    base_v[g.word_off + g.nwords] = rv;
+    base_t[g.word_off + g.nwords] = rt;
+  }
+}
+
 template <int TW>
 __global__ __launch_bounds__(kScanThreads) void k_generate(
     VolGeom g, const float* __restrict__ field, const uint64_t* __restrict__ bits,
     const uint32_t* __restrict__ base_v, const uint32_t* __restrict__ base_t,
     const Ctrl* __restrict__ ctrl, OutPtrs out, unsigned flags, int64_t cap_points,
-    int64_t cap_tris, int64_t* __restrict__ counts, int xcd_order) {
+    int64_t cap_tris, int64_t* __restrict__ counts, int xcd_order, int64_t tile_lo,
+    int64_t ntiles) {
   extern __shared__ uint64_t s_dyn[];
   const bool dedup = (flags & DAX_HIP_MC_DEDUP) != 0;
   const bool normals = (flags & DAX_HIP_MC_NORMALS) != 0 && out.normals != nullptr;
@@ -86,8 +220,7 @@ __global__ __launch_bounds__(kScanThreads) void k_generate(
   }
   if (total_v > 0x7fffffffull || 3ull * total_t > 0x7fffffffull) return;
   if ((int64_t)total_t > cap_tris || (int64_t)n_points > cap_points) return;
-  const int64_t ntiles = (g.nwords + TW - 1) / TW;
-  const int64_t tile = xcd_order ? xcd_tile(blockIdx.x, ntiles) : (int64_t)blockIdx.x;
+  const int64_t tile = tile_lo + (xcd_order ? xcd_tile(blockIdx.x, ntiles) : (int64_t)blockIdx.x);
   generate_tile<TW>(g, field + g.field_off, bits + g.word_off, base_v + g.word_off,
                     base_t + g.word_off, tile * TW, out, dedup, normals, 0, 0, s_dyn);
 }
@@ -117,7 +250,8 @@ Tuning tuning_for(int64_t nx, int64_t ny, int64_t nz) {
 bool valid_tuning(const Tuning& t) {
   const bool cls = (t.cls_vec == 1 && (t.cls_unroll == 8 || t.cls_unroll == 16 || t.cls_unroll == 32)) ||
                    (t.cls_vec == 4 && (t.cls_unroll == 2 || t.cls_unroll == 4 || t.cls_unroll == 8));
-  const bool scan = t.scan_sub == 1 || t.scan_sub == 2 || t.scan_sub == 4 || t.scan_sub == 8;
+  const bool scan = t.scan_sub == 0 || t.scan_sub == 1 || t.scan_sub == 2 || t.scan_sub == 4 ||
+                    t.scan_sub == 8;
   const bool gen = (t.gen_tile == 128 || t.gen_tile == 256 || t.gen_tile == 512 ||
                     t.gen_tile == 1024) &&
                    (t.gen_xcd == 0 || t.gen_xcd == 1);
@@ -283,21 +417,29 @@ int check_ws(const VolGeom& g, size_t ws_bytes, Workspace& L, void* ws) {
   return DAX_HIP_OK;
 }
 
+// Classify row words [w_lo, w_hi) (all when w_hi < 0); `zero` also clears
+// the count/scan state.
 int launch_bits(const VolGeom& g, const float* field, char* ws, const Workspace& L,
-                const Tuning& tu, hipStream_t s) {
+                const Tuning& tu, hipStream_t s, int64_t w_lo = 0, int64_t w_hi = -1,
+                bool zero = true) {
+  if (w_hi < 0) w_hi = g.nwords;
   uint64_t* bits = (uint64_t*)(ws + L.off_bits);
   unsigned long long* status = (unsigned long long*)(ws + L.off_status);
-  const int64_t zero_words = L.ntiles + (int64_t)(sizeof(Ctrl) / 8);
+  const int64_t zero_words = zero ? L.ntiles + (int64_t)(sizeof(Ctrl) / 8) : 0;
   const bool aligned = (g.nx % 64) == 0;
-  const bool aligned16 = aligned && (g.field_off % 4) == 0 && ((uintptr_t)field % 16) == 0;
+  const bool aligned16 = aligned && (g.field_off % 4) == 0 && ((uintptr_t)field % 16) == 0 &&
+                         (w_lo % 4) == 0;
   const int vec = aligned16 ? tu.cls_vec : 1;
   const int unroll = (vec == 1 && tu.cls_vec != 1) ? 16 : tu.cls_unroll;
-  const int64_t units = vec == 4 ? ceil_div(g.nwords, 4) : g.nwords;  // chunks or words
+  const int64_t units = vec == 4 ? ceil_div(w_hi - w_lo, 4) : w_hi - w_lo;  // chunks or words
   const int64_t waves = ceil_div(units, unroll);
   int64_t blocks = std::max<int64_t>(ceil_div(waves * 64, 256), ceil_div(zero_words, 256));
+  blocks = std::max<int64_t>(blocks, 1);
   const dim3 grid((unsigned)blocks), block(256);
-#define DAXHIP_CLS(U, A) k_classify<U, A><<<grid, block, 0, s>>>(g, field, bits, status, zero_words)
-#define DAXHIP_CLS4(U) k_classify_v4<U><<<grid, block, 0, s>>>(g, field, bits, status, zero_words)
+#define DAXHIP_CLS(U, A) \
+  k_classify<U, A><<<grid, block, 0, s>>>(g, field, bits, status, zero_words, w_lo, w_hi)
+#define DAXHIP_CLS4(U) \
+  k_classify_v4<U><<<grid, block, 0, s>>>(g, field, bits, status, zero_words, w_lo, w_hi)
   if (vec == 4) {
     if (unroll == 2) DAXHIP_CLS4(2);
     else if (unroll == 4) DAXHIP_CLS4(4);
@@ -317,10 +459,28 @@ int launch_bits(const VolGeom& g, const float* field, char* ws, const Workspace&
   return DAX_HIP_OK;
 }
 
-int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu, hipStream_t s) {
-  const int sub = tu.scan_sub;  // scan tile = sub * 1024 row words
-  const int64_t ntiles = ceil_div(g.nwords, (int64_t)sub * kTileWords);
-  const dim3 grid((unsigned)std::max<int64_t>(ntiles, 1));
+// Count/scan tiles [t_lo, t_hi) (all when t_hi < 0).  Tiles take their ids
+// from the state's counter in launch order, so consecutive launches over
+// consecutive ranges cover the tiles in order.
+int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu, hipStream_t s,
+                int64_t t_lo = 0, int64_t t_hi = -1) {
+  const int sub = tu.scan_sub;  // scan tile = sub * 1024 row words; 0: reduce-then-scan
+  const int64_t ntiles = ceil_div(g.nwords, (int64_t)std::max(sub, 1) * kTileWords);
+  if (t_hi < 0) t_hi = ntiles;
+  const dim3 grid((unsigned)std::max<int64_t>(t_hi - t_lo, 1));
+  if (sub == 0) {
+    uint32_t* counts = (uint32_t*)(ws + L.off_counts);
+    unsigned long long* slot = (unsigned long long*)(ws + L.off_status);
+    k_count_words<<<grid, dim3(kCountThreads), 0, s>>>(g, (const uint64_t*)(ws + L.off_bits),
+                                                        counts, slot, t_lo);
+    DAXHIP_LAUNCH_CHECK("k_count_words");
+    k_scan_tiles<<<1, dim3(kScanTileThreads), 0, s>>>(slot, t_lo, t_hi, (Ctrl*)(ws + L.off_ctrl));
+    DAXHIP_LAUNCH_CHECK("k_scan_tiles");
+    k_apply_bases<<<grid, dim3(kCountThreads), 0, s>>>(g, counts, slot, (uint32_t*)(ws + L.off_base_v),
+                                                        (uint32_t*)(ws + L.off_base_t), t_lo);
+    DAXHIP_LAUNCH_CHECK("k_apply_bases");
+    return DAX_HIP_OK;
+  }
   const uint64_t* bits = (const uint64_t*)(ws + L.off_bits);
   uint32_t* bv = (uint32_t*)(ws + L.off_base_v);
   uint32_t* bt = (uint32_t*)(ws + L.off_base_t);
@@ -336,37 +496,50 @@ int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu
   return DAX_HIP_OK;
 }
 
+// Generate tiles of TW words [t_lo, t_hi) (all when t_hi < 0).
 template <int TW>
 int launch_generate_tw(const VolGeom& g, const float* field, const char* ws, const Workspace& L,
                        unsigned flags, const OutPtrs& o, int64_t cap_points, int64_t cap_tris,
-                       int64_t* counts, int xcd, hipStream_t s) {
+                       int64_t* counts, int xcd, hipStream_t s, int64_t t_lo, int64_t t_hi) {
   const size_t lds = GenLayout<TW>(g.wpr).bytes();
-  const int64_t ntiles = ceil_div(g.nwords, (int64_t)TW);
-  k_generate<TW><<<dim3((unsigned)std::max<int64_t>(ntiles, 1)), dim3(kScanThreads), lds, s>>>(
+  if (t_hi < 0) t_hi = ceil_div(g.nwords, (int64_t)TW);
+  const int64_t ntiles = std::max<int64_t>(t_hi - t_lo, 1);
+  k_generate<TW><<<dim3((unsigned)ntiles), dim3(kScanThreads), lds, s>>>(
       g, field, (const uint64_t*)(ws + L.off_bits), (const uint32_t*)(ws + L.off_base_v),
       (const uint32_t*)(ws + L.off_base_t), (const Ctrl*)(ws + L.off_ctrl), o, flags, cap_points,
-      cap_tris, counts, xcd);
+      cap_tris, counts, xcd, t_lo, ntiles);
   DAXHIP_LAUNCH_CHECK("k_generate");
   return DAX_HIP_OK;
 }
 
 int launch_generate(const VolGeom& g, const float* field, const char* ws, const Workspace& L,
                     unsigned flags, const OutPtrs& o, int64_t cap_points, int64_t cap_tris,
-                    int64_t* counts, const Tuning& tu, hipStream_t s) {
+                    int64_t* counts, const Tuning& tu, hipStream_t s, int64_t t_lo = 0,
+                    int64_t t_hi = -1) {
   switch (tu.gen_tile) {
     case 128:
       return launch_generate_tw<128>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
-                                     tu.gen_xcd, s);
+                                     tu.gen_xcd, s, t_lo, t_hi);
     case 256:
       return launch_generate_tw<256>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
-                                     tu.gen_xcd, s);
+                                     tu.gen_xcd, s, t_lo, t_hi);
     case 1024:
       return launch_generate_tw<1024>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
-                                      tu.gen_xcd, s);
+                                      tu.gen_xcd, s, t_lo, t_hi);
     default:
       return launch_generate_tw<512>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
-                                     tu.gen_xcd, s);
+                                     tu.gen_xcd, s, t_lo, t_hi);
   }
+}
+
+// The three kernels of a forward over the whole volume.
+int launch_forward(const VolGeom& g, const float* field, char* ws, const Workspace& L,
+                   unsigned flags, const OutPtrs& o, int64_t cap_points, int64_t cap_tris,
+                   int64_t* counts, const Tuning& tu, hipStream_t s) {
+  int rc;
+  if ((rc = launch_bits(g, field, ws, L, tu, s))) return rc;
+  if ((rc = launch_scan(g, ws, L, tu, s))) return rc;
+  return launch_generate(g, field, ws, L, flags, o, cap_points, cap_tris, counts, tu, s);
 }
 
 int check_outputs(const VolGeom& g, const float* field, unsigned flags, const float* vertices,
@@ -481,11 +654,9 @@ int dax_hip_mc_forward(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)d_workspace;
   const Tuning tu = tuning_for(nx, ny, nz);
-  if ((rc = launch_bits(g, d_field, ws, L, tu, s))) return rc;
-  if ((rc = launch_scan(g, ws, L, tu, s))) return rc;
   OutPtrs o{d_vertices, d_normals, d_conn, d_edges, d_interp_first, d_interp_second,
             d_interp_weight};
-  return launch_generate(g, d_field, ws, L, flags, o, cap_points, cap_tris, d_counts, tu, s);
+  return launch_forward(g, d_field, ws, L, flags, o, cap_points, cap_tris, d_counts, tu, s);
 }
 
 int dax_hip_mc_read_counts(const int64_t* d_counts, int64_t cap_points, int64_t cap_tris,
@@ -710,7 +881,7 @@ int dax_hip_mc_autotune(const float* d_field, int64_t nx, int64_t ny, int64_t nz
     }
   }
   // scan variants (each timed launch follows a classify that resets its state)
-  for (int sub : {1, 2, 4, 8}) {
+  for (int sub : {0, 1, 2, 4, 8}) {
     Tuning t = best;
     t.scan_sub = sub;
     float total = 0.0f;

@@ -355,7 +355,13 @@ __device__ __forceinline__ void point_gradient(const VolGeom& g, const float* __
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const float h = g.spacing[d];
-    out[d] = n[d] < 2 ? 0.0f : (hi_v[d] - lo_v[d]) / (wide[d] ? 2.0f * h : h);
+    const float diff = hi_v[d] - lo_v[d];
+    if (n[d] < 2)
+      out[d] = 0.0f;
+    else if ((g.pow2_h >> d) & 1u)  // exact: the same value as the division
+      out[d] = diff * (wide[d] ? 0.5f * g.rcp_h[d] : g.rcp_h[d]);
+    else
+      out[d] = diff / (wide[d] ? 2.0f * h : h);
   }
 }
 
@@ -425,14 +431,17 @@ __device__ __forceinline__ int select_bit(uint64_t m, int k) {
 constexpr int kMaxGenThreads = 256;
 template <int TW>
 struct GenLayout {
-  int segb, segv;
-  __host__ __device__ GenLayout(int wpr) : segb(TW + 2 * wpr + 2), segv(TW + wpr + 2) {}
-  // u64: bits (3 planes), case table, own-word flags fx/fy/fz, active cells,
-  // packed positions; u32: vertex bases (2 planes), triangle bases,
-  // active-cell slots, scan scratch
+  static constexpr int kRoundTris = 5 * kMaxGenThreads;  // triangles of one round of cells
+  int segb, segv, segf;
+  __host__ __device__ GenLayout(int wpr)
+      : segb(TW + 2 * wpr + 2), segv(TW + wpr + 2), segf(TW + wpr + 1) {}
+  // u64: bits (3 planes), case table, edge flags fx/fy/fz of the tile's words
+  // and the next row, active cells, packed positions; u32: vertex bases
+  // (2 planes), triangle bases, active-cell slots, scan scratch, round
+  // offsets and cell info, the round's triangle -> cell map
   __host__ __device__ size_t bytes() const {
-    return (size_t)(3 * segb + 256 + 5 * TW) * 8 +
-           (size_t)(2 * segv + 2 * (TW + 1) + 32 + 2 * kMaxGenThreads + 1) * 4;
+    return (size_t)(3 * segb + 256 + 3 * segf + 2 * TW) * 8 +
+           (size_t)(2 * segv + 2 * (TW + 1) + 32 + 2 * kMaxGenThreads + 1 + kRoundTris) * 4;
   }
 };
 
@@ -466,15 +475,17 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total,
 
 // One generate tile: words [w_begin, w_begin + TW) of a volume.
 //   1. stage bits (planes k, k+1, k+2), vertex bases (k, k+1), triangle bases
-//   2. per word: edge flags, active-cell mask, position; block scan of the
-//      active-cell counts gives every active cell a slot
+//   2. per word: edge flags (also of the next row, which the tile's cells
+//      reference), active-cell mask, position; block scan of the active-cell
+//      counts gives every active cell a slot
 //   3. one thread per output vertex: word by binary search over the vertex
 //      bases, edge by popcount select; coordinates, normal, records
-//   4. one thread per active cell: word by binary search over the slots,
-//      cell by popcount select; a block scan of NumFaces gives its first
-//      triangle; its triangles' corners become vertex ids through the edge
-//      flags of the neighbouring words
-// No per-thread loop depends on how many cells or edges a word holds.
+//   4. rounds of up to blockDim active cells: one thread per cell (word by
+//      binary search over the slots, cell by popcount select) forms its
+//      case; a block scan of NumFaces gives its first triangle and the cell
+//      marks its triangles' entries of the round's triangle -> cell map; one
+//      thread per triangle turns the triangle's edges into vertex ids through
+//      the edge flags (staged for the tile's words and the next row)
 template <int TW>
 __device__ __forceinline__ void generate_tile(const VolGeom& g, const float* __restrict__ f,
                                               const uint64_t* __restrict__ b,
@@ -495,15 +506,18 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const float* __r
   uint64_t* s_b2 = s_b1 + L.segb;
   uint64_t* s_case = s_b2 + L.segb;
   uint64_t* s_fx = s_case + 256;
-  uint64_t* s_fy = s_fx + TW;
-  uint64_t* s_fz = s_fy + TW;
-  uint64_t* s_act = s_fz + TW;
+  uint64_t* s_fy = s_fx + L.segf;
+  uint64_t* s_fz = s_fy + L.segf;
+  uint64_t* s_act = s_fz + L.segf;
   uint64_t* s_pos = s_act + TW;
   uint32_t* s_v0 = reinterpret_cast<uint32_t*>(s_pos + TW);
   uint32_t* s_v1 = s_v0 + L.segv;
   uint32_t* s_t = s_v1 + L.segv;
   uint32_t* s_cs = s_t + TW + 1;
-  uint32_t* s_w = s_cs + TW + 1;  // scan scratch (<= 16 waves)
+  uint32_t* s_w = s_cs + TW + 1;           // scan scratch (<= 16 waves)
+  uint32_t* s_rfirst = s_w + 32;            // [nt + 1]
+  uint32_t* s_rinfo = s_rfirst + nt + 1;    // [nt]: li << 16 | bit << 8 | case
+  uint32_t* s_tcell = s_rinfo + kMaxGenThreads;  // [kRoundTris]
   uint64_t* s_bz[3] = {s_b0, s_b1, s_b2};
   uint32_t* s_vz[2] = {s_v0, s_v1};
 
@@ -553,6 +567,20 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const float* __r
       }
     }
   }
+  // edge flags of the words after the tile up to one row on (their vertices
+  // are corners of the tile's cells)
+  for (int li = nw + tid; li < nw + g.wpr + 1; li += nt) {
+    const int64_t w = w_begin + li;
+    uint64_t fx = 0ull, fy = 0ull, fz = 0ull;
+    if (w < g.nwords) {
+      const WordPos p = word_pos(g, w);
+      const bool nx1 = p.q + 1 < g.wpr;
+      edge_flags(p, s_b0[li], nx1 ? s_b0[li + 1] : 0ull, s_b0[li + g.wpr], s_b1[li], fx, fy, fz);
+    }
+    s_fx[li] = fx;
+    s_fy[li] = fy;
+    s_fz[li] = fz;
+  }
   uint32_t n_cells;
   {
     uint32_t run = block_excl_scan(ncell, n_cells, s_w);
@@ -593,11 +621,7 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const float* __r
     }
   }
 
-  // 4. triangles: rounds of up to blockDim active cells; a block scan of
-  //    NumFaces gives each cell its first triangle, then one thread per
-  //    triangle finds its cell by binary search over those offsets
-  uint32_t* s_rfirst = s_cs + TW + 1 + 32;  // [nt + 1]
-  uint32_t* s_rinfo = s_rfirst + nt + 1;    // [nt]: li << 16 | bit << 8 | case
+  // 4. triangles
   uint32_t carry = 0;
   for (uint32_t c0 = 0; c0 < n_cells; c0 += (uint32_t)nt) {  // uniform
     const uint32_t slot = c0 + (uint32_t)tid;
@@ -625,12 +649,12 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const float* __r
       s_rinfo[tid] = ((uint32_t)li << 16) | ((uint32_t)bit << 8) | (uint32_t)cs;
     }
     uint32_t round_tris;
-    s_rfirst[tid] = block_excl_scan(nf, round_tris, s_w);
-    if (tid == 0) s_rfirst[nt] = round_tris;
+    const uint32_t first = block_excl_scan(nf, round_tris, s_w);
+    for (uint32_t t = 0; t < nf; ++t) s_tcell[first + t] = (uint32_t)tid;  // <= 5 per cell
+    s_rfirst[tid] = first;
     __syncthreads();
-    const int ncell_round = (int)min((uint32_t)nt, n_cells - c0);
     for (uint32_t tl = (uint32_t)tid; tl < round_tris; tl += (uint32_t)nt) {
-      const int c = find_word(s_rfirst, ncell_round, tl);
+      const int c = (int)s_tcell[tl];
       const uint32_t info = s_rinfo[c];
       const int li = (int)(info >> 16), bit = (int)((info >> 8) & 0xff), cs = (int)(info & 0xff);
       const int visit = (int)(tl - s_rfirst[c]);
@@ -647,9 +671,9 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const float* __r
         const int x = q * 64 + bit + dx;
         if (dedup) {
           const int qq = x >> 6, bb = x & 63;
-          const int lw2 = li + (qq - q) + dy * g.wpr;
+          const int lw2 = li + (qq - q) + dy * g.wpr;  // <= nw + wpr
           uint64_t fx, fy, fz;
-          if (dz == 0 && lw2 < nw) {
+          if (dz == 0) {
             fx = s_fx[lw2];
             fy = s_fy[lw2];
             fz = s_fz[lw2];
@@ -663,8 +687,7 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const float* __r
             p2.pmask = low_mask(g.nx - qq * 64);
             p2.xmask = low_mask(g.nx - 1 - qq * 64);
             const bool nx2 = qq + 1 < g.wpr;
-            const uint64_t* sb = s_bz[dz];
-            edge_flags(p2, sb[lw2], nx2 ? sb[lw2 + 1] : 0ull, sb[lw2 + g.wpr], s_bz[dz + 1][lw2],
+            edge_flags(p2, s_b1[lw2], nx2 ? s_b1[lw2 + 1] : 0ull, s_b1[lw2 + g.wpr], s_b2[lw2],
                        fx, fy, fz);
           }
           ids[v] =
@@ -693,7 +716,7 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const float* __r
       out.conn[3 * t + 2] = ids[2];
     }
     carry += round_tris;
-    __syncthreads();  // s_rfirst / s_rinfo reused by the next round
+    __syncthreads();  // s_rfirst / s_rinfo / s_tcell reused by the next round
   }
 }
 
