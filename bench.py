@@ -154,6 +154,10 @@ def _graph_ms(fn, steps, stream):
         mc.destroy_graph(g)
 
 
+def _progress(msg):
+    print(f"[bench] {time.strftime('%H:%M:%S')} {msg}", file=sys.stderr, flush=True)
+
+
 def _upstream(U, seed):
     import torch
     gen = torch.Generator(device="cuda").manual_seed(seed)
@@ -173,6 +177,7 @@ def single_volume(field, iso, label, args, tuning=None):
     T, U = mc.read_counts(cnt)
     del ws
     buf = mc.ForwardBuffers(nx, ny, nz, max(U, 1), max(T, 1), normals=True, edges=True)
+    _progress(f"{label}: classified; tuning")
     if tuning:
         mc.set_tuning((nx, ny, nz), tuning)
         tuned, tuned_ms = mc.get_tuning((nx, ny, nz)), [0.0, 0.0, 0.0]
@@ -192,6 +197,7 @@ def single_volume(field, iso, label, args, tuning=None):
         ms = _timed(lambda: mc.launch_graph(g_fwd, s), 1) / args.steps
         wall = time.perf_counter() - wall0
     mc.destroy_graph(g_fwd)
+    _progress(f"{label}: forward graph {ms:.4f} ms per step")
     steps_ms = []  # the same K forwards launched one by one from Python
     loop_ms = _timed(lambda: mc.forward_into(field, iso, buf), args.steps, steps_ms)
     assert mc.read_counts(buf.counts, buf.cap_points, buf.cap_tris) == (T, U)
@@ -204,6 +210,7 @@ def single_volume(field, iso, label, args, tuning=None):
                      args.steps, s) - t_cls
     t_gen = _graph_ms(lambda: mc.generate_into(field, iso, buf), args.steps, s)
     k_ms = [t_cls, t_cs, t_gen]
+    _progress(f"{label}: kernels " + ", ".join(f"{x:.4f}" for x in k_ms) + " ms")
     r = {"P": P, "C": C, "T": T, "U": U, "ms": ms, "wall": wall, "k_ms": k_ms,
          "tuned": tuned, "tuned_ms": tuned_ms, "steps_ms": steps_ms, "loop_ms": loop_ms}
     if not args.no_bwd:
@@ -217,6 +224,8 @@ def single_volume(field, iso, label, args, tuning=None):
             mc.backward_into(field, iso, buf, dv, dn, scratch, dldf)
         for _ in range(2):
             fb()
+        torch.cuda.synchronize()
+        _progress(f"{label}: eager forward+backward done")
         g_fb = mc.capture(lambda: [fb() for _ in range(args.steps)], s)
         g_b = mc.capture(lambda: [mc.backward_into(field, iso, buf, dv, dn, scratch, dldf)
                                   for _ in range(args.steps)], s)

@@ -138,19 +138,40 @@ __device__ __forceinline__ void vertex_adjoint(const VolGeom& g, const float* __
 // |dq| <= 1, -2 <= dy, dz <= 1 from its own word, inside its segment.
 constexpr int kGatherTile = 256;     // row words per tile
 constexpr int kGatherThreads = 512;  // threads per tile
-// Endpoint records a tile keeps (2 per thread, held in registers until the
+// Endpoint records a tile keeps (4 per thread, held in registers until the
 // flags are no longer needed, then written over them).
-constexpr int kRecPerThread = 2;
+constexpr int kRecPerThread = 4;
 constexpr int kMaxEndpoints = kRecPerThread * kGatherThreads;
 
-__host__ __device__ inline int gather_seg(int wpr) { return kGatherTile + 4 * wpr + 2; }
+__host__ __device__ constexpr int gather_seg(int wpr) { return kGatherTile + 4 * wpr + 2; }
 // G-words: the tile's words with one row above and below in its plane, and
 // the tile's words in the planes below and above.
-__host__ __device__ inline int gather_gwords(int nw, int wpr) { return 3 * nw + 2 * wpr; }
-__host__ __device__ inline size_t gather_lds_bytes(int wpr) {
+__host__ __device__ constexpr int gather_gwords(int nw, int wpr) { return 3 * nw + 2 * wpr; }
+// Dynamic LDS of a gather tile: the staged flags and vertex bases (later the
+// endpoint records), the G-word endpoint masks and their exclusive scan, and
+// the G-word of each endpoint slot.
+__host__ __device__ constexpr size_t gather_flags_bytes(int wpr) {
   const size_t flags = (size_t)4 * gather_seg(wpr) * (3 * sizeof(uint64_t) + sizeof(uint32_t));
+  const size_t recs = (size_t)kMaxEndpoints * 16;
+  return flags > recs ? flags : recs;
+}
+__host__ __device__ constexpr size_t gather_lds_bytes(int wpr) {
   const size_t ng = (size_t)gather_gwords(kGatherTile, wpr);
-  return flags + ng * sizeof(uint64_t) + (ng + 1) * sizeof(uint32_t);
+  return gather_flags_bytes(wpr) + ng * sizeof(uint64_t) + (ng + 1) * sizeof(uint32_t) +
+         (size_t)kMaxEndpoints * sizeof(uint16_t);
+}
+
+// Part `part` (0..3) of the test whether the tile at word w0 stages any
+// active edge: segment dz = part - 2 holds one (the vertex bases count the
+// active edges of a word range).  A tile is quiet when no part holds one.
+__device__ __forceinline__ bool gather_segment_active(const VolGeom& g,
+                                                      const uint32_t* __restrict__ base_v,
+                                                      int64_t w0, int part) {
+  const int64_t plane = (int64_t)g.wpr * g.ny;
+  const int64_t ws0 = w0 + (part - 2) * plane - 2 * g.wpr - 1;
+  const int64_t ws = max(ws0, (int64_t)0);
+  const int64_t we = min(ws0 + gather_seg(g.wpr), g.nwords);
+  return we > ws && base_v[we] != base_v[ws];
 }
 
 struct Stage {
@@ -322,6 +343,12 @@ struct GatherShared {
   uint32_t scan[16];
 };
 
+// Widest rows the gather stages (nx <= 15872): its LDS stays within a CU's
+// 160 KiB.
+constexpr int kMaxGatherWpr = 248;
+static_assert(gather_lds_bytes(kMaxGatherWpr) + sizeof(GatherShared) <= 160 * 1024,
+              "gather tile LDS");
+
 // Staging of one flag slot: the word's inside-bits and those it needs.
 struct SlotLoad {
   uint64_t A, A1, B, C;
@@ -355,15 +382,9 @@ __device__ __forceinline__ void gather_tile(const VolGeom& g, const uint64_t* __
   const int seg = gather_seg(wpr);
   const int64_t plane = (int64_t)wpr * g.ny;
 
-  // 0. a tile with no active edge among its staged words (the vertex bases
-  //    count the edges of a word range) has no receiving point: zeros only
-  bool active = false;
-  if (tid < 4) {
-    const int64_t ws = max(w0 + (tid - 2) * plane - 2 * wpr - 1, (int64_t)0);
-    const int64_t we = min(w0 + (tid - 2) * plane - 2 * wpr - 1 + seg, g.nwords);
-    active = we > ws && base_v[we] != base_v[ws];
-  }
-  if (!__syncthreads_or(active)) {
+  // 0. a tile with no active edge among its staged words has no receiving
+  //    point: zeros only
+  if (!__syncthreads_or(tid < 4 && gather_segment_active(g, base_v, w0, tid))) {
     zero_run(g, out, w0, w_end, tid, kGatherThreads);
     return;
   }
@@ -375,8 +396,10 @@ __device__ __forceinline__ void gather_tile(const VolGeom& g, const uint64_t* __
   uint64_t* s_fz = s_fy + 4 * seg;
   uint32_t* s_bv = reinterpret_cast<uint32_t*>(s_fz + 4 * seg);
   const int ng = gather_gwords(nw, wpr);
-  uint64_t* s_gm = reinterpret_cast<uint64_t*>(s_bv + 4 * seg);
+  uint64_t* s_gm =
+      reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(lds) + gather_flags_bytes(wpr));
   uint32_t* s_gp = reinterpret_cast<uint32_t*>(s_gm + gather_gwords(kGatherTile, wpr));
+  uint16_t* s_eg = reinterpret_cast<uint16_t*>(s_gp + gather_gwords(kGatherTile, wpr) + 1);
   constexpr int kSlots = 3;
   for (int base = 0; base < 4 * seg; base += kSlots * kGatherThreads) {
     SlotLoad L[kSlots];
@@ -480,7 +503,10 @@ __device__ __forceinline__ void gather_tile(const VolGeom& g, const uint64_t* __
       const int gi = tid * ch + u;
       if (gi >= ng) break;
       s_gp[gi] = run;
-      run += (uint32_t)popc(s_gm[gi]);
+      const uint32_t c = (uint32_t)popc(s_gm[gi]);
+      for (uint32_t t = run; t < run + c && t < (uint32_t)kMaxEndpoints; ++t)
+        s_eg[t] = (uint16_t)gi;  // G-word of endpoint slot t
+      run += c;
     }
     if (tid == 0) s_gp[ng] = n_end;
     __syncthreads();
@@ -495,7 +521,7 @@ __device__ __forceinline__ void gather_tile(const VolGeom& g, const uint64_t* __
       const uint32_t e = (uint32_t)tid + (uint32_t)u * kGatherThreads;
       rr[u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
       if (e < n_end) {
-        const int gi = find_word(s_gp, ng, e);
+        const int gi = s_eg[e];
         const int bit = select_bit(s_gm[gi], (int)(e - s_gp[gi]));
         const int dzp = gi < nw + 2 * wpr ? 0 : (gi < 2 * nw + 2 * wpr ? -1 : 1);
         const int64_t wt = dzp == 0 ? w0 - wpr + gi

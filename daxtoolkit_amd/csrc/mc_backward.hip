@@ -19,15 +19,18 @@ __global__ __launch_bounds__(256) void k_vertex_adjoint(VolGeom g, const float* 
   vertex_adjoint(g, field, edges, dldx, dldn, u, out);
 }
 
-// One tile of kGatherTile row words per block.
+// One tile of kGatherTile row words per block, in tile order (round-robin
+// over the XCDs: a contiguous range per XCD leaves the XCDs that hold dense
+// surface working long after the others).
 __global__ __launch_bounds__(kGatherThreads) void k_point_gather(
     VolGeom g, const uint64_t* __restrict__ bits, const uint32_t* __restrict__ base_v,
     const VertexAdj* __restrict__ adj, const int64_t* __restrict__ counts, int64_t n_points,
     float* __restrict__ dldf) {
   __shared__ GatherShared sh;
   extern __shared__ __align__(16) uint64_t s_dyn[];
+  const int64_t tile = blockIdx.x;
   const int64_t n_adj = counts ? min(counts[1], n_points) : n_points;
-  const int64_t w0 = (int64_t)blockIdx.x * kGatherTile;
+  const int64_t w0 = tile * kGatherTile;
   gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj, dldf + g.field_off, w0,
               min(w0 + kGatherTile, g.nwords), sh, s_dyn);
 }
@@ -50,7 +53,8 @@ int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz
                         const int64_t* d_counts, int64_t n_points, const float* d_dl_dvertices,
                         const float* d_dl_dnormals, void* d_scratch, size_t scratch_bytes,
                         float* d_dl_dfield, void* stream) {
-  if (nx < 1 || ny < 1 || nz < 1 || nx * ny * nz > (1LL << 30) || (nx + 63) / 64 > kMaxWpr) {
+  if (nx < 1 || ny < 1 || nz < 1 || nx * ny * nz > (1LL << 30) ||
+      (nx + 63) / 64 > kMaxGatherWpr) {
     set_error("dax_hip_mc_backward: unsupported grid %lld x %lld x %lld", (long long)nx,
               (long long)ny, (long long)nz);
     return DAX_HIP_BAD_VALUE;

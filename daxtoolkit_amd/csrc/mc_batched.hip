@@ -382,6 +382,11 @@ int dax_hip_mc_batch_backward(const float* d_fields, const void* h_header,
     set_error("dax_hip_mc_batch_backward: header/workspace not prepared");
     return DAX_HIP_BAD_VALUE;
   }
+  if (h->max_wpr > kMaxGatherWpr) {
+    set_error("dax_hip_mc_batch_backward: rows of %lld points exceed the %d the adjoint stages",
+              (long long)h->max_wpr * 64, kMaxGatherWpr * 64);
+    return DAX_HIP_BAD_VALUE;
+  }
   if (n_points < 0 || scratch_bytes < (size_t)n_points * sizeof(VertexAdj) ||
       (n_points > 0 && (!d_scratch || !d_edges || !d_dl_dvertices || !d_dl_dnormals)) ||
       !d_fields || !d_dl_dfields) {
@@ -397,9 +402,9 @@ int dax_hip_mc_batch_backward(const float* d_fields, const void* h_header,
   const uint32_t* bv = (const uint32_t*)(body + h->off_base_v);
   const int grid = (int)std::min<int64_t>(persistent_blocks(4), h->ngen);
   if (n_points > 0) {
-    k_batch_vertex_adjoint<<<dim3(grid), dim3(256), 0, s>>>(vols, gen_tiles, h->ngen, d_fields, bv,
-                                                            d_edges, d_dl_dvertices, d_dl_dnormals,
-                                                            d_counts, n_points,
+    k_batch_vertex_adjoint<<<dim3(grid), dim3(256), 0, s>>>(vols, gen_tiles, h->ngen, d_fields,
+                                                            bv, d_edges, d_dl_dvertices,
+                                                            d_dl_dnormals, d_counts, n_points,
                                                             (VertexAdj*)d_scratch);
     DAXHIP_LAUNCH_CHECK("k_batch_vertex_adjoint");
   }

@@ -591,6 +591,125 @@ void oracle_mc_fd_f64(const float* field, const int64_t* dims, const float* orig
   }
 }
 
+// The same adjoint in FP32, in the gather form and operation order of the
+// device adjoint, so that its output can be compared bit for bit:
+//   per vertex (A, B = A + e_axis): t, gA, gB, m, |m|, dL/dm, dL/dt and the
+//     record (sA, GA, sB, GB) with sA = dL/dt * (-(1-t)/den), sB = dL/dt *
+//     (-t/den), GA = (1-t) dL/dm, GB = t dL/dm;
+//   per point p: S_p and G_p summed from 0 over the edges leaving p (axis
+//     0, 1, 2: lower halves) and then entering p (axis 0, 1, 2: upper
+//     halves);
+//   per point q: S_q, then the one-sided terms G_q[d] * (-1/h) on the first
+//     and G_q[d] * (1/h) on the last plane of axis d, then for d = 0, 1, 2
+//     the terms of q - e_d (+G/h next to the first plane, else +G/(2h)) and
+//     of q + e_d (-(G/h) next to the last plane, else -(G/(2h))).
+void oracle_mc_backward_f32(const float* field, const int64_t* dims, const float* origin,
+                            const float* spacing, float iso, const int32_t* first,
+                            const int32_t* second, int64_t n, const float* dldx,
+                            const float* dldn, float* dldf) {
+  const Grid g = make_grid(dims, origin, spacing);
+  const int64_t P = g.points();
+  const int64_t n3[3] = {g.nx, g.ny, g.nz};
+  const int64_t stride[3] = {1, g.nx, g.nx * g.ny};
+  auto grad = [&](int64_t p, float out[3]) {
+    const int64_t idx[3] = {p % g.nx, (p / g.nx) % g.ny, p / (g.nx * g.ny)};
+    for (int d = 0; d < 3; ++d) {
+      const float h = g.spacing[d];
+      if (n3[d] < 2)
+        out[d] = 0.0f;
+      else if (idx[d] == 0)
+        out[d] = (field[p + stride[d]] - field[p]) / h;
+      else if (idx[d] == n3[d] - 1)
+        out[d] = (field[p] - field[p - stride[d]]) / h;
+      else
+        out[d] = (field[p + stride[d]] - field[p - stride[d]]) / (2.0f * h);
+    }
+  };
+  // records and the vertex of each (point, axis) as lower / upper endpoint
+  std::vector<float> rec((size_t)n * 8);
+  std::vector<int64_t> lower((size_t)P * 3, -1), upper((size_t)P * 3, -1);
+  for (int64_t u = 0; u < n; ++u) {
+    const int64_t A = first[u], B = second[u];
+    const int axis = B - A == 1 ? 0 : (B - A == g.nx ? 1 : 2);
+    lower[3 * A + axis] = u;
+    upper[3 * B + axis] = u;
+    const float fA = field[A], fB = field[B];
+    const float den = fB - fA;
+    const float t = (iso - fA) / den;
+    float gA[3], gB[3], m[3];
+    grad(A, gA);
+    grad(B, gB);
+    for (int d = 0; d < 3; ++d) m[d] = gA[d] + t * (gB[d] - gA[d]);
+    const float len2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    const float dn[3] = {dldn[3 * u], dldn[3 * u + 1], dldn[3 * u + 2]};
+    float dm[3] = {0.0f, 0.0f, 0.0f};
+    if (len2 > 0.0f) {
+      const float len = std::sqrt(len2);
+      float nv[3];
+      for (int d = 0; d < 3; ++d) nv[d] = m[d] / len;
+      const float dot = nv[0] * dn[0] + nv[1] * dn[1] + nv[2] * dn[2];
+      for (int d = 0; d < 3; ++d) dm[d] = (dn[d] - nv[d] * dot) / len;
+    }
+    const int64_t ia[3] = {A % g.nx, (A / g.nx) % g.ny, A / (g.nx * g.ny)};
+    float dldt = 0.0f;
+    for (int d = 0; d < 3; ++d) {
+      const float a = g.origin[d] + g.spacing[d] * (float)ia[d];
+      const float b = g.origin[d] + g.spacing[d] * (float)(ia[d] + (d == axis ? 1 : 0));
+      dldt += dldx[3 * u + d] * (b - a);
+    }
+    for (int d = 0; d < 3; ++d) dldt += dm[d] * (gB[d] - gA[d]);
+    float* r = &rec[(size_t)u * 8];
+    r[0] = dldt * (-(1.0f - t) / den);
+    r[4] = dldt * (-t / den);
+    for (int d = 0; d < 3; ++d) {
+      r[1 + d] = (1.0f - t) * dm[d];
+      r[5 + d] = t * dm[d];
+    }
+  }
+  // (S, G) per point
+  std::vector<float> sg((size_t)P * 4, 0.0f);
+  for (int64_t p = 0; p < P; ++p) {
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int a = 0; a < 3; ++a) {
+      const int64_t u = lower[3 * p + a];
+      if (u >= 0)
+        for (int c = 0; c < 4; ++c) acc[c] += rec[(size_t)u * 8 + c];
+    }
+    for (int a = 0; a < 3; ++a) {
+      const int64_t u = upper[3 * p + a];
+      if (u >= 0)
+        for (int c = 0; c < 4; ++c) acc[c] += rec[(size_t)u * 8 + 4 + c];
+    }
+    for (int c = 0; c < 4; ++c) sg[(size_t)p * 4 + c] = acc[c];
+  }
+  for (int64_t q = 0; q < P; ++q) {
+    const int64_t co[3] = {q % g.nx, (q / g.nx) % g.ny, q / (g.nx * g.ny)};
+    float acc = sg[(size_t)q * 4];
+    for (int d = 0; d < 3; ++d) {
+      const float h = g.spacing[d];
+      if (n3[d] < 2) continue;
+      const float G = sg[(size_t)q * 4 + 1 + d];
+      if (co[d] == 0)
+        acc += G * (-1.0f / h);
+      else if (co[d] == n3[d] - 1)
+        acc += G * (1.0f / h);
+    }
+    for (int d = 0; d < 3; ++d) {
+      const float h = g.spacing[d];
+      if (n3[d] < 2) continue;
+      if (co[d] >= 1) {
+        const float G = sg[(size_t)(q - stride[d]) * 4 + 1 + d];
+        acc += (co[d] - 1 == 0) ? G / h : G / (2.0f * h);
+      }
+      if (co[d] <= n3[d] - 2) {
+        const float G = sg[(size_t)(q + stride[d]) * 4 + 1 + d];
+        acc += (co[d] + 1 == n3[d] - 1) ? -(G / h) : -(G / (2.0f * h));
+      }
+    }
+    dldf[q] = acc;
+  }
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -50,6 +50,7 @@ def lib():
         L.oracle_cell_gradient.argtypes = [C.c_void_p] * 5
         L.oracle_mc_backward_f64.argtypes = ([C.c_void_p] * 4 + [C.c_float] + [C.c_void_p] * 2 +
                                              [C.c_int64] + [C.c_void_p] * 3)
+        L.oracle_mc_backward_f32.argtypes = L.oracle_mc_backward_f64.argtypes
         L.oracle_mc_fd_f64.argtypes = ([C.c_void_p] * 4 + [C.c_float] + [C.c_void_p] * 2 +
                                        [C.c_int64] + [C.c_void_p] * 3 + [C.c_int64, C.c_double,
                                                                           C.c_void_p])
@@ -196,6 +197,21 @@ def backward_f64(field, iso, first, second, dldx, dldn, origin=(0, 0, 0), spacin
     dldn = np.ascontiguousarray(dldn, np.float32)
     out = np.zeros(f.size, np.float64)
     lib().oracle_mc_backward_f64(_p(f), _p(_dims(f)), _p(_f3(origin)), _p(_f3(spacing)),
+                                 float(iso), _p(first), _p(second), first.size, _p(dldx),
+                                 _p(dldn), _p(out))
+    return out
+
+
+def backward_f32(field, iso, first, second, dldx, dldn, origin=(0, 0, 0), spacing=(1, 1, 1)):
+    """dL/dfield in FP32 with the device adjoint's operation order (bit-level
+    checker; backward_f64 is the mathematical one)."""
+    f = _field(field)
+    first = np.ascontiguousarray(first, np.int32)
+    second = np.ascontiguousarray(second, np.int32)
+    dldx = np.ascontiguousarray(dldx, np.float32)
+    dldn = np.ascontiguousarray(dldn, np.float32)
+    out = np.zeros(f.size, np.float32)
+    lib().oracle_mc_backward_f32(_p(f), _p(_dims(f)), _p(_f3(origin)), _p(_f3(spacing)),
                                  float(iso), _p(first), _p(second), first.size, _p(dldx),
                                  _p(dldn), _p(out))
     return out

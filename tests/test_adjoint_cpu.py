@@ -1,5 +1,6 @@
 """The float64 torch autograd restatement agrees with the oracle's analytic
-FP64 adjoint (both are then the references of the GPU adjoint)."""
+FP64 adjoint, and the oracle's FP32 restatement of the device operation
+order agrees with both (all three are references of the GPU adjoint)."""
 import numpy as np
 import torch
 
@@ -26,3 +27,25 @@ def test_torch_autograd_matches_oracle_adjoint():
     got = f64.grad.numpy().reshape(-1)
     scale = np.abs(ref).max()
     assert np.abs(got - ref).max() <= 1e-9 * scale
+
+
+def test_fp32_restatement_matches_fp64_adjoint():
+    """FP32 (device order) vs FP64 adjoint: same mathematics, FP32 rounding
+    only; an offset origin and unequal spacings exercise every term."""
+    dims = (21, 18, 16)
+    p = fields.gaussian_mixture_params(7, dims, n_gauss=4, sigma=(3.0, 7.0))
+    f = fields.gaussian_mixture_numpy(dims, p)
+    iso, org, sp = 0.5, (0.5, 1.0, -3.0), (0.7, 1.0, 1.3)
+    r = orc.generate(f, iso, origin=org, spacing=sp)
+    U = r["n_pts"]
+    assert U > 100
+    rng = np.random.default_rng(2)
+    dldx = rng.standard_normal((U, 3)).astype(np.float32)
+    dldn = rng.standard_normal((U, 3)).astype(np.float32)
+    ref = orc.backward_f64(f, iso, r["first"], r["second"], dldx, dldn, origin=org, spacing=sp)
+    got = orc.backward_f32(f, iso, r["first"], r["second"], dldx, dldn, origin=org, spacing=sp)
+    assert got.dtype == np.float32
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-3 * scale)
+    assert err.max() <= 1e-4, err.max()
+    assert np.all(got[ref == 0.0] == 0.0) and np.all(ref[got == 0.0] == 0.0)
