@@ -357,37 +357,32 @@ struct SlotLoad {
   bool in;
 };
 
-// dL/df of the points of words [w0, w_end) of a volume (w_end - w0 <=
-// kGatherTile, blockDim.x == kGatherThreads).  Every thread of the block
-// calls this; `lds` holds gather_lds_bytes(g.wpr).
-//   0. a tile without active edges nearby only writes zeros
+// dL/df of the points of row words [w0, w_end) of a volume (a span of at
+// most kGatherTile words, blockDim.x == kGatherThreads).  Every thread of
+// the block calls this; `lds` holds gather_lds_bytes(g.wpr).
 //   1. stage flags and vertex bases of the planes k-2..k+1
-//   2. receiving masks of the tile's words (thread per word)
+//   2. receiving masks of the span's words (thread per word)
 //   3. endpoint masks of the G-words, restricted to the points some
 //      receiving point's stencil reads; block scan -> compact endpoint slots
 //   4. (S, G) of every such endpoint (thread per endpoint, 6 edge records)
 //   5. zeros for the other points (coalesced) and dL/df of every receiving
 //      point from the 7 (S, G) of its stencil; no barrier follows, so these
 //      stores drain while the next tiles run
-// A tile with more than kMaxEndpoints endpoints computes every receiving
-// point's stencil sums directly from the flags instead of 3-4.
-__device__ __forceinline__ void gather_tile(const VolGeom& g, const uint64_t* __restrict__ b,
-                                            const uint32_t* __restrict__ base_v,
-                                            const VertexAdj* __restrict__ adj, int64_t n_adj,
-                                            float* __restrict__ out, int64_t w0, int64_t w_end,
-                                            GatherShared& sh, void* lds) {
+// A span with more than kMaxEndpoints endpoints writes nothing and returns
+// its endpoint count (kDirect false), or computes every receiving point's
+// stencil sums directly from the flags instead of 3-4 (kDirect true).
+// Returns 0 when its points are written.
+template <bool kDirect>
+__device__ __forceinline__ uint32_t gather_span(const VolGeom& g, const uint64_t* __restrict__ b,
+                                                const uint32_t* __restrict__ base_v,
+                                                const VertexAdj* __restrict__ adj, int64_t n_adj,
+                                                float* __restrict__ out, int64_t w0,
+                                                int64_t w_end, GatherShared& sh, void* lds) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int nw = (int)(w_end - w0);
   const int wpr = g.wpr;
-  const int seg = gather_seg(wpr);
+  const int seg = nw + 4 * wpr + 2;  // staged words per plane
   const int64_t plane = (int64_t)wpr * g.ny;
-
-  // 0. a tile with no active edge among its staged words has no receiving
-  //    point: zeros only
-  if (!__syncthreads_or(tid < 4 && gather_segment_active(g, base_v, w0, tid))) {
-    zero_run(g, out, w0, w_end, tid, kGatherThreads);
-    return;
-  }
 
   // 1. flags and vertex bases: every load of a round is issued before any
   //    is used
@@ -512,6 +507,7 @@ __device__ __forceinline__ void gather_tile(const VolGeom& g, const uint64_t* __
     __syncthreads();
   }
   const bool shared_sums = n_end <= (uint32_t)kMaxEndpoints;  // block-uniform
+  if (!kDirect && !shared_sums) return n_end;
   f32x4* s_rec = reinterpret_cast<f32x4*>(lds);
   if (total > 0 && shared_sums) {
     // 4. (S, G) per endpoint, kept in registers until the flags are done
@@ -574,7 +570,7 @@ __device__ __forceinline__ void gather_tile(const VolGeom& g, const uint64_t* __
     const int bit = select_bit(sh.recv[i], (int)(e - sh.first[i]));
     const WordPos p = word_pos(g, w0 + i);
     const int x = p.q * 64 + bit;
-    float v;
+    float v = 0.0f;
     if (shared_sums) {
       const int g0 = i + wpr;  // G-word of the point's own word
       auto rec = [&](int gi, int bb) -> f32x4 {
@@ -591,12 +587,56 @@ __device__ __forceinline__ void gather_tile(const VolGeom& g, const uint64_t* __
         if (d == 1) return rec(g0 + s * wpr, bit);
         return rec((s < 0 ? nw + 2 * wpr : 2 * nw + 2 * wpr) + i, bit);
       });
-    } else {
+    } else if (kDirect) {
       const GatherCtx c{&g, &st, adj, n_adj, w0 + i, 0, p.q, x, p.j, p.k};
       v = point_adjoint(c);
     }
     out[(int64_t)x + (int64_t)g.nx * (p.j + (int64_t)g.ny * p.k)] = v;
   }
+  return 0;
+}
+
+// A gather tile: zeros only when no active edge lies among its staged words,
+// else the tile as one span.  Returns its endpoint count when that exceeds
+// kMaxEndpoints (nothing written; the tile goes to the span pass).
+__device__ __forceinline__ uint32_t gather_tile(const VolGeom& g, const uint64_t* __restrict__ b,
+                                                const uint32_t* __restrict__ base_v,
+                                                const VertexAdj* __restrict__ adj, int64_t n_adj,
+                                                float* __restrict__ out, int64_t w0,
+                                                int64_t w_end, GatherShared& sh, void* lds) {
+  const int tid = threadIdx.x;
+  if (!__syncthreads_or(tid < 4 && gather_segment_active(g, base_v, w0, tid))) {
+    zero_run(g, out, w0, w_end, tid, kGatherThreads);
+    return 0;
+  }
+  return gather_span<false>(g, b, base_v, adj, n_adj, out, w0, w_end, sh, lds);
+}
+
+// Tiles whose endpoints exceed kMaxEndpoints are listed by the tile pass
+// and redone as kSpansPerTile spans of kGatherTile / kSpansPerTile words
+// (a span reads about 1/kSpansPerTile of the tile's endpoints).  Every
+// listed tile holds more than kMaxEndpoints endpoint slots; a point is a
+// slot of at most 5 tiles (its own, the tiles of the words one row before
+// and after it and one plane below and above) and a volume with U vertices
+// has at most 2U endpoints, so fewer than 10U / kMaxEndpoints < U / 128 + 2
+// tiles are listed.
+constexpr int kSpansPerTile = 4;
+__host__ __device__ constexpr int64_t gather_list_capacity(int64_t n_points) {
+  return n_points / 128 + 2;
+}
+// Scratch of the adjoint: the vertex records, then the list (count, tiles).
+__host__ __device__ constexpr size_t gather_list_offset(int64_t n_points) {
+  return ((size_t)(n_points > 0 ? n_points : 0) * sizeof(VertexAdj) + 15) & ~(size_t)15;
+}
+__host__ __device__ constexpr size_t adjoint_scratch_bytes(int64_t n_points) {
+  return gather_list_offset(n_points) + (size_t)(1 + gather_list_capacity(n_points)) * 4;
+}
+
+// Appends a tile to the list (one thread of the block).
+__device__ __forceinline__ void gather_list_push(uint32_t* __restrict__ list, int64_t cap,
+                                                 uint32_t tile) {
+  const uint32_t i = atomicAdd(&list[0], 1u);
+  if (i < cap) list[1 + i] = tile;
 }
 
 }  // namespace mc

@@ -6,14 +6,17 @@
 namespace daxhip {
 namespace mc {
 
+// One thread per vertex; block 0 also empties the list of dense tiles.
 __global__ __launch_bounds__(256) void k_vertex_adjoint(VolGeom g, const float* __restrict__ field,
                                                         const uint32_t* __restrict__ edges,
                                                         const float* __restrict__ dldx,
                                                         const float* __restrict__ dldn,
                                                         const int64_t* __restrict__ counts,
                                                         int64_t n_points,
-                                                        VertexAdj* __restrict__ out) {
+                                                        VertexAdj* __restrict__ out,
+                                                        uint32_t* __restrict__ list) {
   const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u == 0) list[0] = 0u;
   const int64_t U = counts ? min(counts[1], n_points) : n_points;
   if (u >= U) return;
   vertex_adjoint(g, field, edges, dldx, dldn, u, out);
@@ -21,18 +24,47 @@ __global__ __launch_bounds__(256) void k_vertex_adjoint(VolGeom g, const float* 
 
 // One tile of kGatherTile row words per block, in tile order (round-robin
 // over the XCDs: a contiguous range per XCD leaves the XCDs that hold dense
-// surface working long after the others).
+// surface working long after the others).  A tile whose endpoint records do
+// not fit goes to the list (without records, list == nullptr, every point
+// receives 0).
 __global__ __launch_bounds__(kGatherThreads) void k_point_gather(
     VolGeom g, const uint64_t* __restrict__ bits, const uint32_t* __restrict__ base_v,
     const VertexAdj* __restrict__ adj, const int64_t* __restrict__ counts, int64_t n_points,
-    float* __restrict__ dldf) {
+    float* __restrict__ dldf, uint32_t* __restrict__ list) {
   __shared__ GatherShared sh;
   extern __shared__ __align__(16) uint64_t s_dyn[];
   const int64_t tile = blockIdx.x;
   const int64_t n_adj = counts ? min(counts[1], n_points) : n_points;
-  const int64_t w0 = tile * kGatherTile;
-  gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj, dldf + g.field_off, w0,
-              min(w0 + kGatherTile, g.nwords), sh, s_dyn);
+  const int64_t w0 = tile * kGatherTile, w_end = min(w0 + kGatherTile, g.nwords);
+  if (gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj, dldf + g.field_off, w0,
+                  w_end, sh, s_dyn)) {
+    if (!list)
+      zero_run(g, dldf + g.field_off, w0, w_end, threadIdx.x, kGatherThreads);
+    else if (threadIdx.x == 0)
+      gather_list_push(list, gather_list_capacity(n_points), (uint32_t)tile);
+  }
+}
+
+// The listed tiles as kSpansPerTile spans each (persistent blocks).
+__global__ __launch_bounds__(kGatherThreads) void k_point_gather_spans(
+    VolGeom g, const uint64_t* __restrict__ bits, const uint32_t* __restrict__ base_v,
+    const VertexAdj* __restrict__ adj, const int64_t* __restrict__ counts, int64_t n_points,
+    float* __restrict__ dldf, const uint32_t* __restrict__ list) {
+  __shared__ GatherShared sh;
+  extern __shared__ __align__(16) uint64_t s_dyn[];
+  const int64_t n_adj = counts ? min(counts[1], n_points) : n_points;
+  const int64_t n =
+      min((int64_t)list[0], gather_list_capacity(n_points)) * (int64_t)kSpansPerTile;
+  constexpr int kSpan = kGatherTile / kSpansPerTile;
+  for (int64_t it = blockIdx.x; it < n; it += gridDim.x) {
+    const int64_t tile = list[1 + it / kSpansPerTile];
+    const int64_t w0 = tile * kGatherTile + (it % kSpansPerTile) * kSpan;
+    const int64_t w_end = min(w0 + kSpan, g.nwords);
+    if (w0 < w_end)
+      gather_span<true>(g, bits + g.word_off, base_v + g.word_off, adj, n_adj,
+                        dldf + g.field_off, w0, w_end, sh, s_dyn);
+    __syncthreads();  // LDS reused by the next span
+  }
 }
 
 }  // namespace mc
@@ -41,10 +73,24 @@ __global__ __launch_bounds__(kGatherThreads) void k_point_gather(
 using namespace daxhip;
 using namespace daxhip::mc;
 
+namespace {
+
+// Persistent blocks of the span pass: two per CU.
+int gather_span_blocks() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  return 2 * cus;
+}
+
+}  // namespace
+
 extern "C" {
 
 size_t dax_hip_mc_backward_scratch_bytes(int64_t n_points) {
-  return n_points > 0 ? (size_t)n_points * sizeof(VertexAdj) : 0;
+  return n_points > 0 ? adjoint_scratch_bytes(n_points) : 0;
 }
 
 int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
@@ -87,17 +133,26 @@ int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz
   }
   hipStream_t s = (hipStream_t)stream;
   const char* ws = (const char*)d_workspace;
+  const uint64_t* bits = (const uint64_t*)(ws + L.off_bits);
+  const uint32_t* bv = (const uint32_t*)(ws + L.off_base_v);
+  uint32_t* list =
+      n_points > 0 ? (uint32_t*)((char*)d_scratch + gather_list_offset(n_points)) : nullptr;
   if (n_points > 0) {
     k_vertex_adjoint<<<dim3((unsigned)ceil_div(n_points, 256)), dim3(256), 0, s>>>(
         g, d_field, d_edges, d_dl_dvertices, d_dl_dnormals, d_counts, n_points,
-        (VertexAdj*)d_scratch);
+        (VertexAdj*)d_scratch, list);
     DAXHIP_LAUNCH_CHECK("k_vertex_adjoint");
   }
   k_point_gather<<<dim3((unsigned)ceil_div(g.nwords, kGatherTile)), dim3(kGatherThreads),
-                   gather_lds_bytes(g.wpr), s>>>(
-      g, (const uint64_t*)(ws + L.off_bits), (const uint32_t*)(ws + L.off_base_v),
-      (const VertexAdj*)d_scratch, d_counts, n_points, d_dl_dfield);
+                   gather_lds_bytes(g.wpr), s>>>(g, bits, bv, (const VertexAdj*)d_scratch,
+                                                 d_counts, n_points, d_dl_dfield, list);
   DAXHIP_LAUNCH_CHECK("k_point_gather");
+  if (n_points > 0) {
+    k_point_gather_spans<<<dim3((unsigned)gather_span_blocks()), dim3(kGatherThreads),
+                           gather_lds_bytes(g.wpr), s>>>(g, bits, bv, (const VertexAdj*)d_scratch,
+                                                         d_counts, n_points, d_dl_dfield, list);
+    DAXHIP_LAUNCH_CHECK("k_point_gather_spans");
+  }
   return DAX_HIP_OK;
 }
 

@@ -170,13 +170,16 @@ __global__ void k_batch_offsets(const VolGeom* __restrict__ vols, int64_t nvols,
 // ---- batched adjoint --------------------------------------------------------
 // Same tile table as generate: a block takes a tile of row words, first the
 // adjoint records of the tile's vertices (global ids base_v[w0] ..
-// base_v[w_end]), then, in a second launch, dL/dfield of the tile's points.
+// base_v[w_end]), then, in a second launch (a block per tile), dL/dfield of
+// the tile's points; tiles whose endpoint records do not fit are listed and
+// redone as spans by a third launch.
 __global__ __launch_bounds__(256) void k_batch_vertex_adjoint(
     const VolGeom* __restrict__ vols, const TileRef* __restrict__ tiles, int64_t ntiles,
     const float* __restrict__ fields, const uint32_t* __restrict__ base_v,
     const uint32_t* __restrict__ edges, const float* __restrict__ dldx,
     const float* __restrict__ dldn, const int64_t* __restrict__ counts, int64_t n_points,
-    VertexAdj* __restrict__ out) {
+    VertexAdj* __restrict__ out, uint32_t* __restrict__ list) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) list[0] = 0u;
   const int64_t U = counts ? min(counts[1], n_points) : n_points;
   for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
@@ -190,20 +193,46 @@ __global__ __launch_bounds__(256) void k_batch_vertex_adjoint(
 }
 
 __global__ __launch_bounds__(kGatherThreads) void k_batch_point_gather(
-    const VolGeom* __restrict__ vols, const TileRef* __restrict__ tiles, int64_t ntiles,
+    const VolGeom* __restrict__ vols, const TileRef* __restrict__ tiles,
     const uint64_t* __restrict__ bits, const uint32_t* __restrict__ base_v,
     const VertexAdj* __restrict__ adj, const int64_t* __restrict__ counts, int64_t n_points,
-    float* __restrict__ dldf) {
+    float* __restrict__ dldf, uint32_t* __restrict__ list) {
   static_assert(kGatherTile == kBatchGenTile, "gather tiles are the generate tiles");
   __shared__ GatherShared sh;
   extern __shared__ __align__(16) uint64_t s_dyn[];
   const int64_t n_adj = counts ? min(counts[1], n_points) : n_points;
-  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const TileRef tr = tiles[t];
+  const TileRef tr = tiles[blockIdx.x];
+  const VolGeom g = vols[tr.vol];
+  const int64_t w_end = min((int64_t)tr.w0 + kBatchGenTile, g.nwords);
+  if (gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj, dldf + g.field_off,
+                  tr.w0, w_end, sh, s_dyn)) {
+    if (!list)
+      zero_run(g, dldf + g.field_off, tr.w0, w_end, threadIdx.x, kGatherThreads);
+    else if (threadIdx.x == 0)
+      gather_list_push(list, gather_list_capacity(n_points), blockIdx.x);
+  }
+}
+
+__global__ __launch_bounds__(kGatherThreads) void k_batch_point_gather_spans(
+    const VolGeom* __restrict__ vols, const TileRef* __restrict__ tiles,
+    const uint64_t* __restrict__ bits, const uint32_t* __restrict__ base_v,
+    const VertexAdj* __restrict__ adj, const int64_t* __restrict__ counts, int64_t n_points,
+    float* __restrict__ dldf, const uint32_t* __restrict__ list) {
+  __shared__ GatherShared sh;
+  extern __shared__ __align__(16) uint64_t s_dyn[];
+  const int64_t n_adj = counts ? min(counts[1], n_points) : n_points;
+  const int64_t n =
+      min((int64_t)list[0], gather_list_capacity(n_points)) * (int64_t)kSpansPerTile;
+  constexpr int kSpan = kGatherTile / kSpansPerTile;
+  for (int64_t it = blockIdx.x; it < n; it += gridDim.x) {
+    const TileRef tr = tiles[list[1 + it / kSpansPerTile]];
     const VolGeom g = vols[tr.vol];
-    gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj, dldf + g.field_off, tr.w0,
-                min((int64_t)tr.w0 + kBatchGenTile, g.nwords), sh, s_dyn);
-    __syncthreads();  // LDS reused by the next tile
+    const int64_t w0 = tr.w0 + (it % kSpansPerTile) * kSpan;
+    const int64_t w_end = min(w0 + kSpan, g.nwords);
+    if (w0 < w_end)
+      gather_span<true>(g, bits + g.word_off, base_v + g.word_off, adj, n_adj,
+                        dldf + g.field_off, w0, w_end, sh, s_dyn);
+    __syncthreads();  // LDS reused by the next span
   }
 }
 
@@ -387,11 +416,12 @@ int dax_hip_mc_batch_backward(const float* d_fields, const void* h_header,
               (long long)h->max_wpr * 64, kMaxGatherWpr * 64);
     return DAX_HIP_BAD_VALUE;
   }
-  if (n_points < 0 || scratch_bytes < (size_t)n_points * sizeof(VertexAdj) ||
+  const size_t need = n_points > 0 ? adjoint_scratch_bytes(n_points) : 0;
+  if (n_points < 0 || scratch_bytes < need ||
       (n_points > 0 && (!d_scratch || !d_edges || !d_dl_dvertices || !d_dl_dnormals)) ||
       !d_fields || !d_dl_dfields) {
     set_error("dax_hip_mc_batch_backward: missing buffers or scratch smaller than %zu bytes",
-              (size_t)n_points * sizeof(VertexAdj));
+              need);
     return DAX_HIP_BAD_VALUE;
   }
   hipStream_t s = (hipStream_t)stream;
@@ -401,17 +431,26 @@ int dax_hip_mc_batch_backward(const float* d_fields, const void* h_header,
   const uint64_t* bits = (const uint64_t*)(body + h->off_bits);
   const uint32_t* bv = (const uint32_t*)(body + h->off_base_v);
   const int grid = (int)std::min<int64_t>(persistent_blocks(4), h->ngen);
+  uint32_t* list =
+      n_points > 0 ? (uint32_t*)((char*)d_scratch + gather_list_offset(n_points)) : nullptr;
+  const size_t lds = gather_lds_bytes((int)h->max_wpr);
   if (n_points > 0) {
     k_batch_vertex_adjoint<<<dim3(grid), dim3(256), 0, s>>>(vols, gen_tiles, h->ngen, d_fields,
                                                             bv, d_edges, d_dl_dvertices,
                                                             d_dl_dnormals, d_counts, n_points,
-                                                            (VertexAdj*)d_scratch);
+                                                            (VertexAdj*)d_scratch, list);
     DAXHIP_LAUNCH_CHECK("k_batch_vertex_adjoint");
   }
-  k_batch_point_gather<<<dim3(grid), dim3(kGatherThreads), gather_lds_bytes((int)h->max_wpr), s>>>(vols, gen_tiles, h->ngen, bits, bv,
-                                                        (const VertexAdj*)d_scratch, d_counts,
-                                                        n_points, d_dl_dfields);
+  k_batch_point_gather<<<dim3((unsigned)h->ngen), dim3(kGatherThreads), lds, s>>>(
+      vols, gen_tiles, bits, bv, (const VertexAdj*)d_scratch, d_counts, n_points, d_dl_dfields,
+      list);
   DAXHIP_LAUNCH_CHECK("k_batch_point_gather");
+  if (n_points > 0) {
+    k_batch_point_gather_spans<<<dim3((unsigned)persistent_blocks(2)), dim3(kGatherThreads), lds,
+                                 s>>>(vols, gen_tiles, bits, bv, (const VertexAdj*)d_scratch,
+                                      d_counts, n_points, d_dl_dfields, list);
+    DAXHIP_LAUNCH_CHECK("k_batch_point_gather_spans");
+  }
   return DAX_HIP_OK;
 }
 
