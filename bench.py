@@ -12,13 +12,15 @@ Timed regions are the K steps captured as one HIP graph and launched once
 calls cannot leave the GPU idle; the same K steps launched one by one from
 Python are reported beside it.
 
-Prints one JSON line.  `value` is Gcells/s of the forward; `roofline` is the
-classify kernel (the dominant one) measured with HIP events on the stream the
-kernels run on; `cpu_baseline` times the OpenMP restatement of Dax's
-OpenMP-adapter pipeline on the host cores on the same field.  Beside the
-headline the line carries
+Prints one JSON line.  `value` is Gcells/s of the forward; `roofline` is
+the forward graph (its algorithmic bytes over its HIP-event time on the
+stream the kernels run on), with each kernel's own time, bytes and measured
+HBM traffic under `kernels`; `cpu_baseline` times the OpenMP restatement of
+Dax's OpenMP-adapter pipeline on the host cores on the same field.  Beside
+the headline the line carries
   fwd_bwd  the same field, forward + adjoint (upstream dL/dvertex and
            dL/dnormal N(0,1), seed 1), and its ratio to the forward;
+  fwd_bwd_256  the differentiable workload: the same at 256^3;
   batched  1024 volumes with per-axis extents U{32..160}, 4-16 Gaussians each,
            iso U[0.3,0.7]: the batched forward and forward+backward replayed
            as captured graphs, against a native launch loop of single-volume
@@ -80,7 +82,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=1024, help="volumes of the batched run (0: skip)")
     ap.add_argument("--no-bwd", action="store_true", help="skip the forward+backward timing")
     ap.add_argument("--large", action="store_true", help="also run the 1024^3 Mandelbulb")
-    ap.add_argument("--only", choices=["fwd", "batch", "large"], default=None,
+    ap.add_argument("--only", choices=["fwd", "batch", "large", "diff"], default=None,
                     help="run one section only (profiling); prints its JSON")
     return ap.parse_args()
 
@@ -302,6 +304,23 @@ def batched(args):
     }
 
 
+def differentiable_256(args):
+    """The differentiable workload: 256^3, the same 16-Gaussian field (seed
+    0), forward and forward+backward (upstream N(0,1))."""
+    import torch
+
+    from daxtoolkit_amd import fields
+    n = 256
+    field = fields.gaussian_mixture_torch((n, n, n), fields.gaussian_mixture_params(0, (n, n, n)))
+    torch.cuda.synchronize()
+    r = single_volume(field, 0.5, "256^3", args)
+    return {"workload": "differentiable marching cubes, 256^3 points, 16-Gaussian mixture (seed 0) "
+                        "+ U(-0.01,0.01) noise, iso 0.5, upstream dL/dvertex, dL/dnormal N(0,1)",
+            "triangles": r["T"], "vertices": r["U"], "fwd_ms": round(r["ms"], 5),
+            "fwd_bwd_ms": round(r["fb_ms"], 5), "bwd_ms": round(r["bwd_ms"], 5),
+            "fwd_bwd_over_fwd": round(r["fb_ms"] / r["ms"], 3), "tuning": r["tuned"]}
+
+
 def large(args):
     import torch
 
@@ -346,6 +365,9 @@ def main():
         return
     if args.only == "large":
         print(json.dumps({"large": large(args)}), flush=True)
+        return
+    if args.only == "diff":
+        print(json.dumps({"fwd_bwd_256": differentiable_256(args)}), flush=True)
         return
     n = args.n
     dims = (n, n, n)
@@ -416,6 +438,8 @@ def main():
                            "fwd_bwd_over_fwd": round(r["fb_ms"] / ms, 3),
                            "Gcells_per_s": round(C / (r["fb_ms"] * 1e-3) / 1e9, 3)}
     if args.only != "fwd":
+        if not args.no_bwd:
+            line["fwd_bwd_256"] = differentiable_256(args)
         if args.batch > 0:
             line["batched"] = batched(args)
         if args.large:
