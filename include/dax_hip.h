@@ -341,8 +341,12 @@ int dax_hip_graph_destroy(void* graph_exec);
  * interpolants (t, endpoint gradients) are recomputed from the field.
  * U = min(d_counts[1], n_points) if d_counts is given (device {T, U} of the
  * forward), else n_points.  d_scratch needs
- * dax_hip_mc_backward_scratch_bytes(n_points) bytes.  Deterministic: no
- * atomics, fixed summation order.  d_dl_dfield [P] f32 is fully written. */
+ * dax_hip_mc_backward_scratch_bytes(n_points) bytes (a 32-byte record per
+ * vertex and a list of the tiles dense enough to be redone in spans).
+ * Deterministic: every sum has a fixed order (an atomic counter only fills
+ * the tile list, whose order does not change any value), so the result is
+ * bitwise reproducible.  Rows of at most 15872 points (nx).  d_dl_dfield [P]
+ * f32 is fully written. */
 size_t dax_hip_mc_backward_scratch_bytes(int64_t n_points);
 int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
                         const float* h_origin3, const float* h_spacing3, float iso,
