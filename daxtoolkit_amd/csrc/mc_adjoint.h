@@ -612,19 +612,21 @@ __device__ __forceinline__ uint32_t gather_tile(const VolGeom& g, const uint64_t
   return gather_span<false>(g, b, base_v, adj, n_adj, out, w0, w_end, sh, lds);
 }
 
-// Tiles whose endpoints exceed kMaxEndpoints are listed by the tile pass
-// and redone as kSpansPerTile spans of kGatherTile / kSpansPerTile words
-// (a span reads about 1/kSpansPerTile of the tile's endpoints).  Every
-// listed tile holds more than kMaxEndpoints endpoint slots; a point is a
-// slot of at most 5 tiles (its own, the tiles of the words one row before
-// and after it and one plane below and above) and a volume with U vertices
-// has at most 2U endpoints, so fewer than 10U / kMaxEndpoints < U / 128 + 2
-// tiles are listed.
-constexpr int kSpansPerTile = 4;
+// Tiles whose endpoints exceed kMaxEndpoints are redone in spans: the tile
+// pass appends one item per span, 2..16 spans per tile (the power of two
+// nearest above n_end / 1536), and a span pass runs the items.  An item is
+// (tile << 8) | (log2 spans << 4) | span.  Every listed tile holds more than
+// kMaxEndpoints endpoint slots; a point is a slot of at most 5 tiles (its
+// own, the tiles of the words one row before and after it and one plane
+// below and above) and a volume with U vertices has at most 2U endpoints,
+// so the slots of listed tiles sum to at most 10U, fewer than 10U / 2048
+// tiles are listed, and their items number at most
+// 2 * (10U / 1536 + tiles) < U / 32 + 16.
+constexpr int kSpanEndpoints = 1536;
 __host__ __device__ constexpr int64_t gather_list_capacity(int64_t n_points) {
-  return n_points / 128 + 2;
+  return n_points / 32 + 16;
 }
-// Scratch of the adjoint: the vertex records, then the list (count, tiles).
+// Scratch of the adjoint: the vertex records, then the list (count, items).
 __host__ __device__ constexpr size_t gather_list_offset(int64_t n_points) {
   return ((size_t)(n_points > 0 ? n_points : 0) * sizeof(VertexAdj) + 15) & ~(size_t)15;
 }
@@ -632,11 +634,38 @@ __host__ __device__ constexpr size_t adjoint_scratch_bytes(int64_t n_points) {
   return gather_list_offset(n_points) + (size_t)(1 + gather_list_capacity(n_points)) * 4;
 }
 
-// Appends a tile to the list (one thread of the block).
+// Appends the span items of a tile with n_end endpoints (one thread of the
+// block).
 __device__ __forceinline__ void gather_list_push(uint32_t* __restrict__ list, int64_t cap,
-                                                 uint32_t tile) {
-  const uint32_t i = atomicAdd(&list[0], 1u);
-  if (i < cap) list[1 + i] = tile;
+                                                 uint32_t tile, uint32_t n_end) {
+  int lg = 1;
+  while (lg < 4 && (uint32_t)(kSpanEndpoints << lg) < n_end) ++lg;
+  const uint32_t n = 1u << lg;
+  const uint32_t i = atomicAdd(&list[0], n);
+  for (uint32_t s = 0; s < n; ++s)
+    if (i + s < cap) list[1 + i + s] = (tile << 8) | ((uint32_t)lg << 4) | s;
+}
+
+// The span pass splits every item into `parts` equal pieces, the largest
+// power of two (up to 16) that keeps the pieces within two rounds of its
+// blocks: a short list (a few dense tiles) then runs as many small spans,
+// whose latency is shorter, and a long one as the items themselves.
+__device__ __forceinline__ int gather_item_parts(int64_t n_items, int64_t blocks) {
+  int parts = 1;
+  while (parts < 16 && n_items * parts * 2 <= 2 * blocks) parts *= 2;
+  return parts;
+}
+
+// Word range [w0, w_end) of piece `part` of `parts` of a span item within
+// the tile starting at word tile_w0 of a volume with nwords words (spans
+// are at least 16 words, pieces at least one).
+__device__ __forceinline__ void gather_item_span(uint32_t item, int part, int parts,
+                                                 int64_t tile_w0, int64_t nwords, int64_t& w0,
+                                                 int64_t& w_end) {
+  const int lg = (int)((item >> 4) & 15u), s = (int)(item & 15u);
+  const int len = kGatherTile >> lg, piece = max(len / parts, 1);
+  w0 = tile_w0 + (int64_t)s * len + (int64_t)part * piece;
+  w_end = min(min(w0 + piece, tile_w0 + (int64_t)(s + 1) * len), nwords);
 }
 
 }  // namespace mc

@@ -36,16 +36,17 @@ __global__ __launch_bounds__(kGatherThreads) void k_point_gather(
   const int64_t tile = blockIdx.x;
   const int64_t n_adj = counts ? min(counts[1], n_points) : n_points;
   const int64_t w0 = tile * kGatherTile, w_end = min(w0 + kGatherTile, g.nwords);
-  if (gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj, dldf + g.field_off, w0,
-                  w_end, sh, s_dyn)) {
+  const uint32_t n_end = gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj,
+                                     dldf + g.field_off, w0, w_end, sh, s_dyn);
+  if (n_end) {
     if (!list)
       zero_run(g, dldf + g.field_off, w0, w_end, threadIdx.x, kGatherThreads);
     else if (threadIdx.x == 0)
-      gather_list_push(list, gather_list_capacity(n_points), (uint32_t)tile);
+      gather_list_push(list, gather_list_capacity(n_points), (uint32_t)tile, n_end);
   }
 }
 
-// The listed tiles as kSpansPerTile spans each (persistent blocks).
+// The listed span items (persistent blocks).
 __global__ __launch_bounds__(kGatherThreads) void k_point_gather_spans(
     VolGeom g, const uint64_t* __restrict__ bits, const uint32_t* __restrict__ base_v,
     const VertexAdj* __restrict__ adj, const int64_t* __restrict__ counts, int64_t n_points,
@@ -53,13 +54,13 @@ __global__ __launch_bounds__(kGatherThreads) void k_point_gather_spans(
   __shared__ GatherShared sh;
   extern __shared__ __align__(16) uint64_t s_dyn[];
   const int64_t n_adj = counts ? min(counts[1], n_points) : n_points;
-  const int64_t n =
-      min((int64_t)list[0], gather_list_capacity(n_points)) * (int64_t)kSpansPerTile;
-  constexpr int kSpan = kGatherTile / kSpansPerTile;
-  for (int64_t it = blockIdx.x; it < n; it += gridDim.x) {
-    const int64_t tile = list[1 + it / kSpansPerTile];
-    const int64_t w0 = tile * kGatherTile + (it % kSpansPerTile) * kSpan;
-    const int64_t w_end = min(w0 + kSpan, g.nwords);
+  const int64_t n_items = min((int64_t)list[0], gather_list_capacity(n_points));
+  const int parts = gather_item_parts(n_items, gridDim.x);
+  for (int64_t it = blockIdx.x; it < n_items * parts; it += gridDim.x) {
+    const uint32_t item = list[1 + it / parts];
+    int64_t w0, w_end;
+    gather_item_span(item, (int)(it % parts), parts, (int64_t)(item >> 8) * kGatherTile,
+                     g.nwords, w0, w_end);
     if (w0 < w_end)
       gather_span<true>(g, bits + g.word_off, base_v + g.word_off, adj, n_adj,
                         dldf + g.field_off, w0, w_end, sh, s_dyn);

@@ -204,12 +204,13 @@ __global__ __launch_bounds__(kGatherThreads) void k_batch_point_gather(
   const TileRef tr = tiles[blockIdx.x];
   const VolGeom g = vols[tr.vol];
   const int64_t w_end = min((int64_t)tr.w0 + kBatchGenTile, g.nwords);
-  if (gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj, dldf + g.field_off,
-                  tr.w0, w_end, sh, s_dyn)) {
+  const uint32_t n_end = gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj,
+                                     dldf + g.field_off, tr.w0, w_end, sh, s_dyn);
+  if (n_end) {
     if (!list)
       zero_run(g, dldf + g.field_off, tr.w0, w_end, threadIdx.x, kGatherThreads);
     else if (threadIdx.x == 0)
-      gather_list_push(list, gather_list_capacity(n_points), blockIdx.x);
+      gather_list_push(list, gather_list_capacity(n_points), blockIdx.x, n_end);
   }
 }
 
@@ -221,14 +222,14 @@ __global__ __launch_bounds__(kGatherThreads) void k_batch_point_gather_spans(
   __shared__ GatherShared sh;
   extern __shared__ __align__(16) uint64_t s_dyn[];
   const int64_t n_adj = counts ? min(counts[1], n_points) : n_points;
-  const int64_t n =
-      min((int64_t)list[0], gather_list_capacity(n_points)) * (int64_t)kSpansPerTile;
-  constexpr int kSpan = kGatherTile / kSpansPerTile;
-  for (int64_t it = blockIdx.x; it < n; it += gridDim.x) {
-    const TileRef tr = tiles[list[1 + it / kSpansPerTile]];
+  const int64_t n_items = min((int64_t)list[0], gather_list_capacity(n_points));
+  const int parts = gather_item_parts(n_items, gridDim.x);
+  for (int64_t it = blockIdx.x; it < n_items * parts; it += gridDim.x) {
+    const uint32_t item = list[1 + it / parts];
+    const TileRef tr = tiles[item >> 8];
     const VolGeom g = vols[tr.vol];
-    const int64_t w0 = tr.w0 + (it % kSpansPerTile) * kSpan;
-    const int64_t w_end = min(w0 + kSpan, g.nwords);
+    int64_t w0, w_end;
+    gather_item_span(item, (int)(it % parts), parts, tr.w0, g.nwords, w0, w_end);
     if (w0 < w_end)
       gather_span<true>(g, bits + g.word_off, base_v + g.word_off, adj, n_adj,
                         dldf + g.field_off, w0, w_end, sh, s_dyn);
@@ -409,6 +410,11 @@ int dax_hip_mc_batch_backward(const float* d_fields, const void* h_header,
   if (!h || h->magic != kBatchMagic || !d_workspace ||
       workspace_bytes < kBatchHeaderBytes + h->bytes) {
     set_error("dax_hip_mc_batch_backward: header/workspace not prepared");
+    return DAX_HIP_BAD_VALUE;
+  }
+  if (h->ngen >= (1 << 24)) {
+    set_error("dax_hip_mc_batch_backward: %lld tiles exceed the 2^24 the adjoint lists",
+              (long long)h->ngen);
     return DAX_HIP_BAD_VALUE;
   }
   if (h->max_wpr > kMaxGatherWpr) {
