@@ -148,8 +148,9 @@ __global__ __launch_bounds__(kScanThreads) void k_batch_generate(
     const TileRef tr = tiles[xcd_tile(b, ntiles)];
     const VolGeom g = vols[tr.vol];
     const int64_t vbv = base_v[g.word_off], vbt = base_t[g.word_off];
-    generate_tile<kBatchGenTile>(g, fields + g.field_off, bits + g.word_off, base_v + g.word_off,
-                                 base_t + g.word_off, tr.w0, out, dedup, normals, vbv, vbt, s_dyn);
+    generate_tile<kBatchGenTile>(g, FieldArray{fields + g.field_off}, bits + g.word_off,
+                                 base_v + g.word_off, base_t + g.word_off, tr.w0, out, dedup,
+                                 normals, vbv, vbt, s_dyn);
     __syncthreads();  // LDS reused by the next tile
   }
 }

@@ -202,9 +202,9 @@ __global__ __launch_bounds__(kCountThreads) void k_apply_bases(VolGeom g,
   }
 }
 
-template <int TW>
+template <int TW, class Src>
 __global__ __launch_bounds__(kScanThreads) void k_generate(
-    VolGeom g, const float* __restrict__ field, const uint64_t* __restrict__ bits,
+    VolGeom g, Src field, const uint64_t* __restrict__ bits,
     const uint32_t* __restrict__ base_v, const uint32_t* __restrict__ base_t,
     const Ctrl* __restrict__ ctrl, OutPtrs out, unsigned flags, int64_t cap_points,
     int64_t cap_tris, int64_t* __restrict__ counts, int xcd_order, int64_t tile_lo,
@@ -221,8 +221,37 @@ __global__ __launch_bounds__(kScanThreads) void k_generate(
   if (total_v > 0x7fffffffull || 3ull * total_t > 0x7fffffffull) return;
   if ((int64_t)total_t > cap_tris || (int64_t)n_points > cap_points) return;
   const int64_t tile = tile_lo + (xcd_order ? xcd_tile(blockIdx.x, ntiles) : (int64_t)blockIdx.x);
-  generate_tile<TW>(g, field + g.field_off, bits + g.word_off, base_v + g.word_off,
-                    base_t + g.word_off, tile * TW, out, dedup, normals, 0, 0, s_dyn);
+  generate_tile<TW>(g, field, bits + g.word_off, base_v + g.word_off, base_t + g.word_off,
+                    tile * TW, out, dedup, normals, 0, 0, s_dyn);
+}
+
+// Slice classify: a lane per point computes the plane distance and a ballot
+// forms the word (UNROLL consecutive words per wave); the same launch clears
+// the count/scan state.
+template <int UNROLL>
+__global__ __launch_bounds__(256) void k_classify_plane(VolGeom g, FieldPlane pl,
+                                                        uint64_t* __restrict__ bits,
+                                                        unsigned long long* __restrict__ zero_buf,
+                                                        int64_t zero_words) {
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gtid < zero_words) zero_buf[gtid] = 0ull;
+  const int lane = __lane_id();
+  const int64_t w0 = (gtid >> 6) * UNROLL;
+  uint64_t mine = 0ull;
+#pragma unroll
+  for (int u = 0; u < UNROLL; ++u) {
+    const int64_t w = w0 + u;
+    bool in = false;
+    if (w < g.nwords) {
+      const uint32_t row = fast_div((uint32_t)w, g.div_wpr);
+      const uint32_t k = fast_div(row, g.div_ny);
+      const int x = (int)(w - (int64_t)row * g.wpr) * 64 + lane;
+      if (x < g.nx) in = pl(x, (int)(row - k * (uint32_t)g.ny), (int)k, 0) > g.iso;
+    }
+    const uint64_t b = __ballot(in);
+    if (lane == u) mine = b;
+  }
+  if (lane < UNROLL && w0 + lane < g.nwords) bits[w0 + lane] = mine;
 }
 
 // ---------------------------------------------------------------------------
@@ -320,7 +349,7 @@ __global__ void k_point_gradient(VolGeom g, const float* __restrict__ f, float* 
   const int x = (int)((uint32_t)p - row * (uint32_t)g.nx), y = (int)(row - zz * (uint32_t)g.ny),
             z = (int)zz;
   float gr[3];
-  point_gradient(g, f, x, y, z, gr);
+  point_gradient(g, FieldArray{f}, x, y, z, gr);
   grad[3 * p + 0] = gr[0];
   grad[3 * p + 1] = gr[1];
   grad[3 * p + 2] = gr[2];
@@ -404,6 +433,29 @@ int make_geom(int64_t nx, int64_t ny, int64_t nz, const float* origin, const flo
   g.word_off = 0;
   g.nwords = (int64_t)g.wpr * ny * nz;
   set_geom_divs(g);
+  return DAX_HIP_OK;
+}
+
+// Geometry and plane of a slice; the isovalue is dot(normal, origin) summed
+// from component 0 (dax/worklet/Slice.h:44,104; dax/Types.h:611-621).
+int make_slice(int64_t nx, int64_t ny, int64_t nz, const float* origin, const float* spacing,
+               const float* plane_origin, const float* plane_normal, VolGeom& g,
+               FieldPlane& pl) {
+  if (!plane_origin || !plane_normal) {
+    set_error("dax_hip_mc_slice: plane origin and normal are required");
+    return DAX_HIP_BAD_VALUE;
+  }
+  float iso = plane_normal[0] * plane_origin[0];
+  iso += plane_normal[1] * plane_origin[1];
+  iso += plane_normal[2] * plane_origin[2];
+  const int rc = make_geom(nx, ny, nz, origin, spacing, iso, g);
+  if (rc) return rc;
+  for (int d = 0; d < 3; ++d) {
+    pl.o[d] = g.origin[d];
+    pl.s[d] = g.spacing[d];
+    pl.po[d] = plane_origin[d];
+    pl.pn[d] = plane_normal[d];
+  }
   return DAX_HIP_OK;
 }
 
@@ -497,14 +549,14 @@ int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu
 }
 
 // Generate tiles of TW words [t_lo, t_hi) (all when t_hi < 0).
-template <int TW>
-int launch_generate_tw(const VolGeom& g, const float* field, const char* ws, const Workspace& L,
+template <int TW, class Src>
+int launch_generate_tw(const VolGeom& g, const Src& field, const char* ws, const Workspace& L,
                        unsigned flags, const OutPtrs& o, int64_t cap_points, int64_t cap_tris,
                        int64_t* counts, int xcd, hipStream_t s, int64_t t_lo, int64_t t_hi) {
   const size_t lds = GenLayout<TW>(g.wpr).bytes();
   if (t_hi < 0) t_hi = ceil_div(g.nwords, (int64_t)TW);
   const int64_t ntiles = std::max<int64_t>(t_hi - t_lo, 1);
-  k_generate<TW><<<dim3((unsigned)ntiles), dim3(kScanThreads), lds, s>>>(
+  k_generate<TW, Src><<<dim3((unsigned)ntiles), dim3(kScanThreads), lds, s>>>(
       g, field, (const uint64_t*)(ws + L.off_bits), (const uint32_t*)(ws + L.off_base_v),
       (const uint32_t*)(ws + L.off_base_t), (const Ctrl*)(ws + L.off_ctrl), o, flags, cap_points,
       cap_tris, counts, xcd, t_lo, ntiles);
@@ -512,10 +564,11 @@ int launch_generate_tw(const VolGeom& g, const float* field, const char* ws, con
   return DAX_HIP_OK;
 }
 
-int launch_generate(const VolGeom& g, const float* field, const char* ws, const Workspace& L,
-                    unsigned flags, const OutPtrs& o, int64_t cap_points, int64_t cap_tris,
-                    int64_t* counts, const Tuning& tu, hipStream_t s, int64_t t_lo = 0,
-                    int64_t t_hi = -1) {
+template <class Src>
+int launch_generate_src(const VolGeom& g, const Src& field, const char* ws, const Workspace& L,
+                        unsigned flags, const OutPtrs& o, int64_t cap_points, int64_t cap_tris,
+                        int64_t* counts, const Tuning& tu, hipStream_t s, int64_t t_lo = 0,
+                        int64_t t_hi = -1) {
   switch (tu.gen_tile) {
     case 128:
       return launch_generate_tw<128>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
@@ -530,6 +583,28 @@ int launch_generate(const VolGeom& g, const float* field, const char* ws, const 
       return launch_generate_tw<512>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
                                      tu.gen_xcd, s, t_lo, t_hi);
   }
+}
+
+int launch_generate(const VolGeom& g, const float* field, const char* ws, const Workspace& L,
+                    unsigned flags, const OutPtrs& o, int64_t cap_points, int64_t cap_tris,
+                    int64_t* counts, const Tuning& tu, hipStream_t s, int64_t t_lo = 0,
+                    int64_t t_hi = -1) {
+  return launch_generate_src(g, FieldArray{field + g.field_off}, ws, L, flags, o, cap_points,
+                             cap_tris, counts, tu, s, t_lo, t_hi);
+}
+
+// Slice classify over the whole volume (clears the count/scan state).
+int launch_bits_plane(const VolGeom& g, const FieldPlane& pl, char* ws, const Workspace& L,
+                      hipStream_t s) {
+  constexpr int kUnroll = 8;
+  const int64_t zero_words = L.ntiles + (int64_t)(sizeof(Ctrl) / 8);
+  const int64_t waves = ceil_div(g.nwords, kUnroll);
+  const int64_t blocks =
+      std::max<int64_t>(std::max(ceil_div(waves * 64, 256), ceil_div(zero_words, 256)), 1);
+  k_classify_plane<kUnroll><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
+      g, pl, (uint64_t*)(ws + L.off_bits), (unsigned long long*)(ws + L.off_status), zero_words);
+  DAXHIP_LAUNCH_CHECK("k_classify_plane");
+  return DAX_HIP_OK;
 }
 
 // The three kernels of a forward over the whole volume.
@@ -657,6 +732,55 @@ int dax_hip_mc_forward(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
   OutPtrs o{d_vertices, d_normals, d_conn, d_edges, d_interp_first, d_interp_second,
             d_interp_weight};
   return launch_forward(g, d_field, ws, L, flags, o, cap_points, cap_tris, d_counts, tu, s);
+}
+
+// ---- slice: marching cubes of the plane distance, computed in the kernels
+// (SliceCount / SliceGenerate, dax/worklet/Slice.h:28-173, on a uniform
+// grid; no field in memory).
+int dax_hip_mc_slice_classify(int64_t nx, int64_t ny, int64_t nz, const float* h_origin3,
+                              const float* h_spacing3, const float* h_plane_origin3,
+                              const float* h_plane_normal3, void* d_workspace,
+                              size_t workspace_bytes, int64_t* d_counts, void* stream) {
+  VolGeom g;
+  FieldPlane pl;
+  int rc = make_slice(nx, ny, nz, h_origin3, h_spacing3, h_plane_origin3, h_plane_normal3, g, pl);
+  if (rc) return rc;
+  Workspace L;
+  if ((rc = check_ws(g, workspace_bytes, L, d_workspace))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)d_workspace;
+  if ((rc = launch_bits_plane(g, pl, ws, L, s))) return rc;
+  if ((rc = launch_scan(g, ws, L, tuning_for(nx, ny, nz), s))) return rc;
+  if (d_counts) {
+    k_write_counts<<<1, 1, 0, s>>>((const Ctrl*)(ws + L.off_ctrl), d_counts, DAX_HIP_MC_DEDUP);
+    DAXHIP_LAUNCH_CHECK("k_write_counts");
+  }
+  return DAX_HIP_OK;
+}
+
+int dax_hip_mc_slice(int64_t nx, int64_t ny, int64_t nz, const float* h_origin3,
+                     const float* h_spacing3, const float* h_plane_origin3,
+                     const float* h_plane_normal3, unsigned flags, void* d_workspace,
+                     size_t workspace_bytes, float* d_vertices, float* d_normals, int32_t* d_conn,
+                     uint32_t* d_edges, int32_t* d_interp_first, int32_t* d_interp_second,
+                     float* d_interp_weight, int64_t cap_points, int64_t cap_tris,
+                     int64_t* d_counts, void* stream) {
+  VolGeom g;
+  FieldPlane pl;
+  int rc = make_slice(nx, ny, nz, h_origin3, h_spacing3, h_plane_origin3, h_plane_normal3, g, pl);
+  if (rc) return rc;
+  Workspace L;
+  if ((rc = check_ws(g, workspace_bytes, L, d_workspace))) return rc;
+  static const float kNoField = 0.0f;  // the slice reads no field
+  if ((rc = check_outputs(g, &kNoField, flags, d_vertices, d_conn, d_edges, d_counts))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)d_workspace;
+  const Tuning tu = tuning_for(nx, ny, nz);
+  OutPtrs o{d_vertices, d_normals, d_conn, d_edges, d_interp_first, d_interp_second,
+            d_interp_weight};
+  if ((rc = launch_bits_plane(g, pl, ws, L, s))) return rc;
+  if ((rc = launch_scan(g, ws, L, tu, s))) return rc;
+  return launch_generate_src(g, pl, ws, L, flags, o, cap_points, cap_tris, d_counts, tu, s);
 }
 
 int dax_hip_mc_read_counts(const int64_t* d_counts, int64_t cap_points, int64_t cap_tris,

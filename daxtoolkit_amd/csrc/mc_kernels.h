@@ -332,11 +332,40 @@ struct OutPtrs {
   float* interp_weight;
 };
 
+// Point-field sources of the forward: a float32 field in memory, or the
+// signed plane distance of a slice computed from the point's coordinates.
+// src(x, y, z, p) is the value at point (x, y, z) of flat index p.
+struct FieldArray {
+  const float* __restrict__ f;
+  __device__ __forceinline__ float operator()(int, int, int, int64_t p) const { return f[p]; }
+};
+
+// The field SliceCount / SliceGenerate classify and interpolate
+// (dax/worklet/Slice.h:44-57,104-120): dot(normal, c - origin) at the point
+// coordinates c = grid origin + spacing * ijk (dax/cont/UniformGrid.h:82-89),
+// the dot product summed from component 0 (dax/Types.h:611-621), against
+// the isovalue dot(normal, origin).
+struct FieldPlane {
+  float o[3], s[3];  // grid origin and spacing
+  float po[3], pn[3];  // plane origin and normal
+  __device__ __forceinline__ float operator()(int x, int y, int z, int64_t) const {
+    const int c[3] = {x, y, z};
+    float r = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float t = pn[d] * ((o[d] + s[d] * (float)c[d]) - po[d]);
+      r = d == 0 ? t : r + t;
+    }
+    return r;
+  }
+};
+
 // Central-difference gradient at (x,y,z): interior (f(p+e)-f(p-e))/(2h),
 // first plane (f(p+e)-f(p))/h, last plane (f(p)-f(p-e))/h, and 0 along an
 // axis with a single point.  Written branch-free so all loads issue at once.
-__device__ __forceinline__ void point_gradient(const VolGeom& g, const float* __restrict__ f, int x,
-                                               int y, int z, float out[3]) {
+template <class Src>
+__device__ __forceinline__ void point_gradient(const VolGeom& g, const Src& f, int x, int y,
+                                               int z, float out[3]) {
   const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
   const int64_t p = x + sy * y + sz * z;
   const int c[3] = {x, y, z};
@@ -346,11 +375,10 @@ __device__ __forceinline__ void point_gradient(const VolGeom& g, const float* __
   bool wide[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const int64_t lo = c[d] > 0 ? p - st[d] : p;
-    const int64_t hi = c[d] < n[d] - 1 ? p + st[d] : p;
+    const int dl = c[d] > 0 ? 1 : 0, dh = c[d] < n[d] - 1 ? 1 : 0;
     wide[d] = c[d] > 0 && c[d] < n[d] - 1;
-    lo_v[d] = f[lo];
-    hi_v[d] = f[hi];
+    lo_v[d] = f(x - (d == 0) * dl, y - (d == 1) * dl, z - (d == 2) * dl, p - dl * st[d]);
+    hi_v[d] = f(x + (d == 0) * dh, y + (d == 1) * dh, z + (d == 2) * dh, p + dh * st[d]);
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -366,13 +394,15 @@ __device__ __forceinline__ void point_gradient(const VolGeom& g, const float* __
 }
 
 // One interpolated point on the edge with lower endpoint (x,y,z) along axis.
-__device__ __forceinline__ void emit_point(const VolGeom& g, const float* __restrict__ f, int x,
-                                           int y, int z, int axis, bool normals, float xyz[3],
-                                           float nrm[3], float& w, int64_t& pa, int64_t& pb) {
+template <class Src>
+__device__ __forceinline__ void emit_point(const VolGeom& g, const Src& f, int x, int y, int z,
+                                           int axis, bool normals, float xyz[3], float nrm[3],
+                                           float& w, int64_t& pa, int64_t& pb) {
   const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
   pa = x + sy * y + sz * z;
   pb = pa + (axis == 0 ? 1 : (axis == 1 ? sy : sz));
-  const float fa = f[pa], fb = f[pb];
+  const float fa = f(x, y, z, pa),
+              fb = f(x + (axis == 0), y + (axis == 1), z + (axis == 2), pb);
   float ga[3], gb[3];
   if (normals) {
     point_gradient(g, f, x, y, z, ga);
@@ -486,8 +516,8 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total,
 //      marks its triangles' entries of the round's triangle -> cell map; one
 //      thread per triangle turns the triangle's edges into vertex ids through
 //      the edge flags (staged for the tile's words and the next row)
-template <int TW>
-__device__ __forceinline__ void generate_tile(const VolGeom& g, const float* __restrict__ f,
+template <int TW, class Src>
+__device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
                                               const uint64_t* __restrict__ b,
                                               const uint32_t* __restrict__ bv,
                                               const uint32_t* __restrict__ bt, int64_t w_begin,

@@ -220,6 +220,52 @@ def forward(field: torch.Tensor, iso: float, origin=(0, 0, 0), spacing=(1, 1, 1)
         workspace=buf.workspace, counts=buf.counts)
 
 
+def slice_into(dims, plane_origin, plane_normal, buf: ForwardBuffers, origin=(0, 0, 0),
+               spacing=(1, 1, 1), dedup: bool = True, normals: bool = False) -> None:
+    """Enqueue one slice (the marching-cubes surface of the plane distance the
+    reference's SliceCount / SliceGenerate classify, dax/worklet/Slice.h:28-173,
+    computed in the kernels) into preallocated buffers; dims = (nx, ny, nz)."""
+    nx, ny, nz = (int(v) for v in dims)
+    flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and buf.normals is not None) else 0)
+    _lib.call("dax_hip_mc_slice", nx, ny, nz, _f3(origin), _f3(spacing), _f3(plane_origin),
+              _f3(plane_normal), flags, buf.workspace.data_ptr(), buf.workspace.numel(),
+              buf.vertices.data_ptr(), _ptr(buf.normals), buf.conn.data_ptr(),
+              _ptr(buf.edges) if dedup else None, _ptr(buf.interp_first),
+              _ptr(buf.interp_second), _ptr(buf.interp_weight), buf.cap_points, buf.cap_tris,
+              buf.counts.data_ptr(), _stream())
+
+
+def slice(dims, plane_origin, plane_normal, origin=(0, 0, 0), spacing=(1, 1, 1),
+          dedup: bool = True, normals: bool = False, interp: bool = False,
+          device="cuda") -> Contour:
+    """Slice of the uniform grid dims = (nx, ny, nz) by the plane through
+    plane_origin with normal plane_normal, as Dax's SliceCount then
+    SliceGenerate (dax/worklet/Slice.h:28-173) produce it: count, size the
+    outputs, generate.  No field is read."""
+    nx, ny, nz = (int(v) for v in dims)
+    ws = new_workspace(nx, ny, nz, device)
+    cnt = torch.empty(2, dtype=torch.int64, device=device)
+    _lib.call("dax_hip_mc_slice_classify", nx, ny, nz, _f3(origin), _f3(spacing),
+              _f3(plane_origin), _f3(plane_normal), ws.data_ptr(), ws.numel(), cnt.data_ptr(),
+              _stream())
+    T, U = read_counts(cnt)
+    if not dedup:
+        U = 3 * T
+    buf = ForwardBuffers(nx, ny, nz, max(U, 1), max(T, 1), device, normals=normals,
+                         edges=dedup, interp=interp)
+    slice_into((nx, ny, nz), plane_origin, plane_normal, buf, origin, spacing, dedup, normals)
+    T2, U2 = read_counts(buf.counts, buf.cap_points, buf.cap_tris)
+    assert (T2, U2) == (T, U), (T2, U2, T, U)
+    return Contour(
+        vertices=buf.vertices[:U], connectivity=buf.conn[:T],
+        normals=None if buf.normals is None else buf.normals[:U],
+        edges=None if buf.edges is None else buf.edges[:U],
+        interp_first=None if buf.interp_first is None else buf.interp_first[:U],
+        interp_second=None if buf.interp_second is None else buf.interp_second[:U],
+        interp_weight=None if buf.interp_weight is None else buf.interp_weight[:U],
+        workspace=buf.workspace, counts=buf.counts)
+
+
 def backward(field: torch.Tensor, iso: float, contour: Contour, dl_dvertices: torch.Tensor,
              dl_dnormals: torch.Tensor, origin=(0, 0, 0), spacing=(1, 1, 1),
              out: Optional[torch.Tensor] = None, scratch: Optional[torch.Tensor] = None):

@@ -184,6 +184,30 @@ int dax_hip_mc_forward(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
                        int64_t cap_points, int64_t cap_tris, int64_t* d_counts,
                        void* stream);
 
+/* Slice of a uniform grid by a plane: replaces DispatcherMapCell<SliceCount>
+ * + DispatcherGenerateInterpolatedCells<SliceGenerate> over the grid's
+ * point coordinates (dax/worklet/Slice.h:28-173; the reference test
+ * dax/worklet/testing/UnitTestWorkletSlice.cxx:66-121).  The classified and
+ * interpolated value of a point is dot(plane_normal, c - plane_origin) at
+ * its coordinates c = origin + spacing * ijk, against the isovalue
+ * dot(plane_normal, plane_origin) -- the reference's definition -- computed
+ * in the kernels from the point's indices: no field is read.  Outputs,
+ * flags, capacities and counts as dax_hip_mc_forward (normals are the
+ * central-difference gradients of those values; the reference's slice has
+ * none).  dax_hip_mc_slice_classify is the counting step alone (as
+ * dax_hip_mc_classify). */
+int dax_hip_mc_slice_classify(int64_t nx, int64_t ny, int64_t nz, const float* h_origin3,
+                              const float* h_spacing3, const float* h_plane_origin3,
+                              const float* h_plane_normal3, void* d_workspace,
+                              size_t workspace_bytes, int64_t* d_counts, void* stream);
+int dax_hip_mc_slice(int64_t nx, int64_t ny, int64_t nz, const float* h_origin3,
+                     const float* h_spacing3, const float* h_plane_origin3,
+                     const float* h_plane_normal3, unsigned flags, void* d_workspace,
+                     size_t workspace_bytes, float* d_vertices, float* d_normals, int32_t* d_conn,
+                     uint32_t* d_edges, int32_t* d_interp_first, int32_t* d_interp_second,
+                     float* d_interp_weight, int64_t cap_points, int64_t cap_tris,
+                     int64_t* d_counts, void* stream);
+
 /* Synchronous: reads {T, U} written by a forward (or classify) on `stream`,
  * and returns DAX_HIP_CAPACITY if the generate step was skipped for lack of
  * room, DAX_HIP_EXEC_ERROR if T or U left the int32 dax::Id range. */

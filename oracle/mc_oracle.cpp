@@ -480,6 +480,34 @@ double vertex_loss_f64(const Grid& g, const float* f, const std::unordered_map<i
 
 extern "C" {
 
+// Slice (dax/worklet/Slice.h:28-173) on a uniform grid: SliceCount and
+// SliceGenerate classify and interpolate the per-point value
+// dot(normal, c - origin) of the point coordinates c = grid origin + spacing
+// * ijk (dax/cont/UniformGrid.h:82-89) against the isovalue
+// dot(normal, origin) (Slice.h:44-57,104-120), dot summed from component 0
+// (dax/Types.h:611-621).  With that value as the point field the two
+// worklets are MarchingCubesCount / MarchingCubesGenerate (same table, same
+// edge pairs; on a uniform grid every edge pair lists the lower point id
+// first, so the weights match MarchingCubes.h:148-161).  Returns the
+// isovalue; writes the values of all points.
+float oracle_slice_field(const int64_t* dims, const float* origin, const float* spacing,
+                         const float* plane_origin, const float* plane_normal, float* out) {
+  const Grid g = make_grid(dims, origin, spacing);
+  for (int64_t p = 0; p < g.points(); ++p) {
+    const int64_t ijk[3] = {p % g.nx, (p / g.nx) % g.ny, p / (g.nx * g.ny)};
+    float c[3];
+    for (int d = 0; d < 3; ++d) c[d] = g.origin[d] + g.spacing[d] * (float)ijk[d];
+    float r = plane_normal[0] * (c[0] - plane_origin[0]);
+    r += plane_normal[1] * (c[1] - plane_origin[1]);
+    r += plane_normal[2] * (c[2] - plane_origin[2]);
+    out[p] = r;
+  }
+  float iso = plane_normal[0] * plane_origin[0];
+  iso += plane_normal[1] * plane_origin[1];
+  iso += plane_normal[2] * plane_origin[2];
+  return iso;
+}
+
 // Analytic adjoint in FP64, scatter form: dL/df for every point.
 void oracle_mc_backward_f64(const float* field, const int64_t* dims, const float* origin,
                             const float* spacing, float iso, const int32_t* first,

@@ -51,6 +51,8 @@ def lib():
         L.oracle_mc_backward_f64.argtypes = ([C.c_void_p] * 4 + [C.c_float] + [C.c_void_p] * 2 +
                                              [C.c_int64] + [C.c_void_p] * 3)
         L.oracle_mc_backward_f32.argtypes = L.oracle_mc_backward_f64.argtypes
+        L.oracle_slice_field.argtypes = [C.c_void_p] * 6
+        L.oracle_slice_field.restype = C.c_float
         L.oracle_mc_fd_f64.argtypes = ([C.c_void_p] * 4 + [C.c_float] + [C.c_void_p] * 2 +
                                        [C.c_int64] + [C.c_void_p] * 3 + [C.c_int64, C.c_double,
                                                                           C.c_void_p])
@@ -187,6 +189,19 @@ def cell_gradient(field, origin=(0, 0, 0), spacing=(1, 1, 1)):
     out = np.zeros((C_, 3), np.float32)
     lib().oracle_cell_gradient(_p(f), _p(_dims(f)), _p(_f3(origin)), _p(_f3(spacing)), _p(out))
     return out
+
+
+def slice_field(dims, plane_origin, plane_normal, origin=(0, 0, 0), spacing=(1, 1, 1)):
+    """The point values and isovalue Dax's SliceCount/SliceGenerate classify
+    and interpolate (see mc_oracle.cpp); dims = (nx, ny, nz).  Returns
+    (field [nz, ny, nx] float32, iso float32); oracle.generate on them is
+    the slice."""
+    nx, ny, nz = (int(v) for v in dims)
+    out = np.zeros((nz, ny, nx), np.float32)
+    d = np.array([nx, ny, nz], np.int64)
+    iso = lib().oracle_slice_field(_p(d), _p(_f3(origin)), _p(_f3(spacing)),
+                                   _p(_f3(plane_origin)), _p(_f3(plane_normal)), _p(out))
+    return out, np.float32(iso)
 
 
 def backward_f64(field, iso, first, second, dldx, dldn, origin=(0, 0, 0), spacing=(1, 1, 1)):
