@@ -21,6 +21,8 @@ the headline the line carries
   fwd_bwd  the same field, forward + adjoint (upstream dL/dvertex and
            dL/dnormal N(0,1), seed 1), and its ratio to the forward;
   fwd_bwd_256  the differentiable workload: the same at 256^3;
+  slice_512  a slice of the 512^3 grid by a tilted plane (no field read),
+           with the OpenMP restatement beside it;
   batched  1024 volumes with per-axis extents U{32..160}, 4-16 Gaussians each,
            iso U[0.3,0.7]: the batched forward and forward+backward replayed
            as captured graphs, against a native launch loop of single-volume
@@ -82,7 +84,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=1024, help="volumes of the batched run (0: skip)")
     ap.add_argument("--no-bwd", action="store_true", help="skip the forward+backward timing")
     ap.add_argument("--large", action="store_true", help="also run the 1024^3 Mandelbulb")
-    ap.add_argument("--only", choices=["fwd", "batch", "large", "diff"], default=None,
+    ap.add_argument("--only", choices=["fwd", "batch", "large", "diff", "slice"], default=None,
                     help="run one section only (profiling); prints its JSON")
     return ap.parse_args()
 
@@ -321,6 +323,45 @@ def differentiable_256(args):
             "fwd_bwd_over_fwd": round(r["fb_ms"] / r["ms"], 3), "tuning": r["tuned"]}
 
 
+def slice_512(args):
+    """A slice of the 512^3 uniform grid by a tilted plane (SliceCount /
+    SliceGenerate semantics; the plane distance is computed in the kernels,
+    no field is read), graph-timed like the forward; beside it the OpenMP
+    restatement of the MC pipeline on the materialised plane values."""
+    import torch
+
+    from daxtoolkit_amd import mc
+    n = 512
+    dims = (n, n, n)
+    po, pn = (128.0, 128.0, 100.0), (0.36, 0.48, 0.8)
+    ws_cnt = mc.slice(dims, po, pn, normals=False)
+    T, U = ws_cnt.num_triangles, ws_cnt.num_points
+    del ws_cnt
+    buf = mc.ForwardBuffers(n, n, n, max(U, 1), max(T, 1), normals=False, edges=True)
+    for _ in range(args.warmup):
+        mc.slice_into(dims, po, pn, buf)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    ms = _graph_ms(lambda: mc.slice_into(dims, po, pn, buf), args.steps, s)
+    assert mc.read_counts(buf.counts, buf.cap_points, buf.cap_tris) == (T, U)
+    C = (n - 1) ** 3
+    out = {"workload": f"slice of the {n}^3 grid (origin 0, spacing 1) by the plane "
+                       f"dot({pn}, c - {po}) = dot(n, o) (SliceCount/SliceGenerate), dedup, "
+                       f"edge list, no normals",
+           "triangles": T, "vertices": U, "ms": round(ms, 5),
+           "Gcells_per_s": round(C / (ms * 1e-3) / 1e9, 3),
+           "tris_per_s": round(T / (ms * 1e-3), 1),
+           "out_bytes": 12 * U + 12 * T + 4 * U + n ** 3 // 8}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        f, iso = orc.slice_field(dims, po, pn)
+        cb = cpu_baseline(f, float(iso), C, args.cpu_seconds)
+        cb["sample"] += " (the plane values materialised beforehand, not timed)"
+        out["cpu_baseline"] = cb
+    _progress(f"slice {n}^3: {ms:.4f} ms, T={T}")
+    return out
+
+
 def large(args):
     import torch
 
@@ -368,6 +409,9 @@ def main():
         return
     if args.only == "diff":
         print(json.dumps({"fwd_bwd_256": differentiable_256(args)}), flush=True)
+        return
+    if args.only == "slice":
+        print(json.dumps({"slice_512": slice_512(args)}), flush=True)
         return
     n = args.n
     dims = (n, n, n)
@@ -440,6 +484,7 @@ def main():
     if args.only != "fwd":
         if not args.no_bwd:
             line["fwd_bwd_256"] = differentiable_256(args)
+        line["slice_512"] = slice_512(args)
         if args.batch > 0:
             line["batched"] = batched(args)
         if args.large:

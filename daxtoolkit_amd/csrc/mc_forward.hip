@@ -225,10 +225,51 @@ __global__ __launch_bounds__(kScanThreads) void k_generate(
                     tile * TW, out, dedup, normals, 0, 0, s_dyn);
 }
 
-// Slice classify: a lane per point computes the plane distance and a ballot
-// forms the word (UNROLL consecutive words per wave); the same launch clears
-// the count/scan state.
-template <int UNROLL>
+// Slice classify: a wave per kPlaneRowsPerWave grid rows.  Along a row the
+// plane value pn0 * ((o0 + s0 * x) - po0) + t1 + t2 (t1, t2: the row's y
+// and z terms)
+// is a monotonic function of x: int -> float conversion, multiplication by
+// a constant, addition of a constant and their round-to-nearest are all
+// monotonic.  So "value > iso" changes at most once along the row, and the
+// row's bits are a prefix or a suffix.  A 64-way search (each lane
+// evaluates one candidate x with exactly the operations FieldPlane uses,
+// a ballot finds the first change) locates the change in ceil(log64 nx) + 1
+// rounds; the lanes then write the row's words.  The same launch clears the
+// count/scan state.
+constexpr int kPlaneRowsPerWave = 8;
+
+// One grid row of the slice classify (all lanes of a wave).
+__device__ __forceinline__ void classify_plane_row(const VolGeom& g, const FieldPlane& pl,
+                                                   uint64_t* __restrict__ bits, uint32_t row,
+                                                   int lane) {
+  const uint32_t k = fast_div(row, g.div_ny), j = row - k * (uint32_t)g.ny;
+  const float t1 = pl.pn[1] * ((pl.o[1] + pl.s[1] * (float)(int)j) - pl.po[1]);
+  const float t2 = pl.pn[2] * ((pl.o[2] + pl.s[2] * (float)(int)k) - pl.po[2]);
+  auto inside = [&](int x) -> bool {
+    const float t0 = pl.pn[0] * ((pl.o[0] + pl.s[0] * (float)x) - pl.po[0]);
+    return (t0 + t1) + t2 > g.iso;
+  };
+  const bool p0 = inside(0);
+  // first x with inside(x) != p0, nx if none: invariant lo differs not, hi
+  // differs (nx stands for "differs"), the change lies in (lo, hi]
+  int lo = 0, hi = g.nx;
+  while (hi - lo > 1) {
+    const int step = (hi - lo + 63) / 64;
+    const int x = lo + (lane + 1) * step;
+    const uint64_t d = __ballot(x >= hi || inside(x) != p0);
+    const int f = __builtin_ctzll(d);  // lane 63 reaches hi, so d != 0
+    const int nlo = lo + f * step;
+    hi = min(nlo + step, hi);
+    lo = nlo;
+  }
+  const int change = hi;
+  uint64_t* out = bits + (int64_t)row * g.wpr;
+  for (int q = lane; q < g.wpr; q += 64) {
+    const uint64_t below = low_mask(change - q * 64);  // points x < change
+    out[q] = (p0 ? below : ~below) & low_mask(g.nx - q * 64);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_classify_plane(VolGeom g, FieldPlane pl,
                                                         uint64_t* __restrict__ bits,
                                                         unsigned long long* __restrict__ zero_buf,
@@ -236,22 +277,12 @@ __global__ __launch_bounds__(256) void k_classify_plane(VolGeom g, FieldPlane pl
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gtid < zero_words) zero_buf[gtid] = 0ull;
   const int lane = __lane_id();
-  const int64_t w0 = (gtid >> 6) * UNROLL;
-  uint64_t mine = 0ull;
-#pragma unroll
-  for (int u = 0; u < UNROLL; ++u) {
-    const int64_t w = w0 + u;
-    bool in = false;
-    if (w < g.nwords) {
-      const uint32_t row = fast_div((uint32_t)w, g.div_wpr);
-      const uint32_t k = fast_div(row, g.div_ny);
-      const int x = (int)(w - (int64_t)row * g.wpr) * 64 + lane;
-      if (x < g.nx) in = pl(x, (int)(row - k * (uint32_t)g.ny), (int)k, 0) > g.iso;
-    }
-    const uint64_t b = __ballot(in);
-    if (lane == u) mine = b;
-  }
-  if (lane < UNROLL && w0 + lane < g.nwords) bits[w0 + lane] = mine;
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane(
+      (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  const uint32_t rows = (uint32_t)g.ny * (uint32_t)g.nz;
+  for (uint32_t row = wave * kPlaneRowsPerWave;
+       row < min(rows, (wave + 1) * kPlaneRowsPerWave); ++row)
+    classify_plane_row(g, pl, bits, row, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -596,12 +627,11 @@ int launch_generate(const VolGeom& g, const float* field, const char* ws, const 
 // Slice classify over the whole volume (clears the count/scan state).
 int launch_bits_plane(const VolGeom& g, const FieldPlane& pl, char* ws, const Workspace& L,
                       hipStream_t s) {
-  constexpr int kUnroll = 8;
   const int64_t zero_words = L.ntiles + (int64_t)(sizeof(Ctrl) / 8);
-  const int64_t waves = ceil_div(g.nwords, kUnroll);
+  const int64_t waves = ceil_div((int64_t)g.ny * g.nz, kPlaneRowsPerWave);
   const int64_t blocks =
       std::max<int64_t>(std::max(ceil_div(waves * 64, 256), ceil_div(zero_words, 256)), 1);
-  k_classify_plane<kUnroll><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
+  k_classify_plane<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
       g, pl, (uint64_t*)(ws + L.off_bits), (unsigned long long*)(ws + L.off_status), zero_words);
   DAXHIP_LAUNCH_CHECK("k_classify_plane");
   return DAX_HIP_OK;
