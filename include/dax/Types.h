@@ -35,6 +35,13 @@ struct Tuple {
   DAX_EXEC_CONT_EXPORT explicit Tuple(const T& v) {
     for (int i = 0; i < N; ++i) c[i] = v;
   }
+  // Three components (dax/Types.h:444-449: Tuple<T,3>(x, y, z)).
+  DAX_EXEC_CONT_EXPORT Tuple(T x, T y, T z) {
+    static_assert(N == 3, "three components");
+    c[0] = x;
+    c[1] = y;
+    c[2] = z;
+  }
   DAX_EXEC_CONT_EXPORT T& operator[](int i) { return c[i]; }
   DAX_EXEC_CONT_EXPORT const T& operator[](int i) const { return c[i]; }
   DAX_EXEC_CONT_EXPORT bool operator==(const Tuple& o) const {

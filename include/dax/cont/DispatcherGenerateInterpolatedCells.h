@@ -1,5 +1,6 @@
 // DispatcherGenerateInterpolatedCells (dax/cont/DispatcherGenerateInterpolatedCells.h:
-// 34-309) for MarchingCubesGenerate on a UniformGrid with the HIP adapter.
+// 34-309) for MarchingCubesGenerate and SliceGenerate on a UniformGrid with
+// the HIP adapter.
 //
 // Same interface: constructor (count[, worklet]), SetRemoveDuplicatePoints
 // (default true), SetReleaseCount (default true), GetCount,
@@ -11,9 +12,10 @@
 // SetComputeNormals(true) also produces central-difference point-gradient
 // normals (GetPointNormals()).
 //
-// The HIP path classifies from the field and the worklet's isovalue; the
-// count handle (from DispatcherMapCell<MarchingCubesCount>) is only released
-// per SetReleaseCount, its values are not re-read.
+// The HIP path classifies from the field and the worklet's isovalue (or,
+// for SliceGenerate, from the plane: Invoke(grid, outGrid, coords)); the
+// count handle (from DispatcherMapCell<MarchingCubesCount / SliceCount>) is
+// only released per SetReleaseCount, its values are not re-read.
 #ifndef DAX_HIP_SLICE_DISPATCHERGENERATEINTERPOLATEDCELLS_H
 #define DAX_HIP_SLICE_DISPATCHERGENERATEINTERPOLATEDCELLS_H
 
@@ -23,6 +25,7 @@
 #include <dax/cont/UnstructuredGrid.h>
 #include <dax/cont/internal/HipWorkspace.h>
 #include <dax/worklet/MarchingCubes.h>
+#include <dax/worklet/Slice.h>
 
 namespace dax {
 namespace cont {
@@ -31,18 +34,18 @@ template <class WorkletType, class CountHandleType = ArrayHandle<dax::Id>,
           class DeviceAdapterTag = DAX_DEFAULT_DEVICE_ADAPTER_TAG>
 class DispatcherGenerateInterpolatedCells;
 
-template <class CountHandleType_>
-class DispatcherGenerateInterpolatedCells<dax::worklet::MarchingCubesGenerate, CountHandleType_,
-                                          dax::hip::cont::DeviceAdapterTagHip> {
+namespace internal {
+
+// What the HIP specialisations share: the flags, the count handle, the
+// interpolation records of the last Invoke and CompactPointField.
+template <class WorkletType_, class CountHandleType_>
+class InterpolatedCellsHip {
  public:
   typedef dax::hip::cont::DeviceAdapterTagHip DeviceAdapterTag;
-  typedef dax::worklet::MarchingCubesGenerate WorkletType;
+  typedef WorkletType_ WorkletType;
   typedef CountHandleType_ CountHandleType;
 
-  explicit DispatcherGenerateInterpolatedCells(const CountHandleType& count)
-      : RemoveDuplicatePoints(true), ReleaseCount(true), ComputeNormals(false), Count(count),
-        Worklet() {}
-  DispatcherGenerateInterpolatedCells(const CountHandleType& count, const WorkletType& work)
+  InterpolatedCellsHip(const CountHandleType& count, const WorkletType& work)
       : RemoveDuplicatePoints(true), ReleaseCount(true), ComputeNormals(false), Count(count),
         Worklet(work) {}
 
@@ -56,42 +59,6 @@ class DispatcherGenerateInterpolatedCells<dax::worklet::MarchingCubesGenerate, C
   const ArrayHandle<dax::Vector3, DAX_DEFAULT_ARRAY_CONTAINER_CONTROL_TAG, DeviceAdapterTag>&
   GetPointNormals() const {
     return Normals;
-  }
-
-  template <class CC, class PC, class CField>
-  void Invoke(const UniformGrid<DeviceAdapterTag>& inputGrid,
-              UnstructuredGrid<dax::CellTagTriangle, CC, PC, DeviceAdapterTag>& outputGrid,
-              const ArrayHandle<dax::Scalar, CField, DeviceAdapterTag>& field) {
-    static_assert(sizeof(dax::Id) == 4, "the HIP marching-cubes path writes 32-bit ids");
-    const dax::Id3 d = inputGrid.GetPointDimensions();
-    if (field.GetNumberOfValues() != inputGrid.GetNumberOfPoints())
-      throw ErrorControlBadValue("field size does not match the grid's point count");
-    const float* f = field.PrepareForInput().GetRawPointer();
-    const float iso = Worklet.GetIsoValue();
-    internal::HipWorkspace ws(dax_hip_mc_workspace_bytes(d[0], d[1], d[2]));
-    internal::HipWorkspace counts(2 * sizeof(int64_t));
-    internal::HipCheck(dax_hip_mc_classify(f, d[0], d[1], d[2], iso, ws.Data, ws.Bytes,
-                                           (int64_t*)counts.Data, nullptr));
-    int64_t T = 0, U = 0;
-    internal::HipCheck(dax_hip_mc_read_counts((const int64_t*)counts.Data, INT64_MAX, INT64_MAX,
-                                              &T, &U, nullptr));
-    if (GetReleaseCount()) DoReleaseCount();
-    if (T == 0) return;  // nothing to do (:190-194)
-    if (!RemoveDuplicatePoints) U = 3 * T;
-    const dax::Vector3 o = inputGrid.GetOrigin(), s = inputGrid.GetSpacing();
-    const float origin[3] = {o[0], o[1], o[2]}, spacing[3] = {s[0], s[1], s[2]};
-    int32_t* conn = (int32_t*)outputGrid.GetCellConnections().PrepareForOutput((dax::Id)(3 * T)).GetRawPointer();
-    float* coords = (float*)outputGrid.GetPointCoordinates().PrepareForOutput((dax::Id)U).GetRawPointer();
-    float* normals = ComputeNormals ? (float*)Normals.PrepareForOutput((dax::Id)U).GetRawPointer() : nullptr;
-    int32_t* first = InterpFirst.PrepareForOutput((dax::Id)U).GetRawPointer();
-    int32_t* second = InterpSecond.PrepareForOutput((dax::Id)U).GetRawPointer();
-    float* weight = InterpWeight.PrepareForOutput((dax::Id)U).GetRawPointer();
-    const unsigned flags = (RemoveDuplicatePoints ? DAX_HIP_MC_DEDUP : 0u) |
-                           (ComputeNormals ? DAX_HIP_MC_NORMALS : 0u);
-    internal::HipCheck(dax_hip_mc_generate(f, d[0], d[1], d[2], origin, spacing, iso, flags,
-                                           ws.Data, ws.Bytes, coords, normals, conn, nullptr, first,
-                                           second, weight, U, T, (int64_t*)counts.Data, nullptr));
-    internal::HipCheck(dax_hip_mc_read_counts((const int64_t*)counts.Data, U, T, &T, &U, nullptr));
   }
 
   // out[u] = Lerp(input[first_u], input[second_u], w_u) for every output point
@@ -121,13 +88,130 @@ class DispatcherGenerateInterpolatedCells<dax::worklet::MarchingCubesGenerate, C
     return true;
   }
 
- private:
+ protected:
+  // Sizes the outputs of a classified workspace (counts = {T, U}) and returns
+  // the pointers the generate step writes; false when there is nothing to do.
+  template <class CC, class PC>
+  bool PrepareOutputs(UnstructuredGrid<dax::CellTagTriangle, CC, PC, DeviceAdapterTag>& outputGrid,
+                      const int64_t* d_counts, int64_t& T, int64_t& U, int32_t*& conn,
+                      float*& coords, float*& normals, int32_t*& first, int32_t*& second,
+                      float*& weight) {
+    internal::HipCheck(dax_hip_mc_read_counts(d_counts, INT64_MAX, INT64_MAX, &T, &U, nullptr));
+    if (GetReleaseCount()) DoReleaseCount();
+    if (T == 0) return false;  // nothing to do (:190-194)
+    if (!RemoveDuplicatePoints) U = 3 * T;
+    conn = (int32_t*)outputGrid.GetCellConnections().PrepareForOutput((dax::Id)(3 * T)).GetRawPointer();
+    coords = (float*)outputGrid.GetPointCoordinates().PrepareForOutput((dax::Id)U).GetRawPointer();
+    normals = ComputeNormals ? (float*)Normals.PrepareForOutput((dax::Id)U).GetRawPointer() : nullptr;
+    first = InterpFirst.PrepareForOutput((dax::Id)U).GetRawPointer();
+    second = InterpSecond.PrepareForOutput((dax::Id)U).GetRawPointer();
+    weight = InterpWeight.PrepareForOutput((dax::Id)U).GetRawPointer();
+    return true;
+  }
+  unsigned Flags() const {
+    return (RemoveDuplicatePoints ? DAX_HIP_MC_DEDUP : 0u) | (ComputeNormals ? DAX_HIP_MC_NORMALS : 0u);
+  }
+
   bool RemoveDuplicatePoints, ReleaseCount, ComputeNormals;
   CountHandleType Count;
   WorkletType Worklet;
   ArrayHandle<int32_t, DAX_DEFAULT_ARRAY_CONTAINER_CONTROL_TAG, DeviceAdapterTag> InterpFirst, InterpSecond;
   ArrayHandle<float, DAX_DEFAULT_ARRAY_CONTAINER_CONTROL_TAG, DeviceAdapterTag> InterpWeight;
   ArrayHandle<dax::Vector3, DAX_DEFAULT_ARRAY_CONTAINER_CONTROL_TAG, DeviceAdapterTag> Normals;
+};
+
+}  // namespace internal
+
+template <class CountHandleType_>
+class DispatcherGenerateInterpolatedCells<dax::worklet::MarchingCubesGenerate, CountHandleType_,
+                                          dax::hip::cont::DeviceAdapterTagHip>
+    : public internal::InterpolatedCellsHip<dax::worklet::MarchingCubesGenerate, CountHandleType_> {
+  typedef internal::InterpolatedCellsHip<dax::worklet::MarchingCubesGenerate, CountHandleType_> Base;
+
+ public:
+  typedef typename Base::DeviceAdapterTag DeviceAdapterTag;
+  typedef typename Base::WorkletType WorkletType;
+  typedef typename Base::CountHandleType CountHandleType;
+
+  explicit DispatcherGenerateInterpolatedCells(const CountHandleType& count)
+      : Base(count, WorkletType()) {}
+  DispatcherGenerateInterpolatedCells(const CountHandleType& count, const WorkletType& work)
+      : Base(count, work) {}
+
+  template <class CC, class PC, class CField>
+  void Invoke(const UniformGrid<DeviceAdapterTag>& inputGrid,
+              UnstructuredGrid<dax::CellTagTriangle, CC, PC, DeviceAdapterTag>& outputGrid,
+              const ArrayHandle<dax::Scalar, CField, DeviceAdapterTag>& field) {
+    static_assert(sizeof(dax::Id) == 4, "the HIP marching-cubes path writes 32-bit ids");
+    const dax::Id3 d = inputGrid.GetPointDimensions();
+    if (field.GetNumberOfValues() != inputGrid.GetNumberOfPoints())
+      throw ErrorControlBadValue("field size does not match the grid's point count");
+    const float* f = field.PrepareForInput().GetRawPointer();
+    const float iso = this->Worklet.GetIsoValue();
+    internal::HipWorkspace ws(dax_hip_mc_workspace_bytes(d[0], d[1], d[2]));
+    internal::HipWorkspace counts(2 * sizeof(int64_t));
+    internal::HipCheck(dax_hip_mc_classify(f, d[0], d[1], d[2], iso, ws.Data, ws.Bytes,
+                                           (int64_t*)counts.Data, nullptr));
+    int64_t T = 0, U = 0;
+    int32_t *conn, *first, *second;
+    float *coords, *normals, *weight;
+    if (!this->PrepareOutputs(outputGrid, (const int64_t*)counts.Data, T, U, conn, coords,
+                              normals, first, second, weight))
+      return;
+    const dax::Vector3 o = inputGrid.GetOrigin(), s = inputGrid.GetSpacing();
+    const float origin[3] = {o[0], o[1], o[2]}, spacing[3] = {s[0], s[1], s[2]};
+    internal::HipCheck(dax_hip_mc_generate(f, d[0], d[1], d[2], origin, spacing, iso,
+                                           this->Flags(), ws.Data, ws.Bytes, coords, normals, conn,
+                                           nullptr, first, second, weight, U, T,
+                                           (int64_t*)counts.Data, nullptr));
+    internal::HipCheck(dax_hip_mc_read_counts((const int64_t*)counts.Data, U, T, &T, &U, nullptr));
+  }
+};
+
+template <class CountHandleType_>
+class DispatcherGenerateInterpolatedCells<dax::worklet::SliceGenerate, CountHandleType_,
+                                          dax::hip::cont::DeviceAdapterTagHip>
+    : public internal::InterpolatedCellsHip<dax::worklet::SliceGenerate, CountHandleType_> {
+  typedef internal::InterpolatedCellsHip<dax::worklet::SliceGenerate, CountHandleType_> Base;
+
+ public:
+  typedef typename Base::DeviceAdapterTag DeviceAdapterTag;
+  typedef typename Base::WorkletType WorkletType;
+  typedef typename Base::CountHandleType CountHandleType;
+
+  DispatcherGenerateInterpolatedCells(const CountHandleType& count, const WorkletType& work)
+      : Base(count, work) {}
+
+  // coords: the grid's point coordinates (implicit on a uniform grid; the
+  // kernels form them from the grid's origin and spacing).
+  template <class CC, class PC, class CCoords>
+  void Invoke(const UniformGrid<DeviceAdapterTag>& inputGrid,
+              UnstructuredGrid<dax::CellTagTriangle, CC, PC, DeviceAdapterTag>& outputGrid,
+              const ArrayHandle<dax::Vector3, CCoords, DeviceAdapterTag>& coords) {
+    static_assert(sizeof(dax::Id) == 4, "the HIP slice path writes 32-bit ids");
+    (void)coords;
+    const dax::Id3 d = inputGrid.GetPointDimensions();
+    const dax::Vector3 o = inputGrid.GetOrigin(), s = inputGrid.GetSpacing();
+    const dax::Vector3 po = this->Worklet.GetOrigin(), pn = this->Worklet.GetNormal();
+    const float origin[3] = {o[0], o[1], o[2]}, spacing[3] = {s[0], s[1], s[2]};
+    const float porig[3] = {po[0], po[1], po[2]}, pnorm[3] = {pn[0], pn[1], pn[2]};
+    internal::HipWorkspace ws(dax_hip_mc_workspace_bytes(d[0], d[1], d[2]));
+    internal::HipWorkspace counts(2 * sizeof(int64_t));
+    internal::HipCheck(dax_hip_mc_slice_classify(d[0], d[1], d[2], origin, spacing, porig, pnorm,
+                                                 ws.Data, ws.Bytes, (int64_t*)counts.Data,
+                                                 nullptr));
+    int64_t T = 0, U = 0;
+    int32_t *conn, *first, *second;
+    float *cpts, *normals, *weight;
+    if (!this->PrepareOutputs(outputGrid, (const int64_t*)counts.Data, T, U, conn, cpts, normals,
+                              first, second, weight))
+      return;
+    internal::HipCheck(dax_hip_mc_slice(d[0], d[1], d[2], origin, spacing, porig, pnorm,
+                                        this->Flags(), ws.Data, ws.Bytes, cpts, normals, conn,
+                                        nullptr, first, second, weight, U, T,
+                                        (int64_t*)counts.Data, nullptr));
+    internal::HipCheck(dax_hip_mc_read_counts((const int64_t*)counts.Data, U, T, &T, &U, nullptr));
+  }
 };
 
 }  // namespace cont

@@ -1,6 +1,7 @@
 // DispatcherMapCell (dax/cont/DispatcherMapCell.h:27-64) for the worklets of
 // the marching-cubes path on a UniformGrid with the HIP adapter:
 //   MarchingCubesCount(iso): Invoke(grid, field, count)   count[C] = NumFaces[case]
+//   SliceCount(o, n):        Invoke(grid, coords, count)  the same for the plane
 //   CellGradient:            Invoke(grid, coords, field, gradient)
 #ifndef DAX_HIP_SLICE_DISPATCHERMAPCELL_H
 #define DAX_HIP_SLICE_DISPATCHERMAPCELL_H
@@ -10,6 +11,7 @@
 #include <dax/cont/internal/HipWorkspace.h>
 #include <dax/worklet/CellGradient.h>
 #include <dax/worklet/MarchingCubes.h>
+#include <dax/worklet/Slice.h>
 
 namespace dax {
 namespace cont {
@@ -44,6 +46,37 @@ class DispatcherMapCell<dax::worklet::MarchingCubesCount, dax::hip::cont::Device
 
  private:
   dax::worklet::MarchingCubesCount Worklet;
+};
+
+template <>
+class DispatcherMapCell<dax::worklet::SliceCount, dax::hip::cont::DeviceAdapterTagHip> {
+ public:
+  typedef dax::hip::cont::DeviceAdapterTagHip Tag;
+  explicit DispatcherMapCell(const dax::worklet::SliceCount& w) : Worklet(w) {}
+
+  // coords: the grid's point coordinates (implicit on a uniform grid; the
+  // kernels form them from the grid's origin and spacing).
+  template <class CCoords, class CCount>
+  void Invoke(const UniformGrid<Tag>& grid, const ArrayHandle<dax::Vector3, CCoords, Tag>& coords,
+              ArrayHandle<dax::Id, CCount, Tag>& count) const {
+    static_assert(sizeof(dax::Id) == 4, "the HIP slice path writes 32-bit ids");
+    (void)coords;
+    const dax::Id3 d = grid.GetPointDimensions();
+    const dax::Vector3 o = grid.GetOrigin(), s = grid.GetSpacing();
+    const dax::Vector3 po = Worklet.GetOrigin(), pn = Worklet.GetNormal();
+    const float origin[3] = {o[0], o[1], o[2]}, spacing[3] = {s[0], s[1], s[2]};
+    const float porig[3] = {po[0], po[1], po[2]}, pnorm[3] = {pn[0], pn[1], pn[2]};
+    internal::HipWorkspace ws(dax_hip_mc_workspace_bytes(d[0], d[1], d[2]));
+    internal::HipCheck(dax_hip_mc_slice_classify(d[0], d[1], d[2], origin, spacing, porig, pnorm,
+                                                 ws.Data, ws.Bytes, nullptr, nullptr));
+    int32_t* c = count.PrepareForOutput(grid.GetNumberOfCells()).GetRawPointer();
+    internal::HipCheck(dax_hip_mc_cell_outputs(d[0], d[1], d[2], ws.Data, ws.Bytes, nullptr, c,
+                                               nullptr, nullptr));
+    internal::HipCheck(dax_hip_synchronize(nullptr));
+  }
+
+ private:
+  dax::worklet::SliceCount Worklet;
 };
 
 template <>

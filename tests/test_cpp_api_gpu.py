@@ -9,7 +9,7 @@ pytestmark = pytest.mark.gpu
 
 BIN = Path(__file__).resolve().parent / "cpp" / "bin"
 TESTS = ["UnitTestDeviceAdapterHip", "UnitTestWorkletMarchingCubesHip",
-         "UnitTestWorkletCellGradientHip"]
+         "UnitTestWorkletCellGradientHip", "UnitTestWorkletSliceHip"]
 
 
 @pytest.mark.parametrize("name", TESTS)
