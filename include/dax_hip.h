@@ -208,6 +208,37 @@ int dax_hip_mc_slice(int64_t nx, int64_t ny, int64_t nz, const float* h_origin3,
                      float* d_interp_weight, int64_t cap_points, int64_t cap_tris,
                      int64_t* d_counts, void* stream);
 
+/* Marching tetrahedra on an unstructured tetrahedral grid: replaces
+ * DispatcherMapCell<MarchingTetrahedraCount> +
+ * DispatcherGenerateInterpolatedCells<MarchingTetrahedraGenerate>
+ * (dax/worklet/MarchingTetrahedra.h:29-143; the reference test
+ * dax/worklet/testing/UnitTestWorkletMarchingTetrahedra.cxx:59-135).
+ * d_tets [n_tets][4] int32 point ids, d_field [n_points] f32, d_points
+ * [n_points][3] f32 coordinates.
+ * dax_hip_mt_count (synchronous): d_counts [n_tets] = NumFaces[case],
+ * d_offsets [n_tets] their exclusive scan, *h_tris = T; an id outside
+ * [0, n_points) is DAX_HIP_BAD_VALUE.
+ * dax_hip_mt_generate (synchronous with DAX_HIP_MC_DEDUP): the triangles'
+ * corners as interpolation records (first < second point id, weight
+ * (iso - f_first) / (f_second - f_first)) and coordinates (Lerp of the
+ * endpoints' coordinates); without dedup 3T points in corner order and
+ * identity connectivity (d_first/second/weight required), with dedup the
+ * unique edges sorted by (first, second) and connectivity by lower bound
+ * (the reference's Sort/Unique/LowerBounds, :261-301), d_first/second/
+ * weight optional.  d_vertices [3T][3] and d_conn [3T] (3T is the most
+ * points there can be); *h_points = U.  Dedup needs
+ * dax_hip_mt_scratch_bytes(T) bytes of d_scratch. */
+int dax_hip_mt_count(const int32_t* d_tets, int64_t n_tets, const float* d_field,
+                     int64_t n_points, float iso, int32_t* d_counts, int32_t* d_offsets,
+                     int64_t* h_tris, void* stream);
+size_t dax_hip_mt_scratch_bytes(int64_t n_tris);
+int dax_hip_mt_generate(const int32_t* d_tets, int64_t n_tets, const float* d_field,
+                        const float* d_points, int64_t n_points, float iso, unsigned flags,
+                        const int32_t* d_counts, const int32_t* d_offsets, int64_t n_tris,
+                        void* d_scratch, size_t scratch_bytes, float* d_vertices,
+                        int32_t* d_conn, int32_t* d_first, int32_t* d_second, float* d_weight,
+                        int64_t* h_points, void* stream);
+
 /* Synchronous: reads {T, U} written by a forward (or classify) on `stream`,
  * and returns DAX_HIP_CAPACITY if the generate step was skipped for lack of
  * room, DAX_HIP_EXEC_ERROR if T or U left the int32 dax::Id range. */

@@ -508,6 +508,85 @@ float oracle_slice_field(const int64_t* dims, const float* origin, const float* 
   return iso;
 }
 
+// Marching tetrahedra, Serial-adapter sequence (dax/worklet/MarchingTetrahedra.h:29-143
+// through DispatcherGenerateInterpolatedCells.h:138-301): per tetrahedron
+// NumFaces[case] (case = sum_v (f_v > iso) << v, :37-43); inclusive scan;
+// triangle -> cell by upper bounds and visit index by lower bounds (as the
+// MC path above); per triangle corner the edge from TriTable, endpoints
+// ordered by point id (:117-127), weight (iso - f_A) / (f_B - f_A)
+// (:130-131); dedup: sort by (first, second), unique on the pair, each
+// corner's point by lower bound (:261-301); coordinates Lerp (:107-141).
+// Table rows: the edges of each case's triangles (MarchingTetrahedraTable.h:28-54).
+static const char* const kTetTri[16] = {"",       "032", "014",    "143132", "125",    "031135",
+                                        "254024", "354", "345",    "204245", "015053", "152",
+                                        "341312", "041", "023",    ""};
+static const int kTetEdge[6][2] = {{0, 1}, {1, 2}, {2, 0}, {3, 0}, {3, 1}, {3, 2}};
+
+// The tetrahedron tables in the reference's byte layout: NumFaces[16],
+// TriTable[16][8] with 255 for unused entries.
+void oracle_tet_tables(uint8_t* num_faces, uint8_t* tri) {
+  for (int c = 0; c < 16; ++c) {
+    const size_t n = strlen(kTetTri[c]);
+    num_faces[c] = (uint8_t)(n / 3);
+    for (int k = 0; k < 8; ++k) tri[8 * c + k] = k < (int)n ? (uint8_t)(kTetTri[c][k] - '0') : 255;
+  }
+}
+
+int oracle_mt_generate(const int32_t* tets, int64_t n_tets, const float* field,
+                       const float* points, float iso, int dedup, int64_t cap,
+                       int64_t* n_tris, int64_t* n_pts, int32_t* conn, int32_t* first,
+                       int32_t* second, float* weight, float* coords) {
+  std::vector<int> cls(n_tets);
+  std::vector<Id> incl(n_tets);
+  Id run = 0;
+  for (int64_t c = 0; c < n_tets; ++c) {
+    int k = 0;
+    for (int v = 0; v < 4; ++v) k |= (field[tets[4 * c + v]] > iso) << v;
+    cls[c] = k;
+    run += (Id)(strlen(kTetTri[k]) / 3);
+    incl[c] = run;
+  }
+  const int64_t T = run;
+  *n_tris = T;
+  std::vector<PointAsEdgeInterpolation> rec(3 * T);
+  for (int64_t t = 0; t < T; ++t) {
+    const int64_t c = std::upper_bound(incl.begin(), incl.end(), (Id)t) - incl.begin();
+    const int64_t first_t = c == 0 ? 0 : incl[c - 1];
+    const int s = (int)(t - first_t);
+    const int32_t* id = tets + 4 * c;
+    for (int v = 0; v < 3; ++v) {
+      const int e = kTetTri[cls[c]][3 * s + v] - '0';
+      int va = kTetEdge[e][0], vb = kTetEdge[e][1];
+      if (!(id[va] < id[vb])) std::swap(va, vb);
+      const Scalar w = (iso - field[id[va]]) / (field[id[vb]] - field[id[va]]);
+      rec[3 * t + v] = PointAsEdgeInterpolation{id[va], id[vb], w};
+    }
+  }
+  std::vector<PointAsEdgeInterpolation> pts;
+  if (dedup) {
+    pts = rec;
+    std::sort(pts.begin(), pts.end());
+    pts.erase(std::unique(pts.begin(), pts.end()), pts.end());
+    for (int64_t i = 0; i < 3 * T; ++i)
+      conn[i] = (int32_t)(std::lower_bound(pts.begin(), pts.end(), rec[i]) - pts.begin());
+  } else {
+    pts = rec;
+    for (int64_t i = 0; i < 3 * T; ++i) conn[i] = (int32_t)i;
+  }
+  *n_pts = (int64_t)pts.size();
+  if ((int64_t)pts.size() > cap) return 1;
+  for (size_t u = 0; u < pts.size(); ++u) {
+    first[u] = pts[u].first;
+    second[u] = pts[u].second;
+    weight[u] = pts[u].weight;
+    for (int d = 0; d < 3; ++d) {
+      const Scalar a = points[3 * (int64_t)pts[u].first + d], b = points[3 * (int64_t)pts[u].second + d];
+      coords[3 * u + d] = a + pts[u].weight * (b - a);
+    }
+  }
+  return 0;
+}
+
 // Analytic adjoint in FP64, scatter form: dL/df for every point.
 void oracle_mc_backward_f64(const float* field, const int64_t* dims, const float* origin,
                             const float* spacing, float iso, const int32_t* first,
