@@ -23,6 +23,8 @@ the headline the line carries
   fwd_bwd_256  the differentiable workload: the same at 256^3;
   slice_512  a slice of the 512^3 grid by a tilted plane (no field read),
            with the OpenMP restatement beside it;
+  tetrahedra_128  marching tetrahedra over the 12.3 M Freudenthal
+           tetrahedra of a 128^3 grid, with the serial restatement beside it;
   batched  1024 volumes with per-axis extents U{32..160}, 4-16 Gaussians each,
            iso U[0.3,0.7]: the batched forward and forward+backward replayed
            as captured graphs, against a native launch loop of single-volume
@@ -84,7 +86,8 @@ def parse():
     ap.add_argument("--batch", type=int, default=1024, help="volumes of the batched run (0: skip)")
     ap.add_argument("--no-bwd", action="store_true", help="skip the forward+backward timing")
     ap.add_argument("--large", action="store_true", help="also run the 1024^3 Mandelbulb")
-    ap.add_argument("--only", choices=["fwd", "batch", "large", "diff", "slice"], default=None,
+    ap.add_argument("--only", choices=["fwd", "batch", "large", "diff", "slice", "tets"],
+                    default=None,
                     help="run one section only (profiling); prints its JSON")
     return ap.parse_args()
 
@@ -362,6 +365,44 @@ def slice_512(args):
     return out
 
 
+def tetrahedra_128(args):
+    """Marching tetrahedra over the Freudenthal tetrahedra of a 128^3 grid
+    (6 per voxel, as the reference's test grids) on the 128^3 sphere-distance
+    field, iso 40, dedup on; the call is synchronous (counts and the unique
+    size come back to the host), so each rep is timed whole with HIP events.
+    Beside it the oracle's serial restatement on the same input."""
+    import torch
+
+    from daxtoolkit_amd import fields, mc
+    from oracle import oracle as orc
+    n = 128
+    tets_h, pts_h = orc.freudenthal_tets((n, n, n))
+    f_h = fields.sphere_distance(n)
+    iso = 40.0
+    tets, pts = torch.from_numpy(tets_h).cuda(), torch.from_numpy(pts_h).cuda()
+    f = torch.from_numpy(f_h.reshape(-1)).cuda()
+    out = mc.tetrahedra(tets, f, pts, iso)
+    T, U = out.num_triangles, out.num_points
+    reps = max(3, min(args.steps, 10))
+    ms = _timed(lambda: mc.tetrahedra(tets, f, pts, iso), reps)
+    C = tets_h.shape[0]
+    res = {"workload": f"marching tetrahedra, Freudenthal tetrahedra of a {n}^3 grid "
+                       f"({C} tetrahedra), sphere distance |p - (63.5,63.5,63.5)|, iso 40, "
+                       f"dedup",
+           "tetrahedra": C, "triangles": T, "vertices": U, "ms": round(ms, 4),
+           "Gtets_per_s": round(C / (ms * 1e-3) / 1e9, 3)}
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        ref = orc.mt_generate(tets_h, f_h.reshape(-1), pts_h, iso, dedup=True)
+        sec = time.perf_counter() - t0
+        assert (ref["n_tris"], ref["n_pts"]) == (T, U)
+        res["cpu_baseline"] = {"value": round(C / sec / 1e9, 4), "unit": "Gtets/s", "cores": 1,
+                               "kind": "port", "sample": f"the full input, serial restatement "
+                                                         f"(oracle), one run, {sec:.2f} s"}
+    _progress(f"tetrahedra {n}^3: {ms:.3f} ms, T={T}")
+    return res
+
+
 def large(args):
     import torch
 
@@ -412,6 +453,9 @@ def main():
         return
     if args.only == "slice":
         print(json.dumps({"slice_512": slice_512(args)}), flush=True)
+        return
+    if args.only == "tets":
+        print(json.dumps({"tetrahedra_128": tetrahedra_128(args)}), flush=True)
         return
     n = args.n
     dims = (n, n, n)
@@ -485,6 +529,7 @@ def main():
         if not args.no_bwd:
             line["fwd_bwd_256"] = differentiable_256(args)
         line["slice_512"] = slice_512(args)
+        line["tetrahedra_128"] = tetrahedra_128(args)
         if args.batch > 0:
             line["batched"] = batched(args)
         if args.large:
