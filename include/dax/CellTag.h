@@ -1,4 +1,4 @@
-// Cell tags and traits used by the marching-cubes path
+// Cell tags and traits used by the marching-cubes / -tetrahedra paths
 // (dax/CellTag.h, dax/CellTraits.h:83-145; Voxel's canonical cell is the
 // Hexahedron, CellTraits.h:131-137).
 #ifndef DAX_HIP_SLICE_CELLTAG_H
@@ -9,6 +9,7 @@ namespace dax {
 struct CellTagHexahedron {};
 struct CellTagVoxel {};
 struct CellTagTriangle {};
+struct CellTagTetrahedron {};
 
 template <class CellTag>
 struct CellTraits;
@@ -21,6 +22,11 @@ template <>
 struct CellTraits<CellTagVoxel> {
   enum { NUM_VERTICES = 8, TOPOLOGICAL_DIMENSIONS = 3 };
   typedef CellTagHexahedron CanonicalCellTag;
+};
+template <>
+struct CellTraits<CellTagTetrahedron> {
+  enum { NUM_VERTICES = 4, TOPOLOGICAL_DIMENSIONS = 3 };
+  typedef CellTagTetrahedron CanonicalCellTag;
 };
 template <>
 struct CellTraits<CellTagTriangle> {

@@ -1,5 +1,6 @@
 // DispatcherGenerateInterpolatedCells (dax/cont/DispatcherGenerateInterpolatedCells.h:
-// 34-309) for MarchingCubesGenerate and SliceGenerate on a UniformGrid with
+// 34-309) for MarchingCubesGenerate and SliceGenerate on a UniformGrid and
+// MarchingTetrahedraGenerate on an UnstructuredGrid<CellTagTetrahedron> with
 // the HIP adapter.
 //
 // Same interface: constructor (count[, worklet]), SetRemoveDuplicatePoints
@@ -25,6 +26,7 @@
 #include <dax/cont/UnstructuredGrid.h>
 #include <dax/cont/internal/HipWorkspace.h>
 #include <dax/worklet/MarchingCubes.h>
+#include <dax/worklet/MarchingTetrahedra.h>
 #include <dax/worklet/Slice.h>
 
 namespace dax {
@@ -211,6 +213,62 @@ class DispatcherGenerateInterpolatedCells<dax::worklet::SliceGenerate, CountHand
                                         nullptr, first, second, weight, U, T,
                                         (int64_t*)counts.Data, nullptr));
     internal::HipCheck(dax_hip_mc_read_counts((const int64_t*)counts.Data, U, T, &T, &U, nullptr));
+  }
+};
+
+template <class CountHandleType_>
+class DispatcherGenerateInterpolatedCells<dax::worklet::MarchingTetrahedraGenerate,
+                                          CountHandleType_, dax::hip::cont::DeviceAdapterTagHip>
+    : public internal::InterpolatedCellsHip<dax::worklet::MarchingTetrahedraGenerate,
+                                            CountHandleType_> {
+  typedef internal::InterpolatedCellsHip<dax::worklet::MarchingTetrahedraGenerate,
+                                         CountHandleType_>
+      Base;
+
+ public:
+  typedef typename Base::DeviceAdapterTag DeviceAdapterTag;
+  typedef typename Base::WorkletType WorkletType;
+  typedef typename Base::CountHandleType CountHandleType;
+
+  DispatcherGenerateInterpolatedCells(const CountHandleType& count, const WorkletType& work)
+      : Base(count, work) {}
+
+  // Normals are not produced for tetrahedra.
+  template <class CConn, class CPts, class CC, class PC, class CField>
+  void Invoke(const UnstructuredGrid<dax::CellTagTetrahedron, CConn, CPts, DeviceAdapterTag>& in,
+              UnstructuredGrid<dax::CellTagTriangle, CC, PC, DeviceAdapterTag>& outputGrid,
+              const ArrayHandle<dax::Scalar, CField, DeviceAdapterTag>& field) {
+    static_assert(sizeof(dax::Id) == 4, "the HIP tetrahedra path writes 32-bit ids");
+    if (field.GetNumberOfValues() != in.GetNumberOfPoints())
+      throw ErrorControlBadValue("field size does not match the grid's point count");
+    const dax::Id n = in.GetNumberOfCells(), P = in.GetNumberOfPoints();
+    const int32_t* tets = in.GetCellConnections().PrepareForInput().GetRawPointer();
+    const float* f = field.PrepareForInput().GetRawPointer();
+    const float* pts = (const float*)in.GetPointCoordinates().PrepareForInput().GetRawPointer();
+    const float iso = this->Worklet.GetIsoValue();
+    internal::HipWorkspace counts((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    internal::HipWorkspace offs((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    int64_t T = 0, U = 0;
+    internal::HipCheck(dax_hip_mt_count(tets, n, f, P, iso, (int32_t*)counts.Data,
+                                        (int32_t*)offs.Data, &T, nullptr));
+    if (this->GetReleaseCount()) this->DoReleaseCount();
+    if (T == 0) return;  // nothing to do (:190-194)
+    const dax::Id cap = (dax::Id)(3 * T);
+    int32_t* conn = outputGrid.GetCellConnections().PrepareForOutput(cap).GetRawPointer();
+    float* coords = (float*)outputGrid.GetPointCoordinates().PrepareForOutput(cap).GetRawPointer();
+    int32_t* first = this->InterpFirst.PrepareForOutput(cap).GetRawPointer();
+    int32_t* second = this->InterpSecond.PrepareForOutput(cap).GetRawPointer();
+    float* weight = this->InterpWeight.PrepareForOutput(cap).GetRawPointer();
+    const unsigned flags = this->RemoveDuplicatePoints ? DAX_HIP_MC_DEDUP : 0u;
+    internal::HipWorkspace scratch(this->RemoveDuplicatePoints ? dax_hip_mt_scratch_bytes(T) : 1);
+    internal::HipCheck(dax_hip_mt_generate(tets, n, f, pts, P, iso, flags,
+                                           (const int32_t*)counts.Data, (const int32_t*)offs.Data,
+                                           T, scratch.Data, scratch.Bytes, coords, conn, first,
+                                           second, weight, &U, nullptr));
+    outputGrid.GetPointCoordinates().Shrink((dax::Id)U);
+    this->InterpFirst.Shrink((dax::Id)U);
+    this->InterpSecond.Shrink((dax::Id)U);
+    this->InterpWeight.Shrink((dax::Id)U);
   }
 };
 

@@ -2,15 +2,19 @@
 // the marching-cubes path on a UniformGrid with the HIP adapter:
 //   MarchingCubesCount(iso): Invoke(grid, field, count)   count[C] = NumFaces[case]
 //   SliceCount(o, n):        Invoke(grid, coords, count)  the same for the plane
+//   MarchingTetrahedraCount(iso): Invoke(tetGrid, field, count) on an
+//                            UnstructuredGrid<CellTagTetrahedron>
 //   CellGradient:            Invoke(grid, coords, field, gradient)
 #ifndef DAX_HIP_SLICE_DISPATCHERMAPCELL_H
 #define DAX_HIP_SLICE_DISPATCHERMAPCELL_H
 
 #include <dax/cont/ArrayHandle.h>
 #include <dax/cont/UniformGrid.h>
+#include <dax/cont/UnstructuredGrid.h>
 #include <dax/cont/internal/HipWorkspace.h>
 #include <dax/worklet/CellGradient.h>
 #include <dax/worklet/MarchingCubes.h>
+#include <dax/worklet/MarchingTetrahedra.h>
 #include <dax/worklet/Slice.h>
 
 namespace dax {
@@ -77,6 +81,35 @@ class DispatcherMapCell<dax::worklet::SliceCount, dax::hip::cont::DeviceAdapterT
 
  private:
   dax::worklet::SliceCount Worklet;
+};
+
+template <>
+class DispatcherMapCell<dax::worklet::MarchingTetrahedraCount,
+                        dax::hip::cont::DeviceAdapterTagHip> {
+ public:
+  typedef dax::hip::cont::DeviceAdapterTagHip Tag;
+  explicit DispatcherMapCell(const dax::worklet::MarchingTetrahedraCount& w) : Worklet(w) {}
+
+  template <class CConn, class CPts, class CField, class CCount>
+  void Invoke(const UnstructuredGrid<dax::CellTagTetrahedron, CConn, CPts, Tag>& grid,
+              const ArrayHandle<dax::Scalar, CField, Tag>& field,
+              ArrayHandle<dax::Id, CCount, Tag>& count) const {
+    static_assert(sizeof(dax::Id) == 4, "the HIP tetrahedra path reads 32-bit ids");
+    if (field.GetNumberOfValues() != grid.GetNumberOfPoints())
+      throw ErrorControlBadValue("field size does not match the grid's point count");
+    const dax::Id n = grid.GetNumberOfCells();
+    const int32_t* tets = grid.GetCellConnections().PrepareForInput().GetRawPointer();
+    const float* f = field.PrepareForInput().GetRawPointer();
+    int32_t* c = count.PrepareForOutput(n).GetRawPointer();
+    internal::HipWorkspace offs((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    int64_t T = 0;
+    internal::HipCheck(dax_hip_mt_count(tets, n, f, grid.GetNumberOfPoints(),
+                                        Worklet.GetIsoValue(), c, (int32_t*)offs.Data, &T,
+                                        nullptr));
+  }
+
+ private:
+  dax::worklet::MarchingTetrahedraCount Worklet;
 };
 
 template <>

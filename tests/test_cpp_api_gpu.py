@@ -9,7 +9,8 @@ pytestmark = pytest.mark.gpu
 
 BIN = Path(__file__).resolve().parent / "cpp" / "bin"
 TESTS = ["UnitTestDeviceAdapterHip", "UnitTestWorkletMarchingCubesHip",
-         "UnitTestWorkletCellGradientHip", "UnitTestWorkletSliceHip"]
+         "UnitTestWorkletCellGradientHip", "UnitTestWorkletSliceHip",
+         "UnitTestWorkletMarchingTetrahedraHip"]
 
 
 @pytest.mark.parametrize("name", TESTS)
