@@ -25,6 +25,9 @@ the headline the line carries
            with the OpenMP restatement beside it;
   tetrahedra_128  marching tetrahedra over the 12.3 M Freudenthal
            tetrahedra of a 128^3 grid, with the serial restatement beside it;
+  hexahedra_256  marching cubes on the unstructured hexahedral grid of the
+           256^3 grid's voxels (the general dedup path), with the serial
+           restatement beside it;
   batched  1024 volumes with per-axis extents U{32..160}, 4-16 Gaussians each,
            iso U[0.3,0.7]: the batched forward and forward+backward replayed
            as captured graphs, against a native launch loop of single-volume
@@ -86,7 +89,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=1024, help="volumes of the batched run (0: skip)")
     ap.add_argument("--no-bwd", action="store_true", help="skip the forward+backward timing")
     ap.add_argument("--large", action="store_true", help="also run the 1024^3 Mandelbulb")
-    ap.add_argument("--only", choices=["fwd", "batch", "large", "diff", "slice", "tets"],
+    ap.add_argument("--only", choices=["fwd", "batch", "large", "diff", "slice", "tets", "hex"],
                     default=None,
                     help="run one section only (profiling); prints its JSON")
     return ap.parse_args()
@@ -374,24 +377,27 @@ def tetrahedra_128(args):
     import torch
 
     from daxtoolkit_amd import fields, mc
-    from oracle import oracle as orc
     n = 128
-    tets_h, pts_h = orc.freudenthal_tets((n, n, n))
+    tets, pts = fields.voxel_cells_torch((n, n, n), "tetrahedron")
     f_h = fields.sphere_distance(n)
     iso = 40.0
-    tets, pts = torch.from_numpy(tets_h).cuda(), torch.from_numpy(pts_h).cuda()
     f = torch.from_numpy(f_h.reshape(-1)).cuda()
     out = mc.tetrahedra(tets, f, pts, iso)
     T, U = out.num_triangles, out.num_points
     reps = max(3, min(args.steps, 10))
     ms = _timed(lambda: mc.tetrahedra(tets, f, pts, iso), reps)
-    C = tets_h.shape[0]
+    C = int(tets.shape[0])
     res = {"workload": f"marching tetrahedra, Freudenthal tetrahedra of a {n}^3 grid "
                        f"({C} tetrahedra), sphere distance |p - (63.5,63.5,63.5)|, iso 40, "
                        f"dedup",
            "tetrahedra": C, "triangles": T, "vertices": U, "ms": round(ms, 4),
-           "Gtets_per_s": round(C / (ms * 1e-3) / 1e9, 3)}
+           "Gtets_per_s": round(C / (ms * 1e-3) / 1e9, 3),
+           "alg_bytes": _cells_alg_bytes(C, 4, T, U)}
+    res["GBps"] = round(res["alg_bytes"] / (ms * 1e-3) / 1e9, 1)
+    res["frac_hbm_peak"] = round(res["GBps"] / HBM_PEAK_GBS, 4)
     if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        tets_h, pts_h = tets.cpu().numpy(), pts.cpu().numpy()
         t0 = time.perf_counter()
         ref = orc.mt_generate(tets_h, f_h.reshape(-1), pts_h, iso, dedup=True)
         sec = time.perf_counter() - t0
@@ -400,6 +406,63 @@ def tetrahedra_128(args):
                                "kind": "port", "sample": f"the full input, serial restatement "
                                                          f"(oracle), one run, {sec:.2f} s"}
     _progress(f"tetrahedra {n}^3: {ms:.3f} ms, T={T}")
+    return res
+
+
+def _cells_alg_bytes(C, nv, T, U):
+    """Algorithmic HBM bytes of one unstructured-grid contour: the count pass
+    reads the connectivity (4*nv B per cell) and writes a count (4 B); the
+    scan reads the counts and writes the offsets (8 B per cell); generate
+    reads the counts again (4 B per cell) plus, per output triangle, its
+    cell's ids and offset (4*nv + 4 B) and per corner an 8-byte edge key;
+    dedup sorts and looks up the 3T keys (counted once: read + write,
+    16 B) and writes U points (first, second, weight, xyz: 24 B).  Field
+    and coordinate gathers are counted once per point read (L2/MALL
+    resident for these sizes) and are left out."""
+    return C * (4 * nv + 4 + 8 + 4) + T * (4 * nv + 4) + 3 * T * (8 + 16) + U * 24
+
+
+def hexahedra_256(args):
+    """Marching cubes on the unstructured hexahedral grid of the 256^3
+    uniform grid's voxels (MarchingCubesCount / MarchingCubesGenerate on
+    UnstructuredGrid<Hexahedron>) on the 16-Gaussian field, iso 0.5, dedup
+    (the general sort/unique/lower-bounds path); synchronous like the
+    tetrahedra.  Beside it the oracle's serial restatement."""
+    import torch
+
+    from daxtoolkit_amd import fields, mc
+    n = 256
+    dims = (n, n, n)
+    field = fields.gaussian_mixture_torch(dims, fields.gaussian_mixture_params(0, dims))
+    hexes, pts = fields.voxel_cells_torch(dims, "hexahedron")
+    f = field.reshape(-1)
+    iso = 0.5
+    out = mc.hexahedra(hexes, f, pts, iso)
+    T, U = out.num_triangles, out.num_points
+    reps = max(3, min(args.steps, 10))
+    ms = _timed(lambda: mc.hexahedra(hexes, f, pts, iso), reps)
+    C = int(hexes.shape[0])
+    res = {"workload": f"marching cubes on an unstructured hexahedral grid (the {C} voxels of the "
+                       f"{n}^3 grid as hexahedra), 16-Gaussian mixture (seed 0) + "
+                       f"U(-0.01,0.01) noise, iso 0.5, dedup",
+           "hexahedra": C, "triangles": T, "vertices": U, "ms": round(ms, 4),
+           "Gcells_per_s": round(C / (ms * 1e-3) / 1e9, 3),
+           "alg_bytes": _cells_alg_bytes(C, 8, T, U)}
+    res["GBps"] = round(res["alg_bytes"] / (ms * 1e-3) / 1e9, 1)
+    res["frac_hbm_peak"] = round(res["GBps"] / HBM_PEAK_GBS, 4)
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        hexes_h, pts_h, f_h = hexes.cpu().numpy(), pts.cpu().numpy(), f.cpu().numpy()
+        t0 = time.perf_counter()
+        ref = orc.mc_hex_generate(hexes_h, f_h, pts_h, iso, dedup=True)
+        sec = time.perf_counter() - t0
+        assert (ref["n_tris"], ref["n_pts"]) == (T, U)
+        res["cpu_baseline"] = {"value": round(C / sec / 1e9, 4), "unit": "Gcells/s", "cores": 1,
+                               "kind": "port", "sample": f"the full input, serial restatement "
+                                                         f"(oracle), one run, {sec:.2f} s"}
+    del hexes, pts, field
+    torch.cuda.empty_cache()
+    _progress(f"hexahedra {n}^3: {ms:.3f} ms, T={T}")
     return res
 
 
@@ -456,6 +519,9 @@ def main():
         return
     if args.only == "tets":
         print(json.dumps({"tetrahedra_128": tetrahedra_128(args)}), flush=True)
+        return
+    if args.only == "hex":
+        print(json.dumps({"hexahedra_256": hexahedra_256(args)}), flush=True)
         return
     n = args.n
     dims = (n, n, n)
@@ -530,6 +596,7 @@ def main():
             line["fwd_bwd_256"] = differentiable_256(args)
         line["slice_512"] = slice_512(args)
         line["tetrahedra_128"] = tetrahedra_128(args)
+        line["hexahedra_256"] = hexahedra_256(args)
         if args.batch > 0:
             line["batched"] = batched(args)
         if args.large:

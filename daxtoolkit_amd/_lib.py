@@ -108,6 +108,10 @@ _SIGNATURES = {
     "dax_hip_mt_scratch_bytes": (sz, [i64]),
     "dax_hip_mt_generate": (C.c_int, [vp, i64, vp, vp, i64, f32, C.c_uint, vp, vp, i64, vp, sz,
                                       vp, vp, vp, vp, vp, P_i64, vp]),
+    "dax_hip_mc_hex_count": (C.c_int, [vp, i64, vp, i64, f32, vp, vp, P_i64, vp]),
+    "dax_hip_mc_hex_scratch_bytes": (sz, [i64]),
+    "dax_hip_mc_hex_generate": (C.c_int, [vp, i64, vp, vp, i64, f32, C.c_uint, vp, vp, i64, vp,
+                                          sz, vp, vp, vp, vp, vp, P_i64, vp]),
     "dax_hip_mc_slice": (C.c_int, [i64, i64, i64, vp, vp, vp, vp, C.c_uint, vp, sz, vp, vp, vp,
                                    vp, vp, vp, vp, i64, i64, vp, vp]),
     "dax_hip_mc_bits": (C.c_int, [vp, i64, i64, i64, f32, vp, sz, vp]),

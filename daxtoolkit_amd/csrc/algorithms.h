@@ -1,0 +1,14 @@
+// Internal entry points of the adapter algorithms (algorithms.hip) used by
+// other parts of the library.
+#pragma once
+#include "common.h"
+
+namespace daxhip {
+namespace alg {
+
+// Ascending, stream-ordered LSD radix sort of n unsigned keys that are all
+// below 2^bits (ceil(bits / 8) passes).
+int sort_u64_bits(uint64_t* d_keys, int64_t n, int bits, hipStream_t s);
+
+}  // namespace alg
+}  // namespace daxhip

@@ -17,7 +17,7 @@
 #include <algorithm>
 #include <type_traits>
 
-#include "common.h"
+#include "algorithms.h"
 
 namespace daxhip {
 namespace alg {
@@ -171,21 +171,6 @@ __global__ __launch_bounds__(kNT) void k_tile_scan(const T* __restrict__ in, int
   }
 }
 
-struct Temp {
-  void* p = nullptr;
-  ~Temp() {
-    if (p) (void)hipFree(p);
-  }
-  int alloc(size_t bytes) {
-    hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
-    if (e != hipSuccess) {
-      p = nullptr;
-      return check_hip(e, "hipMalloc (adapter temporary)");
-    }
-    return DAX_HIP_OK;
-  }
-};
-
 template <typename T>
 int scan(const T* in, T* out, int64_t n, int inclusive, T* h_total, hipStream_t s) {
   if (n <= 0) {
@@ -193,7 +178,7 @@ int scan(const T* in, T* out, int64_t n, int inclusive, T* h_total, hipStream_t 
     return DAX_HIP_OK;
   }
   const int64_t tiles = ceil_div(n, (int64_t)kTile);
-  Temp tmp;
+  TempBuf tmp(s);
   int rc = tmp.alloc((size_t)(tiles + 1) * sizeof(T));
   if (rc) return rc;
   T* sums = (T*)tmp.p;
@@ -203,10 +188,11 @@ int scan(const T* in, T* out, int64_t n, int inclusive, T* h_total, hipStream_t 
   DAXHIP_LAUNCH_CHECK("k_scan_sums");
   k_tile_scan<T><<<dim3((unsigned)tiles), dim3(kNT), 0, s>>>(in, n, sums, out, inclusive);
   DAXHIP_LAUNCH_CHECK("k_tile_scan");
+  if (!h_total) return DAX_HIP_OK;  // stream-ordered, no host round trip
   T total;
   DAXHIP_CHECK(hipMemcpyAsync(&total, sums + tiles, sizeof(T), hipMemcpyDeviceToHost, s));
   DAXHIP_CHECK(hipStreamSynchronize(s));
-  if (h_total) *h_total = total;
+  *h_total = total;
   return DAX_HIP_OK;
 }
 
@@ -289,8 +275,10 @@ struct RadixKey<double> {
   }
 };
 
-constexpr int kSortIPT = 16;                 // items per thread per sort tile
-constexpr int kSortTile = kNT * kSortIPT;    // 4096 keys
+// Items per thread per sort tile: 16 (4096-key tiles) for large inputs, 4
+// (1024-key tiles) below kSmallSort keys, so that small sorts still spread
+// over the CUs.
+constexpr int64_t kSmallSort = 1 << 22;
 
 template <typename T>
 __global__ void k_encode(const T* __restrict__ in, typename RadixKey<T>::U* __restrict__ out,
@@ -312,9 +300,10 @@ __global__ void k_decode(const typename RadixKey<T>::U* __restrict__ in, T* __re
 }
 
 // Per-tile digit histograms, digit-major: hist[digit * ntiles + tile].
-template <typename U>
+template <typename U, int kSortIPT>
 __global__ __launch_bounds__(kNT) void k_radix_hist(const U* __restrict__ keys, int64_t n, int shift,
                                                     uint32_t* __restrict__ hist, int64_t ntiles) {
+  constexpr int kSortTile = kNT * kSortIPT;
   __shared__ uint32_t s_h[256];
   s_h[threadIdx.x] = 0;
   __syncthreads();
@@ -331,13 +320,14 @@ __global__ __launch_bounds__(kNT) void k_radix_hist(const U* __restrict__ keys, 
 // Stable scatter: the tile is processed in rounds of 256 keys in index order;
 // within a round each wave ranks its 64 keys per digit with ballots, waves
 // are ordered by a per-round digit-count scan in LDS.
-template <typename U, bool WITH_IDX>
+template <typename U, bool WITH_IDX, int kSortIPT>
 __global__ __launch_bounds__(kNT) void k_radix_scatter(const U* __restrict__ keys_in,
                                                        const int32_t* __restrict__ idx_in,
                                                        int64_t n, int shift,
                                                        const uint32_t* __restrict__ offsets,
                                                        int64_t ntiles, U* __restrict__ keys_out,
                                                        int32_t* __restrict__ idx_out) {
+  constexpr int kSortTile = kNT * kSortIPT;
   __shared__ uint32_t s_base[256];          // running output offset per digit
   __shared__ uint32_t s_wcnt[kNT / 64][256];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -383,30 +373,40 @@ __global__ __launch_bounds__(kNT) void k_radix_scatter(const U* __restrict__ key
   }
 }
 
-template <typename U>
-int radix_sort_encoded(U*& keys, U*& keys_alt, int32_t*& idx, int32_t*& idx_alt, int64_t n,
-                       hipStream_t s) {
+// LSD passes over the low `bits` bits of the encoded keys (8 bits a pass);
+// the result is in `keys` (the buffers are swapped after every pass).
+template <typename U, int kSortIPT>
+int radix_passes(U*& keys, U*& keys_alt, int32_t*& idx, int32_t*& idx_alt, int64_t n, int bits,
+                 hipStream_t s) {
+  constexpr int kSortTile = kNT * kSortIPT;
   const int64_t ntiles = ceil_div(n, (int64_t)kSortTile);
-  Temp th;
+  TempBuf th(s);
   int rc = th.alloc((size_t)ntiles * 256 * sizeof(uint32_t));
   if (rc) return rc;
   uint32_t* hist = (uint32_t*)th.p;
-  for (int shift = 0; shift < (int)(8 * sizeof(U)); shift += 8) {
-    k_radix_hist<U><<<dim3((unsigned)ntiles), dim3(kNT), 0, s>>>(keys, n, shift, hist, ntiles);
+  for (int shift = 0; shift < bits; shift += 8) {
+    k_radix_hist<U, kSortIPT><<<dim3((unsigned)ntiles), dim3(kNT), 0, s>>>(keys, n, shift, hist,
+                                                                          ntiles);
     DAXHIP_LAUNCH_CHECK("k_radix_hist");
-    uint32_t total = 0;
-    if ((rc = scan<uint32_t>(hist, hist, ntiles * 256, 0, &total, s))) return rc;
+    if ((rc = scan<uint32_t>(hist, hist, ntiles * 256, 0, nullptr, s))) return rc;
     if (idx)
-      k_radix_scatter<U, true><<<dim3((unsigned)ntiles), dim3(kNT), 0, s>>>(
+      k_radix_scatter<U, true, kSortIPT><<<dim3((unsigned)ntiles), dim3(kNT), 0, s>>>(
           keys, idx, n, shift, hist, ntiles, keys_alt, idx_alt);
     else
-      k_radix_scatter<U, false><<<dim3((unsigned)ntiles), dim3(kNT), 0, s>>>(
+      k_radix_scatter<U, false, kSortIPT><<<dim3((unsigned)ntiles), dim3(kNT), 0, s>>>(
           keys, nullptr, n, shift, hist, ntiles, keys_alt, nullptr);
     DAXHIP_LAUNCH_CHECK("k_radix_scatter");
     std::swap(keys, keys_alt);
     std::swap(idx, idx_alt);
   }
   return DAX_HIP_OK;
+}
+
+template <typename U>
+int radix_sort_encoded(U*& keys, U*& keys_alt, int32_t*& idx, int32_t*& idx_alt, int64_t n,
+                       hipStream_t s, int bits = 8 * (int)sizeof(U)) {
+  return n < kSmallSort ? radix_passes<U, 4>(keys, keys_alt, idx, idx_alt, n, bits, s)
+                        : radix_passes<U, 16>(keys, keys_alt, idx, idx_alt, n, bits, s);
 }
 
 template <typename T>
@@ -424,7 +424,7 @@ int sort(T* d, int64_t n, int desc, int32_t* perm_out, hipStream_t s) {
     return DAX_HIP_BAD_VALUE;
   }
   typedef typename RadixKey<T>::U U;
-  Temp tk;
+  TempBuf tk(s);
   const bool with_idx = perm_out != nullptr;
   int rc = tk.alloc((size_t)n * (2 * sizeof(U) + (with_idx ? 2 * sizeof(int32_t) : 0)));
   if (rc) return rc;
@@ -440,6 +440,24 @@ int sort(T* d, int64_t n, int desc, int32_t* perm_out, hipStream_t s) {
   DAXHIP_LAUNCH_CHECK("k_decode");
   if (with_idx) DAXHIP_CHECK(hipMemcpyAsync(perm_out, i0, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
   DAXHIP_CHECK(hipStreamSynchronize(s));
+  return DAX_HIP_OK;
+}
+
+// Ascending, stream-ordered sort of unsigned keys that are below 2^bits.
+int sort_u64_bits(uint64_t* d, int64_t n, int bits, hipStream_t s) {
+  if (n <= 1 || bits <= 0) return DAX_HIP_OK;
+  if (n > 0x7fffffffLL || bits > 64) {
+    set_error("sort_u64_bits: more than 2^31-1 keys or more than 64 bits");
+    return DAX_HIP_BAD_VALUE;
+  }
+  TempBuf tk(s);
+  int rc = tk.alloc((size_t)n * sizeof(uint64_t));
+  if (rc) return rc;
+  uint64_t* k0 = d;
+  uint64_t* k1 = (uint64_t*)tk.p;
+  int32_t *i0 = nullptr, *i1 = nullptr;
+  if ((rc = radix_sort_encoded<uint64_t>(k0, k1, i0, i1, n, s, bits))) return rc;
+  if (k0 != d) return copy_bytes(d, k0, (size_t)n * sizeof(uint64_t), s);
   return DAX_HIP_OK;
 }
 
@@ -460,7 +478,7 @@ __global__ void k_gather_bytes(const uint8_t* __restrict__ in, const int32_t* __
 template <typename T>
 int sort_by_key(T* keys, void* values, size_t value_bytes, int64_t n, int desc, hipStream_t s) {
   if (n <= 1) return DAX_HIP_OK;
-  Temp tp;
+  TempBuf tp(s);
   int rc = tp.alloc((size_t)n * (4 + value_bytes));
   if (rc) return rc;
   int32_t* perm = (int32_t*)tp.p;
@@ -516,7 +534,7 @@ __global__ void k_scatter_flagged(const uint8_t* __restrict__ in, int64_t rec,
 // Compact `in` (records of rec bytes) or indices by flags; returns the count.
 int compact(const void* in, int64_t rec, int32_t* flags, int64_t n, void* out, int32_t* idx32,
             int64_t* idx64, int64_t* h_count, hipStream_t s) {
-  Temp tp;
+  TempBuf tp(s);
   int rc = tp.alloc((size_t)n * 4);
   if (rc) return rc;
   int32_t* pos = (int32_t*)tp.p;
@@ -536,7 +554,7 @@ int unique(T* v, int64_t n, int64_t* h_new_n, hipStream_t s) {
     *h_new_n = n;
     return DAX_HIP_OK;
   }
-  Temp tp;
+  TempBuf tp(s);
   int rc = tp.alloc((size_t)n * (4 + sizeof(T)));
   if (rc) return rc;
   int32_t* flags = (int32_t*)tp.p;
@@ -570,14 +588,21 @@ int dax_hip_copy(void* d_dst, const void* d_src, size_t bytes, void* stream) {
   return copy_bytes(d_dst, d_src, bytes, DAXHIP_S(stream));
 }
 
+// The public scans are synchronous whether or not the total is asked for.
 #define DAXHIP_SCAN(SUF, T)                                                                   \
   int dax_hip_scan_inclusive_##SUF(const T* d_in, T* d_out, int64_t n, T* h_total,            \
                                    void* stream) {                                            \
-    return scan<T>(d_in, d_out, n, 1, h_total, DAXHIP_S(stream));                              \
+    T total = T(0);                                                                           \
+    const int rc = scan<T>(d_in, d_out, n, 1, &total, DAXHIP_S(stream));                     \
+    if (!rc && h_total) *h_total = total;                                                     \
+    return rc;                                                                                \
   }                                                                                           \
   int dax_hip_scan_exclusive_##SUF(const T* d_in, T* d_out, int64_t n, T* h_total,            \
                                    void* stream) {                                            \
-    return scan<T>(d_in, d_out, n, 0, h_total, DAXHIP_S(stream));                              \
+    T total = T(0);                                                                           \
+    const int rc = scan<T>(d_in, d_out, n, 0, &total, DAXHIP_S(stream));                     \
+    if (!rc && h_total) *h_total = total;                                                     \
+    return rc;                                                                                \
   }
 DAXHIP_SCAN(i32, int32_t)
 DAXHIP_SCAN(i64, int64_t)
@@ -650,10 +675,10 @@ int dax_hip_stream_compact(const void* d_in, size_t value_bytes, const void* d_s
     set_error("dax_hip_stream_compact: bad arguments");
     return DAX_HIP_BAD_VALUE;
   }
-  Temp tf;
+  hipStream_t s = DAXHIP_S(stream);
+  TempBuf tf(s);
   int rc = tf.alloc((size_t)n * 4);
   if (rc) return rc;
-  hipStream_t s = DAXHIP_S(stream);
   k_stencil_flags<<<dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s>>>(d_stencil, stencil_type, n,
                                                                          (int32_t*)tf.p);
   DAXHIP_LAUNCH_CHECK("k_stencil_flags");

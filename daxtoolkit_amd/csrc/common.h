@@ -15,6 +15,23 @@ namespace daxhip {
 void set_error(const char* fmt, ...);
 int check_hip(hipError_t e, const char* what);
 
+// Temporaries of the library's own launches: stream-ordered allocations from
+// a per-device memory pool that keeps its memory between calls (no device
+// synchronisation on free, no fresh mapping per call).
+int temp_alloc(void** p, size_t bytes, hipStream_t s);
+void temp_free(void* p, hipStream_t s);
+
+// RAII holder of one temporary, released in stream order.
+struct TempBuf {
+  void* p = nullptr;
+  hipStream_t s = nullptr;
+  explicit TempBuf(hipStream_t stream) : s(stream) {}
+  ~TempBuf() { temp_free(p, s); }
+  TempBuf(const TempBuf&) = delete;
+  TempBuf& operator=(const TempBuf&) = delete;
+  int alloc(size_t bytes) { return temp_alloc(&p, bytes, s); }
+};
+
 #define DAXHIP_CHECK(expr)                                  \
   do {                                                      \
     hipError_t _e = (expr);                                 \

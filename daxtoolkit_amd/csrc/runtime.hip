@@ -2,6 +2,8 @@
 // backend), synchronisation and the event timer.
 #include <stdarg.h>
 
+#include <mutex>
+
 #include "common.h"
 
 namespace daxhip {
@@ -22,6 +24,53 @@ int check_hip(hipError_t e, const char* what) {
   (void)hipGetLastError();
   set_error("%s: %s (%d)", what, hipGetErrorString(e), (int)e);
   return e == hipErrorOutOfMemory ? DAX_HIP_OOM : DAX_HIP_EXEC_ERROR;
+}
+
+namespace {
+std::mutex g_pool_mu;
+constexpr int kMaxDevices = 64;
+hipMemPool_t g_pool[kMaxDevices] = {};
+}  // namespace
+
+int temp_alloc(void** p, size_t bytes, hipStream_t s) {
+  *p = nullptr;
+  int dev = 0;
+  DAXHIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("temp_alloc: device %d beyond the pool table", dev);
+    return DAX_HIP_BAD_VALUE;
+  }
+  hipMemPool_t pool;
+  {
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    if (!g_pool[dev]) {
+      hipMemPoolProps props = {};
+      props.allocType = hipMemAllocationTypePinned;
+      props.handleTypes = hipMemHandleTypeNone;
+      props.location.type = hipMemLocationTypeDevice;
+      props.location.id = dev;
+      hipMemPool_t np = nullptr;
+      DAXHIP_CHECK(hipMemPoolCreate(&np, &props));
+      uint64_t keep = UINT64_MAX;  // never return memory to the device at synchronisation
+      hipError_t e = hipMemPoolSetAttribute(np, hipMemPoolAttrReleaseThreshold, &keep);
+      if (e != hipSuccess) {
+        (void)hipMemPoolDestroy(np);
+        return check_hip(e, "hipMemPoolSetAttribute");
+      }
+      g_pool[dev] = np;
+    }
+    pool = g_pool[dev];
+  }
+  hipError_t e = hipMallocFromPoolAsync(p, bytes ? bytes : 1, pool, s);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return check_hip(e, "hipMallocFromPoolAsync (library temporary)");
+  }
+  return DAX_HIP_OK;
+}
+
+void temp_free(void* p, hipStream_t s) {
+  if (p) (void)hipFreeAsync(p, s);
 }
 
 }  // namespace daxhip

@@ -1,0 +1,358 @@
+// Isosurfaces of unstructured grids of one cell shape: the reference's
+// MarchingTetrahedraCount / MarchingTetrahedraGenerate
+// (dax/worklet/MarchingTetrahedra.h:29-143) on UnstructuredGrid<Tetrahedron>
+// and MarchingCubesCount / MarchingCubesGenerate (dax/worklet/MarchingCubes.h:
+// 53-164) on UnstructuredGrid<Hexahedron>, dispatched through
+// DispatcherMapCell and DispatcherGenerateInterpolatedCells
+// (dax/cont/DispatcherGenerateInterpolatedCells.h:138-301).
+//   count    : a thread per cell: case = sum_v (f_v > iso) << v,
+//              NumFaces[case]; cells with an id outside the points are
+//              counted as 0 and reported
+//   scan     : exclusive scan of the counts (first triangle per cell)
+//   generate : a thread per cell writes its triangles' corners: edge from
+//              the table, endpoints ordered by point id, weight
+//              (iso - f_A) / (f_B - f_A)
+//   dedup    : no grid structure to number edges by, so the general path:
+//              each corner's edge as the key A * P + B (P points; the order
+//              of the keys is the order of the (A, B) pairs), keys sorted
+//              over their significant bits, unique, each corner's point by
+//              lower bound -- the reference's Sort / Unique / LowerBounds on
+//              (EdgeIdFirst, EdgeIdSecond) (:261-301)
+//   points   : Lerp(coords[A], coords[B], w) (CompactPointField, :107-141)
+#include "algorithms.h"
+#include "mc_case_table.h"
+
+namespace daxhip {
+namespace ucell {
+
+// Tetrahedra: per case the number of triangles (bits 0-1) and up to 6 edge
+// numbers (3 bits each from bit 2) of MarchingTetrahedraTable.h:28-54, in
+// the table's order.
+__constant__ uint32_t c_tet_cases[16] = {0,      609,    1057, 314246, 1349, 706914,
+                                         558250, 1197,   1421, 726026, 476450, 677,
+                                         285070, 385,    833,  0};
+// Endpoints (cell vertices) of the 6 tetrahedron edges (MarchingTetrahedra.h:110-113).
+__constant__ uint8_t c_tet_edge[6][2] = {{0, 1}, {1, 2}, {2, 0}, {3, 0}, {3, 1}, {3, 2}};
+
+// Hexahedra: the marching-cubes case table (packed as in mc_case_table.h)
+// and the endpoints of the 12 hexahedron edges (MarchingCubes.h:131-135).
+__constant__ uint64_t c_hex_cases[256] = DAX_HIP_MC_CASE_TABLE_INIT;
+__constant__ uint8_t c_hex_edge[12][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}, {4, 5}, {5, 6},
+                                          {7, 6}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+template <int NV>
+struct Shape;
+
+template <>
+struct Shape<4> {
+  __device__ static int faces(int c) { return (int)(c_tet_cases[c] & 3u); }
+  __device__ static int edge(int c, int k) { return (int)((c_tet_cases[c] >> (2 + 3 * k)) & 7u); }
+  __device__ static int end0(int e) { return c_tet_edge[e][0]; }
+  __device__ static int end1(int e) { return c_tet_edge[e][1]; }
+};
+
+template <>
+struct Shape<8> {
+  __device__ static int faces(int c) { return (int)(c_hex_cases[c] >> 60); }
+  __device__ static int edge(int c, int k) { return (int)((c_hex_cases[c] >> (4 * k)) & 15u); }
+  __device__ static int end0(int e) { return c_hex_edge[e][0]; }
+  __device__ static int end1(int e) { return c_hex_edge[e][1]; }
+};
+
+// The NV point ids of cell t; with kVec the rows are read as 16-byte loads
+// (the connectivity is 16-byte aligned).
+template <int NV, bool kVec>
+__device__ __forceinline__ void load_ids(const int32_t* __restrict__ cells, int64_t t, int* id) {
+  if (kVec) {
+    const int4* p = (const int4*)(cells + NV * t);
+#pragma unroll
+    for (int q = 0; q < NV / 4; ++q) {
+      const int4 v = p[q];
+      id[4 * q + 0] = v.x;
+      id[4 * q + 1] = v.y;
+      id[4 * q + 2] = v.z;
+      id[4 * q + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) id[v] = cells[NV * t + v];
+  }
+}
+
+template <int NV, bool kVec>
+__global__ __launch_bounds__(256) void k_count(const int32_t* __restrict__ cells, int64_t n_cells,
+                                               const float* __restrict__ field, int64_t n_points,
+                                               float iso, int32_t* __restrict__ counts,
+                                               int32_t* __restrict__ bad) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_cells) return;
+  int id[NV];
+  load_ids<NV, kVec>(cells, t, id);
+  bool ok = true;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) ok = ok && id[v] >= 0 && id[v] < n_points;
+  if (!ok) {
+    counts[t] = 0;
+    *bad = 1;
+    return;
+  }
+  int c = 0;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) c |= (field[id[v]] > iso) << v;
+  counts[t] = Shape<NV>::faces(c);
+}
+
+__device__ __forceinline__ void lerp3(const float* __restrict__ pts, int64_t a, int64_t b, float w,
+                                      float* out) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float pa = pts[3 * a + d], pb = pts[3 * b + d];
+    out[d] = pa + w * (pb - pa);
+  }
+}
+
+// Corners of every triangle: with keys, the edge key A * P + B of each
+// corner (dedup); without, the corner's point record, coordinates and
+// identity connectivity.
+template <int NV, bool kVec>
+__global__ __launch_bounds__(256) void k_generate(
+    const int32_t* __restrict__ cells, int64_t n_cells, const float* __restrict__ field,
+    const float* __restrict__ pts, int64_t n_points, float iso, const int32_t* __restrict__ counts,
+    const int32_t* __restrict__ offs, uint64_t* __restrict__ keys, int32_t* __restrict__ first,
+    int32_t* __restrict__ second, float* __restrict__ weight, float* __restrict__ verts,
+    int32_t* __restrict__ conn) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_cells) return;
+  const int nf = counts[t];
+  if (nf == 0) return;
+  int id[NV];
+  load_ids<NV, kVec>(cells, t, id);
+  float f[NV];
+  int c = 0;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    f[v] = field[id[v]];
+    c |= (f[v] > iso) << v;
+  }
+  const int64_t base = 3 * (int64_t)offs[t];
+  for (int k = 0; k < 3 * nf; ++k) {
+    const int e = Shape<NV>::edge(c, k);
+    int va = Shape<NV>::end0(e), vb = Shape<NV>::end1(e);
+    int ia = 0, ib = 0;
+    float fa = 0.0f, fb = 0.0f;
+    // registers indexed by a run-time vertex: select instead of a local array
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      if (v == va) {
+        ia = id[v];
+        fa = f[v];
+      }
+      if (v == vb) {
+        ib = id[v];
+        fb = f[v];
+      }
+    }
+    if (!(ia < ib)) {  // endpoints ordered by point id (ties keep the swap)
+      const int si = ia;
+      ia = ib;
+      ib = si;
+      const float sf = fa;
+      fa = fb;
+      fb = sf;
+    }
+    const int64_t i = base + k;
+    if (keys) {
+      keys[i] = (uint64_t)ia * (uint64_t)n_points + (uint64_t)ib;
+    } else {
+      const float w = (iso - fa) / (fb - fa);
+      first[i] = ia;
+      second[i] = ib;
+      weight[i] = w;
+      float xyz[3];
+      lerp3(pts, ia, ib, w, xyz);
+      verts[3 * i + 0] = xyz[0];
+      verts[3 * i + 1] = xyz[1];
+      verts[3 * i + 2] = xyz[2];
+      conn[i] = (int32_t)i;
+    }
+  }
+}
+
+// The unique points (sorted keys): records, weights and coordinates.
+__global__ __launch_bounds__(256) void k_points(const uint64_t* __restrict__ ukeys, int64_t n,
+                                                uint64_t n_points, const float* __restrict__ field,
+                                                const float* __restrict__ pts, float iso,
+                                                int32_t* __restrict__ first,
+                                                int32_t* __restrict__ second,
+                                                float* __restrict__ weight,
+                                                float* __restrict__ verts) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const uint64_t key = ukeys[u];
+  const int32_t a = (int32_t)(key / n_points), b = (int32_t)(key - (uint64_t)a * n_points);
+  const float fa = field[a], fb = field[b];
+  const float w = (iso - fa) / (fb - fa);
+  if (first) first[u] = a;
+  if (second) second[u] = b;
+  if (weight) weight[u] = w;
+  float xyz[3];
+  lerp3(pts, a, b, w, xyz);
+  verts[3 * u + 0] = xyz[0];
+  verts[3 * u + 1] = xyz[1];
+  verts[3 * u + 2] = xyz[2];
+}
+
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+template <int NV>
+int count_cells(const char* name, const int32_t* d_cells, int64_t n_cells, const float* d_field,
+                int64_t n_points, float iso, int32_t* d_counts, int32_t* d_offsets,
+                int64_t* h_tris, void* stream) {
+  if (n_cells < 0 || n_points < 0 || n_points > 0x7fffffffLL || !h_tris ||
+      n_cells > 0x7fffffffLL / NV ||
+      (n_cells > 0 && (!d_cells || !d_field || !d_counts || !d_offsets))) {
+    set_error("%s: bad arguments", name);
+    return DAX_HIP_BAD_VALUE;
+  }
+  *h_tris = 0;
+  if (n_cells == 0) return DAX_HIP_OK;
+  hipStream_t s = (hipStream_t)stream;
+  TempBuf flag(s);  // set when a cell references a point outside [0, n_points)
+  int rc = flag.alloc(sizeof(int32_t));
+  if (rc) return rc;
+  DAXHIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int32_t), s));
+  const dim3 g((unsigned)ceil_div(n_cells, 256)), b(256);
+  if (aligned16(d_cells))
+    k_count<NV, true><<<g, b, 0, s>>>(d_cells, n_cells, d_field, n_points, iso, d_counts,
+                                      (int32_t*)flag.p);
+  else
+    k_count<NV, false><<<g, b, 0, s>>>(d_cells, n_cells, d_field, n_points, iso, d_counts,
+                                       (int32_t*)flag.p);
+  DAXHIP_LAUNCH_CHECK("k_count");
+  int32_t total = 0, bad = 0;
+  if ((rc = dax_hip_scan_exclusive_i32(d_counts, d_offsets, n_cells, &total, stream))) return rc;
+  DAXHIP_CHECK(hipMemcpyAsync(&bad, flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DAXHIP_CHECK(hipStreamSynchronize(s));
+  if (bad) {
+    set_error("%s: a cell references a point outside [0, %lld)", name, (long long)n_points);
+    return DAX_HIP_BAD_VALUE;
+  }
+  *h_tris = total;
+  return DAX_HIP_OK;
+}
+
+inline size_t scratch_bytes(int64_t n_tris) {
+  return n_tris > 0 ? (size_t)n_tris * 3 * 2 * sizeof(uint64_t) : 0;
+}
+
+template <int NV>
+int generate_cells(const char* name, const int32_t* d_cells, int64_t n_cells, const float* d_field,
+                   const float* d_points, int64_t n_points, float iso, unsigned flags,
+                   const int32_t* d_counts, const int32_t* d_offsets, int64_t n_tris,
+                   void* d_scratch, size_t scratch_bytes_, float* d_vertices, int32_t* d_conn,
+                   int32_t* d_first, int32_t* d_second, float* d_weight, int64_t* h_points,
+                   void* stream) {
+  const bool dedup = (flags & DAX_HIP_MC_DEDUP) != 0;
+  if (n_cells < 0 || n_tris < 0 || 3 * n_tris > 0x7fffffffLL || !h_points || n_points < 0 ||
+      n_points > 0x7fffffffLL || (flags & ~(unsigned)DAX_HIP_MC_DEDUP) ||
+      (n_tris > 0 && (!d_cells || !d_field || !d_points || !d_counts || !d_offsets ||
+                      !d_vertices || !d_conn)) ||
+      (!dedup && n_tris > 0 && (!d_first || !d_second || !d_weight)) ||
+      (dedup && n_tris > 0 && (!d_scratch || scratch_bytes_ < scratch_bytes(n_tris)))) {
+    set_error("%s: bad arguments (only DAX_HIP_MC_DEDUP; dedup needs %zu bytes of scratch)", name,
+              scratch_bytes(n_tris));
+    return DAX_HIP_BAD_VALUE;
+  }
+  *h_points = 0;
+  if (n_tris == 0) return DAX_HIP_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 gt((unsigned)ceil_div(n_cells, 256)), b(256);
+  const bool vec = aligned16(d_cells);
+  uint64_t* keys = dedup ? (uint64_t*)d_scratch : nullptr;
+#define DAXHIP_UGEN(VEC)                                                                        \
+  k_generate<NV, VEC><<<gt, b, 0, s>>>(d_cells, n_cells, d_field, d_points, n_points, iso,      \
+                                       d_counts, d_offsets, keys, d_first, d_second, d_weight,  \
+                                       d_vertices, d_conn)
+  if (vec)
+    DAXHIP_UGEN(true);
+  else
+    DAXHIP_UGEN(false);
+#undef DAXHIP_UGEN
+  DAXHIP_LAUNCH_CHECK("k_generate");
+  if (!dedup) {
+    *h_points = 3 * n_tris;
+    return DAX_HIP_OK;
+  }
+  const int64_t n = 3 * n_tris;
+  uint64_t* sorted = keys + n;
+  DAXHIP_CHECK(hipMemcpyAsync(sorted, keys, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                              s));
+  // keys < P^2: sort over the bits that can be set
+  const uint64_t kmax = (uint64_t)n_points * (uint64_t)n_points - 1;
+  const int bits = kmax ? 64 - __builtin_clzll(kmax) : 1;
+  int rc = alg::sort_u64_bits(sorted, n, bits, s);
+  if (rc) return rc;
+  int64_t u = 0;
+  // keys < 2^62 compare the same as signed 64-bit values
+  if ((rc = dax_hip_unique_i64((int64_t*)sorted, n, &u, stream))) return rc;
+  if ((rc = dax_hip_lower_bounds_i64((const int64_t*)sorted, u, (const int64_t*)keys, n, 0, d_conn,
+                                     nullptr, stream)))
+    return rc;
+  k_points<<<dim3((unsigned)ceil_div(u, 256)), b, 0, s>>>(sorted, u, (uint64_t)n_points, d_field,
+                                                          d_points, iso, d_first, d_second,
+                                                          d_weight, d_vertices);
+  DAXHIP_LAUNCH_CHECK("k_points");
+  *h_points = u;
+  return DAX_HIP_OK;
+}
+
+}  // namespace ucell
+}  // namespace daxhip
+
+using namespace daxhip;
+using namespace daxhip::ucell;
+
+extern "C" {
+
+int dax_hip_mt_count(const int32_t* d_tets, int64_t n_tets, const float* d_field,
+                     int64_t n_points, float iso, int32_t* d_counts, int32_t* d_offsets,
+                     int64_t* h_tris, void* stream) {
+  return count_cells<4>("dax_hip_mt_count", d_tets, n_tets, d_field, n_points, iso, d_counts,
+                        d_offsets, h_tris, stream);
+}
+
+size_t dax_hip_mt_scratch_bytes(int64_t n_tris) { return scratch_bytes(n_tris); }
+
+int dax_hip_mt_generate(const int32_t* d_tets, int64_t n_tets, const float* d_field,
+                        const float* d_points, int64_t n_points, float iso, unsigned flags,
+                        const int32_t* d_counts, const int32_t* d_offsets, int64_t n_tris,
+                        void* d_scratch, size_t scratch_bytes_, float* d_vertices,
+                        int32_t* d_conn, int32_t* d_first, int32_t* d_second, float* d_weight,
+                        int64_t* h_points, void* stream) {
+  return generate_cells<4>("dax_hip_mt_generate", d_tets, n_tets, d_field, d_points, n_points, iso,
+                           flags, d_counts, d_offsets, n_tris, d_scratch, scratch_bytes_,
+                           d_vertices, d_conn, d_first, d_second, d_weight, h_points, stream);
+}
+
+int dax_hip_mc_hex_count(const int32_t* d_hexes, int64_t n_hexes, const float* d_field,
+                         int64_t n_points, float iso, int32_t* d_counts, int32_t* d_offsets,
+                         int64_t* h_tris, void* stream) {
+  return count_cells<8>("dax_hip_mc_hex_count", d_hexes, n_hexes, d_field, n_points, iso,
+                        d_counts, d_offsets, h_tris, stream);
+}
+
+size_t dax_hip_mc_hex_scratch_bytes(int64_t n_tris) { return scratch_bytes(n_tris); }
+
+int dax_hip_mc_hex_generate(const int32_t* d_hexes, int64_t n_hexes, const float* d_field,
+                            const float* d_points, int64_t n_points, float iso, unsigned flags,
+                            const int32_t* d_counts, const int32_t* d_offsets, int64_t n_tris,
+                            void* d_scratch, size_t scratch_bytes_, float* d_vertices,
+                            int32_t* d_conn, int32_t* d_first, int32_t* d_second,
+                            float* d_weight, int64_t* h_points, void* stream) {
+  return generate_cells<8>("dax_hip_mc_hex_generate", d_hexes, n_hexes, d_field, d_points,
+                           n_points, iso, flags, d_counts, d_offsets, n_tris, d_scratch,
+                           scratch_bytes_, d_vertices, d_conn, d_first, d_second, d_weight,
+                           h_points, stream);
+}
+
+}  // extern "C"

@@ -167,3 +167,40 @@ def volume_batch_torch(b: VolumeBatch, device="cuda"):
         flat[off:off + s] = gaussian_mixture_torch(d, p, device).reshape(-1)
         off += s
     return flat
+
+
+# Vertex lists of the cells a uniform grid's voxel is split into, in the
+# voxel's corner order (0..7 = (0,0,0) (1,0,0) (1,1,0) (0,1,0) (0,0,1)
+# (1,0,1) (1,1,1) (0,1,1)), as the reference's test grids build them
+# (dax/cont/testing/TestingGridGenerator.h:229-257 (corners), :266-295 (one
+# hexahedron per voxel), :300-339 (6 tetrahedra per voxel, Freudenthal)).
+VOXEL_SPLITS = {
+    "hexahedron": [[0, 1, 2, 3, 4, 5, 6, 7]],
+    "tetrahedron": [[0, 2, 3, 6], [0, 3, 7, 6], [0, 7, 4, 6], [0, 5, 6, 4], [1, 5, 6, 0],
+                    [1, 6, 2, 0]],
+}
+
+
+def voxel_cells_torch(dims, shape: str, origin=(0, 0, 0), spacing=(1, 1, 1), device="cuda"):
+    """An unstructured grid of the voxels of the uniform grid `dims` (points
+    x-fastest, id i + nx*(j + ny*k)): cells [C, V] int32 (voxels x-fastest,
+    each voxel's cells in VOXEL_SPLITS order) and points [P, 3] float32 at
+    origin + spacing*(i, j, k) (one float32 multiply and add per axis)."""
+    import torch
+    nx, ny, nz = (int(v) for v in dims)
+    split = torch.tensor(VOXEL_SPLITS[shape], dtype=torch.int64, device=device)   # [S, V]
+    corner = torch.tensor([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1],
+                           [1, 1, 1], [0, 1, 1]], dtype=torch.int64, device=device)
+    coff = corner[:, 0] + nx * (corner[:, 1] + ny * corner[:, 2])                  # [8]
+    k, j, i = torch.meshgrid(torch.arange(nz - 1, device=device),
+                             torch.arange(ny - 1, device=device),
+                             torch.arange(nx - 1, device=device), indexing="ij")
+    base = (i + nx * (j + ny * k)).reshape(-1)                                     # [C]
+    cells = (base[:, None, None] + coff[split][None]).reshape(-1, split.shape[1])
+    k, j, i = torch.meshgrid(torch.arange(nz, device=device), torch.arange(ny, device=device),
+                             torch.arange(nx, device=device), indexing="ij")
+    o = torch.tensor(origin, dtype=torch.float32, device=device)
+    s = torch.tensor(spacing, dtype=torch.float32, device=device)
+    pts = torch.stack([o[0] + s[0] * i.reshape(-1).float(), o[1] + s[1] * j.reshape(-1).float(),
+                       o[2] + s[2] * k.reshape(-1).float()], dim=1).contiguous()
+    return cells.to(torch.int32).contiguous(), pts

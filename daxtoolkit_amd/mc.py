@@ -266,27 +266,23 @@ def slice(dims, plane_origin, plane_normal, origin=(0, 0, 0), spacing=(1, 1, 1),
         workspace=buf.workspace, counts=buf.counts)
 
 
-def tetrahedra(tets: torch.Tensor, field: torch.Tensor, points: torch.Tensor, iso: float,
-               dedup: bool = True) -> Contour:
-    """Marching tetrahedra on an unstructured tetrahedral grid (the
-    reference's MarchingTetrahedraCount / MarchingTetrahedraGenerate,
-    dax/worklet/MarchingTetrahedra.h:29-143): tets [C, 4] int32 point ids,
-    field [P] float32, points [P, 3] float32 coordinates.  Synchronous."""
+def _cells_contour(prefix: str, nv: int, cells: torch.Tensor, field: torch.Tensor,
+                   points: torch.Tensor, iso: float, dedup: bool) -> Contour:
     import ctypes
-    if (tets.dtype != torch.int32 or tets.dim() != 2 or tets.shape[1] != 4 or
-            not tets.is_contiguous()):
-        raise ValueError("tets must be a contiguous int32 [C, 4] tensor")
+    if (cells.dtype != torch.int32 or cells.dim() != 2 or cells.shape[1] != nv or
+            not cells.is_contiguous()):
+        raise ValueError(f"cells must be a contiguous int32 [C, {nv}] tensor")
     if field.dtype != torch.float32 or not field.is_contiguous():
         raise ValueError("field must be a contiguous float32 tensor")
     P = field.numel()
     if points.dtype != torch.float32 or points.shape != (P, 3) or not points.is_contiguous():
         raise ValueError("points must be a contiguous float32 [P, 3] tensor")
     dev = field.device
-    C = int(tets.shape[0])
+    C = int(cells.shape[0])
     counts = torch.empty(max(C, 1), dtype=torch.int32, device=dev)
     offsets = torch.empty(max(C, 1), dtype=torch.int32, device=dev)
     t = ctypes.c_int64()
-    _lib.call("dax_hip_mt_count", tets.data_ptr(), C, field.data_ptr(), P, float(iso),
+    _lib.call(f"{prefix}_count", cells.data_ptr(), C, field.data_ptr(), P, float(iso),
               counts.data_ptr(), offsets.data_ptr(), ctypes.byref(t), _stream())
     T = int(t.value)
     n = max(3 * T, 1)
@@ -295,10 +291,10 @@ def tetrahedra(tets: torch.Tensor, field: torch.Tensor, points: torch.Tensor, is
     first = torch.empty(n, dtype=torch.int32, device=dev)
     second = torch.empty(n, dtype=torch.int32, device=dev)
     weight = torch.empty(n, dtype=torch.float32, device=dev)
-    nbytes = int(_lib.load().dax_hip_mt_scratch_bytes(T)) if dedup else 0
+    nbytes = int(getattr(_lib.load(), f"{prefix}_scratch_bytes")(T)) if dedup else 0
     scratch = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
     u = ctypes.c_int64()
-    _lib.call("dax_hip_mt_generate", tets.data_ptr(), C, field.data_ptr(), points.data_ptr(), P,
+    _lib.call(f"{prefix}_generate", cells.data_ptr(), C, field.data_ptr(), points.data_ptr(), P,
               float(iso), MC_DEDUP if dedup else 0, counts.data_ptr(), offsets.data_ptr(), T,
               scratch.data_ptr(), scratch.numel(), verts.data_ptr(), conn.data_ptr(),
               first.data_ptr(), second.data_ptr(), weight.data_ptr(), ctypes.byref(u), _stream())
@@ -306,6 +302,25 @@ def tetrahedra(tets: torch.Tensor, field: torch.Tensor, points: torch.Tensor, is
     return Contour(vertices=verts[:U], connectivity=conn[:3 * T].view(T, 3), normals=None,
                    edges=None, interp_first=first[:U], interp_second=second[:U],
                    interp_weight=weight[:U])
+
+
+def tetrahedra(tets: torch.Tensor, field: torch.Tensor, points: torch.Tensor, iso: float,
+               dedup: bool = True) -> Contour:
+    """Marching tetrahedra on an unstructured tetrahedral grid (the
+    reference's MarchingTetrahedraCount / MarchingTetrahedraGenerate,
+    dax/worklet/MarchingTetrahedra.h:29-143): tets [C, 4] int32 point ids,
+    field [P] float32, points [P, 3] float32 coordinates.  Synchronous."""
+    return _cells_contour("dax_hip_mt", 4, tets, field, points, iso, dedup)
+
+
+def hexahedra(hexes: torch.Tensor, field: torch.Tensor, points: torch.Tensor, iso: float,
+              dedup: bool = True) -> Contour:
+    """Marching cubes on an unstructured hexahedral grid (the reference's
+    MarchingCubesCount / MarchingCubesGenerate on an
+    UnstructuredGrid<Hexahedron>, dax/worklet/MarchingCubes.h:53-164): hexes
+    [C, 8] int32 point ids in the hexahedron's vertex order, field [P]
+    float32, points [P, 3] float32 coordinates.  No normals.  Synchronous."""
+    return _cells_contour("dax_hip_mc_hex", 8, hexes, field, points, iso, dedup)
 
 
 def backward(field: torch.Tensor, iso: float, contour: Contour, dl_dvertices: torch.Tensor,

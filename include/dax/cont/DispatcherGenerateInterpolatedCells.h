@@ -1,7 +1,7 @@
 // DispatcherGenerateInterpolatedCells (dax/cont/DispatcherGenerateInterpolatedCells.h:
-// 34-309) for MarchingCubesGenerate and SliceGenerate on a UniformGrid and
-// MarchingTetrahedraGenerate on an UnstructuredGrid<CellTagTetrahedron> with
-// the HIP adapter.
+// 34-309) with the HIP adapter for MarchingCubesGenerate on a UniformGrid or
+// an UnstructuredGrid<CellTagHexahedron>, SliceGenerate on a UniformGrid and
+// MarchingTetrahedraGenerate on an UnstructuredGrid<CellTagTetrahedron>.
 //
 // Same interface: constructor (count[, worklet]), SetRemoveDuplicatePoints
 // (default true), SetReleaseCount (default true), GetCount,
@@ -114,6 +114,47 @@ class InterpolatedCellsHip {
     return (RemoveDuplicatePoints ? DAX_HIP_MC_DEDUP : 0u) | (ComputeNormals ? DAX_HIP_MC_NORMALS : 0u);
   }
 
+  // Invoke on an unstructured grid of NV-vertex cells through the C-ABI
+  // pair count / generate (dax_hip_mt_* or dax_hip_mc_hex_*): count and
+  // scan, outputs sized for 3T points, generate, then the point arrays
+  // shrunk to the U points produced.  Normals are not produced.
+  template <class CellTag, class CConn, class CPts, class CC, class PC, class CField,
+            class CountFn, class ScratchFn, class GenerateFn>
+  void InvokeOnCells(const UnstructuredGrid<CellTag, CConn, CPts, DeviceAdapterTag>& in,
+                     UnstructuredGrid<dax::CellTagTriangle, CC, PC, DeviceAdapterTag>& outputGrid,
+                     const ArrayHandle<dax::Scalar, CField, DeviceAdapterTag>& field, float iso,
+                     CountFn count, ScratchFn scratch_bytes, GenerateFn generate) {
+    static_assert(sizeof(dax::Id) == 4, "the HIP unstructured path reads 32-bit ids");
+    if (field.GetNumberOfValues() != in.GetNumberOfPoints())
+      throw ErrorControlBadValue("field size does not match the grid's point count");
+    const dax::Id n = in.GetNumberOfCells(), P = in.GetNumberOfPoints();
+    const int32_t* cells = in.GetCellConnections().PrepareForInput().GetRawPointer();
+    const float* f = field.PrepareForInput().GetRawPointer();
+    const float* pts = (const float*)in.GetPointCoordinates().PrepareForInput().GetRawPointer();
+    internal::HipWorkspace counts((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    internal::HipWorkspace offs((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    int64_t T = 0, U = 0;
+    internal::HipCheck(count(cells, n, f, P, iso, (int32_t*)counts.Data, (int32_t*)offs.Data, &T,
+                             nullptr));
+    if (GetReleaseCount()) DoReleaseCount();
+    if (T == 0) return;  // nothing to do (:190-194)
+    const dax::Id cap = (dax::Id)(3 * T);
+    int32_t* conn = outputGrid.GetCellConnections().PrepareForOutput(cap).GetRawPointer();
+    float* coords = (float*)outputGrid.GetPointCoordinates().PrepareForOutput(cap).GetRawPointer();
+    int32_t* first = InterpFirst.PrepareForOutput(cap).GetRawPointer();
+    int32_t* second = InterpSecond.PrepareForOutput(cap).GetRawPointer();
+    float* weight = InterpWeight.PrepareForOutput(cap).GetRawPointer();
+    const unsigned flags = RemoveDuplicatePoints ? DAX_HIP_MC_DEDUP : 0u;
+    internal::HipWorkspace scratch(RemoveDuplicatePoints ? scratch_bytes(T) : 1);
+    internal::HipCheck(generate(cells, n, f, pts, P, iso, flags, (const int32_t*)counts.Data,
+                                (const int32_t*)offs.Data, T, scratch.Data, scratch.Bytes, coords,
+                                conn, first, second, weight, &U, nullptr));
+    outputGrid.GetPointCoordinates().Shrink((dax::Id)U);
+    InterpFirst.Shrink((dax::Id)U);
+    InterpSecond.Shrink((dax::Id)U);
+    InterpWeight.Shrink((dax::Id)U);
+  }
+
   bool RemoveDuplicatePoints, ReleaseCount, ComputeNormals;
   CountHandleType Count;
   WorkletType Worklet;
@@ -167,6 +208,19 @@ class DispatcherGenerateInterpolatedCells<dax::worklet::MarchingCubesGenerate, C
                                            nullptr, first, second, weight, U, T,
                                            (int64_t*)counts.Data, nullptr));
     internal::HipCheck(dax_hip_mc_read_counts((const int64_t*)counts.Data, U, T, &T, &U, nullptr));
+  }
+
+  // The same worklet on an UnstructuredGrid<CellTagHexahedron> (the
+  // reference test runs it there too, UnitTestWorkletMarchingCubes.cxx:
+  // 211-216): the general sort/unique/lower-bounds dedup; no normals.
+  template <class CConn, class CPts, class CC, class PC, class CField>
+  void Invoke(const UnstructuredGrid<dax::CellTagHexahedron, CConn, CPts, DeviceAdapterTag>& in,
+              UnstructuredGrid<dax::CellTagTriangle, CC, PC, DeviceAdapterTag>& outputGrid,
+              const ArrayHandle<dax::Scalar, CField, DeviceAdapterTag>& field) {
+    if (this->ComputeNormals)
+      throw ErrorControlBadValue("normals are not produced on an unstructured grid");
+    this->InvokeOnCells(in, outputGrid, field, this->Worklet.GetIsoValue(), dax_hip_mc_hex_count,
+                        dax_hip_mc_hex_scratch_bytes, dax_hip_mc_hex_generate);
   }
 };
 
@@ -238,37 +292,8 @@ class DispatcherGenerateInterpolatedCells<dax::worklet::MarchingTetrahedraGenera
   void Invoke(const UnstructuredGrid<dax::CellTagTetrahedron, CConn, CPts, DeviceAdapterTag>& in,
               UnstructuredGrid<dax::CellTagTriangle, CC, PC, DeviceAdapterTag>& outputGrid,
               const ArrayHandle<dax::Scalar, CField, DeviceAdapterTag>& field) {
-    static_assert(sizeof(dax::Id) == 4, "the HIP tetrahedra path writes 32-bit ids");
-    if (field.GetNumberOfValues() != in.GetNumberOfPoints())
-      throw ErrorControlBadValue("field size does not match the grid's point count");
-    const dax::Id n = in.GetNumberOfCells(), P = in.GetNumberOfPoints();
-    const int32_t* tets = in.GetCellConnections().PrepareForInput().GetRawPointer();
-    const float* f = field.PrepareForInput().GetRawPointer();
-    const float* pts = (const float*)in.GetPointCoordinates().PrepareForInput().GetRawPointer();
-    const float iso = this->Worklet.GetIsoValue();
-    internal::HipWorkspace counts((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
-    internal::HipWorkspace offs((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
-    int64_t T = 0, U = 0;
-    internal::HipCheck(dax_hip_mt_count(tets, n, f, P, iso, (int32_t*)counts.Data,
-                                        (int32_t*)offs.Data, &T, nullptr));
-    if (this->GetReleaseCount()) this->DoReleaseCount();
-    if (T == 0) return;  // nothing to do (:190-194)
-    const dax::Id cap = (dax::Id)(3 * T);
-    int32_t* conn = outputGrid.GetCellConnections().PrepareForOutput(cap).GetRawPointer();
-    float* coords = (float*)outputGrid.GetPointCoordinates().PrepareForOutput(cap).GetRawPointer();
-    int32_t* first = this->InterpFirst.PrepareForOutput(cap).GetRawPointer();
-    int32_t* second = this->InterpSecond.PrepareForOutput(cap).GetRawPointer();
-    float* weight = this->InterpWeight.PrepareForOutput(cap).GetRawPointer();
-    const unsigned flags = this->RemoveDuplicatePoints ? DAX_HIP_MC_DEDUP : 0u;
-    internal::HipWorkspace scratch(this->RemoveDuplicatePoints ? dax_hip_mt_scratch_bytes(T) : 1);
-    internal::HipCheck(dax_hip_mt_generate(tets, n, f, pts, P, iso, flags,
-                                           (const int32_t*)counts.Data, (const int32_t*)offs.Data,
-                                           T, scratch.Data, scratch.Bytes, coords, conn, first,
-                                           second, weight, &U, nullptr));
-    outputGrid.GetPointCoordinates().Shrink((dax::Id)U);
-    this->InterpFirst.Shrink((dax::Id)U);
-    this->InterpSecond.Shrink((dax::Id)U);
-    this->InterpWeight.Shrink((dax::Id)U);
+    this->InvokeOnCells(in, outputGrid, field, this->Worklet.GetIsoValue(), dax_hip_mt_count,
+                        dax_hip_mt_scratch_bytes, dax_hip_mt_generate);
   }
 };
 

@@ -1,6 +1,7 @@
 // DispatcherMapCell (dax/cont/DispatcherMapCell.h:27-64) for the worklets of
-// the marching-cubes path on a UniformGrid with the HIP adapter:
+// the marching-cubes path with the HIP adapter:
 //   MarchingCubesCount(iso): Invoke(grid, field, count)   count[C] = NumFaces[case]
+//                            (grid: UniformGrid or UnstructuredGrid<CellTagHexahedron>)
 //   SliceCount(o, n):        Invoke(grid, coords, count)  the same for the plane
 //   MarchingTetrahedraCount(iso): Invoke(tetGrid, field, count) on an
 //                            UnstructuredGrid<CellTagTetrahedron>
@@ -46,6 +47,25 @@ class DispatcherMapCell<dax::worklet::MarchingCubesCount, dax::hip::cont::Device
     internal::HipCheck(dax_hip_mc_cell_outputs(d[0], d[1], d[2], ws.Data, ws.Bytes, nullptr, c,
                                                nullptr, nullptr));
     internal::HipCheck(dax_hip_synchronize(nullptr));
+  }
+
+  // On an UnstructuredGrid<CellTagHexahedron> (UnitTestWorkletMarchingCubes.cxx:211-216).
+  template <class CConn, class CPts, class CField, class CCount>
+  void Invoke(const UnstructuredGrid<dax::CellTagHexahedron, CConn, CPts, Tag>& grid,
+              const ArrayHandle<dax::Scalar, CField, Tag>& field,
+              ArrayHandle<dax::Id, CCount, Tag>& count) const {
+    static_assert(sizeof(dax::Id) == 4, "the HIP unstructured path reads 32-bit ids");
+    if (field.GetNumberOfValues() != grid.GetNumberOfPoints())
+      throw ErrorControlBadValue("field size does not match the grid's point count");
+    const dax::Id n = grid.GetNumberOfCells();
+    const int32_t* hexes = grid.GetCellConnections().PrepareForInput().GetRawPointer();
+    const float* f = field.PrepareForInput().GetRawPointer();
+    int32_t* c = count.PrepareForOutput(n).GetRawPointer();
+    internal::HipWorkspace offs((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    int64_t T = 0;
+    internal::HipCheck(dax_hip_mc_hex_count(hexes, n, f, grid.GetNumberOfPoints(),
+                                            Worklet.GetIsoValue(), c, (int32_t*)offs.Data, &T,
+                                            nullptr));
   }
 
  private:

@@ -239,6 +239,28 @@ int dax_hip_mt_generate(const int32_t* d_tets, int64_t n_tets, const float* d_fi
                         int32_t* d_conn, int32_t* d_first, int32_t* d_second, float* d_weight,
                         int64_t* h_points, void* stream);
 
+/* Marching cubes on an unstructured hexahedral grid: replaces
+ * DispatcherMapCell<MarchingCubesCount> +
+ * DispatcherGenerateInterpolatedCells<MarchingCubesGenerate> on an
+ * UnstructuredGrid<CellTagHexahedron> (dax/worklet/MarchingCubes.h:53-164;
+ * the reference test runs on it through TryAllGridTypes,
+ * dax/worklet/testing/UnitTestWorkletMarchingCubes.cxx:211-216).
+ * d_hexes [n_hexes][8] int32 point ids in the hexahedron's vertex order
+ * (dax/cont/testing/TestingGridGenerator.h:232-241).  The same contract as
+ * dax_hip_mt_count / dax_hip_mt_generate above, with the marching-cubes
+ * case table and the hexahedron's 12 edges (MarchingCubes.h:131-135).
+ * No normals: the reference computes none on this path. */
+int dax_hip_mc_hex_count(const int32_t* d_hexes, int64_t n_hexes, const float* d_field,
+                         int64_t n_points, float iso, int32_t* d_counts, int32_t* d_offsets,
+                         int64_t* h_tris, void* stream);
+size_t dax_hip_mc_hex_scratch_bytes(int64_t n_tris);
+int dax_hip_mc_hex_generate(const int32_t* d_hexes, int64_t n_hexes, const float* d_field,
+                            const float* d_points, int64_t n_points, float iso, unsigned flags,
+                            const int32_t* d_counts, const int32_t* d_offsets, int64_t n_tris,
+                            void* d_scratch, size_t scratch_bytes, float* d_vertices,
+                            int32_t* d_conn, int32_t* d_first, int32_t* d_second,
+                            float* d_weight, int64_t* h_points, void* stream);
+
 /* Synchronous: reads {T, U} written by a forward (or classify) on `stream`,
  * and returns DAX_HIP_CAPACITY if the generate step was skipped for lack of
  * room, DAX_HIP_EXEC_ERROR if T or U left the int32 dax::Id range. */

@@ -532,18 +532,29 @@ void oracle_tet_tables(uint8_t* num_faces, uint8_t* tri) {
   }
 }
 
-int oracle_mt_generate(const int32_t* tets, int64_t n_tets, const float* field,
-                       const float* points, float iso, int dedup, int64_t cap,
-                       int64_t* n_tris, int64_t* n_pts, int32_t* conn, int32_t* first,
-                       int32_t* second, float* weight, float* coords) {
-  std::vector<int> cls(n_tets);
-  std::vector<Id> incl(n_tets);
+}  // extern "C"
+
+namespace {
+
+// Cells of one shape with NV vertices: case_edges(c) is the case's TriTable
+// row as a string of edge digits (hex digits for the hexahedron), ends the
+// edge -> vertex pairs.  The Serial-adapter sequence shared by marching
+// tetrahedra and marching cubes on an unstructured grid.
+template <int NV, class CaseEdges>
+int unstructured_generate(const int32_t* cells, int64_t n_cells, const float* field,
+                          const float* points, float iso, int dedup, int64_t cap,
+                          int64_t* n_tris, int64_t* n_pts, int32_t* conn, int32_t* first,
+                          int32_t* second, float* weight, float* coords, CaseEdges case_edges,
+                          const int (*ends)[2]) {
+  auto digit = [](char ch) { return ch <= '9' ? ch - '0' : ch - 'a' + 10; };
+  std::vector<int> cls(n_cells);
+  std::vector<Id> incl(n_cells);
   Id run = 0;
-  for (int64_t c = 0; c < n_tets; ++c) {
+  for (int64_t c = 0; c < n_cells; ++c) {
     int k = 0;
-    for (int v = 0; v < 4; ++v) k |= (field[tets[4 * c + v]] > iso) << v;
+    for (int v = 0; v < NV; ++v) k |= (field[cells[NV * c + v]] > iso) << v;
     cls[c] = k;
-    run += (Id)(strlen(kTetTri[k]) / 3);
+    run += (Id)(strlen(case_edges(k)) / 3);
     incl[c] = run;
   }
   const int64_t T = run;
@@ -553,10 +564,10 @@ int oracle_mt_generate(const int32_t* tets, int64_t n_tets, const float* field,
     const int64_t c = std::upper_bound(incl.begin(), incl.end(), (Id)t) - incl.begin();
     const int64_t first_t = c == 0 ? 0 : incl[c - 1];
     const int s = (int)(t - first_t);
-    const int32_t* id = tets + 4 * c;
+    const int32_t* id = cells + NV * c;
     for (int v = 0; v < 3; ++v) {
-      const int e = kTetTri[cls[c]][3 * s + v] - '0';
-      int va = kTetEdge[e][0], vb = kTetEdge[e][1];
+      const int e = digit(case_edges(cls[c])[3 * s + v]);
+      int va = ends[e][0], vb = ends[e][1];
       if (!(id[va] < id[vb])) std::swap(va, vb);
       const Scalar w = (iso - field[id[va]]) / (field[id[vb]] - field[id[va]]);
       rec[3 * t + v] = PointAsEdgeInterpolation{id[va], id[vb], w};
@@ -585,6 +596,36 @@ int oracle_mt_generate(const int32_t* tets, int64_t n_tets, const float* field,
     }
   }
   return 0;
+}
+
+// Hexahedron edges -> vertex pairs (dax/worklet/MarchingCubes.h:131-135).
+const int kHexEdge[12][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}, {4, 5}, {5, 6},
+                             {7, 6}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+}  // namespace
+
+extern "C" {
+
+int oracle_mt_generate(const int32_t* tets, int64_t n_tets, const float* field,
+                       const float* points, float iso, int dedup, int64_t cap,
+                       int64_t* n_tris, int64_t* n_pts, int32_t* conn, int32_t* first,
+                       int32_t* second, float* weight, float* coords) {
+  return unstructured_generate<4>(tets, n_tets, field, points, iso, dedup, cap, n_tris, n_pts,
+                                  conn, first, second, weight, coords,
+                                  [](int c) { return kTetTri[c]; }, kTetEdge);
+}
+
+// Marching cubes on an unstructured hexahedral grid
+// (dax/worklet/MarchingCubes.h:53-164 on UnstructuredGrid<Hexahedron>): the
+// same sequence with the marching-cubes table (GetHexahedronClassification,
+// :38-48) and the hexahedron's edges.
+int oracle_mc_hex_generate(const int32_t* hexes, int64_t n_hexes, const float* field,
+                           const float* points, float iso, int dedup, int64_t cap,
+                           int64_t* n_tris, int64_t* n_pts, int32_t* conn, int32_t* first,
+                           int32_t* second, float* weight, float* coords) {
+  return unstructured_generate<8>(hexes, n_hexes, field, points, iso, dedup, cap, n_tris, n_pts,
+                                  conn, first, second, weight, coords,
+                                  [](int c) { return kOracleCaseEdges[c]; }, kHexEdge);
 }
 
 // Analytic adjoint in FP64, scatter form: dL/df for every point.

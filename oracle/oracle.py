@@ -57,6 +57,8 @@ def lib():
                                           C.c_float, C.c_int, C.c_int64, _i64p, _i64p] +
                                          [C.c_void_p] * 5)
         L.oracle_mt_generate.restype = C.c_int
+        L.oracle_mc_hex_generate.argtypes = L.oracle_mt_generate.argtypes
+        L.oracle_mc_hex_generate.restype = C.c_int
         L.oracle_tet_tables.argtypes = [C.c_void_p, C.c_void_p]
         L.oracle_mc_fd_f64.argtypes = ([C.c_void_p] * 4 + [C.c_float] + [C.c_void_p] * 2 +
                                        [C.c_int64] + [C.c_void_p] * 3 + [C.c_int64, C.c_double,
@@ -209,28 +211,40 @@ def slice_field(dims, plane_origin, plane_normal, origin=(0, 0, 0), spacing=(1, 
     return out, np.float32(iso)
 
 
-def freudenthal_tets(dims, origin=(0, 0, 0), spacing=(1, 1, 1)):
-    """The tetrahedral grid the reference's tests build from a uniform grid
-    (dax/cont/testing/TestingGridGenerator.h:229-257,300-339): each voxel
-    split into 6 tetrahedra (Freudenthal), voxel corners numbered as the
-    hexahedron's.  Returns (tets [6C, 4] int32, points [P, 3] float32)."""
+def _voxel_cells(dims, split, origin, spacing):
     nx, ny, nz = (int(v) for v in dims)
     corner = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
                        (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)], np.int64)
-    split = np.array([[0, 2, 3, 6], [0, 3, 7, 6], [0, 7, 4, 6], [0, 5, 6, 4],
-                      [1, 5, 6, 0], [1, 6, 2, 0]], np.int64)
     k, j, i = np.meshgrid(np.arange(nz - 1), np.arange(ny - 1), np.arange(nx - 1), indexing="ij")
     cell = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1)          # x-fastest cells
-    pts_ijk = cell[:, None, None, :] + corner[split][None, :, :, :]     # [C, 6, 4, 3]
+    pts_ijk = cell[:, None, None, :] + corner[split][None, :, :, :]     # [C, S, V, 3]
     ids = pts_ijk[..., 0] + nx * (pts_ijk[..., 1] + ny * pts_ijk[..., 2])
-    tets = ids.reshape(-1, 4).astype(np.int32)
+    cells = ids.reshape(-1, split.shape[1]).astype(np.int32)
     k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     o = np.asarray(origin, np.float32)
     sp = np.asarray(spacing, np.float32)
     pts = np.stack([o[0] + sp[0] * i.ravel().astype(np.float32),
                     o[1] + sp[1] * j.ravel().astype(np.float32),
                     o[2] + sp[2] * k.ravel().astype(np.float32)], axis=1).astype(np.float32)
-    return tets, pts
+    return cells, pts
+
+
+def freudenthal_tets(dims, origin=(0, 0, 0), spacing=(1, 1, 1)):
+    """The tetrahedral grid the reference's tests build from a uniform grid
+    (dax/cont/testing/TestingGridGenerator.h:229-257,300-339): each voxel
+    split into 6 tetrahedra (Freudenthal), voxel corners numbered as the
+    hexahedron's.  Returns (tets [6C, 4] int32, points [P, 3] float32)."""
+    split = np.array([[0, 2, 3, 6], [0, 3, 7, 6], [0, 7, 4, 6], [0, 5, 6, 4],
+                      [1, 5, 6, 0], [1, 6, 2, 0]], np.int64)
+    return _voxel_cells(dims, split, origin, spacing)
+
+
+def voxel_hexes(dims, origin=(0, 0, 0), spacing=(1, 1, 1)):
+    """The hexahedral grid the reference's tests build from a uniform grid
+    (dax/cont/testing/TestingGridGenerator.h:229-257,266-295): one
+    hexahedron per voxel, vertices in the voxel's corner order.  Returns
+    (hexes [C, 8] int32, points [P, 3] float32)."""
+    return _voxel_cells(dims, np.arange(8, dtype=np.int64)[None, :], origin, spacing)
 
 
 def tet_tables():
@@ -241,25 +255,34 @@ def tet_tables():
     return nf, tri
 
 
-def mt_generate(tets, field, points, iso, dedup=True):
-    """Marching tetrahedra, the Serial-adapter sequence (see mc_oracle.cpp)."""
-    tets = np.ascontiguousarray(tets, np.int32)
+def _unstructured(fn, nv, cells, field, points, iso, dedup):
+    cells = np.ascontiguousarray(cells, np.int32).reshape(-1, nv)
     field = np.ascontiguousarray(field, np.float32).reshape(-1)
     points = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
     nt, npt = C.c_int64(), C.c_int64()
-    cap = 3 * 5 * max(tets.shape[0], 1)
+    cap = 3 * 5 * max(cells.shape[0], 1)
     conn = np.zeros(cap, np.int32)
     first = np.zeros(cap, np.int32)
     second = np.zeros(cap, np.int32)
     weight = np.zeros(cap, np.float32)
     coords = np.zeros((cap, 3), np.float32)
-    rc = lib().oracle_mt_generate(_p(tets), tets.shape[0], _p(field), _p(points), float(iso),
-                                  int(dedup), cap, C.byref(nt), C.byref(npt), _p(conn),
-                                  _p(first), _p(second), _p(weight), _p(coords))
+    rc = fn(_p(cells), cells.shape[0], _p(field), _p(points), float(iso), int(dedup), cap,
+            C.byref(nt), C.byref(npt), _p(conn), _p(first), _p(second), _p(weight), _p(coords))
     assert rc == 0
     T, U = nt.value, npt.value
     return {"n_tris": T, "n_pts": U, "conn": conn[:3 * T], "first": first[:U],
             "second": second[:U], "weight": weight[:U], "coords": coords[:U]}
+
+
+def mt_generate(tets, field, points, iso, dedup=True):
+    """Marching tetrahedra, the Serial-adapter sequence (see mc_oracle.cpp)."""
+    return _unstructured(lib().oracle_mt_generate, 4, tets, field, points, iso, dedup)
+
+
+def mc_hex_generate(hexes, field, points, iso, dedup=True):
+    """Marching cubes on an unstructured hexahedral grid, the Serial-adapter
+    sequence (see mc_oracle.cpp)."""
+    return _unstructured(lib().oracle_mc_hex_generate, 8, hexes, field, points, iso, dedup)
 
 
 def backward_f64(field, iso, first, second, dldx, dldn, origin=(0, 0, 0), spacing=(1, 1, 1)):

@@ -2,7 +2,11 @@
 // (dax/worklet/testing/UnitTestWorkletMarchingCubes.cxx:38-224) on
 // DeviceAdapterTagHip: 26^3 uniform grid, f = x+y+z, iso 70; 3T points
 // without merging, 45 with merging; a secondary linear field interpolated
-// onto the contour matches dot(coords, (-0.5, 1, 2)) to test_equal.
+// onto the contour matches dot(coords, (-0.5, 1, 2)) to test_equal.  As the
+// reference does through TryAllGridTypes (:211-216), the same test runs on
+// the grid's voxels as an UnstructuredGrid<CellTagHexahedron>
+// (dax/cont/testing/TestingGridGenerator.h:266-295), whose merged output
+// must also equal the uniform grid's.
 #include <iostream>
 #include <vector>
 
@@ -119,6 +123,97 @@ void TestMarchingCubes() {
   }
   for (int u : used) DAX_TEST_ASSERT(u == 1, "unused output point");
 }
+
+void TestMarchingCubesHexahedra() {
+  typedef dax::cont::UnstructuredGrid<dax::CellTagHexahedron, ArrayContainer, ArrayContainer,
+                                      DeviceAdapter>
+      HexGridType;
+  typedef dax::cont::UnstructuredGrid<dax::CellTagTriangle, ArrayContainer, ArrayContainer,
+                                      DeviceAdapter>
+      TriGridType;
+  const int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                            {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  std::vector<dax::Vector3> points;
+  for (dax::Id k = 0; k < DIM; ++k)
+    for (dax::Id j = 0; j < DIM; ++j)
+      for (dax::Id i = 0; i < DIM; ++i) points.push_back(dax::Vector3(i, j, k));
+  std::vector<dax::Id> conn;
+  for (dax::Id k = 0; k + 1 < DIM; ++k)
+    for (dax::Id j = 0; j + 1 < DIM; ++j)
+      for (dax::Id i = 0; i + 1 < DIM; ++i)
+        for (int v = 0; v < 8; ++v)
+          conn.push_back((i + corner[v][0]) + DIM * ((j + corner[v][1]) + DIM * (k + corner[v][2])));
+  HexGridType hexGrid(dax::cont::make_ArrayHandle(conn, ArrayContainer(), DeviceAdapter()),
+                      dax::cont::make_ArrayHandle(points, ArrayContainer(), DeviceAdapter()));
+  const dax::Vector3 secondaryGradient = dax::make_Vector3(-0.5f, 1.0f, 2.0f);
+  std::vector<dax::Scalar> field(points.size()), secondary(points.size());
+  for (size_t p = 0; p < points.size(); ++p) {
+    field[p] = points[p][0] + points[p][1] + points[p][2];
+    secondary[p] = dax::dot(points[p], secondaryGradient);
+  }
+  auto fieldHandle = dax::cont::make_ArrayHandle(field, ArrayContainer(), DeviceAdapter());
+  auto secondaryHandle = dax::cont::make_ArrayHandle(secondary, ArrayContainer(), DeviceAdapter());
+
+  typedef dax::cont::ArrayHandle<dax::Id, ArrayContainer, DeviceAdapter> CountHandleType;
+  typedef dax::cont::DispatcherMapCell<dax::worklet::MarchingCubesCount, DeviceAdapter> CellDispatcher;
+  typedef dax::cont::DispatcherGenerateInterpolatedCells<dax::worklet::MarchingCubesGenerate,
+                                                         CountHandleType, DeviceAdapter>
+      InterpolatedDispatcher;
+  std::cout << "Hexahedral grid: count, generate without and with merging." << std::endl;
+  CountHandleType count;
+  CellDispatcher((dax::worklet::MarchingCubesCount(ISOVALUE))).Invoke(hexGrid, fieldHandle, count);
+  DAX_TEST_ASSERT(count.GetNumberOfValues() == hexGrid.GetNumberOfCells(), "count size");
+  InterpolatedDispatcher interp(count, dax::worklet::MarchingCubesGenerate(ISOVALUE));
+  interp.SetRemoveDuplicatePoints(false);
+  interp.SetReleaseCount(false);
+  TriGridType outGrid;
+  interp.Invoke(hexGrid, outGrid, fieldHandle);
+  const dax::Id valid_num_points = 3 * outGrid.GetNumberOfCells();
+  DAX_TEST_ASSERT(valid_num_points == outGrid.GetNumberOfPoints(),
+                  "Incorrect number of points in the output grid when not merging");
+  {
+    auto cp = count.GetPortalConstControl();
+    dax::Id total = 0;
+    for (dax::Id i = 0; i < cp.GetNumberOfValues(); ++i) total += cp.Get(i);
+    DAX_TEST_ASSERT(total == outGrid.GetNumberOfCells(), "count does not sum to the triangles");
+  }
+  dax::cont::ArrayHandle<dax::Scalar, ArrayContainer, DeviceAdapter> interpolatedSecondary;
+  interp.CompactPointField(secondaryHandle, interpolatedSecondary);
+  CheckFieldInterpolation(outGrid.GetPointCoordinates(), interpolatedSecondary, secondaryGradient);
+
+  interp.SetRemoveDuplicatePoints(true);
+  TriGridType merged;
+  interp.Invoke(hexGrid, merged, fieldHandle);
+  std::cout << "Number of points is " << merged.GetNumberOfPoints() << std::endl;
+  DAX_TEST_ASSERT(merged.GetNumberOfPoints() == 45 && merged.GetNumberOfPoints() != valid_num_points,
+                  "We didn't merge to the correct number of points");
+  interp.CompactPointField(secondaryHandle, interpolatedSecondary);
+  CheckFieldInterpolation(merged.GetPointCoordinates(), interpolatedSecondary, secondaryGradient);
+
+  // the same output as the uniform grid's dispatchers
+  dax::cont::UniformGrid<DeviceAdapter> inGrid;
+  inGrid.SetExtent(dax::make_Id3(0, 0, 0), dax::make_Id3(DIM - 1, DIM - 1, DIM - 1));
+  CountHandleType ucount;
+  CellDispatcher((dax::worklet::MarchingCubesCount(ISOVALUE))).Invoke(inGrid, fieldHandle, ucount);
+  InterpolatedDispatcher uinterp(ucount, dax::worklet::MarchingCubesGenerate(ISOVALUE));
+  TriGridType uniformOut;
+  uinterp.Invoke(inGrid, uniformOut, fieldHandle);
+  auto a = merged.GetPointCoordinates().GetPortalConstControl();
+  auto b = uniformOut.GetPointCoordinates().GetPortalConstControl();
+  DAX_TEST_ASSERT(a.GetNumberOfValues() == b.GetNumberOfValues(), "point counts differ");
+  for (dax::Id i = 0; i < a.GetNumberOfValues(); ++i)
+    DAX_TEST_ASSERT(a.Get(i) == b.Get(i), "hexahedral and uniform points differ");
+  auto ca = merged.GetCellConnections().GetPortalConstControl();
+  auto cb = uniformOut.GetCellConnections().GetPortalConstControl();
+  DAX_TEST_ASSERT(ca.GetNumberOfValues() == cb.GetNumberOfValues(), "triangle counts differ");
+  for (dax::Id i = 0; i < ca.GetNumberOfValues(); ++i)
+    DAX_TEST_ASSERT(ca.Get(i) == cb.Get(i), "hexahedral and uniform connectivity differ");
+}
 }  // namespace
 
-int main() { return RunTest(TestMarchingCubes); }
+int main() {
+  return RunTest([] {
+    TestMarchingCubes();
+    TestMarchingCubesHexahedra();
+  });
+}
