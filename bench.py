@@ -28,6 +28,9 @@ the headline the line carries
   hexahedra_256  marching cubes on the unstructured hexahedral grid of the
            256^3 grid's voxels (the general dedup path), with the serial
            restatement beside it;
+  threshold_512  the reference's Threshold benchmark pipeline at 512^3
+           (|p| field, [0, 100], duplicate removal, CompactPointField), with
+           the serial restatement beside it;
   batched  1024 volumes with per-axis extents U{32..160}, 4-16 Gaussians each,
            iso U[0.3,0.7]: the batched forward and forward+backward replayed
            as captured graphs, against a native launch loop of single-volume
@@ -89,7 +92,8 @@ def parse():
     ap.add_argument("--batch", type=int, default=1024, help="volumes of the batched run (0: skip)")
     ap.add_argument("--no-bwd", action="store_true", help="skip the forward+backward timing")
     ap.add_argument("--large", action="store_true", help="also run the 1024^3 Mandelbulb")
-    ap.add_argument("--only", choices=["fwd", "batch", "large", "diff", "slice", "tets", "hex"],
+    ap.add_argument("--only", choices=["fwd", "batch", "large", "diff", "slice", "tets", "hex",
+                                       "threshold"],
                     default=None,
                     help="run one section only (profiling); prints its JSON")
     return ap.parse_args()
@@ -422,6 +426,59 @@ def _cells_alg_bytes(C, nv, T, U):
     return C * (4 * nv + 4 + 8 + 4) + T * (4 * nv + 4) + 3 * T * (8 + 16) + U * 24
 
 
+def threshold_512(args):
+    """The reference's Threshold benchmark pipeline (Benchmarks/Threshold/
+    Pipeline.h:163-190, main.cxx: origin 0, spacing 1) at 512^3: the field
+    is the point magnitude |p| (made beforehand, as the reference does with
+    DispatcherMapField<Magnitude> before its timer starts); timed:
+    ThresholdCount [0, 100], DispatcherGenerateTopology with duplicate
+    removal, and CompactPointField of the field.  Synchronous like the
+    reference (the cell and point counts come back to the host)."""
+    import torch
+
+    from daxtoolkit_amd import topology
+    n = 512
+    ax = torch.arange(n, device="cuda", dtype=torch.float32)
+    z, y, x = torch.meshgrid(ax, ax, ax, indexing="ij")
+    f = torch.sqrt(x * x + y * y + z * z).contiguous()
+    del x, y, z
+    lo, hi = 0.0, 100.0
+
+    def run():
+        out = topology.threshold(f, lo, hi)
+        return out, out.compact_point_field(f.reshape(-1))
+
+    out, vals = run()
+    T, U = out.num_cells, out.num_points
+    reps = max(3, min(args.steps, 10))
+    ms = _timed(run, reps)
+    P, C = n ** 3, (n - 1) ** 3
+    # the bytes the pipeline cannot avoid: the field read, the counts and the
+    # point mask written, the field read again by CompactPointField, and the
+    # outputs (8 ids per cell, 3 coordinates and 1 value per point)
+    alg = 4 * P + 4 * C + 4 * P + 4 * P + 32 * T + 16 * U
+    res = {"workload": f"threshold of the {n}^3 grid (origin 0, spacing 1), field |p|, range "
+                       f"[0, 100], duplicate removal, CompactPointField of the field",
+           "cells": C, "out_cells": T, "out_points": U, "ms": round(ms, 4),
+           "Gcells_per_s": round(C / (ms * 1e-3) / 1e9, 3), "alg_bytes": alg,
+           "GBps": round(alg / (ms * 1e-3) / 1e9, 1)}
+    res["frac_hbm_peak"] = round(res["GBps"] / HBM_PEAK_GBS, 4)
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        f_h = f.cpu().numpy()
+        t0 = time.perf_counter()
+        ref = orc.threshold(f_h, lo, hi)
+        sec = time.perf_counter() - t0
+        assert (ref["n_cells"], ref["n_points"]) == (T, U)
+        res["cpu_baseline"] = {"value": round(C / sec / 1e9, 4), "unit": "Gcells/s", "cores": 1,
+                               "kind": "port", "sample": f"the full input, serial restatement "
+                                                         f"(oracle), one run, {sec:.2f} s"}
+    del f, out, vals
+    torch.cuda.empty_cache()
+    _progress(f"threshold {n}^3: {ms:.3f} ms, T={T}, U={U}")
+    return res
+
+
 def hexahedra_256(args):
     """Marching cubes on the unstructured hexahedral grid of the 256^3
     uniform grid's voxels (MarchingCubesCount / MarchingCubesGenerate on
@@ -523,6 +580,9 @@ def main():
     if args.only == "hex":
         print(json.dumps({"hexahedra_256": hexahedra_256(args)}), flush=True)
         return
+    if args.only == "threshold":
+        print(json.dumps({"threshold_512": threshold_512(args)}), flush=True)
+        return
     n = args.n
     dims = (n, n, n)
     iso = 0.5
@@ -597,6 +657,7 @@ def main():
         line["slice_512"] = slice_512(args)
         line["tetrahedra_128"] = tetrahedra_128(args)
         line["hexahedra_256"] = hexahedra_256(args)
+        line["threshold_512"] = threshold_512(args)
         if args.batch > 0:
             line["batched"] = batched(args)
         if args.large:

@@ -112,6 +112,17 @@ _SIGNATURES = {
     "dax_hip_mc_hex_scratch_bytes": (sz, [i64]),
     "dax_hip_mc_hex_generate": (C.c_int, [vp, i64, vp, vp, i64, f32, C.c_uint, vp, vp, i64, vp,
                                           sz, vp, vp, vp, vp, vp, P_i64, vp]),
+    "dax_hip_threshold_count_uniform": (C.c_int, [vp, i64, i64, i64, f32, f32, vp, vp]),
+    "dax_hip_threshold_count_cells": (C.c_int, [vp, C.c_int, i64, vp, i64, f32, f32, vp, vp]),
+    "dax_hip_topology_workspace_bytes": (sz, [i64, i64]),
+    "dax_hip_topology_prepare_uniform": (C.c_int, [i64, i64, i64, vp, C.c_uint, vp, sz, vp, P_i64,
+                                                   P_i64, vp]),
+    "dax_hip_topology_generate_uniform": (C.c_int, [i64, i64, i64, vp, vp, vp, C.c_uint, vp, sz,
+                                                    vp, vp, vp, vp]),
+    "dax_hip_topology_prepare_cells": (C.c_int, [vp, C.c_int, i64, i64, vp, C.c_uint, vp, sz, vp,
+                                                 P_i64, P_i64, vp]),
+    "dax_hip_topology_generate_cells": (C.c_int, [vp, C.c_int, i64, vp, i64, vp, C.c_uint, vp, sz,
+                                                  vp, vp, vp, vp]),
     "dax_hip_mc_slice": (C.c_int, [i64, i64, i64, vp, vp, vp, vp, C.c_uint, vp, sz, vp, vp, vp,
                                    vp, vp, vp, vp, i64, i64, vp, vp]),
     "dax_hip_mc_bits": (C.c_int, [vp, i64, i64, i64, f32, vp, sz, vp]),

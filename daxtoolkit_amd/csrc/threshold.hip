@@ -1,0 +1,440 @@
+// Threshold: the reference's ThresholdCount (dax/worklet/Threshold.h:30-96,
+// a cell passes when every point value v has min <= v <= max) dispatched by
+// DispatcherMapCell, and ThresholdTopology (:98-110, the output cell's
+// vertices are the input cell's) dispatched by DispatcherGenerateTopology
+// (dax/cont/DispatcherGenerateTopology.h:146-256), on a uniform grid (voxels
+// become hexahedra with the voxel's corner order) or an unstructured grid of
+// one cell shape.
+//   count    : per cell 1 or 0 (int32, the reference's dax::Id count)
+//   prepare  : exclusive scan of the counts (first output cell of each input
+//              cell, the reference's inclusive scan + upper bounds); with
+//              duplicate removal the point mask (1 where a kept cell uses the
+//              point, FillPointMask :211-229) and its exclusive scan, which
+//              is each used point's rank among the used points -- the value
+//              the reference gets by Sort/Unique/LowerBounds of the used ids
+//              (ResolveDuplicatePoints :231-254)
+//   generate : every kept cell writes count x V vertex ids (remapped to the
+//              ranks with duplicate removal), every used point its
+//              coordinates at its rank (StreamCompact of the coordinates)
+// Counts may be any non-negative values (DispatcherGenerateTopology repeats
+// a cell count times); Threshold's are 0 or 1.
+#include "common.h"
+
+namespace daxhip {
+namespace topo {
+
+constexpr int kTX = 64, kTY = 4;  // uniform-grid blocks: 64 x-cells by 4 rows
+constexpr int kZRun = 16;         // planes one thread marches through
+
+__device__ __forceinline__ bool in_range(float v, float lo, float hi) {
+  return v >= lo && v <= hi;
+}
+
+// Uniform grid: thread (i, j) marches over kZRun cell planes keeping the
+// lower plane's four in-range flags in registers.
+__global__ __launch_bounds__(kTX* kTY) void k_count_uniform(const float* __restrict__ f, int nx,
+                                                             int ny, int nz, float lo, float hi,
+                                                             int32_t* __restrict__ counts) {
+  const int cx = nx - 1, cy = ny - 1, cz = nz - 1;
+  const int i = blockIdx.x * kTX + threadIdx.x, j = blockIdx.y * kTY + threadIdx.y;
+  const int k0 = blockIdx.z * kZRun;
+  if (i >= cx || j >= cy || k0 >= cz) return;
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  const float* p = f + i + sy * j + sz * k0;
+  bool a00 = in_range(p[0], lo, hi), a10 = in_range(p[1], lo, hi);
+  bool a01 = in_range(p[sy], lo, hi), a11 = in_range(p[sy + 1], lo, hi);
+  const int k1 = min(k0 + kZRun, cz);
+  int32_t* out = counts + i + (int64_t)cx * (j + (int64_t)cy * k0);
+  const int64_t cplane = (int64_t)cx * cy;
+  for (int k = k0; k < k1; ++k) {
+    p += sz;
+    const bool b00 = in_range(p[0], lo, hi), b10 = in_range(p[1], lo, hi);
+    const bool b01 = in_range(p[sy], lo, hi), b11 = in_range(p[sy + 1], lo, hi);
+    *out = (a00 && a10 && a01 && a11 && b00 && b10 && b01 && b11) ? 1 : 0;
+    out += cplane;
+    a00 = b00;
+    a10 = b10;
+    a01 = b01;
+    a11 = b11;
+  }
+}
+
+// Unstructured grid: a thread per cell of nv vertices (nv <= 8); an id
+// outside [0, n_points) sets *bad and counts 0.
+__global__ __launch_bounds__(256) void k_count_cells(const int32_t* __restrict__ cells, int nv,
+                                                     int64_t n_cells,
+                                                     const float* __restrict__ f,
+                                                     int64_t n_points, float lo, float hi,
+                                                     int32_t* __restrict__ counts,
+                                                     int32_t* __restrict__ bad) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  bool ok = true, keep = true;
+  for (int v = 0; v < nv; ++v) {
+    const int32_t id = cells[c * nv + v];
+    if (id < 0 || id >= n_points) {
+      ok = false;
+      break;
+    }
+    keep = keep && in_range(f[id], lo, hi);
+  }
+  if (!ok) *bad = 1;
+  counts[c] = ok && keep ? 1 : 0;
+}
+
+// Uniform grid point mask: point (i, j, k) is used when one of its (up to 8)
+// cells has a non-zero count.  A thread per point of an x-run of 64, 4 rows,
+// marching over z with the lower cell plane's flags in registers.
+__global__ __launch_bounds__(kTX* kTY) void k_mask_uniform(const int32_t* __restrict__ counts,
+                                                            int nx, int ny, int nz,
+                                                            int32_t* __restrict__ mask) {
+  const int cx = nx - 1, cy = ny - 1, cz = nz - 1;
+  const int i = blockIdx.x * kTX + threadIdx.x, j = blockIdx.y * kTY + threadIdx.y;
+  const int k0 = blockIdx.z * kZRun;
+  if (i >= nx || j >= ny || k0 >= nz) return;
+  // any non-zero count among the cells of cell plane kc adjacent to (i, j)
+  auto plane_any = [&](int kc) -> bool {
+    if (kc < 0 || kc >= cz) return false;
+    const int32_t* q = counts + (int64_t)cx * cy * kc;
+    bool any = false;
+    for (int dj = -1; dj <= 0; ++dj) {
+      const int jj = j + dj;
+      if (jj < 0 || jj >= cy) continue;
+      for (int di = -1; di <= 0; ++di) {
+        const int ii = i + di;
+        if (ii < 0 || ii >= cx) continue;
+        any = any || q[ii + (int64_t)cx * jj] != 0;
+      }
+    }
+    return any;
+  };
+  bool below = plane_any(k0 - 1);
+  const int k1 = min(k0 + kZRun, nz);
+  int32_t* out = mask + i + (int64_t)nx * (j + (int64_t)ny * k0);
+  for (int k = k0; k < k1; ++k) {
+    const bool above = plane_any(k);
+    *out = (below || above) ? 1 : 0;
+    out += (int64_t)nx * ny;
+    below = above;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Unstructured point mask: the kept cells store 1 at their vertices (every
+// store writes the same value); an id outside [0, n_points) sets *bad.
+__global__ __launch_bounds__(256) void k_mark_cells(const int32_t* __restrict__ cells, int nv,
+                                                    int64_t n_cells, int64_t n_points,
+                                                    const int32_t* __restrict__ counts,
+                                                    int32_t* __restrict__ mask,
+                                                    int32_t* __restrict__ bad) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells || counts[c] <= 0) return;
+  for (int v = 0; v < nv; ++v) {
+    const int32_t id = cells[c * nv + v];
+    if (id >= 0 && id < n_points)
+      mask[id] = 1;
+    else
+      *bad = 1;
+  }
+}
+
+// Vertex ids of voxel (i, j, k) in the hexahedron's corner order
+// (TopologyUniform.h:139-159).
+__device__ __forceinline__ void voxel_ids(int64_t c, int nx, int ny, int64_t* ids) {
+  const int cx = nx - 1, cy = ny - 1;
+  const int64_t i = c % cx, r = c / cx, j = r % cy, k = r / cy;
+  const int64_t p = i + (int64_t)nx * (j + (int64_t)ny * k), sy = nx, sz = (int64_t)nx * ny;
+  ids[0] = p;
+  ids[1] = p + 1;
+  ids[2] = p + 1 + sy;
+  ids[3] = p + sy;
+  ids[4] = p + sz;
+  ids[5] = p + 1 + sz;
+  ids[6] = p + 1 + sy + sz;
+  ids[7] = p + sy + sz;
+}
+
+// Every cell with a non-zero count writes its vertex ids count times at its
+// first output cell; with `rank` the ids are the used points' ranks.
+__global__ __launch_bounds__(256) void k_emit(const int32_t* __restrict__ cells, int nv,
+                                              int64_t n_cells, int nx, int ny,
+                                              const int32_t* __restrict__ counts,
+                                              const int32_t* __restrict__ offs,
+                                              const int32_t* __restrict__ rank,
+                                              int32_t* __restrict__ conn) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  const int n = counts[c];
+  if (n <= 0) return;
+  int64_t ids[8];
+  if (cells) {
+    for (int v = 0; v < nv; ++v) ids[v] = cells[c * nv + v];
+  } else {
+    voxel_ids(c, nx, ny, ids);
+  }
+  int32_t out[8];
+  for (int v = 0; v < nv; ++v) out[v] = rank ? rank[ids[v]] : (int32_t)ids[v];
+  int32_t* dst = conn + (int64_t)offs[c] * nv;
+  for (int r = 0; r < n; ++r)
+    for (int v = 0; v < nv; ++v) dst[(int64_t)r * nv + v] = out[v];
+}
+
+// Coordinates of the used points at their ranks: from `points`, or on a
+// uniform grid origin + spacing * (i, j, k).
+__global__ __launch_bounds__(256) void k_compact_points(const float* __restrict__ points,
+                                                        int64_t n_points, int nx, int ny,
+                                                        float ox, float oy, float oz, float hx,
+                                                        float hy, float hz,
+                                                        const int32_t* __restrict__ mask,
+                                                        const int32_t* __restrict__ rank,
+                                                        float* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_points || !mask[p]) return;
+  const int64_t u = rank[p];
+  float x, y, z;
+  if (points) {
+    x = points[3 * p];
+    y = points[3 * p + 1];
+    z = points[3 * p + 2];
+  } else {
+    const int64_t i = p % nx, r = p / nx, j = r % ny, k = r / ny;
+    x = ox + hx * (float)i;
+    y = oy + hy * (float)j;
+    z = oz + hz * (float)k;
+  }
+  out[3 * u] = x;
+  out[3 * u + 1] = y;
+  out[3 * u + 2] = z;
+}
+
+struct TopoWs {
+  size_t off_rank, bytes;
+};
+
+inline TopoWs topo_ws(int64_t n_cells, int64_t n_points) {
+  TopoWs w;
+  w.off_rank = align_up((size_t)(n_cells > 0 ? n_cells : 1) * 4, 256);
+  w.bytes = w.off_rank + align_up((size_t)(n_points > 0 ? n_points : 1) * 4, 256);
+  return w;
+}
+
+struct Geometry {  // a uniform grid (cells == nullptr) or an unstructured one
+  const int32_t* cells;
+  int nv;
+  int64_t n_cells, n_points;
+  int nx, ny, nz;
+};
+
+inline bool uniform_ok(int64_t nx, int64_t ny, int64_t nz) {
+  return nx >= 2 && ny >= 2 && nz >= 2 && nx <= 65535 * (int64_t)kTX && ny <= 65535 * kTY &&
+         nz <= 65535 * (int64_t)kZRun && nx * ny * nz <= 0x7fffffffLL;
+}
+
+int prepare(const Geometry& g, const int32_t* d_counts, unsigned flags, void* d_ws,
+            size_t ws_bytes, int32_t* d_mask, int64_t* h_cells, int64_t* h_points,
+            hipStream_t s) {
+  const bool dedup = (flags & DAX_HIP_MC_DEDUP) != 0;
+  const TopoWs w = topo_ws(g.n_cells, g.n_points);
+  if (!d_counts || !h_cells || !h_points || !d_ws || ws_bytes < w.bytes || (dedup && !d_mask) ||
+      (flags & ~(unsigned)DAX_HIP_MC_DEDUP)) {
+    set_error("topology prepare: bad arguments (workspace of %zu bytes needed)", w.bytes);
+    return DAX_HIP_BAD_VALUE;
+  }
+  *h_cells = 0;
+  *h_points = 0;
+  int32_t* offs = (int32_t*)d_ws;
+  int32_t* rank = (int32_t*)((char*)d_ws + w.off_rank);
+  int32_t total = 0;
+  int rc = dax_hip_scan_exclusive_i32(d_counts, offs, g.n_cells, &total, s);
+  if (rc) return rc;
+  if (total < 0 || (int64_t)total * g.nv > 0x7fffffffLL) {
+    set_error("topology prepare: %lld output cells of %d vertices exceed 2^31 ids",
+              (long long)total, g.nv);
+    return DAX_HIP_BAD_VALUE;
+  }
+  *h_cells = total;
+  if (!dedup) return DAX_HIP_OK;
+  const dim3 b(256), gp((unsigned)ceil_div(g.n_points, 256));
+  if (g.cells) {
+    TempBuf flag(s);
+    if ((rc = flag.alloc(sizeof(int32_t)))) return rc;
+    k_fill_i32<<<dim3(1), b, 0, s>>>((int32_t*)flag.p, 1, 0);
+    DAXHIP_LAUNCH_CHECK("k_fill_i32");
+    k_fill_i32<<<gp, b, 0, s>>>(d_mask, g.n_points, 0);
+    DAXHIP_LAUNCH_CHECK("k_fill_i32");
+    k_mark_cells<<<dim3((unsigned)ceil_div(g.n_cells, 256)), b, 0, s>>>(
+        g.cells, g.nv, g.n_cells, g.n_points, d_counts, d_mask, (int32_t*)flag.p);
+    DAXHIP_LAUNCH_CHECK("k_mark_cells");
+    int32_t bad = 0;
+    DAXHIP_CHECK(hipMemcpyAsync(&bad, flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    DAXHIP_CHECK(hipStreamSynchronize(s));
+    if (bad) {
+      set_error("topology prepare: a kept cell references a point outside [0, %lld)",
+                (long long)g.n_points);
+      return DAX_HIP_BAD_VALUE;
+    }
+  } else {
+    k_mask_uniform<<<dim3((unsigned)ceil_div(g.nx, kTX), (unsigned)ceil_div(g.ny, kTY),
+                          (unsigned)ceil_div(g.nz, kZRun)),
+                     dim3(kTX, kTY), 0, s>>>(d_counts, g.nx, g.ny, g.nz, d_mask);
+    DAXHIP_LAUNCH_CHECK("k_mask_uniform");
+  }
+  int32_t used = 0;
+  if ((rc = dax_hip_scan_exclusive_i32(d_mask, rank, g.n_points, &used, s))) return rc;
+  *h_points = used;
+  return DAX_HIP_OK;
+}
+
+int generate(const Geometry& g, const float* d_points, const float* origin, const float* spacing,
+             const int32_t* d_counts, unsigned flags, const void* d_ws, size_t ws_bytes,
+             const int32_t* d_mask, int32_t* d_conn, float* d_coords, hipStream_t s) {
+  const bool dedup = (flags & DAX_HIP_MC_DEDUP) != 0;
+  const TopoWs w = topo_ws(g.n_cells, g.n_points);
+  if (!d_counts || !d_ws || ws_bytes < w.bytes || !d_conn || (dedup && (!d_mask || !d_coords)) ||
+      (flags & ~(unsigned)DAX_HIP_MC_DEDUP)) {
+    set_error("topology generate: bad arguments (workspace of %zu bytes needed)", w.bytes);
+    return DAX_HIP_BAD_VALUE;
+  }
+  const int32_t* offs = (const int32_t*)d_ws;
+  const int32_t* rank = (const int32_t*)((const char*)d_ws + w.off_rank);
+  const dim3 b(256);
+  k_emit<<<dim3((unsigned)ceil_div(g.n_cells, 256)), b, 0, s>>>(
+      g.cells, g.nv, g.n_cells, g.nx, g.ny, d_counts, offs, dedup ? rank : nullptr, d_conn);
+  DAXHIP_LAUNCH_CHECK("k_emit");
+  if (dedup) {
+    float o[3] = {0, 0, 0}, h[3] = {1, 1, 1};
+    for (int d = 0; d < 3; ++d) {
+      if (origin) o[d] = origin[d];
+      if (spacing) h[d] = spacing[d];
+    }
+    k_compact_points<<<dim3((unsigned)ceil_div(g.n_points, 256)), b, 0, s>>>(
+        d_points, g.n_points, g.nx, g.ny, o[0], o[1], o[2], h[0], h[1], h[2], d_mask, rank,
+        d_coords);
+    DAXHIP_LAUNCH_CHECK("k_compact_points");
+  }
+  return DAX_HIP_OK;
+}
+
+}  // namespace topo
+}  // namespace daxhip
+
+using namespace daxhip;
+using namespace daxhip::topo;
+
+extern "C" {
+
+int dax_hip_threshold_count_uniform(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
+                                    float min_value, float max_value, int32_t* d_counts,
+                                    void* stream) {
+  if (!uniform_ok(nx, ny, nz) || !d_field || !d_counts) {
+    set_error("dax_hip_threshold_count_uniform: bad arguments (grid %lld x %lld x %lld)",
+              (long long)nx, (long long)ny, (long long)nz);
+    return DAX_HIP_BAD_VALUE;
+  }
+  const int cx = (int)nx - 1, cy = (int)ny - 1, cz = (int)nz - 1;
+  k_count_uniform<<<dim3((unsigned)ceil_div(cx, kTX), (unsigned)ceil_div(cy, kTY),
+                         (unsigned)ceil_div(cz, kZRun)),
+                    dim3(kTX, kTY), 0, (hipStream_t)stream>>>(d_field, (int)nx, (int)ny, (int)nz,
+                                                              min_value, max_value, d_counts);
+  DAXHIP_LAUNCH_CHECK("k_count_uniform");
+  return DAX_HIP_OK;
+}
+
+int dax_hip_threshold_count_cells(const int32_t* d_cells, int nv, int64_t n_cells,
+                                  const float* d_field, int64_t n_points, float min_value,
+                                  float max_value, int32_t* d_counts, void* stream) {
+  if (nv < 1 || nv > 8 || n_cells < 0 || n_points < 0 || n_points > 0x7fffffffLL ||
+      n_cells > 0x7fffffffLL / nv || (n_cells > 0 && (!d_cells || !d_field || !d_counts))) {
+    set_error("dax_hip_threshold_count_cells: bad arguments");
+    return DAX_HIP_BAD_VALUE;
+  }
+  if (n_cells == 0) return DAX_HIP_OK;
+  hipStream_t s = (hipStream_t)stream;
+  TempBuf flag(s);
+  int rc = flag.alloc(sizeof(int32_t));
+  if (rc) return rc;
+  DAXHIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int32_t), s));
+  k_count_cells<<<dim3((unsigned)ceil_div(n_cells, 256)), dim3(256), 0, s>>>(
+      d_cells, nv, n_cells, d_field, n_points, min_value, max_value, d_counts, (int32_t*)flag.p);
+  DAXHIP_LAUNCH_CHECK("k_count_cells");
+  int32_t bad = 0;
+  DAXHIP_CHECK(hipMemcpyAsync(&bad, flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DAXHIP_CHECK(hipStreamSynchronize(s));
+  if (bad) {
+    set_error("dax_hip_threshold_count_cells: a cell references a point outside [0, %lld)",
+              (long long)n_points);
+    return DAX_HIP_BAD_VALUE;
+  }
+  return DAX_HIP_OK;
+}
+
+size_t dax_hip_topology_workspace_bytes(int64_t n_cells, int64_t n_points) {
+  return topo_ws(n_cells, n_points).bytes;
+}
+
+int dax_hip_topology_prepare_uniform(int64_t nx, int64_t ny, int64_t nz, const int32_t* d_counts,
+                                     unsigned flags, void* d_workspace, size_t workspace_bytes,
+                                     int32_t* d_point_mask, int64_t* h_cells, int64_t* h_points,
+                                     void* stream) {
+  if (!uniform_ok(nx, ny, nz)) {
+    set_error("dax_hip_topology_prepare_uniform: unsupported grid");
+    return DAX_HIP_BAD_VALUE;
+  }
+  const Geometry g{nullptr, 8, (nx - 1) * (ny - 1) * (nz - 1), nx * ny * nz, (int)nx, (int)ny,
+                   (int)nz};
+  return prepare(g, d_counts, flags, d_workspace, workspace_bytes, d_point_mask, h_cells, h_points,
+                 (hipStream_t)stream);
+}
+
+int dax_hip_topology_generate_uniform(int64_t nx, int64_t ny, int64_t nz, const float* h_origin3,
+                                      const float* h_spacing3, const int32_t* d_counts,
+                                      unsigned flags, const void* d_workspace,
+                                      size_t workspace_bytes, const int32_t* d_point_mask,
+                                      int32_t* d_conn, float* d_coords, void* stream) {
+  if (!uniform_ok(nx, ny, nz)) {
+    set_error("dax_hip_topology_generate_uniform: unsupported grid");
+    return DAX_HIP_BAD_VALUE;
+  }
+  const Geometry g{nullptr, 8, (nx - 1) * (ny - 1) * (nz - 1), nx * ny * nz, (int)nx, (int)ny,
+                   (int)nz};
+  return generate(g, nullptr, h_origin3, h_spacing3, d_counts, flags, d_workspace, workspace_bytes,
+                  d_point_mask, d_conn, d_coords, (hipStream_t)stream);
+}
+
+int dax_hip_topology_prepare_cells(const int32_t* d_cells, int nv, int64_t n_cells,
+                                   int64_t n_points, const int32_t* d_counts, unsigned flags,
+                                   void* d_workspace, size_t workspace_bytes,
+                                   int32_t* d_point_mask, int64_t* h_cells, int64_t* h_points,
+                                   void* stream) {
+  if (nv < 1 || nv > 8 || n_cells < 1 || n_points < 1 || n_points > 0x7fffffffLL ||
+      n_cells > 0x7fffffffLL / nv || !d_cells) {
+    set_error("dax_hip_topology_prepare_cells: bad arguments");
+    return DAX_HIP_BAD_VALUE;
+  }
+  const Geometry g{d_cells, nv, n_cells, n_points, 0, 0, 0};
+  return prepare(g, d_counts, flags, d_workspace, workspace_bytes, d_point_mask, h_cells, h_points,
+                 (hipStream_t)stream);
+}
+
+int dax_hip_topology_generate_cells(const int32_t* d_cells, int nv, int64_t n_cells,
+                                    const float* d_points, int64_t n_points,
+                                    const int32_t* d_counts, unsigned flags,
+                                    const void* d_workspace, size_t workspace_bytes,
+                                    const int32_t* d_point_mask, int32_t* d_conn,
+                                    float* d_coords, void* stream) {
+  if (nv < 1 || nv > 8 || n_cells < 1 || n_points < 1 || n_points > 0x7fffffffLL ||
+      n_cells > 0x7fffffffLL / nv || !d_cells ||
+      ((flags & DAX_HIP_MC_DEDUP) && !d_points)) {
+    set_error("dax_hip_topology_generate_cells: bad arguments");
+    return DAX_HIP_BAD_VALUE;
+  }
+  const Geometry g{d_cells, nv, n_cells, n_points, 0, 0, 0};
+  return generate(g, d_points, nullptr, nullptr, d_counts, flags, d_workspace, workspace_bytes,
+                  d_point_mask, d_conn, d_coords, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+"""Threshold and topology generation on the GPU (the reference's
+ThresholdCount / ThresholdTopology, dax/worklet/Threshold.h:30-110, through
+DispatcherMapCell and DispatcherGenerateTopology,
+dax/cont/DispatcherGenerateTopology.h:146-256).
+
+A uniform grid is given by its field of shape (nz, ny, nx) (and origin,
+spacing); an unstructured grid by `cells` [C, V] int32 point ids and `points`
+[P, 3] float32.  Every call is synchronous where the reference reads a size
+back (the output cell and point counts)."""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Optional
+
+import torch
+
+from . import _lib
+from ._lib import MC_DEDUP
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _f3(v):
+    return (ctypes.c_float * 3)(*[float(x) for x in v])
+
+
+@dataclass
+class Topology:
+    """Output of DispatcherGenerateTopology: connectivity [T, V] (ids of the
+    input's points, or with duplicate removal their ranks among the used
+    points), the used points' coordinates [U, 3] and the point mask [P]
+    (duplicate removal only)."""
+    counts: torch.Tensor
+    connectivity: torch.Tensor
+    points: Optional[torch.Tensor]
+    point_mask: Optional[torch.Tensor]
+
+    @property
+    def num_cells(self) -> int:
+        return int(self.connectivity.shape[0])
+
+    @property
+    def num_points(self) -> int:
+        return 0 if self.points is None else int(self.points.shape[0])
+
+    def compact_point_field(self, values: torch.Tensor) -> torch.Tensor:
+        """CompactPointField (:97-110): StreamCompact of a point field by the
+        point mask; needs duplicate removal."""
+        if self.point_mask is None:
+            raise ValueError("compact_point_field needs duplicate removal")
+        v = values.reshape(self.point_mask.numel(), -1).contiguous()
+        out = torch.empty((max(self.num_points, 1), v.shape[1]), dtype=v.dtype, device=v.device)
+        n = ctypes.c_int64()
+        _lib.call("dax_hip_stream_compact", v.data_ptr(), v.element_size() * v.shape[1],
+                  self.point_mask.data_ptr(), 0, v.shape[0], out.data_ptr(), None, None,
+                  ctypes.byref(n), _stream())
+        res = out[: n.value]
+        return res.reshape(-1) if values.dim() == 1 else res.reshape((n.value,) +
+                                                                     tuple(values.shape[1:]))
+
+
+def _check_cells(cells: torch.Tensor, n_points: int):
+    if (cells.dtype != torch.int32 or cells.dim() != 2 or not 1 <= cells.shape[1] <= 8 or
+            not cells.is_contiguous()):
+        raise ValueError("cells must be a contiguous int32 [C, V] tensor with 1 <= V <= 8")
+    if n_points < 1:
+        raise ValueError("an unstructured grid needs points")
+
+
+def threshold_count(field: torch.Tensor, lo: float, hi: float,
+                    cells: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """ThresholdCount through DispatcherMapCell: 1 per cell whose point
+    values all lie in [lo, hi], else 0 (int32 [C])."""
+    if field.dtype != torch.float32 or not field.is_contiguous():
+        raise ValueError("field must be a contiguous float32 tensor")
+    if cells is None:
+        if field.dim() != 3:
+            raise ValueError("a uniform grid's field has shape (nz, ny, nx)")
+        nz, ny, nx = field.shape
+        counts = torch.empty(max((nx - 1) * (ny - 1) * (nz - 1), 1), dtype=torch.int32,
+                             device=field.device)
+        _lib.call("dax_hip_threshold_count_uniform", field.data_ptr(), nx, ny, nz, float(lo),
+                  float(hi), counts.data_ptr(), _stream())
+        return counts
+    _check_cells(cells, field.numel())
+    C, V = cells.shape
+    counts = torch.empty(max(C, 1), dtype=torch.int32, device=field.device)
+    _lib.call("dax_hip_threshold_count_cells", cells.data_ptr(), V, C, field.data_ptr(),
+              field.numel(), float(lo), float(hi), counts.data_ptr(), _stream())
+    return counts[:C]
+
+
+def generate_topology(counts: torch.Tensor, dims=None, cells: Optional[torch.Tensor] = None,
+                      points: Optional[torch.Tensor] = None, origin=(0, 0, 0), spacing=(1, 1, 1),
+                      dedup: bool = True) -> Topology:
+    """DispatcherGenerateTopology with ThresholdTopology: each cell repeated
+    counts[c] times with its vertices (uniform grids: the voxel's corners in
+    the hexahedron's order)."""
+    if counts.dtype != torch.int32 or not counts.is_contiguous():
+        raise ValueError("counts must be a contiguous int32 tensor")
+    dev = counts.device
+    lib = _lib.load()
+    t, u = ctypes.c_int64(), ctypes.c_int64()
+    flags = MC_DEDUP if dedup else 0
+    if cells is None:
+        nx, ny, nz = (int(v) for v in dims)
+        C, P, V = (nx - 1) * (ny - 1) * (nz - 1), nx * ny * nz, 8
+    else:
+        P = int(points.shape[0]) if points is not None else int(cells.max()) + 1
+        _check_cells(cells, P)
+        if points is not None and (points.dtype != torch.float32 or points.shape != (P, 3) or
+                                   not points.is_contiguous()):
+            raise ValueError("points must be a contiguous float32 [P, 3] tensor")
+        C, V = cells.shape
+    if counts.numel() < C:
+        raise ValueError("one count per cell")
+    ws = torch.empty(int(lib.dax_hip_topology_workspace_bytes(C, P)), dtype=torch.uint8,
+                     device=dev)
+    mask = torch.empty(P, dtype=torch.int32, device=dev) if dedup else None
+    mptr = mask.data_ptr() if dedup else None
+    if cells is None:
+        _lib.call("dax_hip_topology_prepare_uniform", nx, ny, nz, counts.data_ptr(), flags,
+                  ws.data_ptr(), ws.numel(), mptr, ctypes.byref(t), ctypes.byref(u), _stream())
+    else:
+        _lib.call("dax_hip_topology_prepare_cells", cells.data_ptr(), V, C, P, counts.data_ptr(),
+                  flags, ws.data_ptr(), ws.numel(), mptr, ctypes.byref(t), ctypes.byref(u),
+                  _stream())
+    T, U = int(t.value), int(u.value)
+    conn = torch.empty((max(T, 1), V), dtype=torch.int32, device=dev)
+    coords = torch.empty((max(U, 1), 3), dtype=torch.float32, device=dev) if dedup else None
+    cptr = coords.data_ptr() if dedup else None
+    if T > 0:
+        if cells is None:
+            _lib.call("dax_hip_topology_generate_uniform", nx, ny, nz, _f3(origin), _f3(spacing),
+                      counts.data_ptr(), flags, ws.data_ptr(), ws.numel(), mptr,
+                      conn.data_ptr(), cptr, _stream())
+        else:
+            _lib.call("dax_hip_topology_generate_cells", cells.data_ptr(), V, C,
+                      points.data_ptr() if points is not None else None, P, counts.data_ptr(),
+                      flags, ws.data_ptr(), ws.numel(), mptr, conn.data_ptr(), cptr, _stream())
+    return Topology(counts=counts[:C], connectivity=conn[:T],
+                    points=coords[:U] if dedup else None, point_mask=mask)
+
+
+def threshold(field: torch.Tensor, lo: float, hi: float, cells: Optional[torch.Tensor] = None,
+              points: Optional[torch.Tensor] = None, origin=(0, 0, 0), spacing=(1, 1, 1),
+              dedup: bool = True) -> Topology:
+    """The reference's Threshold pipeline (Benchmarks/Threshold/Pipeline.h:
+    163-190): ThresholdCount, then DispatcherGenerateTopology."""
+    counts = threshold_count(field, lo, hi, cells)
+    dims = None if cells is not None else (field.shape[2], field.shape[1], field.shape[0])
+    return generate_topology(counts, dims=dims, cells=cells, points=points, origin=origin,
+                             spacing=spacing, dedup=dedup)

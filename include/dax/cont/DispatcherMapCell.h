@@ -5,6 +5,7 @@
 //   SliceCount(o, n):        Invoke(grid, coords, count)  the same for the plane
 //   MarchingTetrahedraCount(iso): Invoke(tetGrid, field, count) on an
 //                            UnstructuredGrid<CellTagTetrahedron>
+//   ThresholdCount<Scalar>(min, max): Invoke(grid, field, count), any grid
 //   CellGradient:            Invoke(grid, coords, field, gradient)
 #ifndef DAX_HIP_SLICE_DISPATCHERMAPCELL_H
 #define DAX_HIP_SLICE_DISPATCHERMAPCELL_H
@@ -17,6 +18,7 @@
 #include <dax/worklet/MarchingCubes.h>
 #include <dax/worklet/MarchingTetrahedra.h>
 #include <dax/worklet/Slice.h>
+#include <dax/worklet/Threshold.h>
 
 namespace dax {
 namespace cont {
@@ -130,6 +132,48 @@ class DispatcherMapCell<dax::worklet::MarchingTetrahedraCount,
 
  private:
   dax::worklet::MarchingTetrahedraCount Worklet;
+};
+
+template <>
+class DispatcherMapCell<dax::worklet::ThresholdCount<dax::Scalar>,
+                        dax::hip::cont::DeviceAdapterTagHip> {
+ public:
+  typedef dax::hip::cont::DeviceAdapterTagHip Tag;
+  explicit DispatcherMapCell(const dax::worklet::ThresholdCount<dax::Scalar>& w) : Worklet(w) {}
+
+  template <class CField, class CCount>
+  void Invoke(const UniformGrid<Tag>& grid, const ArrayHandle<dax::Scalar, CField, Tag>& field,
+              ArrayHandle<dax::Id, CCount, Tag>& count) const {
+    static_assert(sizeof(dax::Id) == 4, "the HIP threshold path writes 32-bit counts");
+    const dax::Id3 d = grid.GetPointDimensions();
+    if (field.GetNumberOfValues() != grid.GetNumberOfPoints())
+      throw ErrorControlBadValue("field size does not match the grid's point count");
+    const float* f = field.PrepareForInput().GetRawPointer();
+    int32_t* c = count.PrepareForOutput(grid.GetNumberOfCells()).GetRawPointer();
+    internal::HipCheck(dax_hip_threshold_count_uniform(f, d[0], d[1], d[2], Worklet.GetMin(),
+                                                       Worklet.GetMax(), c, nullptr));
+    internal::HipCheck(dax_hip_synchronize(nullptr));
+  }
+
+  template <class CellTag, class CConn, class CPts, class CField, class CCount>
+  void Invoke(const UnstructuredGrid<CellTag, CConn, CPts, Tag>& grid,
+              const ArrayHandle<dax::Scalar, CField, Tag>& field,
+              ArrayHandle<dax::Id, CCount, Tag>& count) const {
+    static_assert(sizeof(dax::Id) == 4, "the HIP threshold path reads 32-bit ids");
+    if (field.GetNumberOfValues() != grid.GetNumberOfPoints())
+      throw ErrorControlBadValue("field size does not match the grid's point count");
+    const dax::Id n = grid.GetNumberOfCells();
+    const int32_t* cells = grid.GetCellConnections().PrepareForInput().GetRawPointer();
+    const float* f = field.PrepareForInput().GetRawPointer();
+    int32_t* c = count.PrepareForOutput(n).GetRawPointer();
+    internal::HipCheck(dax_hip_threshold_count_cells(cells, dax::CellTraits<CellTag>::NUM_VERTICES,
+                                                     n, f, grid.GetNumberOfPoints(),
+                                                     Worklet.GetMin(), Worklet.GetMax(), c,
+                                                     nullptr));
+  }
+
+ private:
+  dax::worklet::ThresholdCount<dax::Scalar> Worklet;
 };
 
 template <>

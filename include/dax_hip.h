@@ -261,6 +261,58 @@ int dax_hip_mc_hex_generate(const int32_t* d_hexes, int64_t n_hexes, const float
                             int32_t* d_conn, int32_t* d_first, int32_t* d_second,
                             float* d_weight, int64_t* h_points, void* stream);
 
+/* Threshold (dax/worklet/Threshold.h:30-110) through DispatcherMapCell and
+ * DispatcherGenerateTopology (dax/cont/DispatcherGenerateTopology.h:146-256).
+ *
+ * dax_hip_threshold_count_uniform / _cells replace
+ * DispatcherMapCell<ThresholdCount<Scalar>>: d_counts[c] = 1 when every
+ * point value v of cell c has min <= v <= max, else 0.  Uniform grids: cells
+ * x-fastest, (nx-1)(ny-1)(nz-1) of them, d_field [nx*ny*nz].  Unstructured:
+ * d_cells [n_cells][nv] int32 point ids (1 <= nv <= 8); an id outside
+ * [0, n_points) is DAX_HIP_BAD_VALUE (synchronous).
+ *
+ * dax_hip_topology_prepare_* / _generate_* replace
+ * DispatcherGenerateTopology<ThresholdTopology> for any non-negative counts
+ * (a cell is repeated count times, in cell order).  prepare (synchronous):
+ * *h_cells = T output cells; with DAX_HIP_MC_DEDUP (RemoveDuplicatePoints,
+ * the reference's default) d_point_mask [n_points] int32 = 1 where a kept
+ * cell uses the point (the reference's PointMask, for CompactPointField =
+ * dax_hip_stream_compact with it as the stencil) and *h_points = U used
+ * points.  generate (after prepare with the same counts and workspace):
+ * d_conn [T][nv] = the input cells' vertex ids (uniform grids: the voxel's 8
+ * corners in the hexahedron's order), remapped with dedup to the points'
+ * ranks among the used points (the reference's Sort/Unique/LowerBounds),
+ * and d_coords [U][3] the used points' coordinates in point order (uniform:
+ * origin + spacing * ijk); without dedup the ids are the input's and no
+ * coordinates are written.  Workspace: dax_hip_topology_workspace_bytes. */
+int dax_hip_threshold_count_uniform(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
+                                    float min_value, float max_value, int32_t* d_counts,
+                                    void* stream);
+int dax_hip_threshold_count_cells(const int32_t* d_cells, int nv, int64_t n_cells,
+                                  const float* d_field, int64_t n_points, float min_value,
+                                  float max_value, int32_t* d_counts, void* stream);
+size_t dax_hip_topology_workspace_bytes(int64_t n_cells, int64_t n_points);
+int dax_hip_topology_prepare_uniform(int64_t nx, int64_t ny, int64_t nz, const int32_t* d_counts,
+                                     unsigned flags, void* d_workspace, size_t workspace_bytes,
+                                     int32_t* d_point_mask, int64_t* h_cells, int64_t* h_points,
+                                     void* stream);
+int dax_hip_topology_generate_uniform(int64_t nx, int64_t ny, int64_t nz, const float* h_origin3,
+                                      const float* h_spacing3, const int32_t* d_counts,
+                                      unsigned flags, const void* d_workspace,
+                                      size_t workspace_bytes, const int32_t* d_point_mask,
+                                      int32_t* d_conn, float* d_coords, void* stream);
+int dax_hip_topology_prepare_cells(const int32_t* d_cells, int nv, int64_t n_cells,
+                                   int64_t n_points, const int32_t* d_counts, unsigned flags,
+                                   void* d_workspace, size_t workspace_bytes,
+                                   int32_t* d_point_mask, int64_t* h_cells, int64_t* h_points,
+                                   void* stream);
+int dax_hip_topology_generate_cells(const int32_t* d_cells, int nv, int64_t n_cells,
+                                    const float* d_points, int64_t n_points,
+                                    const int32_t* d_counts, unsigned flags,
+                                    const void* d_workspace, size_t workspace_bytes,
+                                    const int32_t* d_point_mask, int32_t* d_conn,
+                                    float* d_coords, void* stream);
+
 /* Synchronous: reads {T, U} written by a forward (or classify) on `stream`,
  * and returns DAX_HIP_CAPACITY if the generate step was skipped for lack of
  * room, DAX_HIP_EXEC_ERROR if T or U left the int32 dax::Id range. */

@@ -628,6 +628,83 @@ int oracle_mc_hex_generate(const int32_t* hexes, int64_t n_hexes, const float* f
                                   [](int c) { return kOracleCaseEdges[c]; }, kHexEdge);
 }
 
+// Threshold, Serial-adapter sequence: ThresholdCount per cell (every point
+// value v with lo <= v <= hi, dax/worklet/Threshold.h:30-96) through
+// DispatcherMapCell; DispatcherGenerateTopology
+// (dax/cont/DispatcherGenerateTopology.h:146-256): inclusive scan of the
+// counts, UpperBounds of [0, T) in it (the input cell of each output cell),
+// ThresholdTopology copies the input cell's vertices (:98-110); with dedup
+// the point mask of the used ids (:211-229), StreamCompact of the
+// coordinates by it, and the connectivity replaced by LowerBounds in the
+// sorted unique used ids (:231-254).  cells == nullptr: the uniform grid
+// `dims` (voxels x-fastest, corners in the hexahedron's order, coordinates
+// origin + spacing * ijk).  Returns 1 if cap_cells output cells do not fit.
+int oracle_threshold(const int32_t* cells, int nv, int64_t n_cells, const int64_t* dims,
+                     const float* origin, const float* spacing, const float* points,
+                     int64_t n_points, const float* field, float lo, float hi, int dedup,
+                     int64_t cap_cells, int32_t* counts, int64_t* n_out_cells,
+                     int64_t* n_out_points, int32_t* conn, float* coords, int32_t* mask) {
+  Grid g{};
+  if (!cells) {
+    g = make_grid(dims, origin, spacing);
+    nv = 8;
+    n_cells = g.cells();
+    n_points = g.points();
+  }
+  auto verts_of = [&](int64_t c, Id* v) {
+    if (cells)
+      for (int q = 0; q < nv; ++q) v[q] = cells[c * nv + q];
+    else
+      cell_vertices(g, c, v);  // the uniform grid's voxel topology
+  };
+  std::vector<Id> scanned(n_cells);
+  Id run = 0;
+  for (int64_t c = 0; c < n_cells; ++c) {
+    Id v[8];
+    verts_of(c, v);
+    int valid = 1;
+    for (int q = 0; q < nv; ++q) valid &= field[v[q]] >= lo && field[v[q]] <= hi;
+    counts[c] = valid;
+    run += valid;
+    scanned[c] = run;
+  }
+  const int64_t T = run;
+  *n_out_cells = T;
+  *n_out_points = 0;
+  if (T > cap_cells) return 1;
+  for (int64_t t = 0; t < T; ++t) {
+    const int64_t c = std::upper_bound(scanned.begin(), scanned.end(), (Id)t) - scanned.begin();
+    Id v[8];
+    verts_of(c, v);
+    for (int q = 0; q < nv; ++q) conn[t * nv + q] = v[q];
+  }
+  if (!dedup) return 0;
+  std::fill(mask, mask + n_points, 0);
+  for (int64_t i = 0; i < T * nv; ++i) mask[conn[i]] = 1;
+  int64_t u = 0;
+  for (int64_t p = 0; p < n_points; ++p) {
+    if (!mask[p]) continue;
+    for (int d = 0; d < 3; ++d) {
+      float x;
+      if (cells) {
+        x = points[3 * p + d];
+      } else {
+        const int64_t idx = d == 0 ? p % g.nx : d == 1 ? (p / g.nx) % g.ny : p / (g.nx * g.ny);
+        x = g.origin[d] + g.spacing[d] * (float)idx;
+      }
+      coords[3 * u + d] = x;
+    }
+    ++u;
+  }
+  *n_out_points = u;
+  std::vector<Id> used(conn, conn + T * nv);
+  std::sort(used.begin(), used.end());
+  used.erase(std::unique(used.begin(), used.end()), used.end());
+  for (int64_t i = 0; i < T * nv; ++i)
+    conn[i] = (Id)(std::lower_bound(used.begin(), used.end(), conn[i]) - used.begin());
+  return 0;
+}
+
 // Analytic adjoint in FP64, scatter form: dL/df for every point.
 void oracle_mc_backward_f64(const float* field, const int64_t* dims, const float* origin,
                             const float* spacing, float iso, const int32_t* first,

@@ -57,6 +57,10 @@ def lib():
                                           C.c_float, C.c_int, C.c_int64, _i64p, _i64p] +
                                          [C.c_void_p] * 5)
         L.oracle_mt_generate.restype = C.c_int
+        L.oracle_threshold.argtypes = ([C.c_void_p, C.c_int, C.c_int64] + [C.c_void_p] * 4 +
+                                       [C.c_int64, C.c_void_p, C.c_float, C.c_float, C.c_int,
+                                        C.c_int64, C.c_void_p, _i64p, _i64p] + [C.c_void_p] * 3)
+        L.oracle_threshold.restype = C.c_int
         L.oracle_mc_hex_generate.argtypes = L.oracle_mt_generate.argtypes
         L.oracle_mc_hex_generate.restype = C.c_int
         L.oracle_tet_tables.argtypes = [C.c_void_p, C.c_void_p]
@@ -283,6 +287,46 @@ def mc_hex_generate(hexes, field, points, iso, dedup=True):
     """Marching cubes on an unstructured hexahedral grid, the Serial-adapter
     sequence (see mc_oracle.cpp)."""
     return _unstructured(lib().oracle_mc_hex_generate, 8, hexes, field, points, iso, dedup)
+
+
+def threshold(field, lo, hi, dedup=True, cells=None, points=None, origin=(0, 0, 0),
+              spacing=(1, 1, 1)):
+    """Threshold through the Serial-adapter sequence (see mc_oracle.cpp).
+    Uniform grid: field of shape (nz, ny, nx); unstructured: cells [C, V]
+    int32 and points [P, 3].  Returns counts, n_cells, n_points, conn
+    [n_cells, V], coords [n_points, 3] and mask (dedup), and the compacted
+    field (CompactPointField = StreamCompact of the field by the mask)."""
+    if cells is None:
+        f = _field(field)
+        dims = _dims(f)
+        nv, C_ = 8, int((dims[0] - 1) * (dims[1] - 1) * (dims[2] - 1))
+        P = int(f.size)
+        cptr, pptr = None, None
+        f = f.reshape(-1)
+    else:
+        cells = np.ascontiguousarray(cells, np.int32)
+        nv, C_ = cells.shape[1], cells.shape[0]
+        f = np.ascontiguousarray(field, np.float32).reshape(-1)
+        points = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        P = int(f.size)
+        dims = np.zeros(3, np.int64)
+        cptr, pptr = _p(cells), _p(points)
+    counts = np.zeros(max(C_, 1), np.int32)
+    conn = np.zeros(max(C_, 1) * nv, np.int32)
+    coords = np.zeros((max(P, 1), 3), np.float32)
+    mask = np.zeros(max(P, 1), np.int32)
+    nt, npt = C.c_int64(), C.c_int64()
+    rc = lib().oracle_threshold(cptr, nv, C_, _p(dims), _p(_f3(origin)), _p(_f3(spacing)), pptr, P,
+                                _p(f), float(lo), float(hi), int(dedup), C_, _p(counts),
+                                C.byref(nt), C.byref(npt), _p(conn), _p(coords), _p(mask))
+    assert rc == 0
+    T, U = nt.value, npt.value
+    out = {"counts": counts[:C_], "n_cells": T, "n_points": U, "conn": conn[:T * nv].reshape(T, nv)}
+    if dedup:
+        out["coords"] = coords[:U]
+        out["mask"] = mask[:P]
+        out["field"] = f[mask[:P] != 0]
+    return out
 
 
 def backward_f64(field, iso, first, second, dldx, dldn, origin=(0, 0, 0), spacing=(1, 1, 1)):

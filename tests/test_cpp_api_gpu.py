@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 BIN = Path(__file__).resolve().parent / "cpp" / "bin"
 TESTS = ["UnitTestDeviceAdapterHip", "UnitTestWorkletMarchingCubesHip",
          "UnitTestWorkletCellGradientHip", "UnitTestWorkletSliceHip",
-         "UnitTestWorkletMarchingTetrahedraHip"]
+         "UnitTestWorkletMarchingTetrahedraHip", "UnitTestWorkletThresholdHip"]
 
 
 @pytest.mark.parametrize("name", TESTS)

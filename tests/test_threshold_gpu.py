@@ -1,0 +1,139 @@
+"""GPU Threshold (ThresholdCount / ThresholdTopology through DispatcherMapCell
+and DispatcherGenerateTopology, dax/worklet/Threshold.h:30-110,
+dax/cont/DispatcherGenerateTopology.h:146-256): bit-exact parity with the
+oracle (counts, connectivity, used-point coordinates, point mask,
+CompactPointField) on the reference's unit test, random uniform grids with
+origin and spacing, shuffled tetrahedral grids and hexahedral grids, both
+duplicate-removal modes; cells repeated by counts above 1; and at 256^3 on
+the reference benchmark's field (point magnitude, [0, 100]) the output
+against a torch restatement of the definition."""
+import numpy as np
+import pytest
+import torch
+
+from daxtoolkit_amd import _lib, topology
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(out, ref, dedup):
+    assert out.num_cells == ref["n_cells"]
+    assert np.array_equal(out.counts.cpu().numpy(), ref["counts"])
+    assert np.array_equal(out.connectivity.cpu().numpy(), ref["conn"])
+    if dedup:
+        assert out.num_points == ref["n_points"]
+        assert np.array_equal(out.points.cpu().numpy().view(np.uint32),
+                              ref["coords"].view(np.uint32))
+        assert np.array_equal(out.point_mask.cpu().numpy(), ref["mask"])
+
+
+def test_reference_unit_test_26():
+    n = 26
+    f = np.add.outer(np.add.outer(np.arange(n), np.arange(n)), np.arange(n)).astype(np.float32)
+    print("[case] threshold unit26", flush=True)
+    ft = torch.from_numpy(f).cuda()
+    out = topology.threshold(ft, 70, 82)
+    ref = orc.threshold(f, 70, 82)
+    _check(out, ref, True)
+    vals = out.compact_point_field(ft.reshape(-1)).cpu().numpy()
+    assert np.array_equal(vals, ref["field"])
+    assert out.num_cells == 10 and np.all((vals >= 70) & (vals <= 82))
+
+
+@pytest.mark.parametrize("dedup", [True, False])
+@pytest.mark.parametrize("dims", [(9, 7, 6), (70, 33, 20), (130, 5, 40), (2, 2, 2)])
+def test_uniform_matches_oracle(dims, dedup):
+    rng = np.random.default_rng(sum(dims))
+    nx, ny, nz = dims
+    f = rng.random((nz, ny, nx)).astype(np.float32)
+    origin, spacing = (0.5, -1.0, 2.0), (0.25, 1.5, 1.0)
+    print(f"[case] threshold uniform {dims} dedup={dedup}", flush=True)
+    out = topology.threshold(torch.from_numpy(f).cuda(), 0.03, 0.97, origin=origin,
+                             spacing=spacing, dedup=dedup)
+    ref = orc.threshold(f, 0.03, 0.97, dedup=dedup, origin=origin, spacing=spacing)
+    _check(out, ref, dedup)
+
+
+@pytest.mark.parametrize("dedup", [True, False])
+def test_shuffled_tetrahedra_and_hexahedra_match_oracle(dedup):
+    rng = np.random.default_rng(4)
+    for name, builder, dims in (("tets", orc.freudenthal_tets, (12, 9, 8)),
+                                ("hexes", orc.voxel_hexes, (20, 13, 9))):
+        cells, pts = builder(dims)
+        perm = rng.permutation(pts.shape[0]).astype(np.int32)
+        inv = np.empty_like(perm)
+        inv[perm] = np.arange(perm.size, dtype=np.int32)
+        cells, pts = inv[cells], pts[perm]
+        f = rng.random(pts.shape[0]).astype(np.float32)
+        print(f"[case] threshold {name} {dims} dedup={dedup}", flush=True)
+        ft = torch.from_numpy(f).cuda()
+        out = topology.threshold(ft, 0.1, 0.95, cells=torch.from_numpy(cells).cuda(),
+                                 points=torch.from_numpy(pts).cuda(), dedup=dedup)
+        ref = orc.threshold(f, 0.1, 0.95, dedup=dedup, cells=cells, points=pts)
+        _check(out, ref, dedup)
+        if dedup:
+            assert np.array_equal(out.compact_point_field(ft).cpu().numpy(), ref["field"])
+
+
+def test_counts_above_one_repeat_cells():
+    """DispatcherGenerateTopology with counts 0..3: cell c appears counts[c]
+    times, in cell order."""
+    dims = (7, 6, 5)
+    C = 6 * 5 * 4
+    rng = np.random.default_rng(2)
+    counts = rng.integers(0, 4, C).astype(np.int32)
+    hexes, pts = orc.voxel_hexes(dims)
+    print("[case] topology counts 0..3", flush=True)
+    out = topology.generate_topology(torch.from_numpy(counts).cuda(), dims=dims, dedup=False)
+    assert np.array_equal(out.connectivity.cpu().numpy(), np.repeat(hexes, counts, axis=0))
+    out2 = topology.generate_topology(torch.from_numpy(counts).cuda(), dims=dims, dedup=True)
+    used = np.unique(np.repeat(hexes, counts, axis=0))
+    assert np.array_equal(out2.connectivity.cpu().numpy(),
+                          np.searchsorted(used, np.repeat(hexes, counts, axis=0)))
+    assert np.array_equal(out2.points.cpu().numpy(), pts[used])
+
+
+def test_magnitude_256_against_torch_definition():
+    """The reference benchmark's pipeline (Benchmarks/Threshold/Pipeline.h:
+    163-190) at 256^3: field = |p| (origin 0, spacing 1), [0, 100]."""
+    n = 256
+    print("[case] threshold magnitude 256^3", flush=True)
+    ax = torch.arange(n, device="cuda", dtype=torch.float32)
+    z, y, x = torch.meshgrid(ax, ax, ax, indexing="ij")
+    f = torch.sqrt(x * x + y * y + z * z).contiguous()
+    out = topology.threshold(f, 0.0, 100.0)
+    inr = (f >= 0) & (f <= 100)
+    keep = torch.ones((n - 1, n - 1, n - 1), dtype=torch.bool, device="cuda")
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                keep &= inr[dz:dz + n - 1, dy:dy + n - 1, dx:dx + n - 1]
+    assert torch.equal(out.counts.view(n - 1, n - 1, n - 1), keep.to(torch.int32))
+    T = int(keep.sum())
+    assert out.num_cells == T > 400_000
+    used = torch.zeros((n, n, n), dtype=torch.bool, device="cuda")
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                used[dz:dz + n - 1, dy:dy + n - 1, dx:dx + n - 1] |= keep
+    assert torch.equal(out.point_mask.view(n, n, n), used.to(torch.int32))
+    assert out.num_points == int(used.sum())
+    # every output cell's points are its voxel's corners, in order
+    idx = torch.nonzero(keep.reshape(-1)).reshape(-1)
+    i, j, k = idx % (n - 1), (idx // (n - 1)) % (n - 1), idx // ((n - 1) * (n - 1))
+    corner = torch.tensor([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1],
+                           [1, 1, 1], [0, 1, 1]], device="cuda")
+    expect = torch.stack([i, j, k], 1)[:, None, :] + corner[None]
+    got = out.points[out.connectivity.long()]
+    assert torch.equal(got, expect.to(torch.float32))
+    vals = out.compact_point_field(f.reshape(-1))
+    assert torch.equal(vals, f.reshape(-1)[used.reshape(-1)])
+
+
+def test_bad_ids_are_rejected():
+    pts = torch.zeros((4, 3), device="cuda")
+    f = torch.zeros(4, device="cuda")
+    tets = torch.tensor([[0, 1, 2, 7]], dtype=torch.int32, device="cuda")
+    with pytest.raises(_lib.ErrorControlBadValue):
+        topology.threshold(f, -1.0, 1.0, cells=tets, points=pts)
