@@ -123,6 +123,7 @@ _SIGNATURES = {
                                                  P_i64, P_i64, vp]),
     "dax_hip_topology_generate_cells": (C.c_int, [vp, C.c_int, i64, vp, i64, vp, C.c_uint, vp, sz,
                                                   vp, vp, vp, vp]),
+    "dax_hip_topology_compact_point_field": (C.c_int, [vp, sz, i64, i64, vp, vp, sz, vp, vp]),
     "dax_hip_mc_slice": (C.c_int, [i64, i64, i64, vp, vp, vp, vp, C.c_uint, vp, sz, vp, vp, vp,
                                    vp, vp, vp, vp, i64, i64, vp, vp]),
     "dax_hip_mc_bits": (C.c_int, [vp, i64, i64, i64, f32, vp, sz, vp]),

@@ -62,8 +62,6 @@ int copy_bytes(void* d_dst, const void* d_src, size_t bytes, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // block-wide helpers (256 threads)
 constexpr int kNT = 256;
-constexpr int kIPT = 8;  // items per thread in scans
-constexpr int kTile = kNT * kIPT;
 
 template <typename T>
 __device__ __forceinline__ T shfl_up(T v, int d) {
@@ -89,10 +87,10 @@ __device__ __forceinline__ T wave_scan_incl(T v) {
   return v;
 }
 
-// Exclusive block scan of one value per thread; returns the block total in
-// `total`.
-template <typename T>
-__device__ __forceinline__ T block_scan_excl(T v, T& total, T* s_wave /* [4] */) {
+// Exclusive block scan of one value per thread (NT threads); returns the
+// block total in `total`.
+template <typename T, int NT = kNT>
+__device__ __forceinline__ T block_scan_excl(T v, T& total, T* s_wave /* [NT / 64] */) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const T inc = wave_scan_incl(v);
   if (lane == 63) s_wave[wv] = inc;
@@ -100,7 +98,7 @@ __device__ __forceinline__ T block_scan_excl(T v, T& total, T* s_wave /* [4] */)
   T off = T(0);
   T tot = T(0);
 #pragma unroll
-  for (int q = 0; q < kNT / 64; ++q) {
+  for (int q = 0; q < NT / 64; ++q) {
     if (q < wv) off = off + s_wave[q];
     tot = tot + s_wave[q];
   }
@@ -110,65 +108,113 @@ __device__ __forceinline__ T block_scan_excl(T v, T& total, T* s_wave /* [4] */)
 }
 
 // ---------------------------------------------------------------------------
-// scan
+// scan: reduce-then-scan over tiles of kNT threads x scan_ipt<T>() items; each
+// thread owns 64 contiguous bytes, read and written as 16-byte vectors when
+// the arrays are 16-byte aligned.
 template <typename T>
-__global__ __launch_bounds__(kNT) void k_tile_sums(const T* __restrict__ in, int64_t n,
+__host__ __device__ constexpr int scan_ipt() {
+  return 64 / (int)sizeof(T);
+}
+
+template <typename T>
+__device__ __forceinline__ void load_chunk(const T* __restrict__ in, int64_t base, int64_t n,
+                                           bool vec, T* vals) {
+  constexpr int IPT = scan_ipt<T>();
+  if (vec && base + IPT <= n) {
+    const int4* p = (const int4*)(in + base);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int4 v = p[q];
+      __builtin_memcpy(&vals[q * IPT / 4], &v, 16);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) vals[i] = base + i < n ? in[base + i] : T(0);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_chunk(T* __restrict__ out, int64_t base, int64_t n, bool vec,
+                                            const T* vals) {
+  constexpr int IPT = scan_ipt<T>();
+  if (vec && base + IPT <= n) {
+    int4* p = (int4*)(out + base);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int4 v;
+      __builtin_memcpy(&v, &vals[q * IPT / 4], 16);
+      p[q] = v;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+      if (base + i < n) out[base + i] = vals[i];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_tile_sums(const T* __restrict__ in, int64_t n, bool vec,
                                                    T* __restrict__ sums) {
+  constexpr int IPT = scan_ipt<T>();
   __shared__ T s_wave[kNT / 64];
-  const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kIPT;
+  const int64_t base = ((int64_t)blockIdx.x * kNT + threadIdx.x) * IPT;
+  T vals[IPT];
+  load_chunk(in, base, n, vec, vals);
   T v = T(0);
 #pragma unroll
-  for (int i = 0; i < kIPT; ++i)
-    if (base + i < n) v = v + in[base + i];
+  for (int i = 0; i < IPT; ++i) v = v + vals[i];
   T total;
   (void)block_scan_excl(v, total, s_wave);
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// Exclusive scan of the tile sums in one block (sequential over chunks).
+// Exclusive scan of the n tile sums in one block of kSumsNT threads, each
+// owning a contiguous segment; the grand total goes to *grand.
+constexpr int kSumsNT = 1024;
 template <typename T>
-__global__ __launch_bounds__(kNT) void k_scan_sums(T* __restrict__ sums, int64_t n,
-                                                   T* __restrict__ grand) {
-  __shared__ T s_wave[kNT / 64];
-  T carry = T(0);
-  for (int64_t b = 0; b < n; b += kNT) {
-    const int64_t i = b + threadIdx.x;
-    const T v = i < n ? sums[i] : T(0);
-    T total;
-    const T ex = block_scan_excl(v, total, s_wave);
-    if (i < n) sums[i] = carry + ex;
-    carry = carry + total;
+__global__ __launch_bounds__(kSumsNT) void k_scan_sums(T* __restrict__ sums, int64_t n,
+                                                       T* __restrict__ grand) {
+  __shared__ T s_wave[kSumsNT / 64];
+  const int64_t seg = (n + kSumsNT - 1) / kSumsNT;
+  const int64_t b = (int64_t)threadIdx.x * seg, e = min(b + seg, n);
+  T v = T(0);
+  for (int64_t i = b; i < e; ++i) v = v + sums[i];
+  T total;
+  T run = block_scan_excl<T, kSumsNT>(v, total, s_wave);
+  for (int64_t i = b; i < e; ++i) {
+    const T x = sums[i];
+    sums[i] = run;
+    run = run + x;
   }
-  if (threadIdx.x == 0) *grand = carry;
+  if (threadIdx.x == 0) *grand = total;
 }
 
 template <typename T>
 __global__ __launch_bounds__(kNT) void k_tile_scan(const T* __restrict__ in, int64_t n,
-                                                   const T* __restrict__ prefix,
+                                                   const T* __restrict__ prefix, bool vec,
                                                    T* __restrict__ out, int inclusive) {
+  constexpr int IPT = scan_ipt<T>();
   __shared__ T s_wave[kNT / 64];
-  const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kIPT;
-  T vals[kIPT];
+  const int64_t base = ((int64_t)blockIdx.x * kNT + threadIdx.x) * IPT;
+  T vals[IPT];
+  load_chunk(in, base, n, vec, vals);
   T v = T(0);
 #pragma unroll
-  for (int i = 0; i < kIPT; ++i) {
-    vals[i] = base + i < n ? in[base + i] : T(0);
-    v = v + vals[i];
-  }
+  for (int i = 0; i < IPT; ++i) v = v + vals[i];
   T total;
   T run = prefix[blockIdx.x] + block_scan_excl(v, total, s_wave);
 #pragma unroll
-  for (int i = 0; i < kIPT; ++i) {
-    if (base + i < n) {
-      if (inclusive) {
-        run = run + vals[i];
-        out[base + i] = run;
-      } else {
-        out[base + i] = run;
-        run = run + vals[i];
-      }
+  for (int i = 0; i < IPT; ++i) {
+    const T x = vals[i];
+    if (inclusive) {
+      run = run + x;
+      vals[i] = run;
+    } else {
+      vals[i] = run;
+      run = run + x;
     }
   }
+  store_chunk(out, base, n, vec, vals);
 }
 
 template <typename T>
@@ -177,16 +223,18 @@ int scan(const T* in, T* out, int64_t n, int inclusive, T* h_total, hipStream_t 
     if (h_total) *h_total = T(0);
     return DAX_HIP_OK;
   }
-  const int64_t tiles = ceil_div(n, (int64_t)kTile);
+  const int64_t tile = (int64_t)kNT * scan_ipt<T>();
+  const int64_t tiles = ceil_div(n, tile);
+  const bool vec = (((uintptr_t)in | (uintptr_t)out) & 15u) == 0;
   TempBuf tmp(s);
   int rc = tmp.alloc((size_t)(tiles + 1) * sizeof(T));
   if (rc) return rc;
   T* sums = (T*)tmp.p;
-  k_tile_sums<T><<<dim3((unsigned)tiles), dim3(kNT), 0, s>>>(in, n, sums);
+  k_tile_sums<T><<<dim3((unsigned)tiles), dim3(kNT), 0, s>>>(in, n, vec, sums);
   DAXHIP_LAUNCH_CHECK("k_tile_sums");
-  k_scan_sums<T><<<1, kNT, 0, s>>>(sums, tiles, sums + tiles);
+  k_scan_sums<T><<<1, kSumsNT, 0, s>>>(sums, tiles, sums + tiles);
   DAXHIP_LAUNCH_CHECK("k_scan_sums");
-  k_tile_scan<T><<<dim3((unsigned)tiles), dim3(kNT), 0, s>>>(in, n, sums, out, inclusive);
+  k_tile_scan<T><<<dim3((unsigned)tiles), dim3(kNT), 0, s>>>(in, n, sums, vec, out, inclusive);
   DAXHIP_LAUNCH_CHECK("k_tile_scan");
   if (!h_total) return DAX_HIP_OK;  // stream-ordered, no host round trip
   T total;

@@ -211,6 +211,19 @@ __global__ __launch_bounds__(256) void k_compact_points(const float* __restrict_
   out[3 * u + 2] = z;
 }
 
+// CompactPointField with the ranks of the prepare step: the used points'
+// values, in point order (a StreamCompact by the point mask).
+__global__ __launch_bounds__(256) void k_compact_values(const uint32_t* __restrict__ in, int words,
+                                                        int64_t n_points,
+                                                        const int32_t* __restrict__ mask,
+                                                        const int32_t* __restrict__ rank,
+                                                        uint32_t* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_points || !mask[p]) return;
+  const int64_t u = rank[p];
+  for (int w = 0; w < words; ++w) out[u * words + w] = in[p * words + w];
+}
+
 struct TopoWs {
   size_t off_rank, bytes;
 };
@@ -435,6 +448,26 @@ int dax_hip_topology_generate_cells(const int32_t* d_cells, int nv, int64_t n_ce
   const Geometry g{d_cells, nv, n_cells, n_points, 0, 0, 0};
   return generate(g, d_points, nullptr, nullptr, d_counts, flags, d_workspace, workspace_bytes,
                   d_point_mask, d_conn, d_coords, (hipStream_t)stream);
+}
+
+int dax_hip_topology_compact_point_field(const void* d_in, size_t value_bytes, int64_t n_cells,
+                                         int64_t n_points, const int32_t* d_point_mask,
+                                         const void* d_workspace, size_t workspace_bytes,
+                                         void* d_out, void* stream) {
+  const TopoWs w = topo_ws(n_cells, n_points);
+  if (value_bytes == 0 || value_bytes % 4 || value_bytes > 1024 || n_points < 0 ||
+      n_points > 0x7fffffffLL || !d_workspace || workspace_bytes < w.bytes ||
+      (n_points > 0 && (!d_in || !d_point_mask || !d_out))) {
+    set_error("dax_hip_topology_compact_point_field: bad arguments (values of 4k bytes, "
+              "workspace of %zu bytes)", w.bytes);
+    return DAX_HIP_BAD_VALUE;
+  }
+  if (n_points == 0) return DAX_HIP_OK;
+  k_compact_values<<<dim3((unsigned)ceil_div(n_points, 256)), dim3(256), 0, (hipStream_t)stream>>>(
+      (const uint32_t*)d_in, (int)(value_bytes / 4), n_points, d_point_mask,
+      (const int32_t*)((const char*)d_workspace + w.off_rank), (uint32_t*)d_out);
+  DAXHIP_LAUNCH_CHECK("k_compact_values");
+  return DAX_HIP_OK;
 }
 
 }  // extern "C"

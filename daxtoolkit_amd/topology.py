@@ -37,6 +37,8 @@ class Topology:
     connectivity: torch.Tensor
     points: Optional[torch.Tensor]
     point_mask: Optional[torch.Tensor]
+    workspace: Optional[torch.Tensor] = None   # the prepare step's (ranks of the used points)
+    num_input_cells: int = 0
 
     @property
     def num_cells(self) -> int:
@@ -47,18 +49,21 @@ class Topology:
         return 0 if self.points is None else int(self.points.shape[0])
 
     def compact_point_field(self, values: torch.Tensor) -> torch.Tensor:
-        """CompactPointField (:97-110): StreamCompact of a point field by the
-        point mask; needs duplicate removal."""
+        """CompactPointField (:97-110): the used points' values in point
+        order (a StreamCompact by the point mask); needs duplicate removal."""
         if self.point_mask is None:
             raise ValueError("compact_point_field needs duplicate removal")
-        v = values.reshape(self.point_mask.numel(), -1).contiguous()
+        P = self.point_mask.numel()
+        v = values.reshape(P, -1).contiguous()
+        if v.element_size() * v.shape[1] % 4:
+            raise ValueError("point values of a multiple of 4 bytes")
         out = torch.empty((max(self.num_points, 1), v.shape[1]), dtype=v.dtype, device=v.device)
-        n = ctypes.c_int64()
-        _lib.call("dax_hip_stream_compact", v.data_ptr(), v.element_size() * v.shape[1],
-                  self.point_mask.data_ptr(), 0, v.shape[0], out.data_ptr(), None, None,
-                  ctypes.byref(n), _stream())
-        res = out[: n.value]
-        return res.reshape(-1) if values.dim() == 1 else res.reshape((n.value,) +
+        _lib.call("dax_hip_topology_compact_point_field", v.data_ptr(),
+                  v.element_size() * v.shape[1], self.num_input_cells, P,
+                  self.point_mask.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(),
+                  out.data_ptr(), _stream())
+        res = out[: self.num_points]
+        return res.reshape(-1) if values.dim() == 1 else res.reshape((self.num_points,) +
                                                                      tuple(values.shape[1:]))
 
 
@@ -142,7 +147,8 @@ def generate_topology(counts: torch.Tensor, dims=None, cells: Optional[torch.Ten
                       points.data_ptr() if points is not None else None, P, counts.data_ptr(),
                       flags, ws.data_ptr(), ws.numel(), mptr, conn.data_ptr(), cptr, _stream())
     return Topology(counts=counts[:C], connectivity=conn[:T],
-                    points=coords[:U] if dedup else None, point_mask=mask)
+                    points=coords[:U] if dedup else None, point_mask=mask, workspace=ws,
+                    num_input_cells=C)
 
 
 def threshold(field: torch.Tensor, lo: float, hi: float, cells: Optional[torch.Tensor] = None,

@@ -3,8 +3,9 @@
 //
 // Same interface: constructor (count[, worklet]), SetRemoveDuplicatePoints
 // (default true), SetReleaseCount (default true), GetCount, DoReleaseCount,
-// Invoke(inGrid, outGrid) and CompactPointField(input, output) (a
-// StreamCompact by the point mask; false without duplicate removal).  Input:
+// Invoke(inGrid, outGrid) and CompactPointField(input, output) (the
+// StreamCompact by the point mask, done with the ranks of the used points
+// that Invoke computed; false without duplicate removal).  Input:
 // a UniformGrid (its voxels become hexahedra in the voxel's corner order) or
 // an UnstructuredGrid of one cell shape; every input cell is repeated
 // count[c] times.  With duplicate removal the output holds the used points'
@@ -13,6 +14,8 @@
 // are left as they were (the reference writes only the topology then).
 #ifndef DAX_HIP_SLICE_DISPATCHERGENERATETOPOLOGY_H
 #define DAX_HIP_SLICE_DISPATCHERGENERATETOPOLOGY_H
+
+#include <memory>
 
 #include <dax/CellTag.h>
 #include <dax/cont/ArrayHandle.h>
@@ -50,12 +53,21 @@ class DispatcherGenerateTopology<dax::worklet::ThresholdTopology, CountHandleTyp
   void SetRemoveDuplicatePoints(bool b) { RemoveDuplicatePoints = b; }
   bool GetRemoveDuplicatePoints() const { return RemoveDuplicatePoints; }
 
-  // (:97-110)
+  // (:97-110): the StreamCompact of `input` by the point mask of the last
+  // Invoke, done with that Invoke's ranks of the used points.
   template <typename T, class C1, class C2>
   bool CompactPointField(const ArrayHandle<T, C1, DeviceAdapterTag>& input,
                          ArrayHandle<T, C2, DeviceAdapterTag>& output) {
     if (!RemoveDuplicatePoints) return false;
-    DeviceAdapterAlgorithm<DeviceAdapterTag>::StreamCompact(input, PointMask, output);
+    static_assert(sizeof(T) % 4 == 0, "point values of a multiple of 4 bytes");
+    if (!Ws || input.GetNumberOfValues() != NumPoints)
+      throw ErrorControlBadValue("CompactPointField needs an Invoke on a grid of as many points");
+    const T* in = input.PrepareForInput().GetRawPointer();
+    T* out = output.PrepareForOutput(NumUsed).GetRawPointer();
+    internal::HipCheck(dax_hip_topology_compact_point_field(
+        in, sizeof(T), NumCells, NumPoints, PointMask.PrepareForInput().GetRawPointer(),
+        Ws->Data, Ws->Bytes, out, nullptr));
+    internal::HipCheck(dax_hip_synchronize(nullptr));
     return true;
   }
 
@@ -66,14 +78,14 @@ class DispatcherGenerateTopology<dax::worklet::ThresholdTopology, CountHandleTyp
     const dax::Vector3 o = inGrid.GetOrigin(), s = inGrid.GetSpacing();
     const float origin[3] = {o[0], o[1], o[2]}, spacing[3] = {s[0], s[1], s[2]};
     const int32_t* counts = Counts(inGrid.GetNumberOfCells());
-    internal::HipWorkspace ws(
-        dax_hip_topology_workspace_bytes(inGrid.GetNumberOfCells(), inGrid.GetNumberOfPoints()));
+    internal::HipWorkspace& ws = Workspace(inGrid.GetNumberOfCells(), inGrid.GetNumberOfPoints());
     int32_t* mask = RemoveDuplicatePoints
                         ? PointMask.PrepareForOutput(inGrid.GetNumberOfPoints()).GetRawPointer()
                         : nullptr;
     int64_t T = 0, U = 0;
     internal::HipCheck(dax_hip_topology_prepare_uniform(d[0], d[1], d[2], counts, Flags(), ws.Data,
                                                         ws.Bytes, mask, &T, &U, nullptr));
+    NumUsed = (dax::Id)U;
     int32_t* conn = nullptr;
     float* coords = nullptr;
     PrepareOutputs(outGrid, 8, T, U, conn, coords);
@@ -94,11 +106,12 @@ class DispatcherGenerateTopology<dax::worklet::ThresholdTopology, CountHandleTyp
     const int32_t* cells = inGrid.GetCellConnections().PrepareForInput().GetRawPointer();
     const float* pts = (const float*)inGrid.GetPointCoordinates().PrepareForInput().GetRawPointer();
     const int32_t* counts = Counts(n);
-    internal::HipWorkspace ws(dax_hip_topology_workspace_bytes(n, P));
+    internal::HipWorkspace& ws = Workspace(n, P);
     int32_t* mask = RemoveDuplicatePoints ? PointMask.PrepareForOutput(P).GetRawPointer() : nullptr;
     int64_t T = 0, U = 0;
     internal::HipCheck(dax_hip_topology_prepare_cells(cells, nv, n, P, counts, Flags(), ws.Data,
                                                       ws.Bytes, mask, &T, &U, nullptr));
+    NumUsed = (dax::Id)U;
     int32_t* conn = nullptr;
     float* coords = nullptr;
     PrepareOutputs(outGrid, nv, T, U, conn, coords);
@@ -112,6 +125,16 @@ class DispatcherGenerateTopology<dax::worklet::ThresholdTopology, CountHandleTyp
 
  private:
   unsigned Flags() const { return RemoveDuplicatePoints ? DAX_HIP_MC_DEDUP : 0u; }
+
+  // The workspace of this Invoke, kept for CompactPointField.
+  internal::HipWorkspace& Workspace(dax::Id n_cells, dax::Id n_points) {
+    NumCells = n_cells;
+    NumPoints = n_points;
+    NumUsed = 0;
+    Ws = std::make_shared<internal::HipWorkspace>(
+        dax_hip_topology_workspace_bytes(n_cells, n_points));
+    return *Ws;
+  }
 
   const int32_t* Counts(dax::Id n) const {
     static_assert(sizeof(dax::Id) == 4, "the HIP topology path reads 32-bit counts");
@@ -136,6 +159,8 @@ class DispatcherGenerateTopology<dax::worklet::ThresholdTopology, CountHandleTyp
   bool RemoveDuplicatePoints, ReleaseCount;
   CountHandleType Count;
   PointMaskType PointMask;
+  std::shared_ptr<internal::HipWorkspace> Ws;
+  dax::Id NumCells = 0, NumPoints = 0, NumUsed = 0;
 };
 
 }  // namespace cont

@@ -312,6 +312,15 @@ int dax_hip_topology_generate_cells(const int32_t* d_cells, int nv, int64_t n_ce
                                     const void* d_workspace, size_t workspace_bytes,
                                     const int32_t* d_point_mask, int32_t* d_conn,
                                     float* d_coords, void* stream);
+/* CompactPointField after a prepare with DAX_HIP_MC_DEDUP (same workspace and
+ * mask): d_out[U] = the used points' values of d_in [n_points] (value_bytes
+ * each, a multiple of 4), in point order -- the StreamCompact of the
+ * reference (DispatcherGenerateTopology.h:97-110) done with the prepare
+ * step's ranks. */
+int dax_hip_topology_compact_point_field(const void* d_in, size_t value_bytes, int64_t n_cells,
+                                         int64_t n_points, const int32_t* d_point_mask,
+                                         const void* d_workspace, size_t workspace_bytes,
+                                         void* d_out, void* stream);
 
 /* Synchronous: reads {T, U} written by a forward (or classify) on `stream`,
  * and returns DAX_HIP_CAPACITY if the generate step was skipped for lack of
