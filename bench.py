@@ -31,6 +31,9 @@ the headline the line carries
   threshold_512  the reference's Threshold benchmark pipeline at 512^3
            (|p| field, [0, 100], duplicate removal, CompactPointField), with
            the serial restatement beside it;
+  vtk_write_512  the 512^3 contour written as a VTK legacy file by the
+           library's host writer, with the reference's ofstream writer
+           (restated) beside it;
   batched  1024 volumes with per-axis extents U{32..160}, 4-16 Gaussians each,
            iso U[0.3,0.7]: the batched forward and forward+backward replayed
            as captured graphs, against a native launch loop of single-volume
@@ -93,7 +96,7 @@ def parse():
     ap.add_argument("--no-bwd", action="store_true", help="skip the forward+backward timing")
     ap.add_argument("--large", action="store_true", help="also run the 1024^3 Mandelbulb")
     ap.add_argument("--only", choices=["fwd", "batch", "large", "diff", "slice", "tets", "hex",
-                                       "threshold"],
+                                       "threshold", "vtk"],
                     default=None,
                     help="run one section only (profiling); prints its JSON")
     return ap.parse_args()
@@ -479,6 +482,51 @@ def threshold_512(args):
     return res
 
 
+def vtk_write_512(args):
+    """The 512^3 benchmark contour (dedup) written as a VTK legacy POLYDATA
+    file (the reference benchmark's PrintContentsToStream format) by the
+    library's multi-threaded host writer, wall-clock from host arrays to a
+    closed file under /tmp; beside it the oracle's ofstream restatement of
+    the reference's writer on the same arrays (the files are compared)."""
+    import tempfile
+
+    import torch
+
+    from daxtoolkit_amd import fields, io, mc
+    n = 512
+    dims = (n, n, n)
+    field = fields.gaussian_mixture_torch(dims, fields.gaussian_mixture_params(0, dims))
+    out = mc.forward(field, 0.5, normals=False)
+    pts = out.vertices.cpu().numpy()
+    conn = out.connectivity.reshape(-1, 3).cpu().numpy()
+    del field, out
+    torch.cuda.empty_cache()
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        a, b = f"{d}/lib.vtk", f"{d}/ref.vtk"
+        t0 = time.perf_counter()
+        io.write_vtk_polydata(a, pts, conn)
+        sec = time.perf_counter() - t0
+        size = os.path.getsize(a)
+        res = {"workload": f"VTK legacy POLYDATA file of the {n}^3 benchmark contour "
+                           f"({conn.shape[0]} triangles, {pts.shape[0]} points)",
+               "bytes": size, "s": round(sec, 4), "MB_per_s": round(size / sec / 1e6, 1),
+               "threads": os.cpu_count()}
+        if not args.no_cpu_baseline:
+            from oracle import oracle as orc
+            t0 = time.perf_counter()
+            orc.write_vtk(b, pts, conn, polydata=True)
+            ref_sec = time.perf_counter() - t0
+            with open(a, "rb") as fa, open(b, "rb") as fb:
+                same = fa.read() == fb.read()
+            res["identical_to_reference_stream"] = same
+            res["cpu_baseline"] = {"value": round(size / ref_sec / 1e6, 1), "unit": "MB/s",
+                                   "cores": 1, "kind": "port",
+                                   "sample": f"the same arrays through an ofstream as the "
+                                             f"reference writes them, {ref_sec:.2f} s"}
+    _progress(f"vtk write: {sec:.3f} s, {size / 1e6:.1f} MB")
+    return res
+
+
 def hexahedra_256(args):
     """Marching cubes on the unstructured hexahedral grid of the 256^3
     uniform grid's voxels (MarchingCubesCount / MarchingCubesGenerate on
@@ -583,6 +631,9 @@ def main():
     if args.only == "threshold":
         print(json.dumps({"threshold_512": threshold_512(args)}), flush=True)
         return
+    if args.only == "vtk":
+        print(json.dumps({"vtk_write_512": vtk_write_512(args)}), flush=True)
+        return
     n = args.n
     dims = (n, n, n)
     iso = 0.5
@@ -658,6 +709,7 @@ def main():
         line["tetrahedra_128"] = tetrahedra_128(args)
         line["hexahedra_256"] = hexahedra_256(args)
         line["threshold_512"] = threshold_512(args)
+        line["vtk_write_512"] = vtk_write_512(args)
         if args.batch > 0:
             line["batched"] = batched(args)
         if args.large:

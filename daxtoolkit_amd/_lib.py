@@ -124,6 +124,9 @@ _SIGNATURES = {
     "dax_hip_topology_generate_cells": (C.c_int, [vp, C.c_int, i64, vp, i64, vp, C.c_uint, vp, sz,
                                                   vp, vp, vp, vp]),
     "dax_hip_topology_compact_point_field": (C.c_int, [vp, sz, i64, i64, vp, vp, sz, vp, vp]),
+    "dax_hip_write_vtk_polydata": (C.c_int, [C.c_char_p, vp, i64, vp, i64, C.c_int]),
+    "dax_hip_write_vtk_unstructured": (C.c_int, [C.c_char_p, vp, i64, vp, i64, C.c_int,
+                                                 C.c_int]),
     "dax_hip_mc_slice": (C.c_int, [i64, i64, i64, vp, vp, vp, vp, C.c_uint, vp, sz, vp, vp, vp,
                                    vp, vp, vp, vp, i64, i64, vp, vp]),
     "dax_hip_mc_bits": (C.c_int, [vp, i64, i64, i64, f32, vp, sz, vp]),

@@ -322,6 +322,20 @@ int dax_hip_topology_compact_point_field(const void* d_in, size_t value_bytes, i
                                          const void* d_workspace, size_t workspace_bytes,
                                          void* d_out, void* stream);
 
+/* VTK legacy ASCII files of an output grid held in host memory: the format
+ * of the reference benchmarks' PrintContentsToStream
+ * (Benchmarks/MarchingCubes/Pipeline.h:51-98: DATASET POLYDATA, POLYGONS;
+ * Benchmarks/Threshold/Pipeline.h:99-150: DATASET UNSTRUCTURED_GRID, CELLS,
+ * CELL_TYPES with vtk_cell_type for every cell, e.g. 12 = hexahedron,
+ * 10 = tetrahedron), numbers as an ostream writes them by default.
+ * h_points [n_points][3] f32, h_conn [n_cells][verts_per_cell] int32 (each
+ * id in [0, n_points)).  Host code; formats on all host threads. */
+int dax_hip_write_vtk_polydata(const char* path, const float* h_points, int64_t n_points,
+                               const int32_t* h_conn, int64_t n_cells, int verts_per_cell);
+int dax_hip_write_vtk_unstructured(const char* path, const float* h_points, int64_t n_points,
+                                   const int32_t* h_conn, int64_t n_cells, int verts_per_cell,
+                                   int vtk_cell_type);
+
 /* Synchronous: reads {T, U} written by a forward (or classify) on `stream`,
  * and returns DAX_HIP_CAPACITY if the generate step was skipped for lack of
  * room, DAX_HIP_EXEC_ERROR if T or U left the int32 dax::Id range. */

@@ -30,6 +30,7 @@
 //                     torch float64 autograd restatement in the tests
 //   oracle_mc_pipeline_omp  equality with the serial oracle
 #include <algorithm>
+#include <fstream>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -703,6 +704,42 @@ int oracle_threshold(const int32_t* cells, int nv, int64_t n_cells, const int64_
   for (int64_t i = 0; i < T * nv; ++i)
     conn[i] = (Id)(std::lower_bound(used.begin(), used.end(), conn[i]) - used.begin());
   return 0;
+}
+
+// VTK legacy writers, the reference benchmarks' PrintContentsToStream stream
+// sequence written with an ofstream: POLYDATA/POLYGONS
+// (Benchmarks/MarchingCubes/Pipeline.h:51-98) and UNSTRUCTURED_GRID/CELLS/
+// CELL_TYPES (Benchmarks/Threshold/Pipeline.h:99-150).  Returns 0 on success.
+int oracle_write_vtk(const char* path, int polydata, const float* pts, int64_t n_points,
+                     const int32_t* conn, int64_t n_cells, int nv, int cell_type) {
+  std::ofstream stream(path);
+  if (!stream) return 1;
+  stream << "# vtk DataFile Version 3.0" << std::endl;
+  stream << "vtk output" << std::endl;
+  stream << "ASCII" << std::endl;
+  stream << (polydata ? "DATASET POLYDATA" : "DATASET UNSTRUCTURED_GRID") << std::endl;
+  stream << "POINTS " << n_points << " float" << std::endl;
+  for (int64_t i = 0; i < n_points; ++i) {
+    stream << pts[3 * i] << " " << pts[3 * i + 1] << " " << pts[3 * i + 2] << " ";
+    if (i % 3 == 2) stream << std::endl;
+  }
+  if (n_points % 3 != 0) stream << std::endl;
+  stream << (polydata ? "POLYGONS " : "CELLS ") << n_cells << " " << n_cells * (nv + 1)
+         << std::endl;
+  for (int64_t c = 0; c < n_cells; ++c) {
+    stream << nv << " ";
+    for (int v = 0; v < nv; ++v) {
+      stream << conn[c * nv + v];
+      if (v + 1 < nv) stream << " ";
+    }
+    stream << std::endl;
+  }
+  stream << std::endl;
+  if (!polydata) {
+    stream << "CELL_TYPES " << n_cells << std::endl;
+    for (int64_t c = 0; c < n_cells; ++c) stream << cell_type << std::endl;
+  }
+  return stream.good() ? 0 : 1;
 }
 
 // Analytic adjoint in FP64, scatter form: dL/df for every point.

@@ -61,6 +61,9 @@ def lib():
                                        [C.c_int64, C.c_void_p, C.c_float, C.c_float, C.c_int,
                                         C.c_int64, C.c_void_p, _i64p, _i64p] + [C.c_void_p] * 3)
         L.oracle_threshold.restype = C.c_int
+        L.oracle_write_vtk.argtypes = [C.c_char_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
+                                       C.c_int64, C.c_int, C.c_int]
+        L.oracle_write_vtk.restype = C.c_int
         L.oracle_mc_hex_generate.argtypes = L.oracle_mt_generate.argtypes
         L.oracle_mc_hex_generate.restype = C.c_int
         L.oracle_tet_tables.argtypes = [C.c_void_p, C.c_void_p]
@@ -327,6 +330,17 @@ def threshold(field, lo, hi, dedup=True, cells=None, points=None, origin=(0, 0, 
         out["mask"] = mask[:P]
         out["field"] = f[mask[:P] != 0]
     return out
+
+
+def write_vtk(path, points, conn, polydata=True, cell_type=12):
+    """The reference benchmarks' ASCII VTK writers (see mc_oracle.cpp)."""
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    conn = np.ascontiguousarray(conn, np.int32)
+    nv = conn.shape[1] if conn.ndim == 2 else 3
+    n_cells = conn.size // nv
+    rc = lib().oracle_write_vtk(str(path).encode(), int(polydata), _p(pts), pts.shape[0], _p(conn),
+                                n_cells, nv, cell_type)
+    assert rc == 0
 
 
 def backward_f64(field, iso, first, second, dldx, dldn, origin=(0, 0, 0), spacing=(1, 1, 1)):

@@ -7,13 +7,16 @@
 // the grid's voxels as an UnstructuredGrid<CellTagHexahedron>
 // (dax/cont/testing/TestingGridGenerator.h:266-295), whose merged output
 // must also equal the uniform grid's.
+#include <fstream>
 #include <iostream>
+#include <sstream>
 #include <vector>
 
 #include <dax/cont/DispatcherGenerateInterpolatedCells.h>
 #include <dax/cont/DispatcherMapCell.h>
 #include <dax/cont/UniformGrid.h>
 #include <dax/cont/UnstructuredGrid.h>
+#include <dax/cont/VTKWriter.h>
 
 #include "Testing.h"
 
@@ -122,6 +125,22 @@ void TestMarchingCubes() {
     used[conn.Get(i)] = 1;
   }
   for (int u : used) DAX_TEST_ASSERT(u == 1, "unused output point");
+
+  // the merged contour as a VTK legacy file (the reference benchmark's
+  // PrintContentsToStream format)
+  const std::string path = "/tmp/UnitTestWorkletMarchingCubesHip.vtk";
+  dax::cont::WriteVTK(secondOutGrid, path);
+  std::ifstream in(path);
+  std::stringstream text;
+  text << in.rdbuf();
+  std::ostringstream head;
+  head << "# vtk DataFile Version 3.0\nvtk output\nASCII\nDATASET POLYDATA\nPOINTS "
+       << NumberOfUniquePoints << " float\n";
+  DAX_TEST_ASSERT(text.str().rfind(head.str(), 0) == 0, "VTK header");
+  std::ostringstream polys;
+  polys << "POLYGONS " << secondOutGrid.GetNumberOfCells() << " "
+        << 4 * secondOutGrid.GetNumberOfCells() << "\n";
+  DAX_TEST_ASSERT(text.str().find(polys.str()) != std::string::npos, "VTK polygons");
 }
 
 void TestMarchingCubesHexahedra() {
