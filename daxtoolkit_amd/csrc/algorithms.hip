@@ -168,25 +168,36 @@ __global__ __launch_bounds__(kNT) void k_tile_sums(const T* __restrict__ in, int
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// Exclusive scan of the n tile sums in one block of kSumsNT threads, each
-// owning a contiguous segment; the grand total goes to *grand.
+// Exclusive scan of the n tile sums in one block of kSumsNT threads: chunks
+// of kSumsNT x 16 bytes in order, each thread scanning 16 contiguous bytes
+// (coalesced), the chunk's total carried to the next; the grand total goes
+// to *grand.
 constexpr int kSumsNT = 1024;
 template <typename T>
 __global__ __launch_bounds__(kSumsNT) void k_scan_sums(T* __restrict__ sums, int64_t n,
                                                        T* __restrict__ grand) {
+  constexpr int V = 16 / (int)sizeof(T);
   __shared__ T s_wave[kSumsNT / 64];
-  const int64_t seg = (n + kSumsNT - 1) / kSumsNT;
-  const int64_t b = (int64_t)threadIdx.x * seg, e = min(b + seg, n);
-  T v = T(0);
-  for (int64_t i = b; i < e; ++i) v = v + sums[i];
-  T total;
-  T run = block_scan_excl<T, kSumsNT>(v, total, s_wave);
-  for (int64_t i = b; i < e; ++i) {
-    const T x = sums[i];
-    sums[i] = run;
-    run = run + x;
+  T carry = T(0);
+  for (int64_t c0 = 0; c0 < n; c0 += (int64_t)kSumsNT * V) {
+    const int64_t b = c0 + (int64_t)threadIdx.x * V;
+    T v[V];
+    T t = T(0);
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+      v[q] = b + q < n ? sums[b + q] : T(0);
+      t = t + v[q];
+    }
+    T total;
+    T run = carry + block_scan_excl<T, kSumsNT>(t, total, s_wave);
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+      if (b + q < n) sums[b + q] = run;
+      run = run + v[q];
+    }
+    carry = carry + total;
   }
-  if (threadIdx.x == 0) *grand = total;
+  if (threadIdx.x == 0) *grand = carry;
 }
 
 template <typename T>

@@ -83,37 +83,42 @@ __global__ __launch_bounds__(256) void k_count_cells(const int32_t* __restrict__
 }
 
 // Uniform grid point mask: point (i, j, k) is used when one of its (up to 8)
-// cells has a non-zero count.  A thread per point of an x-run of 64, 4 rows,
-// marching over z with the lower cell plane's flags in registers.
+// cells has a non-zero count.  A wave per 64 points of a row (blockDim.x is
+// the wave), 4 rows per block, marching over z with the lower cell plane's
+// result in a register; per plane each lane reads its cell column x = i of
+// rows j-1 and j and takes column i-1 from the lane below (lane 0 reads it).
 __global__ __launch_bounds__(kTX* kTY) void k_mask_uniform(const int32_t* __restrict__ counts,
                                                             int nx, int ny, int nz,
                                                             int32_t* __restrict__ mask) {
   const int cx = nx - 1, cy = ny - 1, cz = nz - 1;
   const int i = blockIdx.x * kTX + threadIdx.x, j = blockIdx.y * kTY + threadIdx.y;
   const int k0 = blockIdx.z * kZRun;
-  if (i >= nx || j >= ny || k0 >= nz) return;
-  // any non-zero count among the cells of cell plane kc adjacent to (i, j)
-  auto plane_any = [&](int kc) -> bool {
-    if (kc < 0 || kc >= cz) return false;
-    const int32_t* q = counts + (int64_t)cx * cy * kc;
-    bool any = false;
-    for (int dj = -1; dj <= 0; ++dj) {
-      const int jj = j + dj;
-      if (jj < 0 || jj >= cy) continue;
-      for (int di = -1; di <= 0; ++di) {
-        const int ii = i + di;
-        if (ii < 0 || ii >= cx) continue;
-        any = any || q[ii + (int64_t)cx * jj] != 0;
-      }
-    }
+  if (j >= ny || k0 >= nz) return;  // whole waves (one row per wave)
+  const bool lane0 = threadIdx.x == 0;
+  const int64_t row_lo = (int64_t)cx * (j - 1), row_hi = (int64_t)cx * j;
+  const bool has_lo = j - 1 >= 0, has_hi = j < cy;
+  // cell columns x of rows j-1 and j of cell plane kc: any non-zero count
+  auto column = [&](const int32_t* q, int x) -> int {
+    if (x < 0 || x >= cx) return 0;
+    int any = 0;
+    if (has_lo) any |= q[row_lo + x] != 0;
+    if (has_hi) any |= q[row_hi + x] != 0;
     return any;
   };
-  bool below = plane_any(k0 - 1);
+  auto plane_any = [&](int kc) -> int {
+    const bool ok = kc >= 0 && kc < cz;
+    const int32_t* q = counts + (int64_t)cx * cy * (ok ? kc : 0);
+    const int own = ok ? column(q, i) : 0;
+    int left = __shfl_up(own, 1, 64);
+    if (lane0) left = ok ? column(q, i - 1) : 0;
+    return own | left;
+  };
+  int below = plane_any(k0 - 1);
   const int k1 = min(k0 + kZRun, nz);
   int32_t* out = mask + i + (int64_t)nx * (j + (int64_t)ny * k0);
   for (int k = k0; k < k1; ++k) {
-    const bool above = plane_any(k);
-    *out = (below || above) ? 1 : 0;
+    const int above = plane_any(k);
+    if (i < nx) *out = below | above;
     out += (int64_t)nx * ny;
     below = above;
   }
@@ -159,70 +164,112 @@ __device__ __forceinline__ void voxel_ids(int64_t c, int nx, int ny, int64_t* id
 }
 
 // Every cell with a non-zero count writes its vertex ids count times at its
-// first output cell; with `rank` the ids are the used points' ranks.
+// first output cell; with `rank` the ids are the used points' ranks.  A
+// thread per 4 consecutive cells (one 16-byte load of their counts when
+// `vec`: the counts are 16-byte aligned).
 __global__ __launch_bounds__(256) void k_emit(const int32_t* __restrict__ cells, int nv,
                                               int64_t n_cells, int nx, int ny,
-                                              const int32_t* __restrict__ counts,
+                                              const int32_t* __restrict__ counts, bool vec,
                                               const int32_t* __restrict__ offs,
                                               const int32_t* __restrict__ rank,
                                               int32_t* __restrict__ conn) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n_cells) return;
-  const int n = counts[c];
-  if (n <= 0) return;
-  int64_t ids[8];
-  if (cells) {
-    for (int v = 0; v < nv; ++v) ids[v] = cells[c * nv + v];
+  const int64_t c0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (c0 >= n_cells) return;
+  int cnt[4];
+  if (vec && c0 + 4 <= n_cells) {
+    const int4 v = *(const int4*)(counts + c0);
+    cnt[0] = v.x;
+    cnt[1] = v.y;
+    cnt[2] = v.z;
+    cnt[3] = v.w;
   } else {
-    voxel_ids(c, nx, ny, ids);
+    for (int q = 0; q < 4; ++q) cnt[q] = c0 + q < n_cells ? counts[c0 + q] : 0;
   }
-  int32_t out[8];
-  for (int v = 0; v < nv; ++v) out[v] = rank ? rank[ids[v]] : (int32_t)ids[v];
-  int32_t* dst = conn + (int64_t)offs[c] * nv;
-  for (int r = 0; r < n; ++r)
-    for (int v = 0; v < nv; ++v) dst[(int64_t)r * nv + v] = out[v];
+  for (int q = 0; q < 4; ++q) {
+    const int n = cnt[q];
+    if (n <= 0) continue;
+    const int64_t c = c0 + q;
+    int64_t ids[8];
+    if (cells) {
+      for (int v = 0; v < nv; ++v) ids[v] = cells[c * nv + v];
+    } else {
+      voxel_ids(c, nx, ny, ids);
+    }
+    int32_t out[8];
+    for (int v = 0; v < nv; ++v) out[v] = rank ? rank[ids[v]] : (int32_t)ids[v];
+    int32_t* dst = conn + (int64_t)offs[c] * nv;
+    for (int r = 0; r < n; ++r)
+      for (int v = 0; v < nv; ++v) dst[(int64_t)r * nv + v] = out[v];
+  }
+}
+
+// The mask of points p0 .. p0+3 (one 16-byte load when `vec`).
+__device__ __forceinline__ void load_mask4(const int32_t* __restrict__ mask, int64_t p0, int64_t n,
+                                           bool vec, int* m) {
+  if (vec && p0 + 4 <= n) {
+    const int4 v = *(const int4*)(mask + p0);
+    m[0] = v.x;
+    m[1] = v.y;
+    m[2] = v.z;
+    m[3] = v.w;
+  } else {
+    for (int q = 0; q < 4; ++q) m[q] = p0 + q < n ? mask[p0 + q] : 0;
+  }
 }
 
 // Coordinates of the used points at their ranks: from `points`, or on a
-// uniform grid origin + spacing * (i, j, k).
+// uniform grid origin + spacing * (i, j, k).  A thread per 4 points.
 __global__ __launch_bounds__(256) void k_compact_points(const float* __restrict__ points,
                                                         int64_t n_points, int nx, int ny,
                                                         float ox, float oy, float oz, float hx,
                                                         float hy, float hz,
-                                                        const int32_t* __restrict__ mask,
+                                                        const int32_t* __restrict__ mask, bool vec,
                                                         const int32_t* __restrict__ rank,
                                                         float* __restrict__ out) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_points || !mask[p]) return;
-  const int64_t u = rank[p];
-  float x, y, z;
-  if (points) {
-    x = points[3 * p];
-    y = points[3 * p + 1];
-    z = points[3 * p + 2];
-  } else {
-    const int64_t i = p % nx, r = p / nx, j = r % ny, k = r / ny;
-    x = ox + hx * (float)i;
-    y = oy + hy * (float)j;
-    z = oz + hz * (float)k;
+  const int64_t p0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (p0 >= n_points) return;
+  int m[4];
+  load_mask4(mask, p0, n_points, vec, m);
+  for (int q = 0; q < 4; ++q) {
+    if (!m[q]) continue;
+    const int64_t p = p0 + q, u = rank[p];
+    float x, y, z;
+    if (points) {
+      x = points[3 * p];
+      y = points[3 * p + 1];
+      z = points[3 * p + 2];
+    } else {
+      const int64_t i = p % nx, r = p / nx, j = r % ny, k = r / ny;
+      x = ox + hx * (float)i;
+      y = oy + hy * (float)j;
+      z = oz + hz * (float)k;
+    }
+    out[3 * u] = x;
+    out[3 * u + 1] = y;
+    out[3 * u + 2] = z;
   }
-  out[3 * u] = x;
-  out[3 * u + 1] = y;
-  out[3 * u + 2] = z;
 }
 
 // CompactPointField with the ranks of the prepare step: the used points'
-// values, in point order (a StreamCompact by the point mask).
+// values, in point order (a StreamCompact by the point mask).  A thread per
+// 4 points.
 __global__ __launch_bounds__(256) void k_compact_values(const uint32_t* __restrict__ in, int words,
                                                         int64_t n_points,
-                                                        const int32_t* __restrict__ mask,
+                                                        const int32_t* __restrict__ mask, bool vec,
                                                         const int32_t* __restrict__ rank,
                                                         uint32_t* __restrict__ out) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_points || !mask[p]) return;
-  const int64_t u = rank[p];
-  for (int w = 0; w < words; ++w) out[u * words + w] = in[p * words + w];
+  const int64_t p0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (p0 >= n_points) return;
+  int m[4];
+  load_mask4(mask, p0, n_points, vec, m);
+  for (int q = 0; q < 4; ++q) {
+    if (!m[q]) continue;
+    const int64_t p = p0 + q, u = rank[p];
+    for (int w = 0; w < words; ++w) out[u * words + w] = in[p * words + w];
+  }
 }
+
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 struct TopoWs {
   size_t off_rank, bytes;
@@ -315,8 +362,9 @@ int generate(const Geometry& g, const float* d_points, const float* origin, cons
   const int32_t* offs = (const int32_t*)d_ws;
   const int32_t* rank = (const int32_t*)((const char*)d_ws + w.off_rank);
   const dim3 b(256);
-  k_emit<<<dim3((unsigned)ceil_div(g.n_cells, 256)), b, 0, s>>>(
-      g.cells, g.nv, g.n_cells, g.nx, g.ny, d_counts, offs, dedup ? rank : nullptr, d_conn);
+  k_emit<<<dim3((unsigned)ceil_div(g.n_cells, 1024)), b, 0, s>>>(
+      g.cells, g.nv, g.n_cells, g.nx, g.ny, d_counts, aligned16(d_counts), offs,
+      dedup ? rank : nullptr, d_conn);
   DAXHIP_LAUNCH_CHECK("k_emit");
   if (dedup) {
     float o[3] = {0, 0, 0}, h[3] = {1, 1, 1};
@@ -324,9 +372,9 @@ int generate(const Geometry& g, const float* d_points, const float* origin, cons
       if (origin) o[d] = origin[d];
       if (spacing) h[d] = spacing[d];
     }
-    k_compact_points<<<dim3((unsigned)ceil_div(g.n_points, 256)), b, 0, s>>>(
-        d_points, g.n_points, g.nx, g.ny, o[0], o[1], o[2], h[0], h[1], h[2], d_mask, rank,
-        d_coords);
+    k_compact_points<<<dim3((unsigned)ceil_div(g.n_points, 1024)), b, 0, s>>>(
+        d_points, g.n_points, g.nx, g.ny, o[0], o[1], o[2], h[0], h[1], h[2], d_mask,
+        aligned16(d_mask), rank, d_coords);
     DAXHIP_LAUNCH_CHECK("k_compact_points");
   }
   return DAX_HIP_OK;
@@ -463,9 +511,11 @@ int dax_hip_topology_compact_point_field(const void* d_in, size_t value_bytes, i
     return DAX_HIP_BAD_VALUE;
   }
   if (n_points == 0) return DAX_HIP_OK;
-  k_compact_values<<<dim3((unsigned)ceil_div(n_points, 256)), dim3(256), 0, (hipStream_t)stream>>>(
+  k_compact_values<<<dim3((unsigned)ceil_div(n_points, 1024)), dim3(256), 0,
+                     (hipStream_t)stream>>>(
       (const uint32_t*)d_in, (int)(value_bytes / 4), n_points, d_point_mask,
-      (const int32_t*)((const char*)d_workspace + w.off_rank), (uint32_t*)d_out);
+      aligned16(d_point_mask), (const int32_t*)((const char*)d_workspace + w.off_rank),
+      (uint32_t*)d_out);
   DAXHIP_LAUNCH_CHECK("k_compact_values");
   return DAX_HIP_OK;
 }
