@@ -26,6 +26,13 @@ namespace topo {
 constexpr int kTX = 64, kTY = 4;  // uniform-grid blocks: 64 x-cells by 4 rows
 constexpr int kZRun = 16;         // planes one thread marches through
 
+// How prepare laid out the workspace (its first int): per-cell offsets and
+// per-point ranks (any counts, any grid), or bit words (counts 0/1 on a
+// uniform grid: a bit per cell and per point, 64 to a word along x, with
+// per-word exclusive popcount prefixes).  The kernels of both layouts are
+// launched; each returns at once unless the workspace holds its layout.
+constexpr int kModeCells = 0, kModeBits = 1;
+
 __device__ __forceinline__ bool in_range(float v, float lo, float hi) {
   return v >= lo && v <= hi;
 }
@@ -46,6 +53,7 @@ __global__ __launch_bounds__(kTX* kTY) void k_count_uniform(const float* __restr
   const int k1 = min(k0 + kZRun, cz);
   int32_t* out = counts + i + (int64_t)cx * (j + (int64_t)cy * k0);
   const int64_t cplane = (int64_t)cx * cy;
+#pragma unroll 4
   for (int k = k0; k < k1; ++k) {
     p += sz;
     const bool b00 = in_range(p[0], lo, hi), b10 = in_range(p[1], lo, hi);
@@ -167,14 +175,15 @@ __device__ __forceinline__ void voxel_ids(int64_t c, int nx, int ny, int64_t* id
 // first output cell; with `rank` the ids are the used points' ranks.  A
 // thread per 4 consecutive cells (one 16-byte load of their counts when
 // `vec`: the counts are 16-byte aligned).
-__global__ __launch_bounds__(256) void k_emit(const int32_t* __restrict__ cells, int nv,
+__global__ __launch_bounds__(256) void k_emit(const int32_t* __restrict__ mode, int nv,
+                                              const int32_t* __restrict__ cells,
                                               int64_t n_cells, int nx, int ny,
                                               const int32_t* __restrict__ counts, bool vec,
                                               const int32_t* __restrict__ offs,
                                               const int32_t* __restrict__ rank,
                                               int32_t* __restrict__ conn) {
   const int64_t c0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-  if (c0 >= n_cells) return;
+  if (c0 >= n_cells || *mode != kModeCells) return;
   int cnt[4];
   if (vec && c0 + 4 <= n_cells) {
     const int4 v = *(const int4*)(counts + c0);
@@ -225,9 +234,10 @@ __global__ __launch_bounds__(256) void k_compact_points(const float* __restrict_
                                                         float hy, float hz,
                                                         const int32_t* __restrict__ mask, bool vec,
                                                         const int32_t* __restrict__ rank,
+                                                        const int32_t* __restrict__ mode,
                                                         float* __restrict__ out) {
   const int64_t p0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-  if (p0 >= n_points) return;
+  if (p0 >= n_points || *mode != kModeCells) return;
   int m[4];
   load_mask4(mask, p0, n_points, vec, m);
   for (int q = 0; q < 4; ++q) {
@@ -257,9 +267,10 @@ __global__ __launch_bounds__(256) void k_compact_values(const uint32_t* __restri
                                                         int64_t n_points,
                                                         const int32_t* __restrict__ mask, bool vec,
                                                         const int32_t* __restrict__ rank,
+                                                        const int32_t* __restrict__ mode,
                                                         uint32_t* __restrict__ out) {
   const int64_t p0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-  if (p0 >= n_points) return;
+  if (p0 >= n_points || *mode != kModeCells) return;
   int m[4];
   load_mask4(mask, p0, n_points, vec, m);
   for (int q = 0; q < 4; ++q) {
@@ -271,15 +282,218 @@ __global__ __launch_bounds__(256) void k_compact_values(const uint32_t* __restri
 
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
+// ---- bit-word layout (uniform grids, counts 0/1) ----
+// Cell words: cell row (j, k) has wcx = ceil(cx / 64) words, word index
+// (j + cy * k) * wcx + w.  Point words: point row (j, k) has wpx =
+// ceil(nx / 64) words, index (j + ny * k) * wpx + w.
+struct BitGeom {
+  int nx, ny, nz, cx, cy, cz, wcx, wpx;
+  int64_t n_cwords, n_pwords;
+};
+
+inline BitGeom bit_geom(int nx, int ny, int nz) {
+  BitGeom b;
+  b.nx = nx;
+  b.ny = ny;
+  b.nz = nz;
+  b.cx = nx - 1;
+  b.cy = ny - 1;
+  b.cz = nz - 1;
+  b.wcx = (b.cx + 63) / 64;
+  b.wpx = (nx + 63) / 64;
+  b.n_cwords = (int64_t)b.wcx * b.cy * b.cz;
+  b.n_pwords = (int64_t)b.wpx * ny * nz;
+  return b;
+}
+
+// The bit kernels run a wave per grid row (4 waves per block), looping over
+// the row's words; lane l stands for bit l of a word.
+constexpr int kRowWaves = 4;
+
+__device__ __forceinline__ int wave_row() {
+  return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kRowWaves + (threadIdx.x >> 6)));
+}
+
+// Cell row r = j + cy * k: per word the ballot of count != 0 over its 64
+// cells and its popcount; *general = 1 if a count is outside {0, 1}.
+__global__ __launch_bounds__(64 * kRowWaves) void k_cell_bits(BitGeom g,
+                                                              const int32_t* __restrict__ counts,
+                                                              uint64_t* __restrict__ cbits,
+                                                              int32_t* __restrict__ cpop,
+                                                              int32_t* __restrict__ general) {
+  const int r = wave_row(), lane = threadIdx.x & 63;
+  if (r >= g.cy * g.cz) return;  // whole waves
+  const int32_t* row = counts + (int64_t)r * g.cx;
+  bool bad = false;
+  for (int wx = 0; wx < g.wcx; ++wx) {
+    const int x = wx * 64 + lane;
+    const int32_t cnt = x < g.cx ? row[x] : 0;
+    const uint64_t bits = __ballot(cnt != 0);
+    bad = bad || cnt < 0 || cnt > 1;
+    if (lane == 0) {
+      cbits[(int64_t)r * g.wcx + wx] = bits;
+      cpop[(int64_t)r * g.wcx + wx] = __popcll(bits);
+    }
+  }
+  if (bad) *general = 1;
+}
+
+// Point row (j, k): point (i, j, k) is used when a cell of cell rows j-1, j
+// in cell planes k-1, k at x = i-1 or i is kept, i.e. the word is the OR over
+// those four cell rows of c | c << 1 | (the previous word's bit 63).  Writes
+// the words, their popcounts and, if `mask`, the int32 mask of the row.
+__global__ __launch_bounds__(64 * kRowWaves) void k_mask_bits(BitGeom g,
+                                                              const uint64_t* __restrict__ cbits,
+                                                              uint64_t* __restrict__ mbits,
+                                                              int32_t* __restrict__ mpop,
+                                                              int32_t* __restrict__ mask) {
+  const int pr = wave_row(), lane = threadIdx.x & 63;
+  if (pr >= g.ny * g.nz) return;  // whole waves
+  const int j = pr % g.ny, k = pr / g.ny;
+  // the four cell rows (j-1 or j, k-1 or k); a missing one reads row 0 and is
+  // masked out
+  const uint64_t* rows[4];
+  uint64_t keep[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int jc = j - 1 + (q & 1), kc = k - 1 + (q >> 1);
+    const bool ok = kc >= 0 && kc < g.cz && jc >= 0 && jc < g.cy;
+    rows[q] = cbits + (ok ? ((int64_t)jc + (int64_t)g.cy * kc) * g.wcx : 0);
+    keep[q] = ok ? ~0ull : 0ull;
+  }
+  uint64_t carry[4] = {0, 0, 0, 0};  // bit 63 of each cell row's previous word
+  for (int wx = 0; wx < g.wpx; ++wx) {
+    uint64_t m = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint64_t c = (wx < g.wcx ? rows[q][wx] : 0) & keep[q];  // wx <= wcx
+      m |= c | (c << 1) | carry[q];
+      carry[q] = c >> 63;
+    }
+    const int x = wx * 64 + lane;
+    if (mask && x < g.nx) mask[(int64_t)pr * g.nx + x] = (int32_t)((m >> lane) & 1u);
+    if (lane == 0) {
+      mbits[(int64_t)pr * g.wpx + wx] = m;
+      mpop[(int64_t)pr * g.wpx + wx] = __popcll(m);
+    }
+  }
+}
+
+// Rank of point (i, j, k) among the used points.
+__device__ __forceinline__ int32_t bit_rank(const BitGeom& g, const uint64_t* __restrict__ mbits,
+                                            const int32_t* __restrict__ moff, int i, int j, int k) {
+  const int64_t w = ((int64_t)j + (int64_t)g.ny * k) * g.wpx + (i >> 6);
+  const uint64_t below = (1ull << (i & 63)) - 1ull;
+  return moff[w] + __popcll(mbits[w] & below);
+}
+
+// Cell row r = j + cy * k: each kept cell (lane per bit) writes its 8 corner
+// ids (ranks with dedup) at its output cell, the word's first plus the kept
+// cells before it in the word.
+__global__ __launch_bounds__(64 * kRowWaves) void k_emit_bits(BitGeom g,
+                                                              const int32_t* __restrict__ mode,
+                                                              const uint64_t* __restrict__ cbits,
+                                                              const int32_t* __restrict__ coff,
+                                                              const uint64_t* __restrict__ mbits,
+                                                              const int32_t* __restrict__ moff,
+                                                              bool dedup,
+                                                              int32_t* __restrict__ conn) {
+  const int r = wave_row(), lane = threadIdx.x & 63;
+  if (r >= g.cy * g.cz || *mode != kModeBits) return;  // whole waves
+  const int j = r % g.cy, k = r / g.cy;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  static constexpr int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                       {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  for (int wx = 0; wx < g.wcx; ++wx) {
+    const int64_t w = (int64_t)r * g.wcx + wx;
+    const uint64_t bits = cbits[w];
+    if (!((bits >> lane) & 1u)) continue;
+    int32_t* dst = conn + ((int64_t)coff[w] + __popcll(bits & lt)) * 8;
+    const int i = wx * 64 + lane;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const int pi = i + corner[v][0], pj = j + corner[v][1], pk = k + corner[v][2];
+      dst[v] = dedup ? bit_rank(g, mbits, moff, pi, pj, pk)
+                     : (int32_t)(pi + (int64_t)g.nx * (pj + (int64_t)g.ny * pk));
+    }
+  }
+}
+
+// Point row (j, k): the used points' coordinates (origin + spacing * ijk) or
+// values (`words` 4-byte words each) at their ranks, a lane per bit.
+__global__ __launch_bounds__(64 * kRowWaves) void k_compact_bits(
+    BitGeom g, const int32_t* __restrict__ mode, const uint64_t* __restrict__ mbits,
+    const int32_t* __restrict__ moff, float ox, float oy, float oz, float hx, float hy, float hz,
+    float* __restrict__ coords, const uint32_t* __restrict__ in, int words,
+    uint32_t* __restrict__ out) {
+  const int pr = wave_row(), lane = threadIdx.x & 63;
+  if (pr >= g.ny * g.nz || *mode != kModeBits) return;  // whole waves
+  const int j = pr % g.ny, k = pr / g.ny;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (int wx = 0; wx < g.wpx; ++wx) {
+    const int64_t w = (int64_t)pr * g.wpx + wx;
+    const uint64_t m = mbits[w];
+    if (!((m >> lane) & 1u)) continue;
+    const int64_t u = (int64_t)moff[w] + __popcll(m & lt);
+    const int i = wx * 64 + lane;
+    if (coords) {
+      coords[3 * u] = ox + hx * (float)i;
+      coords[3 * u + 1] = oy + hy * (float)j;
+      coords[3 * u + 2] = oz + hz * (float)k;
+    } else {
+      const int64_t p = (int64_t)pr * g.nx + i;
+      for (int q = 0; q < words; ++q) out[u * words + q] = in[p * words + q];
+    }
+  }
+}
+
+// Header of the workspace: [0] layout mode, [1] flag of counts outside
+// {0, 1}, [2..4] the uniform grid's nx, ny, nz, [5] 1 when prepare computed
+// the point ranks (duplicate removal).
+constexpr int kHeaderInts = 6;
+__global__ void k_set_header(int32_t* __restrict__ hdr, int mode, int nx, int ny, int nz,
+                             int ranked) {
+  if (threadIdx.x == 0) {
+    hdr[0] = mode;
+    hdr[1] = 0;
+    hdr[2] = nx;
+    hdr[3] = ny;
+    hdr[4] = nz;
+    hdr[5] = ranked;
+  }
+}
+
+// Workspace: a 256-byte header (the layout's mode, a flag), region A (per-
+// cell offsets, or cell words and their prefixes), region B (per-point
+// ranks, or point words and their prefixes).
+constexpr size_t kHeader = 256;
+
 struct TopoWs {
-  size_t off_rank, bytes;
+  size_t off_a, off_b, bytes;
 };
 
 inline TopoWs topo_ws(int64_t n_cells, int64_t n_points) {
   TopoWs w;
-  w.off_rank = align_up((size_t)(n_cells > 0 ? n_cells : 1) * 4, 256);
-  w.bytes = w.off_rank + align_up((size_t)(n_points > 0 ? n_points : 1) * 4, 256);
+  w.off_a = kHeader;
+  w.off_b = w.off_a + align_up((size_t)(n_cells > 0 ? n_cells : 1) * 4, 256);
+  w.bytes = w.off_b + align_up((size_t)(n_points > 0 ? n_points : 1) * 4, 256);
   return w;
+}
+
+// The bit-word layout inside regions A and B, if it fits there.
+struct BitWs {
+  size_t cbits, coff, mbits, moff;
+  bool fits;
+};
+
+inline BitWs bit_ws(const TopoWs& w, const BitGeom& g) {
+  BitWs b;
+  b.cbits = w.off_a;
+  b.coff = b.cbits + align_up((size_t)g.n_cwords * 8, 256);
+  b.mbits = w.off_b;
+  b.moff = b.mbits + align_up((size_t)g.n_pwords * 8, 256);
+  b.fits = b.coff + (size_t)g.n_cwords * 4 <= w.off_b && b.moff + (size_t)g.n_pwords * 4 <= w.bytes;
+  return b;
 }
 
 struct Geometry {  // a uniform grid (cells == nullptr) or an unstructured one
@@ -306,11 +520,52 @@ int prepare(const Geometry& g, const int32_t* d_counts, unsigned flags, void* d_
   }
   *h_cells = 0;
   *h_points = 0;
-  int32_t* offs = (int32_t*)d_ws;
-  int32_t* rank = (int32_t*)((char*)d_ws + w.off_rank);
+  char* ws = (char*)d_ws;
+  int32_t* hdr = (int32_t*)ws;
+  const dim3 b(256);
+  int rc = DAX_HIP_OK;
+  k_set_header<<<1, 64, 0, s>>>(hdr, kModeCells, g.nx, g.ny, g.nz, dedup ? 1 : 0);
+  DAXHIP_LAUNCH_CHECK("k_set_header");
+  if (!g.cells) {
+    // counts 0/1 on a uniform grid: the bit-word layout
+    const BitGeom bg = bit_geom(g.nx, g.ny, g.nz);
+    const BitWs bw = bit_ws(w, bg);
+    if (bw.fits) {
+      uint64_t* cbits = (uint64_t*)(ws + bw.cbits);
+      int32_t* coff = (int32_t*)(ws + bw.coff);
+      k_cell_bits<<<dim3((unsigned)ceil_div((int64_t)bg.cy * bg.cz, kRowWaves)),
+                    dim3(64 * kRowWaves), 0, s>>>(bg, d_counts, cbits, coff, hdr + 1);
+      DAXHIP_LAUNCH_CHECK("k_cell_bits");
+      int32_t total = 0, general = 0;
+      if ((rc = dax_hip_scan_exclusive_i32(coff, coff, bg.n_cwords, &total, s))) return rc;
+      DAXHIP_CHECK(hipMemcpyAsync(&general, hdr + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      DAXHIP_CHECK(hipStreamSynchronize(s));
+      if (!general) {
+        if ((int64_t)total * 8 > 0x7fffffffLL) {
+          set_error("topology prepare: %lld output cells exceed 2^31 ids", (long long)total);
+          return DAX_HIP_BAD_VALUE;
+        }
+        k_set_header<<<1, 64, 0, s>>>(hdr, kModeBits, g.nx, g.ny, g.nz, dedup ? 1 : 0);
+        DAXHIP_LAUNCH_CHECK("k_set_header");
+        *h_cells = total;
+        if (!dedup) return DAX_HIP_OK;
+        uint64_t* mbits = (uint64_t*)(ws + bw.mbits);
+        int32_t* moff = (int32_t*)(ws + bw.moff);
+        k_mask_bits<<<dim3((unsigned)ceil_div((int64_t)bg.ny * bg.nz, kRowWaves)),
+                      dim3(64 * kRowWaves), 0, s>>>(bg, cbits, mbits, moff, d_mask);
+        DAXHIP_LAUNCH_CHECK("k_mask_bits");
+        int32_t used = 0;
+        if ((rc = dax_hip_scan_exclusive_i32(moff, moff, bg.n_pwords, &used, s))) return rc;
+        *h_points = used;
+        return DAX_HIP_OK;
+      }
+    }
+  }
+  // any counts: per-cell offsets and per-point ranks
+  int32_t* offs = (int32_t*)(ws + w.off_a);
+  int32_t* rank = (int32_t*)(ws + w.off_b);
   int32_t total = 0;
-  int rc = dax_hip_scan_exclusive_i32(d_counts, offs, g.n_cells, &total, s);
-  if (rc) return rc;
+  if ((rc = dax_hip_scan_exclusive_i32(d_counts, offs, g.n_cells, &total, s))) return rc;
   if (total < 0 || (int64_t)total * g.nv > 0x7fffffffLL) {
     set_error("topology prepare: %lld output cells of %d vertices exceed 2^31 ids",
               (long long)total, g.nv);
@@ -318,7 +573,7 @@ int prepare(const Geometry& g, const int32_t* d_counts, unsigned flags, void* d_
   }
   *h_cells = total;
   if (!dedup) return DAX_HIP_OK;
-  const dim3 b(256), gp((unsigned)ceil_div(g.n_points, 256));
+  const dim3 gp((unsigned)ceil_div(g.n_points, 256));
   if (g.cells) {
     TempBuf flag(s);
     if ((rc = flag.alloc(sizeof(int32_t)))) return rc;
@@ -359,23 +614,61 @@ int generate(const Geometry& g, const float* d_points, const float* origin, cons
     set_error("topology generate: bad arguments (workspace of %zu bytes needed)", w.bytes);
     return DAX_HIP_BAD_VALUE;
   }
-  const int32_t* offs = (const int32_t*)d_ws;
-  const int32_t* rank = (const int32_t*)((const char*)d_ws + w.off_rank);
+  const char* ws = (const char*)d_ws;
+  const int32_t* hdr = (const int32_t*)ws;
+  const int32_t* offs = (const int32_t*)(ws + w.off_a);
+  const int32_t* rank = (const int32_t*)(ws + w.off_b);
+  float o[3] = {0, 0, 0}, h[3] = {1, 1, 1};
+  for (int d = 0; d < 3; ++d) {
+    if (origin) o[d] = origin[d];
+    if (spacing) h[d] = spacing[d];
+  }
   const dim3 b(256);
-  k_emit<<<dim3((unsigned)ceil_div(g.n_cells, 1024)), b, 0, s>>>(
-      g.cells, g.nv, g.n_cells, g.nx, g.ny, d_counts, aligned16(d_counts), offs,
-      dedup ? rank : nullptr, d_conn);
-  DAXHIP_LAUNCH_CHECK("k_emit");
-  if (dedup) {
-    float o[3] = {0, 0, 0}, h[3] = {1, 1, 1};
-    for (int d = 0; d < 3; ++d) {
-      if (origin) o[d] = origin[d];
-      if (spacing) h[d] = spacing[d];
+  // the layout prepare chose (the kernels also check it) and whether it
+  // ranked the points
+  int32_t head[kHeaderInts];
+  DAXHIP_CHECK(hipMemcpyAsync(head, hdr, sizeof(head), hipMemcpyDeviceToHost, s));
+  DAXHIP_CHECK(hipStreamSynchronize(s));
+  const int32_t mode = head[0];
+  if (dedup && !head[5]) {
+    set_error("topology generate: duplicate removal after a prepare without it");
+    return DAX_HIP_BAD_VALUE;
+  }
+  if (mode == kModeCells) {
+    k_emit<<<dim3((unsigned)ceil_div(g.n_cells, 1024)), b, 0, s>>>(
+        hdr, g.nv, g.cells, g.n_cells, g.nx, g.ny, d_counts, aligned16(d_counts), offs,
+        dedup ? rank : nullptr, d_conn);
+    DAXHIP_LAUNCH_CHECK("k_emit");
+    if (dedup) {
+      k_compact_points<<<dim3((unsigned)ceil_div(g.n_points, 1024)), b, 0, s>>>(
+          d_points, g.n_points, g.nx, g.ny, o[0], o[1], o[2], h[0], h[1], h[2], d_mask,
+          aligned16(d_mask), rank, hdr, d_coords);
+      DAXHIP_LAUNCH_CHECK("k_compact_points");
     }
-    k_compact_points<<<dim3((unsigned)ceil_div(g.n_points, 1024)), b, 0, s>>>(
-        d_points, g.n_points, g.nx, g.ny, o[0], o[1], o[2], h[0], h[1], h[2], d_mask,
-        aligned16(d_mask), rank, d_coords);
-    DAXHIP_LAUNCH_CHECK("k_compact_points");
+  } else if (!g.cells) {
+    const BitGeom bg = bit_geom(g.nx, g.ny, g.nz);
+    const BitWs bw = bit_ws(w, bg);
+    if (!bw.fits) {
+      set_error("topology generate: workspace of another grid");
+      return DAX_HIP_BAD_VALUE;
+    }
+    {
+      const uint64_t* cbits = (const uint64_t*)(ws + bw.cbits);
+      const int32_t* coff = (const int32_t*)(ws + bw.coff);
+      const uint64_t* mbits = (const uint64_t*)(ws + bw.mbits);
+      const int32_t* moff = (const int32_t*)(ws + bw.moff);
+      k_emit_bits<<<dim3((unsigned)ceil_div((int64_t)bg.cy * bg.cz, kRowWaves)),
+                    dim3(64 * kRowWaves), 0, s>>>(bg, hdr, cbits, coff, mbits, moff, dedup,
+                                                  d_conn);
+      DAXHIP_LAUNCH_CHECK("k_emit_bits");
+      if (dedup) {
+        k_compact_bits<<<dim3((unsigned)ceil_div((int64_t)bg.ny * bg.nz, kRowWaves)),
+                         dim3(64 * kRowWaves), 0, s>>>(bg, hdr, mbits, moff, o[0], o[1], o[2],
+                                                       h[0], h[1], h[2], d_coords, nullptr, 0,
+                                                       nullptr);
+        DAXHIP_LAUNCH_CHECK("k_compact_bits");
+      }
+    }
   }
   return DAX_HIP_OK;
 }
@@ -511,11 +804,33 @@ int dax_hip_topology_compact_point_field(const void* d_in, size_t value_bytes, i
     return DAX_HIP_BAD_VALUE;
   }
   if (n_points == 0) return DAX_HIP_OK;
-  k_compact_values<<<dim3((unsigned)ceil_div(n_points, 1024)), dim3(256), 0,
-                     (hipStream_t)stream>>>(
-      (const uint32_t*)d_in, (int)(value_bytes / 4), n_points, d_point_mask,
-      aligned16(d_point_mask), (const int32_t*)((const char*)d_workspace + w.off_rank),
-      (uint32_t*)d_out);
+  hipStream_t s = (hipStream_t)stream;
+  const char* ws = (const char*)d_workspace;
+  int32_t hdr[kHeaderInts];
+  DAXHIP_CHECK(hipMemcpyAsync(hdr, ws, sizeof(hdr), hipMemcpyDeviceToHost, s));
+  DAXHIP_CHECK(hipStreamSynchronize(s));
+  if (!hdr[5]) {
+    set_error("dax_hip_topology_compact_point_field: the prepare step did not remove duplicates");
+    return DAX_HIP_BAD_VALUE;
+  }
+  const int words = (int)(value_bytes / 4);
+  if (hdr[0] == kModeBits) {
+    const BitGeom bg = bit_geom(hdr[2], hdr[3], hdr[4]);
+    const BitWs bw = bit_ws(w, bg);
+    if ((int64_t)hdr[2] * hdr[3] * hdr[4] != n_points || !bw.fits) {
+      set_error("dax_hip_topology_compact_point_field: workspace of another grid");
+      return DAX_HIP_BAD_VALUE;
+    }
+    k_compact_bits<<<dim3((unsigned)ceil_div((int64_t)bg.ny * bg.nz, kRowWaves)),
+                     dim3(64 * kRowWaves), 0, s>>>(
+        bg, (const int32_t*)ws, (const uint64_t*)(ws + bw.mbits), (const int32_t*)(ws + bw.moff),
+        0, 0, 0, 0, 0, 0, nullptr, (const uint32_t*)d_in, words, (uint32_t*)d_out);
+    DAXHIP_LAUNCH_CHECK("k_compact_bits");
+    return DAX_HIP_OK;
+  }
+  k_compact_values<<<dim3((unsigned)ceil_div(n_points, 1024)), dim3(256), 0, s>>>(
+      (const uint32_t*)d_in, words, n_points, d_point_mask, aligned16(d_point_mask),
+      (const int32_t*)(ws + w.off_b), (const int32_t*)ws, (uint32_t*)d_out);
   DAXHIP_LAUNCH_CHECK("k_compact_values");
   return DAX_HIP_OK;
 }

@@ -137,3 +137,35 @@ def test_bad_ids_are_rejected():
     tets = torch.tensor([[0, 1, 2, 7]], dtype=torch.int32, device="cuda")
     with pytest.raises(_lib.ErrorControlBadValue):
         topology.threshold(f, -1.0, 1.0, cells=tets, points=pts)
+
+
+def test_dedup_generate_after_plain_prepare_is_rejected():
+    """generate / CompactPointField with duplicate removal need a prepare that
+    ranked the points: otherwise an error, not a read of unset ranks."""
+    import ctypes
+    dims = (40, 30, 20)
+    rng = np.random.default_rng(6)
+    f = torch.from_numpy(rng.random(dims[::-1]).astype(np.float32)).cuda()
+    counts = topology.threshold_count(f, 0.1, 0.9)
+    C, P = counts.numel(), f.numel()
+    lib = _lib.load()
+    ws = torch.empty(int(lib.dax_hip_topology_workspace_bytes(C, P)), dtype=torch.uint8,
+                     device="cuda")
+    mask = torch.empty(P, dtype=torch.int32, device="cuda")
+    t, u = ctypes.c_int64(), ctypes.c_int64()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    print("[case] topology flags mismatch", flush=True)
+    _lib.call("dax_hip_topology_prepare_uniform", *dims, counts.data_ptr(), 0, ws.data_ptr(),
+              ws.numel(), None, ctypes.byref(t), ctypes.byref(u), s)
+    conn = torch.empty((max(t.value, 1), 8), dtype=torch.int32, device="cuda")
+    coords = torch.empty((P, 3), dtype=torch.float32, device="cuda")
+    o3 = (ctypes.c_float * 3)(0, 0, 0)
+    h3 = (ctypes.c_float * 3)(1, 1, 1)
+    with pytest.raises(_lib.ErrorControlBadValue):
+        _lib.call("dax_hip_topology_generate_uniform", *dims, o3, h3, counts.data_ptr(), 1,
+                  ws.data_ptr(), ws.numel(), mask.data_ptr(), conn.data_ptr(), coords.data_ptr(),
+                  s)
+    out = torch.empty(P, dtype=torch.float32, device="cuda")
+    with pytest.raises(_lib.ErrorControlBadValue):
+        _lib.call("dax_hip_topology_compact_point_field", f.data_ptr(), 4, C, P, mask.data_ptr(),
+                  ws.data_ptr(), ws.numel(), out.data_ptr(), s)
