@@ -325,14 +325,22 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_cell_bits(BitGeom g,
   if (r >= g.cy * g.cz) return;  // whole waves
   const int32_t* row = counts + (int64_t)r * g.cx;
   bool bad = false;
-  for (int wx = 0; wx < g.wcx; ++wx) {
-    const int x = wx * 64 + lane;
-    const int32_t cnt = x < g.cx ? row[x] : 0;
-    const uint64_t bits = __ballot(cnt != 0);
-    bad = bad || cnt < 0 || cnt > 1;
-    if (lane == 0) {
-      cbits[(int64_t)r * g.wcx + wx] = bits;
-      cpop[(int64_t)r * g.wcx + wx] = __popcll(bits);
+  for (int w0 = 0; w0 < g.wcx; w0 += 4) {  // four words' loads in flight
+    int32_t cnt[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int x = (w0 + q) * 64 + lane;
+      cnt[q] = x < g.cx ? row[x] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (w0 + q >= g.wcx) break;
+      const uint64_t bits = __ballot(cnt[q] != 0);
+      bad = bad || cnt[q] < 0 || cnt[q] > 1;
+      if (lane == 0) {
+        cbits[(int64_t)r * g.wcx + w0 + q] = bits;
+        cpop[(int64_t)r * g.wcx + w0 + q] = __popcll(bits);
+      }
     }
   }
   if (bad) *general = 1;
@@ -362,19 +370,47 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_mask_bits(BitGeom g,
     keep[q] = ok ? ~0ull : 0ull;
   }
   uint64_t carry[4] = {0, 0, 0, 0};  // bit 63 of each cell row's previous word
-  for (int wx = 0; wx < g.wpx; ++wx) {
-    uint64_t m = 0;
+  // with 16-byte aligned rows the mask goes out 4 words (256 points) at a
+  // time, lane l writing points 4l .. 4l+3 as one 16-byte store
+  const bool vec = mask && (g.nx & 3) == 0 && ((uintptr_t)mask & 15u) == 0;
+  int32_t* mrow = mask ? mask + (int64_t)pr * g.nx : nullptr;
+  for (int w0 = 0; w0 < g.wpx; w0 += 4) {
+    uint64_t m4[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint64_t c = (wx < g.wcx ? rows[q][wx] : 0) & keep[q];  // wx <= wcx
-      m |= c | (c << 1) | carry[q];
-      carry[q] = c >> 63;
+    for (int t = 0; t < 4; ++t) {
+      const int wx = w0 + t;
+      uint64_t m = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint64_t c = (wx < g.wcx ? rows[q][wx] : 0) & keep[q];  // wx <= wcx
+        m |= c | (c << 1) | carry[q];
+        carry[q] = c >> 63;
+      }
+      m4[t] = wx < g.wpx ? m : 0;
+      if (lane == 0 && wx < g.wpx) {
+        mbits[(int64_t)pr * g.wpx + wx] = m;
+        mpop[(int64_t)pr * g.wpx + wx] = __popcll(m);
+      }
     }
-    const int x = wx * 64 + lane;
-    if (mask && x < g.nx) mask[(int64_t)pr * g.nx + x] = (int32_t)((m >> lane) & 1u);
-    if (lane == 0) {
-      mbits[(int64_t)pr * g.wpx + wx] = m;
-      mpop[(int64_t)pr * g.wpx + wx] = __popcll(m);
+    if (!mask) continue;
+    if (vec) {
+      const int x = w0 * 64 + 4 * lane;  // points x .. x+3, all in word w0 + lane / 16
+      if (x < g.nx) {
+        const uint64_t m = m4[lane >> 4];
+        const int sh = x & 63;
+        int4 v;
+        v.x = (int)((m >> sh) & 1u);
+        v.y = (int)((m >> (sh + 1)) & 1u);
+        v.z = (int)((m >> (sh + 2)) & 1u);
+        v.w = (int)((m >> (sh + 3)) & 1u);
+        *(int4*)(mrow + x) = v;
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int x = (w0 + t) * 64 + lane;
+        if (x < g.nx) mrow[x] = (int32_t)((m4[t] >> lane) & 1u);
+      }
     }
   }
 }
@@ -404,9 +440,17 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_emit_bits(BitGeom g,
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   static constexpr int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
                                        {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
-  for (int wx = 0; wx < g.wcx; ++wx) {
+  const uint64_t* crow = cbits + (int64_t)r * g.wcx;
+  for (int w0 = 0; w0 < g.wcx; w0 += 4) {
+    uint64_t b4[4];  // four words' loads in flight
+#pragma unroll
+    for (int t = 0; t < 4; ++t) b4[t] = w0 + t < g.wcx ? crow[w0 + t] : 0;
+    if (!(b4[0] | b4[1] | b4[2] | b4[3])) continue;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+    const int wx = w0 + t;
     const int64_t w = (int64_t)r * g.wcx + wx;
-    const uint64_t bits = cbits[w];
+    const uint64_t bits = b4[t];
     if (!((bits >> lane) & 1u)) continue;
     int32_t* dst = conn + ((int64_t)coff[w] + __popcll(bits & lt)) * 8;
     const int i = wx * 64 + lane;
@@ -415,6 +459,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_emit_bits(BitGeom g,
       const int pi = i + corner[v][0], pj = j + corner[v][1], pk = k + corner[v][2];
       dst[v] = dedup ? bit_rank(g, mbits, moff, pi, pj, pk)
                      : (int32_t)(pi + (int64_t)g.nx * (pj + (int64_t)g.ny * pk));
+    }
     }
   }
 }
@@ -430,9 +475,17 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_compact_bits(
   if (pr >= g.ny * g.nz || *mode != kModeBits) return;  // whole waves
   const int j = pr % g.ny, k = pr / g.ny;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  for (int wx = 0; wx < g.wpx; ++wx) {
+  const uint64_t* mrow = mbits + (int64_t)pr * g.wpx;
+  for (int w0 = 0; w0 < g.wpx; w0 += 4) {
+    uint64_t m4[4];  // four words' loads in flight
+#pragma unroll
+    for (int t = 0; t < 4; ++t) m4[t] = w0 + t < g.wpx ? mrow[w0 + t] : 0;
+    if (!(m4[0] | m4[1] | m4[2] | m4[3])) continue;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+    const int wx = w0 + t;
     const int64_t w = (int64_t)pr * g.wpx + wx;
-    const uint64_t m = mbits[w];
+    const uint64_t m = m4[t];
     if (!((m >> lane) & 1u)) continue;
     const int64_t u = (int64_t)moff[w] + __popcll(m & lt);
     const int i = wx * 64 + lane;
@@ -443,6 +496,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_compact_bits(
     } else {
       const int64_t p = (int64_t)pr * g.nx + i;
       for (int q = 0; q < words; ++q) out[u * words + q] = in[p * words + q];
+    }
     }
   }
 }
