@@ -67,6 +67,46 @@ __global__ __launch_bounds__(kTX* kTY) void k_count_uniform(const float* __restr
   }
 }
 
+// The same with rows of 16-byte aligned float4s (nx % 4 == 0): lane l of a
+// wave covers cells x0 .. x0+3 (x0 = 4 * (64 * blockIdx.x + l)) from one
+// float4 per point row and the next lane's first value (lane 63 loads it).
+__device__ __forceinline__ unsigned range_bits5(const float* __restrict__ row, int x0, int nx,
+                                                float lo, float hi) {
+  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (x0 < nx) v = *(const float4*)(row + x0);
+  float nxt = __shfl_down(v.x, 1, 64);
+  if ((threadIdx.x & 63) == 63) nxt = x0 + 4 < nx ? row[x0 + 4] : 0.f;
+  return (unsigned)in_range(v.x, lo, hi) | (unsigned)in_range(v.y, lo, hi) << 1 |
+         (unsigned)in_range(v.z, lo, hi) << 2 | (unsigned)in_range(v.w, lo, hi) << 3 |
+         (unsigned)in_range(nxt, lo, hi) << 4;
+}
+
+__global__ __launch_bounds__(kTX* kTY) void k_count_uniform4(const float* __restrict__ f, int nx,
+                                                              int ny, int nz, float lo, float hi,
+                                                              int32_t* __restrict__ counts) {
+  const int cx = nx - 1, cy = ny - 1, cz = nz - 1;
+  const int x0 = 4 * (blockIdx.x * kTX + threadIdx.x), j = blockIdx.y * kTY + threadIdx.y;
+  const int k0 = blockIdx.z * kZRun;
+  if (j >= cy || k0 >= cz) return;  // whole waves (one row per wave)
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  const float* p = f + sy * j + sz * k0;
+  // bit q: point x0+q in range in both rows j and j+1 of the plane
+  unsigned a = range_bits5(p, x0, nx, lo, hi) & range_bits5(p + sy, x0, nx, lo, hi);
+  const int k1 = min(k0 + kZRun, cz);
+  int32_t* out = counts + x0 + (int64_t)cx * (j + (int64_t)cy * k0);
+  const int64_t cplane = (int64_t)cx * cy;
+  for (int k = k0; k < k1; ++k) {
+    p += sz;
+    const unsigned b = range_bits5(p, x0, nx, lo, hi) & range_bits5(p + sy, x0, nx, lo, hi);
+    const unsigned both = a & b, kept = both & (both >> 1);  // cell q: points q and q+1
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (x0 + q < cx) out[q] = (int32_t)((kept >> q) & 1u);
+    out += cplane;
+    a = b;
+  }
+}
+
 // Unstructured grid: a thread per cell of nv vertices (nv <= 8); an id
 // outside [0, n_points) sets *bad and counts 0.
 __global__ __launch_bounds__(256) void k_count_cells(const int32_t* __restrict__ cells, int nv,
@@ -744,6 +784,14 @@ int dax_hip_threshold_count_uniform(const float* d_field, int64_t nx, int64_t ny
     return DAX_HIP_BAD_VALUE;
   }
   const int cx = (int)nx - 1, cy = (int)ny - 1, cz = (int)nz - 1;
+  if ((nx & 3) == 0 && ((uintptr_t)d_field & 15u) == 0) {
+    k_count_uniform4<<<dim3((unsigned)ceil_div(nx, 4 * kTX), (unsigned)ceil_div(cy, kTY),
+                            (unsigned)ceil_div(cz, kZRun)),
+                       dim3(kTX, kTY), 0, (hipStream_t)stream>>>(
+        d_field, (int)nx, (int)ny, (int)nz, min_value, max_value, d_counts);
+    DAXHIP_LAUNCH_CHECK("k_count_uniform4");
+    return DAX_HIP_OK;
+  }
   k_count_uniform<<<dim3((unsigned)ceil_div(cx, kTX), (unsigned)ceil_div(cy, kTY),
                          (unsigned)ceil_div(cz, kZRun)),
                     dim3(kTX, kTY), 0, (hipStream_t)stream>>>(d_field, (int)nx, (int)ny, (int)nz,

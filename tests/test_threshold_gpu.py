@@ -42,7 +42,8 @@ def test_reference_unit_test_26():
 
 
 @pytest.mark.parametrize("dedup", [True, False])
-@pytest.mark.parametrize("dims", [(9, 7, 6), (70, 33, 20), (130, 5, 40), (2, 2, 2)])
+@pytest.mark.parametrize("dims", [(9, 7, 6), (70, 33, 20), (130, 5, 40), (2, 2, 2), (68, 9, 7),
+                                  (64, 3, 2), (260, 6, 19)])
 def test_uniform_matches_oracle(dims, dedup):
     rng = np.random.default_rng(sum(dims))
     nx, ny, nz = dims
