@@ -9,13 +9,16 @@
 // NUM_VERTICES ids per cell, and the output cells are the passing input
 // cells in order with their vertices' coordinates.  Also: the closed-form
 // cell count, and without duplicate removal the input's point ids.
+#include <fstream>
 #include <iostream>
+#include <sstream>
 #include <vector>
 
 #include <dax/cont/DispatcherGenerateTopology.h>
 #include <dax/cont/DispatcherMapCell.h>
 #include <dax/cont/UniformGrid.h>
 #include <dax/cont/UnstructuredGrid.h>
+#include <dax/cont/VTKWriter.h>
 
 #include "Testing.h"
 
@@ -128,6 +131,26 @@ void TestThreshold() {
     DAX_TEST_ASSERT(topo.CompactPointField(fieldHandle, result), "CompactPointField failed");
     DAX_TEST_ASSERT(outGrid.GetNumberOfCells() == closedForm, "wrong number of cells");
     CheckOutput(MakeCells(8), 8, points, field, outGrid, result);
+    // the output as a VTK legacy file (the reference threshold benchmark's
+    // PrintContentsToStream format: UNSTRUCTURED_GRID, CELLS, CELL_TYPES 12)
+    const std::string path = "/tmp/UnitTestWorkletThresholdHip.vtk";
+    dax::cont::WriteVTK(outGrid, path);
+    std::ifstream in(path);
+    std::stringstream text;
+    text << in.rdbuf();
+    std::ostringstream head, cells, types;
+    head << "# vtk DataFile Version 3.0\nvtk output\nASCII\nDATASET UNSTRUCTURED_GRID\nPOINTS "
+         << outGrid.GetNumberOfPoints() << " float\n";
+    cells << "CELLS " << closedForm << " " << 9 * closedForm << "\n";
+    types << "\nCELL_TYPES " << closedForm << "\n";
+    DAX_TEST_ASSERT(text.str().rfind(head.str(), 0) == 0, "VTK header");
+    DAX_TEST_ASSERT(text.str().find(cells.str()) != std::string::npos, "VTK cells");
+    const size_t at = text.str().find(types.str());
+    DAX_TEST_ASSERT(at != std::string::npos, "VTK cell types");
+    std::string rest = text.str().substr(at + types.str().size());
+    std::string expect;
+    for (dax::Id c = 0; c < closedForm; ++c) expect += "12\n";
+    DAX_TEST_ASSERT(rest == expect, "VTK cell type list");
   }
   for (int nv : {8, 4}) {
     std::cout << (nv == 8 ? "Hexahedra" : "Tetrahedra") << std::endl;
