@@ -114,17 +114,29 @@ __device__ __forceinline__ void lerp3(const float* __restrict__ pts, int64_t a, 
 // Corners of every triangle: with keys, the edge key A * P + B of each
 // corner (dedup); without, the corner's point record, coordinates and
 // identity connectivity.
+//
+// A thread per output triangle (the reference's UpperBounds and visit index,
+// DispatcherGenerateInterpolatedCells.h:196-212): its cell is the last one
+// whose exclusive offset is <= the triangle, its visit index the difference.
 template <int NV, bool kVec>
 __global__ __launch_bounds__(256) void k_generate(
     const int32_t* __restrict__ cells, int64_t n_cells, const float* __restrict__ field,
-    const float* __restrict__ pts, int64_t n_points, float iso, const int32_t* __restrict__ counts,
+    const float* __restrict__ pts, int64_t n_points, float iso, int64_t n_tris,
     const int32_t* __restrict__ offs, uint64_t* __restrict__ keys, int32_t* __restrict__ first,
     int32_t* __restrict__ second, float* __restrict__ weight, float* __restrict__ verts,
     int32_t* __restrict__ conn) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_cells) return;
-  const int nf = counts[t];
-  if (nf == 0) return;
+  const int64_t tri = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tri >= n_tris) return;
+  int64_t lo = 0, hi = n_cells;  // offs[0] = 0 <= tri
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)offs[mid] <= tri)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int64_t t = lo;
+  const int visit = (int)(tri - offs[t]);
   int id[NV];
   load_ids<NV, kVec>(cells, t, id);
   float f[NV];
@@ -134,8 +146,9 @@ __global__ __launch_bounds__(256) void k_generate(
     f[v] = field[id[v]];
     c |= (f[v] > iso) << v;
   }
+  if (visit >= Shape<NV>::faces(c)) return;  // offsets not from this field's counts
   const int64_t base = 3 * (int64_t)offs[t];
-  for (int k = 0; k < 3 * nf; ++k) {
+  for (int k = 3 * visit; k < 3 * visit + 3; ++k) {
     const int e = Shape<NV>::edge(c, k);
     int va = Shape<NV>::end0(e), vb = Shape<NV>::end1(e);
     int ia = 0, ib = 0;
@@ -266,12 +279,12 @@ int generate_cells(const char* name, const int32_t* d_cells, int64_t n_cells, co
   *h_points = 0;
   if (n_tris == 0) return DAX_HIP_OK;
   hipStream_t s = (hipStream_t)stream;
-  const dim3 gt((unsigned)ceil_div(n_cells, 256)), b(256);
+  const dim3 gt((unsigned)ceil_div(n_tris, 256)), b(256);
   const bool vec = aligned16(d_cells);
   uint64_t* keys = dedup ? (uint64_t*)d_scratch : nullptr;
 #define DAXHIP_UGEN(VEC)                                                                        \
   k_generate<NV, VEC><<<gt, b, 0, s>>>(d_cells, n_cells, d_field, d_points, n_points, iso,      \
-                                       d_counts, d_offsets, keys, d_first, d_second, d_weight,  \
+                                       n_tris, d_offsets, keys, d_first, d_second, d_weight,    \
                                        d_vertices, d_conn)
   if (vec)
     DAXHIP_UGEN(true);
