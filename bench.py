@@ -429,6 +429,104 @@ def _cells_alg_bytes(C, nv, T, U):
     return C * (4 * nv + 4 + 8) + T * (4 * nv + 4) + 3 * T * (8 + 16) + U * 24
 
 
+def threshold_512(args):
+    """The reference's Threshold benchmark pipeline (Benchmarks/Threshold/
+    Pipeline.h:163-190, main.cxx: origin 0, spacing 1) at 512^3: the field
+    is the point magnitude |p| (made beforehand, as the reference does with
+    DispatcherMapField<Magnitude> before its timer starts); timed:
+    ThresholdCount [0, 100], DispatcherGenerateTopology with duplicate
+    removal, and CompactPointField of the field.  Synchronous like the
+    reference (the cell and point counts come back to the host)."""
+    import torch
+
+    from daxtoolkit_amd import topology
+    n = 512
+    ax = torch.arange(n, device="cuda", dtype=torch.float32)
+    z, y, x = torch.meshgrid(ax, ax, ax, indexing="ij")
+    f = torch.sqrt(x * x + y * y + z * z).contiguous()
+    del x, y, z
+    lo, hi = 0.0, 100.0
+
+    def run():
+        out = topology.threshold(f, lo, hi)
+        return out, out.compact_point_field(f.reshape(-1))
+
+    out, vals = run()
+    T, U = out.num_cells, out.num_points
+    reps = max(3, min(args.steps, 10))
+    ms = _timed(run, reps)
+    P, C = n ** 3, (n - 1) ** 3
+    # the bytes the pipeline cannot avoid: the field read, the counts and the
+    # point mask written, the field read again by CompactPointField, and the
+    # outputs (8 ids per cell, 3 coordinates and 1 value per point)
+    alg = 4 * P + 4 * C + 4 * P + 4 * P + 32 * T + 16 * U
+    res = {"workload": f"threshold of the {n}^3 grid (origin 0, spacing 1), field |p|, range "
+                       f"[0, 100], duplicate removal, CompactPointField of the field",
+           "cells": C, "out_cells": T, "out_points": U, "ms": round(ms, 4),
+           "Gcells_per_s": round(C / (ms * 1e-3) / 1e9, 3), "alg_bytes": alg,
+           "GBps": round(alg / (ms * 1e-3) / 1e9, 1)}
+    res["frac_hbm_peak"] = round(res["GBps"] / HBM_PEAK_GBS, 4)
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        f_h = f.cpu().numpy()
+        t0 = time.perf_counter()
+        ref = orc.threshold(f_h, lo, hi)
+        sec = time.perf_counter() - t0
+        assert (ref["n_cells"], ref["n_points"]) == (T, U)
+        res["cpu_baseline"] = {"value": round(C / sec / 1e9, 4), "unit": "Gcells/s", "cores": 1,
+                               "kind": "port", "sample": f"the full input, serial restatement "
+                                                         f"(oracle), one run, {sec:.2f} s"}
+    del f, out, vals
+    torch.cuda.empty_cache()
+    _progress(f"threshold {n}^3: {ms:.3f} ms, T={T}, U={U}")
+    return res
+
+
+def vtk_write_512(args):
+    """The 512^3 benchmark contour (dedup) written as a VTK legacy POLYDATA
+    file (the reference benchmark's PrintContentsToStream format) by the
+    library's multi-threaded host writer, wall-clock from host arrays to a
+    closed file under /tmp; beside it the oracle's ofstream restatement of
+    the reference's writer on the same arrays (the files are compared)."""
+    import tempfile
+
+    import torch
+
+    from daxtoolkit_amd import fields, io, mc
+    n = 512
+    dims = (n, n, n)
+    field = fields.gaussian_mixture_torch(dims, fields.gaussian_mixture_params(0, dims))
+    out = mc.forward(field, 0.5, normals=False)
+    pts = out.vertices.cpu().numpy()
+    conn = out.connectivity.reshape(-1, 3).cpu().numpy()
+    del field, out
+    torch.cuda.empty_cache()
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        a, b = f"{d}/lib.vtk", f"{d}/ref.vtk"
+        t0 = time.perf_counter()
+        io.write_vtk_polydata(a, pts, conn)
+        sec = time.perf_counter() - t0
+        size = os.path.getsize(a)
+        res = {"workload": f"VTK legacy POLYDATA file of the {n}^3 benchmark contour "
+                           f"({conn.shape[0]} triangles, {pts.shape[0]} points)",
+               "bytes": size, "s": round(sec, 4), "MB_per_s": round(size / sec / 1e6, 1),
+               "threads": os.cpu_count()}
+        if not args.no_cpu_baseline:
+            from oracle import oracle as orc
+            t0 = time.perf_counter()
+            orc.write_vtk(b, pts, conn, polydata=True)
+            ref_sec = time.perf_counter() - t0
+            with open(a, "rb") as fa, open(b, "rb") as fb:
+                same = fa.read() == fb.read()
+            res["identical_to_reference_stream"] = same
+            res["cpu_baseline"] = {"value": round(size / ref_sec / 1e6, 1), "unit": "MB/s",
+                                   "cores": 1, "kind": "port",
+                                   "sample": f"the same arrays through an ofstream as the "
+                                             f"reference writes them, {ref_sec:.2f} s"}
+    _progress(f"vtk write: {sec:.3f} s, {size / 1e6:.1f} MB")
+    return res
+
+
 def hexahedra_256(args):
     """Marching cubes on the unstructured hexahedral grid of the 256^3
     uniform grid's voxels (MarchingCubesCount / MarchingCubesGenerate on

@@ -2,7 +2,11 @@
 // backend), synchronisation and the event timer.
 #include <stdarg.h>
 
+#include <algorithm>
+#include <map>
 #include <mutex>
+#include <utility>
+#include <vector>
 
 #include "common.h"
 
@@ -30,6 +34,50 @@ namespace {
 std::mutex g_pool_mu;
 constexpr int kMaxDevices = 64;
 hipMemPool_t g_pool[kMaxDevices] = {};
+
+// A grow-only block per (device, stream) handed out as a stack: the
+// temporaries of one call nest (their holders are scoped), so they are
+// released in reverse order.  Reuse needs no synchronisation: every user of
+// a block runs on its stream, in order.  Allocations that do not fit come
+// from the pool, and the block grows to the high-water mark the next time it
+// is idle.  Blocks above kArenaMax are not kept.
+constexpr size_t kArenaMax = (size_t)256 << 20;
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, top = 0, want = 0;
+  struct Live {
+    size_t off, size;
+    bool freed;
+  };
+  std::vector<Live> live;
+};
+std::map<std::pair<int, uintptr_t>, Arena> g_arenas;  // guarded by g_pool_mu
+
+hipMemPool_t pool_of(int dev, int* rc) {  // g_pool_mu held
+  *rc = DAX_HIP_OK;
+  if (!g_pool[dev]) {
+    hipMemPoolProps props = {};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    hipMemPool_t np = nullptr;
+    hipError_t e = hipMemPoolCreate(&np, &props);
+    if (e != hipSuccess) {
+      *rc = check_hip(e, "hipMemPoolCreate");
+      return nullptr;
+    }
+    uint64_t keep = UINT64_MAX;  // never return memory to the device at synchronisation
+    e = hipMemPoolSetAttribute(np, hipMemPoolAttrReleaseThreshold, &keep);
+    if (e != hipSuccess) {
+      (void)hipMemPoolDestroy(np);
+      *rc = check_hip(e, "hipMemPoolSetAttribute");
+      return nullptr;
+    }
+    g_pool[dev] = np;
+  }
+  return g_pool[dev];
+}
 }  // namespace
 
 int temp_alloc(void** p, size_t bytes, hipStream_t s) {
@@ -40,28 +88,37 @@ int temp_alloc(void** p, size_t bytes, hipStream_t s) {
     set_error("temp_alloc: device %d beyond the pool table", dev);
     return DAX_HIP_BAD_VALUE;
   }
+  bytes = align_up(bytes ? bytes : 1, 256);
   hipMemPool_t pool;
   {
     std::lock_guard<std::mutex> lock(g_pool_mu);
-    if (!g_pool[dev]) {
-      hipMemPoolProps props = {};
-      props.allocType = hipMemAllocationTypePinned;
-      props.handleTypes = hipMemHandleTypeNone;
-      props.location.type = hipMemLocationTypeDevice;
-      props.location.id = dev;
-      hipMemPool_t np = nullptr;
-      DAXHIP_CHECK(hipMemPoolCreate(&np, &props));
-      uint64_t keep = UINT64_MAX;  // never return memory to the device at synchronisation
-      hipError_t e = hipMemPoolSetAttribute(np, hipMemPoolAttrReleaseThreshold, &keep);
-      if (e != hipSuccess) {
-        (void)hipMemPoolDestroy(np);
-        return check_hip(e, "hipMemPoolSetAttribute");
+    int rc = DAX_HIP_OK;
+    pool = pool_of(dev, &rc);
+    if (rc) return rc;
+    Arena& a = g_arenas[{dev, (uintptr_t)s}];
+    if (a.top == 0 && a.want > a.cap) {  // idle: grow to the high-water mark
+      if (a.base) (void)hipFreeAsync(a.base, s);
+      a.base = nullptr;
+      a.cap = 0;
+      void* nb = nullptr;
+      if (hipMallocFromPoolAsync(&nb, a.want, pool, s) == hipSuccess) {
+        a.base = (char*)nb;
+        a.cap = a.want;
+      } else {
+        (void)hipGetLastError();
       }
-      g_pool[dev] = np;
     }
-    pool = g_pool[dev];
+    if (bytes <= kArenaMax) {
+      if (a.top + bytes <= a.cap) {
+        *p = a.base + a.top;
+        a.live.push_back({a.top, bytes, false});
+        a.top += bytes;
+        return DAX_HIP_OK;
+      }
+      a.want = std::max(a.want, std::min(kArenaMax, a.top + bytes));
+    }
   }
-  hipError_t e = hipMallocFromPoolAsync(p, bytes ? bytes : 1, pool, s);
+  hipError_t e = hipMallocFromPoolAsync(p, bytes, pool, s);
   if (e != hipSuccess) {
     *p = nullptr;
     return check_hip(e, "hipMallocFromPoolAsync (library temporary)");
@@ -70,7 +127,28 @@ int temp_alloc(void** p, size_t bytes, hipStream_t s) {
 }
 
 void temp_free(void* p, hipStream_t s) {
-  if (p) (void)hipFreeAsync(p, s);
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    for (auto& kv : g_arenas) {
+      Arena& a = kv.second;
+      if (kv.first.second != (uintptr_t)s || !a.base || (char*)p < a.base ||
+          (char*)p >= a.base + a.cap)
+        continue;
+      const size_t off = (size_t)((char*)p - a.base);
+      for (auto it = a.live.rbegin(); it != a.live.rend(); ++it)
+        if (it->off == off && !it->freed) {
+          it->freed = true;
+          break;
+        }
+      while (!a.live.empty() && a.live.back().freed) {
+        a.top = a.live.back().off;
+        a.live.pop_back();
+      }
+      return;
+    }
+  }
+  (void)hipFreeAsync(p, s);
 }
 
 }  // namespace daxhip
