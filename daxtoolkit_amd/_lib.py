@@ -14,6 +14,7 @@ _lib = None
 OK, OOM, EXEC_ERROR, BAD_VALUE, CAPACITY = 0, 1, 2, 3, 4
 MC_DEDUP = 1
 MC_NORMALS = 2
+TOPOLOGY_MASK_OPTIONAL = 4
 
 
 class DaxHipError(RuntimeError):
@@ -115,6 +116,8 @@ _SIGNATURES = {
     "dax_hip_threshold_count_uniform": (C.c_int, [vp, i64, i64, i64, f32, f32, vp, vp]),
     "dax_hip_threshold_count_cells": (C.c_int, [vp, C.c_int, i64, vp, i64, f32, f32, vp, vp]),
     "dax_hip_topology_workspace_bytes": (sz, [i64, i64]),
+    "dax_hip_threshold_prepare_uniform": (C.c_int, [vp, i64, i64, i64, f32, f32, C.c_uint, vp, vp,
+                                                    sz, vp, P_i64, P_i64, vp]),
     "dax_hip_topology_prepare_uniform": (C.c_int, [i64, i64, i64, vp, C.c_uint, vp, sz, vp, P_i64,
                                                    P_i64, vp]),
     "dax_hip_topology_generate_uniform": (C.c_int, [i64, i64, i64, vp, vp, vp, C.c_uint, vp, sz,

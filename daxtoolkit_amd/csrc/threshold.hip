@@ -107,6 +107,58 @@ __global__ __launch_bounds__(kTX* kTY) void k_count_uniform4(const float* __rest
   }
 }
 
+// Bits 0..15 of x moved to bits 0, 4, 8, ..., 60.
+__device__ __forceinline__ uint64_t spread16(uint64_t x) {
+  x &= 0xFFFFull;
+  x = (x | (x << 24)) & 0x000000FF000000FFull;
+  x = (x | (x << 12)) & 0x000F000F000F000Full;
+  x = (x | (x << 6)) & 0x0303030303030303ull;
+  x = (x | (x << 3)) & 0x1111111111111111ull;
+  return x;
+}
+
+// k_count_uniform4 that also writes the cell words of the bit-word layout: a
+// wave covers cells 256 * blockIdx.x .. +255 of its row, i.e. words
+// 4 * blockIdx.x .. +3; lane l holds cells 4l .. 4l+3, so word q is the four
+// ballots of the lanes' bits, each spread to every fourth bit.
+__global__ __launch_bounds__(kTX* kTY) void k_threshold_words4(
+    const float* __restrict__ f, int nx, int ny, int nz, float lo, float hi, int wcx,
+    int32_t* __restrict__ counts, uint64_t* __restrict__ cbits, int32_t* __restrict__ cpop) {
+  const int cx = nx - 1, cy = ny - 1, cz = nz - 1;
+  const int lane = threadIdx.x;
+  const int x0 = 4 * (blockIdx.x * kTX + lane), j = blockIdx.y * kTY + threadIdx.y;
+  const int k0 = blockIdx.z * kZRun;
+  if (j >= cy || k0 >= cz) return;  // whole waves (one row per wave)
+  const unsigned valid = x0 >= cx ? 0u : (x0 + 4 <= cx ? 0xFu : (1u << (cx - x0)) - 1u);
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  const float* p = f + sy * j + sz * k0;
+  unsigned a = range_bits5(p, x0, nx, lo, hi) & range_bits5(p + sy, x0, nx, lo, hi);
+  const int k1 = min(k0 + kZRun, cz);
+  int32_t* out = counts + x0 + (int64_t)cx * (j + (int64_t)cy * k0);
+  const int64_t cplane = (int64_t)cx * cy;
+  for (int k = k0; k < k1; ++k) {
+    p += sz;
+    const unsigned b = range_bits5(p, x0, nx, lo, hi) & range_bits5(p + sy, x0, nx, lo, hi);
+    const unsigned both = a & b, kept = both & (both >> 1) & valid;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (x0 + q < cx) out[q] = (int32_t)((kept >> q) & 1u);
+    const uint64_t b0 = __ballot(kept & 1u), b1 = __ballot((kept >> 1) & 1u);
+    const uint64_t b2 = __ballot((kept >> 2) & 1u), b3 = __ballot((kept >> 3) & 1u);
+    const int wx = 4 * (int)blockIdx.x + lane;
+    if (lane < 4 && wx < wcx) {
+      const int sh = 16 * lane;
+      const uint64_t w = spread16(b0 >> sh) | spread16(b1 >> sh) << 1 | spread16(b2 >> sh) << 2 |
+                         spread16(b3 >> sh) << 3;
+      const int64_t wi = ((int64_t)j + (int64_t)cy * k) * wcx + wx;
+      cbits[wi] = w;
+      cpop[wi] = __popcll(w);
+    }
+    out += cplane;
+    a = b;
+  }
+}
+
 // Unstructured grid: a thread per cell of nv vertices (nv <= 8); an id
 // outside [0, n_points) sets *bad and counts 0.
 __global__ __launch_bounds__(256) void k_count_cells(const int32_t* __restrict__ cells, int nv,
@@ -602,13 +654,16 @@ inline bool uniform_ok(int64_t nx, int64_t ny, int64_t nz) {
          nz <= 65535 * (int64_t)kZRun && nx * ny * nz <= 0x7fffffffLL;
 }
 
+// words_ready: the cell words and popcounts of the bit-word layout are
+// already in the workspace (k_threshold_words4) and the counts are 0/1.
 int prepare(const Geometry& g, const int32_t* d_counts, unsigned flags, void* d_ws,
             size_t ws_bytes, int32_t* d_mask, int64_t* h_cells, int64_t* h_points,
-            hipStream_t s) {
+            hipStream_t s, bool words_ready = false) {
   const bool dedup = (flags & DAX_HIP_MC_DEDUP) != 0;
+  const bool mask_optional = (flags & DAX_HIP_TOPOLOGY_MASK_OPTIONAL) != 0;
   const TopoWs w = topo_ws(g.n_cells, g.n_points);
   if (!d_counts || !h_cells || !h_points || !d_ws || ws_bytes < w.bytes || (dedup && !d_mask) ||
-      (flags & ~(unsigned)DAX_HIP_MC_DEDUP)) {
+      (flags & ~(unsigned)(DAX_HIP_MC_DEDUP | DAX_HIP_TOPOLOGY_MASK_OPTIONAL))) {
     set_error("topology prepare: bad arguments (workspace of %zu bytes needed)", w.bytes);
     return DAX_HIP_BAD_VALUE;
   }
@@ -618,8 +673,10 @@ int prepare(const Geometry& g, const int32_t* d_counts, unsigned flags, void* d_
   int32_t* hdr = (int32_t*)ws;
   const dim3 b(256);
   int rc = DAX_HIP_OK;
-  k_set_header<<<1, 64, 0, s>>>(hdr, kModeCells, g.nx, g.ny, g.nz, dedup ? 1 : 0);
-  DAXHIP_LAUNCH_CHECK("k_set_header");
+  if (!words_ready) {
+    k_set_header<<<1, 64, 0, s>>>(hdr, kModeCells, g.nx, g.ny, g.nz, dedup ? 1 : 0);
+    DAXHIP_LAUNCH_CHECK("k_set_header");
+  }
   if (!g.cells) {
     // counts 0/1 on a uniform grid: the bit-word layout
     const BitGeom bg = bit_geom(g.nx, g.ny, g.nz);
@@ -627,13 +684,17 @@ int prepare(const Geometry& g, const int32_t* d_counts, unsigned flags, void* d_
     if (bw.fits) {
       uint64_t* cbits = (uint64_t*)(ws + bw.cbits);
       int32_t* coff = (int32_t*)(ws + bw.coff);
-      k_cell_bits<<<dim3((unsigned)ceil_div((int64_t)bg.cy * bg.cz, kRowWaves)),
-                    dim3(64 * kRowWaves), 0, s>>>(bg, d_counts, cbits, coff, hdr + 1);
-      DAXHIP_LAUNCH_CHECK("k_cell_bits");
+      if (!words_ready) {
+        k_cell_bits<<<dim3((unsigned)ceil_div((int64_t)bg.cy * bg.cz, kRowWaves)),
+                      dim3(64 * kRowWaves), 0, s>>>(bg, d_counts, cbits, coff, hdr + 1);
+        DAXHIP_LAUNCH_CHECK("k_cell_bits");
+      }
       int32_t total = 0, general = 0;
       if ((rc = dax_hip_scan_exclusive_i32(coff, coff, bg.n_cwords, &total, s))) return rc;
-      DAXHIP_CHECK(hipMemcpyAsync(&general, hdr + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-      DAXHIP_CHECK(hipStreamSynchronize(s));
+      if (!words_ready) {
+        DAXHIP_CHECK(hipMemcpyAsync(&general, hdr + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        DAXHIP_CHECK(hipStreamSynchronize(s));
+      }
       if (!general) {
         if ((int64_t)total * 8 > 0x7fffffffLL) {
           set_error("topology prepare: %lld output cells exceed 2^31 ids", (long long)total);
@@ -646,7 +707,8 @@ int prepare(const Geometry& g, const int32_t* d_counts, unsigned flags, void* d_
         uint64_t* mbits = (uint64_t*)(ws + bw.mbits);
         int32_t* moff = (int32_t*)(ws + bw.moff);
         k_mask_bits<<<dim3((unsigned)ceil_div((int64_t)bg.ny * bg.nz, kRowWaves)),
-                      dim3(64 * kRowWaves), 0, s>>>(bg, cbits, mbits, moff, d_mask);
+                      dim3(64 * kRowWaves), 0, s>>>(bg, cbits, mbits, moff,
+                                                    mask_optional ? nullptr : d_mask);
         DAXHIP_LAUNCH_CHECK("k_mask_bits");
         int32_t used = 0;
         if ((rc = dax_hip_scan_exclusive_i32(moff, moff, bg.n_pwords, &used, s))) return rc;
@@ -828,6 +890,46 @@ int dax_hip_threshold_count_cells(const int32_t* d_cells, int nv, int64_t n_cell
   return DAX_HIP_OK;
 }
 
+int dax_hip_threshold_prepare_uniform(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
+                                      float min_value, float max_value, unsigned flags,
+                                      int32_t* d_counts, void* d_workspace,
+                                      size_t workspace_bytes, int32_t* d_point_mask,
+                                      int64_t* h_cells, int64_t* h_points, void* stream) {
+  if (!uniform_ok(nx, ny, nz) || !d_field || !d_counts) {
+    set_error("dax_hip_threshold_prepare_uniform: bad arguments (grid %lld x %lld x %lld)",
+              (long long)nx, (long long)ny, (long long)nz);
+    return DAX_HIP_BAD_VALUE;
+  }
+  const Geometry g{nullptr, 8, (nx - 1) * (ny - 1) * (nz - 1), nx * ny * nz, (int)nx, (int)ny,
+                   (int)nz};
+  const TopoWs w = topo_ws(g.n_cells, g.n_points);
+  const BitGeom bg = bit_geom(g.nx, g.ny, g.nz);
+  const BitWs bw = bit_ws(w, bg);
+  hipStream_t s = (hipStream_t)stream;
+  const bool fused = (nx & 3) == 0 && ((uintptr_t)d_field & 15u) == 0 && bw.fits &&
+                     d_workspace && workspace_bytes >= w.bytes;
+  if (!fused) {  // ThresholdCount, then the generic prepare
+    int rc = dax_hip_threshold_count_uniform(d_field, nx, ny, nz, min_value, max_value, d_counts,
+                                             stream);
+    if (rc) return rc;
+    return prepare(g, d_counts, flags, d_workspace, workspace_bytes, d_point_mask, h_cells,
+                   h_points, s);
+  }
+  char* ws = (char*)d_workspace;
+  const bool dedup = (flags & DAX_HIP_MC_DEDUP) != 0;
+  k_set_header<<<1, 64, 0, s>>>((int32_t*)ws, kModeCells, g.nx, g.ny, g.nz, dedup ? 1 : 0);
+  DAXHIP_LAUNCH_CHECK("k_set_header");
+  k_threshold_words4<<<dim3((unsigned)ceil_div(nx, 4 * kTX), (unsigned)ceil_div(bg.cy, kTY),
+                            (unsigned)ceil_div(bg.cz, kZRun)),
+                       dim3(kTX, kTY), 0, s>>>(d_field, (int)nx, (int)ny, (int)nz, min_value,
+                                               max_value, bg.wcx, d_counts,
+                                               (uint64_t*)(ws + bw.cbits),
+                                               (int32_t*)(ws + bw.coff));
+  DAXHIP_LAUNCH_CHECK("k_threshold_words4");
+  return prepare(g, d_counts, flags, d_workspace, workspace_bytes, d_point_mask, h_cells, h_points,
+                 s, true);
+}
+
 size_t dax_hip_topology_workspace_bytes(int64_t n_cells, int64_t n_points) {
   return topo_ws(n_cells, n_points).bytes;
 }
@@ -900,7 +1002,7 @@ int dax_hip_topology_compact_point_field(const void* d_in, size_t value_bytes, i
   const TopoWs w = topo_ws(n_cells, n_points);
   if (value_bytes == 0 || value_bytes % 4 || value_bytes > 1024 || n_points < 0 ||
       n_points > 0x7fffffffLL || !d_workspace || workspace_bytes < w.bytes ||
-      (n_points > 0 && (!d_in || !d_point_mask || !d_out))) {
+      (n_points > 0 && (!d_in || !d_out))) {
     set_error("dax_hip_topology_compact_point_field: bad arguments (values of 4k bytes, "
               "workspace of %zu bytes)", w.bytes);
     return DAX_HIP_BAD_VALUE;
@@ -929,6 +1031,10 @@ int dax_hip_topology_compact_point_field(const void* d_in, size_t value_bytes, i
         0, 0, 0, 0, 0, 0, nullptr, (const uint32_t*)d_in, words, (uint32_t*)d_out);
     DAXHIP_LAUNCH_CHECK("k_compact_bits");
     return DAX_HIP_OK;
+  }
+  if (!d_point_mask) {
+    set_error("dax_hip_topology_compact_point_field: the point mask is needed for this layout");
+    return DAX_HIP_BAD_VALUE;
   }
   k_compact_values<<<dim3((unsigned)ceil_div(n_points, 1024)), dim3(256), 0, s>>>(
       (const uint32_t*)d_in, words, n_points, d_point_mask, aligned16(d_point_mask),

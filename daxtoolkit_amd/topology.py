@@ -16,7 +16,7 @@ from typing import Optional
 import torch
 
 from . import _lib
-from ._lib import MC_DEDUP
+from ._lib import MC_DEDUP, TOPOLOGY_MASK_OPTIONAL
 
 
 def _stream():
@@ -39,6 +39,7 @@ class Topology:
     point_mask: Optional[torch.Tensor]
     workspace: Optional[torch.Tensor] = None   # the prepare step's (ranks of the used points)
     num_input_cells: int = 0
+    _mask_buffer: Optional[torch.Tensor] = None  # what prepare was given for the mask
 
     @property
     def num_cells(self) -> int:
@@ -51,16 +52,16 @@ class Topology:
     def compact_point_field(self, values: torch.Tensor) -> torch.Tensor:
         """CompactPointField (:97-110): the used points' values in point
         order (a StreamCompact by the point mask); needs duplicate removal."""
-        if self.point_mask is None:
+        if self._mask_buffer is None:
             raise ValueError("compact_point_field needs duplicate removal")
-        P = self.point_mask.numel()
+        P = self._mask_buffer.numel()
         v = values.reshape(P, -1).contiguous()
         if v.element_size() * v.shape[1] % 4:
             raise ValueError("point values of a multiple of 4 bytes")
         out = torch.empty((max(self.num_points, 1), v.shape[1]), dtype=v.dtype, device=v.device)
         _lib.call("dax_hip_topology_compact_point_field", v.data_ptr(),
                   v.element_size() * v.shape[1], self.num_input_cells, P,
-                  self.point_mask.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(),
+                  self._mask_buffer.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(),
                   out.data_ptr(), _stream())
         res = out[: self.num_points]
         return res.reshape(-1) if values.dim() == 1 else res.reshape((self.num_points,) +
@@ -98,6 +99,35 @@ def threshold_count(field: torch.Tensor, lo: float, hi: float,
     return counts[:C]
 
 
+def _generate(counts, dims, cells, points, origin, spacing, dedup, ws, mask, T, U,
+              mask_valid=True) -> Topology:
+    dev = counts.device
+    if cells is None:
+        nx, ny, nz = (int(v) for v in dims)
+        C, P, V = (nx - 1) * (ny - 1) * (nz - 1), nx * ny * nz, 8
+    else:
+        C, V = cells.shape
+        P = mask.numel() if mask is not None else int(points.shape[0])
+    flags = MC_DEDUP if dedup else 0
+    mptr = mask.data_ptr() if dedup else None
+    conn = torch.empty((max(T, 1), V), dtype=torch.int32, device=dev)
+    coords = torch.empty((max(U, 1), 3), dtype=torch.float32, device=dev) if dedup else None
+    cptr = coords.data_ptr() if dedup else None
+    if T > 0:
+        if cells is None:
+            _lib.call("dax_hip_topology_generate_uniform", nx, ny, nz, _f3(origin), _f3(spacing),
+                      counts.data_ptr(), flags, ws.data_ptr(), ws.numel(), mptr,
+                      conn.data_ptr(), cptr, _stream())
+        else:
+            _lib.call("dax_hip_topology_generate_cells", cells.data_ptr(), V, C,
+                      points.data_ptr() if points is not None else None, P, counts.data_ptr(),
+                      flags, ws.data_ptr(), ws.numel(), mptr, conn.data_ptr(), cptr, _stream())
+    return Topology(counts=counts[:C], connectivity=conn[:T],
+                    points=coords[:U] if dedup else None,
+                    point_mask=mask if (dedup and mask_valid) else None,
+                    workspace=ws, num_input_cells=C, _mask_buffer=mask)
+
+
 def generate_topology(counts: torch.Tensor, dims=None, cells: Optional[torch.Tensor] = None,
                       points: Optional[torch.Tensor] = None, origin=(0, 0, 0), spacing=(1, 1, 1),
                       dedup: bool = True) -> Topology:
@@ -133,30 +163,35 @@ def generate_topology(counts: torch.Tensor, dims=None, cells: Optional[torch.Ten
         _lib.call("dax_hip_topology_prepare_cells", cells.data_ptr(), V, C, P, counts.data_ptr(),
                   flags, ws.data_ptr(), ws.numel(), mptr, ctypes.byref(t), ctypes.byref(u),
                   _stream())
-    T, U = int(t.value), int(u.value)
-    conn = torch.empty((max(T, 1), V), dtype=torch.int32, device=dev)
-    coords = torch.empty((max(U, 1), 3), dtype=torch.float32, device=dev) if dedup else None
-    cptr = coords.data_ptr() if dedup else None
-    if T > 0:
-        if cells is None:
-            _lib.call("dax_hip_topology_generate_uniform", nx, ny, nz, _f3(origin), _f3(spacing),
-                      counts.data_ptr(), flags, ws.data_ptr(), ws.numel(), mptr,
-                      conn.data_ptr(), cptr, _stream())
-        else:
-            _lib.call("dax_hip_topology_generate_cells", cells.data_ptr(), V, C,
-                      points.data_ptr() if points is not None else None, P, counts.data_ptr(),
-                      flags, ws.data_ptr(), ws.numel(), mptr, conn.data_ptr(), cptr, _stream())
-    return Topology(counts=counts[:C], connectivity=conn[:T],
-                    points=coords[:U] if dedup else None, point_mask=mask, workspace=ws,
-                    num_input_cells=C)
+    return _generate(counts, dims, cells, points, origin, spacing, dedup, ws, mask,
+                     int(t.value), int(u.value))
 
 
 def threshold(field: torch.Tensor, lo: float, hi: float, cells: Optional[torch.Tensor] = None,
               points: Optional[torch.Tensor] = None, origin=(0, 0, 0), spacing=(1, 1, 1),
-              dedup: bool = True) -> Topology:
+              dedup: bool = True, point_mask: bool = True) -> Topology:
     """The reference's Threshold pipeline (Benchmarks/Threshold/Pipeline.h:
-    163-190): ThresholdCount, then DispatcherGenerateTopology."""
-    counts = threshold_count(field, lo, hi, cells)
-    dims = None if cells is not None else (field.shape[2], field.shape[1], field.shape[0])
-    return generate_topology(counts, dims=dims, cells=cells, points=points, origin=origin,
-                             spacing=spacing, dedup=dedup)
+    163-190): ThresholdCount, then DispatcherGenerateTopology.  On a uniform
+    grid both run in one call (dax_hip_threshold_prepare_uniform); with
+    point_mask=False the int32 point mask is not kept (CompactPointField
+    does not need it)."""
+    if cells is not None or field.dim() != 3:
+        counts = threshold_count(field, lo, hi, cells)
+        return generate_topology(counts, cells=cells, points=points, origin=origin,
+                                 spacing=spacing, dedup=dedup)
+    if field.dtype != torch.float32 or not field.is_contiguous():
+        raise ValueError("field must be a contiguous float32 tensor")
+    nz, ny, nx = field.shape
+    C, P = (nx - 1) * (ny - 1) * (nz - 1), nx * ny * nz
+    dev = field.device
+    counts = torch.empty(max(C, 1), dtype=torch.int32, device=dev)
+    ws = torch.empty(int(_lib.load().dax_hip_topology_workspace_bytes(C, P)), dtype=torch.uint8,
+                     device=dev)
+    mask = torch.empty(P, dtype=torch.int32, device=dev) if dedup else None
+    flags = (MC_DEDUP if dedup else 0) | (0 if point_mask else TOPOLOGY_MASK_OPTIONAL)
+    t, u = ctypes.c_int64(), ctypes.c_int64()
+    _lib.call("dax_hip_threshold_prepare_uniform", field.data_ptr(), nx, ny, nz, float(lo),
+              float(hi), flags, counts.data_ptr(), ws.data_ptr(), ws.numel(),
+              mask.data_ptr() if dedup else None, ctypes.byref(t), ctypes.byref(u), _stream())
+    return _generate(counts, (nx, ny, nz), None, None, origin, spacing, dedup, ws, mask,
+                     int(t.value), int(u.value), mask_valid=point_mask)

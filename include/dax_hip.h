@@ -163,6 +163,8 @@ size_t dax_hip_mc_workspace_bytes(int64_t nx, int64_t ny, int64_t nz);
 /* Flags for dax_hip_mc_forward. */
 #define DAX_HIP_MC_DEDUP 1u          /* RemoveDuplicatePoints(true) */
 #define DAX_HIP_MC_NORMALS 2u        /* write d_normals */
+#define DAX_HIP_TOPOLOGY_MASK_OPTIONAL 4u /* topology prepare: the int32 point mask may be
+                                            left unwritten (CompactPointField still works) */
 
 /* classify + count/scan + generate in one call, graph-capturable (no host
  * read-back).  d_counts (device int64[2]) receives {T, U}: triangles and
@@ -292,6 +294,18 @@ int dax_hip_threshold_count_cells(const int32_t* d_cells, int nv, int64_t n_cell
                                   const float* d_field, int64_t n_points, float min_value,
                                   float max_value, int32_t* d_counts, void* stream);
 size_t dax_hip_topology_workspace_bytes(int64_t n_cells, int64_t n_points);
+/* ThresholdCount and the prepare step in one call on a uniform grid (the
+ * reference's Threshold pipeline, Benchmarks/Threshold/Pipeline.h:163-190):
+ * d_counts as dax_hip_threshold_count_uniform, then as
+ * dax_hip_topology_prepare_uniform on them.  With rows of 16-byte aligned
+ * float4s (nx % 4 == 0) the count kernel also forms the bit-word layout's
+ * cell words.  With DAX_HIP_TOPOLOGY_MASK_OPTIONAL, d_point_mask may be
+ * left unwritten where that layout is used; pass the buffer anyway. */
+int dax_hip_threshold_prepare_uniform(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
+                                      float min_value, float max_value, unsigned flags,
+                                      int32_t* d_counts, void* d_workspace,
+                                      size_t workspace_bytes, int32_t* d_point_mask,
+                                      int64_t* h_cells, int64_t* h_points, void* stream);
 int dax_hip_topology_prepare_uniform(int64_t nx, int64_t ny, int64_t nz, const int32_t* d_counts,
                                      unsigned flags, void* d_workspace, size_t workspace_bytes,
                                      int32_t* d_point_mask, int64_t* h_cells, int64_t* h_points,

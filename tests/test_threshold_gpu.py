@@ -170,3 +170,22 @@ def test_dedup_generate_after_plain_prepare_is_rejected():
     with pytest.raises(_lib.ErrorControlBadValue):
         _lib.call("dax_hip_topology_compact_point_field", f.data_ptr(), 4, C, P, mask.data_ptr(),
                   ws.data_ptr(), ws.numel(), out.data_ptr(), s)
+
+
+@pytest.mark.parametrize("dims", [(68, 20, 9), (256, 7, 5), (70, 33, 20)])
+def test_fused_prepare_without_mask_matches_oracle(dims):
+    """ThresholdCount and prepare in one call (the count kernel forming the
+    cell words when nx % 4 == 0), the point mask left out: the same counts,
+    connectivity, coordinates and CompactPointField as the oracle."""
+    rng = np.random.default_rng(sum(dims) + 5)
+    f = rng.random(dims[::-1]).astype(np.float32)
+    print(f"[case] threshold fused {dims}", flush=True)
+    ft = torch.from_numpy(f).cuda()
+    out = topology.threshold(ft, 0.05, 0.97, point_mask=False)
+    ref = orc.threshold(f, 0.05, 0.97)
+    assert out.point_mask is None
+    assert out.num_cells == ref["n_cells"] and out.num_points == ref["n_points"]
+    assert np.array_equal(out.counts.cpu().numpy(), ref["counts"])
+    assert np.array_equal(out.connectivity.cpu().numpy(), ref["conn"])
+    assert np.array_equal(out.points.cpu().numpy(), ref["coords"])
+    assert np.array_equal(out.compact_point_field(ft.reshape(-1)).cpu().numpy(), ref["field"])
