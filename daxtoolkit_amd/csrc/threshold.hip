@@ -507,6 +507,30 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_mask_bits(BitGeom g,
   }
 }
 
+// The point words without the int32 mask: a lane per word (32-bit index
+// arithmetic; every point word has its own independent loads).
+__global__ __launch_bounds__(256) void k_mask_words(BitGeom g, const uint64_t* __restrict__ cbits,
+                                                    uint64_t* __restrict__ mbits,
+                                                    int32_t* __restrict__ mpop) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= (uint32_t)g.n_pwords) return;
+  const uint32_t prow = w / (uint32_t)g.wpx;
+  const int wx = (int)(w - prow * (uint32_t)g.wpx);
+  const int j = (int)(prow % (uint32_t)g.ny), k = (int)(prow / (uint32_t)g.ny);
+  uint64_t m = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int jc = j - 1 + (q & 1), kc = k - 1 + (q >> 1);
+    if (kc < 0 || kc >= g.cz || jc < 0 || jc >= g.cy) continue;
+    const uint64_t* r = cbits + ((int64_t)jc + (int64_t)g.cy * kc) * g.wcx;
+    const uint64_t c = wx < g.wcx ? r[wx] : 0;     // wx <= wcx
+    const uint64_t prev = wx > 0 ? r[wx - 1] : 0;  // wx - 1 < wcx
+    m |= c | (c << 1) | (prev >> 63);
+  }
+  mbits[w] = m;
+  mpop[w] = __popcll(m);
+}
+
 // Rank of point (i, j, k) among the used points.
 __device__ __forceinline__ int32_t bit_rank(const BitGeom& g, const uint64_t* __restrict__ mbits,
                                             const int32_t* __restrict__ moff, int i, int j, int k) {
@@ -706,10 +730,15 @@ int prepare(const Geometry& g, const int32_t* d_counts, unsigned flags, void* d_
         if (!dedup) return DAX_HIP_OK;
         uint64_t* mbits = (uint64_t*)(ws + bw.mbits);
         int32_t* moff = (int32_t*)(ws + bw.moff);
-        k_mask_bits<<<dim3((unsigned)ceil_div((int64_t)bg.ny * bg.nz, kRowWaves)),
-                      dim3(64 * kRowWaves), 0, s>>>(bg, cbits, mbits, moff,
-                                                    mask_optional ? nullptr : d_mask);
-        DAXHIP_LAUNCH_CHECK("k_mask_bits");
+        if (mask_optional) {
+          k_mask_words<<<dim3((unsigned)ceil_div(bg.n_pwords, 256)), b, 0, s>>>(bg, cbits, mbits,
+                                                                                moff);
+          DAXHIP_LAUNCH_CHECK("k_mask_words");
+        } else {
+          k_mask_bits<<<dim3((unsigned)ceil_div((int64_t)bg.ny * bg.nz, kRowWaves)),
+                        dim3(64 * kRowWaves), 0, s>>>(bg, cbits, mbits, moff, d_mask);
+          DAXHIP_LAUNCH_CHECK("k_mask_bits");
+        }
         int32_t used = 0;
         if ((rc = dax_hip_scan_exclusive_i32(moff, moff, bg.n_pwords, &used, s))) return rc;
         *h_points = used;
