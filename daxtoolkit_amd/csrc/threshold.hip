@@ -67,6 +67,27 @@ __global__ __launch_bounds__(kTX* kTY) void k_count_uniform(const float* __restr
   }
 }
 
+// The counts of a wave's 256 cells (lane l holds cells 4l .. 4l+3 as bits
+// of `kept`) written through the wave's LDS slice so that each store
+// instruction covers 64 consecutive cells; n = cells of the 256 in the row.
+__device__ __forceinline__ void store_counts256(int32_t* s_w, unsigned kept, int32_t* out, int n) {
+  const int lane = threadIdx.x & 63;
+  int4 v;
+  v.x = (int)(kept & 1u);
+  v.y = (int)((kept >> 1) & 1u);
+  v.z = (int)((kept >> 2) & 1u);
+  v.w = (int)((kept >> 3) & 1u);
+  *(int4*)(s_w + 4 * lane) = v;
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int x = lane + 64 * q;
+    const int32_t c = s_w[x];
+    if (x < n) out[x] = c;
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 // The same with rows of 16-byte aligned float4s (nx % 4 == 0): lane l of a
 // wave covers cells x0 .. x0+3 (x0 = 4 * (64 * blockIdx.x + l)) from one
 // float4 per point row and the next lane's first value (lane 63 loads it).
@@ -91,17 +112,17 @@ __global__ __launch_bounds__(kTX* kTY) void k_count_uniform4(const float* __rest
   const int64_t sy = nx, sz = (int64_t)nx * ny;
   const float* p = f + sy * j + sz * k0;
   // bit q: point x0+q in range in both rows j and j+1 of the plane
+  __shared__ __align__(16) int32_t s_cnt[kTY][256];
   unsigned a = range_bits5(p, x0, nx, lo, hi) & range_bits5(p + sy, x0, nx, lo, hi);
   const int k1 = min(k0 + kZRun, cz);
-  int32_t* out = counts + x0 + (int64_t)cx * (j + (int64_t)cy * k0);
+  const int xw = 256 * (int)blockIdx.x;  // the wave's first cell
+  int32_t* out = counts + xw + (int64_t)cx * (j + (int64_t)cy * k0);
   const int64_t cplane = (int64_t)cx * cy;
   for (int k = k0; k < k1; ++k) {
     p += sz;
     const unsigned b = range_bits5(p, x0, nx, lo, hi) & range_bits5(p + sy, x0, nx, lo, hi);
     const unsigned both = a & b, kept = both & (both >> 1);  // cell q: points q and q+1
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (x0 + q < cx) out[q] = (int32_t)((kept >> q) & 1u);
+    store_counts256(s_cnt[threadIdx.y], kept, out, min(256, cx - xw));
     out += cplane;
     a = b;
   }
@@ -129,20 +150,20 @@ __global__ __launch_bounds__(kTX* kTY) void k_threshold_words4(
   const int x0 = 4 * (blockIdx.x * kTX + lane), j = blockIdx.y * kTY + threadIdx.y;
   const int k0 = blockIdx.z * kZRun;
   if (j >= cy || k0 >= cz) return;  // whole waves (one row per wave)
+  __shared__ __align__(16) int32_t s_cnt[kTY][256];
   const unsigned valid = x0 >= cx ? 0u : (x0 + 4 <= cx ? 0xFu : (1u << (cx - x0)) - 1u);
   const int64_t sy = nx, sz = (int64_t)nx * ny;
   const float* p = f + sy * j + sz * k0;
   unsigned a = range_bits5(p, x0, nx, lo, hi) & range_bits5(p + sy, x0, nx, lo, hi);
   const int k1 = min(k0 + kZRun, cz);
-  int32_t* out = counts + x0 + (int64_t)cx * (j + (int64_t)cy * k0);
+  const int xw = 256 * (int)blockIdx.x;  // the wave's first cell
+  int32_t* out = counts + xw + (int64_t)cx * (j + (int64_t)cy * k0);
   const int64_t cplane = (int64_t)cx * cy;
   for (int k = k0; k < k1; ++k) {
     p += sz;
     const unsigned b = range_bits5(p, x0, nx, lo, hi) & range_bits5(p + sy, x0, nx, lo, hi);
     const unsigned both = a & b, kept = both & (both >> 1) & valid;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (x0 + q < cx) out[q] = (int32_t)((kept >> q) & 1u);
+    store_counts256(s_cnt[threadIdx.y], kept, out, min(256, cx - xw));
     const uint64_t b0 = __ballot(kept & 1u), b1 = __ballot((kept >> 1) & 1u);
     const uint64_t b2 = __ballot((kept >> 2) & 1u), b3 = __ballot((kept >> 3) & 1u);
     const int wx = 4 * (int)blockIdx.x + lane;
