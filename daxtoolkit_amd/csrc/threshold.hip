@@ -419,7 +419,7 @@ inline BitGeom bit_geom(int nx, int ny, int nz) {
   return b;
 }
 
-// The bit kernels run a wave per grid row (4 waves per block), looping over
+// The row kernels run a wave per grid row (4 waves per block), looping over
 // the row's words; lane l stands for bit l of a word.
 constexpr int kRowWaves = 4;
 
@@ -560,88 +560,102 @@ __device__ __forceinline__ int32_t bit_rank(const BitGeom& g, const uint64_t* __
   return moff[w] + __popcll(mbits[w] & below);
 }
 
-// Cell row r = j + cy * k: each kept cell (lane per bit) writes its 8 corner
-// ids (ranks with dedup) at its output cell, the word's first plus the kept
-// cells before it in the word.
-__global__ __launch_bounds__(64 * kRowWaves) void k_emit_bits(BitGeom g,
-                                                              const int32_t* __restrict__ mode,
-                                                              const uint64_t* __restrict__ cbits,
-                                                              const int32_t* __restrict__ coff,
-                                                              const uint64_t* __restrict__ mbits,
-                                                              const int32_t* __restrict__ moff,
-                                                              bool dedup,
-                                                              int32_t* __restrict__ conn) {
-  const int r = wave_row(), lane = threadIdx.x & 63;
-  if (r >= g.cy * g.cz || *mode != kModeBits) return;  // whole waves
-  const int j = r % g.cy, k = r / g.cy;
-  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+// Last word w with off[w] <= v (off: exclusive prefix of the words'
+// popcounts, off[0] = 0 <= v): the word holding the v-th set bit.
+__device__ __forceinline__ int64_t word_of(const int32_t* __restrict__ off, int64_t n, int64_t v) {
+  int64_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)off[mid] <= v)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Position of the k-th set bit of m (k < popcount(m)).
+__device__ __forceinline__ int nth_bit(uint64_t m, int k) {
+  int lo = 0, hi = 63;  // largest i with popc(m & ((1 << i) - 1)) <= k
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (__popcll(m & ((1ull << mid) - 1ull)) <= k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// A thread per output cell t: its cell is the t-th kept cell (word by binary
+// search over the cell words' prefixes, then the bit); it writes the 8
+// corner ids (ranks with dedup).
+__global__ __launch_bounds__(256) void k_emit_rank(BitGeom g, const int32_t* __restrict__ mode,
+                                                   const uint64_t* __restrict__ cbits,
+                                                   const int32_t* __restrict__ coff,
+                                                   const uint64_t* __restrict__ mbits,
+                                                   const int32_t* __restrict__ moff, bool dedup,
+                                                   int64_t n_out, int32_t* __restrict__ conn) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_out || *mode != kModeBits) return;
+  const int64_t w = word_of(coff, g.n_cwords, t);
+  const uint64_t bits = cbits[w];
+  const int r = (int)(t - coff[w]);
+  if (r >= __popcll(bits)) return;  // prefixes not of these words
+  const int b = nth_bit(bits, r);
+  const int64_t row = w / g.wcx;
+  const int wx = (int)(w - row * g.wcx);
+  const int j = (int)(row % g.cy), k = (int)(row / g.cy), i = wx * 64 + b;
   static constexpr int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
                                        {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
-  const uint64_t* crow = cbits + (int64_t)r * g.wcx;
-  for (int w0 = 0; w0 < g.wcx; w0 += 4) {
-    uint64_t b4[4];  // four words' loads in flight
+  int32_t* dst = conn + t * 8;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) b4[t] = w0 + t < g.wcx ? crow[w0 + t] : 0;
-    if (!(b4[0] | b4[1] | b4[2] | b4[3])) continue;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-    const int wx = w0 + t;
-    const int64_t w = (int64_t)r * g.wcx + wx;
-    const uint64_t bits = b4[t];
-    if (!((bits >> lane) & 1u)) continue;
-    int32_t* dst = conn + ((int64_t)coff[w] + __popcll(bits & lt)) * 8;
-    const int i = wx * 64 + lane;
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      const int pi = i + corner[v][0], pj = j + corner[v][1], pk = k + corner[v][2];
-      dst[v] = dedup ? bit_rank(g, mbits, moff, pi, pj, pk)
-                     : (int32_t)(pi + (int64_t)g.nx * (pj + (int64_t)g.ny * pk));
-    }
-    }
+  for (int v = 0; v < 8; ++v) {
+    const int pi = i + corner[v][0], pj = j + corner[v][1], pk = k + corner[v][2];
+    dst[v] = dedup ? bit_rank(g, mbits, moff, pi, pj, pk)
+                   : (int32_t)(pi + (int64_t)g.nx * (pj + (int64_t)g.ny * pk));
   }
 }
 
-// Point row (j, k): the used points' coordinates (origin + spacing * ijk) or
-// values (`words` 4-byte words each) at their ranks, a lane per bit.
-__global__ __launch_bounds__(64 * kRowWaves) void k_compact_bits(
+// A thread per used point u: its point is the u-th used one; writes its
+// coordinates (origin + spacing * ijk) or values (`words` 4-byte words).
+__global__ __launch_bounds__(256) void k_compact_rank(
     BitGeom g, const int32_t* __restrict__ mode, const uint64_t* __restrict__ mbits,
-    const int32_t* __restrict__ moff, float ox, float oy, float oz, float hx, float hy, float hz,
-    float* __restrict__ coords, const uint32_t* __restrict__ in, int words,
+    const int32_t* __restrict__ moff, int64_t n_used, float ox, float oy, float oz, float hx,
+    float hy, float hz, float* __restrict__ coords, const uint32_t* __restrict__ in, int words,
     uint32_t* __restrict__ out) {
-  const int pr = wave_row(), lane = threadIdx.x & 63;
-  if (pr >= g.ny * g.nz || *mode != kModeBits) return;  // whole waves
-  const int j = pr % g.ny, k = pr / g.ny;
-  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  const uint64_t* mrow = mbits + (int64_t)pr * g.wpx;
-  for (int w0 = 0; w0 < g.wpx; w0 += 4) {
-    uint64_t m4[4];  // four words' loads in flight
-#pragma unroll
-    for (int t = 0; t < 4; ++t) m4[t] = w0 + t < g.wpx ? mrow[w0 + t] : 0;
-    if (!(m4[0] | m4[1] | m4[2] | m4[3])) continue;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-    const int wx = w0 + t;
-    const int64_t w = (int64_t)pr * g.wpx + wx;
-    const uint64_t m = m4[t];
-    if (!((m >> lane) & 1u)) continue;
-    const int64_t u = (int64_t)moff[w] + __popcll(m & lt);
-    const int i = wx * 64 + lane;
-    if (coords) {
-      coords[3 * u] = ox + hx * (float)i;
-      coords[3 * u + 1] = oy + hy * (float)j;
-      coords[3 * u + 2] = oz + hz * (float)k;
-    } else {
-      const int64_t p = (int64_t)pr * g.nx + i;
-      for (int q = 0; q < words; ++q) out[u * words + q] = in[p * words + q];
-    }
-    }
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n_used || *mode != kModeBits) return;
+  const int64_t w = word_of(moff, g.n_pwords, u);
+  const uint64_t m = mbits[w];
+  const int r = (int)(u - moff[w]);
+  if (r >= __popcll(m)) return;  // prefixes not of these words
+  const int b = nth_bit(m, r);
+  const int64_t prow = w / g.wpx;
+  const int wx = (int)(w - prow * g.wpx);
+  const int i = wx * 64 + b;
+  if (coords) {
+    const int j = (int)(prow % g.ny), k = (int)(prow / g.ny);
+    coords[3 * u] = ox + hx * (float)i;
+    coords[3 * u + 1] = oy + hy * (float)j;
+    coords[3 * u + 2] = oz + hz * (float)k;
+  } else {
+    const int64_t p = prow * g.nx + i;
+    for (int q = 0; q < words; ++q) out[u * words + q] = in[p * words + q];
   }
 }
 
 // Header of the workspace: [0] layout mode, [1] flag of counts outside
 // {0, 1}, [2..4] the uniform grid's nx, ny, nz, [5] 1 when prepare computed
-// the point ranks (duplicate removal).
-constexpr int kHeaderInts = 6;
+// the point ranks (duplicate removal), [6] output cells, [7] used points.
+constexpr int kHeaderInts = 8;
+
+__global__ void k_set_sizes(int32_t* __restrict__ hdr, int32_t cells, int32_t points) {
+  if (threadIdx.x == 0) {
+    hdr[6] = cells;
+    hdr[7] = points;
+  }
+}
 __global__ void k_set_header(int32_t* __restrict__ hdr, int mode, int nx, int ny, int nz,
                              int ranked) {
   if (threadIdx.x == 0) {
@@ -748,7 +762,11 @@ int prepare(const Geometry& g, const int32_t* d_counts, unsigned flags, void* d_
         k_set_header<<<1, 64, 0, s>>>(hdr, kModeBits, g.nx, g.ny, g.nz, dedup ? 1 : 0);
         DAXHIP_LAUNCH_CHECK("k_set_header");
         *h_cells = total;
-        if (!dedup) return DAX_HIP_OK;
+        if (!dedup) {
+          k_set_sizes<<<1, 64, 0, s>>>(hdr, total, 0);
+          DAXHIP_LAUNCH_CHECK("k_set_sizes");
+          return DAX_HIP_OK;
+        }
         uint64_t* mbits = (uint64_t*)(ws + bw.mbits);
         int32_t* moff = (int32_t*)(ws + bw.moff);
         if (mask_optional) {
@@ -762,6 +780,8 @@ int prepare(const Geometry& g, const int32_t* d_counts, unsigned flags, void* d_
         }
         int32_t used = 0;
         if ((rc = dax_hip_scan_exclusive_i32(moff, moff, bg.n_pwords, &used, s))) return rc;
+        k_set_sizes<<<1, 64, 0, s>>>(hdr, total, used);
+        DAXHIP_LAUNCH_CHECK("k_set_sizes");
         *h_points = used;
         return DAX_HIP_OK;
       }
@@ -863,16 +883,17 @@ int generate(const Geometry& g, const float* d_points, const float* origin, cons
       const int32_t* coff = (const int32_t*)(ws + bw.coff);
       const uint64_t* mbits = (const uint64_t*)(ws + bw.mbits);
       const int32_t* moff = (const int32_t*)(ws + bw.moff);
-      k_emit_bits<<<dim3((unsigned)ceil_div((int64_t)bg.cy * bg.cz, kRowWaves)),
-                    dim3(64 * kRowWaves), 0, s>>>(bg, hdr, cbits, coff, mbits, moff, dedup,
-                                                  d_conn);
-      DAXHIP_LAUNCH_CHECK("k_emit_bits");
-      if (dedup) {
-        k_compact_bits<<<dim3((unsigned)ceil_div((int64_t)bg.ny * bg.nz, kRowWaves)),
-                         dim3(64 * kRowWaves), 0, s>>>(bg, hdr, mbits, moff, o[0], o[1], o[2],
-                                                       h[0], h[1], h[2], d_coords, nullptr, 0,
-                                                       nullptr);
-        DAXHIP_LAUNCH_CHECK("k_compact_bits");
+      const int64_t n_out = head[6], n_used = head[7];
+      if (n_out > 0) {
+        k_emit_rank<<<dim3((unsigned)ceil_div(n_out, 256)), b, 0, s>>>(
+            bg, hdr, cbits, coff, mbits, moff, dedup, n_out, d_conn);
+        DAXHIP_LAUNCH_CHECK("k_emit_rank");
+      }
+      if (dedup && n_used > 0) {
+        k_compact_rank<<<dim3((unsigned)ceil_div(n_used, 256)), b, 0, s>>>(
+            bg, hdr, mbits, moff, n_used, o[0], o[1], o[2], h[0], h[1], h[2], d_coords, nullptr,
+            0, nullptr);
+        DAXHIP_LAUNCH_CHECK("k_compact_rank");
       }
     }
   }
@@ -1075,11 +1096,13 @@ int dax_hip_topology_compact_point_field(const void* d_in, size_t value_bytes, i
       set_error("dax_hip_topology_compact_point_field: workspace of another grid");
       return DAX_HIP_BAD_VALUE;
     }
-    k_compact_bits<<<dim3((unsigned)ceil_div((int64_t)bg.ny * bg.nz, kRowWaves)),
-                     dim3(64 * kRowWaves), 0, s>>>(
-        bg, (const int32_t*)ws, (const uint64_t*)(ws + bw.mbits), (const int32_t*)(ws + bw.moff),
-        0, 0, 0, 0, 0, 0, nullptr, (const uint32_t*)d_in, words, (uint32_t*)d_out);
-    DAXHIP_LAUNCH_CHECK("k_compact_bits");
+    const int64_t n_used = hdr[7];
+    if (n_used > 0) {
+      k_compact_rank<<<dim3((unsigned)ceil_div(n_used, 256)), dim3(256), 0, s>>>(
+          bg, (const int32_t*)ws, (const uint64_t*)(ws + bw.mbits), (const int32_t*)(ws + bw.moff),
+          n_used, 0, 0, 0, 0, 0, 0, nullptr, (const uint32_t*)d_in, words, (uint32_t*)d_out);
+      DAXHIP_LAUNCH_CHECK("k_compact_rank");
+    }
     return DAX_HIP_OK;
   }
   if (!d_point_mask) {
