@@ -2,7 +2,7 @@
 
 Native library: ``libdaxhip.so`` (C-ABI in ``include/dax_hip.h``), built from
 ``daxtoolkit_amd/csrc``.  The C++ host API mirroring Dax (``include/dax``)
-and this Python driver both call that C-ABI.
+and this Python front end both call that C-ABI.
 """
 from . import _lib  # noqa: F401
 from ._lib import (DaxHipError, ErrorCapacity, ErrorControlBadValue,  # noqa: F401

@@ -1,4 +1,4 @@
-"""Python driver of the marching-cubes C-ABI over torch device tensors.
+"""Python front end of the marching-cubes C-ABI over torch device tensors.
 
 torch only provides device memory and the stream; all computation is in
 libdaxhip.so.  Field tensors are float32, C-contiguous, shape (nz, ny, nx) on

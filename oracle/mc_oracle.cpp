@@ -14,8 +14,8 @@
 //
 // Pinning (what checks each function; see tests/test_oracle.py):
 //   tables            SHA-256 of the expanded TriTable/NumFaces bytes
-//   oracle_mc_count   brute force over all 256 corner sign patterns; survey
-//                     cross-check values for the 128^3 sphere
+//   oracle_mc_count   brute force over all 256 corner sign patterns; known
+//                     counts for the 128^3 sphere
 //   scans / bounds    the reference conformance suite's golden vectors
 //                     (TestingDeviceAdapter.h:621-717, 940-1001)
 //   oracle_mc_generate  UnitTestWorkletMarchingCubes.cxx (26^3, x+y+z, iso 70:

@@ -125,7 +125,9 @@ def test_edge_flag_numbering_brute_force():
         assert list(zip(r["first"].tolist(), r["second"].tolist())) == edges
 
 
-def test_sphere_survey_values():
+def test_sphere_known_values():
+    """128^3 sphere distance, iso 40: known active-cell, triangle and point
+    counts of this field (30,146 cells, 60,284 triangles, 30,144 points)."""
     f = fields.sphere_distance(128)
     cases, cnt = orc.count(f, 40.0)
     assert cnt.size == 2048383
