@@ -10,5 +10,9 @@ namespace alg {
 // below 2^bits (ceil(bits / 8) passes).
 int sort_u64_bits(uint64_t* d_keys, int64_t n, int bits, hipStream_t s);
 
+// Exclusive scan of n int32 values (in place allowed), stream-ordered: no
+// total is read back.
+int scan_exclusive_i32_async(const int32_t* d_in, int32_t* d_out, int64_t n, hipStream_t s);
+
 }  // namespace alg
 }  // namespace daxhip

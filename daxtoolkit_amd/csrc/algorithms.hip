@@ -502,6 +502,10 @@ int sort(T* d, int64_t n, int desc, int32_t* perm_out, hipStream_t s) {
   return DAX_HIP_OK;
 }
 
+int scan_exclusive_i32_async(const int32_t* d_in, int32_t* d_out, int64_t n, hipStream_t s) {
+  return scan<int32_t>(d_in, d_out, n, 0, nullptr, s);
+}
+
 // Ascending, stream-ordered sort of unsigned keys that are below 2^bits.
 int sort_u64_bits(uint64_t* d, int64_t n, int bits, hipStream_t s) {
   if (n <= 1 || bits <= 0) return DAX_HIP_OK;

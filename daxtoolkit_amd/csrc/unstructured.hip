@@ -15,9 +15,10 @@
 //   dedup    : no grid structure to number edges by, so the general path:
 //              each corner's edge as the key A * P + B (P points; the order
 //              of the keys is the order of the (A, B) pairs), keys sorted
-//              over their significant bits, unique, each corner's point by
-//              lower bound -- the reference's Sort / Unique / LowerBounds on
-//              (EdgeIdFirst, EdgeIdSecond) (:261-301)
+//              over their significant bits, the unique keys numbered by a
+//              scan of the run starts, each corner's point the number of
+//              its key's first position -- the reference's Sort / Unique /
+//              LowerBounds on (EdgeIdFirst, EdgeIdSecond) (:261-301)
 //   points   : Lerp(coords[A], coords[B], w) (CompactPointField, :107-141)
 #include "algorithms.h"
 #include "mc_case_table.h"
@@ -191,17 +192,57 @@ __global__ __launch_bounds__(256) void k_generate(
   }
 }
 
+// {triangle total, bad-id flag} for one read-back: the last offset plus the
+// last count.
+__global__ void k_count_result(const int32_t* __restrict__ offs,
+                               const int32_t* __restrict__ counts, int64_t n,
+                               const int32_t* __restrict__ bad, int32_t* __restrict__ out2) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    out2[0] = offs[n - 1] + counts[n - 1];
+    out2[1] = *bad;
+  }
+}
+
+// 1 where a sorted key starts a run of equal keys.
+__global__ __launch_bounds__(256) void k_run_starts(const uint64_t* __restrict__ sorted,
+                                                    int64_t n, int32_t* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = (i == 0 || sorted[i] != sorted[i - 1]) ? 1 : 0;
+}
+
+// Each corner's point: the rank of its key's first position in the sorted
+// keys (the reference's LowerBounds in the unique keys).
+__global__ __launch_bounds__(256) void k_corner_points(const uint64_t* __restrict__ keys,
+                                                       const uint64_t* __restrict__ sorted,
+                                                       const int32_t* __restrict__ rank, int64_t n,
+                                                       int32_t* __restrict__ conn) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const uint64_t key = keys[c];
+  int64_t lo = 0, hi = n;  // first position with sorted[pos] >= key
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (sorted[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  conn[c] = rank[lo];
+}
+
 // The unique points (sorted keys): records, weights and coordinates.
-__global__ __launch_bounds__(256) void k_points(const uint64_t* __restrict__ ukeys, int64_t n,
+__global__ __launch_bounds__(256) void k_points(const uint64_t* __restrict__ sorted,
+                                                const int32_t* __restrict__ rank, int64_t n,
                                                 uint64_t n_points, const float* __restrict__ field,
                                                 const float* __restrict__ pts, float iso,
                                                 int32_t* __restrict__ first,
                                                 int32_t* __restrict__ second,
                                                 float* __restrict__ weight,
                                                 float* __restrict__ verts) {
-  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= n) return;
-  const uint64_t key = ukeys[u];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || (i > 0 && sorted[i] == sorted[i - 1])) return;  // a run's first key only
+  const int64_t u = rank[i];
+  const uint64_t key = sorted[i];
   const int32_t a = (int32_t)(key / n_points), b = (int32_t)(key - (uint64_t)a * n_points);
   const float fa = field[a], fb = field[b];
   const float w = (iso - fa) / (fb - fa);
@@ -242,10 +283,16 @@ int count_cells(const char* name, const int32_t* d_cells, int64_t n_cells, const
     k_count<NV, false><<<g, b, 0, s>>>(d_cells, n_cells, d_field, n_points, iso, d_counts,
                                        (int32_t*)flag.p);
   DAXHIP_LAUNCH_CHECK("k_count");
-  int32_t total = 0, bad = 0;
-  if ((rc = dax_hip_scan_exclusive_i32(d_counts, d_offsets, n_cells, &total, stream))) return rc;
-  DAXHIP_CHECK(hipMemcpyAsync(&bad, flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if ((rc = alg::scan_exclusive_i32_async(d_counts, d_offsets, n_cells, s))) return rc;
+  TempBuf res(s);  // {total, bad} read back together
+  if ((rc = res.alloc(2 * sizeof(int32_t)))) return rc;
+  k_count_result<<<1, 64, 0, s>>>(d_offsets, d_counts, n_cells, (const int32_t*)flag.p,
+                                  (int32_t*)res.p);
+  DAXHIP_LAUNCH_CHECK("k_count_result");
+  int32_t h_res[2] = {0, 0};
+  DAXHIP_CHECK(hipMemcpyAsync(h_res, res.p, sizeof(h_res), hipMemcpyDeviceToHost, s));
   DAXHIP_CHECK(hipStreamSynchronize(s));
+  const int32_t total = h_res[0], bad = h_res[1];
   if (bad) {
     set_error("%s: a cell references a point outside [0, %lld)", name, (long long)n_points);
     return DAX_HIP_BAD_VALUE;
@@ -305,16 +352,22 @@ int generate_cells(const char* name, const int32_t* d_cells, int64_t n_cells, co
   const int bits = kmax ? 64 - __builtin_clzll(kmax) : 1;
   int rc = alg::sort_u64_bits(sorted, n, bits, s);
   if (rc) return rc;
-  int64_t u = 0;
-  // keys < 2^62 compare the same as signed 64-bit values
-  if ((rc = dax_hip_unique_i64((int64_t*)sorted, n, &u, stream))) return rc;
-  if ((rc = dax_hip_lower_bounds_i64((const int64_t*)sorted, u, (const int64_t*)keys, n, 0, d_conn,
-                                     nullptr, stream)))
-    return rc;
-  k_points<<<dim3((unsigned)ceil_div(u, 256)), b, 0, s>>>(sorted, u, (uint64_t)n_points, d_field,
-                                                          d_points, iso, d_first, d_second,
-                                                          d_weight, d_vertices);
+  // unique points: the runs of equal sorted keys, numbered by a scan of the
+  // run starts (the reference's Unique); each corner's point by the first
+  // position of its key (its LowerBounds)
+  TempBuf rk(s);
+  if ((rc = rk.alloc((size_t)n * sizeof(int32_t)))) return rc;
+  int32_t* rank = (int32_t*)rk.p;
+  const dim3 gn((unsigned)ceil_div(n, 256));
+  k_run_starts<<<gn, b, 0, s>>>(sorted, n, rank);
+  DAXHIP_LAUNCH_CHECK("k_run_starts");
+  int32_t u = 0;
+  if ((rc = dax_hip_scan_exclusive_i32(rank, rank, n, &u, stream))) return rc;
+  k_points<<<gn, b, 0, s>>>(sorted, rank, n, (uint64_t)n_points, d_field, d_points, iso, d_first,
+                            d_second, d_weight, d_vertices);
   DAXHIP_LAUNCH_CHECK("k_points");
+  k_corner_points<<<gn, b, 0, s>>>(keys, sorted, rank, n, d_conn);
+  DAXHIP_LAUNCH_CHECK("k_corner_points");
   *h_points = u;
   return DAX_HIP_OK;
 }
