@@ -91,15 +91,33 @@ __device__ __forceinline__ void store_counts256(int32_t* s_w, unsigned kept, int
 // The same with rows of 16-byte aligned float4s (nx % 4 == 0): lane l of a
 // wave covers cells x0 .. x0+3 (x0 = 4 * (64 * blockIdx.x + l)) from one
 // float4 per point row and the next lane's first value (lane 63 loads it).
+// The loads (load_row5) and the range bits (row5_bits) are separate so that
+// a plane's rows can be loaded while the previous plane is processed.
+struct Row5 {
+  float4 v;
+  float tail;  // lane 63: the next lane's first value
+};
+
+__device__ __forceinline__ Row5 load_row5(const float* __restrict__ row, int x0, int nx) {
+  Row5 r;
+  r.v = make_float4(0.f, 0.f, 0.f, 0.f);
+  r.tail = 0.f;
+  if (x0 < nx) r.v = *(const float4*)(row + x0);
+  if ((threadIdx.x & 63) == 63 && x0 + 4 < nx) r.tail = row[x0 + 4];
+  return r;
+}
+
+__device__ __forceinline__ unsigned row5_bits(const Row5& r, float lo, float hi) {
+  float nxt = __shfl_down(r.v.x, 1, 64);
+  if ((threadIdx.x & 63) == 63) nxt = r.tail;
+  return (unsigned)in_range(r.v.x, lo, hi) | (unsigned)in_range(r.v.y, lo, hi) << 1 |
+         (unsigned)in_range(r.v.z, lo, hi) << 2 | (unsigned)in_range(r.v.w, lo, hi) << 3 |
+         (unsigned)in_range(nxt, lo, hi) << 4;
+}
+
 __device__ __forceinline__ unsigned range_bits5(const float* __restrict__ row, int x0, int nx,
                                                 float lo, float hi) {
-  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (x0 < nx) v = *(const float4*)(row + x0);
-  float nxt = __shfl_down(v.x, 1, 64);
-  if ((threadIdx.x & 63) == 63) nxt = x0 + 4 < nx ? row[x0 + 4] : 0.f;
-  return (unsigned)in_range(v.x, lo, hi) | (unsigned)in_range(v.y, lo, hi) << 1 |
-         (unsigned)in_range(v.z, lo, hi) << 2 | (unsigned)in_range(v.w, lo, hi) << 3 |
-         (unsigned)in_range(nxt, lo, hi) << 4;
+  return row5_bits(load_row5(row, x0, nx), lo, hi);
 }
 
 __global__ __launch_bounds__(kTX* kTY) void k_count_uniform4(const float* __restrict__ f, int nx,
@@ -118,9 +136,16 @@ __global__ __launch_bounds__(kTX* kTY) void k_count_uniform4(const float* __rest
   const int xw = 256 * (int)blockIdx.x;  // the wave's first cell
   int32_t* out = counts + xw + (int64_t)cx * (j + (int64_t)cy * k0);
   const int64_t cplane = (int64_t)cx * cy;
+  // point plane k + 1 is loaded while plane k is processed (k + 1 <= k1 <= nz - 1)
+  p += sz;
+  Row5 n0 = load_row5(p, x0, nx), n1 = load_row5(p + sy, x0, nx);
   for (int k = k0; k < k1; ++k) {
-    p += sz;
-    const unsigned b = range_bits5(p, x0, nx, lo, hi) & range_bits5(p + sy, x0, nx, lo, hi);
+    const unsigned b = row5_bits(n0, lo, hi) & row5_bits(n1, lo, hi);
+    if (k + 1 < k1) {
+      p += sz;
+      n0 = load_row5(p, x0, nx);
+      n1 = load_row5(p + sy, x0, nx);
+    }
     const unsigned both = a & b, kept = both & (both >> 1);  // cell q: points q and q+1
     store_counts256(s_cnt[threadIdx.y], kept, out, min(256, cx - xw));
     out += cplane;
@@ -159,9 +184,16 @@ __global__ __launch_bounds__(kTX* kTY) void k_threshold_words4(
   const int xw = 256 * (int)blockIdx.x;  // the wave's first cell
   int32_t* out = counts + xw + (int64_t)cx * (j + (int64_t)cy * k0);
   const int64_t cplane = (int64_t)cx * cy;
+  // point plane k + 1 is loaded while plane k is processed (k + 1 <= k1 <= nz - 1)
+  p += sz;
+  Row5 n0 = load_row5(p, x0, nx), n1 = load_row5(p + sy, x0, nx);
   for (int k = k0; k < k1; ++k) {
-    p += sz;
-    const unsigned b = range_bits5(p, x0, nx, lo, hi) & range_bits5(p + sy, x0, nx, lo, hi);
+    const unsigned b = row5_bits(n0, lo, hi) & row5_bits(n1, lo, hi);
+    if (k + 1 < k1) {
+      p += sz;
+      n0 = load_row5(p, x0, nx);
+      n1 = load_row5(p + sy, x0, nx);
+    }
     const unsigned both = a & b, kept = both & (both >> 1) & valid;
     store_counts256(s_cnt[threadIdx.y], kept, out, min(256, cx - xw));
     const uint64_t b0 = __ballot(kept & 1u), b1 = __ballot((kept >> 1) & 1u);
