@@ -600,8 +600,9 @@ def large(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the volume would be split into z-slabs per GPU, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported (the volume "
+              "would be split into z-slabs with a 2-plane halo, one slab per GPU)",
+              file=sys.stderr)
         sys.exit(2)
     import torch
 
