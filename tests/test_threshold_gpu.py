@@ -189,3 +189,23 @@ def test_fused_prepare_without_mask_matches_oracle(dims):
     assert np.array_equal(out.connectivity.cpu().numpy(), ref["conn"])
     assert np.array_equal(out.points.cpu().numpy(), ref["coords"])
     assert np.array_equal(out.compact_point_field(ft.reshape(-1)).cpu().numpy(), ref["field"])
+
+
+@pytest.mark.parametrize("dims", [(64, 5, 2), (132, 6, 17), (68, 4, 18), (260, 3, 34)])
+def test_count_only_matches_oracle(dims):
+    """ThresholdCount alone (DispatcherMapCell, the float4-row count kernel
+    when nx % 4 == 0) over z-runs of 1, 16 + 0, 16 + 1 and 2 x 16 + 1 cell
+    planes, then DispatcherGenerateTopology on its counts: the same counts,
+    connectivity and coordinates as the oracle."""
+    rng = np.random.default_rng(sum(dims) + 11)
+    f = rng.random(dims[::-1]).astype(np.float32)
+    print(f"[case] threshold count only {dims}", flush=True)
+    ft = torch.from_numpy(f).cuda()
+    counts = topology.threshold_count(ft, 0.04, 0.96)
+    ref = orc.threshold(f, 0.04, 0.96)
+    C = (dims[0] - 1) * (dims[1] - 1) * (dims[2] - 1)
+    assert np.array_equal(counts[:C].cpu().numpy(), ref["counts"])
+    out = topology.generate_topology(counts, dims=dims)
+    assert out.num_cells == ref["n_cells"] and out.num_points == ref["n_points"]
+    assert np.array_equal(out.connectivity.cpu().numpy(), ref["conn"])
+    assert np.array_equal(out.points.cpu().numpy(), ref["coords"])
