@@ -358,7 +358,19 @@ __global__ void k_decode(const typename RadixKey<T>::U* __restrict__ in, T* __re
   out[i] = RadixKey<T>::dec(desc ? ~u : u);
 }
 
-// Per-tile digit histograms, digit-major: hist[digit * ntiles + tile].
+// The sort tile of block L.  Workgroups are dispatched round-robin over the
+// 8 XCDs, so block L takes tile (L % 8) * (N8 / 8) + L / 8 (N8: the tile
+// count rounded down to a multiple of 8; the rest keep their index), and
+// each XCD works through a contiguous run of tiles: the histogram words of
+// consecutive tiles, which share cache lines, fill whole lines in one L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t ntiles) {
+  const int64_t n8 = ntiles & ~(int64_t)7;
+  const int64_t b = blockIdx.x;
+  return b < n8 ? (b % 8) * (n8 / 8) + b / 8 : b;
+}
+
+// Per-tile digit histograms, digit-major: hist[digit * ntiles + tile]; a
+// block writes one word to each of the 256 rows.
 template <typename U, int kSortIPT>
 __global__ __launch_bounds__(kNT) void k_radix_hist(const U* __restrict__ keys, int64_t n, int shift,
                                                     uint32_t* __restrict__ hist, int64_t ntiles) {
@@ -366,14 +378,15 @@ __global__ __launch_bounds__(kNT) void k_radix_hist(const U* __restrict__ keys, 
   __shared__ uint32_t s_h[256];
   s_h[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  const int64_t tile = xcd_tile(ntiles);
+  const int64_t base = tile * kSortTile;
 #pragma unroll 4
   for (int i = 0; i < kSortIPT; ++i) {
     const int64_t k = base + (int64_t)i * kNT + threadIdx.x;
     if (k < n) atomicAdd(&s_h[(uint32_t)(keys[k] >> shift) & 0xffu], 1u);
   }
   __syncthreads();
-  hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
+  hist[(int64_t)threadIdx.x * ntiles + tile] = s_h[threadIdx.x];
 }
 
 // Stable scatter: the tile is processed in rounds of 256 keys in index order;
