@@ -7,8 +7,10 @@ namespace daxhip {
 namespace alg {
 
 // Ascending, stream-ordered LSD radix sort of n unsigned keys that are all
-// below 2^bits (ceil(bits / 8) passes).
-int sort_u64_bits(uint64_t* d_keys, int64_t n, int bits, hipStream_t s);
+// below 2^bits (ceil(bits / 8) passes); d_vals, when given, holds n int32
+// values that move with their keys (stable).
+int sort_u64_bits(uint64_t* d_keys, int64_t n, int bits, hipStream_t s,
+                  int32_t* d_vals = nullptr);
 
 // Exclusive scan of n int32 values (in place allowed), stream-ordered: no
 // total is read back.

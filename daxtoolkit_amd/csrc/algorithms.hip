@@ -520,20 +520,23 @@ int scan_exclusive_i32_async(const int32_t* d_in, int32_t* d_out, int64_t n, hip
 }
 
 // Ascending, stream-ordered sort of unsigned keys that are below 2^bits.
-int sort_u64_bits(uint64_t* d, int64_t n, int bits, hipStream_t s) {
+int sort_u64_bits(uint64_t* d, int64_t n, int bits, hipStream_t s, int32_t* d_vals) {
   if (n <= 1 || bits <= 0) return DAX_HIP_OK;
   if (n > 0x7fffffffLL || bits > 64) {
     set_error("sort_u64_bits: more than 2^31-1 keys or more than 64 bits");
     return DAX_HIP_BAD_VALUE;
   }
-  TempBuf tk(s);
+  TempBuf tk(s), tv(s);
   int rc = tk.alloc((size_t)n * sizeof(uint64_t));
   if (rc) return rc;
+  if (d_vals && (rc = tv.alloc((size_t)n * sizeof(int32_t)))) return rc;
   uint64_t* k0 = d;
   uint64_t* k1 = (uint64_t*)tk.p;
-  int32_t *i0 = nullptr, *i1 = nullptr;
+  int32_t* i0 = d_vals;
+  int32_t* i1 = d_vals ? (int32_t*)tv.p : nullptr;
   if ((rc = radix_sort_encoded<uint64_t>(k0, k1, i0, i1, n, s, bits))) return rc;
-  if (k0 != d) return copy_bytes(d, k0, (size_t)n * sizeof(uint64_t), s);
+  if (k0 != d && (rc = copy_bytes(d, k0, (size_t)n * sizeof(uint64_t), s))) return rc;
+  if (d_vals && i0 != d_vals) return copy_bytes(d_vals, i0, (size_t)n * sizeof(int32_t), s);
   return DAX_HIP_OK;
 }
 

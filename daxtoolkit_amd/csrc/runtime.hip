@@ -40,7 +40,9 @@ hipMemPool_t g_pool[kMaxDevices] = {};
 // released in reverse order.  Reuse needs no synchronisation: every user of
 // a block runs on its stream, in order.  Allocations that do not fit come
 // from the pool, and the block grows to the high-water mark the next time it
-// is idle.  Blocks above kArenaMax are not kept.
+// is idle.  The mark counts the spilled temporaries live at the time too, so
+// one growth covers a call's whole peak.  Blocks above kArenaMax are not
+// kept.
 constexpr size_t kArenaMax = (size_t)256 << 20;
 struct Arena {
   char* base = nullptr;
@@ -50,6 +52,8 @@ struct Arena {
     bool freed;
   };
   std::vector<Live> live;
+  std::vector<std::pair<void*, size_t>> spills;  // live temporaries taken from the pool
+  size_t spilled = 0;                            // their bytes
 };
 std::map<std::pair<int, uintptr_t>, Arena> g_arenas;  // guarded by g_pool_mu
 
@@ -115,13 +119,15 @@ int temp_alloc(void** p, size_t bytes, hipStream_t s) {
         a.top += bytes;
         return DAX_HIP_OK;
       }
-      a.want = std::max(a.want, std::min(kArenaMax, a.top + bytes));
+      a.want = std::max(a.want, std::min(kArenaMax, a.top + a.spilled + bytes));
     }
-  }
-  hipError_t e = hipMallocFromPoolAsync(p, bytes, pool, s);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return check_hip(e, "hipMallocFromPoolAsync (library temporary)");
+    hipError_t e = hipMallocFromPoolAsync(p, bytes, pool, s);
+    if (e != hipSuccess) {
+      *p = nullptr;
+      return check_hip(e, "hipMallocFromPoolAsync (library temporary)");
+    }
+    a.spills.push_back({*p, bytes});
+    a.spilled += bytes;
   }
   return DAX_HIP_OK;
 }
@@ -132,9 +138,16 @@ void temp_free(void* p, hipStream_t s) {
     std::lock_guard<std::mutex> lock(g_pool_mu);
     for (auto& kv : g_arenas) {
       Arena& a = kv.second;
-      if (kv.first.second != (uintptr_t)s || !a.base || (char*)p < a.base ||
-          (char*)p >= a.base + a.cap)
-        continue;
+      if (kv.first.second != (uintptr_t)s) continue;
+      bool spill = false;  // a temporary taken from the pool: returned below
+      for (size_t i = 0; i < a.spills.size() && !spill; ++i)
+        if (a.spills[i].first == p) {
+          a.spilled -= a.spills[i].second;
+          a.spills.erase(a.spills.begin() + (std::ptrdiff_t)i);
+          spill = true;
+        }
+      if (spill) break;
+      if (!a.base || (char*)p < a.base || (char*)p >= a.base + a.cap) continue;
       const size_t off = (size_t)((char*)p - a.base);
       for (auto it = a.live.rbegin(); it != a.live.rend(); ++it)
         if (it->off == off && !it->freed) {

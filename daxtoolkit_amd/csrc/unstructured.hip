@@ -210,24 +210,26 @@ __global__ __launch_bounds__(256) void k_run_starts(const uint64_t* __restrict__
   if (i < n) flag[i] = (i == 0 || sorted[i] != sorted[i - 1]) ? 1 : 0;
 }
 
-// Each corner's point: the rank of its key's first position in the sorted
-// keys (the reference's LowerBounds in the unique keys).
-__global__ __launch_bounds__(256) void k_corner_points(const uint64_t* __restrict__ keys,
-                                                       const uint64_t* __restrict__ sorted,
+// Corner indices 0 .. n-1, carried through the key sort.
+__global__ __launch_bounds__(256) void k_iota(int32_t* __restrict__ idx, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) idx[i] = (int32_t)i;
+}
+
+// Each corner's point (the reference's LowerBounds of its key in the unique
+// keys): sorted position i holds corner idx[i], whose point is the number of
+// its run, i.e. the run starts up to and including i, less one; `rank` is
+// the exclusive scan of the run starts, so a position inside a run (not a
+// start) subtracts one.
+__global__ __launch_bounds__(256) void k_corner_points(const uint64_t* __restrict__ sorted,
+                                                       const int32_t* __restrict__ idx,
                                                        const int32_t* __restrict__ rank, int64_t n,
                                                        int32_t* __restrict__ conn) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  const uint64_t key = keys[c];
-  int64_t lo = 0, hi = n;  // first position with sorted[pos] >= key
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (sorted[mid] < key)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  conn[c] = rank[lo];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool start = i == 0 || sorted[i] != sorted[i - 1];
+  const int32_t c = idx[i];
+  if (c >= 0 && c < n) conn[c] = rank[i] - (start ? 0 : 1);  // always in range: a permutation
 }
 
 // The unique points (sorted keys): records, weights and coordinates.
@@ -344,21 +346,24 @@ int generate_cells(const char* name, const int32_t* d_cells, int64_t n_cells, co
     return DAX_HIP_OK;
   }
   const int64_t n = 3 * n_tris;
-  uint64_t* sorted = keys + n;
-  DAXHIP_CHECK(hipMemcpyAsync(sorted, keys, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice,
-                              s));
+  // the keys are sorted in place, carrying their corner indices (held in
+  // the second half of the scratch)
+  uint64_t* sorted = keys;
+  int32_t* idx = (int32_t*)(keys + n);
+  const dim3 gn((unsigned)ceil_div(n, 256));
+  k_iota<<<gn, b, 0, s>>>(idx, n);
+  DAXHIP_LAUNCH_CHECK("k_iota");
   // keys < P^2: sort over the bits that can be set
   const uint64_t kmax = (uint64_t)n_points * (uint64_t)n_points - 1;
   const int bits = kmax ? 64 - __builtin_clzll(kmax) : 1;
-  int rc = alg::sort_u64_bits(sorted, n, bits, s);
+  int rc = alg::sort_u64_bits(sorted, n, bits, s, idx);
   if (rc) return rc;
   // unique points: the runs of equal sorted keys, numbered by a scan of the
-  // run starts (the reference's Unique); each corner's point by the first
-  // position of its key (its LowerBounds)
+  // run starts (the reference's Unique); each corner's point is its run's
+  // number (the reference's LowerBounds)
   TempBuf rk(s);
   if ((rc = rk.alloc((size_t)n * sizeof(int32_t)))) return rc;
   int32_t* rank = (int32_t*)rk.p;
-  const dim3 gn((unsigned)ceil_div(n, 256));
   k_run_starts<<<gn, b, 0, s>>>(sorted, n, rank);
   DAXHIP_LAUNCH_CHECK("k_run_starts");
   int32_t u = 0;
@@ -366,7 +371,7 @@ int generate_cells(const char* name, const int32_t* d_cells, int64_t n_cells, co
   k_points<<<gn, b, 0, s>>>(sorted, rank, n, (uint64_t)n_points, d_field, d_points, iso, d_first,
                             d_second, d_weight, d_vertices);
   DAXHIP_LAUNCH_CHECK("k_points");
-  k_corner_points<<<gn, b, 0, s>>>(keys, sorted, rank, n, d_conn);
+  k_corner_points<<<gn, b, 0, s>>>(sorted, idx, rank, n, d_conn);
   DAXHIP_LAUNCH_CHECK("k_corner_points");
   *h_points = u;
   return DAX_HIP_OK;
