@@ -113,8 +113,9 @@ __device__ __forceinline__ void lerp3(const float* __restrict__ pts, int64_t a, 
 }
 
 // Corners of every triangle: with keys, the edge key A * P + B of each
-// corner (dedup); without, the corner's point record, coordinates and
-// identity connectivity.
+// corner and, after the 3T keys, the corner's index (dedup: the index moves
+// with its key through the sort); without, the corner's point record,
+// coordinates and identity connectivity.
 //
 // A thread per output triangle (the reference's UpperBounds and visit index,
 // DispatcherGenerateInterpolatedCells.h:196-212): its cell is the last one
@@ -177,6 +178,7 @@ __global__ __launch_bounds__(256) void k_generate(
     const int64_t i = base + k;
     if (keys) {
       keys[i] = (uint64_t)ia * (uint64_t)n_points + (uint64_t)ib;
+      ((int32_t*)(keys + 3 * n_tris))[i] = (int32_t)i;
     } else {
       const float w = (iso - fa) / (fb - fa);
       first[i] = ia;
@@ -208,12 +210,6 @@ __global__ __launch_bounds__(256) void k_run_starts(const uint64_t* __restrict__
                                                     int64_t n, int32_t* __restrict__ flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) flag[i] = (i == 0 || sorted[i] != sorted[i - 1]) ? 1 : 0;
-}
-
-// Corner indices 0 .. n-1, carried through the key sort.
-__global__ __launch_bounds__(256) void k_iota(int32_t* __restrict__ idx, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) idx[i] = (int32_t)i;
 }
 
 // Each corner's point (the reference's LowerBounds of its key in the unique
@@ -346,13 +342,11 @@ int generate_cells(const char* name, const int32_t* d_cells, int64_t n_cells, co
     return DAX_HIP_OK;
   }
   const int64_t n = 3 * n_tris;
-  // the keys are sorted in place, carrying their corner indices (held in
-  // the second half of the scratch)
+  // the keys are sorted in place, carrying their corner indices (generate
+  // wrote them to the second half of the scratch)
   uint64_t* sorted = keys;
   int32_t* idx = (int32_t*)(keys + n);
   const dim3 gn((unsigned)ceil_div(n, 256));
-  k_iota<<<gn, b, 0, s>>>(idx, n);
-  DAXHIP_LAUNCH_CHECK("k_iota");
   // keys < P^2: sort over the bits that can be set
   const uint64_t kmax = (uint64_t)n_points * (uint64_t)n_points - 1;
   const int bits = kmax ? 64 - __builtin_clzll(kmax) : 1;
