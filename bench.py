@@ -62,8 +62,7 @@ def _kernel_variants(n, tuned):
     unroll, vec, sub, tile = tuned[0], tuned[1], tuned[2], tuned[3]
     cls = (f"k_classify_v4<{unroll}>" if vec == 4 else
            f"k_classify<{unroll}, {'true' if n % 64 == 0 else 'false'}>")
-    scan = {1: "<256, 4>", 2: "<512, 4>", 4: "<1024, 4>", 8: "<1024, 8>"}[sub]
-    return cls, f"k_count_scan{scan}", f"k_generate<{tile}, FieldArray>"
+    return cls, f"k_count<{sub}>", f"k_generate<{tile}, FieldArray>"
 
 
 def _measured_traffic(n, tuned):

@@ -113,7 +113,6 @@ __global__ __launch_bounds__(kBatchScanNT) void k_batch_count_scan(
   __shared__ ScanShared sh;
   __shared__ uint32_t s_pk[ScanTileLds<kBatchScanNT, kBatchScanWPT>::kSize];
   __shared__ int64_t s_tile;
-  for (int i = threadIdx.x; i < 256; i += kBatchScanNT) sh.faces[i] = (uint8_t)(c_case_table[i] >> 60);
   while (true) {
     __syncthreads();  // s_tile and sh reused across iterations
     if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(&ctrl->tile_counter, 1u);

@@ -22,6 +22,7 @@ namespace daxhip {
 namespace mc {
 
 constexpr int kTileWords = 1024;     // row words per scan tile
+constexpr int kCountTile = 128;      // row words per count tile: its prefixes fit 16 bits
 constexpr int kScanThreads = 256;    // threads of the count/scan and generate blocks
 constexpr int kWordsPerThread = kTileWords / kScanThreads;
 constexpr int kMaxWpr = 256;         // nx <= 16384
@@ -179,6 +180,31 @@ __device__ __forceinline__ void cell_planes(const WordPos& p, uint64_t A, uint64
   cp.active = (p.has_y && p.has_z) ? ((any & ~all) & p.xmask) : 0ull;
 }
 
+// Triangles of a word's cells: NumFaces[case] summed over the active cells,
+// 64 cells at a time.  For every case with mixed corners
+//   NumFaces = V - 2F - 4H,
+// V = inside corners, F = cube faces whose four corners are inside, H = 1
+// when exactly two corners are outside and they are the ends of a body
+// diagonal (the table's two-triangle cases 125, 190, 215, 235).  Checked
+// against MarchingCubesTable.h:28-45 for all 254 mixed cases.  Faces in the
+// corner order of TopologyUniform.h:139-159: z = 0 {0,1,2,3}, z = 1
+// {4,5,6,7}, y = 0 {0,1,5,4}, y = 1 {3,2,6,7}, x = 0 {0,3,7,4}, x = 1
+// {1,2,6,5}; body diagonals (0,6), (1,7), (2,4), (3,5).
+__device__ __forceinline__ uint32_t word_triangles(const CellPlanes& cp) {
+  const uint64_t a = cp.active;
+  const uint64_t* c = cp.c;
+  uint32_t v = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v += (uint32_t)popc(c[i] & a);
+  const uint64_t p01 = c[0] & c[1], p23 = c[2] & c[3], p45 = c[4] & c[5], p67 = c[6] & c[7];
+  const uint32_t f = (uint32_t)(popc(p01 & p23 & a) + popc(p45 & p67 & a) + popc(p01 & p45 & a) +
+                                popc(p23 & p67 & a) + popc(c[0] & c[3] & c[4] & c[7] & a) +
+                                popc(c[1] & c[2] & c[5] & c[6] & a));
+  const uint64_t h = (p23 & p45 & ((c[1] & ~c[0] & c[7] & ~c[6]) | (c[0] & ~c[1] & c[6] & ~c[7]))) |
+                     (p01 & p67 & ((c[3] & ~c[2] & c[5] & ~c[4]) | (c[2] & ~c[3] & c[4] & ~c[5])));
+  return v - 2u * f - 4u * (uint32_t)popc(h & a);
+}
+
 __device__ __forceinline__ int cell_case(const CellPlanes& cp, int i) {
   int c = 0;
 #pragma unroll
@@ -195,39 +221,32 @@ __device__ __forceinline__ int edge_rank(uint64_t fx, uint64_t fy, uint64_t fz, 
   return r;
 }
 
-// Inverse of edge_rank: the k-th active edge of a word (k < count).
-__device__ __forceinline__ void edge_select(uint64_t fx, uint64_t fy, uint64_t fz, int k, int& i,
-                                            int& a) {
-  // largest i with (#edges at bits < i) <= k
-  int lo = 0, hi = 63;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    const uint64_t m = (1ull << mid) - 1ull;
-    const int c = popc(fx & m) + popc(fy & m) + popc(fz & m);
-    if (c <= k)
-      lo = mid;
-    else
-      hi = mid - 1;
+// Inverse of edge_rank: the k-th active edge of a word in (bit, axis x<y<z)
+// order (k < popc(fx) + popc(fy) + popc(fz)): the 32-bit half that holds
+// it, then halving steps, each counting the three flag sets below the split.
+__device__ __forceinline__ void edge_select(uint64_t fx, uint64_t fy, uint64_t fz, int k, int& bit,
+                                            int& axis) {
+  const int nlo = __popc((uint32_t)fx) + __popc((uint32_t)fy) + __popc((uint32_t)fz);
+  const bool up = k >= nlo;
+  uint32_t x = up ? (uint32_t)(fx >> 32) : (uint32_t)fx;
+  uint32_t y = up ? (uint32_t)(fy >> 32) : (uint32_t)fy;
+  uint32_t z = up ? (uint32_t)(fz >> 32) : (uint32_t)fz;
+  int r = up ? k - nlo : k, pos = up ? 32 : 0;
+#pragma unroll
+  for (int w = 16; w > 0; w >>= 1) {
+    const uint32_t m = (1u << w) - 1u;
+    const int c = __popc(x & m) + __popc(y & m) + __popc(z & m);
+    const bool hi = r >= c;
+    r = hi ? r - c : r;
+    x = hi ? x >> w : x;
+    y = hi ? y >> w : y;
+    z = hi ? z >> w : z;
+    pos += hi ? w : 0;
   }
-  i = lo;
-  const uint64_t m = (i == 0) ? 0ull : ((1ull << i) - 1ull);
-  int r = k - (popc(fx & m) + popc(fy & m) + popc(fz & m));
-  const int bx = (int)((fx >> i) & 1ull), by = (int)((fy >> i) & 1ull);
-  if (bx) {
-    if (r == 0) {
-      a = 0;
-      return;
-    }
-    --r;
-  }
-  if (by) {
-    if (r == 0) {
-      a = 1;
-      return;
-    }
-    --r;
-  }
-  a = 2;
+  // bit `pos` holds its set edges x, y, z in that order; r < their count
+  const int bx = (int)(x & 1u), by = (int)(y & 1u);
+  bit = pos;
+  axis = r < bx ? 0 : (r < bx + by ? 1 : 2);
 }
 
 // Decoupled look-back tile status: flag in bits 62-63 (0 = not ready,
@@ -249,10 +268,16 @@ struct Ctrl {
   unsigned long long total_t;
 };
 
-// Workspace layout for a word space of `nwords` row words.
+// Workspace layout for a word space of `nwords` row words.  off_counts
+// holds every word's exclusive vertex / triangle prefix within its count
+// tile of kCountTile words (v | t << 16: at most 128 * 192 and 127 * 320,
+// both below 2^16), off_tpre the exclusive prefix (v | t << 32) of every
+// count tile plus the totals after the last, so a word's bases are
+// tpre[w / kCountTile] + its local prefix.
 struct Workspace {
-  size_t off_bits, off_base_v, off_base_t, off_status, off_ctrl, off_counts, bytes;
-  int64_t nwords, ntiles;
+  size_t off_bits, off_base_v, off_base_t, off_status, off_ctrl, off_counts, off_tsum, off_tpre,
+      bytes;
+  int64_t nwords, ntiles, n_ctiles;
 };
 
 inline Workspace workspace_layout(int64_t nwords) {
@@ -270,8 +295,13 @@ inline Workspace workspace_layout(int64_t nwords) {
   off = off + (size_t)w.ntiles * 8;
   w.off_ctrl = off;
   off = align_up(off + sizeof(Ctrl), 256);
-  w.off_counts = off;  // packed per-word counts of the reduce-then-scan path
+  w.off_counts = off;  // packed per-word counts
   off = align_up(off + (size_t)w.ntiles * kTileWords * 4, 256);
+  w.n_ctiles = ceil_div(nwords, kCountTile);
+  w.off_tsum = off;  // per count tile sums
+  off = align_up(off + (size_t)(w.n_ctiles + 1) * 8, 256);
+  w.off_tpre = off;  // per count tile exclusive prefixes, then the totals
+  off = align_up(off + (size_t)(w.n_ctiles + 1) * 8, 256);
   w.bytes = off;
   return w;
 }

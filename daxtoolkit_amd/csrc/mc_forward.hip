@@ -54,158 +54,108 @@ __global__ __launch_bounds__(256) void k_classify_v4(VolGeom g, const float* __r
   classify_chunks_v4<UNROLL>(g, field + g.field_off, bits + g.word_off, c0, w_hi);
 }
 
-template <int NT, int WPT>
-__global__ __launch_bounds__(NT) void k_count_scan(VolGeom g, const uint64_t* __restrict__ bits,
-                                                   uint32_t* __restrict__ base_v,
-                                                   uint32_t* __restrict__ base_t,
-                                                   unsigned long long* __restrict__ status,
-                                                   Ctrl* __restrict__ ctrl, int64_t ntiles) {
-  __shared__ ScanShared sh;
-  __shared__ uint32_t s_pk[ScanTileLds<NT, WPT>::kSize];
-  __shared__ int64_t s_tile;
-  if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(&ctrl->tile_counter, 1u);
-  for (int i = threadIdx.x; i < 256; i += NT) sh.faces[i] = (uint8_t)(c_case_table[i] >> 60);
-  __syncthreads();
-  const int64_t tile = s_tile;
-  if (tile >= ntiles) return;  // scan state not reset by a classify launch
-  count_scan_tile<NT, WPT>(g, bits + g.word_off, base_v, base_t, status, ctrl, tile,
-                           tile * (int64_t)NT * WPT, sh, s_pk);
+// ---- count and scan --------------------------------------------------------
+// 1. k_count: a wave per two count tiles of kCountTile row words,
+//    consecutive lanes on consecutive words (coalesced loads of the word and
+//    its +x/+y/+z neighbours): every word's prefix within its tile and the
+//    tiles' sums.  No inter-tile dependency.
+// 2. k_scan_tsums: one block scans the tile sums into exclusive prefixes
+//    (the totals after the last) and the 64-bit totals of the control block.
+// 3. the bases of every word (tile prefix + local prefix): the generate
+//    kernel stores them for its words; k_bases stores them alone, for calls
+//    that classify without generating.
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_count(VolGeom g, const uint64_t* __restrict__ bits,
+                                                      uint32_t* __restrict__ lpre,
+                                                      unsigned long long* __restrict__ tsum,
+                                                      int64_t n_ctiles) {
+  const int64_t pair = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (2 * pair >= n_ctiles) return;
+  count_tile_pair(g, bits + g.word_off, lpre, tsum, pair, n_ctiles);
 }
 
-// ---- reduce-then-scan count/scan (tiles of kTileWords row words) ----------
-// 1. k_count_words: a thread per row word (consecutive threads on
-//    consecutive words): packed vertex / triangle counts, and the tile's sums
-//    in its status slot.  No inter-tile dependency.
-// 2. k_scan_tiles: one block scans the tiles' sums, continuing the totals
-//    already in the control block (so consecutive launches over consecutive
-//    tile ranges scan the whole volume), and stores each tile's prefix in
-//    its slot.
-// 3. k_apply_bases: a block per tile turns its prefix and the word counts
-//    into the vertex and triangle base of every word.
-constexpr int kCountThreads = 256;
-__global__ __launch_bounds__(kCountThreads) void k_count_words(VolGeom g,
-                                                               const uint64_t* __restrict__ b,
-                                                               uint32_t* __restrict__ counts,
-                                                               unsigned long long* __restrict__ slot,
-                                                               int64_t t_lo) {
-  __shared__ uint8_t faces[256];
-  __shared__ uint32_t s_v[kCountThreads / 64], s_t[kCountThreads / 64];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  for (int i = tid; i < 256; i += kCountThreads) faces[i] = (uint8_t)(c_case_table[i] >> 60);
-  __syncthreads();
-  const int64_t tile = t_lo + blockIdx.x;
-  const int64_t plane = (int64_t)g.wpr * g.ny;
-  const int wpr = g.wpr;
-  uint32_t sv = 0, st = 0;
+// One block of 16 waves; wave q scans a contiguous range of the tile sums
+// 64 at a time (coalesced loads, DPP scans of the 32-bit halves, 64-bit
+// carries): pass 1 sums the range, the block scans the 16 range sums, pass 2
+// writes the prefixes.  A tile's sums are below 2^16, so 64 of them fit the
+// 32-bit scans.
+constexpr int kTsumThreads = 1024;
+constexpr int kTsumUnroll = 16;  // 64-entry steps in flight per wave
+
+__global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long long* __restrict__ tsum,
+                                                             unsigned long long* __restrict__ tpre,
+                                                             int64_t n, Ctrl* __restrict__ ctrl) {
+  constexpr int kWaves = kTsumThreads / 64;
+  __shared__ unsigned long long s_v[kWaves], s_t[kWaves];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t span = ceil_div(ceil_div(n, kWaves), 64 * kTsumUnroll) * 64 * kTsumUnroll;
+  const int64_t r0 = min((int64_t)wv * span, n), r1 = min(r0 + span, n);
+  unsigned long long sv = 0, st = 0;
+  for (int64_t e = r0; e < r1; e += 64 * kTsumUnroll) {
+    unsigned long long x[kTsumUnroll];
 #pragma unroll
-  for (int u = 0; u < kTileWords / kCountThreads; ++u) {
-    const int64_t w = tile * kTileWords + u * kCountThreads + tid;
-    uint32_t pk = 0;
-    if (w < g.nwords) {
-      const WordPos p = word_pos(g, w);
-      const bool nx1 = p.q + 1 < wpr;
-      const bool hy = w + wpr < g.nwords, hz = w + plane < g.nwords,
-                 hyz = w + plane + wpr < g.nwords;
-      const uint64_t A = b[w], A1 = nx1 ? b[w + 1] : 0ull;
-      const uint64_t B = hy ? b[w + wpr] : 0ull, B1 = (hy && nx1) ? b[w + wpr + 1] : 0ull;
-      const uint64_t C = hz ? b[w + plane] : 0ull, C1 = (hz && nx1) ? b[w + plane + 1] : 0ull;
-      const uint64_t D = hyz ? b[w + plane + wpr] : 0ull;
-      const uint64_t D1 = (hyz && nx1) ? b[w + plane + wpr + 1] : 0ull;
-      uint32_t nv, nt;
-      word_counts_regs(g, p, A, A1, B, B1, C, C1, D, D1, faces, nv, nt);
-      pk = nv | (nt << 16);
-      sv += nv;
-      st += nt;
+    for (int u = 0; u < kTsumUnroll; ++u) {
+      const int64_t i = e + 64 * u + lane;
+      x[u] = i < r1 ? tsum[i] : 0ull;
     }
-    counts[tile * kTileWords + u * kCountThreads + tid] = pk;
+#pragma unroll
+    for (int u = 0; u < kTsumUnroll; ++u) {
+      sv += wave_sum((uint32_t)x[u]);
+      st += wave_sum((uint32_t)(x[u] >> 32));
+    }
   }
-  sv = wave_sum(sv);
-  st = wave_sum(st);
   if (lane == 0) {
     s_v[wv] = sv;
     s_t[wv] = st;
   }
   __syncthreads();
-  if (tid == 0) {
-    uint32_t tv = 0, tt = 0;
-    for (int q = 0; q < kCountThreads / 64; ++q) {
-      tv += s_v[q];
-      tt += s_t[q];
+  unsigned long long cv = 0, ct = 0, tv = 0, tt = 0;
+#pragma unroll
+  for (int q = 0; q < kWaves; ++q) {
+    if (q < wv) {
+      cv += s_v[q];
+      ct += s_t[q];
     }
-    slot[tile] = (unsigned long long)tv | ((unsigned long long)tt << 32);
+    tv += s_v[q];
+    tt += s_t[q];
+  }
+  for (int64_t e = r0; e < r1; e += 64 * kTsumUnroll) {
+    unsigned long long x[kTsumUnroll];
+#pragma unroll
+    for (int u = 0; u < kTsumUnroll; ++u) {
+      const int64_t i = e + 64 * u + lane;
+      x[u] = i < r1 ? tsum[i] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < kTsumUnroll; ++u) {
+      const uint32_t v = (uint32_t)x[u], t = (uint32_t)(x[u] >> 32);
+      const uint32_t iv = wave_incl_scan(v), it = wave_incl_scan(t);
+      const int64_t i = e + 64 * u + lane;
+      if (i < r1)
+        tpre[i] = ((cv + iv - v) & 0xffffffffull) | ((ct + it - t) << 32);
+      cv += wave_last(iv);
+      ct += wave_last(it);
+    }
+  }
+  if (threadIdx.x == 0) {
+    tpre[n] = (tv & 0xffffffffull) | (tt << 32);
+    ctrl->total_v = tv;
+    ctrl->total_t = tt;
   }
 }
 
-constexpr int kScanTileThreads = 1024;
-__global__ __launch_bounds__(kScanTileThreads) void k_scan_tiles(unsigned long long* __restrict__ slot,
-                                                                 int64_t t_lo, int64_t t_hi,
-                                                                 Ctrl* __restrict__ ctrl) {
-  __shared__ uint32_t s_w[32];
-  const int tid = threadIdx.x;
-  uint64_t run_v = ctrl->total_v, run_t = ctrl->total_t;
-  __syncthreads();  // every thread read the running totals before thread 0 updates them
-  for (int64_t base = t_lo; base < t_hi; base += kScanTileThreads) {
-    const int64_t t = base + tid;
-    const unsigned long long a = t < t_hi ? slot[t] : 0ull;
-    const uint32_t v = (uint32_t)(a & 0xffffffffull), tt = (uint32_t)(a >> 32);
-    uint32_t tot_v, tot_t;
-    const uint32_t ex_v = block_excl_scan(v, tot_v, s_w);
-    const uint32_t ex_t = block_excl_scan(tt, tot_t, s_w + 16);
-    if (t < t_hi)
-      slot[t] = (unsigned long long)(uint32_t)(run_v + ex_v) |
-                ((unsigned long long)(uint32_t)(run_t + ex_t) << 32);
-    run_v += tot_v;
-    run_t += tot_t;
-  }
-  if (tid == 0) {
-    ctrl->total_v = run_v;
-    ctrl->total_t = run_t;
-  }
-}
-
-__global__ __launch_bounds__(kCountThreads) void k_apply_bases(VolGeom g,
-                                                               const uint32_t* __restrict__ counts,
-                                                               const unsigned long long* __restrict__ slot,
-                                                               uint32_t* __restrict__ base_v,
-                                                               uint32_t* __restrict__ base_t,
-                                                               int64_t t_lo) {
-  __shared__ uint32_t s_w[32];
-  constexpr int WPT = kTileWords / kCountThreads;
-  const int tid = threadIdx.x;
-  const int64_t tile = t_lo + blockIdx.x;
-  const unsigned long long pre = slot[tile];
-  const int64_t w0 = tile * kTileWords + (int64_t)tid * WPT;
-  uint32_t pk[WPT];
-  uint32_t sv = 0, st = 0;
-#pragma unroll
-  for (int u = 0; u < WPT; ++u) {
-    pk[u] = counts[w0 + u];  // 0 beyond the last word
-    sv += pk[u] & 0xffffu;
-    st += pk[u] >> 16;
-  }
-  uint32_t tot_v, tot_t;
-  uint32_t rv = (uint32_t)(pre & 0xffffffffull) + block_excl_scan(sv, tot_v, s_w);
-  uint32_t rt = (uint32_t)(pre >> 32) + block_excl_scan(st, tot_t, s_w + 16);
-#pragma unroll
-  for (int u = 0; u < WPT; ++u) {
-    const int64_t w = w0 + u;
-    if (w < g.nwords) {
-      base_v[g.word_off + w] = rv;
-      base_t[g.word_off + w] = rt;
-    }
-    rv += pk[u] & 0xffffu;
-    rt += pk[u] >> 16;
-  }
-  if (w0 < g.nwords && g.nwords <= w0 + WPT) {  // owner of the last word: entry nwords
-    base_v[g.word_off + g.nwords] = rv;
-    base_t[g.word_off + g.nwords] = rt;
-  }
+// Bases of every row word (and entry nwords): tile prefix + local prefix.
+__global__ __launch_bounds__(256) void k_bases(VolGeom g, CountSrc cs) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w > g.nwords) return;
+  const unsigned long long p = word_bases(cs, w);
+  cs.base_v[g.word_off + w] = (uint32_t)p;
+  cs.base_t[g.word_off + w] = (uint32_t)(p >> 32);
 }
 
 template <int TW, class Src>
 __global__ __launch_bounds__(kScanThreads) void k_generate(
-    VolGeom g, Src field, const uint64_t* __restrict__ bits,
-    const uint32_t* __restrict__ base_v, const uint32_t* __restrict__ base_t,
+    VolGeom g, Src field, const uint64_t* __restrict__ bits, CountSrc cs,
     const Ctrl* __restrict__ ctrl, OutPtrs out, unsigned flags, int64_t cap_points,
     int64_t cap_tris, int64_t* __restrict__ counts, int xcd_order, int64_t tile_lo,
     int64_t ntiles) {
@@ -218,11 +168,13 @@ __global__ __launch_bounds__(kScanThreads) void k_generate(
     counts[0] = (int64_t)total_t;
     counts[1] = (int64_t)n_points;
   }
-  if (total_v > 0x7fffffffull || 3ull * total_t > 0x7fffffffull) return;
-  if ((int64_t)total_t > cap_tris || (int64_t)n_points > cap_points) return;
+  // too many ids for dax::Id or too little room: the bases only (the adjoint
+  // and the per-cell outputs read them), no output
+  const bool emit = total_v <= 0x7fffffffull && 3ull * total_t <= 0x7fffffffull &&
+                    (int64_t)total_t <= cap_tris && (int64_t)n_points <= cap_points;
   const int64_t tile = tile_lo + (xcd_order ? xcd_tile(blockIdx.x, ntiles) : (int64_t)blockIdx.x);
-  generate_tile<TW>(g, field, bits + g.word_off, base_v + g.word_off, base_t + g.word_off,
-                    tile * TW, out, dedup, normals, 0, 0, s_dyn);
+  generate_tile<TW, Src, true>(g, field, bits + g.word_off, nullptr, nullptr, tile * TW, out,
+                               dedup, normals, 0, 0, s_dyn, cs, emit);
 }
 
 // Slice classify: a wave per kPlaneRowsPerWave grid rows.  Along a row the
@@ -292,7 +244,7 @@ __global__ __launch_bounds__(256) void k_classify_plane(VolGeom g, FieldPlane pl
 struct Tuning {
   int cls_unroll = 16;  // words (vec 1) or 256-point chunks (vec 4) per wave
   int cls_vec = 1;      // floats per lane per load
-  int scan_sub = 4;     // scan tile = scan_sub * 1024 words
+  int scan_sub = 4;     // count tiles (waves) per count block
   int gen_tile = 512;   // generate tile words
   int gen_xcd = 0;      // 1: each XCD takes a contiguous range of generate tiles
 };
@@ -310,8 +262,7 @@ Tuning tuning_for(int64_t nx, int64_t ny, int64_t nz) {
 bool valid_tuning(const Tuning& t) {
   const bool cls = (t.cls_vec == 1 && (t.cls_unroll == 8 || t.cls_unroll == 16 || t.cls_unroll == 32)) ||
                    (t.cls_vec == 4 && (t.cls_unroll == 2 || t.cls_unroll == 4 || t.cls_unroll == 8));
-  const bool scan = t.scan_sub == 0 || t.scan_sub == 1 || t.scan_sub == 2 || t.scan_sub == 4 ||
-                    t.scan_sub == 8;
+  const bool scan = t.scan_sub == 1 || t.scan_sub == 2 || t.scan_sub == 4 || t.scan_sub == 8;
   const bool gen = (t.gen_tile == 128 || t.gen_tile == 256 || t.gen_tile == 512 ||
                     t.gen_tile == 1024) &&
                    (t.gen_xcd == 0 || t.gen_xcd == 1);
@@ -542,40 +493,39 @@ int launch_bits(const VolGeom& g, const float* field, char* ws, const Workspace&
   return DAX_HIP_OK;
 }
 
-// Count/scan tiles [t_lo, t_hi) (all when t_hi < 0).  Tiles take their ids
-// from the state's counter in launch order, so consecutive launches over
-// consecutive ranges cover the tiles in order.
+CountSrc count_src(const char* ws, const Workspace& L) {
+  char* wsm = const_cast<char*>(ws);
+  return CountSrc{(const uint32_t*)(ws + L.off_counts), (const unsigned long long*)(ws + L.off_tpre),
+                  L.n_ctiles, (uint32_t*)(wsm + L.off_base_v), (uint32_t*)(wsm + L.off_base_t)};
+}
+
+// Count + tile-sum scan over the whole volume (scan_sub waves per count
+// block); `bases` also stores every word's bases (calls that stop before
+// the generate kernel, which otherwise stores them).
 int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu, hipStream_t s,
-                int64_t t_lo = 0, int64_t t_hi = -1) {
-  const int sub = tu.scan_sub;  // scan tile = sub * 1024 row words; 0: reduce-then-scan
-  const int64_t ntiles = ceil_div(g.nwords, (int64_t)std::max(sub, 1) * kTileWords);
-  if (t_hi < 0) t_hi = ntiles;
-  const dim3 grid((unsigned)std::max<int64_t>(t_hi - t_lo, 1));
-  if (sub == 0) {
-    uint32_t* counts = (uint32_t*)(ws + L.off_counts);
-    unsigned long long* slot = (unsigned long long*)(ws + L.off_status);
-    k_count_words<<<grid, dim3(kCountThreads), 0, s>>>(g, (const uint64_t*)(ws + L.off_bits),
-                                                        counts, slot, t_lo);
-    DAXHIP_LAUNCH_CHECK("k_count_words");
-    k_scan_tiles<<<1, dim3(kScanTileThreads), 0, s>>>(slot, t_lo, t_hi, (Ctrl*)(ws + L.off_ctrl));
-    DAXHIP_LAUNCH_CHECK("k_scan_tiles");
-    k_apply_bases<<<grid, dim3(kCountThreads), 0, s>>>(g, counts, slot, (uint32_t*)(ws + L.off_base_v),
-                                                        (uint32_t*)(ws + L.off_base_t), t_lo);
-    DAXHIP_LAUNCH_CHECK("k_apply_bases");
-    return DAX_HIP_OK;
-  }
+                bool bases = false) {
   const uint64_t* bits = (const uint64_t*)(ws + L.off_bits);
-  uint32_t* bv = (uint32_t*)(ws + L.off_base_v);
-  uint32_t* bt = (uint32_t*)(ws + L.off_base_t);
-  unsigned long long* st = (unsigned long long*)(ws + L.off_status);
-  Ctrl* ctrl = (Ctrl*)(ws + L.off_ctrl);
-  switch (sub) {
-    case 1: k_count_scan<256, 4><<<grid, dim3(256), 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
-    case 2: k_count_scan<512, 4><<<grid, dim3(512), 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
-    case 8: k_count_scan<1024, 8><<<grid, dim3(1024), 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
-    default: k_count_scan<1024, 4><<<grid, dim3(1024), 0, s>>>(g, bits, bv, bt, st, ctrl, ntiles); break;
+  uint32_t* lpre = (uint32_t*)(ws + L.off_counts);
+  unsigned long long* tsum = (unsigned long long*)(ws + L.off_tsum);
+  unsigned long long* tpre = (unsigned long long*)(ws + L.off_tpre);
+  const int64_t n = L.n_ctiles;
+  if (n > 0) {
+    const int waves = tu.scan_sub;
+    const dim3 grid((unsigned)ceil_div(ceil_div(n, 2), waves)), block(64 * waves);
+    switch (waves) {
+      case 1: k_count<1><<<grid, block, 0, s>>>(g, bits, lpre, tsum, n); break;
+      case 2: k_count<2><<<grid, block, 0, s>>>(g, bits, lpre, tsum, n); break;
+      case 8: k_count<8><<<grid, block, 0, s>>>(g, bits, lpre, tsum, n); break;
+      default: k_count<4><<<grid, block, 0, s>>>(g, bits, lpre, tsum, n); break;
+    }
+    DAXHIP_LAUNCH_CHECK("k_count");
   }
-  DAXHIP_LAUNCH_CHECK("k_count_scan");
+  k_scan_tsums<<<1, dim3(kTsumThreads), 0, s>>>(tsum, tpre, n, (Ctrl*)(ws + L.off_ctrl));
+  DAXHIP_LAUNCH_CHECK("k_scan_tsums");
+  if (bases) {
+    k_bases<<<dim3((unsigned)ceil_div(g.nwords + 1, 256)), dim3(256), 0, s>>>(g, count_src(ws, L));
+    DAXHIP_LAUNCH_CHECK("k_bases");
+  }
   return DAX_HIP_OK;
 }
 
@@ -587,10 +537,10 @@ int launch_generate_tw(const VolGeom& g, const Src& field, const char* ws, const
   const size_t lds = GenLayout<TW>(g.wpr).bytes();
   if (t_hi < 0) t_hi = ceil_div(g.nwords, (int64_t)TW);
   const int64_t ntiles = std::max<int64_t>(t_hi - t_lo, 1);
+  const CountSrc cs = count_src(ws, L);
   k_generate<TW, Src><<<dim3((unsigned)ntiles), dim3(kScanThreads), lds, s>>>(
-      g, field, (const uint64_t*)(ws + L.off_bits), (const uint32_t*)(ws + L.off_base_v),
-      (const uint32_t*)(ws + L.off_base_t), (const Ctrl*)(ws + L.off_ctrl), o, flags, cap_points,
-      cap_tris, counts, xcd, t_lo, ntiles);
+      g, field, (const uint64_t*)(ws + L.off_bits), cs, (const Ctrl*)(ws + L.off_ctrl), o, flags,
+      cap_points, cap_tris, counts, xcd, t_lo, ntiles);
   DAXHIP_LAUNCH_CHECK("k_generate");
   return DAX_HIP_OK;
 }
@@ -735,7 +685,7 @@ int dax_hip_mc_classify(const float* d_field, int64_t nx, int64_t ny, int64_t nz
   char* ws = (char*)d_workspace;
   const Tuning tu = tuning_for(nx, ny, nz);
   if ((rc = launch_bits(g, d_field, ws, L, tu, s))) return rc;
-  if ((rc = launch_scan(g, ws, L, tu, s))) return rc;
+  if ((rc = launch_scan(g, ws, L, tu, s, true))) return rc;
   if (d_counts) {
     k_write_counts<<<1, 1, 0, s>>>((const Ctrl*)(ws + L.off_ctrl), d_counts, DAX_HIP_MC_DEDUP);
     DAXHIP_LAUNCH_CHECK("k_write_counts");
@@ -780,7 +730,7 @@ int dax_hip_mc_slice_classify(int64_t nx, int64_t ny, int64_t nz, const float* h
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)d_workspace;
   if ((rc = launch_bits_plane(g, pl, ws, L, s))) return rc;
-  if ((rc = launch_scan(g, ws, L, tuning_for(nx, ny, nz), s))) return rc;
+  if ((rc = launch_scan(g, ws, L, tuning_for(nx, ny, nz), s, true))) return rc;
   if (d_counts) {
     k_write_counts<<<1, 1, 0, s>>>((const Ctrl*)(ws + L.off_ctrl), d_counts, DAX_HIP_MC_DEDUP);
     DAXHIP_LAUNCH_CHECK("k_write_counts");
@@ -1035,7 +985,7 @@ int dax_hip_mc_autotune(const float* d_field, int64_t nx, int64_t ny, int64_t nz
     }
   }
   // scan variants (each timed launch follows a classify that resets its state)
-  for (int sub : {0, 1, 2, 4, 8}) {
+  for (int sub : {1, 2, 4, 8}) {
     Tuning t = best;
     t.scan_sub = sub;
     float total = 0.0f;

@@ -135,29 +135,68 @@ __device__ __forceinline__ void classify_chunks_v4(const VolGeom& g, const float
 // global memory (L2), counts, and the block scans the per-thread sums; wave 0
 // then runs the decoupled look-back for the tile's global prefix.
 struct ScanShared {
-  uint8_t faces[256];
   uint32_t wave_v[16], wave_t[16];
   uint32_t prefix[2];
 };
 
+// Active edges (vertices) and triangles of one row word from its eight
+// neighbour words (A = the word, 1 = the next word along x, B = +y row,
+// C = +z plane, D = +y+z).
 __device__ __forceinline__ void word_counts_regs(const VolGeom& g, const WordPos& p, uint64_t A,
                                                  uint64_t A1, uint64_t B, uint64_t B1, uint64_t C,
                                                  uint64_t C1, uint64_t D, uint64_t D1,
-                                                 const uint8_t* faces, uint32_t& nv,
-                                                 uint32_t& nt) {
+                                                 uint32_t& nv, uint32_t& nt) {
   uint64_t fx, fy, fz;
   edge_flags(p, A, A1, B, C, fx, fy, fz);
   nv = (uint32_t)(popc(fx) + popc(fy) + popc(fz));
   CellPlanes cp;
   cell_planes(p, A, A1, B, B1, C, C1, D, D1, cp);
-  uint64_t act = cp.active;
-  uint32_t t = 0;
-  while (act) {
-    const int bit = __builtin_ctzll(act);
-    act &= act - 1;
-    t += faces[cell_case(cp, bit)];
+  nt = cp.active ? word_triangles(cp) : 0u;
+}
+
+// Loads the eight neighbour words of row word w (volume-local) and counts it.
+__device__ __forceinline__ void word_counts(const VolGeom& g, const uint64_t* __restrict__ b,
+                                            int64_t w, uint32_t& nv, uint32_t& nt) {
+  const int64_t plane = (int64_t)g.wpr * g.ny;
+  const int wpr = g.wpr;
+  const WordPos p = word_pos(g, w);
+  const bool nx1 = p.q + 1 < wpr;
+  const bool hy = w + wpr < g.nwords, hz = w + plane < g.nwords, hyz = w + plane + wpr < g.nwords;
+  const uint64_t A = b[w], A1 = nx1 ? b[w + 1] : 0ull;
+  const uint64_t B = hy ? b[w + wpr] : 0ull, B1 = (hy && nx1) ? b[w + wpr + 1] : 0ull;
+  const uint64_t C = hz ? b[w + plane] : 0ull, C1 = (hz && nx1) ? b[w + plane + 1] : 0ull;
+  const uint64_t D = hyz ? b[w + plane + wpr] : 0ull;
+  const uint64_t D1 = (hyz && nx1) ? b[w + plane + wpr + 1] : 0ull;
+  word_counts_regs(g, p, A, A1, B, B1, C, C1, D, D1, nv, nt);
+}
+
+// Two count tiles of kCountTile row words, one wave: lane l counts words
+// 2*kCountTile*pair + 64u + l (u = 0, 1 the first tile, 2, 3 the second) and
+// writes each word's exclusive prefix within its tile (v | t << 16); lane 0
+// writes each tile's sums (v | t << 32).
+__device__ __forceinline__ void count_tile_pair(const VolGeom& g, const uint64_t* __restrict__ b,
+                                                uint32_t* __restrict__ lpre,
+                                                unsigned long long* __restrict__ tsum,
+                                                int64_t pair, int64_t n_ctiles) {
+  static_assert(kCountTile == 128, "two 64-word steps per count tile");
+  const int lane = __lane_id();
+  uint32_t rv = 0, rt = 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t w = pair * (2 * kCountTile) + u * 64 + lane;
+    uint32_t nv = 0, nt = 0;
+    if (w < g.nwords) word_counts(g, b, w, nv, nt);
+    const uint32_t iv = wave_incl_scan(nv), it = wave_incl_scan(nt);
+    if (pair * 2 + (u >> 1) < n_ctiles) lpre[w] = (rv + iv - nv) | ((rt + it - nt) << 16);
+    rv += wave_last(iv);
+    rt += wave_last(it);
+    if (u & 1) {  // end of a tile
+      const int64_t tile = pair * 2 + (u >> 1);
+      if (lane == 0 && tile < n_ctiles)
+        tsum[tile] = (unsigned long long)rv | ((unsigned long long)rt << 32);
+      rv = rt = 0;
+    }
   }
-  nt = t;
 }
 
 // Decoupled look-back by wave 0 over the tile states; returns the exclusive
@@ -257,7 +296,7 @@ __device__ __forceinline__ void count_scan_tile(const VolGeom& g, const uint64_t
       const uint64_t D = hyz ? b[w + plane + wpr] : 0ull;
       const uint64_t D1 = (hyz && nx1) ? b[w + plane + wpr + 1] : 0ull;
       uint32_t nv, nt;
-      word_counts_regs(g, p, A, A1, B, B1, C, C1, D, D1, sh.faces, nv, nt);
+      word_counts_regs(g, p, A, A1, B, B1, C, C1, D, D1, nv, nt);
       pk = nv | (nt << 16);
     }
     s_pk[pad32(i)] = pk;
@@ -444,46 +483,68 @@ __device__ __forceinline__ int find_word(const uint32_t* s, int n, uint32_t v) {
   return lo;
 }
 
-// Position of the k-th set bit of m (k < popcount(m)).
+// Position of the k-th set bit of m (k < popcount(m)): the half of m that
+// holds it, then halving steps on 32 bits.
 __device__ __forceinline__ int select_bit(uint64_t m, int k) {
-  int lo = 0, hi = 63;  // largest i with popc(m & ((1<<i)-1)) <= k
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (popc(m & ((1ull << mid) - 1ull)) <= k)
-      lo = mid;
-    else
-      hi = mid - 1;
+  const uint32_t lo = (uint32_t)m;
+  const int nlo = __popc(lo);
+  const bool up = k >= nlo;
+  uint32_t x = up ? (uint32_t)(m >> 32) : lo;
+  int r = up ? k - nlo : k, pos = up ? 32 : 0;
+#pragma unroll
+  for (int w = 16; w > 0; w >>= 1) {
+    const int c = __popc(x & ((1u << w) - 1u));
+    const bool hi = r >= c;
+    r = hi ? r - c : r;
+    x = hi ? x >> w : x;
+    pos += hi ? w : 0;
   }
-  return lo;
+  return pos;
 }
+
+// Edge codes of the 12 cell edges, 5 bits each (edge_code).
+__host__ __device__ constexpr uint64_t edge_codes_packed() {
+  uint64_t r = 0;
+  for (int e = 0; e < 12; ++e) r |= (uint64_t)edge_code(e) << (5 * e);
+  return r;
+}
+constexpr uint64_t kEdgeCodes = edge_codes_packed();
 
 // LDS layout of a generate tile of TW words (wpr known at run time).
 constexpr int kMaxGenThreads = 256;
 template <int TW>
 struct GenLayout {
   static constexpr int kRoundTris = 5 * kMaxGenThreads;  // triangles of one round of cells
-  int segb, segv, segf;
-  __host__ __device__ GenLayout(int wpr)
-      : segb(TW + 2 * wpr + 2), segv(TW + wpr + 2), segf(TW + wpr + 1) {}
-  // u64: bits (3 planes), case table, edge flags fx/fy/fz of the tile's words
-  // and the next row, active cells, packed positions; u32: vertex bases
+  int segb, segv;
+  __host__ __device__ GenLayout(int wpr) : segb(TW + 2 * wpr + 2), segv(TW + wpr + 2) {}
+  // u64: bits (3 planes), case table, active cells; u32: vertex bases
   // (2 planes), triangle bases, active-cell slots, scan scratch, round
-  // offsets and cell info, the round's triangle -> cell map
+  // offsets and cell info; u8: the round's triangle -> cell map
   __host__ __device__ size_t bytes() const {
-    return (size_t)(3 * segb + 256 + 3 * segf + 2 * TW) * 8 +
-           (size_t)(2 * segv + 2 * (TW + 1) + 32 + 2 * kMaxGenThreads + 1 + kRoundTris) * 4;
+    return (size_t)(3 * segb + 256 + TW) * 8 +
+           (size_t)(2 * segv + 2 * (TW + 1) + 32 + 2 * kMaxGenThreads + 1) * 4 + kRoundTris;
   }
 };
 
-// Word position of a word with cells: q < kMaxWpr (8 bits); every axis has
-// at least 2 points and the grid at most 2^30, so j, k < 2^28.
-__device__ __forceinline__ uint64_t pack_pos(int q, int j, int k) {
-  return (uint64_t)(uint32_t)q | ((uint64_t)(uint32_t)j << 8) | ((uint64_t)(uint32_t)k << 36);
+// Edge flags of the word at LDS index lw of plane sA (its +z plane sC) with
+// position p; branch-free (the +x word is read and masked).
+__device__ __forceinline__ void lds_flags(const VolGeom& g, const uint64_t* sA, const uint64_t* sC,
+                                          int lw, const WordPos& p, uint64_t& fx, uint64_t& fy,
+                                          uint64_t& fz) {
+  const uint64_t A = sA[lw], A1n = sA[lw + 1], B = sA[lw + g.wpr], C = sC[lw];
+  edge_flags(p, A, p.q + 1 < g.wpr ? A1n : 0ull, B, C, fx, fy, fz);
 }
-__device__ __forceinline__ void unpack_pos(uint64_t v, int& q, int& j, int& k) {
-  q = (int)(v & 0xffu);
-  j = (int)((v >> 8) & 0xfffffffu);
-  k = (int)(v >> 36);
+
+__device__ __forceinline__ WordPos make_pos(const VolGeom& g, int q, int j, int k) {
+  WordPos p;
+  p.q = q;
+  p.j = j;
+  p.k = k;
+  p.has_y = j + 1 < g.ny;
+  p.has_z = k + 1 < g.nz;
+  p.pmask = low_mask(g.nx - q * 64);
+  p.xmask = low_mask(g.nx - 1 - q * 64);
+  return p;
 }
 
 // Exclusive scan of one value per thread over the block (thread order);
@@ -503,11 +564,35 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total,
   return off + inc - v;
 }
 
+// Where the generate kernel of the single-volume forward takes its word
+// bases from: the count kernel's per-word prefixes within their count tiles
+// (lpre, v | t << 16) and the exclusive prefix of every count tile (tpre,
+// v | t << 32, n_ctiles + 1 entries, the totals last).  It writes the bases
+// of its words to base_v / base_t, which the adjoint and the per-cell
+// outputs read.
+struct CountSrc {
+  const uint32_t* __restrict__ lpre;
+  const unsigned long long* __restrict__ tpre;
+  int64_t n_ctiles;
+  uint32_t* __restrict__ base_v;
+  uint32_t* __restrict__ base_t;
+};
+
+// Vertex and triangle base of word w (v | t << 32); past the last count
+// tile the totals.
+__device__ __forceinline__ unsigned long long word_bases(const CountSrc& cs, int64_t w) {
+  const int64_t ct = w / kCountTile;
+  if (ct >= cs.n_ctiles) return cs.tpre[cs.n_ctiles];
+  const unsigned long long p = cs.tpre[ct];
+  const uint32_t l = cs.lpre[w];
+  return (unsigned long long)((uint32_t)p + (l & 0xffffu)) |
+         ((unsigned long long)((uint32_t)(p >> 32) + (l >> 16)) << 32);
+}
+
 // One generate tile: words [w_begin, w_begin + TW) of a volume.
 //   1. stage bits (planes k, k+1, k+2), vertex bases (k, k+1), triangle bases
-//   2. per word: edge flags (also of the next row, which the tile's cells
-//      reference), active-cell mask, position; block scan of the active-cell
-//      counts gives every active cell a slot
+//   2. per word: active-cell mask; block scan of the active-cell counts
+//      gives every active cell a slot
 //   3. one thread per output vertex: word by binary search over the vertex
 //      bases, edge by popcount select; coordinates, normal, records
 //   4. rounds of up to blockDim active cells: one thread per cell (word by
@@ -515,41 +600,59 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total,
 //      case; a block scan of NumFaces gives its first triangle and the cell
 //      marks its triangles' entries of the round's triangle -> cell map; one
 //      thread per triangle turns the triangle's edges into vertex ids through
-//      the edge flags (staged for the tile's words and the next row)
-template <int TW, class Src>
+//      the edge flags, formed where they are read from the staged bits
+//
+// FROM_COUNTS: the bases come from the count kernel's per-word prefixes and
+// count-tile prefixes (cs) instead of bv / bt; the tile writes the bases of
+// its words (and, for the last tile, entry nwords) to cs.base_v / base_t,
+// and with emit false only that.
+template <int TW, class Src, bool FROM_COUNTS = false>
 __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
                                               const uint64_t* __restrict__ b,
                                               const uint32_t* __restrict__ bv,
                                               const uint32_t* __restrict__ bt, int64_t w_begin,
                                               const OutPtrs& out, bool dedup, bool normals,
                                               int64_t vol_base_v, int64_t vol_base_t,
-                                              uint64_t* s_dyn) {
+                                              uint64_t* s_dyn, const CountSrc& cs = CountSrc{},
+                                              bool emit = true) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int nw = (int)min((int64_t)TW, g.nwords - w_begin);
-  const uint32_t tv0 = bv[w_begin], tv1 = bv[w_begin + nw];
-  const uint32_t tt0 = bt[w_begin], tt1 = bt[w_begin + nw];
-  if (tv0 == tv1 && tt0 == tt1) return;  // uniform: no output in this tile
+  if constexpr (FROM_COUNTS) {
+    // count tiles without output: constant bases, nothing else to do
+    const int64_t c0 = w_begin / kCountTile;
+    const int64_t c1 = min((w_begin + nw + kCountTile - 1) / kCountTile, cs.n_ctiles);
+    const unsigned long long p0 = cs.tpre[c0], p1 = cs.tpre[c1];
+    if (p0 == p1) {
+      const uint32_t v0 = (uint32_t)p0, t0 = (uint32_t)(p0 >> 32);
+      for (int i = tid; i < nw; i += nt) {
+        cs.base_v[w_begin + i] = v0;
+        cs.base_t[w_begin + i] = t0;
+      }
+      if (tid == 0 && w_begin + nw == g.nwords) {
+        cs.base_v[g.nwords] = v0;
+        cs.base_t[g.nwords] = t0;
+      }
+      return;
+    }
+  } else {
+    if (bv[w_begin] == bv[w_begin + nw] && bt[w_begin] == bt[w_begin + nw])
+      return;  // uniform: no output in this tile
+  }
 
   const GenLayout<TW> L(g.wpr);
   uint64_t* s_b0 = s_dyn;
   uint64_t* s_b1 = s_b0 + L.segb;
   uint64_t* s_b2 = s_b1 + L.segb;
   uint64_t* s_case = s_b2 + L.segb;
-  uint64_t* s_fx = s_case + 256;
-  uint64_t* s_fy = s_fx + L.segf;
-  uint64_t* s_fz = s_fy + L.segf;
-  uint64_t* s_act = s_fz + L.segf;
-  uint64_t* s_pos = s_act + TW;
-  uint32_t* s_v0 = reinterpret_cast<uint32_t*>(s_pos + TW);
+  uint64_t* s_act = s_case + 256;
+  uint32_t* s_v0 = reinterpret_cast<uint32_t*>(s_act + TW);
   uint32_t* s_v1 = s_v0 + L.segv;
   uint32_t* s_t = s_v1 + L.segv;
   uint32_t* s_cs = s_t + TW + 1;
-  uint32_t* s_w = s_cs + TW + 1;           // scan scratch (<= 16 waves)
-  uint32_t* s_rfirst = s_w + 32;            // [nt + 1]
-  uint32_t* s_rinfo = s_rfirst + nt + 1;    // [nt]: li << 16 | bit << 8 | case
-  uint32_t* s_tcell = s_rinfo + kMaxGenThreads;  // [kRoundTris]
-  uint64_t* s_bz[3] = {s_b0, s_b1, s_b2};
-  uint32_t* s_vz[2] = {s_v0, s_v1};
+  uint32_t* s_w = s_cs + TW + 1;                     // scan scratch (<= 16 waves)
+  uint32_t* s_rfirst = s_w + 32;                     // [nt + 1]
+  uint32_t* s_rinfo = s_rfirst + kMaxGenThreads + 1;  // [nt]: li << 16 | bit << 8 | case
+  uint8_t* s_tcell = reinterpret_cast<uint8_t*>(s_rinfo + kMaxGenThreads);  // [kRoundTris]
 
   const int64_t plane = (int64_t)g.wpr * g.ny;
   for (int i = tid; i < L.segb; i += nt) {
@@ -558,16 +661,39 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
     s_b1[i] = (w + plane < g.nwords) ? b[w + plane] : 0ull;
     s_b2[i] = (w + 2 * plane < g.nwords) ? b[w + 2 * plane] : 0ull;
   }
-  for (int i = tid; i < L.segv; i += nt) {
-    const int64_t w = w_begin + i;
-    s_v0[i] = (w <= g.nwords) ? bv[w] : 0u;
-    s_v1[i] = (w + plane <= g.nwords) ? bv[w + plane] : 0u;
+  if constexpr (FROM_COUNTS) {
+    for (int i = tid; i < L.segv; i += nt) {
+      const unsigned long long p0 = word_bases(cs, w_begin + i);
+      const unsigned long long p1 = word_bases(cs, w_begin + plane + i);
+      s_v0[i] = (uint32_t)p0;
+      s_v1[i] = (uint32_t)p1;
+      if (i <= nw) s_t[i] = (uint32_t)(p0 >> 32);
+    }
+  } else {
+    for (int i = tid; i < L.segv; i += nt) {
+      const int64_t w = w_begin + i;
+      s_v0[i] = (w <= g.nwords) ? bv[w] : 0u;
+      s_v1[i] = (w + plane <= g.nwords) ? bv[w + plane] : 0u;
+    }
+    for (int i = tid; i <= nw; i += nt) s_t[i] = bt[w_begin + i];
   }
-  for (int i = tid; i <= nw; i += nt) s_t[i] = bt[w_begin + i];
   for (int i = tid; i < 256; i += nt) s_case[i] = c_case_table[i];
   __syncthreads();
+  const uint32_t tv0 = s_v0[0], tv1 = s_v0[nw];
+  const uint32_t tt0 = s_t[0], tt1 = s_t[nw];
+  if constexpr (FROM_COUNTS) {
+    for (int i = tid; i < nw; i += nt) {
+      cs.base_v[w_begin + i] = s_v0[i];
+      cs.base_t[w_begin + i] = s_t[i];
+    }
+    if (tid == 0 && w_begin + nw == g.nwords) {
+      cs.base_v[g.nwords] = tv1;
+      cs.base_t[g.nwords] = tt1;
+    }
+    if (!emit || (tv0 == tv1 && tt0 == tt1)) return;  // bases only, or no output here
+  }
 
-  // 2. per word (each thread a run of consecutive words)
+  // 2. per word (each thread a run of consecutive words): active cells
   const int wpt = (TW + nt - 1) / nt;
   uint32_t ncell = 0;
   {
@@ -579,37 +705,15 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
         const int li = i0 + u;
         const WordPos p = it.pos(g);
         const bool nx1 = p.q + 1 < g.wpr;
-        const uint64_t A = s_b0[li], A1 = nx1 ? s_b0[li + 1] : 0ull;
-        const uint64_t B = s_b0[li + g.wpr], B1 = nx1 ? s_b0[li + g.wpr + 1] : 0ull;
-        const uint64_t C = s_b1[li], C1 = nx1 ? s_b1[li + 1] : 0ull;
-        const uint64_t D = s_b1[li + g.wpr], D1 = nx1 ? s_b1[li + g.wpr + 1] : 0ull;
-        uint64_t fx, fy, fz;
-        edge_flags(p, A, A1, B, C, fx, fy, fz);
         CellPlanes cp;
-        cell_planes(p, A, A1, B, B1, C, C1, D, D1, cp);
-        s_fx[li] = fx;
-        s_fy[li] = fy;
-        s_fz[li] = fz;
+        cell_planes(p, s_b0[li], nx1 ? s_b0[li + 1] : 0ull, s_b0[li + g.wpr],
+                    nx1 ? s_b0[li + g.wpr + 1] : 0ull, s_b1[li], nx1 ? s_b1[li + 1] : 0ull,
+                    s_b1[li + g.wpr], nx1 ? s_b1[li + g.wpr + 1] : 0ull, cp);
         s_act[li] = cp.active;
-        s_pos[li] = pack_pos(p.q, p.j, p.k);
         ncell += (uint32_t)popc(cp.active);
         it.next(g);
       }
     }
-  }
-  // edge flags of the words after the tile up to one row on (their vertices
-  // are corners of the tile's cells)
-  for (int li = nw + tid; li < nw + g.wpr + 1; li += nt) {
-    const int64_t w = w_begin + li;
-    uint64_t fx = 0ull, fy = 0ull, fz = 0ull;
-    if (w < g.nwords) {
-      const WordPos p = word_pos(g, w);
-      const bool nx1 = p.q + 1 < g.wpr;
-      edge_flags(p, s_b0[li], nx1 ? s_b0[li + 1] : 0ull, s_b0[li + g.wpr], s_b1[li], fx, fy, fz);
-    }
-    s_fx[li] = fx;
-    s_fy[li] = fy;
-    s_fz[li] = fz;
   }
   uint32_t n_cells;
   {
@@ -628,13 +732,14 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
   if (dedup) {
     for (int vl = tid; vl < nV; vl += nt) {
       const int li = find_word(s_v0, nw, tv0 + (uint32_t)vl);
+      const WordPos p = word_pos(g, w_begin + li);
+      uint64_t fx, fy, fz;
+      lds_flags(g, s_b0, s_b1, li, p, fx, fy, fz);
       int bit, axis;
-      edge_select(s_fx[li], s_fy[li], s_fz[li], (int)(tv0 + (uint32_t)vl - s_v0[li]), bit, axis);
-      int q, j, k;
-      unpack_pos(s_pos[li], q, j, k);
+      edge_select(fx, fy, fz, (int)(tv0 + (uint32_t)vl - s_v0[li]), bit, axis);
       float xyz[3], nrm[3], w;
       int64_t pa, pb;
-      emit_point(g, f, q * 64 + bit, j, k, axis, normals, xyz, nrm, w, pa, pb);
+      emit_point(g, f, p.q * 64 + bit, p.j, p.k, axis, normals, xyz, nrm, w, pa, pb);
       const int64_t gv = (int64_t)tv0 + vl;
       out.vertices[3 * gv + 0] = xyz[0];
       out.vertices[3 * gv + 1] = xyz[1];
@@ -659,17 +764,8 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
     if (slot < n_cells) {
       const int li = find_word(s_cs, nw, slot);
       const int bit = select_bit(s_act[li], (int)(slot - s_cs[li]));
-      int q, j, k;
-      unpack_pos(s_pos[li], q, j, k);
-      WordPos p;
-      p.q = q;
-      p.j = j;
-      p.k = k;
-      p.has_y = j + 1 < g.ny;
-      p.has_z = k + 1 < g.nz;
-      p.pmask = low_mask(g.nx - q * 64);
-      p.xmask = low_mask(g.nx - 1 - q * 64);
-      const bool nx1 = q + 1 < g.wpr;
+      const WordPos p = word_pos(g, w_begin + li);
+      const bool nx1 = p.q + 1 < g.wpr;
       CellPlanes cp;
       cell_planes(p, s_b0[li], nx1 ? s_b0[li + 1] : 0ull, s_b0[li + g.wpr],
                   nx1 ? s_b0[li + g.wpr + 1] : 0ull, s_b1[li], nx1 ? s_b1[li + 1] : 0ull,
@@ -680,7 +776,7 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
     }
     uint32_t round_tris;
     const uint32_t first = block_excl_scan(nf, round_tris, s_w);
-    for (uint32_t t = 0; t < nf; ++t) s_tcell[first + t] = (uint32_t)tid;  // <= 5 per cell
+    for (uint32_t t = 0; t < nf; ++t) s_tcell[first + t] = (uint8_t)tid;  // <= 5 per cell
     s_rfirst[tid] = first;
     __syncthreads();
     for (uint32_t tl = (uint32_t)tid; tl < round_tris; tl += (uint32_t)nt) {
@@ -688,45 +784,29 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
       const uint32_t info = s_rinfo[c];
       const int li = (int)(info >> 16), bit = (int)((info >> 8) & 0xff), cs = (int)(info & 0xff);
       const int visit = (int)(tl - s_rfirst[c]);
-      int q, j, k;
-      unpack_pos(s_pos[li], q, j, k);
+      const WordPos p = word_pos(g, w_begin + li);
       const uint64_t packed = s_case[cs];
       const int64_t t = (int64_t)tt0 + carry + tl;
       int32_t ids[3];
 #pragma unroll
       for (int v = 0; v < 3; ++v) {
         const int e = (int)((packed >> (4 * (3 * visit + v))) & 0xF);
-        const uint8_t code = edge_code(e);
+        const int code = (int)((kEdgeCodes >> (5 * e)) & 31u);
         const int dx = code & 1, dy = (code >> 1) & 1, dz = (code >> 2) & 1, axis = code >> 3;
-        const int x = q * 64 + bit + dx;
+        const int x = p.q * 64 + bit + dx;
         if (dedup) {
           const int qq = x >> 6, bb = x & 63;
-          const int lw2 = li + (qq - q) + dy * g.wpr;  // <= nw + wpr
+          const int lw2 = li + (qq - p.q) + dy * g.wpr;  // <= nw + wpr
+          const WordPos p2 = make_pos(g, qq, p.j + dy, p.k + dz);
           uint64_t fx, fy, fz;
-          if (dz == 0) {
-            fx = s_fx[lw2];
-            fy = s_fy[lw2];
-            fz = s_fz[lw2];
-          } else {
-            WordPos p2;
-            p2.q = qq;
-            p2.j = j + dy;
-            p2.k = k + dz;
-            p2.has_y = p2.j + 1 < g.ny;
-            p2.has_z = p2.k + 1 < g.nz;
-            p2.pmask = low_mask(g.nx - qq * 64);
-            p2.xmask = low_mask(g.nx - 1 - qq * 64);
-            const bool nx2 = qq + 1 < g.wpr;
-            edge_flags(p2, s_b1[lw2], nx2 ? s_b1[lw2 + 1] : 0ull, s_b1[lw2 + g.wpr], s_b2[lw2],
-                       fx, fy, fz);
-          }
-          ids[v] =
-              (int32_t)((int64_t)s_vz[dz][lw2] + edge_rank(fx, fy, fz, bb, axis) - vol_base_v);
+          lds_flags(g, dz ? s_b1 : s_b0, dz ? s_b2 : s_b1, lw2, p2, fx, fy, fz);
+          const uint32_t base = (dz ? s_v1 : s_v0)[lw2];
+          ids[v] = (int32_t)((int64_t)base + edge_rank(fx, fy, fz, bb, axis) - vol_base_v);
         } else {
           const int64_t corner = 3 * t + v;
           float xyz[3], nrm[3], w;
           int64_t pa, pb;
-          emit_point(g, f, x, j + dy, k + dz, axis, normals, xyz, nrm, w, pa, pb);
+          emit_point(g, f, x, p.j + dy, p.k + dz, axis, normals, xyz, nrm, w, pa, pb);
           out.vertices[3 * corner + 0] = xyz[0];
           out.vertices[3 * corner + 1] = xyz[1];
           out.vertices[3 * corner + 2] = xyz[2];

@@ -18,6 +18,8 @@
 //              coordinates at its rank (StreamCompact of the coordinates)
 // Counts may be any non-negative values (DispatcherGenerateTopology repeats
 // a cell count times); Threshold's are 0 or 1.
+#include <algorithm>
+
 #include "common.h"
 
 namespace daxhip {
@@ -682,6 +684,30 @@ __global__ __launch_bounds__(256) void k_compact_rank(
 // the point ranks (duplicate removal), [6] output cells, [7] used points.
 constexpr int kHeaderInts = 8;
 
+// Sum of the caller's per-cell counts in 64 bits (out[0]) and a flag for
+// negative counts (out[1]).  The int32 scan of the counts wraps past 2^31-1
+// without notice (three counts of 2^31-1 wrap to a positive total), so the
+// general path checks the exact sum before it scans.
+__global__ __launch_bounds__(256) void k_count_total_i64(const int32_t* __restrict__ counts,
+                                                         int64_t n,
+                                                         unsigned long long* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  unsigned long long sum = 0ull;
+  bool neg = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int32_t c = counts[i];
+    neg |= c < 0;
+    sum += c > 0 ? (unsigned long long)c : 0ull;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+  const bool any_neg = __ballot(neg) != 0ull;
+  if ((threadIdx.x & 63) == 0) {
+    if (sum) atomicAdd(&out[0], sum);
+    if (any_neg) atomicOr(&out[1], 1ull);
+  }
+}
+
 __global__ void k_set_sizes(int32_t* __restrict__ hdr, int32_t cells, int32_t points) {
   if (threadIdx.x == 0) {
     hdr[6] = cells;
@@ -819,16 +845,36 @@ int prepare(const Geometry& g, const int32_t* d_counts, unsigned flags, void* d_
       }
     }
   }
-  // any counts: per-cell offsets and per-point ranks
+  // any counts: per-cell offsets and per-point ranks.  The exact sum and the
+  // sign of every count are checked first: the scan below is int32.
+  {
+    TempBuf chk(s);
+    if ((rc = chk.alloc(2 * sizeof(unsigned long long)))) return rc;
+    unsigned long long* d_chk = (unsigned long long*)chk.p;
+    k_fill_i32<<<dim3(1), b, 0, s>>>((int32_t*)d_chk, 4, 0);
+    DAXHIP_LAUNCH_CHECK("k_fill_i32");
+    if (g.n_cells > 0) {
+      const int64_t blocks = std::min<int64_t>(ceil_div(g.n_cells, 256), 2048);
+      k_count_total_i64<<<dim3((unsigned)blocks), b, 0, s>>>(d_counts, g.n_cells, d_chk);
+      DAXHIP_LAUNCH_CHECK("k_count_total_i64");
+    }
+    unsigned long long h_chk[2] = {0ull, 0ull};
+    DAXHIP_CHECK(hipMemcpyAsync(h_chk, d_chk, sizeof(h_chk), hipMemcpyDeviceToHost, s));
+    DAXHIP_CHECK(hipStreamSynchronize(s));
+    if (h_chk[1]) {
+      set_error("topology prepare: a cell count is negative");
+      return DAX_HIP_BAD_VALUE;
+    }
+    if (h_chk[0] > 0x7fffffffull / (unsigned long long)std::max(g.nv, 1)) {
+      set_error("topology prepare: %llu output cells of %d vertices exceed 2^31 ids", h_chk[0],
+                g.nv);
+      return DAX_HIP_BAD_VALUE;
+    }
+  }
   int32_t* offs = (int32_t*)(ws + w.off_a);
   int32_t* rank = (int32_t*)(ws + w.off_b);
   int32_t total = 0;
   if ((rc = dax_hip_scan_exclusive_i32(d_counts, offs, g.n_cells, &total, s))) return rc;
-  if (total < 0 || (int64_t)total * g.nv > 0x7fffffffLL) {
-    set_error("topology prepare: %lld output cells of %d vertices exceed 2^31 ids",
-              (long long)total, g.nv);
-    return DAX_HIP_BAD_VALUE;
-  }
   *h_cells = total;
   if (!dedup) return DAX_HIP_OK;
   const dim3 gp((unsigned)ceil_div(g.n_points, 256));

@@ -44,6 +44,30 @@ def _check_field(field: torch.Tensor) -> tuple[int, int, int]:
     return int(nx), int(ny), int(nz)
 
 
+def check_tensor(t: Optional[torch.Tensor], name: str, dtype: torch.dtype, min_numel: int,
+                 device=None, exact: bool = False) -> None:
+    """Reject a tensor the kernels would index out of bounds or misread:
+    wrong dtype, not contiguous, fewer than `min_numel` elements (exactly
+    `min_numel` when `exact`), or not on `device` (a HIP device when None).
+    The kernels take raw pointers and index them by grid or contour ids, so
+    these checks run before every call."""
+    if t is None:
+        raise ValueError(f"{name} is required")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype} (got {t.dtype})")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    n = t.numel()
+    if (n != min_numel) if exact else (n < min_numel):
+        rel = "exactly" if exact else "at least"
+        raise ValueError(f"{name} must hold {rel} {min_numel} elements (got {n})")
+    if device is not None:
+        if t.device != torch.device(device):
+            raise ValueError(f"{name} must be on {device} (got {t.device})")
+    elif not t.is_cuda:
+        raise ValueError(f"{name} must be on a HIP device (got {t.device})")
+
+
 def workspace_bytes(nx: int, ny: int, nz: int) -> int:
     return int(_lib.load().dax_hip_mc_workspace_bytes(nx, ny, nz))
 
@@ -64,6 +88,7 @@ class Contour:
     interp_weight: Optional[torch.Tensor]
     workspace: Optional[torch.Tensor] = None   # inside-bits + vertex bases (for backward)
     counts: Optional[torch.Tensor] = None      # device {T, U}
+    grid_points: Optional[int] = None          # points of the input grid (interp ids index it)
 
     @property
     def num_triangles(self) -> int:
@@ -96,6 +121,7 @@ def read_counts(counts: torch.Tensor, cap_points: int = 2**62, cap_tris: int = 2
 def cell_outputs(field: torch.Tensor, workspace: torch.Tensor):
     """Per-cell case (u8), NumFaces (i32) and exclusive triangle offset (i32)."""
     nx, ny, nz = _check_field(field)
+    check_tensor(workspace, "workspace", torch.uint8, workspace_bytes(nx, ny, nz), field.device)
     C = max(nx - 1, 0) * max(ny - 1, 0) * max(nz - 1, 0)
     dev = field.device
     cases = torch.empty(C, dtype=torch.uint8, device=dev)
@@ -128,10 +154,22 @@ class ForwardBuffers:
         self.counts = torch.zeros(2, dtype=torch.int64, device=device)
 
 
+def _check_buf(buf: "ForwardBuffers", dims: tuple[int, int, int], device) -> None:
+    """The buffers were made for this grid and live on the field's device."""
+    if (buf.nx, buf.ny, buf.nz) != tuple(dims):
+        raise ValueError(f"buffers were made for a {buf.nx}x{buf.ny}x{buf.nz} grid, "
+                         f"not {dims[0]}x{dims[1]}x{dims[2]}")
+    check_tensor(buf.workspace, "workspace", torch.uint8, workspace_bytes(*dims), device)
+    check_tensor(buf.vertices, "vertices", torch.float32, 3 * buf.cap_points, device)
+    check_tensor(buf.conn, "connectivity", torch.int32, 3 * buf.cap_tris, device)
+    check_tensor(buf.counts, "counts", torch.int64, 2, device)
+
+
 def forward_into(field: torch.Tensor, iso: float, buf: ForwardBuffers, origin=(0, 0, 0),
                  spacing=(1, 1, 1), dedup: bool = True, normals: bool = True) -> None:
     """Enqueue one forward into preallocated buffers (no host synchronisation)."""
     nx, ny, nz = _check_field(field)
+    _check_buf(buf, (nx, ny, nz), field.device)
     flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and buf.normals is not None) else 0)
     _lib.call("dax_hip_mc_forward", field.data_ptr(), nx, ny, nz, _f3(origin), _f3(spacing),
               float(iso), flags, buf.workspace.data_ptr(), buf.workspace.numel(),
@@ -145,6 +183,7 @@ def classify_into(field: torch.Tensor, iso: float, buf: ForwardBuffers) -> None:
     """The forward's first kernel alone: inside-bits into buf.workspace (and
     the count/scan state cleared)."""
     nx, ny, nz = _check_field(field)
+    _check_buf(buf, (nx, ny, nz), field.device)
     _lib.call("dax_hip_mc_bits", field.data_ptr(), nx, ny, nz, float(iso), buf.workspace.data_ptr(),
               buf.workspace.numel(), _stream())
 
@@ -159,6 +198,7 @@ def generate_into(field: torch.Tensor, iso: float, buf: ForwardBuffers, origin=(
                   spacing=(1, 1, 1), dedup: bool = True, normals: bool = True) -> None:
     """The forward's third kernel alone (after classify and count/scan)."""
     nx, ny, nz = _check_field(field)
+    _check_buf(buf, (nx, ny, nz), field.device)
     flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and buf.normals is not None) else 0)
     _lib.call("dax_hip_mc_generate", field.data_ptr(), nx, ny, nz, _f3(origin), _f3(spacing),
               float(iso), flags, buf.workspace.data_ptr(), buf.workspace.numel(),
@@ -175,6 +215,7 @@ def forward_phases(field: torch.Tensor, iso: float, buf: ForwardBuffers, origin=
     torch.cuda.Event) is given they are recorded before, between and after
     the kernels on the current stream."""
     nx, ny, nz = _check_field(field)
+    _check_buf(buf, (nx, ny, nz), field.device)
     flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and buf.normals is not None) else 0)
     ws = buf.workspace
     s = _stream()
@@ -217,7 +258,7 @@ def forward(field: torch.Tensor, iso: float, origin=(0, 0, 0), spacing=(1, 1, 1)
         interp_first=None if buf.interp_first is None else buf.interp_first[:U],
         interp_second=None if buf.interp_second is None else buf.interp_second[:U],
         interp_weight=None if buf.interp_weight is None else buf.interp_weight[:U],
-        workspace=buf.workspace, counts=buf.counts)
+        workspace=buf.workspace, counts=buf.counts, grid_points=nx * ny * nz)
 
 
 def slice_into(dims, plane_origin, plane_normal, buf: ForwardBuffers, origin=(0, 0, 0),
@@ -263,7 +304,7 @@ def slice(dims, plane_origin, plane_normal, origin=(0, 0, 0), spacing=(1, 1, 1),
         interp_first=None if buf.interp_first is None else buf.interp_first[:U],
         interp_second=None if buf.interp_second is None else buf.interp_second[:U],
         interp_weight=None if buf.interp_weight is None else buf.interp_weight[:U],
-        workspace=buf.workspace, counts=buf.counts)
+        workspace=buf.workspace, counts=buf.counts, grid_points=nx * ny * nz)
 
 
 def _cells_contour(prefix: str, nv: int, cells: torch.Tensor, field: torch.Tensor,
@@ -301,7 +342,7 @@ def _cells_contour(prefix: str, nv: int, cells: torch.Tensor, field: torch.Tenso
     U = int(u.value)
     return Contour(vertices=verts[:U], connectivity=conn[:3 * T].view(T, 3), normals=None,
                    edges=None, interp_first=first[:U], interp_second=second[:U],
-                   interp_weight=weight[:U])
+                   interp_weight=weight[:U], grid_points=P)
 
 
 def tetrahedra(tets: torch.Tensor, field: torch.Tensor, points: torch.Tensor, iso: float,
@@ -335,12 +376,18 @@ def backward(field: torch.Tensor, iso: float, contour: Contour, dl_dvertices: to
     for t in (dl_dvertices, dl_dnormals):
         if t.shape != (U, 3) or t.dtype != torch.float32 or not t.is_contiguous():
             raise ValueError("upstream gradients must be contiguous float32 [U, 3]")
+    check_tensor(contour.edges, "edges", torch.int32, U, field.device)
+    check_tensor(contour.workspace, "workspace", torch.uint8, workspace_bytes(nx, ny, nz),
+                 field.device)
+    check_tensor(contour.counts, "counts", torch.int64, 2, field.device)
     lib = _lib.load()
     nbytes = int(lib.dax_hip_mc_backward_scratch_bytes(U))
     if scratch is None or scratch.numel() < nbytes:
         scratch = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=field.device)
+    check_tensor(scratch, "scratch", torch.uint8, nbytes, field.device)
     if out is None:
         out = torch.empty_like(field)
+    check_tensor(out, "out", torch.float32, nx * ny * nz, field.device, exact=True)
     _lib.call("dax_hip_mc_backward", field.data_ptr(), nx, ny, nz, _f3(origin), _f3(spacing),
               float(iso), contour.workspace.data_ptr(), contour.workspace.numel(),
               contour.edges.data_ptr(), contour.counts.data_ptr(), U, dl_dvertices.data_ptr(),
@@ -356,8 +403,16 @@ def backward_into(field: torch.Tensor, iso: float, buf: ForwardBuffers,
     with edges) with preallocated scratch and output (no synchronisation).
     Upstream gradients cover at least the forward's U points."""
     nx, ny, nz = _check_field(field)
+    _check_buf(buf, (nx, ny, nz), field.device)
     if buf.edges is None:
         raise ValueError("backward needs forward buffers with edges")
+    dev = field.device
+    check_tensor(buf.edges, "edges", torch.int32, buf.cap_points, dev)
+    check_tensor(dl_dvertices, "dl_dvertices", torch.float32, 3 * buf.cap_points, dev)
+    check_tensor(dl_dnormals, "dl_dnormals", torch.float32, 3 * buf.cap_points, dev)
+    check_tensor(scratch, "scratch", torch.uint8,
+                 int(_lib.load().dax_hip_mc_backward_scratch_bytes(buf.cap_points)), dev)
+    check_tensor(out, "out", torch.float32, nx * ny * nz, dev, exact=True)
     _lib.call("dax_hip_mc_backward", field.data_ptr(), nx, ny, nz, _f3(origin), _f3(spacing),
               float(iso), buf.workspace.data_ptr(), buf.workspace.numel(), buf.edges.data_ptr(),
               buf.counts.data_ptr(), buf.cap_points, dl_dvertices.data_ptr(),
@@ -401,7 +456,14 @@ def interpolate_scalar(values: torch.Tensor, contour: Contour) -> torch.Tensor:
     """CompactPointField of a scalar point field onto the contour points."""
     if contour.interp_first is None:
         raise ValueError("forward(..., interp=True) is needed to interpolate point fields")
+    if contour.grid_points is None:
+        raise ValueError("the contour does not record its grid's point count")
     n = contour.num_points
+    dev = contour.interp_first.device
+    check_tensor(values, "values", torch.float32, contour.grid_points, dev, exact=True)
+    for name in ("interp_first", "interp_second"):
+        check_tensor(getattr(contour, name), name, torch.int32, n, dev)
+    check_tensor(contour.interp_weight, "interp_weight", torch.float32, n, dev)
     out = torch.empty(n, dtype=torch.float32, device=values.device)
     _lib.call("dax_hip_mc_interpolate_scalar", values.data_ptr(), contour.interp_first.data_ptr(),
               contour.interp_second.data_ptr(), contour.interp_weight.data_ptr(), n,
@@ -477,10 +539,27 @@ class Batch:
     def sizes(self, fields, dedup=True):
         """Run once without room for output to learn {T, U}."""
         probe = self.alloc(0, 0, normals=False, edges=False)
+        self.check_buffers(fields, probe)
         self._call("dax_hip_mc_batch_forward", fields, probe, MC_DEDUP if dedup else 0)
         return read_counts(self.counts)
 
+    def check_buffers(self, fields, bufs, need_edges=False):
+        """The flat fields hold every volume and the outputs their capacities."""
+        dev = torch.device(self.device) if not isinstance(self.device, torch.device) else self.device
+        if dev.type == "cuda" and dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        check_tensor(fields, "fields", torch.float32, self.total_points, dev)
+        cp, ct = int(bufs["cap_points"]), int(bufs["cap_tris"])
+        check_tensor(bufs["vertices"], "vertices", torch.float32, 3 * cp, dev)
+        check_tensor(bufs["conn"], "conn", torch.int32, 3 * ct, dev)
+        if bufs.get("normals") is not None:
+            check_tensor(bufs["normals"], "normals", torch.float32, 3 * cp, dev)
+        if bufs.get("edges") is not None or need_edges:
+            check_tensor(bufs.get("edges"), "edges", torch.int32, cp, dev)
+        return dev
+
     def forward_into(self, fields, bufs, dedup=True, normals=True):
+        self.check_buffers(fields, bufs)
         flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and bufs.get("normals") is not None) else 0)
         if not dedup and bufs.get("edges") is not None:
             bufs = dict(bufs, edges=None)
@@ -491,6 +570,7 @@ class Batch:
         torch stream); returns the graph handle."""
         import ctypes
         flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and bufs.get("normals") is not None) else 0)
+        self.check_buffers(fields, bufs)
         h = ctypes.c_void_p()
         with torch.cuda.stream(stream):
             self._call("dax_hip_mc_batch_graph_create", fields, bufs, flags, extra=(ctypes.byref(h),))
@@ -512,11 +592,13 @@ class Batch:
         """dL/dfields (flat, like `fields`) of the last batch forward into
         `bufs` (run with edges) for upstream [U, 3] gradients."""
         U = bufs["cap_points"]
-        for t in (dl_dvertices, dl_dnormals):
-            if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() < 3 * U:
-                raise ValueError("upstream gradients must be contiguous float32 [U, 3]")
         if bufs.get("edges") is None:
             raise ValueError("the batch backward needs a forward with the edge list")
+        dev = self.check_buffers(fields, bufs, need_edges=True)
+        check_tensor(dl_dvertices, "dl_dvertices", torch.float32, 3 * U, dev)
+        check_tensor(dl_dnormals, "dl_dnormals", torch.float32, 3 * U, dev)
+        check_tensor(scratch, "scratch", torch.uint8, self.scratch_bytes(U), dev)
+        check_tensor(out, "out", torch.float32, self.total_points, dev)
         _lib.call("dax_hip_mc_batch_backward", fields.data_ptr(), self.header,
                   self.workspace.data_ptr(), self.workspace.numel(), bufs["edges"].data_ptr(),
                   self.counts.data_ptr(), U, dl_dvertices.data_ptr(), dl_dnormals.data_ptr(),
@@ -537,12 +619,23 @@ class Batch:
 
         import numpy as np
         p = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+        dev = self.check_buffers(fields, bufs)
+        need = max(int(_lib.load().dax_hip_mc_workspace_bytes(*map(int, d))) for d in self.dims)
+        check_tensor(workspace, "workspace", torch.uint8, need, dev)
+        check_tensor(counts_scratch, "counts_scratch", torch.int64, 2, dev)
         offs = np.ascontiguousarray(vol_offsets_host, dtype=np.int64).reshape(self.n, 2)
         flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and bufs.get("normals") is not None) else 0)
         dv = dn = sc = df = None
         sc_bytes = 0
         if grads is not None:
             dv, dn, scr, dfo = grads
+            if bufs.get("edges") is None:
+                raise ValueError("the loop backward needs forward buffers with the edge list")
+            U = int(bufs["cap_points"])
+            check_tensor(dv, "dl_dvertices", torch.float32, 3 * U, dev)
+            check_tensor(dn, "dl_dnormals", torch.float32, 3 * U, dev)
+            check_tensor(scr, "scratch", torch.uint8, self.scratch_bytes(U), dev)
+            check_tensor(dfo, "dl_dfields", torch.float32, self.total_points, dev)
             dv, dn, sc, df, sc_bytes = dv.data_ptr(), dn.data_ptr(), scr.data_ptr(), dfo.data_ptr(), scr.numel()
         _lib.call("dax_hip_mc_volume_loop", self.n, p(self.dims), p(self.field_offsets),
                   p(self.isos), p(self.origins), p(self.spacings), fields.data_ptr(), flags,

@@ -356,12 +356,15 @@ int dax_hip_write_vtk_unstructured(const char* path, const float* h_points, int6
 int dax_hip_mc_read_counts(const int64_t* d_counts, int64_t cap_points, int64_t cap_tris,
                            int64_t* h_tris, int64_t* h_points, void* stream);
 
-/* The three kernels of dax_hip_mc_forward as separate calls on one
- * workspace, in this order (each enqueues exactly one kernel):
- *   dax_hip_mc_bits      classify: field -> inside-bits (also resets the
- *                        scan state of the workspace)
- *   dax_hip_mc_scan      per-word counts + decoupled look-back scan
- *   dax_hip_mc_generate  outputs, as dax_hip_mc_forward (d_counts written)
+/* The three phases of dax_hip_mc_forward as separate calls on one
+ * workspace, in this order:
+ *   dax_hip_mc_bits      classify: field -> inside-bits (one kernel; also
+ *                        resets the scan state of the workspace)
+ *   dax_hip_mc_scan      per-word vertex / triangle counts, then the scan
+ *                        of the 256-word count tiles' sums (two kernels)
+ *   dax_hip_mc_generate  outputs, as dax_hip_mc_forward (d_counts written),
+ *                        and every row word's vertex / triangle base in the
+ *                        workspace (one kernel)
  * Together they replace DispatcherMapCell<MarchingCubesCount>
  * (dax/cont/DispatcherMapCell.h:27-64), the ScanInclusive/UpperBounds of
  * DispatcherGenerateInterpolatedCells.h:180-223 and its generate/resolve
@@ -400,9 +403,8 @@ int dax_hip_mc_cell_outputs(int64_t nx, int64_t ny, int64_t nz, const void* d_wo
  * params5 = {classify_unroll, classify_vec, scan_sub, gen_tile, gen_xcd}:
  *   classify_vec 1 (one float per lane, unroll 8/16/32 row words per wave)
  *   or 4 (float4 per lane, unroll 2/4/8 chunks of 256 points; rows of a
- *   multiple of 64 points); scan_sub 1/2/4/8 (single-pass scan with
- *   decoupled look-back over tiles of scan_sub*1024 row words) or 0
- *   (reduce-then-scan: per-word counts, one-block tile scan, bases); gen_tile
+ *   multiple of 64 points); scan_sub 1/2/4/8 (count tiles of 256 row words,
+ *   one wave each, per block of the count kernel); gen_tile
  *   128/256/512/1024 (row words per generate block); gen_xcd 1 gives each
  *   XCD a contiguous range of generate tiles, 0 keeps the plain block order.
  * A shape of (0,0,0) sets or gets the default for shapes not tuned. */

@@ -180,3 +180,81 @@ def test_batched_fwd_bwd_graph_replays():
                 assert torch.equal(bufs[k], r), f"replay {rep} {k}"
     finally:
         mc.Batch.destroy(g)
+
+
+def test_workload_1024_volumes():
+    """The batched workload at its own size: 1024 volumes with per-axis point
+    extents U{32..160}, 4-16 Gaussians each (sigma U[20,60]), iso U[0.3,0.7]
+    (seed 0).  The batched forward and forward+adjoint, eager and replayed
+    as one captured graph, equal the per-volume launch loop bit for bit over
+    all 1024 volumes; 16 volumes (the smallest and the largest among them)
+    equal the oracle: connectivity and coordinates bit for bit, normals
+    within 1e-5 of the oracle's."""
+    vb = fields.volume_batch(1024, seed=0)
+    print("[case] batched workload 1024 volumes", flush=True)
+    flat = fields.volume_batch_torch(vb)
+    b = mc.Batch(vb.dims, vb.isos)
+    T, U = b.sizes(flat)
+    assert T > 5_000_000 and U > 2_000_000
+    bufs = b.alloc(U, T)
+    dv, dn = _upstream(U, 7)
+    scratch = torch.empty(max(b.scratch_bytes(U), 1), dtype=torch.uint8, device="cuda")
+    dldf = torch.full_like(flat, float("nan"))
+    b.forward_into(flat, bufs)
+    b.backward(flat, bufs, dv, dn, scratch, dldf)
+    torch.cuda.synchronize()
+    assert mc.read_counts(b.counts) == (T, U)
+    assert not torch.isnan(dldf).any()
+    offs = b.vol_offsets.cpu().numpy()
+
+    # the same as one captured graph (fresh outputs), replayed twice
+    gbufs = b.alloc(U, T)
+    gout = torch.empty_like(flat)
+    gscratch = torch.empty_like(scratch)
+    s = torch.cuda.Stream()
+    g = mc.capture(lambda: (b.forward_into(flat, gbufs),
+                            b.backward(flat, gbufs, dv, dn, gscratch, gout)), s)
+    try:
+        for rep in range(2):
+            gout.fill_(float("nan"))
+            gbufs["conn"].fill_(-1)
+            torch.cuda.synchronize()
+            mc.Batch.launch(g, s)
+            s.synchronize()
+            for k, n in (("vertices", U), ("normals", U), ("conn", T), ("edges", U)):
+                assert torch.equal(gbufs[k][:n], bufs[k][:n]), f"graph replay {rep} {k}"
+            assert torch.equal(gout, dldf), f"graph replay {rep} adjoint"
+    finally:
+        mc.Batch.destroy(g)
+    del gbufs, gout, gscratch
+
+    # the per-volume launch loop
+    lbufs = b.alloc(U, T)
+    ldldf = torch.full_like(flat, float("nan"))
+    cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
+    b.loop(flat, lbufs, offs, b.loop_workspace(), cnt,
+           grads=(dv, dn, torch.empty_like(scratch), ldldf))
+    torch.cuda.synchronize()
+    for k, n in (("vertices", U), ("normals", U), ("conn", T), ("edges", U)):
+        assert torch.equal(bufs[k][:n], lbufs[k][:n]), k
+    assert torch.equal(dldf, ldldf)
+    del lbufs, ldldf
+
+    # oracle on a seeded sample, the smallest and the largest volume included
+    sizes = np.array([d[0] * d[1] * d[2] for d in vb.dims])
+    rng = np.random.default_rng(11)
+    sample = {int(np.argmin(sizes)), int(np.argmax(sizes))}
+    while len(sample) < 16:
+        sample.add(int(rng.integers(0, 1024)))
+    per = b.split(bufs)
+    for v in sorted(sample):
+        nx, ny, nz = vb.dims[v]
+        off = int(b.field_offsets[v])
+        host = flat[off:off + int(sizes[v])].cpu().numpy().reshape(nz, ny, nx)
+        verts, nrms, conn = per[v]
+        ref = orc.generate(host, vb.isos[v])
+        assert conn.shape[0] == ref["n_tris"] and verts.shape[0] == ref["n_pts"], v
+        assert np.array_equal(conn.cpu().numpy().reshape(-1), ref["conn"]), v
+        assert np.array_equal(verts.cpu().numpy().view(np.uint32), ref["coords"].view(np.uint32)), v
+        nref = orc.normals(host, ref["first"], ref["second"], ref["weight"])
+        assert np.allclose(nrms.cpu().numpy(), nref, rtol=1e-5, atol=1e-5), v

@@ -132,6 +132,25 @@ def test_magnitude_256_against_torch_definition():
     assert torch.equal(vals, f.reshape(-1)[used.reshape(-1)])
 
 
+def test_counts_whose_sum_wraps_int32_are_rejected():
+    """DispatcherGenerateTopology scans the counts in int32: three counts of
+    2^31-1 wrap to a positive total, and a negative count shifts every later
+    offset.  Both are rejected from the exact 64-bit sum before any cell is
+    emitted, on a uniform and on an unstructured grid."""
+    dims = (4, 2, 2)  # three voxels
+    hexes, pts = orc.voxel_hexes(dims)
+    cells = torch.from_numpy(np.ascontiguousarray(hexes, dtype=np.int32)).cuda()
+    points = torch.from_numpy(np.ascontiguousarray(pts, dtype=np.float32)).cuda()
+    for counts in ([2**31 - 1] * 3, [1, -1, 1], [2**28, 0, 0]):
+        c = torch.tensor(counts, dtype=torch.int32, device="cuda")
+        for dedup in (False, True):
+            print(f"[case] topology counts {counts} dedup={dedup}", flush=True)
+            with pytest.raises(_lib.ErrorControlBadValue):
+                topology.generate_topology(c, dims=dims, dedup=dedup)
+            with pytest.raises(_lib.ErrorControlBadValue):
+                topology.generate_topology(c, cells=cells, points=points, dedup=dedup)
+
+
 def test_bad_ids_are_rejected():
     pts = torch.zeros((4, 3), device="cuda")
     f = torch.zeros(4, device="cuda")
