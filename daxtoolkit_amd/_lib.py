@@ -60,6 +60,8 @@ _SIGNATURES = {
     "dax_hip_memset": (C.c_int, [vp, C.c_int, sz, vp]),
     "dax_hip_synchronize": (C.c_int, [vp]),
     "dax_hip_copy": (C.c_int, [vp, vp, sz, vp]),
+    "dax_hip_magnitude_vec3": (C.c_int, [vp, i64, vp, vp]),
+    "dax_hip_reduce_sum_i32": (C.c_int, [vp, i64, vp, vp]),
     "dax_hip_scan_inclusive_i32": (C.c_int, [vp, vp, i64, vp, vp]),
     "dax_hip_scan_exclusive_i32": (C.c_int, [vp, vp, i64, vp, vp]),
     "dax_hip_lower_bounds_i32": (C.c_int, [vp, i64, vp, i64, C.c_int, vp, vp, vp]),

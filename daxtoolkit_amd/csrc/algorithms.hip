@@ -656,7 +656,70 @@ using namespace daxhip::alg;
 
 #define DAXHIP_S(stream) ((hipStream_t)(stream))
 
+namespace daxhip {
+// Magnitude (dax/worklet/Magnitude.h:26-33): Sqrt(dot(v, v)) with the dot
+// product summed from component 0 (dax/math/VectorAnalysis.h:95-130,
+// dax/Types.h:611-621), one thread per vector.
+__global__ __launch_bounds__(256) void k_magnitude_vec3(const float* __restrict__ in, int64_t n,
+                                                        float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = in[3 * i], y = in[3 * i + 1], z = in[3 * i + 2];
+  float d = x * x;
+  d = d + y * y;
+  d = d + z * z;
+  out[i] = sqrtf(d);
+}
+
+// Sum of n int32 values in 64 bits (integer addition: the order of the
+// per-wave atomics does not change the result).
+__global__ __launch_bounds__(256) void k_sum_i32(const int32_t* __restrict__ in, int64_t n,
+                                                 unsigned long long* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  long long s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) s += in[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, (unsigned long long)s);
+}
+}  // namespace daxhip
+
 extern "C" {
+
+int dax_hip_reduce_sum_i32(const int32_t* d_in, int64_t n, int64_t* h_total, void* stream) {
+  if (n < 0 || (n > 0 && !d_in) || !h_total) {
+    daxhip::set_error("dax_hip_reduce_sum_i32: bad arguments");
+    return DAX_HIP_BAD_VALUE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  daxhip::TempBuf acc(s);
+  int rc;
+  if ((rc = acc.alloc(sizeof(unsigned long long)))) return rc;
+  DAXHIP_CHECK(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long), s));
+  if (n > 0) {
+    const int64_t blocks = std::min<int64_t>(daxhip::ceil_div(n, 256), 2048);
+    daxhip::k_sum_i32<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
+        d_in, n, (unsigned long long*)acc.p);
+    DAXHIP_LAUNCH_CHECK("k_sum_i32");
+  }
+  unsigned long long t = 0;
+  DAXHIP_CHECK(hipMemcpyAsync(&t, acc.p, sizeof(t), hipMemcpyDeviceToHost, s));
+  DAXHIP_CHECK(hipStreamSynchronize(s));
+  *h_total = (int64_t)t;
+  return DAX_HIP_OK;
+}
+
+int dax_hip_magnitude_vec3(const float* d_in, int64_t n, float* d_out, void* stream) {
+  if (n < 0 || (n > 0 && (!d_in || !d_out))) {
+    daxhip::set_error("dax_hip_magnitude_vec3: bad arguments");
+    return DAX_HIP_BAD_VALUE;
+  }
+  if (n == 0) return DAX_HIP_OK;
+  daxhip::k_magnitude_vec3<<<dim3((unsigned)daxhip::ceil_div(n, 256)), dim3(256), 0,
+                             (hipStream_t)stream>>>(d_in, n, d_out);
+  DAXHIP_LAUNCH_CHECK("k_magnitude_vec3");
+  return DAX_HIP_OK;
+}
 
 int dax_hip_copy(void* d_dst, const void* d_src, size_t bytes, void* stream) {
   if (bytes == 0) return DAX_HIP_OK;

@@ -271,12 +271,13 @@ struct Ctrl {
 // Workspace layout for a word space of `nwords` row words.  off_counts
 // holds every word's exclusive vertex / triangle prefix within its count
 // tile of kCountTile words (v | t << 16: at most 128 * 192 and 127 * 320,
-// both below 2^16), off_tpre the exclusive prefix (v | t << 32) of every
-// count tile plus the totals after the last, so a word's bases are
-// tpre[w / kCountTile] + its local prefix.
+// both below 2^16); off_tsum every count tile's exclusive prefix within its
+// count block (v | t << 32); off_bsum the count blocks' sums and off_tpre
+// their exclusive prefixes, the totals after the last.  A word's bases are
+// block prefix + tile prefix + local prefix.
 struct Workspace {
-  size_t off_bits, off_base_v, off_base_t, off_status, off_ctrl, off_counts, off_tsum, off_tpre,
-      bytes;
+  size_t off_bits, off_base_v, off_base_t, off_status, off_ctrl, off_counts, off_tsum, off_bsum,
+      off_tpre, bytes;
   int64_t nwords, ntiles, n_ctiles;
 };
 
@@ -298,9 +299,11 @@ inline Workspace workspace_layout(int64_t nwords) {
   w.off_counts = off;  // packed per-word counts
   off = align_up(off + (size_t)w.ntiles * kTileWords * 4, 256);
   w.n_ctiles = ceil_div(nwords, kCountTile);
-  w.off_tsum = off;  // per count tile sums
+  w.off_tsum = off;  // per count tile: exclusive prefix within its block
   off = align_up(off + (size_t)(w.n_ctiles + 1) * 8, 256);
-  w.off_tpre = off;  // per count tile exclusive prefixes, then the totals
+  w.off_bsum = off;  // per count block (at least one tile each): sums
+  off = align_up(off + (size_t)(w.n_ctiles + 1) * 8, 256);
+  w.off_tpre = off;  // per count block: exclusive prefixes, then the totals
   off = align_up(off + (size_t)(w.n_ctiles + 1) * 8, 256);
   w.bytes = off;
   return w;

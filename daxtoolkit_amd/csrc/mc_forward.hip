@@ -55,92 +55,86 @@ __global__ __launch_bounds__(256) void k_classify_v4(VolGeom g, const float* __r
 }
 
 // ---- count and scan --------------------------------------------------------
-// 1. k_count: a wave per two count tiles of kCountTile row words,
-//    consecutive lanes on consecutive words (coalesced loads of the word and
-//    its +x/+y/+z neighbours): every word's prefix within its tile and the
-//    tiles' sums.  No inter-tile dependency.
-// 2. k_scan_tsums: one block scans the tile sums into exclusive prefixes
+// 1. k_count: a block of WAVES waves, each wave two count tiles of
+//    kCountTile row words (four consecutive words per lane, coalesced loads
+//    of the words and their +x/+y/+z neighbours): every word's prefix within
+//    its tile, every tile's prefix within the block and the block's sums.
+//    No inter-block dependency.
+// 2. k_scan_tsums: one block scans the blocks' sums into exclusive prefixes
 //    (the totals after the last) and the 64-bit totals of the control block.
-// 3. the bases of every word (tile prefix + local prefix): the generate
+// 3. the bases of every word (block + tile + local prefix): the generate
 //    kernel stores them for its words; k_bases stores them alone, for calls
 //    that classify without generating.
 template <int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_count(VolGeom g, const uint64_t* __restrict__ bits,
                                                       uint32_t* __restrict__ lpre,
-                                                      unsigned long long* __restrict__ tsum,
+                                                      unsigned long long* __restrict__ tloc,
+                                                      unsigned long long* __restrict__ bsum,
                                                       int64_t n_ctiles) {
-  const int64_t pair = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
-  if (2 * pair >= n_ctiles) return;
-  count_tile_pair(g, bits + g.word_off, lpre, tsum, pair, n_ctiles);
-}
-
-// One block of 16 waves; wave q scans a contiguous range of the tile sums
-// 64 at a time (coalesced loads, DPP scans of the 32-bit halves, 64-bit
-// carries): pass 1 sums the range, the block scans the 16 range sums, pass 2
-// writes the prefixes.  A tile's sums are below 2^16, so 64 of them fit the
-// 32-bit scans.
-constexpr int kTsumThreads = 1024;
-constexpr int kTsumUnroll = 16;  // 64-entry steps in flight per wave
-
-__global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long long* __restrict__ tsum,
-                                                             unsigned long long* __restrict__ tpre,
-                                                             int64_t n, Ctrl* __restrict__ ctrl) {
-  constexpr int kWaves = kTsumThreads / 64;
-  __shared__ unsigned long long s_v[kWaves], s_t[kWaves];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t span = ceil_div(ceil_div(n, kWaves), 64 * kTsumUnroll) * 64 * kTsumUnroll;
-  const int64_t r0 = min((int64_t)wv * span, n), r1 = min(r0 + span, n);
-  unsigned long long sv = 0, st = 0;
-  for (int64_t e = r0; e < r1; e += 64 * kTsumUnroll) {
-    unsigned long long x[kTsumUnroll];
-#pragma unroll
-    for (int u = 0; u < kTsumUnroll; ++u) {
-      const int64_t i = e + 64 * u + lane;
-      x[u] = i < r1 ? tsum[i] : 0ull;
-    }
-#pragma unroll
-    for (int u = 0; u < kTsumUnroll; ++u) {
-      sv += wave_sum((uint32_t)x[u]);
-      st += wave_sum((uint32_t)(x[u] >> 32));
-    }
-  }
-  if (lane == 0) {
-    s_v[wv] = sv;
-    s_t[wv] = st;
+  __shared__ unsigned long long s_ts[2 * WAVES];
+  const int wv = threadIdx.x >> 6;
+  const int64_t pair = (int64_t)blockIdx.x * WAVES + wv;
+  unsigned long long s0 = 0ull, s1 = 0ull;
+  if (2 * pair < n_ctiles) count_tile_pair(g, bits + g.word_off, lpre, pair, n_ctiles, s0, s1);
+  if ((threadIdx.x & 63) == 0) {
+    s_ts[2 * wv] = s0;
+    s_ts[2 * wv + 1] = s1;
   }
   __syncthreads();
-  unsigned long long cv = 0, ct = 0, tv = 0, tt = 0;
-#pragma unroll
-  for (int q = 0; q < kWaves; ++q) {
-    if (q < wv) {
-      cv += s_v[q];
-      ct += s_t[q];
-    }
-    tv += s_v[q];
-    tt += s_t[q];
-  }
-  for (int64_t e = r0; e < r1; e += 64 * kTsumUnroll) {
-    unsigned long long x[kTsumUnroll];
-#pragma unroll
-    for (int u = 0; u < kTsumUnroll; ++u) {
-      const int64_t i = e + 64 * u + lane;
-      x[u] = i < r1 ? tsum[i] : 0ull;
-    }
-#pragma unroll
-    for (int u = 0; u < kTsumUnroll; ++u) {
-      const uint32_t v = (uint32_t)x[u], t = (uint32_t)(x[u] >> 32);
-      const uint32_t iv = wave_incl_scan(v), it = wave_incl_scan(t);
-      const int64_t i = e + 64 * u + lane;
-      if (i < r1)
-        tpre[i] = ((cv + iv - v) & 0xffffffffull) | ((ct + it - t) << 32);
-      cv += wave_last(iv);
-      ct += wave_last(it);
-    }
-  }
   if (threadIdx.x == 0) {
-    tpre[n] = (tv & 0xffffffffull) | (tt << 32);
-    ctrl->total_v = tv;
-    ctrl->total_t = tt;
+    unsigned long long run = 0ull;
+    for (int i = 0; i < 2 * WAVES; ++i) {
+      const int64_t tile = (int64_t)blockIdx.x * 2 * WAVES + i;
+      if (tile < n_ctiles) tloc[tile] = run;
+      run = add_vt(run, s_ts[i]);
+    }
+    bsum[blockIdx.x] = run;
+  }
+}
+
+// One block scans the n count-block sums (v | t << 32, each below 2^32)
+// 1024 at a time: a coalesced load, wave scans of the halves, the block's
+// wave sums in 64 bits, 64-bit carries between rounds.
+constexpr int kTsumThreads = 1024;
+__global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long long* __restrict__ bsum,
+                                                             unsigned long long* __restrict__ bpre,
+                                                             int64_t n, Ctrl* __restrict__ ctrl) {
+  constexpr int kWaves = kTsumThreads / 64;
+  __shared__ uint32_t s_v[kWaves], s_t[kWaves];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  unsigned long long cv = 0, ct = 0;  // 64-bit carries
+  for (int64_t e0 = 0; e0 < n; e0 += kTsumThreads) {
+    const int64_t e = e0 + tid;
+    const unsigned long long x = e < n ? bsum[e] : 0ull;
+    const uint32_t v = (uint32_t)x, t = (uint32_t)(x >> 32);
+    // a round's sums stay below 2^32 only for the wave partials; the
+    // cross-wave totals are formed in 64 bits
+    const uint32_t iv = wave_incl_scan(v), it = wave_incl_scan(t);
+    if (lane == 63) {
+      s_v[wv] = iv;
+      s_t[wv] = it;
+    }
+    __syncthreads();
+    unsigned long long ov = 0, ot = 0, rv = 0, rt = 0;
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) {
+      if (q < wv) {
+        ov += s_v[q];
+        ot += s_t[q];
+      }
+      rv += s_v[q];
+      rt += s_t[q];
+    }
+    __syncthreads();
+    if (e < n)
+      bpre[e] = ((cv + ov + iv - v) & 0xffffffffull) | ((ct + ot + it - t) << 32);
+    cv += rv;
+    ct += rt;
+  }
+  if (tid == 0) {
+    bpre[n] = (cv & 0xffffffffull) | (ct << 32);
+    ctrl->total_v = cv;
+    ctrl->total_t = ct;
   }
 }
 
@@ -493,37 +487,43 @@ int launch_bits(const VolGeom& g, const float* field, char* ws, const Workspace&
   return DAX_HIP_OK;
 }
 
-CountSrc count_src(const char* ws, const Workspace& L) {
+CountSrc count_src(const char* ws, const Workspace& L, const Tuning& tu) {
   char* wsm = const_cast<char*>(ws);
-  return CountSrc{(const uint32_t*)(ws + L.off_counts), (const unsigned long long*)(ws + L.off_tpre),
-                  L.n_ctiles, (uint32_t*)(wsm + L.off_base_v), (uint32_t*)(wsm + L.off_base_t)};
+  return CountSrc{(const uint32_t*)(ws + L.off_counts),
+                  (const unsigned long long*)(ws + L.off_tsum),
+                  (const unsigned long long*)(ws + L.off_tpre), L.n_ctiles, 2 * tu.scan_sub,
+                  (uint32_t*)(wsm + L.off_base_v), (uint32_t*)(wsm + L.off_base_t)};
 }
 
-// Count + tile-sum scan over the whole volume (scan_sub waves per count
-// block); `bases` also stores every word's bases (calls that stop before
-// the generate kernel, which otherwise stores them).
+// Count + block-sum scan over the whole volume (scan_sub waves, i.e.
+// 2 * scan_sub count tiles, per count block); `bases` also stores every
+// word's bases (calls that stop before the generate kernel, which otherwise
+// stores them).
 int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu, hipStream_t s,
                 bool bases = false) {
   const uint64_t* bits = (const uint64_t*)(ws + L.off_bits);
   uint32_t* lpre = (uint32_t*)(ws + L.off_counts);
-  unsigned long long* tsum = (unsigned long long*)(ws + L.off_tsum);
-  unsigned long long* tpre = (unsigned long long*)(ws + L.off_tpre);
+  unsigned long long* tloc = (unsigned long long*)(ws + L.off_tsum);
+  unsigned long long* bsum = (unsigned long long*)(ws + L.off_bsum);
+  unsigned long long* bpre = (unsigned long long*)(ws + L.off_tpre);
   const int64_t n = L.n_ctiles;
-  if (n > 0) {
-    const int waves = tu.scan_sub;
-    const dim3 grid((unsigned)ceil_div(ceil_div(n, 2), waves)), block(64 * waves);
+  const int waves = tu.scan_sub;
+  const int64_t nblocks = ceil_div(n, 2 * waves);
+  if (nblocks > 0) {
+    const dim3 grid((unsigned)nblocks), block(64 * waves);
     switch (waves) {
-      case 1: k_count<1><<<grid, block, 0, s>>>(g, bits, lpre, tsum, n); break;
-      case 2: k_count<2><<<grid, block, 0, s>>>(g, bits, lpre, tsum, n); break;
-      case 8: k_count<8><<<grid, block, 0, s>>>(g, bits, lpre, tsum, n); break;
-      default: k_count<4><<<grid, block, 0, s>>>(g, bits, lpre, tsum, n); break;
+      case 1: k_count<1><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, n); break;
+      case 2: k_count<2><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, n); break;
+      case 8: k_count<8><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, n); break;
+      default: k_count<4><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, n); break;
     }
     DAXHIP_LAUNCH_CHECK("k_count");
   }
-  k_scan_tsums<<<1, dim3(kTsumThreads), 0, s>>>(tsum, tpre, n, (Ctrl*)(ws + L.off_ctrl));
+  k_scan_tsums<<<1, dim3(kTsumThreads), 0, s>>>(bsum, bpre, nblocks, (Ctrl*)(ws + L.off_ctrl));
   DAXHIP_LAUNCH_CHECK("k_scan_tsums");
   if (bases) {
-    k_bases<<<dim3((unsigned)ceil_div(g.nwords + 1, 256)), dim3(256), 0, s>>>(g, count_src(ws, L));
+    k_bases<<<dim3((unsigned)ceil_div(g.nwords + 1, 256)), dim3(256), 0, s>>>(
+        g, count_src(ws, L, tu));
     DAXHIP_LAUNCH_CHECK("k_bases");
   }
   return DAX_HIP_OK;
@@ -533,11 +533,13 @@ int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu
 template <int TW, class Src>
 int launch_generate_tw(const VolGeom& g, const Src& field, const char* ws, const Workspace& L,
                        unsigned flags, const OutPtrs& o, int64_t cap_points, int64_t cap_tris,
-                       int64_t* counts, int xcd, hipStream_t s, int64_t t_lo, int64_t t_hi) {
+                       int64_t* counts, const Tuning& tu, hipStream_t s, int64_t t_lo,
+                       int64_t t_hi) {
+  const int xcd = tu.gen_xcd;
   const size_t lds = GenLayout<TW>(g.wpr).bytes();
   if (t_hi < 0) t_hi = ceil_div(g.nwords, (int64_t)TW);
   const int64_t ntiles = std::max<int64_t>(t_hi - t_lo, 1);
-  const CountSrc cs = count_src(ws, L);
+  const CountSrc cs = count_src(ws, L, tu);
   k_generate<TW, Src><<<dim3((unsigned)ntiles), dim3(kScanThreads), lds, s>>>(
       g, field, (const uint64_t*)(ws + L.off_bits), cs, (const Ctrl*)(ws + L.off_ctrl), o, flags,
       cap_points, cap_tris, counts, xcd, t_lo, ntiles);
@@ -553,16 +555,16 @@ int launch_generate_src(const VolGeom& g, const Src& field, const char* ws, cons
   switch (tu.gen_tile) {
     case 128:
       return launch_generate_tw<128>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
-                                     tu.gen_xcd, s, t_lo, t_hi);
+                                     tu, s, t_lo, t_hi);
     case 256:
       return launch_generate_tw<256>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
-                                     tu.gen_xcd, s, t_lo, t_hi);
+                                     tu, s, t_lo, t_hi);
     case 1024:
       return launch_generate_tw<1024>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
-                                      tu.gen_xcd, s, t_lo, t_hi);
+                                      tu, s, t_lo, t_hi);
     default:
       return launch_generate_tw<512>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
-                                     tu.gen_xcd, s, t_lo, t_hi);
+                                     tu, s, t_lo, t_hi);
   }
 }
 

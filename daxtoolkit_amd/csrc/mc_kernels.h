@@ -170,32 +170,67 @@ __device__ __forceinline__ void word_counts(const VolGeom& g, const uint64_t* __
   word_counts_regs(g, p, A, A1, B, B1, C, C1, D, D1, nv, nt);
 }
 
-// Two count tiles of kCountTile row words, one wave: lane l counts words
-// 2*kCountTile*pair + 64u + l (u = 0, 1 the first tile, 2, 3 the second) and
-// writes each word's exclusive prefix within its tile (v | t << 16); lane 0
-// writes each tile's sums (v | t << 32).
+// Two count tiles of kCountTile row words, one wave: lane l counts the four
+// consecutive words 2*kCountTile*pair + 4l .. +3 (lanes 0-31 the first
+// tile, 32-63 the second), reusing each loaded word as its predecessor's +x
+// neighbour, and writes each word's exclusive prefix within its tile
+// (v | t << 16).  Returns the two tiles' sums (v | t << 32, wave-uniform).
 __device__ __forceinline__ void count_tile_pair(const VolGeom& g, const uint64_t* __restrict__ b,
-                                                uint32_t* __restrict__ lpre,
-                                                unsigned long long* __restrict__ tsum,
-                                                int64_t pair, int64_t n_ctiles) {
-  static_assert(kCountTile == 128, "two 64-word steps per count tile");
+                                                uint32_t* __restrict__ lpre, int64_t pair,
+                                                int64_t n_ctiles, unsigned long long& sum0,
+                                                unsigned long long& sum1) {
+  static_assert(kCountTile == 128, "32 lanes of four words per count tile");
   const int lane = __lane_id();
-  uint32_t rv = 0, rt = 0;
+  const int64_t plane = (int64_t)g.wpr * g.ny;
+  const int wpr = g.wpr;
+  const int64_t w0 = pair * (2 * kCountTile) + 4 * lane;
+  uint64_t A[5], B[5], C[5], D[5];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int64_t w = pair * (2 * kCountTile) + u * 64 + lane;
-    uint32_t nv = 0, nt = 0;
-    if (w < g.nwords) word_counts(g, b, w, nv, nt);
-    const uint32_t iv = wave_incl_scan(nv), it = wave_incl_scan(nt);
-    if (pair * 2 + (u >> 1) < n_ctiles) lpre[w] = (rv + iv - nv) | ((rt + it - nt) << 16);
-    rv += wave_last(iv);
-    rt += wave_last(it);
-    if (u & 1) {  // end of a tile
-      const int64_t tile = pair * 2 + (u >> 1);
-      if (lane == 0 && tile < n_ctiles)
-        tsum[tile] = (unsigned long long)rv | ((unsigned long long)rt << 32);
-      rv = rt = 0;
+  for (int i = 0; i < 5; ++i) {
+    const int64_t w = w0 + i;
+    A[i] = w < g.nwords ? b[w] : 0ull;
+    B[i] = w + wpr < g.nwords ? b[w + wpr] : 0ull;
+    C[i] = w + plane < g.nwords ? b[w + plane] : 0ull;
+    D[i] = w + plane + wpr < g.nwords ? b[w + plane + wpr] : 0ull;
+  }
+  uint32_t nv[4] = {0u, 0u, 0u, 0u}, nt[4] = {0u, 0u, 0u, 0u};
+  if (w0 < g.nwords) {
+    WordIter wi;
+    wi.init(g, (int)w0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (w0 + i < g.nwords) {
+        const WordPos p = wi.pos(g);
+        const bool nx1 = p.q + 1 < wpr;
+        word_counts_regs(g, p, A[i], nx1 ? A[i + 1] : 0ull, B[i], nx1 ? B[i + 1] : 0ull, C[i],
+                         nx1 ? C[i + 1] : 0ull, D[i], nx1 ? D[i + 1] : 0ull, nv[i], nt[i]);
+        wi.next(g);
+      }
     }
+  }
+  const uint32_t sv = nv[0] + nv[1] + nv[2] + nv[3], st = nt[0] + nt[1] + nt[2] + nt[3];
+  uint32_t iv = wave_incl_scan(sv), it = wave_incl_scan(st);
+  const uint32_t hv = (uint32_t)__builtin_amdgcn_readlane((int)iv, 31);
+  const uint32_t ht = (uint32_t)__builtin_amdgcn_readlane((int)it, 31);
+  const uint32_t ev = (uint32_t)__builtin_amdgcn_readlane((int)iv, 63);
+  const uint32_t et = (uint32_t)__builtin_amdgcn_readlane((int)it, 63);
+  sum0 = (unsigned long long)hv | ((unsigned long long)ht << 32);
+  sum1 = (unsigned long long)(ev - hv) | ((unsigned long long)(et - ht) << 32);
+  if (lane >= 32) {  // the second tile's scan starts at lane 32
+    iv -= hv;
+    it -= ht;
+  }
+  const int64_t tile = pair * 2 + (lane >> 5);
+  if (tile < n_ctiles) {
+    uint32_t rv = iv - sv, rt = it - st;
+    uint32_t out[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      out[i] = rv | (rt << 16);
+      rv += nv[i];
+      rt += nt[i];
+    }
+    *reinterpret_cast<uint4*>(lpre + w0) = make_uint4(out[0], out[1], out[2], out[3]);
   }
 }
 
@@ -566,27 +601,40 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total,
 
 // Where the generate kernel of the single-volume forward takes its word
 // bases from: the count kernel's per-word prefixes within their count tiles
-// (lpre, v | t << 16) and the exclusive prefix of every count tile (tpre,
-// v | t << 32, n_ctiles + 1 entries, the totals last).  It writes the bases
-// of its words to base_v / base_t, which the adjoint and the per-cell
-// outputs read.
+// (lpre, v | t << 16), every count tile's prefix within its count block of
+// tpb tiles (tloc, v | t << 32) and the count blocks' exclusive prefixes
+// (bpre, the totals after the last).  It writes the bases of its words to
+// base_v / base_t, which the adjoint and the per-cell outputs read.
 struct CountSrc {
   const uint32_t* __restrict__ lpre;
-  const unsigned long long* __restrict__ tpre;
+  const unsigned long long* __restrict__ tloc;
+  const unsigned long long* __restrict__ bpre;
   int64_t n_ctiles;
+  int tpb;
   uint32_t* __restrict__ base_v;
   uint32_t* __restrict__ base_t;
 };
+
+__device__ __forceinline__ unsigned long long add_vt(unsigned long long a, unsigned long long b) {
+  return (unsigned long long)((uint32_t)a + (uint32_t)b) |
+         ((unsigned long long)((uint32_t)(a >> 32) + (uint32_t)(b >> 32)) << 32);
+}
+
+// Exclusive vertex / triangle prefix (v | t << 32) of count tile c; for c at
+// or past the last tile, the totals.
+__device__ __forceinline__ unsigned long long ctile_prefix(const CountSrc& cs, int64_t c) {
+  if (c >= cs.n_ctiles) return cs.bpre[(cs.n_ctiles + cs.tpb - 1) / cs.tpb];
+  return add_vt(cs.bpre[c / cs.tpb], cs.tloc[c]);
+}
 
 // Vertex and triangle base of word w (v | t << 32); past the last count
 // tile the totals.
 __device__ __forceinline__ unsigned long long word_bases(const CountSrc& cs, int64_t w) {
   const int64_t ct = w / kCountTile;
-  if (ct >= cs.n_ctiles) return cs.tpre[cs.n_ctiles];
-  const unsigned long long p = cs.tpre[ct];
+  const unsigned long long p = ctile_prefix(cs, ct);
+  if (ct >= cs.n_ctiles) return p;
   const uint32_t l = cs.lpre[w];
-  return (unsigned long long)((uint32_t)p + (l & 0xffffu)) |
-         ((unsigned long long)((uint32_t)(p >> 32) + (l >> 16)) << 32);
+  return add_vt(p, (unsigned long long)(l & 0xffffu) | ((unsigned long long)(l >> 16) << 32));
 }
 
 // One generate tile: words [w_begin, w_begin + TW) of a volume.
@@ -621,7 +669,7 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
     // count tiles without output: constant bases, nothing else to do
     const int64_t c0 = w_begin / kCountTile;
     const int64_t c1 = min((w_begin + nw + kCountTile - 1) / kCountTile, cs.n_ctiles);
-    const unsigned long long p0 = cs.tpre[c0], p1 = cs.tpre[c1];
+    const unsigned long long p0 = ctile_prefix(cs, c0), p1 = ctile_prefix(cs, c1);
     if (p0 == p1) {
       const uint32_t v0 = (uint32_t)p0, t0 = (uint32_t)(p0 >> 32);
       for (int i = tid; i < nw; i += nt) {

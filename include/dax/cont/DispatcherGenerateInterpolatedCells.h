@@ -15,8 +15,10 @@
 //
 // The HIP path classifies from the field and the worklet's isovalue (or,
 // for SliceGenerate, from the plane: Invoke(grid, outGrid, coords)); the
-// count handle (from DispatcherMapCell<MarchingCubesCount / SliceCount>) is
-// only released per SetReleaseCount, its values are not re-read.
+// count handle (from DispatcherMapCell<MarchingCubesCount / SliceCount>)
+// must hold the same total T as that classification (the reference takes T
+// from its scan, :180-183): ErrorControlBadValue otherwise.  It is then
+// released per SetReleaseCount.
 #ifndef DAX_HIP_SLICE_DISPATCHERGENERATEINTERPOLATEDCELLS_H
 #define DAX_HIP_SLICE_DISPATCHERGENERATEINTERPOLATEDCELLS_H
 
@@ -91,6 +93,23 @@ class InterpolatedCellsHip {
   }
 
  protected:
+  // The reference takes T from ScanInclusive of the count handle (:180-183).
+  // The HIP path classifies again from the field and the worklet, so the
+  // handle's total must equal that classification's T: a count array from
+  // another isovalue or field would make the reference's output differ.
+  void CheckCountTotal(int64_t T) {
+    const dax::Id n = Count.GetNumberOfValues();
+    int64_t total = 0;
+    if (n > 0)
+      internal::HipCheck(dax_hip_reduce_sum_i32(
+          reinterpret_cast<const int32_t*>(Count.PrepareForInput().GetRawPointer()), n, &total,
+          nullptr));
+    if (total != T)
+      throw ErrorControlBadValue(
+          "the count array's total differs from the cells this dispatcher generates (was it "
+          "counted with the same worklet, isovalue and field?)");
+  }
+
   // Sizes the outputs of a classified workspace (counts = {T, U}) and returns
   // the pointers the generate step writes; false when there is nothing to do.
   template <class CC, class PC>
@@ -99,6 +118,7 @@ class InterpolatedCellsHip {
                       float*& coords, float*& normals, int32_t*& first, int32_t*& second,
                       float*& weight) {
     internal::HipCheck(dax_hip_mc_read_counts(d_counts, INT64_MAX, INT64_MAX, &T, &U, nullptr));
+    CheckCountTotal(T);
     if (GetReleaseCount()) DoReleaseCount();
     if (T == 0) return false;  // nothing to do (:190-194)
     if (!RemoveDuplicatePoints) U = 3 * T;
@@ -136,6 +156,7 @@ class InterpolatedCellsHip {
     int64_t T = 0, U = 0;
     internal::HipCheck(count(cells, n, f, P, iso, (int32_t*)counts.Data, (int32_t*)offs.Data, &T,
                              nullptr));
+    CheckCountTotal(T);
     if (GetReleaseCount()) DoReleaseCount();
     if (T == 0) return;  // nothing to do (:190-194)
     const dax::Id cap = (dax::Id)(3 * T);

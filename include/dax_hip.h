@@ -65,6 +65,17 @@ int dax_hip_synchronize(void* stream);
 
 /* ---- DeviceAdapterAlgorithm<DeviceAdapterTagHip> ----------------------- */
 
+/* DispatcherMapField<Magnitude> (dax/cont/DispatcherMapField.h:30-61 with
+ * dax/worklet/Magnitude.h:26-33): out[i] = Sqrt(dot(in[i], in[i])) over n
+ * packed float32 triples, the dot product summed from component 0. */
+int dax_hip_magnitude_vec3(const float* d_in, int64_t n, float* d_out, void* stream);
+
+/* The total ScanInclusive returns (DeviceAdapterAlgorithm.h:114-117;
+ * DispatcherGenerateInterpolatedCells.h:180-183 takes T from it) without
+ * writing the scan: *h_total = sum of n int32 values, in 64 bits.
+ * Synchronous. */
+int dax_hip_reduce_sum_i32(const int32_t* d_in, int64_t n, int64_t* h_total, void* stream);
+
 /* Copy (DeviceAdapterAlgorithm.h:56-60; Thrust CopyPortal
  * DeviceAdapterAlgorithmThrust.h:76-83): a device copy kernel. */
 int dax_hip_copy(void* d_dst, const void* d_src, size_t bytes, void* stream);

@@ -141,6 +141,22 @@ void TestMarchingCubes() {
   polys << "POLYGONS " << secondOutGrid.GetNumberOfCells() << " "
         << 4 * secondOutGrid.GetNumberOfCells() << "\n";
   DAX_TEST_ASSERT(text.str().find(polys.str()) != std::string::npos, "VTK polygons");
+
+  // A count array from another isovalue: the reference would generate from
+  // that count, the HIP dispatcher's own classification gives another
+  // total, so it refuses the call.
+  {
+    InterpolatedDispatcher other(count, dax::worklet::MarchingCubesGenerate(isoValue + 1));
+    other.SetReleaseCount(false);
+    UnstructuredGridType badGrid;
+    bool threw = false;
+    try {
+      other.Invoke(inGrid, badGrid, fieldHandle);
+    } catch (const dax::cont::ErrorControlBadValue&) {
+      threw = true;
+    }
+    DAX_TEST_ASSERT(threw, "a count array of another isovalue was accepted");
+  }
 }
 
 void TestMarchingCubesHexahedra() {
