@@ -20,7 +20,9 @@ Dax's OpenMP-adapter pipeline on the host cores on the same field.  Beside
 the headline the line carries
   fwd_bwd  the same field, forward + adjoint (upstream dL/dvertex and
            dL/dnormal N(0,1), seed 1), and its ratio to the forward;
-  fwd_bwd_256  the differentiable workload: the same at 256^3;
+  fwd_bwd_256  the differentiable workload: the same at 256^3, hot (steps back
+           to back, field in the MALL) and cold (MALL flushed before each
+           step; the headline ratio);
   slice_512  a slice of the 512^3 grid by a tilted plane (no field read),
            with the OpenMP restatement beside it;
   tetrahedra_128  marching tetrahedra over the 12.3 M Freudenthal
@@ -38,8 +40,13 @@ the headline the line carries
            iso U[0.3,0.7]: the batched forward and forward+backward replayed
            as captured graphs, against a native launch loop of single-volume
            forwards (+ backwards) into the same outputs;
-  large    (--large) 1024^3 Mandelbulb (power 8, 10 iterations, bailout 2,
-           iso 0.5) forward and forward+backward.
+  large    1024^3 Mandelbulb (power 8, 10 iterations, bailout 2, iso 0.5)
+           forward and forward+backward (--no-large skips it);
+  dispatcher_pipeline_512  the reference benchmark's timed region
+           (Benchmarks/MarchingCubes/Pipeline.h:110-132: classify, then
+           generate with duplicate removal, |p| field at 512^3, iso 100)
+           through the C++ dispatchers, as a Dax user calls them (the
+           program tests/cpp/bin/BenchmarkPipelineMarchingCubesHip).
 """
 from __future__ import annotations
 
@@ -74,7 +81,7 @@ def _measured_traffic(n, tuned):
     if n != 512 or not path.exists():
         return None
     d = json.loads(path.read_text())["hbm_bytes_per_dispatch"]
-    roles = ("k_classify_v4", "k_count_scan", "k_generate")
+    roles = ("k_classify_v4", "k_count", "k_generate")
     return {r: d.get(v) for r, v in zip(roles, _kernel_variants(n, tuned))}
 
 
@@ -93,9 +100,9 @@ def parse():
                     help="only run the device copy kernel on 512 MiB (counter calibration)")
     ap.add_argument("--batch", type=int, default=1024, help="volumes of the batched run (0: skip)")
     ap.add_argument("--no-bwd", action="store_true", help="skip the forward+backward timing")
-    ap.add_argument("--large", action="store_true", help="also run the 1024^3 Mandelbulb")
+    ap.add_argument("--no-large", action="store_true", help="skip the 1024^3 Mandelbulb")
     ap.add_argument("--only", choices=["fwd", "batch", "large", "diff", "slice", "tets", "hex",
-                                       "threshold", "vtk"],
+                                       "threshold", "vtk", "dispatch"],
                     default=None,
                     help="run one section only (profiling); prints its JSON")
     return ap.parse_args()
@@ -182,8 +189,29 @@ def _upstream(U, seed):
     return dv, dn
 
 
-def single_volume(field, iso, label, args, tuning=None):
-    """Forward (+ phases, + fwd+bwd) of one resident volume."""
+def _flusher():
+    """A launch that moves 512 MiB through the HBM (a 256 MiB copy): between
+    timed steps it leaves nothing of the previous step in the 256 MB MALL."""
+    import torch
+
+    from daxtoolkit_amd import _lib
+    n = 256 << 20
+    src = torch.zeros(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    return lambda: _lib.call("dax_hip_copy", dst.data_ptr(), src.data_ptr(), n,
+                             torch.cuda.current_stream().cuda_stream)
+
+
+def _cold_ms(step, flush, steps, stream):
+    """Mean ms of step() with the MALL flushed before each: the graph of
+    (flush, step) x K minus the graph of flush x K."""
+    return max(0.0, _graph_ms(lambda: (flush(), step()), steps, stream) -
+               _graph_ms(flush, steps, stream))
+
+
+def single_volume(field, iso, label, args, tuning=None, cold=False):
+    """Forward (+ phases, + fwd+bwd) of one resident volume; with cold, also
+    the forward and forward+backward with the MALL flushed before each step."""
     import torch
 
     from daxtoolkit_amd import mc
@@ -253,6 +281,13 @@ def single_volume(field, iso, label, args, tuning=None):
         mc.destroy_graph(g_b)
         print(f"[bench] {label}: fwd {ms:.4f} ms, fwd+bwd {r['fb_ms']:.4f} ms",
               file=sys.stderr, flush=True)
+    if cold:
+        flush = _flusher()
+        r["ms_cold"] = _cold_ms(lambda: mc.forward_into(field, iso, buf), flush, args.steps, s)
+        if not args.no_bwd:
+            r["fb_ms_cold"] = _cold_ms(fb, flush, args.steps, s)
+        _progress(f"{label}: cold (MALL flushed) fwd {r['ms_cold']:.4f} ms"
+                  + (f", fwd+bwd {r['fb_ms_cold']:.4f} ms" if "fb_ms_cold" in r else ""))
     return r
 
 
@@ -270,6 +305,9 @@ def batched(args):
     print(f"[bench] batch: {args.batch} volumes, {P} points, T={T} U={U}", file=sys.stderr,
           flush=True)
     bufs = b.alloc(U, T)
+    tuned, tuned_ms = b.autotune(flat, bufs, reps=3)
+    _progress(f"batch: autotuned {tuned} (classify, count/scan, generate "
+              + ", ".join(f"{x:.3f}" for x in tuned_ms) + " ms)")
     dv, dn = _upstream(U, 2)
     scratch = torch.empty(max(b.scratch_bytes(U), 1), dtype=torch.uint8, device="cuda")
     dldf = torch.empty_like(flat)
@@ -315,6 +353,9 @@ def batched(args):
         "volumes_per_s_fwd_bwd": round(args.batch / (ms_fb * 1e-3), 1),
         "Gcells_per_s_fwd": round(C / (ms_f * 1e-3) / 1e9, 3),
         "loop_outputs_bit_identical": same, "steps": steps,
+        "tuning": {"classify_unroll": tuned[0], "classify_blocks_per_cu": tuned[1],
+                   "scan_threads": tuned[2], "gen_blocks_per_cu": tuned[3], "gen_xcd": tuned[4],
+                   "autotune_phase_ms": [round(x, 4) for x in tuned_ms]},
     }
 
 
@@ -327,12 +368,20 @@ def differentiable_256(args):
     n = 256
     field = fields.gaussian_mixture_torch((n, n, n), fields.gaussian_mixture_params(0, (n, n, n)))
     torch.cuda.synchronize()
-    r = single_volume(field, 0.5, "256^3", args)
+    r = single_volume(field, 0.5, "256^3", args, cold=True)
     return {"workload": "differentiable marching cubes, 256^3 points, 16-Gaussian mixture (seed 0) "
                         "+ U(-0.01,0.01) noise, iso 0.5, upstream dL/dvertex, dL/dnormal N(0,1)",
-            "triangles": r["T"], "vertices": r["U"], "fwd_ms": round(r["ms"], 5),
-            "fwd_bwd_ms": round(r["fb_ms"], 5), "bwd_ms": round(r["bwd_ms"], 5),
-            "fwd_bwd_over_fwd": round(r["fb_ms"] / r["ms"], 3), "tuning": r["tuned"]}
+            "triangles": r["T"], "vertices": r["U"],
+            "hot": {"note": "K steps back to back: the 64 MB field stays in the 256 MB MALL",
+                    "fwd_ms": round(r["ms"], 5), "fwd_bwd_ms": round(r["fb_ms"], 5),
+                    "bwd_ms": round(r["bwd_ms"], 5),
+                    "fwd_bwd_over_fwd": round(r["fb_ms"] / r["ms"], 3)},
+            "cold": {"note": "512 MiB of copy traffic before each step (not timed): every step "
+                             "reads the field from HBM",
+                     "fwd_ms": round(r["ms_cold"], 5), "fwd_bwd_ms": round(r["fb_ms_cold"], 5),
+                     "fwd_bwd_over_fwd": round(r["fb_ms_cold"] / r["ms_cold"], 3)},
+            "fwd_ms": round(r["ms_cold"], 5), "fwd_bwd_ms": round(r["fb_ms_cold"], 5),
+            "fwd_bwd_over_fwd": round(r["fb_ms_cold"] / r["ms_cold"], 3), "tuning": r["tuned"]}
 
 
 def slice_512(args):
@@ -596,6 +645,32 @@ def large(args):
     return out
 
 
+def dispatcher_pipeline(n, reps=7):
+    """The reference benchmark's timed region through the C++ dispatchers:
+    the compiled program prints one JSON line with per-repetition seconds,
+    each from the adapter's Timer<DeviceAdapterTagHip> around classify +
+    generate into a fresh output grid, as the reference brackets them
+    (Pipeline.h:110-132)."""
+    import subprocess
+    exe = ROOT / "tests" / "cpp" / "bin" / "BenchmarkPipelineMarchingCubesHip"
+    if not exe.exists():
+        return {"skipped": f"{exe.relative_to(ROOT)} is not built"}
+    p = subprocess.run([str(exe), f"--size={n}", "--pipeline=2", f"--reps={reps}"],
+                       capture_output=True, text=True, timeout=600)
+    lines = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+    if p.returncode != 0 or not lines:
+        return {"failed": p.returncode, "stderr": p.stderr[-500:]}
+    d = json.loads(lines[-1])
+    secs = sorted(d["seconds"][1:] or d["seconds"])  # the first repetition allocates
+    med = statistics.median(secs)
+    C = (n - 1) ** 3
+    return {"workload": f"reference MC benchmark pipeline 2 (duplicate removal), {n}^3 grid, "
+                        f"field |p| (DispatcherMapField<Magnitude>), iso 100",
+            "triangles": d["triangles"], "points": d["points"],
+            "median_ms": round(med * 1e3, 4), "min_ms": round(secs[0] * 1e3, 4),
+            "Gcells_per_s": round(C / med / 1e9, 3), "reps": len(d["seconds"])}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -635,6 +710,9 @@ def main():
     if args.only == "vtk":
         print(json.dumps({"vtk_write_512": vtk_write_512(args)}), flush=True)
         return
+    if args.only == "dispatch":
+        print(json.dumps({"dispatcher_pipeline_512": dispatcher_pipeline(512)}), flush=True)
+        return
     n = args.n
     dims = (n, n, n)
     iso = 0.5
@@ -648,10 +726,10 @@ def main():
     W = ((n + 63) // 64) * n * n                       # row words
     b_fwd = 4 * P + 12 * U + 12 * U + 4 * 3 * T + 4 * U  # field + verts + normals + conn + edges
     fwd_gbs = b_fwd / (ms * 1e-3) / 1e9
-    names = ["k_classify_v4", "k_count_scan", "k_generate"]
+    names = ["k_classify_v4", "k_count", "k_generate"]
     alg = [4 * P + P // 8,                 # field read once, one inside bit per point written
-           P // 8 + 8 * W,                 # inside-bits read, two u32 bases per row word written
-           12 * U + 12 * U + 4 * 3 * T + 4 * U]  # the forward's outputs
+           P // 8 + 4 * W,                 # inside-bits read, one u32 word prefix per row word written
+           12 * U + 12 * U + 4 * 3 * T + 4 * U + 8 * W]  # the outputs and two u32 bases per word
     traffic = _measured_traffic(n, tuned)
     kernels = {}
     for nm, t_k, a_k in zip(names, k_ms, alg):
@@ -691,7 +769,7 @@ def main():
         "tuning": {"classify_unroll": tuned[0], "classify_vec": tuned[1], "scan_sub": tuned[2],
                    "gen_tile": tuned[3], "gen_xcd": tuned[4],
                    "autotune_ms": [round(x, 5) for x in r["tuned_ms"]]},
-        "roofline": {"bound": "hbm", "scope": "forward graph: k_classify_v4, k_count_scan, k_generate",
+        "roofline": {"bound": "hbm", "scope": "forward graph: k_classify_v4, k_count (+ k_scan_tsums), k_generate",
                      "achieved": round(fwd_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(fwd_gbs / HBM_PEAK_GBS, 4),
                      "traffic": (sum(traffic[k] for k in names)
@@ -713,8 +791,9 @@ def main():
         line["vtk_write_512"] = vtk_write_512(args)
         if args.batch > 0:
             line["batched"] = batched(args)
-        if args.large:
+        if not args.no_large:
             line["large"] = large(args)
+        line["dispatcher_pipeline_512"] = dispatcher_pipeline(512)
     if not args.no_cpu_baseline and args.only is None:
         f_host = field.cpu().numpy()
         line["cpu_baseline"] = cpu_baseline(f_host, iso, C, args.cpu_seconds)

@@ -154,6 +154,10 @@ _SIGNATURES = {
     "dax_hip_mc_batch_graph_create": (C.c_int, [vp, vp, vp, sz, C.c_uint, vp, vp, vp, vp, i64,
                                                 i64, vp, vp, vp, C.POINTER(vp)]),
     "dax_hip_mc_batch_backward": (C.c_int, [vp, vp, vp, sz, vp, vp, i64, vp, vp, vp, sz, vp, vp]),
+    "dax_hip_mc_batch_set_tuning": (C.c_int, [vp, vp]),
+    "dax_hip_mc_batch_get_tuning": (C.c_int, [vp, vp]),
+    "dax_hip_mc_batch_autotune": (C.c_int, [vp, vp, vp, sz, C.c_uint, vp, vp, vp, vp, i64, i64,
+                                            vp, C.c_int, vp, vp, vp]),
     "dax_hip_mc_volume_loop": (C.c_int, [i64, vp, vp, vp, vp, vp, vp, C.c_uint, vp, sz, vp, i64,
                                          i64, vp, vp, vp, vp, vp, vp, vp, vp, sz, vp, vp]),
     "dax_hip_graph_begin_capture": (C.c_int, [vp]),

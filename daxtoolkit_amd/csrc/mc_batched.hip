@@ -16,6 +16,9 @@
 // over tiles); the four launches of a forward are graph-capturable (no host
 // read-back, scan state cleared by a kernel).
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "mc_adjoint.h"
@@ -29,10 +32,27 @@ struct TileRef {
   int32_t w0;  // first row word, local to the volume
 };
 
-constexpr int kBatchScanNT = 256, kBatchScanWPT = 4;
-constexpr int kBatchScanTile = kBatchScanNT * kBatchScanWPT;
+// Tile tables are built at prepare: 1024-word scan tiles, 256-word
+// generate (and classify) tiles.  The launch shapes over them are tuned.
+constexpr int kBatchScanTile = 1024;
 constexpr int kBatchGenTile = 256;
-constexpr int kBatchClsUnroll = 16;
+
+// Launch shapes of the batch forward, picked by dax_hip_mc_batch_autotune
+// and cached per batch shape class: classify unroll (row words per wave
+// step) and persistent blocks per CU, threads per scan tile (the tile's
+// 1024 words over 256, 512 or 1024 threads), generate blocks per CU and
+// its XCD-contiguous tile order.
+struct BatchTuning {
+  int32_t cls_unroll, cls_blocks, scan_threads, gen_blocks, gen_xcd, pad;
+};
+constexpr BatchTuning kBatchTuningDefault{16, 4, 256, 4, 1, 0};
+bool valid_batch_tuning(const BatchTuning& t) {
+  return (t.cls_unroll == 8 || t.cls_unroll == 16 || t.cls_unroll == 32) &&
+         (t.cls_blocks == 2 || t.cls_blocks == 4 || t.cls_blocks == 8) &&
+         (t.scan_threads == 256 || t.scan_threads == 512 || t.scan_threads == 1024) &&
+         (t.gen_blocks == 2 || t.gen_blocks == 4 || t.gen_blocks == 8) &&
+         (t.gen_xcd == 0 || t.gen_xcd == 1);
+}
 
 struct BatchLayout {
   size_t off_vols, off_scan_tiles, off_gen_tiles, off_bits, off_base_v, off_base_t, off_status,
@@ -79,11 +99,13 @@ struct BatchHeader {
   int64_t magic, nvols, nwords, nscan, ngen, max_wpr;
   size_t off_vols, off_scan_tiles, off_gen_tiles, off_bits, off_base_v, off_base_t, off_status,
       off_ctrl, bytes;
+  BatchTuning tune;
 };
 constexpr int64_t kBatchMagic = 0x4441584842415443LL;  // "DAXHBATC"
 constexpr size_t kBatchHeaderBytes = 256;
 static_assert(sizeof(BatchHeader) <= kBatchHeaderBytes, "header fits its reserved bytes");
 
+template <int UNROLL>
 __global__ __launch_bounds__(256) void k_batch_classify(const VolGeom* __restrict__ vols,
                                                         const TileRef* __restrict__ tiles,
                                                         int64_t ntiles,
@@ -99,19 +121,21 @@ __global__ __launch_bounds__(256) void k_batch_classify(const VolGeom* __restric
     const TileRef tr = tiles[t];
     const VolGeom g = vols[tr.vol];
     const int64_t w_end = min((int64_t)tr.w0 + kBatchGenTile, g.nwords);
-    for (int64_t w0 = tr.w0 + (int64_t)wv * kBatchClsUnroll; w0 < w_end;
-         w0 += (int64_t)(blockDim.x >> 6) * kBatchClsUnroll)
-      classify_words<kBatchClsUnroll, false>(g, fields + g.field_off, bits + g.word_off, w0, w_end);
+    for (int64_t w0 = tr.w0 + (int64_t)wv * UNROLL; w0 < w_end;
+         w0 += (int64_t)(blockDim.x >> 6) * UNROLL)
+      classify_words<UNROLL, false>(g, fields + g.field_off, bits + g.word_off, w0, w_end);
   }
 }
 
-__global__ __launch_bounds__(kBatchScanNT) void k_batch_count_scan(
+template <int NT>
+__global__ __launch_bounds__(NT) void k_batch_count_scan(
     const VolGeom* __restrict__ vols, const TileRef* __restrict__ tiles, int64_t ntiles,
     const uint64_t* __restrict__ bits, uint32_t* __restrict__ base_v,
     uint32_t* __restrict__ base_t, unsigned long long* __restrict__ status,
     Ctrl* __restrict__ ctrl) {
   __shared__ ScanShared sh;
-  __shared__ uint32_t s_pk[ScanTileLds<kBatchScanNT, kBatchScanWPT>::kSize];
+  constexpr int WPT = kBatchScanTile / NT;
+  __shared__ uint32_t s_pk[ScanTileLds<NT, WPT>::kSize];
   __shared__ int64_t s_tile;
   while (true) {
     __syncthreads();  // s_tile and sh reused across iterations
@@ -121,8 +145,8 @@ __global__ __launch_bounds__(kBatchScanNT) void k_batch_count_scan(
     if (t >= ntiles) break;
     const TileRef tr = tiles[t];
     const VolGeom g = vols[tr.vol];
-    count_scan_tile<kBatchScanNT, kBatchScanWPT>(g, bits + g.word_off, base_v, base_t, status, ctrl,
-                                                t, tr.w0, sh, s_pk);
+    count_scan_tile<NT, WPT>(g, bits + g.word_off, base_v, base_t, status, ctrl, t, tr.w0, sh,
+                             s_pk);
   }
 }
 
@@ -131,7 +155,7 @@ __global__ __launch_bounds__(kScanThreads) void k_batch_generate(
     const float* __restrict__ fields, const uint64_t* __restrict__ bits,
     const uint32_t* __restrict__ base_v, const uint32_t* __restrict__ base_t,
     const Ctrl* __restrict__ ctrl, OutPtrs out, unsigned flags, int64_t cap_points,
-    int64_t cap_tris, int64_t* __restrict__ counts) {
+    int64_t cap_tris, int64_t* __restrict__ counts, int xcd) {
   extern __shared__ uint64_t s_dyn[];
   const bool dedup = (flags & DAX_HIP_MC_DEDUP) != 0;
   const bool normals = (flags & DAX_HIP_MC_NORMALS) != 0 && out.normals != nullptr;
@@ -144,7 +168,7 @@ __global__ __launch_bounds__(kScanThreads) void k_batch_generate(
   if (total_v > 0x7fffffffull || 3ull * total_t > 0x7fffffffull) return;
   if ((int64_t)total_t > cap_tris || (int64_t)n_points > cap_points) return;
   for (int64_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
-    const TileRef tr = tiles[xcd_tile(b, ntiles)];
+    const TileRef tr = tiles[xcd ? xcd_tile(b, ntiles) : b];
     const VolGeom g = vols[tr.vol];
     const int64_t vbv = base_v[g.word_off], vbt = base_t[g.word_off];
     generate_tile<kBatchGenTile>(g, FieldArray{fields + g.field_off}, bits + g.word_off,
@@ -254,6 +278,90 @@ int persistent_blocks(int per_cu) {
   return cus * per_cu;
 }
 
+// Tuning cache per batch shape class: (volumes, ceil(log2(row words)),
+// widest row in words).
+std::mutex g_btune_mu;
+std::map<std::tuple<int64_t, int, int64_t>, BatchTuning> g_btune_cache;
+
+std::tuple<int64_t, int, int64_t> batch_class(const BatchHeader& h) {
+  int lg = 0;
+  while ((1LL << lg) < h.nwords) ++lg;
+  return std::make_tuple(h.nvols, lg, h.max_wpr);
+}
+
+// Device pointers of a prepared workspace.
+struct BatchPtrs {
+  const VolGeom* vols;
+  const TileRef* scan_tiles;
+  const TileRef* gen_tiles;
+  uint64_t* bits;
+  uint32_t* bv;
+  uint32_t* bt;
+  unsigned long long* status;
+  Ctrl* ctrl;
+};
+
+BatchPtrs batch_ptrs(const BatchHeader& h, void* d_workspace) {
+  char* body = (char*)d_workspace + kBatchHeaderBytes;
+  return BatchPtrs{(const VolGeom*)(body + h.off_vols), (const TileRef*)(body + h.off_scan_tiles),
+                   (const TileRef*)(body + h.off_gen_tiles), (uint64_t*)(body + h.off_bits),
+                   (uint32_t*)(body + h.off_base_v), (uint32_t*)(body + h.off_base_t),
+                   (unsigned long long*)(body + h.off_status), (Ctrl*)(body + h.off_ctrl)};
+}
+
+// Classify (which also clears the scan's status and control words).
+int batch_classify(const BatchHeader& h, const BatchPtrs& p, const float* d_fields,
+                   const BatchTuning& t, hipStream_t s) {
+  const int64_t zero_words = h.nscan + (int64_t)(sizeof(Ctrl) / 8);
+  const dim3 grid((unsigned)persistent_blocks(t.cls_blocks)), blk(256);
+  switch (t.cls_unroll) {
+    case 8:
+      k_batch_classify<8><<<grid, blk, 0, s>>>(p.vols, p.gen_tiles, h.ngen, d_fields, p.bits,
+                                               p.status, zero_words);
+      break;
+    case 32:
+      k_batch_classify<32><<<grid, blk, 0, s>>>(p.vols, p.gen_tiles, h.ngen, d_fields, p.bits,
+                                                p.status, zero_words);
+      break;
+    default:
+      k_batch_classify<16><<<grid, blk, 0, s>>>(p.vols, p.gen_tiles, h.ngen, d_fields, p.bits,
+                                                p.status, zero_words);
+  }
+  DAXHIP_LAUNCH_CHECK("k_batch_classify");
+  return DAX_HIP_OK;
+}
+
+int batch_scan(const BatchHeader& h, const BatchPtrs& p, const BatchTuning& t, hipStream_t s) {
+  const dim3 grid((unsigned)std::min<int64_t>(persistent_blocks(4), h.nscan));
+  switch (t.scan_threads) {
+    case 512:
+      k_batch_count_scan<512><<<grid, dim3(512), 0, s>>>(p.vols, p.scan_tiles, h.nscan, p.bits,
+                                                         p.bv, p.bt, p.status, p.ctrl);
+      break;
+    case 1024:
+      k_batch_count_scan<1024><<<grid, dim3(1024), 0, s>>>(p.vols, p.scan_tiles, h.nscan, p.bits,
+                                                           p.bv, p.bt, p.status, p.ctrl);
+      break;
+    default:
+      k_batch_count_scan<256><<<grid, dim3(256), 0, s>>>(p.vols, p.scan_tiles, h.nscan, p.bits,
+                                                         p.bv, p.bt, p.status, p.ctrl);
+  }
+  DAXHIP_LAUNCH_CHECK("k_batch_count_scan");
+  return DAX_HIP_OK;
+}
+
+int batch_generate(const BatchHeader& h, const BatchPtrs& p, const float* d_fields,
+                   unsigned flags, const OutPtrs& o, int64_t cap_points, int64_t cap_tris,
+                   int64_t* d_counts, const BatchTuning& t, hipStream_t s) {
+  const size_t lds = GenLayout<kBatchGenTile>((int)h.max_wpr).bytes();
+  k_batch_generate<<<dim3((unsigned)std::min<int64_t>(persistent_blocks(t.gen_blocks), h.ngen)),
+                     dim3(kScanThreads), lds, s>>>(p.vols, p.gen_tiles, h.ngen, d_fields, p.bits,
+                                                   p.bv, p.bt, p.ctrl, o, flags, cap_points,
+                                                   cap_tris, d_counts, t.gen_xcd);
+  DAXHIP_LAUNCH_CHECK("k_batch_generate");
+  return DAX_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -325,6 +433,12 @@ int dax_hip_mc_batch_prepare(int64_t n_volumes, const int64_t* h_dims3,
   h.off_status = L.off_status;
   h.off_ctrl = L.off_ctrl;
   h.bytes = L.bytes;
+  h.tune = kBatchTuningDefault;
+  {
+    std::lock_guard<std::mutex> lk(g_btune_mu);
+    const auto it = g_btune_cache.find(batch_class(h));
+    if (it != g_btune_cache.end()) h.tune = it->second;
+  }
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)d_workspace;
   char* body = ws + kBatchHeaderBytes;
@@ -365,35 +479,162 @@ int dax_hip_mc_batch_forward(const float* d_fields, const void* h_header, void* 
     return DAX_HIP_BAD_VALUE;
   }
   hipStream_t s = (hipStream_t)stream;
-  char* body = (char*)d_workspace + kBatchHeaderBytes;
-  const VolGeom* vols = (const VolGeom*)(body + h->off_vols);
-  const TileRef* scan_tiles = (const TileRef*)(body + h->off_scan_tiles);
-  const TileRef* gen_tiles = (const TileRef*)(body + h->off_gen_tiles);
-  uint64_t* bits = (uint64_t*)(body + h->off_bits);
-  uint32_t* bv = (uint32_t*)(body + h->off_base_v);
-  uint32_t* bt = (uint32_t*)(body + h->off_base_t);
-  unsigned long long* status = (unsigned long long*)(body + h->off_status);
-  Ctrl* ctrl = (Ctrl*)(body + h->off_ctrl);
-  const int64_t zero_words = h->nscan + (int64_t)(sizeof(Ctrl) / 8);
-  const int grid = persistent_blocks(4);
-  k_batch_classify<<<dim3(grid), dim3(256), 0, s>>>(vols, gen_tiles, h->ngen, d_fields, bits, status,
-                                                     zero_words);
-  DAXHIP_LAUNCH_CHECK("k_batch_classify");
-  k_batch_count_scan<<<dim3((unsigned)std::min<int64_t>(grid, h->nscan)), dim3(kBatchScanNT), 0, s>>>(
-      vols, scan_tiles, h->nscan, bits, bv, bt, status, ctrl);
-  DAXHIP_LAUNCH_CHECK("k_batch_count_scan");
+  const BatchPtrs p = batch_ptrs(*h, d_workspace);
+  const BatchTuning t = valid_batch_tuning(h->tune) ? h->tune : kBatchTuningDefault;
+  int rc;
+  if ((rc = batch_classify(*h, p, d_fields, t, s))) return rc;
+  if ((rc = batch_scan(*h, p, t, s))) return rc;
   OutPtrs o{d_vertices, d_normals, d_conn, d_edges, nullptr, nullptr, nullptr};
-  const size_t lds = GenLayout<kBatchGenTile>((int)h->max_wpr).bytes();
-  k_batch_generate<<<dim3((unsigned)std::min<int64_t>(grid, h->ngen)), dim3(kScanThreads), lds, s>>>(
-      vols, gen_tiles, h->ngen, d_fields, bits, bv, bt, ctrl, o, flags, cap_points, cap_tris,
-      d_counts);
-  DAXHIP_LAUNCH_CHECK("k_batch_generate");
+  if ((rc = batch_generate(*h, p, d_fields, flags, o, cap_points, cap_tris, d_counts, t, s)))
+    return rc;
   if (d_vol_offsets) {
     k_batch_offsets<<<dim3((unsigned)ceil_div(h->nvols, 256)), dim3(256), 0, s>>>(
-        vols, h->nvols, bv, bt, flags, d_vol_offsets);
+        p.vols, h->nvols, p.bv, p.bt, flags, d_vol_offsets);
     DAXHIP_LAUNCH_CHECK("k_batch_offsets");
   }
   return DAX_HIP_OK;
+}
+
+// Launch shapes of a prepared batch: {classify unroll, classify blocks per
+// CU, scan threads per tile, generate blocks per CU, generate XCD order}.
+int dax_hip_mc_batch_set_tuning(void* h_header, const int* h_params5) {
+  BatchHeader* h = (BatchHeader*)h_header;
+  if (!h || h->magic != kBatchMagic || !h_params5) {
+    set_error("dax_hip_mc_batch_set_tuning: header not prepared or NULL parameters");
+    return DAX_HIP_BAD_VALUE;
+  }
+  const BatchTuning t{h_params5[0], h_params5[1], h_params5[2], h_params5[3], h_params5[4], 0};
+  if (!valid_batch_tuning(t)) {
+    set_error("dax_hip_mc_batch_set_tuning: invalid parameters {%d,%d,%d,%d,%d}", t.cls_unroll,
+              t.cls_blocks, t.scan_threads, t.gen_blocks, t.gen_xcd);
+    return DAX_HIP_BAD_VALUE;
+  }
+  h->tune = t;
+  return DAX_HIP_OK;
+}
+
+int dax_hip_mc_batch_get_tuning(const void* h_header, int* h_params5) {
+  const BatchHeader* h = (const BatchHeader*)h_header;
+  if (!h || h->magic != kBatchMagic || !h_params5) {
+    set_error("dax_hip_mc_batch_get_tuning: header not prepared or NULL parameters");
+    return DAX_HIP_BAD_VALUE;
+  }
+  const BatchTuning t = valid_batch_tuning(h->tune) ? h->tune : kBatchTuningDefault;
+  h_params5[0] = t.cls_unroll;
+  h_params5[1] = t.cls_blocks;
+  h_params5[2] = t.scan_threads;
+  h_params5[3] = t.gen_blocks;
+  h_params5[4] = t.gen_xcd;
+  return DAX_HIP_OK;
+}
+
+// Times every launch shape of each phase on this batch (reps launches each,
+// HIP events on `stream`; synchronous), keeps the fastest in the header and
+// in the cache for the batch's shape class (later prepares of that class
+// start from it).  h_best5: the choice; h_best_ms3: its phase times.
+int dax_hip_mc_batch_autotune(const float* d_fields, void* h_header, void* d_workspace,
+                              size_t workspace_bytes, unsigned flags, float* d_vertices,
+                              float* d_normals, int32_t* d_conn, uint32_t* d_edges,
+                              int64_t cap_points, int64_t cap_tris, int64_t* d_counts, int reps,
+                              int* h_best5, float* h_best_ms3, void* stream) {
+  BatchHeader* h = (BatchHeader*)h_header;
+  if (!h || h->magic != kBatchMagic || !d_workspace ||
+      workspace_bytes < kBatchHeaderBytes + h->bytes || !d_fields || !d_vertices || !d_conn ||
+      !d_counts) {
+    set_error("dax_hip_mc_batch_autotune: header/workspace not prepared or outputs missing");
+    return DAX_HIP_BAD_VALUE;
+  }
+  if (reps < 1) reps = 1;
+  hipStream_t s = (hipStream_t)stream;
+  const BatchPtrs p = batch_ptrs(*h, d_workspace);
+  OutPtrs o{d_vertices, d_normals, d_conn, d_edges, nullptr, nullptr, nullptr};
+  BatchTuning best = valid_batch_tuning(h->tune) ? h->tune : kBatchTuningDefault;
+  float best_ms[3] = {1e30f, 1e30f, 1e30f};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  DAXHIP_CHECK(hipEventCreate(&e0));
+  DAXHIP_CHECK(hipEventCreate(&e1));
+  int rc = DAX_HIP_OK;
+  auto elapsed = [&]() -> float {
+    float m = 0.0f;
+    (void)hipEventSynchronize(e1);
+    (void)hipEventElapsedTime(&m, e0, e1);
+    return m;
+  };
+  // classify
+  for (int u : {8, 16, 32})
+    for (int bpc : {2, 4, 8}) {
+      BatchTuning t = best;
+      t.cls_unroll = u;
+      t.cls_blocks = bpc;
+      if ((rc = batch_classify(*h, p, d_fields, t, s))) goto done;  // warm
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < reps; ++r)
+        if ((rc = batch_classify(*h, p, d_fields, t, s))) goto done;
+      (void)hipEventRecord(e1, s);
+      const float ms = elapsed() / reps;
+      if (ms < best_ms[0]) {
+        best_ms[0] = ms;
+        best.cls_unroll = u;
+        best.cls_blocks = bpc;
+      }
+    }
+  // scan: each timed launch follows a classify that resets its state
+  for (int nt : {256, 512, 1024}) {
+    BatchTuning t = best;
+    t.scan_threads = nt;
+    float total = 0.0f;
+    for (int r = 0; r < reps; ++r) {
+      if ((rc = batch_classify(*h, p, d_fields, best, s))) goto done;
+      (void)hipEventRecord(e0, s);
+      if ((rc = batch_scan(*h, p, t, s))) goto done;
+      (void)hipEventRecord(e1, s);
+      total += elapsed();
+    }
+    if (total / reps < best_ms[1]) {
+      best_ms[1] = total / reps;
+      best.scan_threads = nt;
+    }
+  }
+  // generate over one classify + scan
+  if ((rc = batch_classify(*h, p, d_fields, best, s))) goto done;
+  if ((rc = batch_scan(*h, p, best, s))) goto done;
+  for (int bpc : {2, 4, 8})
+    for (int xcd : {0, 1}) {
+      BatchTuning t = best;
+      t.gen_blocks = bpc;
+      t.gen_xcd = xcd;
+      if ((rc = batch_generate(*h, p, d_fields, flags, o, cap_points, cap_tris, d_counts, t, s)))
+        goto done;
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < reps; ++r)
+        if ((rc = batch_generate(*h, p, d_fields, flags, o, cap_points, cap_tris, d_counts, t, s)))
+          goto done;
+      (void)hipEventRecord(e1, s);
+      const float ms = elapsed() / reps;
+      if (ms < best_ms[2]) {
+        best_ms[2] = ms;
+        best.gen_blocks = bpc;
+        best.gen_xcd = xcd;
+      }
+    }
+  h->tune = best;
+  {
+    std::lock_guard<std::mutex> lk(g_btune_mu);
+    g_btune_cache[batch_class(*h)] = best;
+  }
+  if (h_best5) {
+    h_best5[0] = best.cls_unroll;
+    h_best5[1] = best.cls_blocks;
+    h_best5[2] = best.scan_threads;
+    h_best5[3] = best.gen_blocks;
+    h_best5[4] = best.gen_xcd;
+  }
+  if (h_best_ms3)
+    for (int i = 0; i < 3; ++i) h_best_ms3[i] = best_ms[i];
+done:
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
 }
 
 // Batched adjoint: dL/dfield of every volume (at the fields' offsets) from

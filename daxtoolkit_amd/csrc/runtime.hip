@@ -191,6 +191,13 @@ int dax_hip_device_count(int* h_count) {
   return DAX_HIP_OK;
 }
 
+// Execution arrays come from the device's memory pool (kept, never
+// trimmed at synchronisation) in the order of the null stream, which the
+// adapter's calls run on: a free returns the block once the work queued
+// before it has finished, and a later allocation of a similar size reuses it
+// without mapping new pages (a fresh 512 MB hipMalloc/hipFree pair costs
+// milliseconds).  Out of memory, the pool's idle blocks are released and the
+// allocation is tried once more.
 int dax_hip_malloc(void** d_ptr, size_t bytes) {
   if (!d_ptr) {
     set_error("dax_hip_malloc: NULL output pointer");
@@ -198,10 +205,29 @@ int dax_hip_malloc(void** d_ptr, size_t bytes) {
   }
   *d_ptr = nullptr;
   if (bytes == 0) return DAX_HIP_OK;
-  hipError_t e = hipMalloc(d_ptr, bytes);
+  int dev = 0;
+  DAXHIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("dax_hip_malloc: device %d beyond the pool table", dev);
+    return DAX_HIP_BAD_VALUE;
+  }
+  hipMemPool_t pool;
+  {
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    int rc = DAX_HIP_OK;
+    pool = pool_of(dev, &rc);
+    if (rc) return rc;
+  }
+  hipError_t e = hipMallocFromPoolAsync(d_ptr, bytes, pool, nullptr);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(nullptr);
+    (void)hipMemPoolTrimTo(pool, 0);
+    e = hipMallocFromPoolAsync(d_ptr, bytes, pool, nullptr);
+  }
   if (e != hipSuccess) {
     *d_ptr = nullptr;
-    int rc = check_hip(e, "hipMalloc");
+    int rc = check_hip(e, "hipMallocFromPoolAsync");
     if (rc == DAX_HIP_OOM)
       set_error("Failed to allocate execution array with %zu bytes.", bytes);
     return rc;
@@ -211,7 +237,7 @@ int dax_hip_malloc(void** d_ptr, size_t bytes) {
 
 int dax_hip_free(void* d_ptr) {
   if (!d_ptr) return DAX_HIP_OK;
-  DAXHIP_CHECK(hipFree(d_ptr));
+  DAXHIP_CHECK(hipFreeAsync(d_ptr, nullptr));
   return DAX_HIP_OK;
 }
 

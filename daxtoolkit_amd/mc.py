@@ -536,6 +536,35 @@ class Batch:
                 "edges": torch.empty(max(cap_points, 1), dtype=torch.int32, device=dev) if edges else None,
                 "cap_points": int(cap_points), "cap_tris": int(cap_tris)}
 
+    def set_tuning(self, params5):
+        """Launch shapes {classify unroll, classify blocks/CU, scan threads,
+        generate blocks/CU, generate XCD order} for this batch."""
+        import ctypes
+        arr = (ctypes.c_int * 5)(*[int(x) for x in params5])
+        _lib.call("dax_hip_mc_batch_set_tuning", self.header, arr)
+
+    def get_tuning(self):
+        import ctypes
+        arr = (ctypes.c_int * 5)()
+        _lib.call("dax_hip_mc_batch_get_tuning", self.header, arr)
+        return list(arr)
+
+    def autotune(self, fields, bufs, reps=5, dedup=True, normals=True):
+        """Time every launch shape of each phase on this batch and keep the
+        fastest (also cached for later batches of the same shape class);
+        returns (params5, ms3).  Synchronous."""
+        import ctypes
+        flags = (MC_DEDUP if dedup else 0) | (MC_NORMALS if (normals and bufs.get("normals") is not None) else 0)
+        self.check_buffers(fields, bufs)
+        best = (ctypes.c_int * 5)()
+        ms = (ctypes.c_float * 3)()
+        _lib.call("dax_hip_mc_batch_autotune", fields.data_ptr(), self.header,
+                  self.workspace.data_ptr(), self.workspace.numel(), flags,
+                  bufs["vertices"].data_ptr(), _ptr(bufs.get("normals")), bufs["conn"].data_ptr(),
+                  _ptr(bufs.get("edges")), bufs["cap_points"], bufs["cap_tris"],
+                  self.counts.data_ptr(), int(reps), best, ms, _stream())
+        return list(best), list(ms)
+
     def sizes(self, fields, dedup=True):
         """Run once without room for output to learn {T, U}."""
         probe = self.alloc(0, 0, normals=False, edges=False)

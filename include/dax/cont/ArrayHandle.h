@@ -178,10 +178,15 @@ class ArrayHandle {
     if (Internals->ControlArrayValid) Internals->ControlArray.resize((size_t)n);
   }
 
+  // Frees the execution array without copying it to the control side, as
+  // dax/cont/ArrayHandle.h:232-239 does: data held only in the execution
+  // environment is gone afterwards (DispatcherGenerateInterpolatedCells
+  // releases its count array this way).
   void ReleaseResourcesExecution() {
-    if (Internals->ExecutionArrayValid && !Internals->ControlArrayValid) SyncControlArray();
-    Internals->Execution->ReleaseResources();
-    Internals->ExecutionArrayValid = false;
+    if (Internals->ExecutionArrayValid) {
+      Internals->Execution->ReleaseResources();
+      Internals->ExecutionArrayValid = false;
+    }
   }
   void ReleaseResources() {
     Internals->Execution->ReleaseResources();

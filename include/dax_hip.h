@@ -51,7 +51,10 @@ int dax_hip_device_count(int* h_count);
  * Replaces ArrayManagerExecutionThrustDevice's device_vector storage
  * (dax/thrust/cont/internal/ArrayManagerExecutionThrustDevice.h:65-186) and
  * the CUDA manager's OOM handling (ArrayManagerExecutionCuda.h:33-85):
- * allocation failure returns DAX_HIP_OOM with the sticky error cleared. */
+ * allocation failure returns DAX_HIP_OOM with the sticky error cleared.
+ * Blocks come from the device's memory pool in null-stream order (a freed
+ * block is reused once the work queued before the free has finished); work
+ * on other streams that uses a block must finish before it is freed. */
 int dax_hip_malloc(void** d_ptr, size_t bytes);
 int dax_hip_free(void* d_ptr);
 /* LoadDataForInput / RetrieveOutputData / CopyInto transfers. */
@@ -476,6 +479,25 @@ int dax_hip_mc_batch_graph_create(const float* d_fields, const void* h_header, v
                                   float* d_normals, int32_t* d_conn, uint32_t* d_edges,
                                   int64_t cap_points, int64_t cap_tris, int64_t* d_counts,
                                   int64_t* d_vol_offsets, void* stream, void** h_graph_exec);
+/* Launch shapes of the batch forward, held in the header:
+ * {classify unroll 8|16|32, classify blocks per CU 2|4|8, scan threads per
+ * 1024-word tile 256|512|1024, generate blocks per CU 2|4|8, generate
+ * XCD-contiguous tile order 0|1}.  prepare starts from the shape cached for
+ * the batch's class (volume count, ceil(log2(row words)), widest row) or the
+ * default {16, 4, 256, 4, 1}.  Every shape gives the same outputs.
+ *   dax_hip_mc_batch_set_tuning / _get_tuning  write / read the header's
+ *   dax_hip_mc_batch_autotune  times each phase's shapes on this batch (reps
+ *       launches each, synchronous), keeps the fastest in the header and the
+ *       class cache; h_best5 the choice, h_best_ms3 its classify, count/scan
+ *       and generate times.  Outputs as for dax_hip_mc_batch_forward.
+ * No reference counterpart (the reference runs one volume per call). */
+int dax_hip_mc_batch_set_tuning(void* h_header, const int* h_params5);
+int dax_hip_mc_batch_get_tuning(const void* h_header, int* h_params5);
+int dax_hip_mc_batch_autotune(const float* d_fields, void* h_header, void* d_workspace,
+                              size_t workspace_bytes, unsigned flags, float* d_vertices,
+                              float* d_normals, int32_t* d_conn, uint32_t* d_edges,
+                              int64_t cap_points, int64_t cap_tris, int64_t* d_counts, int reps,
+                              int* h_best5, float* h_best_ms3, void* stream);
 /* Batched adjoint: dL/dfield of every volume (written at the fields'
  * offsets) from dL/dvertex and dL/dnormal [U][3] of the concatenated outputs
  * of the last dax_hip_mc_batch_forward on this workspace (run with the edge

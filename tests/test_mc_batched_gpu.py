@@ -50,6 +50,45 @@ def test_batched_equals_single_and_oracle():
         assert np.array_equal(verts.cpu().numpy().view(np.uint32), ref["coords"].view(np.uint32))
 
 
+def test_batched_tunings_bit_identical():
+    """Every launch shape of the batch forward (each value of each parameter,
+    and the autotuned choice) gives bit-identical outputs, volume offsets
+    and counts; invalid shapes are rejected."""
+    dims, isos, origins, spacings, host = _make_batch(40, 5, 110, 4)
+    print("[case] batched tunings", flush=True)
+    flat = torch.from_numpy(np.concatenate([h.reshape(-1) for h in host])).cuda()
+    b = mc.Batch(dims, isos, origins, spacings)
+    assert b.get_tuning() == [16, 4, 256, 4, 1]
+    T, U = b.sizes(flat)
+    ref = b.alloc(U, T)
+    b.forward_into(flat, ref)
+    ref_off = b.vol_offsets.clone()
+    torch.cuda.synchronize()
+    variants = [[8, 2, 512, 2, 0], [32, 8, 1024, 8, 1], [16, 2, 1024, 8, 0], [8, 8, 256, 2, 1],
+                [32, 4, 512, 4, 0]]
+    for v in variants:
+        b.set_tuning(v)
+        assert b.get_tuning() == v
+        out = b.alloc(U, T)
+        b.forward_into(flat, out)
+        torch.cuda.synchronize()
+        assert mc.read_counts(b.counts) == (T, U), v
+        assert torch.equal(b.vol_offsets, ref_off), v
+        for k in ("vertices", "normals", "conn", "edges"):
+            assert torch.equal(out[k], ref[k]), (v, k)
+    best, ms = b.autotune(flat, b.alloc(U, T), reps=2)
+    assert b.get_tuning() == best and all(x > 0 for x in ms), (best, ms)
+    out = b.alloc(U, T)
+    b.forward_into(flat, out)
+    torch.cuda.synchronize()
+    for k in ("vertices", "normals", "conn", "edges"):
+        assert torch.equal(out[k], ref[k]), ("autotuned", best, k)
+    # a later batch of the same shape class starts from the cached choice
+    assert mc.Batch(dims, isos, origins, spacings).get_tuning() == best
+    with pytest.raises(Exception, match="invalid parameters"):
+        b.set_tuning([12, 4, 256, 4, 1])
+
+
 def test_batched_nodedup():
     dims, isos, origins, spacings, host = _make_batch(10, 8, 70, 1)
     print("[case] batched nodedup", flush=True)
