@@ -104,6 +104,8 @@ _SIGNATURES = {
     "dax_hip_mc_workspace_bytes": (sz, [i64, i64, i64]),
     "dax_hip_mc_forward": (C.c_int, [vp, i64, i64, i64, vp, vp, f32, C.c_uint, vp, sz,
                                      vp, vp, vp, vp, vp, vp, vp, i64, i64, vp, vp]),
+    "dax_hip_mc_forward_noalloc": (C.c_int, [vp, i64, i64, i64, vp, vp, f32, C.c_uint,
+                                             vp, vp, vp, vp, vp, vp, vp, i64, i64, vp, vp]),
     "dax_hip_mc_read_counts": (C.c_int, [vp, i64, i64, P_i64, P_i64, vp]),
     "dax_hip_mc_classify": (C.c_int, [vp, i64, i64, i64, f32, vp, sz, vp, vp]),
     "dax_hip_mc_slice_classify": (C.c_int, [i64, i64, i64, vp, vp, vp, vp, vp, sz, vp, vp]),

@@ -716,6 +716,70 @@ int dax_hip_mc_forward(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
   return launch_forward(g, d_field, ws, L, flags, o, cap_points, cap_tris, d_counts, tu, s);
 }
 
+// ---- forward without allocation -------------------------------------------
+// dax_hip_mc_forward with its workspace in a file-scope device array, for
+// callers that cannot allocate (code under graph capture, serving loops):
+// grids whose workspace fits kNoAllocScratchBytes (48 MiB: up to 512^3
+// points).  Every GPU the library runs on holds the array until the process
+// ends.  The forward clears what it needs cleared (its classify zeroes the
+// scan's status and control words); calls that overlap on one GPU share the
+// array and race.
+constexpr size_t kNoAllocScratchBytes = (size_t)48 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kNoAllocDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kNoAllocDevices] = {};  // device address of `scratch`, per device
+
+// The current device's copy of `scratch`, looked up once per device (with
+// that device current); host code uses this address, never the array's name.
+int scratch_on_device(void** ptr) {
+  int dev = 0;
+  DAXHIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices) {
+    set_error("dax_hip_mc_forward_noalloc: device %d beyond the scratch table", dev);
+    return DAX_HIP_BAD_VALUE;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return check_hip(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                                                "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return DAX_HIP_OK;
+}
+}  // namespace
+
+int dax_hip_mc_forward_noalloc(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
+                               const float* h_origin3, const float* h_spacing3, float iso,
+                               unsigned flags, float* d_vertices, float* d_normals,
+                               int32_t* d_conn, uint32_t* d_edges, int32_t* d_interp_first,
+                               int32_t* d_interp_second, float* d_interp_weight,
+                               int64_t cap_points, int64_t cap_tris, int64_t* d_counts,
+                               void* stream) {
+  VolGeom g;
+  int rc = make_geom(nx, ny, nz, h_origin3, h_spacing3, iso, g);
+  if (rc) return rc;
+  const Workspace L = workspace_layout(g.nwords);
+  if (L.bytes > kNoAllocScratchBytes) {
+    set_error("dax_hip_mc_forward_noalloc: a %lld x %lld x %lld grid needs %zu bytes of scratch, "
+              "%zu are available",
+              (long long)nx, (long long)ny, (long long)nz, L.bytes, kNoAllocScratchBytes);
+    return DAX_HIP_BAD_VALUE;
+  }
+  if ((rc = check_outputs(g, d_field, flags, d_vertices, d_conn, d_edges, d_counts))) return rc;
+  void* ws = nullptr;
+  if ((rc = scratch_on_device(&ws))) return rc;
+  OutPtrs o{d_vertices, d_normals, d_conn, d_edges, d_interp_first, d_interp_second,
+            d_interp_weight};
+  return launch_forward(g, d_field, (char*)ws, L, flags, o, cap_points, cap_tris, d_counts,
+                        tuning_for(nx, ny, nz), (hipStream_t)stream);
+}
+
 // ---- slice: marching cubes of the plane distance, computed in the kernels
 // (SliceCount / SliceGenerate, dax/worklet/Slice.h:28-173, on a uniform
 // grid; no field in memory).

@@ -199,6 +199,20 @@ int dax_hip_mc_forward(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
                        int32_t* d_interp_second, float* d_interp_weight,
                        int64_t cap_points, int64_t cap_tris, int64_t* d_counts,
                        void* stream);
+/* dax_hip_mc_forward that allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops).  The workspace is a 48 MiB array the
+ * library holds on each GPU it runs on (grids up to 512^3 points; larger
+ * ones return DAX_HIP_BAD_VALUE); every call on a GPU uses that GPU's array,
+ * so calls that overlap on one GPU race.  Arguments as dax_hip_mc_forward
+ * without the workspace. */
+int dax_hip_mc_forward_noalloc(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
+                               const float* h_origin3, const float* h_spacing3, float iso,
+                               unsigned flags, float* d_vertices, float* d_normals,
+                               int32_t* d_conn, uint32_t* d_edges, int32_t* d_interp_first,
+                               int32_t* d_interp_second, float* d_interp_weight,
+                               int64_t cap_points, int64_t cap_tris, int64_t* d_counts,
+                               void* stream);
 
 /* Slice of a uniform grid by a plane: replaces DispatcherMapCell<SliceCount>
  * + DispatcherGenerateInterpolatedCells<SliceGenerate> over the grid's
