@@ -673,11 +673,30 @@ __global__ __launch_bounds__(256) void k_magnitude_vec3(const float* __restrict_
 
 // Sum of n int32 values in 64 bits (integer addition: the order of the
 // per-wave atomics does not change the result).
+// Sum of n int32 values into *out (64-bit): 16-byte loads, four in flight
+// per thread, the unaligned head and the tail by scalar loads.
 __global__ __launch_bounds__(256) void k_sum_i32(const int32_t* __restrict__ in, int64_t n,
                                                  unsigned long long* __restrict__ out) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t head = min(n, (int64_t)(((16 - (reinterpret_cast<uintptr_t>(in) & 15)) & 15) / 4));
+  const int4* in4 = reinterpret_cast<const int4*>(in + head);
+  const int64_t n4 = (n - head) >> 2;
   long long s = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) s += in[i];
+  if (tid < head) s += in[tid];
+  for (int64_t t = head + 4 * n4 + tid; t < n; t += stride) s += in[t];
+  int64_t i = tid;
+  for (; i + 3 * stride < n4; i += 4 * stride) {
+    int4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = in4[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s += (long long)v[u].x + v[u].y + v[u].z + v[u].w;
+  }
+  for (; i < n4; i += stride) {
+    const int4 v = in4[i];
+    s += (long long)v.x + v.y + v.z + v.w;
+  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, (unsigned long long)s);
@@ -697,7 +716,7 @@ int dax_hip_reduce_sum_i32(const int32_t* d_in, int64_t n, int64_t* h_total, voi
   if ((rc = acc.alloc(sizeof(unsigned long long)))) return rc;
   DAXHIP_CHECK(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long), s));
   if (n > 0) {
-    const int64_t blocks = std::min<int64_t>(daxhip::ceil_div(n, 256), 2048);
+    const int64_t blocks = std::min<int64_t>(daxhip::ceil_div(n, 1024), 4096);
     daxhip::k_sum_i32<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
         d_in, n, (unsigned long long*)acc.p);
     DAXHIP_LAUNCH_CHECK("k_sum_i32");

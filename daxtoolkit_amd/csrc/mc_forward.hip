@@ -235,10 +235,13 @@ __global__ __launch_bounds__(256) void k_classify_plane(VolGeom g, FieldPlane pl
 // Tuning: per grid shape, the classify variant, scan sub-tiles and generate
 // tile words.  Defaults are used until dax_hip_mc_autotune (or
 // dax_hip_mc_set_tuning) records a choice for a shape.
+// The defaults are the 512^3 workload's autotuned choice (classify falls
+// back to one float per lane, 16 words per wave, on rows that are not
+// 64-point aligned).
 struct Tuning {
-  int cls_unroll = 16;  // words (vec 1) or 256-point chunks (vec 4) per wave
-  int cls_vec = 1;      // floats per lane per load
-  int scan_sub = 4;     // count tiles (waves) per count block
+  int cls_unroll = 4;   // words (vec 1) or 256-point chunks (vec 4) per wave
+  int cls_vec = 4;      // floats per lane per load
+  int scan_sub = 8;     // count tiles (waves) per count block
   int gen_tile = 512;   // generate tile words
   int gen_xcd = 0;      // 1: each XCD takes a contiguous range of generate tiles
 };
@@ -265,34 +268,62 @@ bool valid_tuning(const Tuning& t) {
 
 // ---------------------------------------------------------------------------
 // Per-cell outputs of the count step: case byte, NumFaces[case] and the
-// exclusive triangle offset.  One wavefront per row word; lane = cell.
+// exclusive triangle offset.  A wavefront takes kCellWords (8) consecutive
+// row words: one load instruction brings all their neighbour words (lane
+// 8u+v holds neighbour v of word u), each word then reads its eight from
+// the lanes and writes its 64 cells (lane = cell).
+constexpr int kCellWords = 8;
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+  return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
 __global__ __launch_bounds__(256) void k_cell_outputs(VolGeom g, const uint64_t* __restrict__ bits,
                                                       const uint32_t* __restrict__ base_t,
                                                       uint8_t* __restrict__ cases,
                                                       int32_t* __restrict__ count,
                                                       int32_t* __restrict__ offsets) {
+  __shared__ uint8_t s_nf[256];  // NumFaces per case
+  s_nf[threadIdx.x] = (uint8_t)(c_case_table[threadIdx.x] >> 60);
+  __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (w >= g.nwords) return;
-  const WordPos p = word_pos(g, w);
-  if (!(p.has_y && p.has_z)) return;  // no cells in this row
+  const int64_t w0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kCellWords;
+  if (w0 >= g.nwords) return;  // wave-uniform
   const uint64_t* b = bits + g.word_off;
   const int64_t plane = (int64_t)g.wpr * g.ny;
-  const bool nx1 = p.q + 1 < g.wpr;
-  CellPlanes cp;
-  cell_planes(p, b[w], nx1 ? b[w + 1] : 0ull, b[w + g.wpr], nx1 ? b[w + g.wpr + 1] : 0ull,
-              b[w + plane], nx1 ? b[w + plane + 1] : 0ull, b[w + plane + g.wpr],
-              nx1 ? b[w + plane + g.wpr + 1] : 0ull, cp);
-  const bool valid = ((p.xmask >> lane) & 1ull) != 0;
-  const int cs = cell_case(cp, lane);
-  const uint32_t nf = valid ? (uint32_t)(c_case_table[cs] >> 60) : 0u;
-  const uint32_t incl = wave_incl_scan(nf);
-  if (!valid) return;
+  uint64_t mine = 0ull;  // neighbour (lane & 7) of word w0 + (lane >> 3)
+  {
+    const int64_t w = w0 + (lane >> 3);
+    const int v = lane & 7;
+    if (w < g.nwords) {
+      const WordPos p = word_pos(g, w);
+      if (p.has_y && p.has_z && (!(v & 1) || p.q + 1 < g.wpr))
+        mine = b[w + (v & 1) + ((v >> 1) & 1) * g.wpr + (v >> 2) * plane];
+    }
+  }
   const int64_t cx = g.nx - 1, cy = g.ny - 1;
-  const int64_t cell = (int64_t)p.q * 64 + lane + cx * (p.j + cy * (int64_t)p.k);
-  if (cases) cases[cell] = (uint8_t)cs;
-  if (count) count[cell] = (int32_t)nf;
-  if (offsets) offsets[cell] = (int32_t)(base_t[g.word_off + w] + incl - nf);
+#pragma unroll
+  for (int u = 0; u < kCellWords; ++u) {
+    const int64_t w = w0 + u;
+    const WordPos p = word_pos(g, w < g.nwords ? w : w0);
+    if (w >= g.nwords || !(p.has_y && p.has_z)) continue;  // wave-uniform: no cells
+    uint64_t nb[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) nb[v] = readlane64(mine, 8 * u + v);
+    CellPlanes cp;
+    cell_planes(p, nb[0], nb[1], nb[2], nb[3], nb[4], nb[5], nb[6], nb[7], cp);
+    const bool valid = ((p.xmask >> lane) & 1ull) != 0;
+    const int cs = cell_case(cp, lane);
+    const uint32_t nf = valid ? (uint32_t)s_nf[cs] : 0u;
+    const uint32_t incl = offsets ? wave_incl_scan(nf) : 0u;
+    if (valid) {
+      const int64_t cell = (int64_t)p.q * 64 + lane + cx * (p.j + cy * (int64_t)p.k);
+      if (cases) cases[cell] = (uint8_t)cs;
+      if (count) count[cell] = (int32_t)nf;
+      if (offsets) offsets[cell] = (int32_t)(base_t[g.word_off + w] + incl - nf);
+    }
+  }
 }
 
 __global__ void k_interpolate_scalar(const float* __restrict__ in, const int32_t* __restrict__ first,
@@ -858,7 +889,7 @@ int dax_hip_mc_cell_outputs(int64_t nx, int64_t ny, int64_t nz, const void* d_wo
   Workspace L;
   if ((rc = check_ws(g, workspace_bytes, L, (void*)d_workspace))) return rc;
   const char* ws = (const char*)d_workspace;
-  const int64_t blocks = ceil_div(g.nwords * 64, 256);
+  const int64_t blocks = ceil_div(ceil_div(g.nwords, (int64_t)kCellWords) * 64, 256);
   if (blocks == 0) return DAX_HIP_OK;
   k_cell_outputs<<<dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream>>>(
       g, (const uint64_t*)(ws + L.off_bits), (const uint32_t*)(ws + L.off_base_t), d_cases,

@@ -156,3 +156,16 @@ def test_copy_ragged():
         _lib.call("dax_hip_copy", dst.data_ptr() + 1, src.data_ptr() + 1, n, S())
         torch.cuda.synchronize()
         assert torch.equal(dst[1:], src[1:]) and dst[0] == 0
+
+
+@pytest.mark.parametrize("n,off", [(1, 0), (3, 1), (4097, 0), (4097, 3), (1_000_003, 2)])
+def test_reduce_sum_i32(n, off):
+    """dax_hip_reduce_sum_i32 (the count-handle total check of the generate
+    dispatcher) against numpy's int64 sum, at unaligned starts and ragged
+    tails, with values whose int32 sum would wrap."""
+    rng = np.random.default_rng(n + off)
+    a = rng.integers(-2**31, 2**31 - 1, size=n + off, dtype=np.int64).astype(np.int32)
+    d = dev(a)
+    total = ctypes.c_int64()
+    _lib.call("dax_hip_reduce_sum_i32", d.data_ptr() + 4 * off, n, ctypes.byref(total), S())
+    assert total.value == int(a[off:].astype(np.int64).sum())
