@@ -274,10 +274,12 @@ struct Ctrl {
 // both below 2^16); off_tsum every count tile's exclusive prefix within its
 // count block (v | t << 32); off_bsum the count blocks' sums and off_tpre
 // their exclusive prefixes, the totals after the last.  A word's bases are
-// block prefix + tile prefix + local prefix.
+// block prefix + tile prefix + local prefix.  off_cw: every count tile's
+// output size (vertices + triangles); off_order: the generate tile size the
+// order was built for (0: none), then the generate tiles in dispatch order.
 struct Workspace {
   size_t off_bits, off_base_v, off_base_t, off_status, off_ctrl, off_counts, off_tsum, off_bsum,
-      off_tpre, bytes;
+      off_tpre, off_cw, off_order, bytes;
   int64_t nwords, ntiles, n_ctiles;
 };
 
@@ -305,6 +307,10 @@ inline Workspace workspace_layout(int64_t nwords) {
   off = align_up(off + (size_t)(w.n_ctiles + 1) * 8, 256);
   w.off_tpre = off;  // per count block: exclusive prefixes, then the totals
   off = align_up(off + (size_t)(w.n_ctiles + 1) * 8, 256);
+  w.off_cw = off;  // per count tile: vertices + triangles
+  off = align_up(off + (size_t)(w.n_ctiles + 1) * 4, 256);
+  w.off_order = off;  // generate tiles, heaviest first: the tile size, then the tiles
+  off = align_up(off + (size_t)(w.n_ctiles + 2) * 4, 256);
   w.bytes = off;
   return w;
 }

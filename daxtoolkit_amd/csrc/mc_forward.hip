@@ -58,7 +58,8 @@ __global__ __launch_bounds__(256) void k_classify_v4(VolGeom g, const float* __r
 // 1. k_count: a block of WAVES waves, each wave two count tiles of
 //    kCountTile row words (four consecutive words per lane, coalesced loads
 //    of the words and their +x/+y/+z neighbours): every word's prefix within
-//    its tile, every tile's prefix within the block and the block's sums.
+//    its tile, every tile's prefix within the block and output size, and the
+//    block's sums.
 //    No inter-block dependency.
 // 2. k_scan_tsums: one block scans the blocks' sums into exclusive prefixes
 //    (the totals after the last) and the 64-bit totals of the control block.
@@ -70,6 +71,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_count(VolGeom g, const uint64_t*
                                                       uint32_t* __restrict__ lpre,
                                                       unsigned long long* __restrict__ tloc,
                                                       unsigned long long* __restrict__ bsum,
+                                                      uint32_t* __restrict__ cw,
                                                       int64_t n_ctiles) {
   __shared__ unsigned long long s_ts[2 * WAVES];
   const int wv = threadIdx.x >> 6;
@@ -79,6 +81,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_count(VolGeom g, const uint64_t*
   if ((threadIdx.x & 63) == 0) {
     s_ts[2 * wv] = s0;
     s_ts[2 * wv + 1] = s1;
+    if (2 * pair < n_ctiles) cw[2 * pair] = (uint32_t)s0 + (uint32_t)(s0 >> 32);
+    if (2 * pair + 1 < n_ctiles) cw[2 * pair + 1] = (uint32_t)s1 + (uint32_t)(s1 >> 32);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -92,16 +96,79 @@ __global__ __launch_bounds__(64 * WAVES) void k_count(VolGeom g, const uint64_t*
   }
 }
 
+// Dispatch order of the generate tiles, built by the scan block: first the
+// heavy tiles (output size, vertices + triangles, at least 1.5 times the
+// mean), then the other tiles with output, then the empty ones; within a
+// class by wave, round and lane of the block, i.e. by tile index.  The
+// generate grid dispatches its blocks roughly in index order as slots free
+// up, so the longest tiles start first and the short ones fill the tail (a
+// coarse longest-processing-time-first schedule).  Every tile writes at its
+// bases, so the order changes no output.
+constexpr int kGenClasses = 3;
+constexpr int kOrderMaxTiles = 32768;  // generate tiles the order covers (a byte of LDS each)
+constexpr int kOrderRegRounds = 8;     // rounds of tile weights loaded before the scan
+struct GenOrder {
+  const uint32_t* __restrict__ cw;  // per count tile: vertices + triangles
+  int64_t n_ctiles, nwords;
+  int tw;           // generate tile words: 128, 256, 512 or 1024
+  uint32_t* order;  // [0]: tw (0: no order), [1 + i]: the i-th tile to dispatch
+};
+
+// Output size of generate tile t (t * cpt < n_ctiles): its count tiles'
+// sizes, one vector load when cpt is 2, 4 or 8 (the region is 256-byte
+// aligned, so the loads are aligned).
+__device__ __forceinline__ uint32_t gen_tile_weight(const GenOrder& go, int cpt, uint32_t t) {
+  const uint32_t c0 = t * (uint32_t)cpt;
+  if (c0 + (uint32_t)cpt > (uint32_t)go.n_ctiles) {  // the last, partial tile
+    uint32_t w = 0u;
+    for (uint32_t c = c0; c < (uint32_t)go.n_ctiles; ++c) w += go.cw[c];
+    return w;
+  }
+  switch (cpt) {
+    case 1: return go.cw[c0];
+    case 2: {
+      const uint2 v = *reinterpret_cast<const uint2*>(go.cw + c0);
+      return v.x + v.y;
+    }
+    case 4: {
+      const uint4 v = *reinterpret_cast<const uint4*>(go.cw + c0);
+      return (v.x + v.y) + (v.z + v.w);
+    }
+    default: {
+      const uint4 v = *reinterpret_cast<const uint4*>(go.cw + c0);
+      const uint4 u = *reinterpret_cast<const uint4*>(go.cw + c0 + 4);
+      return ((v.x + v.y) + (v.z + v.w)) + ((u.x + u.y) + (u.z + u.w));
+    }
+  }
+}
+
 // One block scans the n count-block sums (v | t << 32, each below 2^32)
 // 1024 at a time: a coalesced load, wave scans of the halves, the block's
-// wave sums in 64 bits, 64-bit carries between rounds.
+// wave sums in 64 bits, 64-bit carries between rounds.  The same block
+// builds the generate tiles' dispatch order: the weights of the first
+// kOrderRegRounds rounds of tiles are loaded before the scan (the latencies
+// overlap), the classes follow from the totals, per-wave class counts come
+// from ballots, and each entry's slot from its wave's ballot rank.
 constexpr int kTsumThreads = 1024;
 __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long long* __restrict__ bsum,
                                                              unsigned long long* __restrict__ bpre,
-                                                             int64_t n, Ctrl* __restrict__ ctrl) {
+                                                             int64_t n, Ctrl* __restrict__ ctrl,
+                                                             GenOrder go) {
   constexpr int kWaves = kTsumThreads / 64;
   __shared__ uint32_t s_v[kWaves], s_t[kWaves];
+  __shared__ uint32_t s_base[kGenClasses * kWaves];
+  __shared__ uint8_t s_cls[kOrderMaxTiles];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const uint32_t ng = (uint32_t)ceil_div(go.nwords, (int64_t)go.tw);
+  const bool ordered = ng <= (uint32_t)kOrderMaxTiles;  // uniform
+  const int cpt = go.tw / kCountTile;
+  uint32_t wreg[kOrderRegRounds];
+#pragma unroll
+  for (int r = 0; r < kOrderRegRounds; ++r) {
+    const uint32_t t = (uint32_t)tid + (uint32_t)r * kTsumThreads;
+    wreg[r] = (ordered && t < ng) ? gen_tile_weight(go, cpt, t) : 0u;
+  }
+
   unsigned long long cv = 0, ct = 0;  // 64-bit carries
   for (int64_t e0 = 0; e0 < n; e0 += kTsumThreads) {
     const int64_t e = e0 + tid;
@@ -136,6 +203,63 @@ __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long
     ctrl->total_v = cv;
     ctrl->total_t = ct;
   }
+  if (!ordered) {
+    if (tid == 0) go.order[0] = 0u;
+    return;
+  }
+
+  // classes: heavy when 2 * w * ng >= 3 * (total vertices + triangles)
+  const unsigned long long tot3 = 3ull * (cv + ct);  // every thread holds the totals
+  auto cls_of = [&](uint32_t w) -> int {
+    return w == 0u ? 2 : (2ull * w * ng >= tot3 ? 0 : 1);
+  };
+  const uint32_t rounds = (ng + kTsumThreads - 1) / kTsumThreads;  // uniform
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t t = (uint32_t)tid + r * kTsumThreads;
+    uint32_t w = 0u;
+#pragma unroll
+    for (int q = 0; q < kOrderRegRounds; ++q)
+      if (q == (int)r) w = wreg[q];
+    if (r >= (uint32_t)kOrderRegRounds && t < ng) w = gen_tile_weight(go, cpt, t);
+    if (t < ng) s_cls[t] = (uint8_t)cls_of(w);
+  }
+  __syncthreads();
+  uint32_t cnt[kGenClasses] = {0u, 0u, 0u};  // the wave's entries of each class (uniform)
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t t = (uint32_t)tid + r * kTsumThreads;
+    const int c = t < ng ? (int)s_cls[t] : -1;
+#pragma unroll
+    for (int k = 0; k < kGenClasses; ++k) cnt[k] += (uint32_t)__popcll(__ballot(c == k));
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < kGenClasses; ++k) s_base[k * kWaves + wv] = cnt[k];
+  }
+  __syncthreads();
+  if (wv == 0) {  // exclusive scan of the kGenClasses * kWaves <= 64 counts
+    const uint32_t x = lane < kGenClasses * kWaves ? s_base[lane] : 0u;
+    const uint32_t inc = wave_incl_scan(x);
+    if (lane < kGenClasses * kWaves) s_base[lane] = 1u + inc - x;  // entry 0: the tile size
+    if (lane == 0) go.order[0] = (uint32_t)go.tw;
+  }
+  __syncthreads();
+  uint32_t run[kGenClasses] = {0u, 0u, 0u};  // the wave's entries of each class placed so far
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t t = (uint32_t)tid + r * kTsumThreads;
+    const int c = t < ng ? (int)s_cls[t] : -1;
+    uint64_t mk[kGenClasses];
+#pragma unroll
+    for (int k = 0; k < kGenClasses; ++k) mk[k] = __ballot(c == k);
+    if (t < ng) {
+      const uint64_t m = c == 0 ? mk[0] : (c == 1 ? mk[1] : mk[2]);
+      const uint32_t before = c == 0 ? run[0] : (c == 1 ? run[1] : run[2]);
+      const uint32_t rank =
+          __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      go.order[s_base[c * kWaves + wv] + before + rank] = t;
+    }
+#pragma unroll
+    for (int k = 0; k < kGenClasses; ++k) run[k] += (uint32_t)__popcll(mk[k]);
+  }
 }
 
 // Bases of every row word (and entry nwords): tile prefix + local prefix.
@@ -152,7 +276,7 @@ __global__ __launch_bounds__(kScanThreads) void k_generate(
     VolGeom g, Src field, const uint64_t* __restrict__ bits, CountSrc cs,
     const Ctrl* __restrict__ ctrl, OutPtrs out, unsigned flags, int64_t cap_points,
     int64_t cap_tris, int64_t* __restrict__ counts, int xcd_order, int64_t tile_lo,
-    int64_t ntiles) {
+    int64_t ntiles, const uint32_t* __restrict__ order) {
   extern __shared__ uint64_t s_dyn[];
   const bool dedup = (flags & DAX_HIP_MC_DEDUP) != 0;
   const bool normals = (flags & DAX_HIP_MC_NORMALS) != 0 && out.normals != nullptr;
@@ -166,7 +290,17 @@ __global__ __launch_bounds__(kScanThreads) void k_generate(
   // and the per-cell outputs read them), no output
   const bool emit = total_v <= 0x7fffffffull && 3ull * total_t <= 0x7fffffffull &&
                     (int64_t)total_t <= cap_tris && (int64_t)n_points <= cap_points;
-  const int64_t tile = tile_lo + (xcd_order ? xcd_tile(blockIdx.x, ntiles) : (int64_t)blockIdx.x);
+  // the scan's dispatch order when it was built for this tile size (whole
+  // volume launches only), else index order
+  uint32_t tag = 0u, ord = 0u;
+  if (order != nullptr) {
+    tag = order[0];
+    ord = order[1 + blockIdx.x];
+  }
+  const int64_t tile =
+      (tag == (uint32_t)TW && (int64_t)ord < ntiles)
+          ? (int64_t)ord
+          : tile_lo + (xcd_order ? xcd_tile(blockIdx.x, ntiles) : (int64_t)blockIdx.x);
   generate_tile<TW, Src, true>(g, field, bits + g.word_off, nullptr, nullptr, tile * TW, out,
                                dedup, normals, 0, 0, s_dyn, cs, emit);
 }
@@ -518,11 +652,20 @@ int launch_bits(const VolGeom& g, const float* field, char* ws, const Workspace&
   return DAX_HIP_OK;
 }
 
+// Generate tile words of a tuning (launch_generate_src's dispatch).
+int gen_tile_words(const Tuning& tu) {
+  return (tu.gen_tile == 128 || tu.gen_tile == 256 || tu.gen_tile == 1024) ? tu.gen_tile : 512;
+}
+
+// Where the generate kernel reads the word bases; a count block holds two
+// count tiles per wave, with the waves of launch_scan's dispatch (1, 2, 8,
+// else 4).
 CountSrc count_src(const char* ws, const Workspace& L, const Tuning& tu) {
   char* wsm = const_cast<char*>(ws);
   return CountSrc{(const uint32_t*)(ws + L.off_counts),
                   (const unsigned long long*)(ws + L.off_tsum),
-                  (const unsigned long long*)(ws + L.off_tpre), L.n_ctiles, 2 * tu.scan_sub,
+                  (const unsigned long long*)(ws + L.off_tpre), L.n_ctiles,
+                  tu.scan_sub == 1 ? 1 : (tu.scan_sub == 2 ? 2 : (tu.scan_sub == 8 ? 4 : 3)),
                   (uint32_t*)(wsm + L.off_base_v), (uint32_t*)(wsm + L.off_base_t)};
 }
 
@@ -537,20 +680,23 @@ int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu
   unsigned long long* tloc = (unsigned long long*)(ws + L.off_tsum);
   unsigned long long* bsum = (unsigned long long*)(ws + L.off_bsum);
   unsigned long long* bpre = (unsigned long long*)(ws + L.off_tpre);
+  uint32_t* cw = (uint32_t*)(ws + L.off_cw);
   const int64_t n = L.n_ctiles;
   const int waves = tu.scan_sub;
   const int64_t nblocks = ceil_div(n, 2 * waves);
   if (nblocks > 0) {
     const dim3 grid((unsigned)nblocks), block(64 * waves);
     switch (waves) {
-      case 1: k_count<1><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, n); break;
-      case 2: k_count<2><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, n); break;
-      case 8: k_count<8><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, n); break;
-      default: k_count<4><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, n); break;
+      case 1: k_count<1><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, cw, n); break;
+      case 2: k_count<2><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, cw, n); break;
+      case 8: k_count<8><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, cw, n); break;
+      default: k_count<4><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, cw, n); break;
     }
     DAXHIP_LAUNCH_CHECK("k_count");
   }
-  k_scan_tsums<<<1, dim3(kTsumThreads), 0, s>>>(bsum, bpre, nblocks, (Ctrl*)(ws + L.off_ctrl));
+  // with the dispatch order of the generate tiles
+  const GenOrder go{cw, L.n_ctiles, g.nwords, gen_tile_words(tu), (uint32_t*)(ws + L.off_order)};
+  k_scan_tsums<<<1, dim3(kTsumThreads), 0, s>>>(bsum, bpre, nblocks, (Ctrl*)(ws + L.off_ctrl), go);
   DAXHIP_LAUNCH_CHECK("k_scan_tsums");
   if (bases) {
     k_bases<<<dim3((unsigned)ceil_div(g.nwords + 1, 256)), dim3(256), 0, s>>>(
@@ -568,12 +714,15 @@ int launch_generate_tw(const VolGeom& g, const Src& field, const char* ws, const
                        int64_t t_hi) {
   const int xcd = tu.gen_xcd;
   const size_t lds = GenLayout<TW>(g.wpr).bytes();
-  if (t_hi < 0) t_hi = ceil_div(g.nwords, (int64_t)TW);
+  const int64_t all = ceil_div(g.nwords, (int64_t)TW);
+  if (t_hi < 0) t_hi = all;
   const int64_t ntiles = std::max<int64_t>(t_hi - t_lo, 1);
   const CountSrc cs = count_src(ws, L, tu);
+  const uint32_t* order =
+      (t_lo == 0 && t_hi == all) ? (const uint32_t*)(ws + L.off_order) : nullptr;
   k_generate<TW, Src><<<dim3((unsigned)ntiles), dim3(kScanThreads), lds, s>>>(
       g, field, (const uint64_t*)(ws + L.off_bits), cs, (const Ctrl*)(ws + L.off_ctrl), o, flags,
-      cap_points, cap_tris, counts, xcd, t_lo, ntiles);
+      cap_points, cap_tris, counts, xcd, t_lo, ntiles, order);
   DAXHIP_LAUNCH_CHECK("k_generate");
   return DAX_HIP_OK;
 }

@@ -408,10 +408,11 @@ struct OutPtrs {
 
 // Point-field sources of the forward: a float32 field in memory, or the
 // signed plane distance of a slice computed from the point's coordinates.
-// src(x, y, z, p) is the value at point (x, y, z) of flat index p.
+// src(x, y, z, p) is the value at point (x, y, z) of flat index p (a volume
+// holds fewer than 2^31 points, so 32-bit index arithmetic is exact).
 struct FieldArray {
   const float* __restrict__ f;
-  __device__ __forceinline__ float operator()(int, int, int, int64_t p) const { return f[p]; }
+  __device__ __forceinline__ float operator()(int, int, int, int32_t p) const { return f[p]; }
 };
 
 // The field SliceCount / SliceGenerate classify and interpolate
@@ -422,7 +423,7 @@ struct FieldArray {
 struct FieldPlane {
   float o[3], s[3];  // grid origin and spacing
   float po[3], pn[3];  // plane origin and normal
-  __device__ __forceinline__ float operator()(int x, int y, int z, int64_t) const {
+  __device__ __forceinline__ float operator()(int x, int y, int z, int32_t) const {
     const int c[3] = {x, y, z};
     float r = 0.0f;
 #pragma unroll
@@ -440,11 +441,11 @@ struct FieldPlane {
 template <class Src>
 __device__ __forceinline__ void point_gradient(const VolGeom& g, const Src& f, int x, int y,
                                                int z, float out[3]) {
-  const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-  const int64_t p = x + sy * y + sz * z;
+  const int32_t sy = g.nx, sz = g.nx * g.ny;
+  const int32_t p = x + sy * y + sz * z;
   const int c[3] = {x, y, z};
   const int n[3] = {g.nx, g.ny, g.nz};
-  const int64_t st[3] = {1, sy, sz};
+  const int32_t st[3] = {1, sy, sz};
   float lo_v[3], hi_v[3];
   bool wide[3];
 #pragma unroll
@@ -471,8 +472,8 @@ __device__ __forceinline__ void point_gradient(const VolGeom& g, const Src& f, i
 template <class Src>
 __device__ __forceinline__ void emit_point(const VolGeom& g, const Src& f, int x, int y, int z,
                                            int axis, bool normals, float xyz[3], float nrm[3],
-                                           float& w, int64_t& pa, int64_t& pb) {
-  const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
+                                           float& w, int32_t& pa, int32_t& pb) {
+  const int32_t sy = g.nx, sz = g.nx * g.ny;
   pa = x + sy * y + sz * z;
   pb = pa + (axis == 0 ? 1 : (axis == 1 ? sy : sz));
   const float fa = f(x, y, z, pa),
@@ -603,14 +604,15 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total,
 // bases from: the count kernel's per-word prefixes within their count tiles
 // (lpre, v | t << 16), every count tile's prefix within its count block of
 // tpb tiles (tloc, v | t << 32) and the count blocks' exclusive prefixes
-// (bpre, the totals after the last).  It writes the bases of its words to
+// (bpre, the totals after the last); the tiles per block are a power of
+// two.  It writes the bases of its words to
 // base_v / base_t, which the adjoint and the per-cell outputs read.
 struct CountSrc {
   const uint32_t* __restrict__ lpre;
   const unsigned long long* __restrict__ tloc;
   const unsigned long long* __restrict__ bpre;
   int64_t n_ctiles;
-  int tpb;
+  int tpb_shift;  // log2 of the count tiles per count block
   uint32_t* __restrict__ base_v;
   uint32_t* __restrict__ base_t;
 };
@@ -623,8 +625,9 @@ __device__ __forceinline__ unsigned long long add_vt(unsigned long long a, unsig
 // Exclusive vertex / triangle prefix (v | t << 32) of count tile c; for c at
 // or past the last tile, the totals.
 __device__ __forceinline__ unsigned long long ctile_prefix(const CountSrc& cs, int64_t c) {
-  if (c >= cs.n_ctiles) return cs.bpre[(cs.n_ctiles + cs.tpb - 1) / cs.tpb];
-  return add_vt(cs.bpre[c / cs.tpb], cs.tloc[c]);
+  if (c >= cs.n_ctiles)
+    return cs.bpre[(cs.n_ctiles + (1 << cs.tpb_shift) - 1) >> cs.tpb_shift];
+  return add_vt(cs.bpre[c >> cs.tpb_shift], cs.tloc[c]);
 }
 
 // Vertex and triangle base of word w (v | t << 32); past the last count
@@ -786,7 +789,7 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
       int bit, axis;
       edge_select(fx, fy, fz, (int)(tv0 + (uint32_t)vl - s_v0[li]), bit, axis);
       float xyz[3], nrm[3], w;
-      int64_t pa, pb;
+      int32_t pa, pb;
       emit_point(g, f, p.q * 64 + bit, p.j, p.k, axis, normals, xyz, nrm, w, pa, pb);
       const int64_t gv = (int64_t)tv0 + vl;
       out.vertices[3 * gv + 0] = xyz[0];
@@ -853,7 +856,7 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
         } else {
           const int64_t corner = 3 * t + v;
           float xyz[3], nrm[3], w;
-          int64_t pa, pb;
+          int32_t pa, pb;
           emit_point(g, f, x, p.j + dy, p.k + dz, axis, normals, xyz, nrm, w, pa, pb);
           out.vertices[3 * corner + 0] = xyz[0];
           out.vertices[3 * corner + 1] = xyz[1];
