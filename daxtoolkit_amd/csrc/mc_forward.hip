@@ -113,6 +113,7 @@ struct GenOrder {
   int tw;           // generate tile words: 128, 256, 512 or 1024
   uint32_t* order;  // [0]: tw (0: no order), [1 + i]: the i-th tile to dispatch
 };
+constexpr uint32_t kOrderHasOutput = 0x80000000u;  // order entry flag: the tile has output
 
 // Output size of generate tile t (t * cpt < n_ctiles): its count tiles'
 // sizes, one vector load when cpt is 2, 4 or 8 (the region is 256-byte
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long
       const uint32_t before = c == 0 ? run[0] : (c == 1 ? run[1] : run[2]);
       const uint32_t rank =
           __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-      go.order[s_base[c * kWaves + wv] + before + rank] = t;
+      go.order[s_base[c * kWaves + wv] + before + rank] = t | (c < 2 ? kOrderHasOutput : 0u);
     }
 #pragma unroll
     for (int k = 0; k < kGenClasses; ++k) run[k] += (uint32_t)__popcll(mk[k]);
@@ -272,7 +273,7 @@ __global__ __launch_bounds__(256) void k_bases(VolGeom g, CountSrc cs) {
 }
 
 template <int TW, class Src>
-__global__ __launch_bounds__(kScanThreads) void k_generate(
+__global__ __launch_bounds__(kScanThreads, 5) void k_generate(
     VolGeom g, Src field, const uint64_t* __restrict__ bits, CountSrc cs,
     const Ctrl* __restrict__ ctrl, OutPtrs out, unsigned flags, int64_t cap_points,
     int64_t cap_tris, int64_t* __restrict__ counts, int xcd_order, int64_t tile_lo,
@@ -297,12 +298,13 @@ __global__ __launch_bounds__(kScanThreads) void k_generate(
     tag = order[0];
     ord = order[1 + blockIdx.x];
   }
+  const bool ordered = tag == (uint32_t)TW && (int64_t)(ord & ~kOrderHasOutput) < ntiles;
   const int64_t tile =
-      (tag == (uint32_t)TW && (int64_t)ord < ntiles)
-          ? (int64_t)ord
-          : tile_lo + (xcd_order ? xcd_tile(blockIdx.x, ntiles) : (int64_t)blockIdx.x);
+      ordered ? (int64_t)(ord & ~kOrderHasOutput)
+              : tile_lo + (xcd_order ? xcd_tile(blockIdx.x, ntiles) : (int64_t)blockIdx.x);
   generate_tile<TW, Src, true>(g, field, bits + g.word_off, nullptr, nullptr, tile * TW, out,
-                               dedup, normals, 0, 0, s_dyn, cs, emit);
+                               dedup, normals, 0, 0, s_dyn, cs, emit,
+                               ordered && (ord & kOrderHasOutput) != 0u);
 }
 
 // Slice classify: a wave per kPlaneRowsPerWave grid rows.  Along a row the

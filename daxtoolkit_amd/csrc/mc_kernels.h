@@ -437,53 +437,86 @@ struct FieldPlane {
 
 // Central-difference gradient at (x,y,z): interior (f(p+e)-f(p-e))/(2h),
 // first plane (f(p+e)-f(p))/h, last plane (f(p)-f(p-e))/h, and 0 along an
-// axis with a single point.  Written branch-free so all loads issue at once.
+// axis with a single point.  Split into the loads (branch-free, so they all
+// issue at once) and the arithmetic, so callers can overlap other work with
+// the loads' latency.
 template <class Src>
-__device__ __forceinline__ void point_gradient(const VolGeom& g, const Src& f, int x, int y,
-                                               int z, float out[3]) {
+__device__ __forceinline__ void gradient_loads(const VolGeom& g, const Src& f, int x, int y, int z,
+                                               float lo_v[3], float hi_v[3]) {
   const int32_t sy = g.nx, sz = g.nx * g.ny;
   const int32_t p = x + sy * y + sz * z;
   const int c[3] = {x, y, z};
   const int n[3] = {g.nx, g.ny, g.nz};
   const int32_t st[3] = {1, sy, sz};
-  float lo_v[3], hi_v[3];
-  bool wide[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int dl = c[d] > 0 ? 1 : 0, dh = c[d] < n[d] - 1 ? 1 : 0;
-    wide[d] = c[d] > 0 && c[d] < n[d] - 1;
     lo_v[d] = f(x - (d == 0) * dl, y - (d == 1) * dl, z - (d == 2) * dl, p - dl * st[d]);
     hi_v[d] = f(x + (d == 0) * dh, y + (d == 1) * dh, z + (d == 2) * dh, p + dh * st[d]);
   }
+}
+
+__device__ __forceinline__ void gradient_finish(const VolGeom& g, int x, int y, int z,
+                                                const float lo_v[3], const float hi_v[3],
+                                                float out[3]) {
+  const int c[3] = {x, y, z};
+  const int n[3] = {g.nx, g.ny, g.nz};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
+    const bool wide = c[d] > 0 && c[d] < n[d] - 1;
     const float h = g.spacing[d];
     const float diff = hi_v[d] - lo_v[d];
     if (n[d] < 2)
       out[d] = 0.0f;
     else if ((g.pow2_h >> d) & 1u)  // exact: the same value as the division
-      out[d] = diff * (wide[d] ? 0.5f * g.rcp_h[d] : g.rcp_h[d]);
+      out[d] = diff * (wide ? 0.5f * g.rcp_h[d] : g.rcp_h[d]);
     else
-      out[d] = diff / (wide[d] ? 2.0f * h : h);
+      out[d] = diff / (wide ? 2.0f * h : h);
   }
 }
 
-// One interpolated point on the edge with lower endpoint (x,y,z) along axis.
 template <class Src>
-__device__ __forceinline__ void emit_point(const VolGeom& g, const Src& f, int x, int y, int z,
-                                           int axis, bool normals, float xyz[3], float nrm[3],
-                                           float& w, int32_t& pa, int32_t& pb) {
+__device__ __forceinline__ void point_gradient(const VolGeom& g, const Src& f, int x, int y,
+                                               int z, float out[3]) {
+  float lo_v[3], hi_v[3];
+  gradient_loads(g, f, x, y, z, lo_v, hi_v);
+  gradient_finish(g, x, y, z, lo_v, hi_v, out);
+}
+
+// Field values of one interpolated point on the edge with lower endpoint
+// (x,y,z) along axis: the end values and, with normals, the central-
+// difference neighbours of both ends.
+struct PointLoads {
+  float fa, fb;
+  float alo[3], ahi[3], blo[3], bhi[3];
+};
+
+template <class Src>
+__device__ __forceinline__ void point_loads(const VolGeom& g, const Src& f, int x, int y, int z,
+                                            int axis, bool normals, PointLoads& L, int32_t& pa,
+                                            int32_t& pb) {
   const int32_t sy = g.nx, sz = g.nx * g.ny;
   pa = x + sy * y + sz * z;
   pb = pa + (axis == 0 ? 1 : (axis == 1 ? sy : sz));
-  const float fa = f(x, y, z, pa),
-              fb = f(x + (axis == 0), y + (axis == 1), z + (axis == 2), pb);
+  L.fa = f(x, y, z, pa);
+  L.fb = f(x + (axis == 0), y + (axis == 1), z + (axis == 2), pb);
+  if (normals) {
+    gradient_loads(g, f, x, y, z, L.alo, L.ahi);
+    gradient_loads(g, f, x + (axis == 0), y + (axis == 1), z + (axis == 2), L.blo, L.bhi);
+  }
+}
+
+// Coordinates, unit normal (central differences interpolated along the edge)
+// and weight of the point whose field values L holds.
+__device__ __forceinline__ void point_finish(const VolGeom& g, int x, int y, int z, int axis,
+                                             bool normals, const PointLoads& L, float xyz[3],
+                                             float nrm[3], float& w) {
   float ga[3], gb[3];
   if (normals) {
-    point_gradient(g, f, x, y, z, ga);
-    point_gradient(g, f, x + (axis == 0), y + (axis == 1), z + (axis == 2), gb);
+    gradient_finish(g, x, y, z, L.alo, L.ahi, ga);
+    gradient_finish(g, x + (axis == 0), y + (axis == 1), z + (axis == 2), L.blo, L.bhi, gb);
   }
-  w = (g.iso - fa) / (fb - fa);
+  w = (g.iso - L.fa) / (L.fb - L.fa);
   const int ca[3] = {x, y, z};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -504,6 +537,16 @@ __device__ __forceinline__ void emit_point(const VolGeom& g, const Src& f, int x
       nrm[0] = nrm[1] = nrm[2] = 0.0f;
     }
   }
+}
+
+// One interpolated point on the edge with lower endpoint (x,y,z) along axis.
+template <class Src>
+__device__ __forceinline__ void emit_point(const VolGeom& g, const Src& f, int x, int y, int z,
+                                           int axis, bool normals, float xyz[3], float nrm[3],
+                                           float& w, int32_t& pa, int32_t& pb) {
+  PointLoads L;
+  point_loads(g, f, x, y, z, axis, normals, L, pa, pb);
+  point_finish(g, x, y, z, axis, normals, L, xyz, nrm, w);
 }
 
 // index i with s[i] <= v < s[i+1], s[0..n]
@@ -656,7 +699,8 @@ __device__ __forceinline__ unsigned long long word_bases(const CountSrc& cs, int
 // FROM_COUNTS: the bases come from the count kernel's per-word prefixes and
 // count-tile prefixes (cs) instead of bv / bt; the tile writes the bases of
 // its words (and, for the last tile, entry nwords) to cs.base_v / base_t,
-// and with emit false only that.
+// and with emit false only that.  has_output skips the check for a tile
+// without output (one round trip).
 template <int TW, class Src, bool FROM_COUNTS = false>
 __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
                                               const uint64_t* __restrict__ b,
@@ -665,14 +709,16 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
                                               const OutPtrs& out, bool dedup, bool normals,
                                               int64_t vol_base_v, int64_t vol_base_t,
                                               uint64_t* s_dyn, const CountSrc& cs = CountSrc{},
-                                              bool emit = true) {
+                                              bool emit = true, bool has_output = false) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int nw = (int)min((int64_t)TW, g.nwords - w_begin);
   if constexpr (FROM_COUNTS) {
-    // count tiles without output: constant bases, nothing else to do
+    // count tiles without output (has_output: known not to be the case):
+    // constant bases, nothing else to do
     const int64_t c0 = w_begin / kCountTile;
     const int64_t c1 = min((w_begin + nw + kCountTile - 1) / kCountTile, cs.n_ctiles);
-    const unsigned long long p0 = ctile_prefix(cs, c0), p1 = ctile_prefix(cs, c1);
+    const unsigned long long p0 = has_output ? 0ull : ctile_prefix(cs, c0);
+    const unsigned long long p1 = has_output ? 1ull : ctile_prefix(cs, c1);
     if (p0 == p1) {
       const uint32_t v0 = (uint32_t)p0, t0 = (uint32_t)(p0 >> 32);
       for (int i = tid; i < nw; i += nt) {
@@ -778,33 +824,45 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
   }
   __syncthreads();
 
-  // 3. vertices (dedup: one per active edge, in id order)
+  // 3. vertices (dedup: one per active edge, in id order).  A thread's
+  //    first vertex is located and its field values loaded here, and
+  //    finished after the triangle rounds: the loads' latency overlaps the
+  //    rounds' LDS work (their barriers wait for LDS operations only).
   const int nV = (int)(tv1 - tv0);
-  if (dedup) {
-    for (int vl = tid; vl < nV; vl += nt) {
-      const int li = find_word(s_v0, nw, tv0 + (uint32_t)vl);
-      const WordPos p = word_pos(g, w_begin + li);
-      uint64_t fx, fy, fz;
-      lds_flags(g, s_b0, s_b1, li, p, fx, fy, fz);
-      int bit, axis;
-      edge_select(fx, fy, fz, (int)(tv0 + (uint32_t)vl - s_v0[li]), bit, axis);
-      float xyz[3], nrm[3], w;
-      int32_t pa, pb;
-      emit_point(g, f, p.q * 64 + bit, p.j, p.k, axis, normals, xyz, nrm, w, pa, pb);
-      const int64_t gv = (int64_t)tv0 + vl;
-      out.vertices[3 * gv + 0] = xyz[0];
-      out.vertices[3 * gv + 1] = xyz[1];
-      out.vertices[3 * gv + 2] = xyz[2];
-      if (normals) {
-        out.normals[3 * gv + 0] = nrm[0];
-        out.normals[3 * gv + 1] = nrm[1];
-        out.normals[3 * gv + 2] = nrm[2];
-      }
-      if (out.edges) out.edges[gv] = ((uint32_t)axis << 30) | (uint32_t)pa;
-      if (out.interp_first) out.interp_first[gv] = (int32_t)pa;
-      if (out.interp_second) out.interp_second[gv] = (int32_t)pb;
-      if (out.interp_weight) out.interp_weight[gv] = w;
+  auto locate = [&](int vl, int& x, int& y, int& z, int& axis) {
+    const int li = find_word(s_v0, nw, tv0 + (uint32_t)vl);
+    const WordPos p = word_pos(g, w_begin + li);
+    uint64_t fx, fy, fz;
+    lds_flags(g, s_b0, s_b1, li, p, fx, fy, fz);
+    int bit;
+    edge_select(fx, fy, fz, (int)(tv0 + (uint32_t)vl - s_v0[li]), bit, axis);
+    x = p.q * 64 + bit;
+    y = p.j;
+    z = p.k;
+  };
+  auto store_vertex = [&](int vl, int axis, const float xyz[3], const float nrm[3], float w,
+                          int32_t pa, int32_t pb) {
+    const int64_t gv = (int64_t)tv0 + vl;
+    out.vertices[3 * gv + 0] = xyz[0];
+    out.vertices[3 * gv + 1] = xyz[1];
+    out.vertices[3 * gv + 2] = xyz[2];
+    if (normals) {
+      out.normals[3 * gv + 0] = nrm[0];
+      out.normals[3 * gv + 1] = nrm[1];
+      out.normals[3 * gv + 2] = nrm[2];
     }
+    if (out.edges) out.edges[gv] = ((uint32_t)axis << 30) | (uint32_t)pa;
+    if (out.interp_first) out.interp_first[gv] = (int32_t)pa;
+    if (out.interp_second) out.interp_second[gv] = (int32_t)pb;
+    if (out.interp_weight) out.interp_weight[gv] = w;
+  };
+  const bool first = dedup && tid < nV;
+  int fx0 = 0, fy0 = 0, fz0 = 0, fax = 0;
+  int32_t fpa = 0, fpb = 0;
+  PointLoads fl;
+  if (first) {
+    locate(tid, fx0, fy0, fz0, fax);
+    point_loads(g, f, fx0, fy0, fz0, fax, normals, fl, fpa, fpb);
   }
 
   // 4. triangles
@@ -878,6 +936,22 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
     }
     carry += round_tris;
     __syncthreads();  // s_rfirst / s_rinfo / s_tcell reused by the next round
+  }
+
+  if (first) {
+    float xyz[3], nrm[3], w;
+    point_finish(g, fx0, fy0, fz0, fax, normals, fl, xyz, nrm, w);
+    store_vertex(tid, fax, xyz, nrm, w, fpa, fpb);
+  }
+  if (dedup) {
+    for (int vl = tid + nt; vl < nV; vl += nt) {
+      int x, y, z, axis;
+      locate(vl, x, y, z, axis);
+      float xyz[3], nrm[3], w;
+      int32_t pa, pb;
+      emit_point(g, f, x, y, z, axis, normals, xyz, nrm, w, pa, pb);
+      store_vertex(vl, axis, xyz, nrm, w, pa, pb);
+    }
   }
 }
 
