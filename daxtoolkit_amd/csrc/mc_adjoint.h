@@ -136,7 +136,7 @@ __device__ __forceinline__ void vertex_adjoint(const VolGeom& g, const float* __
 // segment dz holds words [w0 + dz*plane - 2*wpr - 1, w_end + dz*plane +
 // 2*wpr + 1).  Every lower endpoint of an edge a tile point reads lies at
 // |dq| <= 1, -2 <= dy, dz <= 1 from its own word, inside its segment.
-constexpr int kGatherTile = 256;     // row words per tile
+constexpr int kGatherTile = kGatherTileWords;  // row words per tile
 constexpr int kGatherThreads = 512;  // threads per tile
 // Endpoint records a tile keeps (4 per thread, held in registers until the
 // flags are no longer needed, then written over them).

@@ -22,18 +22,21 @@ __global__ __launch_bounds__(256) void k_vertex_adjoint(VolGeom g, const float* 
   vertex_adjoint(g, field, edges, dldx, dldn, u, out);
 }
 
-// One tile of kGatherTile row words per block, in tile order (round-robin
-// over the XCDs: a contiguous range per XCD leaves the XCDs that hold dense
-// surface working long after the others).  A tile whose endpoint records do
-// not fit goes to the list (without records, list == nullptr, every point
-// receives 0).
+// One tile of kGatherTile row words per block, in the forward scan's
+// dispatch order (tiles with active edges of their own first, so the long
+// tiles start early and the quiet ones fill the tail) when it holds one,
+// else in tile order.  A tile whose endpoint records do not fit goes to the
+// list (without records, list == nullptr, every point receives 0).
 __global__ __launch_bounds__(kGatherThreads) void k_point_gather(
     VolGeom g, const uint64_t* __restrict__ bits, const uint32_t* __restrict__ base_v,
     const VertexAdj* __restrict__ adj, const int64_t* __restrict__ counts, int64_t n_points,
-    float* __restrict__ dldf, uint32_t* __restrict__ list) {
+    float* __restrict__ dldf, uint32_t* __restrict__ list, const uint32_t* __restrict__ order) {
   __shared__ GatherShared sh;
   extern __shared__ __align__(16) uint64_t s_dyn[];
-  const int64_t tile = blockIdx.x;
+  const uint32_t tag = order[0], ord = order[1 + blockIdx.x] & 0x7fffffffu;
+  const int64_t tile =
+      (tag == (uint32_t)kGatherTile && (int64_t)ord < (int64_t)gridDim.x) ? (int64_t)ord
+                                                                            : (int64_t)blockIdx.x;
   const int64_t n_adj = counts ? min(counts[1], n_points) : n_points;
   const int64_t w0 = tile * kGatherTile, w_end = min(w0 + kGatherTile, g.nwords);
   const uint32_t n_end = gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj,
@@ -146,7 +149,8 @@ int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz
   }
   k_point_gather<<<dim3((unsigned)ceil_div(g.nwords, kGatherTile)), dim3(kGatherThreads),
                    gather_lds_bytes(g.wpr), s>>>(g, bits, bv, (const VertexAdj*)d_scratch,
-                                                 d_counts, n_points, d_dl_dfield, list);
+                                                 d_counts, n_points, d_dl_dfield, list,
+                                                 (const uint32_t*)(ws + L.off_gorder));
   DAXHIP_LAUNCH_CHECK("k_point_gather");
   if (n_points > 0) {
     k_point_gather_spans<<<dim3((unsigned)gather_span_blocks()), dim3(kGatherThreads),

@@ -26,6 +26,7 @@ constexpr int kCountTile = 128;      // row words per count tile: its prefixes f
 constexpr int kScanThreads = 256;    // threads of the count/scan and generate blocks
 constexpr int kWordsPerThread = kTileWords / kScanThreads;
 constexpr int kMaxWpr = 256;         // nx <= 16384
+constexpr int kGatherTileWords = 256;  // row words per tile of the adjoint's gather
 
 // Geometry of one volume in the (possibly batched) word space.
 // Division by an invariant d >= 1 for numerators below 2^31: q = (hi32(n*m)
@@ -274,12 +275,14 @@ struct Ctrl {
 // both below 2^16); off_tsum every count tile's exclusive prefix within its
 // count block (v | t << 32); off_bsum the count blocks' sums and off_tpre
 // their exclusive prefixes, the totals after the last.  A word's bases are
-// block prefix + tile prefix + local prefix.  off_cw: every count tile's
-// output size (vertices + triangles); off_order: the generate tile size the
-// order was built for (0: none), then the generate tiles in dispatch order.
+// block prefix + tile prefix + local prefix.  off_gcls / off_tcls: the
+// dispatch class of every generate / gather tile (a byte each, from the
+// count kernel); off_order: the generate tile size the order was built for
+// (0: none), then the generate tiles in dispatch order; off_gorder: the
+// same for the adjoint's gather tiles.
 struct Workspace {
   size_t off_bits, off_base_v, off_base_t, off_status, off_ctrl, off_counts, off_tsum, off_bsum,
-      off_tpre, off_cw, off_order, bytes;
+      off_tpre, off_gcls, off_tcls, off_order, off_gorder, bytes;
   int64_t nwords, ntiles, n_ctiles;
 };
 
@@ -307,10 +310,14 @@ inline Workspace workspace_layout(int64_t nwords) {
   off = align_up(off + (size_t)(w.n_ctiles + 1) * 8, 256);
   w.off_tpre = off;  // per count block: exclusive prefixes, then the totals
   off = align_up(off + (size_t)(w.n_ctiles + 1) * 8, 256);
-  w.off_cw = off;  // per count tile: vertices + triangles
-  off = align_up(off + (size_t)(w.n_ctiles + 1) * 4, 256);
+  w.off_gcls = off;  // per generate tile (at least kCountTile words): dispatch class
+  off = align_up(off + (size_t)w.n_ctiles + 16, 256);
+  w.off_tcls = off;  // per gather tile: dispatch class
+  off = align_up(off + (size_t)ceil_div(nwords, kGatherTileWords) + 16, 256);
   w.off_order = off;  // generate tiles, heaviest first: the tile size, then the tiles
   off = align_up(off + (size_t)(w.n_ctiles + 2) * 4, 256);
+  w.off_gorder = off;  // the adjoint's gather tiles, those with output first
+  off = align_up(off + (size_t)(ceil_div(nwords, kGatherTileWords) + 2) * 4, 256);
   w.bytes = off;
   return w;
 }

@@ -66,14 +66,25 @@ __global__ __launch_bounds__(256) void k_classify_v4(VolGeom g, const float* __r
 // 3. the bases of every word (block + tile + local prefix): the generate
 //    kernel stores them for its words; k_bases stores them alone, for calls
 //    that classify without generating.
+// Dispatch classes of the generate and gather tiles (a byte per tile):
+// generate tiles 0 heavy (at least 5/4 outputs, vertices + triangles, per
+// word), 1 other with output, 2 none; gather tiles 0 with output of their
+// own, 2 none.  tw = 0: no generate classes (the count blocks do not cover
+// whole generate tiles).
+struct TileClasses {
+  uint8_t* gcls;
+  uint8_t* tcls;
+  int tw;
+};
+
 template <int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_count(VolGeom g, const uint64_t* __restrict__ bits,
                                                       uint32_t* __restrict__ lpre,
                                                       unsigned long long* __restrict__ tloc,
                                                       unsigned long long* __restrict__ bsum,
-                                                      uint32_t* __restrict__ cw,
-                                                      int64_t n_ctiles) {
-  __shared__ unsigned long long s_ts[2 * WAVES];
+                                                      TileClasses tc, int64_t n_ctiles) {
+  constexpr int kBlockTiles = 2 * WAVES;  // count tiles per block
+  __shared__ unsigned long long s_ts[kBlockTiles];
   const int wv = threadIdx.x >> 6;
   const int64_t pair = (int64_t)blockIdx.x * WAVES + wv;
   unsigned long long s0 = 0ull, s1 = 0ull;
@@ -81,94 +92,150 @@ __global__ __launch_bounds__(64 * WAVES) void k_count(VolGeom g, const uint64_t*
   if ((threadIdx.x & 63) == 0) {
     s_ts[2 * wv] = s0;
     s_ts[2 * wv + 1] = s1;
-    if (2 * pair < n_ctiles) cw[2 * pair] = (uint32_t)s0 + (uint32_t)(s0 >> 32);
-    if (2 * pair + 1 < n_ctiles) cw[2 * pair + 1] = (uint32_t)s1 + (uint32_t)(s1 >> 32);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long run = 0ull;
-    for (int i = 0; i < 2 * WAVES; ++i) {
-      const int64_t tile = (int64_t)blockIdx.x * 2 * WAVES + i;
+    for (int i = 0; i < kBlockTiles; ++i) {
+      const int64_t tile = (int64_t)blockIdx.x * kBlockTiles + i;
       if (tile < n_ctiles) tloc[tile] = run;
       run = add_vt(run, s_ts[i]);
     }
     bsum[blockIdx.x] = run;
+  } else if (threadIdx.x < 64) {  // the block's generate and gather tiles
+    const int lane = threadIdx.x;
+    auto weight = [&](int first, int n) -> uint32_t {  // outputs of count tiles [first, +n)
+      uint32_t w = 0u;
+      for (int i = 0; i < n; ++i)
+        if ((int64_t)blockIdx.x * kBlockTiles + first + i < n_ctiles)
+          w += (uint32_t)s_ts[first + i] + (uint32_t)(s_ts[first + i] >> 32);
+      return w;
+    };
+    constexpr int gct = kGatherTileWords / kCountTile;
+    const int l = lane - 1;
+    if (l >= 0 && l < (kBlockTiles + gct - 1) / gct) {
+      const int64_t c0 = (int64_t)blockIdx.x * kBlockTiles + l * gct;
+      if (c0 < n_ctiles) tc.tcls[c0 / gct] = weight(l * gct, gct) == 0u ? 2 : 0;
+    }
+    const int cpt = tc.tw / kCountTile;
+    if (tc.tw > 0 && l >= 0 && l < kBlockTiles / cpt) {
+      const int64_t c0 = (int64_t)blockIdx.x * kBlockTiles + l * cpt;
+      if (c0 < n_ctiles) {
+        const uint32_t w = weight(l * cpt, cpt);
+        tc.gcls[c0 / cpt] = w == 0u ? 2 : (4ull * w >= 5ull * (uint32_t)tc.tw ? 0 : 1);
+      }
+    }
   }
 }
 
-// Dispatch order of the generate tiles, built by the scan block: first the
-// heavy tiles (output size, vertices + triangles, at least 1.5 times the
-// mean), then the other tiles with output, then the empty ones; within a
-// class by wave, round and lane of the block, i.e. by tile index.  The
-// generate grid dispatches its blocks roughly in index order as slots free
-// up, so the longest tiles start first and the short ones fill the tail (a
-// coarse longest-processing-time-first schedule).  Every tile writes at its
-// bases, so the order changes no output.
-constexpr int kGenClasses = 3;
-constexpr int kOrderMaxTiles = 32768;  // generate tiles the order covers (a byte of LDS each)
-constexpr int kOrderRegRounds = 8;     // rounds of tile weights loaded before the scan
+// Dispatch orders, written by the scan block from the count kernel's tile
+// classes: generate tiles heavy first, then the other tiles with output,
+// then the empty ones; the adjoint's gather tiles (kGatherTileWords words)
+// with output of their own first, then the others.  Within a class, tiles
+// keep index order.  A grid dispatches its blocks roughly in index order as
+// slots free up, so the longest tiles start first and the short ones fill
+// the tail (a coarse longest-processing-time-first schedule).  Every tile
+// writes at fixed offsets, so an order changes no output.  Each thread of
+// the scan block takes a contiguous chunk of tiles; two block scans of its
+// class counts give its slots.
+constexpr int kOrderMaxTiles = 32768;  // tiles an order covers: chunks of at most 32
+constexpr uint32_t kOrderHasOutput = 0x80000000u;  // order entry flag: class 0 or 1
 struct GenOrder {
-  const uint32_t* __restrict__ cw;  // per count tile: vertices + triangles
-  int64_t n_ctiles, nwords;
-  int tw;           // generate tile words: 128, 256, 512 or 1024
-  uint32_t* order;  // [0]: tw (0: no order), [1 + i]: the i-th tile to dispatch
+  const uint8_t* __restrict__ gcls;  // classes of the generate tiles
+  const uint8_t* __restrict__ tcls;  // classes of the gather tiles
+  int64_t nwords;
+  int tw;            // generate tile words: 128, 256, 512 or 1024
+  bool gen_classes;  // gcls holds the classes
+  uint32_t* order;   // [0]: tw (0: no order), [1 + i]: the i-th tile to dispatch
+  uint32_t* gorder;  // [0]: kGatherTileWords (0: no order), then the gather tiles
 };
-constexpr uint32_t kOrderHasOutput = 0x80000000u;  // order entry flag: the tile has output
 
-// Output size of generate tile t (t * cpt < n_ctiles): its count tiles'
-// sizes, one vector load when cpt is 2, 4 or 8 (the region is 256-byte
-// aligned, so the loads are aligned).
-__device__ __forceinline__ uint32_t gen_tile_weight(const GenOrder& go, int cpt, uint32_t t) {
-  const uint32_t c0 = t * (uint32_t)cpt;
-  if (c0 + (uint32_t)cpt > (uint32_t)go.n_ctiles) {  // the last, partial tile
-    uint32_t w = 0u;
-    for (uint32_t c = c0; c < (uint32_t)go.n_ctiles; ++c) w += go.cw[c];
-    return w;
+// Classes of tiles [t0, t0 + K) (those below ntl), two bits each.
+__device__ __forceinline__ uint64_t load_classes(const uint8_t* __restrict__ cls, uint32_t t0,
+                                                 uint32_t K, uint32_t ntl) {
+  uint64_t r = 0ull;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if ((uint32_t)k < K && t0 + k < ntl) r |= (uint64_t)(cls[t0 + k] & 3u) << (2 * k);
+  for (uint32_t k = 8; k < K && t0 + k < ntl; ++k) r |= (uint64_t)(cls[t0 + k] & 3u) << (2 * k);
+  return r;
+}
+
+constexpr int kTsumThreads = 1024;
+constexpr int kTsumWaves = kTsumThreads / 64;
+
+// Exclusive block scans of two counts per thread (all threads call it).
+__device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& ea, uint32_t& eb,
+                                            uint32_t& ta, uint32_t& tb, uint32_t* s_a,
+                                            uint32_t* s_b) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t ia = wave_incl_scan(a), ib = wave_incl_scan(b);
+  if (lane == 63) {
+    s_a[wv] = ia;
+    s_b[wv] = ib;
   }
-  switch (cpt) {
-    case 1: return go.cw[c0];
-    case 2: {
-      const uint2 v = *reinterpret_cast<const uint2*>(go.cw + c0);
-      return v.x + v.y;
+  __syncthreads();
+  uint32_t oa = 0u, ob = 0u;
+  ta = tb = 0u;
+#pragma unroll
+  for (int q = 0; q < kTsumWaves; ++q) {
+    if (q < wv) {
+      oa += s_a[q];
+      ob += s_b[q];
     }
-    case 4: {
-      const uint4 v = *reinterpret_cast<const uint4*>(go.cw + c0);
-      return (v.x + v.y) + (v.z + v.w);
-    }
-    default: {
-      const uint4 v = *reinterpret_cast<const uint4*>(go.cw + c0);
-      const uint4 u = *reinterpret_cast<const uint4*>(go.cw + c0 + 4);
-      return ((v.x + v.y) + (v.z + v.w)) + ((u.x + u.y) + (u.z + u.w));
-    }
+    ta += s_a[q];
+    tb += s_b[q];
   }
+  __syncthreads();  // s_a / s_b reusable
+  ea = oa + ia - a;
+  eb = ob + ib - b;
+}
+
+// order[1 ..] = tiles [0, ntl) of class 0, then 1, then 2, each by index,
+// flagged kOrderHasOutput in classes 0 and 1; order[0] = tag.  The thread
+// owns tiles [tid * K, tid * K + K); cls holds their classes, two bits each.
+__device__ __forceinline__ void write_order(uint64_t cls, uint32_t K, uint32_t ntl,
+                                            uint32_t* order, uint32_t tag, uint32_t* s_a,
+                                            uint32_t* s_b) {
+  const uint32_t t0 = (uint32_t)threadIdx.x * K;
+  const uint32_t n = t0 < ntl ? min(K, ntl - t0) : 0u;
+  uint32_t c0 = 0u, c1 = 0u;
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t c = (uint32_t)(cls >> (2 * k)) & 3u;
+    c0 += c == 0u;
+    c1 += c == 1u;
+  }
+  uint32_t e0, e1, tot0, tot1;
+  block_scan2(c0, c1, e0, e1, tot0, tot1, s_a, s_b);
+  uint32_t s0 = 1u + e0, s1 = 1u + tot0 + e1, s2 = 1u + tot0 + tot1 + (t0 - e0 - e1);
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t c = (uint32_t)(cls >> (2 * k)) & 3u;
+    const uint32_t slot = c == 0u ? s0++ : (c == 1u ? s1++ : s2++);
+    order[slot] = (t0 + k) | (c < 2u ? kOrderHasOutput : 0u);
+  }
+  if (threadIdx.x == 0) order[0] = tag;
 }
 
 // One block scans the n count-block sums (v | t << 32, each below 2^32)
 // 1024 at a time: a coalesced load, wave scans of the halves, the block's
-// wave sums in 64 bits, 64-bit carries between rounds.  The same block
-// builds the generate tiles' dispatch order: the weights of the first
-// kOrderRegRounds rounds of tiles are loaded before the scan (the latencies
-// overlap), the classes follow from the totals, per-wave class counts come
-// from ballots, and each entry's slot from its wave's ballot rank.
-constexpr int kTsumThreads = 1024;
+// wave sums in 64 bits, 64-bit carries between rounds.  The same block then
+// writes the two dispatch orders.
 __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long long* __restrict__ bsum,
                                                              unsigned long long* __restrict__ bpre,
                                                              int64_t n, Ctrl* __restrict__ ctrl,
                                                              GenOrder go) {
-  constexpr int kWaves = kTsumThreads / 64;
+  constexpr int kWaves = kTsumWaves;
   __shared__ uint32_t s_v[kWaves], s_t[kWaves];
-  __shared__ uint32_t s_base[kGenClasses * kWaves];
-  __shared__ uint8_t s_cls[kOrderMaxTiles];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const uint32_t ng = (uint32_t)ceil_div(go.nwords, (int64_t)go.tw);
-  const bool ordered = ng <= (uint32_t)kOrderMaxTiles;  // uniform
-  const int cpt = go.tw / kCountTile;
-  uint32_t wreg[kOrderRegRounds];
-#pragma unroll
-  for (int r = 0; r < kOrderRegRounds; ++r) {
-    const uint32_t t = (uint32_t)tid + (uint32_t)r * kTsumThreads;
-    wreg[r] = (ordered && t < ng) ? gen_tile_weight(go, cpt, t) : 0u;
-  }
+  const uint32_t ngt = (uint32_t)ceil_div(go.nwords, (int64_t)kGatherTileWords);
+  const bool ordered = go.gen_classes && ng <= (uint32_t)kOrderMaxTiles;  // uniform
+  const bool gordered = ngt <= (uint32_t)kOrderMaxTiles;                 // uniform
+  const uint32_t Kg = (ng + kTsumThreads - 1) / kTsumThreads;
+  const uint32_t Kt = (ngt + kTsumThreads - 1) / kTsumThreads;
+  // loaded before the scan: the latencies overlap
+  const uint64_t gcls = ordered ? load_classes(go.gcls, (uint32_t)tid * Kg, Kg, ng) : 0ull;
+  const uint64_t tcls = gordered ? load_classes(go.tcls, (uint32_t)tid * Kt, Kt, ngt) : 0ull;
 
   unsigned long long cv = 0, ct = 0;  // 64-bit carries
   for (int64_t e0 = 0; e0 < n; e0 += kTsumThreads) {
@@ -204,63 +271,15 @@ __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long
     ctrl->total_v = cv;
     ctrl->total_t = ct;
   }
-  if (!ordered) {
-    if (tid == 0) go.order[0] = 0u;
-    return;
-  }
 
-  // classes: heavy when 2 * w * ng >= 3 * (total vertices + triangles)
-  const unsigned long long tot3 = 3ull * (cv + ct);  // every thread holds the totals
-  auto cls_of = [&](uint32_t w) -> int {
-    return w == 0u ? 2 : (2ull * w * ng >= tot3 ? 0 : 1);
-  };
-  const uint32_t rounds = (ng + kTsumThreads - 1) / kTsumThreads;  // uniform
-  for (uint32_t r = 0; r < rounds; ++r) {
-    const uint32_t t = (uint32_t)tid + r * kTsumThreads;
-    uint32_t w = 0u;
-#pragma unroll
-    for (int q = 0; q < kOrderRegRounds; ++q)
-      if (q == (int)r) w = wreg[q];
-    if (r >= (uint32_t)kOrderRegRounds && t < ng) w = gen_tile_weight(go, cpt, t);
-    if (t < ng) s_cls[t] = (uint8_t)cls_of(w);
-  }
-  __syncthreads();
-  uint32_t cnt[kGenClasses] = {0u, 0u, 0u};  // the wave's entries of each class (uniform)
-  for (uint32_t r = 0; r < rounds; ++r) {
-    const uint32_t t = (uint32_t)tid + r * kTsumThreads;
-    const int c = t < ng ? (int)s_cls[t] : -1;
-#pragma unroll
-    for (int k = 0; k < kGenClasses; ++k) cnt[k] += (uint32_t)__popcll(__ballot(c == k));
-  }
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < kGenClasses; ++k) s_base[k * kWaves + wv] = cnt[k];
-  }
-  __syncthreads();
-  if (wv == 0) {  // exclusive scan of the kGenClasses * kWaves <= 64 counts
-    const uint32_t x = lane < kGenClasses * kWaves ? s_base[lane] : 0u;
-    const uint32_t inc = wave_incl_scan(x);
-    if (lane < kGenClasses * kWaves) s_base[lane] = 1u + inc - x;  // entry 0: the tile size
-    if (lane == 0) go.order[0] = (uint32_t)go.tw;
-  }
-  __syncthreads();
-  uint32_t run[kGenClasses] = {0u, 0u, 0u};  // the wave's entries of each class placed so far
-  for (uint32_t r = 0; r < rounds; ++r) {
-    const uint32_t t = (uint32_t)tid + r * kTsumThreads;
-    const int c = t < ng ? (int)s_cls[t] : -1;
-    uint64_t mk[kGenClasses];
-#pragma unroll
-    for (int k = 0; k < kGenClasses; ++k) mk[k] = __ballot(c == k);
-    if (t < ng) {
-      const uint64_t m = c == 0 ? mk[0] : (c == 1 ? mk[1] : mk[2]);
-      const uint32_t before = c == 0 ? run[0] : (c == 1 ? run[1] : run[2]);
-      const uint32_t rank =
-          __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-      go.order[s_base[c * kWaves + wv] + before + rank] = t | (c < 2 ? kOrderHasOutput : 0u);
-    }
-#pragma unroll
-    for (int k = 0; k < kGenClasses; ++k) run[k] += (uint32_t)__popcll(mk[k]);
-  }
+  if (ordered)
+    write_order(gcls, Kg, ng, go.order, (uint32_t)go.tw, s_v, s_t);
+  else if (tid == 0)
+    go.order[0] = 0u;
+  if (gordered)
+    write_order(tcls, Kt, ngt, go.gorder, (uint32_t)kGatherTileWords, s_v, s_t);
+  else if (tid == 0)
+    go.gorder[0] = 0u;
 }
 
 // Bases of every row word (and entry nwords): tile prefix + local prefix.
@@ -682,22 +701,27 @@ int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu
   unsigned long long* tloc = (unsigned long long*)(ws + L.off_tsum);
   unsigned long long* bsum = (unsigned long long*)(ws + L.off_bsum);
   unsigned long long* bpre = (unsigned long long*)(ws + L.off_tpre);
-  uint32_t* cw = (uint32_t*)(ws + L.off_cw);
   const int64_t n = L.n_ctiles;
   const int waves = tu.scan_sub;
+  // generate classes when a count block covers whole generate tiles
+  const int tw = gen_tile_words(tu);
+  const int block_words = 2 * (waves == 1 || waves == 2 || waves == 8 ? waves : 4) * kCountTile;
+  const TileClasses tc{(uint8_t*)(ws + L.off_gcls), (uint8_t*)(ws + L.off_tcls),
+                       tw <= block_words ? tw : 0};
   const int64_t nblocks = ceil_div(n, 2 * waves);
   if (nblocks > 0) {
     const dim3 grid((unsigned)nblocks), block(64 * waves);
     switch (waves) {
-      case 1: k_count<1><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, cw, n); break;
-      case 2: k_count<2><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, cw, n); break;
-      case 8: k_count<8><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, cw, n); break;
-      default: k_count<4><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, cw, n); break;
+      case 1: k_count<1><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, tc, n); break;
+      case 2: k_count<2><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, tc, n); break;
+      case 8: k_count<8><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, tc, n); break;
+      default: k_count<4><<<grid, block, 0, s>>>(g, bits, lpre, tloc, bsum, tc, n); break;
     }
     DAXHIP_LAUNCH_CHECK("k_count");
   }
   // with the dispatch order of the generate tiles
-  const GenOrder go{cw, L.n_ctiles, g.nwords, gen_tile_words(tu), (uint32_t*)(ws + L.off_order)};
+  const GenOrder go{tc.gcls, tc.tcls, g.nwords, tw, tc.tw > 0, (uint32_t*)(ws + L.off_order),
+                    (uint32_t*)(ws + L.off_gorder)};
   k_scan_tsums<<<1, dim3(kTsumThreads), 0, s>>>(bsum, bpre, nblocks, (Ctrl*)(ws + L.off_ctrl), go);
   DAXHIP_LAUNCH_CHECK("k_scan_tsums");
   if (bases) {
