@@ -138,7 +138,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_count(VolGeom g, const uint64_t*
 // writes at fixed offsets, so an order changes no output.  Each thread of
 // the scan block takes a contiguous chunk of tiles; two block scans of its
 // class counts give its slots.
-constexpr int kOrderMaxTiles = 32768;  // tiles an order covers: chunks of at most 32
 constexpr uint32_t kOrderHasOutput = 0x80000000u;  // order entry flag: class 0 or 1
 struct GenOrder {
   const uint8_t* __restrict__ gcls;  // classes of the generate tiles
@@ -147,17 +146,39 @@ struct GenOrder {
   int tw;            // generate tile words: 128, 256, 512 or 1024
   bool gen_classes;  // gcls holds the classes
   uint32_t* order;   // [0]: tw (0: no order), [1 + i]: the i-th tile to dispatch
-  uint32_t* gorder;  // [0]: kGatherTileWords (0: no order), then the gather tiles
+  uint32_t* gorder;  // [0]: kGatherTileWords, then the gather tiles
 };
 
-// Classes of tiles [t0, t0 + K) (those below ntl), two bits each.
+// A thread of the scan block owns a chunk of K consecutive tiles.  Up to
+// kOrderRegTiles of them have their classes loaded before the scan, two bits
+// each in a register (the latencies overlap the scan's): byte loads for K
+// up to 8, 4-byte loads for K a multiple of 4 up to 32.  Larger chunks (K a
+// multiple of 16, volumes of more than 2^28 points) read their class bytes
+// in 16-byte vectors while the order is written.  The class arrays end in
+// 16 bytes of padding.
+constexpr int kOrderRegTiles = 32;
+__device__ __forceinline__ uint32_t order_chunk(uint32_t ntl, uint32_t threads) {
+  const uint32_t k = (ntl + threads - 1) / threads;
+  return k <= 8u ? k : (k <= (uint32_t)kOrderRegTiles ? (k + 3u) / 4u * 4u : (k + 15u) / 16u * 16u);
+}
 __device__ __forceinline__ uint64_t load_classes(const uint8_t* __restrict__ cls, uint32_t t0,
                                                  uint32_t K, uint32_t ntl) {
   uint64_t r = 0ull;
+  if (K <= 8u) {
 #pragma unroll
-  for (int k = 0; k < 8; ++k)
-    if ((uint32_t)k < K && t0 + k < ntl) r |= (uint64_t)(cls[t0 + k] & 3u) << (2 * k);
-  for (uint32_t k = 8; k < K && t0 + k < ntl; ++k) r |= (uint64_t)(cls[t0 + k] & 3u) << (2 * k);
+    for (int k = 0; k < 8; ++k)
+      if ((uint32_t)k < K && t0 + k < ntl) r |= (uint64_t)(cls[t0 + k] & 3u) << (2 * k);
+  } else if (K <= (uint32_t)kOrderRegTiles) {
+#pragma unroll
+    for (int v = 0; v < kOrderRegTiles / 4; ++v) {
+      const uint32_t t = t0 + 4u * v;
+      if (4u * v < K && t < ntl) {
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(cls + t);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r |= (uint64_t)((w >> (8 * j)) & 3u) << (2 * (4 * v + j));
+      }
+    }
+  }
   return r;
 }
 
@@ -191,28 +212,45 @@ __device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& ea
   eb = ob + ib - b;
 }
 
+// Calls fn(k, class) for the thread's tiles k = 0 .. n-1 in order: from reg
+// (K <= kOrderRegTiles) or from the class bytes at cls + t0.
+template <bool LARGE, class Fn>
+__device__ __forceinline__ void for_classes(uint64_t reg, const uint8_t* __restrict__ cls,
+                                            uint32_t t0, uint32_t K, uint32_t n, Fn&& fn) {
+  if (!LARGE || K <= (uint32_t)kOrderRegTiles) {
+    for (uint32_t k = 0; k < n; ++k) fn(k, (uint32_t)(reg >> (2 * k)) & 3u);
+    return;
+  }
+  for (uint32_t v = 0; v < n; v += 16u) {
+    const uint4 q = *reinterpret_cast<const uint4*>(cls + t0 + v);
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (v + j < n) fn(v + j, (w[j >> 2] >> (8 * (j & 3))) & 3u);
+  }
+}
+
 // order[1 ..] = tiles [0, ntl) of class 0, then 1, then 2, each by index,
 // flagged kOrderHasOutput in classes 0 and 1; order[0] = tag.  The thread
-// owns tiles [tid * K, tid * K + K); cls holds their classes, two bits each.
-__device__ __forceinline__ void write_order(uint64_t cls, uint32_t K, uint32_t ntl,
-                                            uint32_t* order, uint32_t tag, uint32_t* s_a,
-                                            uint32_t* s_b) {
+// owns tiles [tid * K, tid * K + K).
+template <bool LARGE>
+__device__ __forceinline__ void write_order(uint64_t reg, const uint8_t* __restrict__ cls,
+                                            uint32_t K, uint32_t ntl, uint32_t* order,
+                                            uint32_t tag, uint32_t* s_a, uint32_t* s_b) {
   const uint32_t t0 = (uint32_t)threadIdx.x * K;
   const uint32_t n = t0 < ntl ? min(K, ntl - t0) : 0u;
   uint32_t c0 = 0u, c1 = 0u;
-  for (uint32_t k = 0; k < n; ++k) {
-    const uint32_t c = (uint32_t)(cls >> (2 * k)) & 3u;
+  for_classes<LARGE>(reg, cls, t0, K, n, [&](uint32_t, uint32_t c) {
     c0 += c == 0u;
     c1 += c == 1u;
-  }
+  });
   uint32_t e0, e1, tot0, tot1;
   block_scan2(c0, c1, e0, e1, tot0, tot1, s_a, s_b);
   uint32_t s0 = 1u + e0, s1 = 1u + tot0 + e1, s2 = 1u + tot0 + tot1 + (t0 - e0 - e1);
-  for (uint32_t k = 0; k < n; ++k) {
-    const uint32_t c = (uint32_t)(cls >> (2 * k)) & 3u;
+  for_classes<LARGE>(reg, cls, t0, K, n, [&](uint32_t k, uint32_t c) {
     const uint32_t slot = c == 0u ? s0++ : (c == 1u ? s1++ : s2++);
     order[slot] = (t0 + k) | (c < 2u ? kOrderHasOutput : 0u);
-  }
+  });
   if (threadIdx.x == 0) order[0] = tag;
 }
 
@@ -220,6 +258,9 @@ __device__ __forceinline__ void write_order(uint64_t cls, uint32_t K, uint32_t n
 // 1024 at a time: a coalesced load, wave scans of the halves, the block's
 // wave sums in 64 bits, 64-bit carries between rounds.  The same block then
 // writes the two dispatch orders.
+// LARGE: some thread's chunk exceeds kOrderRegTiles tiles (a separate
+// instance, so the common one stays small).
+template <bool LARGE>
 __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long long* __restrict__ bsum,
                                                              unsigned long long* __restrict__ bpre,
                                                              int64_t n, Ctrl* __restrict__ ctrl,
@@ -229,13 +270,10 @@ __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const uint32_t ng = (uint32_t)ceil_div(go.nwords, (int64_t)go.tw);
   const uint32_t ngt = (uint32_t)ceil_div(go.nwords, (int64_t)kGatherTileWords);
-  const bool ordered = go.gen_classes && ng <= (uint32_t)kOrderMaxTiles;  // uniform
-  const bool gordered = ngt <= (uint32_t)kOrderMaxTiles;                 // uniform
-  const uint32_t Kg = (ng + kTsumThreads - 1) / kTsumThreads;
-  const uint32_t Kt = (ngt + kTsumThreads - 1) / kTsumThreads;
-  // loaded before the scan: the latencies overlap
+  const uint32_t Kg = order_chunk(ng, kTsumThreads), Kt = order_chunk(ngt, kTsumThreads);
+  const bool ordered = go.gen_classes;  // uniform
   const uint64_t gcls = ordered ? load_classes(go.gcls, (uint32_t)tid * Kg, Kg, ng) : 0ull;
-  const uint64_t tcls = gordered ? load_classes(go.tcls, (uint32_t)tid * Kt, Kt, ngt) : 0ull;
+  const uint64_t tcls = load_classes(go.tcls, (uint32_t)tid * Kt, Kt, ngt);
 
   unsigned long long cv = 0, ct = 0;  // 64-bit carries
   for (int64_t e0 = 0; e0 < n; e0 += kTsumThreads) {
@@ -273,13 +311,10 @@ __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long
   }
 
   if (ordered)
-    write_order(gcls, Kg, ng, go.order, (uint32_t)go.tw, s_v, s_t);
+    write_order<LARGE>(gcls, go.gcls, Kg, ng, go.order, (uint32_t)go.tw, s_v, s_t);
   else if (tid == 0)
     go.order[0] = 0u;
-  if (gordered)
-    write_order(tcls, Kt, ngt, go.gorder, (uint32_t)kGatherTileWords, s_v, s_t);
-  else if (tid == 0)
-    go.gorder[0] = 0u;
+  write_order<LARGE>(tcls, go.tcls, Kt, ngt, go.gorder, (uint32_t)kGatherTileWords, s_v, s_t);
 }
 
 // Bases of every row word (and entry nwords): tile prefix + local prefix.
@@ -722,7 +757,11 @@ int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu
   // with the dispatch order of the generate tiles
   const GenOrder go{tc.gcls, tc.tcls, g.nwords, tw, tc.tw > 0, (uint32_t*)(ws + L.off_order),
                     (uint32_t*)(ws + L.off_gorder)};
-  k_scan_tsums<<<1, dim3(kTsumThreads), 0, s>>>(bsum, bpre, nblocks, (Ctrl*)(ws + L.off_ctrl), go);
+  const int64_t most = std::max(ceil_div(g.nwords, (int64_t)tw), ceil_div(g.nwords, (int64_t)kGatherTileWords));
+  if (ceil_div(most, (int64_t)kTsumThreads) > kOrderRegTiles)
+    k_scan_tsums<true><<<1, dim3(kTsumThreads), 0, s>>>(bsum, bpre, nblocks, (Ctrl*)(ws + L.off_ctrl), go);
+  else
+    k_scan_tsums<false><<<1, dim3(kTsumThreads), 0, s>>>(bsum, bpre, nblocks, (Ctrl*)(ws + L.off_ctrl), go);
   DAXHIP_LAUNCH_CHECK("k_scan_tsums");
   if (bases) {
     k_bases<<<dim3((unsigned)ceil_div(g.nwords + 1, 256)), dim3(256), 0, s>>>(
