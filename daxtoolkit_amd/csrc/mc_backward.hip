@@ -6,26 +6,34 @@
 namespace daxhip {
 namespace mc {
 
-// One thread per vertex; block 0 also empties the list of dense tiles.
-__global__ __launch_bounds__(256) void k_vertex_adjoint(VolGeom g, const float* __restrict__ field,
-                                                        const uint32_t* __restrict__ edges,
-                                                        const float* __restrict__ dldx,
-                                                        const float* __restrict__ dldn,
-                                                        const int64_t* __restrict__ counts,
-                                                        int64_t n_points,
-                                                        VertexAdj* __restrict__ out,
-                                                        uint32_t* __restrict__ list) {
+// One thread per vertex; block 0 also empties the list of dense tiles and
+// writes the gather tiles' dispatch order (those with active edges of their
+// own first, from the forward count kernel's classes; the other blocks'
+// records hide its time).
+constexpr int kVertexAdjThreads = 256;
+__global__ __launch_bounds__(kVertexAdjThreads) void k_vertex_adjoint(
+    VolGeom g, const float* __restrict__ field, const uint32_t* __restrict__ edges,
+    const float* __restrict__ dldx, const float* __restrict__ dldn,
+    const int64_t* __restrict__ counts, int64_t n_points, VertexAdj* __restrict__ out,
+    uint32_t* __restrict__ list, const uint8_t* __restrict__ tcls, uint32_t* __restrict__ gorder) {
   const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u == 0) list[0] = 0u;
+  if (blockIdx.x == 0) {  // uniform
+    __shared__ uint32_t s_a[kVertexAdjThreads / 64], s_b[kVertexAdjThreads / 64];
+    const uint32_t ngt = (uint32_t)ceil_div(g.nwords, (int64_t)kGatherTile);
+    const uint32_t K = order_chunk(ngt, kVertexAdjThreads);
+    const uint64_t reg = load_classes(tcls, threadIdx.x * K, K, ngt);
+    write_order<true>(reg, tcls, K, ngt, gorder, (uint32_t)kGatherTile, s_a, s_b);
+  }
   const int64_t U = counts ? min(counts[1], n_points) : n_points;
   if (u >= U) return;
   vertex_adjoint(g, field, edges, dldx, dldn, u, out);
 }
 
-// One tile of kGatherTile row words per block, in the forward scan's
-// dispatch order (tiles with active edges of their own first, so the long
-// tiles start early and the quiet ones fill the tail) when it holds one,
-// else in tile order.  A tile whose endpoint records do not fit goes to the
+// One tile of kGatherTile row words per block, in the dispatch order the
+// vertex-record launch wrote (tiles with active edges of their own first, so
+// the long tiles start early and the quiet ones fill the tail), else (no
+// vertices: the forward left the order cleared) in tile order.  A tile whose endpoint records do not fit goes to the
 // list (without records, list == nullptr, every point receives 0).
 __global__ __launch_bounds__(kGatherThreads) void k_point_gather(
     VolGeom g, const uint64_t* __restrict__ bits, const uint32_t* __restrict__ base_v,
@@ -99,7 +107,7 @@ size_t dax_hip_mc_backward_scratch_bytes(int64_t n_points) {
 
 int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
                         const float* h_origin3, const float* h_spacing3, float iso,
-                        const void* d_workspace, size_t workspace_bytes, const uint32_t* d_edges,
+                        void* d_workspace, size_t workspace_bytes, const uint32_t* d_edges,
                         const int64_t* d_counts, int64_t n_points, const float* d_dl_dvertices,
                         const float* d_dl_dnormals, void* d_scratch, size_t scratch_bytes,
                         float* d_dl_dfield, void* stream) {
@@ -136,15 +144,17 @@ int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz
     return DAX_HIP_BAD_VALUE;
   }
   hipStream_t s = (hipStream_t)stream;
-  const char* ws = (const char*)d_workspace;
+  char* ws = (char*)d_workspace;
   const uint64_t* bits = (const uint64_t*)(ws + L.off_bits);
   const uint32_t* bv = (const uint32_t*)(ws + L.off_base_v);
   uint32_t* list =
       n_points > 0 ? (uint32_t*)((char*)d_scratch + gather_list_offset(n_points)) : nullptr;
   if (n_points > 0) {
-    k_vertex_adjoint<<<dim3((unsigned)ceil_div(n_points, 256)), dim3(256), 0, s>>>(
+    k_vertex_adjoint<<<dim3((unsigned)ceil_div(n_points, kVertexAdjThreads)),
+                       dim3(kVertexAdjThreads), 0, s>>>(
         g, d_field, d_edges, d_dl_dvertices, d_dl_dnormals, d_counts, n_points,
-        (VertexAdj*)d_scratch, list);
+        (VertexAdj*)d_scratch, list, (const uint8_t*)(ws + L.off_tcls),
+        (uint32_t*)(ws + L.off_gorder));
     DAXHIP_LAUNCH_CHECK("k_vertex_adjoint");
   }
   k_point_gather<<<dim3((unsigned)ceil_div(g.nwords, kGatherTile)), dim3(kGatherThreads),

@@ -128,131 +128,27 @@ __global__ __launch_bounds__(64 * WAVES) void k_count(VolGeom g, const uint64_t*
   }
 }
 
-// Dispatch orders, written by the scan block from the count kernel's tile
-// classes: generate tiles heavy first, then the other tiles with output,
-// then the empty ones; the adjoint's gather tiles (kGatherTileWords words)
-// with output of their own first, then the others.  Within a class, tiles
-// keep index order.  A grid dispatches its blocks roughly in index order as
-// slots free up, so the longest tiles start first and the short ones fill
-// the tail (a coarse longest-processing-time-first schedule).  Every tile
-// writes at fixed offsets, so an order changes no output.  Each thread of
-// the scan block takes a contiguous chunk of tiles; two block scans of its
-// class counts give its slots.
-constexpr uint32_t kOrderHasOutput = 0x80000000u;  // order entry flag: class 0 or 1
+// Dispatch order of the generate tiles, written by the scan block from the
+// count kernel's tile classes: heavy tiles first, then the other tiles with
+// output, then the empty ones; within a class, index order.  A grid
+// dispatches its blocks roughly in index order as slots free up, so the
+// longest tiles start first and the short ones fill the tail (a coarse
+// longest-processing-time-first schedule).  Every tile writes at fixed
+// offsets, so an order changes no output.  Each thread of the scan block
+// takes a contiguous chunk of tiles; two block scans of its class counts
+// give its slots.  The adjoint orders its gather tiles the same way, from
+// the count kernel's gather classes.
 struct GenOrder {
   const uint8_t* __restrict__ gcls;  // classes of the generate tiles
-  const uint8_t* __restrict__ tcls;  // classes of the gather tiles
   int64_t nwords;
   int tw;            // generate tile words: 128, 256, 512 or 1024
   bool gen_classes;  // gcls holds the classes
   uint32_t* order;   // [0]: tw (0: no order), [1 + i]: the i-th tile to dispatch
-  uint32_t* gorder;  // [0]: kGatherTileWords, then the gather tiles
+  uint32_t* gorder;  // the gather tiles' order: entry 0 cleared (the adjoint writes it)
 };
-
-// A thread of the scan block owns a chunk of K consecutive tiles.  Up to
-// kOrderRegTiles of them have their classes loaded before the scan, two bits
-// each in a register (the latencies overlap the scan's): byte loads for K
-// up to 8, 4-byte loads for K a multiple of 4 up to 32.  Larger chunks (K a
-// multiple of 16, volumes of more than 2^28 points) read their class bytes
-// in 16-byte vectors while the order is written.  The class arrays end in
-// 16 bytes of padding.
-constexpr int kOrderRegTiles = 32;
-__device__ __forceinline__ uint32_t order_chunk(uint32_t ntl, uint32_t threads) {
-  const uint32_t k = (ntl + threads - 1) / threads;
-  return k <= 8u ? k : (k <= (uint32_t)kOrderRegTiles ? (k + 3u) / 4u * 4u : (k + 15u) / 16u * 16u);
-}
-__device__ __forceinline__ uint64_t load_classes(const uint8_t* __restrict__ cls, uint32_t t0,
-                                                 uint32_t K, uint32_t ntl) {
-  uint64_t r = 0ull;
-  if (K <= 8u) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if ((uint32_t)k < K && t0 + k < ntl) r |= (uint64_t)(cls[t0 + k] & 3u) << (2 * k);
-  } else if (K <= (uint32_t)kOrderRegTiles) {
-#pragma unroll
-    for (int v = 0; v < kOrderRegTiles / 4; ++v) {
-      const uint32_t t = t0 + 4u * v;
-      if (4u * v < K && t < ntl) {
-        const uint32_t w = *reinterpret_cast<const uint32_t*>(cls + t);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) r |= (uint64_t)((w >> (8 * j)) & 3u) << (2 * (4 * v + j));
-      }
-    }
-  }
-  return r;
-}
 
 constexpr int kTsumThreads = 1024;
 constexpr int kTsumWaves = kTsumThreads / 64;
-
-// Exclusive block scans of two counts per thread (all threads call it).
-__device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& ea, uint32_t& eb,
-                                            uint32_t& ta, uint32_t& tb, uint32_t* s_a,
-                                            uint32_t* s_b) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const uint32_t ia = wave_incl_scan(a), ib = wave_incl_scan(b);
-  if (lane == 63) {
-    s_a[wv] = ia;
-    s_b[wv] = ib;
-  }
-  __syncthreads();
-  uint32_t oa = 0u, ob = 0u;
-  ta = tb = 0u;
-#pragma unroll
-  for (int q = 0; q < kTsumWaves; ++q) {
-    if (q < wv) {
-      oa += s_a[q];
-      ob += s_b[q];
-    }
-    ta += s_a[q];
-    tb += s_b[q];
-  }
-  __syncthreads();  // s_a / s_b reusable
-  ea = oa + ia - a;
-  eb = ob + ib - b;
-}
-
-// Calls fn(k, class) for the thread's tiles k = 0 .. n-1 in order: from reg
-// (K <= kOrderRegTiles) or from the class bytes at cls + t0.
-template <bool LARGE, class Fn>
-__device__ __forceinline__ void for_classes(uint64_t reg, const uint8_t* __restrict__ cls,
-                                            uint32_t t0, uint32_t K, uint32_t n, Fn&& fn) {
-  if (!LARGE || K <= (uint32_t)kOrderRegTiles) {
-    for (uint32_t k = 0; k < n; ++k) fn(k, (uint32_t)(reg >> (2 * k)) & 3u);
-    return;
-  }
-  for (uint32_t v = 0; v < n; v += 16u) {
-    const uint4 q = *reinterpret_cast<const uint4*>(cls + t0 + v);
-    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-    for (int j = 0; j < 16; ++j)
-      if (v + j < n) fn(v + j, (w[j >> 2] >> (8 * (j & 3))) & 3u);
-  }
-}
-
-// order[1 ..] = tiles [0, ntl) of class 0, then 1, then 2, each by index,
-// flagged kOrderHasOutput in classes 0 and 1; order[0] = tag.  The thread
-// owns tiles [tid * K, tid * K + K).
-template <bool LARGE>
-__device__ __forceinline__ void write_order(uint64_t reg, const uint8_t* __restrict__ cls,
-                                            uint32_t K, uint32_t ntl, uint32_t* order,
-                                            uint32_t tag, uint32_t* s_a, uint32_t* s_b) {
-  const uint32_t t0 = (uint32_t)threadIdx.x * K;
-  const uint32_t n = t0 < ntl ? min(K, ntl - t0) : 0u;
-  uint32_t c0 = 0u, c1 = 0u;
-  for_classes<LARGE>(reg, cls, t0, K, n, [&](uint32_t, uint32_t c) {
-    c0 += c == 0u;
-    c1 += c == 1u;
-  });
-  uint32_t e0, e1, tot0, tot1;
-  block_scan2(c0, c1, e0, e1, tot0, tot1, s_a, s_b);
-  uint32_t s0 = 1u + e0, s1 = 1u + tot0 + e1, s2 = 1u + tot0 + tot1 + (t0 - e0 - e1);
-  for_classes<LARGE>(reg, cls, t0, K, n, [&](uint32_t k, uint32_t c) {
-    const uint32_t slot = c == 0u ? s0++ : (c == 1u ? s1++ : s2++);
-    order[slot] = (t0 + k) | (c < 2u ? kOrderHasOutput : 0u);
-  });
-  if (threadIdx.x == 0) order[0] = tag;
-}
 
 // One block scans the n count-block sums (v | t << 32, each below 2^32)
 // 1024 at a time: a coalesced load, wave scans of the halves, the block's
@@ -269,11 +165,9 @@ __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long
   __shared__ uint32_t s_v[kWaves], s_t[kWaves];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const uint32_t ng = (uint32_t)ceil_div(go.nwords, (int64_t)go.tw);
-  const uint32_t ngt = (uint32_t)ceil_div(go.nwords, (int64_t)kGatherTileWords);
-  const uint32_t Kg = order_chunk(ng, kTsumThreads), Kt = order_chunk(ngt, kTsumThreads);
+  const uint32_t Kg = order_chunk(ng, kTsumThreads);
   const bool ordered = go.gen_classes;  // uniform
   const uint64_t gcls = ordered ? load_classes(go.gcls, (uint32_t)tid * Kg, Kg, ng) : 0ull;
-  const uint64_t tcls = load_classes(go.tcls, (uint32_t)tid * Kt, Kt, ngt);
 
   unsigned long long cv = 0, ct = 0;  // 64-bit carries
   for (int64_t e0 = 0; e0 < n; e0 += kTsumThreads) {
@@ -314,7 +208,8 @@ __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long
     write_order<LARGE>(gcls, go.gcls, Kg, ng, go.order, (uint32_t)go.tw, s_v, s_t);
   else if (tid == 0)
     go.order[0] = 0u;
-  write_order<LARGE>(tcls, go.tcls, Kt, ngt, go.gorder, (uint32_t)kGatherTileWords, s_v, s_t);
+  // the adjoint builds the gather tiles' order itself; until then there is none
+  if (tid == 0) go.gorder[0] = 0u;
 }
 
 // Bases of every row word (and entry nwords): tile prefix + local prefix.
@@ -755,10 +650,9 @@ int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu
     DAXHIP_LAUNCH_CHECK("k_count");
   }
   // with the dispatch order of the generate tiles
-  const GenOrder go{tc.gcls, tc.tcls, g.nwords, tw, tc.tw > 0, (uint32_t*)(ws + L.off_order),
+  const GenOrder go{tc.gcls, g.nwords, tw, tc.tw > 0, (uint32_t*)(ws + L.off_order),
                     (uint32_t*)(ws + L.off_gorder)};
-  const int64_t most = std::max(ceil_div(g.nwords, (int64_t)tw), ceil_div(g.nwords, (int64_t)kGatherTileWords));
-  if (ceil_div(most, (int64_t)kTsumThreads) > kOrderRegTiles)
+  if (ceil_div(ceil_div(g.nwords, (int64_t)tw), (int64_t)kTsumThreads) > kOrderRegTiles)
     k_scan_tsums<true><<<1, dim3(kTsumThreads), 0, s>>>(bsum, bpre, nblocks, (Ctrl*)(ws + L.off_ctrl), go);
   else
     k_scan_tsums<false><<<1, dim3(kTsumThreads), 0, s>>>(bsum, bpre, nblocks, (Ctrl*)(ws + L.off_ctrl), go);

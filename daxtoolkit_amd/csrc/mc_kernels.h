@@ -589,6 +589,115 @@ __host__ __device__ constexpr uint64_t edge_codes_packed() {
 }
 constexpr uint64_t kEdgeCodes = edge_codes_packed();
 
+// ---------------------------------------------------------------------------
+// Dispatch orders of tile grids (generate tiles, the adjoint's gather tiles):
+// tiles of class 0 first, then class 1, then class 2, each in index order,
+// from a class byte per tile.  The grid's blocks take order[1 + blockIdx].
+constexpr uint32_t kOrderHasOutput = 0x80000000u;  // order entry flag: class 0 or 1
+
+// A thread of the ordering block owns a chunk of K consecutive tiles.  Up to
+// kOrderRegTiles of them have their classes loaded before the scan, two bits
+// each in a register (the latencies overlap the scan's): byte loads for K
+// up to 8, 4-byte loads for K a multiple of 4 up to 32.  Larger chunks (K a
+// multiple of 16, volumes of more than 2^28 points) read their class bytes
+// in 16-byte vectors while the order is written.  The class arrays end in
+// 16 bytes of padding.
+constexpr int kOrderRegTiles = 32;
+__device__ __forceinline__ uint32_t order_chunk(uint32_t ntl, uint32_t threads) {
+  const uint32_t k = (ntl + threads - 1) / threads;
+  return k <= 8u ? k : (k <= (uint32_t)kOrderRegTiles ? (k + 3u) / 4u * 4u : (k + 15u) / 16u * 16u);
+}
+__device__ __forceinline__ uint64_t load_classes(const uint8_t* __restrict__ cls, uint32_t t0,
+                                                 uint32_t K, uint32_t ntl) {
+  uint64_t r = 0ull;
+  if (K <= 8u) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if ((uint32_t)k < K && t0 + k < ntl) r |= (uint64_t)(cls[t0 + k] & 3u) << (2 * k);
+  } else if (K <= (uint32_t)kOrderRegTiles) {
+#pragma unroll
+    for (int v = 0; v < kOrderRegTiles / 4; ++v) {
+      const uint32_t t = t0 + 4u * v;
+      if (4u * v < K && t < ntl) {
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(cls + t);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r |= (uint64_t)((w >> (8 * j)) & 3u) << (2 * (4 * v + j));
+      }
+    }
+  }
+  return r;
+}
+
+// Exclusive block scans of two counts per thread (all threads call it; s_a
+// and s_b hold a word per wave).
+__device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& ea, uint32_t& eb,
+                                            uint32_t& ta, uint32_t& tb, uint32_t* s_a,
+                                            uint32_t* s_b) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t ia = wave_incl_scan(a), ib = wave_incl_scan(b);
+  if (lane == 63) {
+    s_a[wv] = ia;
+    s_b[wv] = ib;
+  }
+  __syncthreads();
+  uint32_t oa = 0u, ob = 0u;
+  ta = tb = 0u;
+  const int nwv = (int)(blockDim.x >> 6);
+  for (int q = 0; q < nwv; ++q) {
+    if (q < wv) {
+      oa += s_a[q];
+      ob += s_b[q];
+    }
+    ta += s_a[q];
+    tb += s_b[q];
+  }
+  __syncthreads();  // s_a / s_b reusable
+  ea = oa + ia - a;
+  eb = ob + ib - b;
+}
+
+// Calls fn(k, class) for the thread's tiles k = 0 .. n-1 in order: from reg
+// (K <= kOrderRegTiles) or from the class bytes at cls + t0.
+template <bool LARGE, class Fn>
+__device__ __forceinline__ void for_classes(uint64_t reg, const uint8_t* __restrict__ cls,
+                                            uint32_t t0, uint32_t K, uint32_t n, Fn&& fn) {
+  if (!LARGE || K <= (uint32_t)kOrderRegTiles) {
+    for (uint32_t k = 0; k < n; ++k) fn(k, (uint32_t)(reg >> (2 * k)) & 3u);
+    return;
+  }
+  for (uint32_t v = 0; v < n; v += 16u) {
+    const uint4 q = *reinterpret_cast<const uint4*>(cls + t0 + v);
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (v + j < n) fn(v + j, (w[j >> 2] >> (8 * (j & 3))) & 3u);
+  }
+}
+
+// order[1 ..] = tiles [0, ntl) of class 0, then 1, then 2, each by index,
+// flagged kOrderHasOutput in classes 0 and 1; order[0] = tag.  The thread
+// owns tiles [tid * K, tid * K + K).
+template <bool LARGE>
+__device__ __forceinline__ void write_order(uint64_t reg, const uint8_t* __restrict__ cls,
+                                            uint32_t K, uint32_t ntl, uint32_t* order,
+                                            uint32_t tag, uint32_t* s_a, uint32_t* s_b) {
+  const uint32_t t0 = (uint32_t)threadIdx.x * K;
+  const uint32_t n = t0 < ntl ? min(K, ntl - t0) : 0u;
+  uint32_t c0 = 0u, c1 = 0u;
+  for_classes<LARGE>(reg, cls, t0, K, n, [&](uint32_t, uint32_t c) {
+    c0 += c == 0u;
+    c1 += c == 1u;
+  });
+  uint32_t e0, e1, tot0, tot1;
+  block_scan2(c0, c1, e0, e1, tot0, tot1, s_a, s_b);
+  uint32_t s0 = 1u + e0, s1 = 1u + tot0 + e1, s2 = 1u + tot0 + tot1 + (t0 - e0 - e1);
+  for_classes<LARGE>(reg, cls, t0, K, n, [&](uint32_t k, uint32_t c) {
+    const uint32_t slot = c == 0u ? s0++ : (c == 1u ? s1++ : s2++);
+    order[slot] = (t0 + k) | (c < 2u ? kOrderHasOutput : 0u);
+  });
+  if (threadIdx.x == 0) order[0] = tag;
+}
+
 // LDS layout of a generate tile of TW words (wpr known at run time).
 constexpr int kMaxGenThreads = 256;
 template <int TW>

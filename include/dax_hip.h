@@ -552,7 +552,8 @@ int dax_hip_graph_destroy(void* graph_exec);
  * dL/dfield of L(vertices, normals) given dL/dvertices [U][3] and
  * dL/dnormals [U][3], with the topology of the forward held fixed.  Uses the
  * forward's workspace (inside-bits and vertex bases must still hold the
- * forward of this field and iso) and its compact edge list d_edges; the
+ * forward of this field and iso; the backward writes only the dispatch order
+ * of its gather tiles there) and its compact edge list d_edges; the
  * interpolants (t, endpoint gradients) are recomputed from the field.
  * U = min(d_counts[1], n_points) if d_counts is given (device {T, U} of the
  * forward), else n_points.  d_scratch needs
@@ -565,7 +566,7 @@ int dax_hip_graph_destroy(void* graph_exec);
 size_t dax_hip_mc_backward_scratch_bytes(int64_t n_points);
 int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz,
                         const float* h_origin3, const float* h_spacing3, float iso,
-                        const void* d_workspace, size_t workspace_bytes, const uint32_t* d_edges,
+                        void* d_workspace, size_t workspace_bytes, const uint32_t* d_edges,
                         const int64_t* d_counts, int64_t n_points, const float* d_dl_dvertices,
                         const float* d_dl_dnormals, void* d_scratch, size_t scratch_bytes,
                         float* d_dl_dfield, void* stream);
