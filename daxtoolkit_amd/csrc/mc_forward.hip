@@ -86,7 +86,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_count(VolGeom g, const uint64_t*
   constexpr int kBlockTiles = 2 * WAVES;  // count tiles per block
   __shared__ unsigned long long s_ts[kBlockTiles];
   const int wv = threadIdx.x >> 6;
-  const int64_t pair = (int64_t)blockIdx.x * WAVES + wv;
+  // XCD-contiguous block order: the block one plane ahead, whose words this
+  // block reads as +z neighbours, runs on the same XCD (same L2)
+  const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t pair = blk * WAVES + wv;
   unsigned long long s0 = 0ull, s1 = 0ull;
   if (2 * pair < n_ctiles) count_tile_pair(g, bits + g.word_off, lpre, pair, n_ctiles, s0, s1);
   if ((threadIdx.x & 63) == 0) {
@@ -97,29 +100,29 @@ __global__ __launch_bounds__(64 * WAVES) void k_count(VolGeom g, const uint64_t*
   if (threadIdx.x == 0) {
     unsigned long long run = 0ull;
     for (int i = 0; i < kBlockTiles; ++i) {
-      const int64_t tile = (int64_t)blockIdx.x * kBlockTiles + i;
+      const int64_t tile = blk * kBlockTiles + i;
       if (tile < n_ctiles) tloc[tile] = run;
       run = add_vt(run, s_ts[i]);
     }
-    bsum[blockIdx.x] = run;
+    bsum[blk] = run;
   } else if (threadIdx.x < 64) {  // the block's generate and gather tiles
     const int lane = threadIdx.x;
     auto weight = [&](int first, int n) -> uint32_t {  // outputs of count tiles [first, +n)
       uint32_t w = 0u;
       for (int i = 0; i < n; ++i)
-        if ((int64_t)blockIdx.x * kBlockTiles + first + i < n_ctiles)
+        if (blk * kBlockTiles + first + i < n_ctiles)
           w += (uint32_t)s_ts[first + i] + (uint32_t)(s_ts[first + i] >> 32);
       return w;
     };
     constexpr int gct = kGatherTileWords / kCountTile;
     const int l = lane - 1;
     if (l >= 0 && l < (kBlockTiles + gct - 1) / gct) {
-      const int64_t c0 = (int64_t)blockIdx.x * kBlockTiles + l * gct;
+      const int64_t c0 = blk * kBlockTiles + l * gct;
       if (c0 < n_ctiles) tc.tcls[c0 / gct] = weight(l * gct, gct) == 0u ? 2 : 0;
     }
     const int cpt = tc.tw / kCountTile;
     if (tc.tw > 0 && l >= 0 && l < kBlockTiles / cpt) {
-      const int64_t c0 = (int64_t)blockIdx.x * kBlockTiles + l * cpt;
+      const int64_t c0 = blk * kBlockTiles + l * cpt;
       if (c0 < n_ctiles) {
         const uint32_t w = weight(l * cpt, cpt);
         tc.gcls[c0 / cpt] = w == 0u ? 2 : (4ull * w >= 5ull * (uint32_t)tc.tw ? 0 : 1);
