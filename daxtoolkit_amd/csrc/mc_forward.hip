@@ -144,7 +144,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_count(VolGeom g, const uint64_t*
 struct GenOrder {
   const uint8_t* __restrict__ gcls;  // classes of the generate tiles
   int64_t nwords;
-  int tw;            // generate tile words: 128, 256, 512 or 1024
+  int tw;            // generate tile words: 64, 128, 256, 512 or 1024
   bool gen_classes;  // gcls holds the classes
   uint32_t* order;   // [0]: tw (0: no order), [1 + i]: the i-th tile to dispatch
   uint32_t* gorder;  // the gather tiles' order: entry 0 cleared (the adjoint writes it)
@@ -348,8 +348,8 @@ bool valid_tuning(const Tuning& t) {
   const bool cls = (t.cls_vec == 1 && (t.cls_unroll == 8 || t.cls_unroll == 16 || t.cls_unroll == 32)) ||
                    (t.cls_vec == 4 && (t.cls_unroll == 2 || t.cls_unroll == 4 || t.cls_unroll == 8));
   const bool scan = t.scan_sub == 1 || t.scan_sub == 2 || t.scan_sub == 4 || t.scan_sub == 8;
-  const bool gen = (t.gen_tile == 128 || t.gen_tile == 256 || t.gen_tile == 512 ||
-                    t.gen_tile == 1024) &&
+  const bool gen = (t.gen_tile == 64 || t.gen_tile == 128 || t.gen_tile == 256 ||
+                    t.gen_tile == 512 || t.gen_tile == 1024) &&
                    (t.gen_xcd == 0 || t.gen_xcd == 1);
   return cls && scan && gen;
 }
@@ -608,7 +608,9 @@ int launch_bits(const VolGeom& g, const float* field, char* ws, const Workspace&
 
 // Generate tile words of a tuning (launch_generate_src's dispatch).
 int gen_tile_words(const Tuning& tu) {
-  return (tu.gen_tile == 128 || tu.gen_tile == 256 || tu.gen_tile == 1024) ? tu.gen_tile : 512;
+  return (tu.gen_tile == 64 || tu.gen_tile == 128 || tu.gen_tile == 256 || tu.gen_tile == 1024)
+             ? tu.gen_tile
+             : 512;
 }
 
 // Where the generate kernel reads the word bases; a count block holds two
@@ -640,7 +642,7 @@ int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu
   const int tw = gen_tile_words(tu);
   const int block_words = 2 * (waves == 1 || waves == 2 || waves == 8 ? waves : 4) * kCountTile;
   const TileClasses tc{(uint8_t*)(ws + L.off_gcls), (uint8_t*)(ws + L.off_tcls),
-                       tw <= block_words ? tw : 0};
+                       tw >= kCountTile && tw <= block_words ? tw : 0};
   const int64_t nblocks = ceil_div(n, 2 * waves);
   if (nblocks > 0) {
     const dim3 grid((unsigned)nblocks), block(64 * waves);
@@ -695,6 +697,9 @@ int launch_generate_src(const VolGeom& g, const Src& field, const char* ws, cons
                         int64_t* counts, const Tuning& tu, hipStream_t s, int64_t t_lo = 0,
                         int64_t t_hi = -1) {
   switch (tu.gen_tile) {
+    case 64:
+      return launch_generate_tw<64>(g, field, ws, L, flags, o, cap_points, cap_tris, counts, tu,
+                                    s, t_lo, t_hi);
     case 128:
       return launch_generate_tw<128>(g, field, ws, L, flags, o, cap_points, cap_tris, counts,
                                      tu, s, t_lo, t_hi);
@@ -1210,14 +1215,15 @@ int dax_hip_mc_autotune(const float* d_field, int64_t nx, int64_t ny, int64_t nz
       best.scan_sub = sub;
     }
   }
-  // generate variants over one classify + scan
+  // generate variants over one classify; the scan again for each tile size,
+  // so that its dispatch order is the one that variant runs with
   if ((rc = launch_bits(g, d_field, ws, L, best, s))) return rc;
-  if ((rc = launch_scan(g, ws, L, best, s))) return rc;
-  for (int tv = 0; tv < 8; ++tv) {
-    const int tw = 128 << (tv % 4);
+  for (int tv = 0; tv < 10; ++tv) {
+    const int tw = 64 << (tv % 5);
     Tuning t = best;
     t.gen_tile = tw;
-    t.gen_xcd = tv / 4;
+    t.gen_xcd = tv / 5;
+    if ((rc = launch_scan(g, ws, L, t, s))) return rc;
     if ((rc = launch_generate(g, d_field, ws, L, flags, o, cap_points, cap_tris, d_counts, t, s)))
       return rc;
     DAXHIP_CHECK(hipEventRecord(ev.a, s));

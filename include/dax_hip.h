@@ -431,9 +431,9 @@ int dax_hip_mc_cell_outputs(int64_t nx, int64_t ny, int64_t nz, const void* d_wo
  * params5 = {classify_unroll, classify_vec, scan_sub, gen_tile, gen_xcd}:
  *   classify_vec 1 (one float per lane, unroll 8/16/32 row words per wave)
  *   or 4 (float4 per lane, unroll 2/4/8 chunks of 256 points; rows of a
- *   multiple of 64 points); scan_sub 1/2/4/8 (count tiles of 256 row words,
- *   one wave each, per block of the count kernel); gen_tile
- *   128/256/512/1024 (row words per generate block); gen_xcd 1 gives each
+ *   multiple of 64 points); scan_sub 1/2/4/8 (waves per block of the count
+ *   kernel, two count tiles of 128 row words per wave); gen_tile
+ *   64/128/256/512/1024 (row words per generate block); gen_xcd 1 gives each
  *   XCD a contiguous range of generate tiles, 0 keeps the plain block order.
  * A shape of (0,0,0) sets or gets the default for shapes not tuned. */
 int dax_hip_mc_set_tuning(int64_t nx, int64_t ny, int64_t nz, const int* h_params5);

@@ -207,7 +207,7 @@ def test_every_tuning_variant_is_bit_identical(shape_kind):
     default = mc.get_tuning((0, 0, 0))
     variants = ([[u, 1, 4, 512] for u in (8, 16, 32)] + [[u, 4, 4, 512] for u in (2, 4, 8)] +
                 [[16, 1, s, 512] for s in (1, 2, 4, 8)] +
-                [[16, 1, 4, t, x] for t in (128, 256, 512, 1024) for x in (0, 1)])
+                [[16, 1, 4, t, x] for t in (64, 128, 256, 512, 1024) for x in (0, 1)])
     ref = None
     try:
         for v in variants:
