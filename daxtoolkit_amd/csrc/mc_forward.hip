@@ -146,6 +146,7 @@ struct GenOrder {
   int64_t nwords;
   int tw;            // generate tile words: 64, 128, 256, 512 or 1024
   bool gen_classes;  // gcls holds the classes
+  uint32_t unit_sp;  // > 0: order units of kOrderUnitTiles tiles a plane (unit_sp tiles) apart
   uint32_t* order;   // [0]: tw (0: no order), [1 + i]: the i-th tile to dispatch
   uint32_t* gorder;  // the gather tiles' order: entry 0 cleared (the adjoint writes it)
 };
@@ -170,7 +171,12 @@ __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long
   const uint32_t ng = (uint32_t)ceil_div(go.nwords, (int64_t)go.tw);
   const uint32_t Kg = order_chunk(ng, kTsumThreads);
   const bool ordered = go.gen_classes;  // uniform
-  const uint64_t gcls = ordered ? load_classes(go.gcls, (uint32_t)tid * Kg, Kg, ng) : 0ull;
+  const bool units = ordered && go.unit_sp > 0u;
+  const uint32_t nunits = units ? ng / kOrderUnitTiles : 0u;
+  const uint32_t Ku = units ? order_chunk(nunits, kTsumThreads) : 0u;
+  const uint64_t gcls =
+      units ? load_unit_classes(go.gcls, Ku, nunits, go.unit_sp)
+            : (ordered ? load_classes(go.gcls, (uint32_t)tid * Kg, Kg, ng) : 0ull);
 
   unsigned long long cv = 0, ct = 0;  // 64-bit carries
   for (int64_t e0 = 0; e0 < n; e0 += kTsumThreads) {
@@ -207,7 +213,9 @@ __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long
     ctrl->total_t = ct;
   }
 
-  if (ordered)
+  if (units)
+    write_order_units(gcls, Ku, nunits, go.unit_sp, go.order, (uint32_t)go.tw, s_v, s_t);
+  else if (ordered)
     write_order<LARGE>(gcls, go.gcls, Kg, ng, go.order, (uint32_t)go.tw, s_v, s_t);
   else if (tid == 0)
     go.order[0] = 0u;
@@ -654,9 +662,16 @@ int launch_scan(const VolGeom& g, char* ws, const Workspace& L, const Tuning& tu
     }
     DAXHIP_LAUNCH_CHECK("k_count");
   }
-  // with the dispatch order of the generate tiles
-  const GenOrder go{tc.gcls, g.nwords, tw, tc.tw > 0, (uint32_t*)(ws + L.off_order),
-                    (uint32_t*)(ws + L.off_gorder)};
+  // with the dispatch order of the generate tiles, in units of two tiles a
+  // plane apart when tiles split planes evenly and the units fill groups of 8
+  const int64_t ng = ceil_div(g.nwords, (int64_t)tw);
+  const int64_t plane = (int64_t)g.wpr * g.ny;
+  const int64_t sp = plane % tw == 0 ? plane / tw : 0;
+  constexpr int64_t kUnitG = kOrderUnitTiles;
+  const bool units = tc.tw > 0 && sp > 0 && ng % (kUnitG * sp) == 0 && (ng / kUnitG) % 8 == 0 &&
+                     ceil_div(ng / kUnitG, (int64_t)kTsumThreads) * kUnitG <= kOrderRegTiles;
+  const GenOrder go{tc.gcls, g.nwords, tw, tc.tw > 0, units ? (uint32_t)sp : 0u,
+                    (uint32_t*)(ws + L.off_order), (uint32_t*)(ws + L.off_gorder)};
   if (ceil_div(ceil_div(g.nwords, (int64_t)tw), (int64_t)kTsumThreads) > kOrderRegTiles)
     k_scan_tsums<true><<<1, dim3(kTsumThreads), 0, s>>>(bsum, bpre, nblocks, (Ctrl*)(ws + L.off_ctrl), go);
   else

@@ -698,6 +698,76 @@ __device__ __forceinline__ void write_order(uint64_t reg, const uint8_t* __restr
   if (threadIdx.x == 0) order[0] = tag;
 }
 
+// The same order over units of G tiles that lie a plane apart: unit u =
+// zg * sp + r (sp tiles per plane) holds tiles (G zg + m) sp + r, m < G,
+// and takes the lowest class of its tiles.  The unit of rank R puts its
+// tiles at positions 8 G (R / 8) + R % 8 + 8 m: blocks go to the XCDs
+// round robin, so a unit's tiles run on one XCD, in consecutive rounds, and
+// the field planes they share (three of four for neighbouring planes) are
+// fetched into that XCD's L2 once.  Needs every unit whole and nunits a
+// multiple of 8; K G <= kOrderRegTiles (the tile classes are loaded before
+// the scan, two bits each, by load_unit_classes).
+constexpr uint32_t kOrderUnitTiles = 2;  // G
+__device__ __forceinline__ uint64_t load_unit_classes(const uint8_t* __restrict__ cls, uint32_t K,
+                                                      uint32_t nunits, uint32_t sp) {
+  constexpr uint32_t G = kOrderUnitTiles;
+  const uint32_t u0 = (uint32_t)threadIdx.x * K;
+  const uint32_t n = u0 < nunits ? min(K, nunits - u0) : 0u;
+  uint64_t reg = 0ull;  // class of tile m of unit k at bits 2 (k G + m)
+  uint32_t zg = n ? u0 / sp : 0u, r = n ? u0 - zg * sp : 0u;
+#pragma unroll
+  for (int k = 0; k < kOrderRegTiles / (int)G; ++k) {
+    if ((uint32_t)k < n) {
+#pragma unroll
+      for (uint32_t m = 0; m < G; ++m)
+        reg |= (uint64_t)(cls[(G * zg + m) * sp + r] & 3u) << (2u * (k * G + m));
+      if (++r == sp) {
+        r = 0u;
+        ++zg;
+      }
+    }
+  }
+  return reg;
+}
+__device__ __forceinline__ void write_order_units(uint64_t reg, uint32_t K, uint32_t nunits,
+                                                  uint32_t sp, uint32_t* order, uint32_t tag,
+                                                  uint32_t* s_a, uint32_t* s_b) {
+  constexpr uint32_t G = kOrderUnitTiles;
+  const uint32_t u0 = (uint32_t)threadIdx.x * K;
+  const uint32_t n = u0 < nunits ? min(K, nunits - u0) : 0u;
+  auto unit_class = [&](uint32_t k) {
+    uint32_t c = 3u;
+#pragma unroll
+    for (uint32_t m = 0; m < G; ++m) c = min(c, (uint32_t)(reg >> (2u * (k * G + m))) & 3u);
+    return c;
+  };
+  uint32_t c0 = 0u, c1 = 0u;
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t c = unit_class(k);
+    c0 += c == 0u;
+    c1 += c == 1u;
+  }
+  uint32_t e0, e1, tot0, tot1;
+  block_scan2(c0, c1, e0, e1, tot0, tot1, s_a, s_b);
+  uint32_t s0 = e0, s1 = tot0 + e1, s2 = tot0 + tot1 + (u0 - e0 - e1);
+  uint32_t zg = n ? u0 / sp : 0u, r = n ? u0 - zg * sp : 0u;
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t c = unit_class(k);
+    const uint32_t R = c == 0u ? s0++ : (c == 1u ? s1++ : s2++);
+    const uint32_t base = 1u + 8u * G * (R >> 3) + (R & 7u);
+#pragma unroll
+    for (uint32_t m = 0; m < G; ++m) {
+      const uint32_t tc = (uint32_t)(reg >> (2u * (k * G + m))) & 3u;
+      order[base + 8u * m] = ((G * zg + m) * sp + r) | (tc < 2u ? kOrderHasOutput : 0u);
+    }
+    if (++r == sp) {
+      r = 0u;
+      ++zg;
+    }
+  }
+  if (threadIdx.x == 0) order[0] = tag;
+}
+
 // LDS layout of a generate tile of TW words (wpr known at run time).
 constexpr int kMaxGenThreads = 256;
 template <int TW>

@@ -114,3 +114,27 @@ def test_adjoint_deterministic_and_local(bulb):
         uniform = (hi == lo)[0, 0, z0 - h0: z0 - h0 + (z1 - z0)]
         assert int((g1[z0:z1][uniform] != 0).sum()) == 0
         del b, hi, lo, uniform
+
+
+@pytest.mark.parametrize("tuning", [[4, 4, 8, 128, 0], [4, 4, 2, 256, 1]])
+def test_small_generate_tiles_same_output(bulb, tuning):
+    """Generate tiles of 128 and 256 words: 131072 and 65536 tiles, so each of
+    the scan block's 1024 threads orders more than 32 tiles (the vector
+    chunk path of the dispatch order).  The output is the default tiling's,
+    bit for bit, in the same order."""
+    f, out = bulb
+    shape = (N, N, N)
+    saved = mc.get_tuning(shape)
+    print(f"[case] mandelbulb forward tuning {tuning}", flush=True)
+    try:
+        mc.set_tuning(shape, tuning)
+        o2 = mc.forward(f, ISO, edges=True)
+        torch.cuda.synchronize()
+    finally:
+        mc.set_tuning(shape, saved)
+    assert (o2.num_points, o2.num_triangles) == (out.num_points, out.num_triangles)
+    assert torch.equal(o2.edges, out.edges)
+    assert torch.equal(o2.connectivity, out.connectivity)
+    assert torch.equal(o2.vertices.view(torch.int32), out.vertices.view(torch.int32))
+    assert torch.equal(o2.normals.view(torch.int32), out.normals.view(torch.int32))
+    del o2
