@@ -146,6 +146,22 @@ def cpu_baseline(f_host, iso, cells, budget_s):
     }
 
 
+def _omp_baseline(run, units, unit, what, budget_s):
+    """Median of up to 5 runs (within budget_s) of an OpenMP restatement:
+    run(threads) -> (seconds, *counts).  Returns the cpu_baseline entry and
+    the counts of the last run."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    times, counts = [], ()
+    t_start = time.perf_counter()
+    while len(times) < 5 and (len(times) == 0 or time.perf_counter() - t_start < budget_s):
+        sec, *counts = run(threads)
+        times.append(sec)
+    med = statistics.median(times)
+    return {"value": round(units / med / 1e9, 4), "unit": unit, "cores": threads, "kind": "port",
+            "sample": f"the full input, {what}, median of {len(times)} runs, {med:.3f} s each"}, \
+        tuple(counts)
+
+
 def _timed(fn, steps, per_step=None):
     """Mean ms of fn() over `steps` calls, HIP events on the current stream
     (one event between consecutive steps; the per-step times go into
@@ -453,13 +469,14 @@ def tetrahedra_128(args):
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
         tets_h, pts_h = tets.cpu().numpy(), pts.cpu().numpy()
-        t0 = time.perf_counter()
-        ref = orc.mt_generate(tets_h, f_h.reshape(-1), pts_h, iso, dedup=True)
-        sec = time.perf_counter() - t0
-        assert (ref["n_tris"], ref["n_pts"]) == (T, U)
-        res["cpu_baseline"] = {"value": round(C / sec / 1e9, 4), "unit": "Gtets/s", "cores": 1,
-                               "kind": "port", "sample": f"the full input, serial restatement "
-                                                         f"(oracle), one run, {sec:.2f} s"}
+        cb, counts = _omp_baseline(
+            lambda th: orc.unstructured_pipeline_omp(tets_h, f_h.reshape(-1), pts_h, iso,
+                                                     dedup=True, threads=th),
+            C, "Gtets/s", "OpenMP restatement of Dax's OpenMP-adapter marching-tetrahedra "
+            "sequence (classify, scan, upper/lower bounds, interpolation records, parallel "
+            "sort+unique dedup, coordinates)", args.cpu_seconds)
+        assert counts == (T, U)
+        res["cpu_baseline"] = cb
     _progress(f"tetrahedra {n}^3: {ms:.3f} ms, T={T}")
     return res
 
@@ -518,13 +535,13 @@ def threshold_512(args):
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
         f_h = f.cpu().numpy()
-        t0 = time.perf_counter()
-        ref = orc.threshold(f_h, lo, hi)
-        sec = time.perf_counter() - t0
-        assert (ref["n_cells"], ref["n_points"]) == (T, U)
-        res["cpu_baseline"] = {"value": round(C / sec / 1e9, 4), "unit": "Gcells/s", "cores": 1,
-                               "kind": "port", "sample": f"the full input, serial restatement "
-                                                         f"(oracle), one run, {sec:.2f} s"}
+        cb, counts = _omp_baseline(
+            lambda th: orc.threshold_pipeline_omp(f_h, lo, hi, dedup=True, threads=th),
+            C, "Gcells/s", "OpenMP restatement of Dax's OpenMP-adapter threshold sequence "
+            "(count, scan, upper bounds, topology, point mask, stream compact, parallel "
+            "sort+unique, lower bounds)", args.cpu_seconds)
+        assert counts == (T, U)
+        res["cpu_baseline"] = cb
     del f, out, vals
     torch.cuda.empty_cache()
     _progress(f"threshold {n}^3: {ms:.3f} ms, T={T}, U={U}")
@@ -607,13 +624,14 @@ def hexahedra_256(args):
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
         hexes_h, pts_h, f_h = hexes.cpu().numpy(), pts.cpu().numpy(), f.cpu().numpy()
-        t0 = time.perf_counter()
-        ref = orc.mc_hex_generate(hexes_h, f_h, pts_h, iso, dedup=True)
-        sec = time.perf_counter() - t0
-        assert (ref["n_tris"], ref["n_pts"]) == (T, U)
-        res["cpu_baseline"] = {"value": round(C / sec / 1e9, 4), "unit": "Gcells/s", "cores": 1,
-                               "kind": "port", "sample": f"the full input, serial restatement "
-                                                         f"(oracle), one run, {sec:.2f} s"}
+        cb, counts = _omp_baseline(
+            lambda th: orc.unstructured_pipeline_omp(hexes_h, f_h, pts_h, iso, dedup=True,
+                                                     threads=th),
+            C, "Gcells/s", "OpenMP restatement of Dax's OpenMP-adapter marching-cubes sequence "
+            "on hexahedra (classify, scan, upper/lower bounds, interpolation records, parallel "
+            "sort+unique dedup, coordinates)", args.cpu_seconds)
+        assert counts == (T, U)
+        res["cpu_baseline"] = cb
     del hexes, pts, field
     torch.cuda.empty_cache()
     _progress(f"hexahedra {n}^3: {ms:.3f} ms, T={T}")

@@ -1075,3 +1075,222 @@ extern "C" double oracle_mc_pipeline_omp(const float* field, const int64_t* dims
   return -1.0;
 #endif
 }
+
+// ---------------------------------------------------------------------------
+// CPU baselines of the unstructured-grid and threshold paths: the
+// Serial-adapter sequences of unstructured_generate and oracle_threshold
+// with every step on the OpenMP threads, as Dax's OpenMP adapter runs them
+// (dax/openmp/cont/internal/DeviceAdapterAlgorithmOpenMP.h:34-144 over the
+// Thrust algorithms of DeviceAdapterAlgorithmThrust.h).  Inputs already in
+// memory; each returns seconds and the output counts.
+namespace {
+
+int omp_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+double wall_seconds() {
+#ifdef _OPENMP
+  return omp_get_wtime();
+#else
+  return -1.0;
+#endif
+}
+
+// Blocked two-pass inclusive scan of n counts (Thrust inclusive_scan on
+// the OpenMP backend); returns the total.
+int64_t omp_inclusive_scan(const int32_t* count, Id* scan, int64_t n) {
+  const int nth = omp_threads();
+  std::vector<int64_t> part(nth + 1, 0);
+#pragma omp parallel num_threads(nth)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    const int64_t lo = n * tid / nth, hi = n * (tid + 1) / nth;
+    int64_t s = 0;
+    for (int64_t c = lo; c < hi; ++c) s += count[c];
+    part[tid + 1] = s;
+#pragma omp barrier
+#pragma omp single
+    for (int i = 0; i < nth; ++i) part[i + 1] += part[i];
+    int64_t run = part[tid];
+    for (int64_t c = lo; c < hi; ++c) {
+      run += count[c];
+      scan[c] = (Id)run;
+    }
+  }
+  return part[nth];
+}
+
+template <class It>
+void omp_sort(It b, It e) {
+#ifdef _OPENMP
+  __gnu_parallel::sort(b, e);
+#else
+  std::sort(b, e);
+#endif
+}
+
+// unstructured_generate's sequence (MarchingTetrahedra.h:29-143 /
+// MarchingCubes.h:53-164 through DispatcherGenerateInterpolatedCells.h:
+// 138-301): classify, scan, upper bounds (triangle -> cell), lower bounds
+// (visit index), the corners' interpolation records, sort + unique + lower
+// bounds (dedup), coordinates.
+template <int NV, class CaseEdges>
+double unstructured_pipeline_omp(const int32_t* cells, int64_t n_cells, const float* field,
+                                 const float* points, float iso, int dedup,
+                                 CaseEdges case_edges, const int (*ends)[2], int64_t* n_tris,
+                                 int64_t* n_pts) {
+  const double t0 = wall_seconds();
+  auto digit = [](char ch) { return ch <= '9' ? ch - '0' : ch - 'a' + 10; };
+  int32_t num_faces[1 << NV];
+  for (int k = 0; k < (1 << NV); ++k) num_faces[k] = (int32_t)(strlen(case_edges(k)) / 3);
+  std::vector<uint8_t> cls(n_cells);
+  std::vector<int32_t> count(n_cells);
+  std::vector<Id> incl(n_cells);
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < n_cells; ++c) {
+    int k = 0;
+    for (int v = 0; v < NV; ++v) k |= (field[cells[NV * c + v]] > iso) << v;
+    cls[c] = (uint8_t)k;
+    count[c] = num_faces[k];
+  }
+  const int64_t T = omp_inclusive_scan(count.data(), incl.data(), n_cells);
+  *n_tris = T;
+  *n_pts = 0;
+  if (T == 0) return wall_seconds() - t0;
+  std::vector<Id> vcr(T), visit(T);
+#pragma omp parallel for schedule(static)
+  for (int64_t t = 0; t < T; ++t)
+    vcr[t] = (Id)(std::upper_bound(incl.begin(), incl.end(), (Id)t) - incl.begin());
+#pragma omp parallel for schedule(static)
+  for (int64_t t = 0; t < T; ++t)
+    visit[t] = (Id)(t - (std::lower_bound(vcr.begin(), vcr.end(), vcr[t]) - vcr.begin()));
+  std::vector<PointAsEdgeInterpolation> rec(3 * T);
+#pragma omp parallel for schedule(static)
+  for (int64_t t = 0; t < T; ++t) {
+    const int64_t c = vcr[t];
+    const int32_t* id = cells + NV * c;
+    for (int v = 0; v < 3; ++v) {
+      const int e = digit(case_edges(cls[c])[3 * visit[t] + v]);
+      int va = ends[e][0], vb = ends[e][1];
+      if (!(id[va] < id[vb])) std::swap(va, vb);
+      const Scalar w = (iso - field[id[va]]) / (field[id[vb]] - field[id[va]]);
+      rec[3 * t + v] = PointAsEdgeInterpolation{id[va], id[vb], w};
+    }
+  }
+  std::vector<PointAsEdgeInterpolation> pts(rec);
+  std::vector<int32_t> conn(3 * T);
+  if (dedup) {
+    omp_sort(pts.begin(), pts.end());
+    pts.erase(std::unique(pts.begin(), pts.end()), pts.end());
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < 3 * T; ++i)
+      conn[i] = (int32_t)(std::lower_bound(pts.begin(), pts.end(), rec[i]) - pts.begin());
+  } else {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < 3 * T; ++i) conn[i] = (int32_t)i;
+  }
+  const int64_t U = (int64_t)pts.size();
+  std::vector<float> coords(3 * U);
+#pragma omp parallel for schedule(static)
+  for (int64_t u = 0; u < U; ++u)
+    for (int d = 0; d < 3; ++d) {
+      const Scalar a = points[3 * (int64_t)pts[u].first + d];
+      const Scalar b = points[3 * (int64_t)pts[u].second + d];
+      coords[3 * u + d] = a + pts[u].weight * (b - a);
+    }
+  *n_pts = U;
+  return wall_seconds() - t0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Marching tetrahedra (nv 4) or marching cubes on hexahedra (nv 8).
+double oracle_unstructured_pipeline_omp(const int32_t* cells, int nv, int64_t n_cells,
+                                        const float* field, const float* points, float iso,
+                                        int dedup, int threads, int64_t* n_tris,
+                                        int64_t* n_pts) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#endif
+  if (nv == 4)
+    return unstructured_pipeline_omp<4>(cells, n_cells, field, points, iso, dedup,
+                                        [](int c) { return kTetTri[c]; }, kTetEdge, n_tris,
+                                        n_pts);
+  return unstructured_pipeline_omp<8>(cells, n_cells, field, points, iso, dedup,
+                                      [](int c) { return kOracleCaseEdges[c]; }, kHexEdge,
+                                      n_tris, n_pts);
+}
+
+// oracle_threshold's sequence on the uniform grid `dims` (ThresholdCount
+// through DispatcherMapCell, then DispatcherGenerateTopology.h:146-256):
+// per-cell count, scan, upper bounds and the cells' vertex ids, and with
+// dedup the point mask, StreamCompact of the coordinates by it, sort +
+// unique of the used ids and lower bounds of the connectivity in them, and
+// the field compacted by the mask (CompactPointField).
+double oracle_threshold_pipeline_omp(const int64_t* dims, const float* origin,
+                                     const float* spacing, const float* field, float lo,
+                                     float hi, int dedup, int threads, int64_t* n_out_cells,
+                                     int64_t* n_out_points) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#endif
+  const double t0 = wall_seconds();
+  const Grid g = make_grid(dims, origin, spacing);
+  const int64_t n_cells = g.cells(), n_points = g.points();
+  std::vector<int32_t> count(n_cells);
+  std::vector<Id> scanned(n_cells);
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < n_cells; ++c) {
+    Id v[8];
+    cell_vertices(g, c, v);
+    int valid = 1;
+    for (int q = 0; q < 8; ++q) valid &= field[v[q]] >= lo && field[v[q]] <= hi;
+    count[c] = valid;
+  }
+  const int64_t T = omp_inclusive_scan(count.data(), scanned.data(), n_cells);
+  *n_out_cells = T;
+  *n_out_points = 0;
+  std::vector<Id> conn(8 * T);
+#pragma omp parallel for schedule(static)
+  for (int64_t t = 0; t < T; ++t) {
+    const int64_t c = std::upper_bound(scanned.begin(), scanned.end(), (Id)t) - scanned.begin();
+    cell_vertices(g, c, &conn[8 * t]);
+  }
+  if (!dedup) return wall_seconds() - t0;
+  std::vector<int32_t> mask(n_points);
+#pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p < n_points; ++p) mask[p] = 0;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < 8 * T; ++i) __atomic_store_n(&mask[conn[i]], 1, __ATOMIC_RELAXED);
+  std::vector<Id> pos(n_points);
+  const int64_t U = omp_inclusive_scan(mask.data(), pos.data(), n_points);
+  std::vector<float> coords(3 * U), values(U);  // CompactPointField of the field too
+#pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p < n_points; ++p) {
+    if (!mask[p]) continue;
+    const int64_t u = pos[p] - 1;
+    const int64_t ijk[3] = {p % g.nx, (p / g.nx) % g.ny, p / (g.nx * g.ny)};
+    for (int d = 0; d < 3; ++d) coords[3 * u + d] = g.origin[d] + g.spacing[d] * (float)ijk[d];
+    values[u] = field[p];
+  }
+  std::vector<Id> used(conn);
+  omp_sort(used.begin(), used.end());
+  used.erase(std::unique(used.begin(), used.end()), used.end());
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < 8 * T; ++i)
+    conn[i] = (Id)(std::lower_bound(used.begin(), used.end(), conn[i]) - used.begin());
+  *n_out_points = U;
+  return wall_seconds() - t0;
+}
+
+}  // extern "C"

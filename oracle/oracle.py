@@ -73,6 +73,14 @@ def lib():
         L.oracle_mc_pipeline_omp.argtypes = [C.c_void_p] * 4 + [C.c_float, C.c_int, C.c_int,
                                                                  _i64p, _i64p]
         L.oracle_mc_pipeline_omp.restype = C.c_double
+        L.oracle_unstructured_pipeline_omp.argtypes = [C.c_void_p, C.c_int, C.c_int64,
+                                                       C.c_void_p, C.c_void_p, C.c_float, C.c_int,
+                                                       C.c_int, _i64p, _i64p]
+        L.oracle_unstructured_pipeline_omp.restype = C.c_double
+        L.oracle_threshold_pipeline_omp.argtypes = [C.c_void_p] * 4 + [C.c_float, C.c_float,
+                                                                        C.c_int, C.c_int, _i64p,
+                                                                        _i64p]
+        L.oracle_threshold_pipeline_omp.restype = C.c_double
         _lib = L
     return _lib
 
@@ -395,3 +403,31 @@ def pipeline_omp(field, iso, dedup=True, threads=0, origin=(0, 0, 0), spacing=(1
                                        float(iso), int(dedup), int(threads), C.byref(nt),
                                        C.byref(npt))
     return float(sec), int(nt.value), int(npt.value)
+
+
+def unstructured_pipeline_omp(cells, field, points, iso, dedup=True, threads=0):
+    """Time the OpenMP restatement of the unstructured-grid contour (cells
+    [C, 4]: marching tetrahedra, [C, 8]: marching cubes on hexahedra).
+    Returns (seconds, T, U)."""
+    cells = np.ascontiguousarray(cells, np.int32)
+    nv = cells.shape[1]
+    assert nv in (4, 8)
+    field = np.ascontiguousarray(field, np.float32).reshape(-1)
+    points = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    nt, npt = C.c_int64(), C.c_int64()
+    sec = lib().oracle_unstructured_pipeline_omp(_p(cells), nv, cells.shape[0], _p(field),
+                                                 _p(points), float(iso), int(dedup), int(threads),
+                                                 C.byref(nt), C.byref(npt))
+    return float(sec), int(nt.value), int(npt.value)
+
+
+def threshold_pipeline_omp(field, lo, hi, dedup=True, threads=0, origin=(0, 0, 0),
+                           spacing=(1, 1, 1)):
+    """Time the OpenMP restatement of threshold on a uniform grid (field of
+    shape (nz, ny, nx)).  Returns (seconds, output cells, output points)."""
+    f = _field(field)
+    nc, npt = C.c_int64(), C.c_int64()
+    sec = lib().oracle_threshold_pipeline_omp(_p(_dims(f)), _p(_f3(origin)), _p(_f3(spacing)),
+                                              _p(f), float(lo), float(hi), int(dedup),
+                                              int(threads), C.byref(nc), C.byref(npt))
+    return float(sec), int(nc.value), int(npt.value)

@@ -71,6 +71,13 @@ void oracle_mc_backward_f32(const float* field, const int64_t* dims, const float
 double oracle_mc_pipeline_omp(const float* field, const int64_t* dims, const float* origin,
                               const float* spacing, float iso, int dedup, int threads,
                               int64_t* n_tris, int64_t* n_pts);
+double oracle_unstructured_pipeline_omp(const int32_t* cells, int nv, int64_t n_cells,
+                                        const float* field, const float* points, float iso,
+                                        int dedup, int threads, int64_t* n_tris, int64_t* n_pts);
+double oracle_threshold_pipeline_omp(const int64_t* dims, const float* origin,
+                                     const float* spacing, const float* field, float lo,
+                                     float hi, int dedup, int threads, int64_t* n_out_cells,
+                                     int64_t* n_out_points);
 }
 
 namespace {
@@ -254,12 +261,19 @@ int main() {
                                (int64_t)cap_t, &T, &U, conn.data(), first.data(), second.data(),
                                w.data(), xyz.data()) == 0,
             "tetrahedra");
+      int64_t To = 0, Uo = 0;
+      oracle_unstructured_pipeline_omp(tets.data(), 4, nt, rnd.f.data(), pts_t.data(), 0.1f,
+                                       dedup, 2, &To, &Uo);
+      check(To == T && Uo == U, "OpenMP tetrahedra equal the serial ones");
       std::vector<int32_t> hconn(cap_h), hfirst(cap_h), hsecond(cap_h);
       std::vector<float> hw(cap_h), hxyz(3 * cap_h);
       check(oracle_mc_hex_generate(hexes.data(), nh, rnd.f.data(), pts_h.data(), 0.1f, dedup,
                                    (int64_t)cap_h, &T, &U, hconn.data(), hfirst.data(),
                                    hsecond.data(), hw.data(), hxyz.data()) == 0,
             "hexahedra");
+      oracle_unstructured_pipeline_omp(hexes.data(), 8, nh, rnd.f.data(), pts_h.data(), 0.1f,
+                                       dedup, 2, &To, &Uo);
+      check(To == T && Uo == U, "OpenMP hexahedra equal the serial ones");
       const Contour& ref = dedup ? rc : rp;
       Contour unit = generate(rnd, 0.1f, dedup, org0, sp1);
       check(T == ref.T && U == unit.U, "hexahedra equal the uniform grid");
@@ -287,6 +301,10 @@ int main() {
                            lconn.data(), lcoords.data(), lmask.data()) == 0,
           "threshold 26^3");
     check(oc > 0, "threshold 26^3 keeps cells");
+    int64_t oc3 = 0, op3 = 0;
+    oracle_threshold_pipeline_omp(lin.dims, org0, sp1, lin.f.data(), 70.0f, 82.0f, 1, 2, &oc3,
+                                  &op3);
+    check(oc3 == oc && op3 == op, "OpenMP threshold equals the serial one");
     // VTK writer
     check(oracle_write_vtk("/tmp/dax_oracle_sanitize.vtk", 1, rc.coords.data(), rc.U,
                            rc.conn.data(), rc.T, 3, 5) == 0,

@@ -215,3 +215,28 @@ def test_pipeline_omp_matches_serial():
     sec, T, U = orc.pipeline_omp(f, 20.0, dedup=True, threads=2)
     r = orc.generate(f, 20.0)
     assert (T, U) == (r["n_tris"], r["n_pts"]) and sec >= 0
+
+
+@pytest.mark.parametrize("dedup", [True, False])
+def test_unstructured_pipeline_omp_matches_serial(dedup):
+    """The OpenMP baselines of marching tetrahedra and of marching cubes on
+    hexahedra give the serial sequence's triangle and point counts (4
+    threads, so every parallel step splits its range)."""
+    rng = np.random.default_rng(4)
+    for builder, serial in ((orc.freudenthal_tets, orc.mt_generate),
+                            (orc.voxel_hexes, orc.mc_hex_generate)):
+        cells, pts = builder((21, 17, 13))
+        f = rng.standard_normal(pts.shape[0]).astype(np.float32)
+        sec, T, U = orc.unstructured_pipeline_omp(cells, f, pts, 0.3, dedup=dedup, threads=4)
+        r = serial(cells, f, pts, 0.3, dedup=dedup)
+        assert (T, U) == (r["n_tris"], r["n_pts"]) and T > 1000 and sec >= 0
+
+
+@pytest.mark.parametrize("dedup", [True, False])
+def test_threshold_pipeline_omp_matches_serial(dedup):
+    rng = np.random.default_rng(6)
+    f = rng.random((13, 17, 21)).astype(np.float32)
+    sec, T, U = orc.threshold_pipeline_omp(f, 0.05, 0.95, dedup=dedup, threads=4)
+    r = orc.threshold(f, 0.05, 0.95, dedup=dedup)
+    assert T == r["n_cells"] and T > 100 and sec >= 0
+    assert U == (r["n_points"] if dedup else 0)
