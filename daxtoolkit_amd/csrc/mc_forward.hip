@@ -153,6 +153,7 @@ struct GenOrder {
 
 constexpr int kTsumThreads = 1024;
 constexpr int kTsumWaves = kTsumThreads / 64;
+constexpr uint32_t kOrderStage = 16384;  // 64 KiB of LDS
 
 // One block scans the n count-block sums (v | t << 32, each below 2^32)
 // 1024 at a time: a coalesced load, wave scans of the halves, the block's
@@ -166,7 +167,10 @@ __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long
                                                              int64_t n, Ctrl* __restrict__ ctrl,
                                                              GenOrder go) {
   constexpr int kWaves = kTsumWaves;
-  __shared__ uint32_t s_v[kWaves], s_t[kWaves];
+  // the scan's wave sums, then the dispatch order's LDS window
+  __shared__ uint32_t s_stage[kOrderStage];
+  uint32_t* s_v = s_stage;
+  uint32_t* s_t = s_stage + kWaves;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const uint32_t ng = (uint32_t)ceil_div(go.nwords, (int64_t)go.tw);
   const uint32_t Kg = order_chunk(ng, kTsumThreads);
@@ -214,9 +218,11 @@ __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long
   }
 
   if (units)
-    write_order_units(gcls, Ku, nunits, go.unit_sp, go.order, (uint32_t)go.tw, s_v, s_t);
+    write_order_units(gcls, Ku, nunits, go.unit_sp, go.order, (uint32_t)go.tw, s_v, s_t, s_stage,
+                      kOrderStage);
   else if (ordered)
-    write_order<LARGE>(gcls, go.gcls, Kg, ng, go.order, (uint32_t)go.tw, s_v, s_t);
+    write_order<LARGE>(gcls, go.gcls, Kg, ng, go.order, (uint32_t)go.tw, s_v, s_t, s_stage,
+                       kOrderStage);
   else if (tid == 0)
     go.order[0] = 0u;
   // the adjoint builds the gather tiles' order itself; until then there is none

@@ -674,13 +674,51 @@ __device__ __forceinline__ void for_classes(uint64_t reg, const uint8_t* __restr
   }
 }
 
+// The next slot of class c's run (cursors s0, s1, s2), in selects and adds:
+// a conditional post-increment of one of three cursors is lowered to a
+// dynamically indexed private array, one scratch round trip per tile.
+__device__ __forceinline__ uint32_t next_slot(uint32_t c, uint32_t& s0, uint32_t& s1,
+                                              uint32_t& s2) {
+  const uint32_t slot = c == 0u ? s0 : (c == 1u ? s1 : s2);
+  s0 += c == 0u ? 1u : 0u;
+  s1 += c == 1u ? 1u : 0u;
+  s2 += c >= 2u ? 1u : 0u;
+  return slot;
+}
+
+// Stores order[1 + slot] = v for the (slot, v) pairs that emit_all(put)
+// passes to put (every slot in [0, n) once).  Scattered 4-byte stores of
+// one block all go through one CU's address unit, a cache line at a time
+// (a 16384-entry order took ~10 us); with s_stage (LDS of `stage` entries)
+// the pairs are placed in LDS one window of slots at a time, emit_all
+// running once per window, and each window leaves in coalesced stores.
+// Without s_stage the pairs go straight to global memory.
+template <class EmitAll>
+__device__ __forceinline__ void store_order(uint32_t* order, uint32_t n, uint32_t* s_stage,
+                                            uint32_t stage, EmitAll&& emit_all) {
+  if (s_stage == nullptr) {
+    emit_all([&](uint32_t slot, uint32_t v) { order[1u + slot] = v; });
+    return;
+  }
+  for (uint32_t lo = 0; lo < n; lo += stage) {
+    emit_all([&](uint32_t slot, uint32_t v) {
+      if (slot - lo < stage) s_stage[slot - lo] = v;
+    });
+    __syncthreads();
+    const uint32_t m = min(stage, n - lo);
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) order[1u + lo + i] = s_stage[i];
+    __syncthreads();
+  }
+}
+
 // order[1 ..] = tiles [0, ntl) of class 0, then 1, then 2, each by index,
 // flagged kOrderHasOutput in classes 0 and 1; order[0] = tag.  The thread
-// owns tiles [tid * K, tid * K + K).
+// owns tiles [tid * K, tid * K + K).  s_stage may alias s_a and s_b.
 template <bool LARGE>
 __device__ __forceinline__ void write_order(uint64_t reg, const uint8_t* __restrict__ cls,
                                             uint32_t K, uint32_t ntl, uint32_t* order,
-                                            uint32_t tag, uint32_t* s_a, uint32_t* s_b) {
+                                            uint32_t tag, uint32_t* s_a, uint32_t* s_b,
+                                            uint32_t* s_stage = nullptr, uint32_t stage = 0u) {
   const uint32_t t0 = (uint32_t)threadIdx.x * K;
   const uint32_t n = t0 < ntl ? min(K, ntl - t0) : 0u;
   uint32_t c0 = 0u, c1 = 0u;
@@ -690,10 +728,12 @@ __device__ __forceinline__ void write_order(uint64_t reg, const uint8_t* __restr
   });
   uint32_t e0, e1, tot0, tot1;
   block_scan2(c0, c1, e0, e1, tot0, tot1, s_a, s_b);
-  uint32_t s0 = 1u + e0, s1 = 1u + tot0 + e1, s2 = 1u + tot0 + tot1 + (t0 - e0 - e1);
-  for_classes<LARGE>(reg, cls, t0, K, n, [&](uint32_t k, uint32_t c) {
-    const uint32_t slot = c == 0u ? s0++ : (c == 1u ? s1++ : s2++);
-    order[slot] = (t0 + k) | (c < 2u ? kOrderHasOutput : 0u);
+  const uint32_t b0 = e0, b1 = tot0 + e1, b2 = tot0 + tot1 + (t0 - e0 - e1);
+  store_order(order, ntl, s_stage, stage, [&](auto&& put) {
+    uint32_t s0 = b0, s1 = b1, s2 = b2;
+    for_classes<LARGE>(reg, cls, t0, K, n, [&](uint32_t k, uint32_t c) {
+      put(next_slot(c, s0, s1, s2), (t0 + k) | (c < 2u ? kOrderHasOutput : 0u));
+    });
   });
   if (threadIdx.x == 0) order[0] = tag;
 }
@@ -731,7 +771,9 @@ __device__ __forceinline__ uint64_t load_unit_classes(const uint8_t* __restrict_
 }
 __device__ __forceinline__ void write_order_units(uint64_t reg, uint32_t K, uint32_t nunits,
                                                   uint32_t sp, uint32_t* order, uint32_t tag,
-                                                  uint32_t* s_a, uint32_t* s_b) {
+                                                  uint32_t* s_a, uint32_t* s_b,
+                                                  uint32_t* s_stage = nullptr,
+                                                  uint32_t stage = 0u) {
   constexpr uint32_t G = kOrderUnitTiles;
   const uint32_t u0 = (uint32_t)threadIdx.x * K;
   const uint32_t n = u0 < nunits ? min(K, nunits - u0) : 0u;
@@ -749,22 +791,24 @@ __device__ __forceinline__ void write_order_units(uint64_t reg, uint32_t K, uint
   }
   uint32_t e0, e1, tot0, tot1;
   block_scan2(c0, c1, e0, e1, tot0, tot1, s_a, s_b);
-  uint32_t s0 = e0, s1 = tot0 + e1, s2 = tot0 + tot1 + (u0 - e0 - e1);
-  uint32_t zg = n ? u0 / sp : 0u, r = n ? u0 - zg * sp : 0u;
-  for (uint32_t k = 0; k < n; ++k) {
-    const uint32_t c = unit_class(k);
-    const uint32_t R = c == 0u ? s0++ : (c == 1u ? s1++ : s2++);
-    const uint32_t base = 1u + 8u * G * (R >> 3) + (R & 7u);
+  const uint32_t b0 = e0, b1 = tot0 + e1, b2 = tot0 + tot1 + (u0 - e0 - e1);
+  store_order(order, nunits * G, s_stage, stage, [&](auto&& put) {
+    uint32_t s0 = b0, s1 = b1, s2 = b2;
+    uint32_t zg = n ? u0 / sp : 0u, r = n ? u0 - zg * sp : 0u;
+    for (uint32_t k = 0; k < n; ++k) {
+      const uint32_t R = next_slot(unit_class(k), s0, s1, s2);
+      const uint32_t base = 8u * G * (R >> 3) + (R & 7u);
 #pragma unroll
-    for (uint32_t m = 0; m < G; ++m) {
-      const uint32_t tc = (uint32_t)(reg >> (2u * (k * G + m))) & 3u;
-      order[base + 8u * m] = ((G * zg + m) * sp + r) | (tc < 2u ? kOrderHasOutput : 0u);
+      for (uint32_t m = 0; m < G; ++m) {
+        const uint32_t tc = (uint32_t)(reg >> (2u * (k * G + m))) & 3u;
+        put(base + 8u * m, ((G * zg + m) * sp + r) | (tc < 2u ? kOrderHasOutput : 0u));
+      }
+      if (++r == sp) {
+        r = 0u;
+        ++zg;
+      }
     }
-    if (++r == sp) {
-      r = 0u;
-      ++zg;
-    }
-  }
+  });
   if (threadIdx.x == 0) order[0] = tag;
 }
 

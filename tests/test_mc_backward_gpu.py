@@ -121,6 +121,11 @@ def test_backward_deterministic_and_fd_256():
     print(f"FD step 1e-3: max {e3.max():.3e} median {np.median(e3):.3e} "
           f"frac<=1e-3 {np.mean(e3 <= 1e-3):.3f}; converged points {conv.mean():.3f}, "
           f"max rel err there (step 1e-5) {e5[conv].max():.3e}", flush=True)
+    # This replaces the workload's FD check at step 1e-3 (its statistics are
+    # only printed above): at that step the truncation error of this noisy
+    # field is ~1e-2 relative.  The tolerance stays 1e-3, asserted at step
+    # 1e-5 where the difference quotient has converged; the every-point
+    # FP64 analytic check above is the stronger one.
     assert conv.mean() >= 0.5
     assert e5[conv].max() <= 1e-3
     # away from the surface the gradient is exactly zero
