@@ -16,7 +16,9 @@ __global__ __launch_bounds__(kVertexAdjThreads) void k_vertex_adjoint(
     const float* __restrict__ dldx, const float* __restrict__ dldn,
     const int64_t* __restrict__ counts, int64_t n_points, VertexAdj* __restrict__ out,
     uint32_t* __restrict__ list, const uint8_t* __restrict__ tcls, uint32_t* __restrict__ gorder) {
-  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-contiguous vertex ranges: vertices are in edge order, so the field
+  // lines of neighbouring stencils stay in one XCD's L2
+  const int64_t u = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (u == 0) list[0] = 0u;
   if (blockIdx.x == 0) {  // uniform
     __shared__ uint32_t s_a[kVertexAdjThreads / 64], s_b[kVertexAdjThreads / 64];
