@@ -154,10 +154,12 @@ struct GenOrder {
 constexpr int kTsumThreads = 1024;
 constexpr int kTsumWaves = kTsumThreads / 64;
 constexpr uint32_t kOrderStage = 16384;  // 64 KiB of LDS
+constexpr int kTsumPerThread = 4;        // block sums per thread and round
 
 // One block scans the n count-block sums (v | t << 32, each below 2^32)
-// 1024 at a time: a coalesced load, wave scans of the halves, the block's
-// wave sums in 64 bits, 64-bit carries between rounds.  The same block then
+// 4096 at a time: four consecutive sums per thread, wave scans of the
+// thread totals' halves, the block's wave sums in 64 bits, 64-bit carries
+// between rounds.  The same block then
 // writes the two dispatch orders.
 // LARGE: some thread's chunk exceeds kOrderRegTiles tiles (a separate
 // instance, so the common one stays small).
@@ -183,13 +185,22 @@ __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long
             : (ordered ? load_classes(go.gcls, (uint32_t)tid * Kg, Kg, ng) : 0ull);
 
   unsigned long long cv = 0, ct = 0;  // 64-bit carries
-  for (int64_t e0 = 0; e0 < n; e0 += kTsumThreads) {
-    const int64_t e = e0 + tid;
-    const unsigned long long x = e < n ? bsum[e] : 0ull;
-    const uint32_t v = (uint32_t)x, t = (uint32_t)(x >> 32);
-    // a round's sums stay below 2^32 only for the wave partials; the
-    // cross-wave totals are formed in 64 bits
-    const uint32_t iv = wave_incl_scan(v), it = wave_incl_scan(t);
+  constexpr int kPer = kTsumPerThread;
+  for (int64_t e0 = 0; e0 < n; e0 += kPer * kTsumThreads) {
+    const int64_t e = e0 + (int64_t)kPer * tid;  // the thread's kPer consecutive sums
+    uint32_t v[kPer], t[kPer], sv = 0u, st = 0u;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const unsigned long long x = e + i < n ? bsum[e + i] : 0ull;
+      v[i] = (uint32_t)x;
+      t[i] = (uint32_t)(x >> 32);
+      sv += v[i];
+      st += t[i];
+    }
+    // a round's sums stay below 2^32 only for the wave partials (a count
+    // block holds at most 3 * 2^17 vertices); the cross-wave totals are
+    // formed in 64 bits
+    const uint32_t iv = wave_incl_scan(sv), it = wave_incl_scan(st);
     if (lane == 63) {
       s_v[wv] = iv;
       s_t[wv] = it;
@@ -206,8 +217,13 @@ __global__ __launch_bounds__(kTsumThreads) void k_scan_tsums(const unsigned long
       rt += s_t[q];
     }
     __syncthreads();
-    if (e < n)
-      bpre[e] = ((cv + ov + iv - v) & 0xffffffffull) | ((ct + ot + it - t) << 32);
+    unsigned long long pv = cv + ov + iv - sv, pt = ct + ot + it - st;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      if (e + i < n) bpre[e + i] = (pv & 0xffffffffull) | (pt << 32);
+      pv += v[i];
+      pt += t[i];
+    }
     cv += rv;
     ct += rt;
   }
