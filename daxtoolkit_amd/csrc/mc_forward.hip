@@ -1234,46 +1234,45 @@ int dax_hip_mc_autotune(const float* d_field, int64_t nx, int64_t ny, int64_t nz
       best.cls_vec = t.cls_vec;
     }
   }
-  // scan variants (each timed launch follows a classify that resets its state)
+  // count/scan and generate variants together: the count block size decides
+  // whether the count kernel classifies the generate tiles, and the tile
+  // size the dispatch order the scan block builds (and its cost).  Each
+  // repetition follows a classify, as in the forward; events split the
+  // scan's time from generate's.
+  EventPair ev2;
+  const Tuning base = best;
   for (int sub : {1, 2, 4, 8}) {
-    Tuning t = best;
-    t.scan_sub = sub;
-    float total = 0.0f;
-    for (int r = 0; r < reps; ++r) {
-      if ((rc = launch_bits(g, d_field, ws, L, best, s))) return rc;
-      DAXHIP_CHECK(hipEventRecord(ev.a, s));
-      if ((rc = launch_scan(g, ws, L, t, s))) return rc;
-      DAXHIP_CHECK(hipEventRecord(ev.b, s));
-      total += ev.ms();
-    }
-    const float ms = total / reps;
-    if (ms < best_ms[1]) {
-      best_ms[1] = ms;
-      best.scan_sub = sub;
-    }
-  }
-  // generate variants over one classify; the scan again for each tile size,
-  // so that its dispatch order is the one that variant runs with
-  if ((rc = launch_bits(g, d_field, ws, L, best, s))) return rc;
-  for (int tv = 0; tv < 10; ++tv) {
-    const int tw = 64 << (tv % 5);
-    Tuning t = best;
-    t.gen_tile = tw;
-    t.gen_xcd = tv / 5;
-    if ((rc = launch_scan(g, ws, L, t, s))) return rc;
-    if ((rc = launch_generate(g, d_field, ws, L, flags, o, cap_points, cap_tris, d_counts, t, s)))
-      return rc;
-    DAXHIP_CHECK(hipEventRecord(ev.a, s));
-    for (int r = 0; r < reps; ++r)
-      if ((rc = launch_generate(g, d_field, ws, L, flags, o, cap_points, cap_tris, d_counts, t,
-                                s)))
-        return rc;
-    DAXHIP_CHECK(hipEventRecord(ev.b, s));
-    const float ms = ev.ms() / reps;
-    if (ms < best_ms[2]) {
-      best_ms[2] = ms;
-      best.gen_tile = tw;
-      best.gen_xcd = t.gen_xcd;
+    for (int tv = 0; tv < 10; ++tv) {
+      Tuning t = base;
+      t.scan_sub = sub;
+      t.gen_tile = 64 << (tv % 5);
+      t.gen_xcd = tv / 5;
+      float ts = 0.0f, tg = 0.0f;
+      for (int r = 0; r < reps; ++r) {
+        if ((rc = launch_bits(g, d_field, ws, L, base, s))) return rc;
+        DAXHIP_CHECK(hipEventRecord(ev.a, s));
+        if ((rc = launch_scan(g, ws, L, t, s))) return rc;
+        DAXHIP_CHECK(hipEventRecord(ev2.a, s));
+        if ((rc = launch_generate(g, d_field, ws, L, flags, o, cap_points, cap_tris, d_counts, t,
+                                  s)))
+          return rc;
+        DAXHIP_CHECK(hipEventRecord(ev.b, s));
+        DAXHIP_CHECK(hipEventSynchronize(ev.b));
+        float m1 = 0.0f, m2 = 0.0f;
+        DAXHIP_CHECK(hipEventElapsedTime(&m1, ev.a, ev2.a));
+        DAXHIP_CHECK(hipEventElapsedTime(&m2, ev2.a, ev.b));
+        ts += m1;
+        tg += m2;
+      }
+      ts /= reps;
+      tg /= reps;
+      if (ts + tg < best_ms[1] + best_ms[2]) {
+        best_ms[1] = ts;
+        best_ms[2] = tg;
+        best.scan_sub = sub;
+        best.gen_tile = t.gen_tile;
+        best.gen_xcd = t.gen_xcd;
+      }
     }
   }
   {

@@ -438,10 +438,13 @@ int dax_hip_mc_cell_outputs(int64_t nx, int64_t ny, int64_t nz, const void* d_wo
  * A shape of (0,0,0) sets or gets the default for shapes not tuned. */
 int dax_hip_mc_set_tuning(int64_t nx, int64_t ny, int64_t nz, const int* h_params5);
 int dax_hip_mc_get_tuning(int64_t nx, int64_t ny, int64_t nz, int* h_params5);
-/* Times every variant of each kernel with HIP events (reps launches each),
- * keeps the fastest per kernel for this shape and returns it (h_best5) with
- * the per-kernel milliseconds (h_best_ms3, may be NULL).  Synchronous.  The
- * outputs and workspace are those of dax_hip_mc_forward. */
+/* Times the classify variants with HIP events (reps launches each), then
+ * every (scan_sub, gen_tile, gen_xcd) combination as count/scan + generate
+ * (reps repetitions, each after a classify; the count block size and the
+ * tile size together decide the generate tiles' dispatch order), keeps the
+ * fastest for this shape and returns it (h_best5) with the milliseconds of
+ * classify, count/scan and generate (h_best_ms3, may be NULL).
+ * Synchronous.  The outputs and workspace are those of dax_hip_mc_forward. */
 int dax_hip_mc_autotune(const float* d_field, int64_t nx, int64_t ny, int64_t nz, float iso,
                         unsigned flags, void* d_workspace, size_t workspace_bytes,
                         float* d_vertices, float* d_normals, int32_t* d_conn, uint32_t* d_edges,
