@@ -72,15 +72,38 @@ def _kernel_variants(n, tuned):
     return cls, f"k_count<{sub}>", f"k_generate<{tile}, FieldArray>"
 
 
+def kernel_sources_sha256():
+    """SHA-256 over the library's kernel sources (daxtoolkit_amd/csrc/*.hip
+    and *.h, by name), the build a traffic summary was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in sorted((ROOT / "daxtoolkit_amd" / "csrc").glob("*")):
+        if f.suffix in (".hip", ".h"):
+            h.update(f.name.encode())
+            h.update(f.read_bytes())
+    return h.hexdigest()
+
+
+TRAFFIC_NOTE = None
+
+
 def _measured_traffic(n, tuned):
     """HBM bytes per dispatch of the forward's kernels (by role) from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the 512^3
     workload, scaled by the copy-kernel calibration; None for another
-    workload or a kernel variant the summary does not hold."""
+    workload, a kernel variant the summary does not hold, or a summary
+    measured on other kernel sources (stale)."""
+    global TRAFFIC_NOTE
     path = ROOT / "profiles" / "forward_512_traffic.json"
     if n != 512 or not path.exists():
         return None
-    d = json.loads(path.read_text())["hbm_bytes_per_dispatch"]
+    j = json.loads(path.read_text())
+    if j.get("sources_sha256") != kernel_sources_sha256():
+        TRAFFIC_NOTE = ("profiles/forward_512_traffic.json was measured on other kernel "
+                        "sources: traffic not reported")
+        return None
+    TRAFFIC_NOTE = "profiles/forward_512_traffic.json, measured on these kernel sources"
+    d = j["hbm_bytes_per_dispatch"]
     roles = ("k_classify_v4", "k_count", "k_generate")
     return {r: d.get(v) for r, v in zip(roles, _kernel_variants(n, tuned))}
 
@@ -792,6 +815,7 @@ def main():
                      "frac": round(fwd_gbs / HBM_PEAK_GBS, 4),
                      "traffic": (sum(traffic[k] for k in names)
                                  if traffic and all(traffic.get(k) for k in names) else None),
+                     "traffic_source": TRAFFIC_NOTE,
                      "kernel_variants": list(_kernel_variants(n, tuned)),
                      "dominant_kernel": dominant, "kernels": kernels},
     }
