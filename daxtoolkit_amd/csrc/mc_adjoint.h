@@ -41,23 +41,9 @@ static_assert(sizeof(VertexAdj) == 32, "32-byte vertex records");
 
 __device__ __forceinline__ void grad_at(const VolGeom& g, const float* f, int x, int y, int z,
                                         float out[3]) {
-  const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-  const int64_t p = x + sy * y + sz * z;
-  const int c[3] = {x, y, z};
-  const int n[3] = {g.nx, g.ny, g.nz};
-  const int64_t st[3] = {1, sy, sz};
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const float h = g.spacing[d];
-    if (n[d] < 2)
-      out[d] = 0.0f;
-    else if (c[d] == 0)
-      out[d] = (f[p + st[d]] - f[p]) / h;
-    else if (c[d] == n[d] - 1)
-      out[d] = (f[p] - f[p - st[d]]) / h;
-    else
-      out[d] = (f[p + st[d]] - f[p - st[d]]) / (2.0f * h);
-  }
+  // branch-free loads (point_gradient): the six values of a gradient, and
+  // both gradients of a vertex, are in flight together
+  point_gradient(g, FieldArray{f}, x, y, z, out);
 }
 
 // Adjoint record of vertex u (global index into edges/dldx/dldn/out) of the
@@ -84,6 +70,9 @@ __device__ __forceinline__ void vertex_adjoint(const VolGeom& g, const float* __
     return;
   }
   const int64_t B = A + (axis == 0 ? 1 : (axis == 1 ? sy : sz));
+  // the upstream gradients do not depend on the field: load them first
+  const float dn[3] = {dldn[3 * u + 0], dldn[3 * u + 1], dldn[3 * u + 2]};
+  const float dx[3] = {dldx[3 * u + 0], dldx[3 * u + 1], dldx[3 * u + 2]};
   const float fA = f[A], fB = f[B];
   const float den = fB - fA;
   const float t = (g.iso - fA) / den;
@@ -94,7 +83,6 @@ __device__ __forceinline__ void vertex_adjoint(const VolGeom& g, const float* __
 #pragma unroll
   for (int d = 0; d < 3; ++d) m[d] = gA[d] + t * (gB[d] - gA[d]);
   const float len2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
-  float dn[3] = {dldn[3 * u + 0], dldn[3 * u + 1], dldn[3 * u + 2]};
   float dm[3] = {0.0f, 0.0f, 0.0f};
   if (len2 > 0.0f) {
     const float len = sqrtf(len2);
@@ -111,7 +99,7 @@ __device__ __forceinline__ void vertex_adjoint(const VolGeom& g, const float* __
   for (int d = 0; d < 3; ++d) {
     const float a = g.origin[d] + g.spacing[d] * (float)ca[d];
     const float b = g.origin[d] + g.spacing[d] * (float)(ca[d] + (d == axis ? 1 : 0));
-    dldt += dldx[3 * u + d] * (b - a);
+    dldt += dx[d] * (b - a);
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) dldt += dm[d] * (gB[d] - gA[d]);

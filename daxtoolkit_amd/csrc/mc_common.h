@@ -65,7 +65,6 @@ struct VolGeom {
   float rcp_h[3];
 };
 
-// Host: fill the division constants after nx, ny, wpr are set.
 // Host: fill the division constants after nx, ny, wpr and spacing are set.
 inline void set_geom_divs(VolGeom& g) {
   g.div_wpr = make_fast_div((uint32_t)g.wpr);

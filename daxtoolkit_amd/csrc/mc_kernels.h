@@ -607,23 +607,35 @@ __device__ __forceinline__ uint32_t order_chunk(uint32_t ntl, uint32_t threads) 
   const uint32_t k = (ntl + threads - 1) / threads;
   return k <= 8u ? k : (k <= (uint32_t)kOrderRegTiles ? (k + 3u) / 4u * 4u : (k + 15u) / 16u * 16u);
 }
+// The loads are unconditional (an address outside the thread's tiles is
+// replaced by the first byte, its value dropped), so they issue together
+// instead of one conditional load and wait at a time.
 __device__ __forceinline__ uint64_t load_classes(const uint8_t* __restrict__ cls, uint32_t t0,
                                                  uint32_t K, uint32_t ntl) {
   uint64_t r = 0ull;
   if (K <= 8u) {
+    uint32_t b[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if ((uint32_t)k < K && t0 + k < ntl) r |= (uint64_t)(cls[t0 + k] & 3u) << (2 * k);
+    for (int k = 0; k < 8; ++k) {
+      const bool ok = (uint32_t)k < K && t0 + k < ntl;
+      b[k] = cls[ok ? t0 + k : 0u];
+      b[k] = ok ? b[k] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r |= (uint64_t)(b[k] & 3u) << (2 * k);
   } else if (K <= (uint32_t)kOrderRegTiles) {
+    uint32_t w[kOrderRegTiles / 4];
 #pragma unroll
     for (int v = 0; v < kOrderRegTiles / 4; ++v) {
       const uint32_t t = t0 + 4u * v;
-      if (4u * v < K && t < ntl) {
-        const uint32_t w = *reinterpret_cast<const uint32_t*>(cls + t);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) r |= (uint64_t)((w >> (8 * j)) & 3u) << (2 * (4 * v + j));
-      }
+      const bool ok = 4u * v < K && t < ntl;  // t0 and K multiples of 4: aligned
+      w[v] = *reinterpret_cast<const uint32_t*>(cls + (ok ? t : 0u));
+      w[v] = ok ? w[v] : 0u;
     }
+#pragma unroll
+    for (int v = 0; v < kOrderRegTiles / 4; ++v)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) r |= (uint64_t)((w[v] >> (8 * j)) & 3u) << (2 * (4 * v + j));
   }
   return r;
 }
@@ -753,20 +765,24 @@ __device__ __forceinline__ uint64_t load_unit_classes(const uint8_t* __restrict_
   constexpr uint32_t G = kOrderUnitTiles;
   const uint32_t u0 = (uint32_t)threadIdx.x * K;
   const uint32_t n = u0 < nunits ? min(K, nunits - u0) : 0u;
-  uint64_t reg = 0ull;  // class of tile m of unit k at bits 2 (k G + m)
   uint32_t zg = n ? u0 / sp : 0u, r = n ? u0 - zg * sp : 0u;
+  uint32_t c[kOrderRegTiles];  // unconditional loads, as in load_classes
 #pragma unroll
   for (int k = 0; k < kOrderRegTiles / (int)G; ++k) {
-    if ((uint32_t)k < n) {
+    const bool ok = (uint32_t)k < n;
 #pragma unroll
-      for (uint32_t m = 0; m < G; ++m)
-        reg |= (uint64_t)(cls[(G * zg + m) * sp + r] & 3u) << (2u * (k * G + m));
-      if (++r == sp) {
-        r = 0u;
-        ++zg;
-      }
+    for (uint32_t m = 0; m < G; ++m) {
+      c[k * G + m] = cls[ok ? (G * zg + m) * sp + r : 0u];
+      c[k * G + m] = ok ? c[k * G + m] : 0u;
+    }
+    if (++r == sp) {
+      r = 0u;
+      ++zg;
     }
   }
+  uint64_t reg = 0ull;  // class of tile m of unit k at bits 2 (k G + m)
+#pragma unroll
+  for (int k = 0; k < kOrderRegTiles; ++k) reg |= (uint64_t)(c[k] & 3u) << (2 * k);
   return reg;
 }
 __device__ __forceinline__ void write_order_units(uint64_t reg, uint32_t K, uint32_t nunits,
