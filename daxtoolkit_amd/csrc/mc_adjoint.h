@@ -209,7 +209,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // (S_p, dL/dg_p) at p = (x,j,k) + (dx,dy,dz): over the 3 edges leaving p
 // (lower halves of their records) and the 3 edges entering p (upper halves),
-// summed in that order.  The six record loads are issued together.
+// summed in that order.
 __device__ __forceinline__ f32x4 point_sums(const GatherCtx& c, int dx, int dy, int dz) {
   int64_t vid[6];
 #pragma unroll
@@ -219,9 +219,16 @@ __device__ __forceinline__ f32x4 point_sums(const GatherCtx& c, int dx, int dy, 
   }
   const f32x4* rec = reinterpret_cast<const f32x4*>(c.adj);
   f32x4 r[6];
+  // unconditional loads (an inactive edge reads record 0 and is dropped
+  // below), so the six are in flight together instead of one conditional
+  // load and wait at a time; every record index is below n_adj
+  if (c.n_adj > 0) {
 #pragma unroll
-  for (int i = 0; i < 6; ++i)
-    r[i] = vid[i] >= 0 ? rec[2 * vid[i] + (i < 3 ? 0 : 1)] : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < 6; ++i) r[i] = rec[vid[i] >= 0 ? 2 * vid[i] + (i < 3 ? 0 : 1) : 0];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) r[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  }
   f32x4 acc{0.0f, 0.0f, 0.0f, 0.0f};  // S, G[0..2]
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
