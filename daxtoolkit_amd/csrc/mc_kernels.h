@@ -922,6 +922,21 @@ __device__ __forceinline__ unsigned long long word_bases(const CountSrc& cs, int
   return add_vt(p, (unsigned long long)(l & 0xffffu) | ((unsigned long long)(l >> 16) << 32));
 }
 
+// The same with every load issued (clamped indices, results selected), so
+// that the loads of several words can be in flight together.
+__device__ __forceinline__ unsigned long long word_bases_nb(const CountSrc& cs, int64_t w) {
+  const int64_t ct = w / kCountTile;
+  const bool past = ct >= cs.n_ctiles;
+  const int64_t bi = past ? (cs.n_ctiles + (1 << cs.tpb_shift) - 1) >> cs.tpb_shift
+                          : ct >> cs.tpb_shift;
+  const unsigned long long bp = cs.bpre[bi];
+  const unsigned long long tl = cs.tloc[past ? 0 : ct];
+  const uint32_t l = cs.lpre[past ? 0 : w];
+  const unsigned long long full =
+      add_vt(add_vt(bp, tl), (unsigned long long)(l & 0xffffu) | ((unsigned long long)(l >> 16) << 32));
+  return past ? bp : full;
+}
+
 // One generate tile: words [w_begin, w_begin + TW) of a volume.
 //   1. stage bits (planes k, k+1, k+2), vertex bases (k, k+1), triangle bases
 //   2. per word: active-cell mask; block scan of the active-cell counts
@@ -990,26 +1005,68 @@ __device__ __forceinline__ void generate_tile(const VolGeom& g, const Src& f,
   uint32_t* s_rinfo = s_rfirst + kMaxGenThreads + 1;  // [nt]: li << 16 | bit << 8 | case
   uint8_t* s_tcell = reinterpret_cast<uint8_t*>(s_rinfo + kMaxGenThreads);  // [kRoundTris]
 
+  // staging: each thread issues the loads of up to kStageU of its words
+  // (clamped indices, every load unconditional) before storing any of them,
+  // so they are in flight together rather than one conditional load and
+  // wait at a time
+  constexpr int kStageU = 4;
   const int64_t plane = (int64_t)g.wpr * g.ny;
-  for (int i = tid; i < L.segb; i += nt) {
-    const int64_t w = w_begin + i;
-    s_b0[i] = (w < g.nwords) ? b[w] : 0ull;
-    s_b1[i] = (w + plane < g.nwords) ? b[w + plane] : 0ull;
-    s_b2[i] = (w + 2 * plane < g.nwords) ? b[w + 2 * plane] : 0ull;
+  for (int i0 = tid; i0 < L.segb; i0 += kStageU * nt) {
+    uint64_t v[kStageU][3];
+#pragma unroll
+    for (int u = 0; u < kStageU; ++u) {
+      const int64_t w = w_begin + i0 + u * nt;
+#pragma unroll
+      for (int z = 0; z < 3; ++z) v[u][z] = b[w + z * plane < g.nwords ? w + z * plane : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < kStageU; ++u) {
+      const int i = i0 + u * nt;
+      if (i < L.segb) {
+        const int64_t w = w_begin + i;
+        s_b0[i] = (w < g.nwords) ? v[u][0] : 0ull;
+        s_b1[i] = (w + plane < g.nwords) ? v[u][1] : 0ull;
+        s_b2[i] = (w + 2 * plane < g.nwords) ? v[u][2] : 0ull;
+      }
+    }
   }
   if constexpr (FROM_COUNTS) {
-    for (int i = tid; i < L.segv; i += nt) {
-      const unsigned long long p0 = word_bases(cs, w_begin + i);
-      const unsigned long long p1 = word_bases(cs, w_begin + plane + i);
-      s_v0[i] = (uint32_t)p0;
-      s_v1[i] = (uint32_t)p1;
-      if (i <= nw) s_t[i] = (uint32_t)(p0 >> 32);
+    for (int i0 = tid; i0 < L.segv; i0 += kStageU * nt) {
+      unsigned long long p0[kStageU], p1[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const int i = i0 + u * nt;
+        p0[u] = word_bases_nb(cs, w_begin + i);
+        p1[u] = word_bases_nb(cs, w_begin + plane + i);
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const int i = i0 + u * nt;
+        if (i < L.segv) {
+          s_v0[i] = (uint32_t)p0[u];
+          s_v1[i] = (uint32_t)p1[u];
+          if (i <= nw) s_t[i] = (uint32_t)(p0[u] >> 32);
+        }
+      }
     }
   } else {
-    for (int i = tid; i < L.segv; i += nt) {
-      const int64_t w = w_begin + i;
-      s_v0[i] = (w <= g.nwords) ? bv[w] : 0u;
-      s_v1[i] = (w + plane <= g.nwords) ? bv[w + plane] : 0u;
+    for (int i0 = tid; i0 < L.segv; i0 += kStageU * nt) {
+      uint32_t a0[kStageU], a1[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const int64_t w = w_begin + i0 + u * nt;
+        a0[u] = bv[w <= g.nwords ? w : 0];
+        a1[u] = bv[w + plane <= g.nwords ? w + plane : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const int i = i0 + u * nt;
+        if (i < L.segv) {
+          const int64_t w = w_begin + i;
+          s_v0[i] = (w <= g.nwords) ? a0[u] : 0u;
+          s_v1[i] = (w + plane <= g.nwords) ? a1[u] : 0u;
+        }
+      }
     }
     for (int i = tid; i <= nw; i += nt) s_t[i] = bt[w_begin + i];
   }
