@@ -120,7 +120,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_count(VolGeom g, const uint64_t*
       const int64_t c0 = blk * kBlockTiles + l * gct;
       if (c0 < n_ctiles) {
         const uint32_t w = weight(l * gct, gct);
-        tc.tcls[c0 / gct] = w == 0u ? 2 : (4u * w >= 5u * (uint32_t)kGatherTileWords ? 0 : 1);
+        tc.tcls[c0 / gct] = w == 0u ? 2 : (4u * w >= 8u * (uint32_t)kGatherTileWords ? 0 : 1);
       }
     }
     const int cpt = tc.tw / kCountTile;
