@@ -167,6 +167,7 @@ _SIGNATURES = {
     "dax_hip_graph_launch": (C.c_int, [vp, vp]),
     "dax_hip_graph_destroy": (C.c_int, [vp]),
     "dax_hip_mc_backward_scratch_bytes": (sz, [i64]),
+    "dax_hip_mc_backward_set_span_threshold": (C.c_int, [i64]),
     "dax_hip_mc_backward": (C.c_int, [vp, i64, i64, i64, vp, vp, f32, vp, sz, vp, vp, i64,
                                       vp, vp, vp, sz, vp, vp]),
     "dax_hip_mc_interpolate_vec3": (C.c_int, [vp, vp, vp, vp, i64, vp, vp]),

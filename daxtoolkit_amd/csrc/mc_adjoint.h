@@ -373,7 +373,12 @@ __device__ __forceinline__ uint32_t gather_span(const VolGeom& g, const uint64_t
                                                 const VertexAdj* __restrict__ adj, int64_t n_adj,
                                                 float* __restrict__ out, int64_t w0,
                                                 int64_t w_end, GatherShared& sh, void* lds) {
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // tid is opaque to the compiler, so what is derived from it is not hoisted
+  // out of a caller's loop over spans (it would stay live across the loop,
+  // about 30 more VGPRs)
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int lane = tid & 63, wv = tid >> 6;
   const int nw = (int)(w_end - w0);
   const int wpr = g.wpr;
   const int seg = nw + 4 * wpr + 2;  // staged words per plane
@@ -591,33 +596,23 @@ __device__ __forceinline__ uint32_t gather_span(const VolGeom& g, const uint64_t
   return 0;
 }
 
-// A gather tile: zeros only when no active edge lies among its staged words,
-// else the tile as one span.  Returns its endpoint count when that exceeds
-// kMaxEndpoints (nothing written; the tile goes to the span pass).
-__device__ __forceinline__ uint32_t gather_tile(const VolGeom& g, const uint64_t* __restrict__ b,
-                                                const uint32_t* __restrict__ base_v,
-                                                const VertexAdj* __restrict__ adj, int64_t n_adj,
-                                                float* __restrict__ out, int64_t w0,
-                                                int64_t w_end, GatherShared& sh, void* lds) {
-  const int tid = threadIdx.x;
-  if (!__syncthreads_or(tid < 4 && gather_segment_active(g, base_v, w0, tid))) {
-    zero_run(g, out, w0, w_end, tid, kGatherThreads);
-    return 0;
-  }
-  return gather_span<false>(g, b, base_v, adj, n_adj, out, w0, w_end, sh, lds);
-}
-
-// Tiles whose endpoints exceed kMaxEndpoints are redone in spans: the tile
-// pass appends one item per span, 2..16 spans per tile (the power of two
-// nearest above n_end / 1536), and a span pass runs the items.  An item is
-// (tile << 8) | (log2 spans << 4) | span.  Every listed tile holds more than
-// kMaxEndpoints endpoint slots; a point is a slot of at most 5 tiles (its
-// own, the tiles of the words one row before and after it and one plane
-// below and above) and a volume with U vertices has at most 2U endpoints,
-// so the slots of listed tiles sum to at most 10U, fewer than 10U / 2048
-// tiles are listed, and their items number at most
-// 2 * (10U / 1536 + tiles) < U / 32 + 16.
+// Tiles whose endpoints exceed kMaxEndpoints are redone in spans of
+// kGatherTile >> lg words, lg in 1..4 the smallest with at most
+// kSpanEndpoints << lg endpoints (2..16 spans).  A list item is
+// (tile << 8) | (lg << 4) | span.  With many tiles per CU, a tile with
+// lg <= kTileSpanLg runs its spans itself, one after another, and lists
+// only a span that still does not fit: its block then takes a few tiles'
+// time among many rounds of tiles, and the span pass (a launch later, its
+// spans staged again) is spared.  Otherwise, and for denser tiles, the tile
+// lists all its spans, and a span pass runs the items.  The listed word ranges are disjoint and a
+// point is an endpoint slot of at most 5 of them (its own, the ranges of
+// the words one row before and after it and one plane below and above),
+// so with U vertices (at most 2U endpoints) the slots of listed ranges sum
+// to at most 10U.  A listed tile with n slots gives 2^lg < 2n / 1536 items
+// and a listed span with n > 2048 slots one, so the items number fewer
+// than 20U / 1536 < U / 32 + 16.
 constexpr int kSpanEndpoints = 1536;
+constexpr int kTileSpanLg = 2;
 __host__ __device__ constexpr int64_t gather_list_capacity(int64_t n_points) {
   return n_points / 32 + 16;
 }
@@ -629,16 +624,59 @@ __host__ __device__ constexpr size_t adjoint_scratch_bytes(int64_t n_points) {
   return gather_list_offset(n_points) + (size_t)(1 + gather_list_capacity(n_points)) * 4;
 }
 
-// Appends the span items of a tile with n_end endpoints (one thread of the
-// block).
-__device__ __forceinline__ void gather_list_push(uint32_t* __restrict__ list, int64_t cap,
-                                                 uint32_t tile, uint32_t n_end) {
+__device__ __forceinline__ int gather_span_lg(uint32_t n_end) {
   int lg = 1;
   while (lg < 4 && (uint32_t)(kSpanEndpoints << lg) < n_end) ++lg;
-  const uint32_t n = 1u << lg;
+  return lg;
+}
+
+// Appends items (tile, lg, s0 .. s0 + n - 1) (one thread of the block).
+__device__ __forceinline__ void gather_list_push(uint32_t* __restrict__ list, int64_t cap,
+                                                 uint32_t tile, int lg, uint32_t s0, uint32_t n) {
   const uint32_t i = atomicAdd(&list[0], n);
   for (uint32_t s = 0; s < n; ++s)
-    if (i + s < cap) list[1 + i + s] = (tile << 8) | ((uint32_t)lg << 4) | s;
+    if (i + s < cap) list[1 + i + s] = (tile << 8) | ((uint32_t)lg << 4) | (s0 + s);
+}
+
+// A gather tile [w0, w_end) with list id `tile`: zeros only when no active
+// edge lies among its staged words, else the tile as one span, or (too many
+// endpoints) as the spans above, run here when lg <= max_lg (0 or
+// kTileSpanLg).  Without a list (no vertex records) a tile with active edges
+// is zeroed.
+__device__ __forceinline__ void gather_tile(const VolGeom& g, const uint64_t* __restrict__ b,
+                                            const uint32_t* __restrict__ base_v,
+                                            const VertexAdj* __restrict__ adj, int64_t n_adj,
+                                            float* __restrict__ out, int64_t w0, int64_t w_end,
+                                            GatherShared& sh, void* lds,
+                                            uint32_t* __restrict__ list, int64_t cap,
+                                            uint32_t tile, int max_lg) {
+  const int tid = threadIdx.x;
+  if (!__syncthreads_or(tid < 4 && gather_segment_active(g, base_v, w0, tid))) {
+    zero_run(g, out, w0, w_end, tid, kGatherThreads);
+    return;
+  }
+  // block-uniform counts are read as scalars, so the span bounds stay scalars
+  const uint32_t n_end = __builtin_amdgcn_readfirstlane(
+      gather_span<false>(g, b, base_v, adj, n_adj, out, w0, w_end, sh, lds));
+  if (!n_end) return;
+  if (!list) {
+    zero_run(g, out, w0, w_end, tid, kGatherThreads);
+    return;
+  }
+  const int lg = gather_span_lg(n_end);
+  if (lg > max_lg) {
+    if (tid == 0) gather_list_push(list, cap, tile, lg, 0u, 1u << lg);
+    return;
+  }
+  const int len = kGatherTile >> lg;
+  for (int s = 0; s < (1 << lg); ++s) {
+    const int64_t a = w0 + (int64_t)s * len, e = min(a + len, w_end);
+    if (a >= e) break;
+    __syncthreads();  // LDS reused by this span
+    const uint32_t n_span = __builtin_amdgcn_readfirstlane(
+        gather_span<false>(g, b, base_v, adj, n_adj, out, a, e, sh, lds));
+    if (n_span && tid == 0) gather_list_push(list, cap, tile, lg, (uint32_t)s, 1u);
+  }
 }
 
 // The span pass splits every item into `parts` equal pieces, the largest

@@ -3,6 +3,8 @@
 // device code are in mc_adjoint.h.
 #include "mc_adjoint.h"
 
+#include <atomic>
+
 namespace daxhip {
 namespace mc {
 
@@ -35,12 +37,14 @@ __global__ __launch_bounds__(kVertexAdjThreads) void k_vertex_adjoint(
 // One tile of kGatherTile row words per block, in the dispatch order the
 // vertex-record launch wrote (tiles with active edges of their own first, so
 // the long tiles start early and the quiet ones fill the tail), else (no
-// vertices: the forward left the order cleared) in tile order.  A tile whose endpoint records do not fit goes to the
-// list (without records, list == nullptr, every point receives 0).
+// vertices: the forward left the order cleared) in tile order.  A tile whose
+// endpoint records do not fit runs in spans or goes to the list (without
+// records, list == nullptr, every point receives 0).
 __global__ __launch_bounds__(kGatherThreads) void k_point_gather(
     VolGeom g, const uint64_t* __restrict__ bits, const uint32_t* __restrict__ base_v,
     const VertexAdj* __restrict__ adj, const int64_t* __restrict__ counts, int64_t n_points,
-    float* __restrict__ dldf, uint32_t* __restrict__ list, const uint32_t* __restrict__ order) {
+    float* __restrict__ dldf, uint32_t* __restrict__ list, const uint32_t* __restrict__ order,
+    int span_lg) {
   __shared__ GatherShared sh;
   extern __shared__ __align__(16) uint64_t s_dyn[];
   const uint32_t tag = order[0], ord = order[1 + blockIdx.x] & 0x7fffffffu;
@@ -49,14 +53,8 @@ __global__ __launch_bounds__(kGatherThreads) void k_point_gather(
                                                                             : (int64_t)blockIdx.x;
   const int64_t n_adj = counts ? min(counts[1], n_points) : n_points;
   const int64_t w0 = tile * kGatherTile, w_end = min(w0 + kGatherTile, g.nwords);
-  const uint32_t n_end = gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj,
-                                     dldf + g.field_off, w0, w_end, sh, s_dyn);
-  if (n_end) {
-    if (!list)
-      zero_run(g, dldf + g.field_off, w0, w_end, threadIdx.x, kGatherThreads);
-    else if (threadIdx.x == 0)
-      gather_list_push(list, gather_list_capacity(n_points), (uint32_t)tile, n_end);
-  }
+  gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj, dldf + g.field_off, w0,
+              w_end, sh, s_dyn, list, gather_list_capacity(n_points), (uint32_t)tile, span_lg);
 }
 
 // The listed span items (persistent blocks).
@@ -89,19 +87,41 @@ using namespace daxhip::mc;
 
 namespace {
 
-// Persistent blocks of the span pass: two per CU.
-int gather_span_blocks() {
+int device_cus() {
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       cus <= 0)
     cus = 256;
-  return 2 * cus;
+  return cus;
+}
+
+
+// Dense tiles run their spans in their own block when the gather holds at
+// least this many tiles per CU (by default 64: 1024^3 1.55 + 0.47 ms ->
+// 1.92 + 0.01 ms); with fewer rounds of tiles such a block is the critical
+// path (256^3: 49 + 29 us -> 88 + 15 us).
+std::atomic<int64_t> g_span_tiles_per_cu{64};
+
+int gather_tile_span_lg(int64_t n_tiles, int cus) {
+  return n_tiles >= g_span_tiles_per_cu.load(std::memory_order_relaxed) * (int64_t)cus
+             ? kTileSpanLg
+             : 0;
 }
 
 }  // namespace
 
 extern "C" {
+
+int dax_hip_mc_backward_set_span_threshold(int64_t tiles_per_cu) {
+  if (tiles_per_cu < 0) {
+    set_error("dax_hip_mc_backward_set_span_threshold: negative threshold %lld",
+              (long long)tiles_per_cu);
+    return DAX_HIP_BAD_VALUE;
+  }
+  g_span_tiles_per_cu.store(tiles_per_cu, std::memory_order_relaxed);
+  return DAX_HIP_OK;
+}
 
 size_t dax_hip_mc_backward_scratch_bytes(int64_t n_points) {
   return n_points > 0 ? adjoint_scratch_bytes(n_points) : 0;
@@ -151,6 +171,7 @@ int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz
   const uint32_t* bv = (const uint32_t*)(ws + L.off_base_v);
   uint32_t* list =
       n_points > 0 ? (uint32_t*)((char*)d_scratch + gather_list_offset(n_points)) : nullptr;
+  const int cus = device_cus();
   if (n_points > 0) {
     k_vertex_adjoint<<<dim3((unsigned)ceil_div(n_points, kVertexAdjThreads)),
                        dim3(kVertexAdjThreads), 0, s>>>(
@@ -162,10 +183,12 @@ int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz
   k_point_gather<<<dim3((unsigned)ceil_div(g.nwords, kGatherTile)), dim3(kGatherThreads),
                    gather_lds_bytes(g.wpr), s>>>(g, bits, bv, (const VertexAdj*)d_scratch,
                                                  d_counts, n_points, d_dl_dfield, list,
-                                                 (const uint32_t*)(ws + L.off_gorder));
+                                                 (const uint32_t*)(ws + L.off_gorder),
+                                                 gather_tile_span_lg(ceil_div(g.nwords, kGatherTile), cus));
   DAXHIP_LAUNCH_CHECK("k_point_gather");
   if (n_points > 0) {
-    k_point_gather_spans<<<dim3((unsigned)gather_span_blocks()), dim3(kGatherThreads),
+    // persistent blocks: two per CU
+    k_point_gather_spans<<<dim3((unsigned)(2 * cus)), dim3(kGatherThreads),
                            gather_lds_bytes(g.wpr), s>>>(g, bits, bv, (const VertexAdj*)d_scratch,
                                                          d_counts, n_points, d_dl_dfield, list);
     DAXHIP_LAUNCH_CHECK("k_point_gather_spans");

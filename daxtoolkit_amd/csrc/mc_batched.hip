@@ -228,14 +228,8 @@ __global__ __launch_bounds__(kGatherThreads) void k_batch_point_gather(
   const TileRef tr = tiles[blockIdx.x];
   const VolGeom g = vols[tr.vol];
   const int64_t w_end = min((int64_t)tr.w0 + kBatchGenTile, g.nwords);
-  const uint32_t n_end = gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj,
-                                     dldf + g.field_off, tr.w0, w_end, sh, s_dyn);
-  if (n_end) {
-    if (!list)
-      zero_run(g, dldf + g.field_off, tr.w0, w_end, threadIdx.x, kGatherThreads);
-    else if (threadIdx.x == 0)
-      gather_list_push(list, gather_list_capacity(n_points), blockIdx.x, n_end);
-  }
+  gather_tile(g, bits + g.word_off, base_v + g.word_off, adj, n_adj, dldf + g.field_off, tr.w0,
+              w_end, sh, s_dyn, list, gather_list_capacity(n_points), blockIdx.x, 0);
 }
 
 __global__ __launch_bounds__(kGatherThreads) void k_batch_point_gather_spans(

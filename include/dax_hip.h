@@ -574,6 +574,14 @@ int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz
                         const float* d_dl_dnormals, void* d_scratch, size_t scratch_bytes,
                         float* d_dl_dfield, void* stream);
 
+/* No reference counterpart.  Gather tiles too dense for their block's
+ * endpoint records are redone in spans: by their own block, one span after
+ * another, when the volume has at least `tiles_per_cu` gather tiles per CU
+ * (many rounds of tiles hide such a block), else in a separate span pass
+ * over many blocks.  Default 64; 0 keeps the spans in their tile's block at
+ * any size.  Process-wide; the result does not depend on it (bit for bit). */
+int dax_hip_mc_backward_set_span_threshold(int64_t tiles_per_cu);
+
 /* The same for a Vector3 point field ([n][3] f32, e.g. coordinates). */
 int dax_hip_mc_interpolate_vec3(const float* d_in, const int32_t* d_first,
                                 const int32_t* d_second, const float* d_weight, int64_t n,

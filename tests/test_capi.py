@@ -44,3 +44,11 @@ def test_workspace_size_is_host_only():
     b = lib.dax_hip_mc_workspace_bytes(512, 512, 512)
     # bit words (P/8) + two 32-bit bases per word + tile states
     assert 512**3 // 8 <= b < 512**3 // 8 * 3
+
+
+def test_span_threshold_setter_is_host_only():
+    lib = daxtoolkit_amd.load_library()
+    assert lib.dax_hip_mc_backward_set_span_threshold(-1) != 0
+    assert b"negative" in lib.dax_hip_last_error()
+    assert lib.dax_hip_mc_backward_set_span_threshold(0) == 0
+    assert lib.dax_hip_mc_backward_set_span_threshold(64) == 0

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 import torch
 
-from daxtoolkit_amd import fields, mc
+from daxtoolkit_amd import _lib, fields, mc
 from mc_autograd import MarchingCubesFunction, torch_reference_loss
 from oracle import oracle as orc
 
@@ -174,6 +174,85 @@ def test_backward_dense_surface():
         got = g1.cpu().numpy().reshape(-1).astype(np.float64)
         assert np.array_equal(got == 0.0, ref == 0.0) or np.abs(got[ref == 0.0]).max() == 0.0
         assert _close(got, ref, 1e-4, 1e-3) <= 1e-4
+
+
+def _gather_tile_endpoints(f, iso, rows):
+    """Endpoint slots of a gather tile of `rows` full rows (one plane) and of
+    its 2^lg spans: the endpoints (points with an active incident edge) of
+    its rows, of the row before and after it, and of its rows in the planes
+    below and above."""
+    nz, ny, _ = f.shape
+    b = f > iso
+    ends = np.zeros_like(b)
+    for ax in range(3):
+        a = np.moveaxis(b, ax, 0)
+        e = np.moveaxis(ends, ax, 0)
+        d = a[1:] != a[:-1]
+        e[1:] |= d
+        e[:-1] |= d
+    rc = ends.sum(axis=2)
+
+    def slots(k, j0, j1):
+        s = rc[k, j0:j1].sum() + (rc[k, j0 - 1] if j0 > 0 else 0) + (rc[k, j1] if j1 < ny else 0)
+        return int(s + (rc[k - 1, j0:j1].sum() if k > 0 else 0) +
+                   (rc[k + 1, j0:j1].sum() if k + 1 < nz else 0))
+
+    tiles = []
+    for k in range(nz):
+        for j0 in range(0, ny, rows):
+            n = slots(k, j0, j0 + rows)
+            lg = 1
+            while lg < 4 and (1536 << lg) < n:
+                lg += 1
+            sub = rows >> lg
+            tiles.append((n, lg, [slots(k, j0 + s * sub, j0 + (s + 1) * sub) for s in range(1 << lg)]))
+    return tiles
+
+
+@pytest.mark.parametrize("tiles_per_cu", [64, 0])
+def test_backward_dense_tiles_in_spans(tiles_per_cu):
+    """Gather tiles (256 row words: here one 64 x 256 plane) with more
+    endpoint slots than their 2048 records.  With few tiles per CU every
+    such tile is listed for the span pass; with threshold 0 a tile of up to
+    6144 slots runs as 2 or 4 spans in its own block, listing only a span
+    that still does not fit, and a denser tile is listed whole.  The field
+    (zero, with a noisy strip, a noisy half plane and a box face) holds
+    every kind; the adjoint is checked bit for bit against the oracle and
+    for determinism."""
+    nx, ny, nz = 256, 64, 12
+    rng = np.random.default_rng(3)
+    f = np.zeros((nz, ny, nx), np.float32)
+    f[3, 20:30, :] = rng.integers(0, 2, (10, nx))
+    f[5, :, :128] = rng.integers(0, 2, (ny, 128))
+    f[8, :, :20] = 1.0
+    iso = 0.5
+    tiles = _gather_tile_endpoints(f, iso, 64)
+    in_block = [t for t in tiles if 2048 < t[0] <= 6144]
+    assert any(t[1] == 1 and max(t[2]) <= 2048 for t in in_block)
+    assert any(t[1] == 2 and max(t[2]) <= 2048 for t in in_block)
+    assert any(max(t[2]) > 2048 for t in in_block)  # a span listed from its block
+    assert any(t[0] > 6144 for t in tiles)          # tiles listed whole
+    print(f"[case] adjoint dense tiles in spans {(nx, ny, nz)} threshold {tiles_per_cu}",
+          flush=True)
+    field = torch.from_numpy(f).cuda()
+    out = mc.forward(field, iso, edges=True)
+    U = out.num_points
+    gen = torch.Generator().manual_seed(11)
+    dx = torch.randn((U, 3), generator=gen, dtype=torch.float32)
+    dn = torch.randn((U, 3), generator=gen, dtype=torch.float32)
+    lib = _lib.load()
+    assert lib.dax_hip_mc_backward_set_span_threshold(tiles_per_cu) == 0
+    try:
+        g1 = mc.backward(field, iso, out, dx.cuda(), dn.cuda())
+        g2 = mc.backward(field, iso, out, dx.cuda(), dn.cuda())
+        torch.cuda.synchronize()
+    finally:
+        assert lib.dax_hip_mc_backward_set_span_threshold(64) == 0
+    assert torch.equal(g1, g2)
+    r = orc.generate(f, iso)
+    assert r["n_pts"] == U
+    assert _bit_equal(g1.cpu().numpy().reshape(-1),
+                      orc.backward_f32(f, iso, r["first"], r["second"], dx.numpy(), dn.numpy()))
 
 
 @pytest.mark.parametrize("dims", [(4000, 3, 4), (15872, 2, 3)])
