@@ -680,12 +680,14 @@ __device__ __forceinline__ void gather_tile(const VolGeom& g, const uint64_t* __
 }
 
 // The span pass splits every item into `parts` equal pieces, the largest
-// power of two (up to 16) that keeps the pieces within two rounds of its
+// power of two (up to 16) that keeps the pieces within one round of its
 // blocks: a short list (a few dense tiles) then runs as many small spans,
-// whose latency is shorter, and a long one as the items themselves.
+// whose latency is shorter, and a long one as the items themselves.  A
+// piece is a chain of dependent phases (about 11 us at 8 words), so a
+// second round costs more than longer pieces save.
 __device__ __forceinline__ int gather_item_parts(int64_t n_items, int64_t blocks) {
   int parts = 1;
-  while (parts < 16 && n_items * parts * 2 <= 2 * blocks) parts *= 2;
+  while (parts < 16 && n_items * parts * 2 <= blocks) parts *= 2;
   return parts;
 }
 

@@ -187,8 +187,8 @@ int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz
                                                  gather_tile_span_lg(ceil_div(g.nwords, kGatherTile), cus));
   DAXHIP_LAUNCH_CHECK("k_point_gather");
   if (n_points > 0) {
-    // persistent blocks: two per CU
-    k_point_gather_spans<<<dim3((unsigned)(2 * cus)), dim3(kGatherThreads),
+    // persistent blocks: three per CU, as many as the LDS allows
+    k_point_gather_spans<<<dim3((unsigned)(3 * cus)), dim3(kGatherThreads),
                            gather_lds_bytes(g.wpr), s>>>(g, bits, bv, (const VertexAdj*)d_scratch,
                                                          d_counts, n_points, d_dl_dfield, list);
     DAXHIP_LAUNCH_CHECK("k_point_gather_spans");

@@ -687,7 +687,7 @@ int dax_hip_mc_batch_backward(const float* d_fields, const void* h_header,
       list);
   DAXHIP_LAUNCH_CHECK("k_batch_point_gather");
   if (n_points > 0) {
-    k_batch_point_gather_spans<<<dim3((unsigned)persistent_blocks(2)), dim3(kGatherThreads), lds,
+    k_batch_point_gather_spans<<<dim3((unsigned)persistent_blocks(3)), dim3(kGatherThreads), lds,
                                  s>>>(vols, gen_tiles, bits, bv, (const VertexAdj*)d_scratch,
                                       d_counts, n_points, d_dl_dfields, list);
     DAXHIP_LAUNCH_CHECK("k_batch_point_gather_spans");
