@@ -98,10 +98,11 @@ int device_cus() {
 
 
 // Dense tiles run their spans in their own block when the gather holds at
-// least this many tiles per CU (by default 64: 1024^3 1.55 + 0.47 ms ->
-// 1.92 + 0.01 ms); with fewer rounds of tiles such a block is the critical
-// path (256^3: 49 + 29 us -> 88 + 15 us).
-std::atomic<int64_t> g_span_tiles_per_cu{64};
+// least this many tiles per CU (by default 16: 1024^3, 256 per CU, 1.55 +
+// 0.47 ms -> 1.92 + 0.01 ms; 512^3, 32 per CU, forward + backward 0.369 ->
+// 0.359 ms); with fewer rounds of tiles such a block is the critical path
+// (256^3, 4 per CU: 49 + 29 us -> 88 + 15 us).
+std::atomic<int64_t> g_span_tiles_per_cu{16};
 
 int gather_tile_span_lg(int64_t n_tiles, int cus) {
   return n_tiles >= g_span_tiles_per_cu.load(std::memory_order_relaxed) * (int64_t)cus

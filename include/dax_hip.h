@@ -578,7 +578,7 @@ int dax_hip_mc_backward(const float* d_field, int64_t nx, int64_t ny, int64_t nz
  * endpoint records are redone in spans: by their own block, one span after
  * another, when the volume has at least `tiles_per_cu` gather tiles per CU
  * (many rounds of tiles hide such a block), else in a separate span pass
- * over many blocks.  Default 64; 0 keeps the spans in their tile's block at
+ * over many blocks.  Default 16; 0 keeps the spans in their tile's block at
  * any size.  Process-wide; the result does not depend on it (bit for bit). */
 int dax_hip_mc_backward_set_span_threshold(int64_t tiles_per_cu);
 

@@ -51,4 +51,4 @@ def test_span_threshold_setter_is_host_only():
     assert lib.dax_hip_mc_backward_set_span_threshold(-1) != 0
     assert b"negative" in lib.dax_hip_last_error()
     assert lib.dax_hip_mc_backward_set_span_threshold(0) == 0
-    assert lib.dax_hip_mc_backward_set_span_threshold(64) == 0
+    assert lib.dax_hip_mc_backward_set_span_threshold(16) == 0

@@ -209,7 +209,7 @@ def _gather_tile_endpoints(f, iso, rows):
     return tiles
 
 
-@pytest.mark.parametrize("tiles_per_cu", [64, 0])
+@pytest.mark.parametrize("tiles_per_cu", [16, 0])
 def test_backward_dense_tiles_in_spans(tiles_per_cu):
     """Gather tiles (256 row words: here one 64 x 256 plane) with more
     endpoint slots than their 2048 records.  With few tiles per CU every
@@ -247,7 +247,7 @@ def test_backward_dense_tiles_in_spans(tiles_per_cu):
         g2 = mc.backward(field, iso, out, dx.cuda(), dn.cuda())
         torch.cuda.synchronize()
     finally:
-        assert lib.dax_hip_mc_backward_set_span_threshold(64) == 0
+        assert lib.dax_hip_mc_backward_set_span_threshold(16) == 0
     assert torch.equal(g1, g2)
     r = orc.generate(f, iso)
     assert r["n_pts"] == U
