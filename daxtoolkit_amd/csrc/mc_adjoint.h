@@ -680,14 +680,17 @@ __device__ __forceinline__ void gather_tile(const VolGeom& g, const uint64_t* __
 }
 
 // The span pass splits every item into `parts` equal pieces, the largest
-// power of two (up to 16) that keeps the pieces within one round of its
-// blocks: a short list (a few dense tiles) then runs as many small spans,
-// whose latency is shorter, and a long one as the items themselves.  A
-// piece is a chain of dependent phases (about 11 us at 8 words), so a
-// second round costs more than longer pieces save.
-__device__ __forceinline__ int gather_item_parts(int64_t n_items, int64_t blocks) {
+// power of two (up to 16) that keeps the pieces within 2 * half_blocks: a
+// short list (a few dense tiles) then runs as many small spans, whose
+// latency is shorter, and a long one as the items themselves.  The span
+// kernels pass half their grid, so the pieces fit one round: a piece is a
+// chain of dependent phases (about 11 us at 8 words), and a second round
+// costs more than longer pieces save.  (Written this way, the span kernel
+// compiles to 76 VGPRs, three blocks per CU; `n * parts * 2 <= blocks`
+// gives 93.)
+__device__ __forceinline__ int gather_item_parts(int64_t n_items, int64_t half_blocks) {
   int parts = 1;
-  while (parts < 16 && n_items * parts * 2 <= blocks) parts *= 2;
+  while (parts < 16 && n_items * parts * 2 <= 2 * half_blocks) parts *= 2;
   return parts;
 }
 

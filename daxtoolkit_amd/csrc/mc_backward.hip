@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_point_gather_spans(
   extern __shared__ __align__(16) uint64_t s_dyn[];
   const int64_t n_adj = counts ? min(counts[1], n_points) : n_points;
   const int64_t n_items = min((int64_t)list[0], gather_list_capacity(n_points));
-  const int parts = gather_item_parts(n_items, gridDim.x);
+  const int parts = gather_item_parts(n_items, gridDim.x / 2);
   for (int64_t it = blockIdx.x; it < n_items * parts; it += gridDim.x) {
     const uint32_t item = list[1 + it / parts];
     int64_t w0, w_end;
